@@ -1,0 +1,388 @@
+/*
+ * lace.h -- pure-C ABI of the MI355X-native LACE preconditioned-solve path.
+ *
+ * Two groups of entry points live here.
+ *
+ *  1. Drop-in entry points.  Same symbol names, argument lists and container
+ *     layout as the reference's include/sparse.h, with host-array semantics:
+ *     the caller's data_d_matrix arrays are authoritative, the library uploads
+ *     them, runs the hot path on the GPU and writes results back into host
+ *     arrays allocated with the reference allocator conventions (64-byte
+ *     aligned, zero-filled, released with free()/data_zmfree).
+ *     Differences from the reference are limited to:
+ *       - errors are returned (DEV_* codes), never exit();
+ *       - FGMRES returns info = 0 when converged (the reference inverts it,
+ *         src/fgmres.cpp:495-497);
+ *       - C++ default arguments are gone (sparse.h:152,665); pass every one.
+ *
+ *  2. lace_* device-resident entry points.  Opaque handles that keep A, the
+ *     factors and the Krylov workspace in HBM across calls; no counterpart in
+ *     the reference, which copies host<->device every sweep
+ *     (src/parilu_v0_3_gpu.cpp:211).
+ *
+ * Struct layouts reproduce include/sparse_types.h:14-41,131-176 exactly,
+ * including the 64-byte alignment of the pointer members, so a binary built
+ * against the reference headers can pass its containers here unchanged.
+ */
+#ifndef LACE_MI355X_H
+#define LACE_MI355X_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* every declaration below is exported from the shared library */
+#pragma GCC visibility push(default)
+
+#define DEV_ALIGN 64
+
+typedef int    data_int_t;   /* include/dense_types.h:95-99 */
+typedef int    data_index_t;
+typedef double dataType;     /* include/dense_types.h:105 */
+typedef double real_Double_t;
+
+/* Return codes: include/dense_types.h:134-160 */
+#define DEV_SUCCESS               0
+#define DEV_ERR                  -100
+#define DEV_ERR_NOT_INITIALIZED  -101
+#define DEV_ERR_NOT_SUPPORTED    -103
+#define DEV_ERR_ILLEGAL_VALUE    -104
+#define DEV_ERR_NOT_FOUND        -105
+#define DEV_ERR_ALLOCATION       -106
+#define DEV_ERR_HOST_ALLOC       -112
+#define DEV_ERR_DEVICE_ALLOC     -113
+#define DEV_ERR_INVALID_PTR      -115
+#define DEV_ERR_UNKNOWN          -116
+#define DEV_ERR_NOT_IMPLEMENTED  -117
+#define DEV_ERR_NAN              -118
+#define DEV_SLOW_CONVERGENCE     -201
+#define DEV_DIVERGENCE           -202
+#define DEV_ERR_BADPRECOND       -204
+#define DEV_NOTCONVERGED         -205
+/* library-specific: a HIP runtime call failed */
+#define DEV_ERR_HIP              -301
+
+/* Enumerations: include/dense_types.h:175-388 (MAGMA numbering kept). */
+typedef enum { MagmaRowMajor = 101, MagmaColMajor = 102 } data_order_t;
+typedef enum {
+  MagmaUpper = 121, MagmaLower = 122, MagmaFull = 123, MagmaHessenberg = 124
+} data_uplo_t;
+typedef enum {
+  Magma_CSR = 611, Magma_ELLPACKT = 612, Magma_ELL = 613, Magma_DENSE = 614,
+  Magma_BCSR = 615, Magma_CSC = 616, Magma_HYB = 617, Magma_COO = 618,
+  Magma_ELLRT = 619, Magma_SPMVFUNCTION = 620, Magma_SELLP = 621,
+  Magma_ELLD = 622, Magma_CSRLIST = 623, Magma_CSRD = 624, Magma_CSRL = 627,
+  Magma_CSRU = 628, Magma_CSRCOO = 629, Magma_CUCSR = 630, Magma_COOLIST = 631,
+  Magma_CSR5 = 632, Magma_CSCD = 633, Magma_CSCL = 634, Magma_CSCU = 635,
+  Magma_CSCCOO = 636, Magma_DENSEL = 637, Magma_DENSEU = 638,
+  Magma_DENSED = 639, Magma_BCSRD = 640, Magma_BCSRL = 641, Magma_BCSRU = 642,
+  Magma_BCSC = 643, Magma_BCSCD = 644, Magma_BCSCL = 645, Magma_BCSCU = 646
+} data_storage_t;
+typedef enum { Magma_GENERAL = 581, Magma_SYMMETRIC = 582 } data_symmetry_t;
+typedef enum {
+  Magma_ORDERED = 591, Magma_DIAGFIRST = 592, Magma_UNITY = 593,
+  Magma_VALUE = 594, Magma_ZERO = 595, Magma_NODIAG = 596
+} data_diagorder_t;
+
+/* include/sparse_types.h:14-41 -- field order and alignment unchanged. */
+typedef struct data_d_matrix {
+  data_storage_t   storage_type;
+  data_symmetry_t  sym;
+  data_diagorder_t diagorder_type;
+  data_uplo_t      fill_mode;
+  data_int_t       num_rows;
+  data_int_t       num_cols;
+  data_int_t       nnz;
+  data_int_t       max_nnz_row;
+  data_int_t       diameter;
+  data_int_t       true_nnz;
+  dataType        *val    __attribute__((aligned(DEV_ALIGN)));
+  dataType        *diag   __attribute__((aligned(DEV_ALIGN)));
+  data_int_t      *row    __attribute__((aligned(DEV_ALIGN)));
+  data_int_t      *rowidx __attribute__((aligned(DEV_ALIGN)));
+  data_int_t      *col    __attribute__((aligned(DEV_ALIGN)));
+  data_int_t      *list;
+  data_int_t      *blockinfo;
+  data_int_t       blocksize;
+  data_int_t       ldblock;
+  data_int_t       numblocks;
+  data_int_t       alignment;
+  data_order_t     major;
+  data_int_t       ld;
+  data_int_t       pad_rows;
+  data_int_t       pad_cols;
+} __attribute__((aligned(DEV_ALIGN))) data_d_matrix;
+
+/* include/sparse_types.h:131-144 */
+typedef struct data_z_preconditioner_log {
+  data_int_t sweeps;
+  data_int_t maxSweeps;
+  dataType   tol;
+  dataType   finalStep;
+  dataType   A_Frobenius;
+  dataType   precond_generation_time;
+  dataType   initial_residual;
+  dataType   initial_nonlinear_residual;
+  dataType   residual;
+  dataType   nonlinear_residual;
+  data_int_t omp_num_threads;
+} __attribute__((aligned(DEV_ALIGN))) data_d_preconditioner_log;
+
+/* include/sparse_types.h:147-156 */
+typedef struct data_z_gmres_log {
+  data_int_t restarts;
+  data_int_t search_directions;
+  dataType   solve_time;
+  dataType   initial_residual;
+  dataType   final_residual;
+  dataType   scaled_residual;
+  dataType   original_residual;
+} __attribute__((aligned(DEV_ALIGN))) data_d_gmres_log;
+
+/* include/sparse_types.h:158-176.  user_csrtrsv_choice: 0 exact triangular
+ * solves (the reference's mkl_dcsrtrsv; here level-scheduled on the GPU),
+ * 1 ParCSRTRSV iterated sweeps to tol 1.0 (src/fgmres.cpp:129; here
+ * synchronous Jacobi sweeps), 2 no preconditioner.  This library adds
+ * 3 = fixed Jacobi-k sweeps with k = LACE_JACOBI_SWEEPS_DEFAULT. */
+typedef struct data_z_gmres_param {
+  data_int_t search_max;
+  data_int_t tol_type;
+  dataType   rtol;
+  data_int_t reorth;
+  data_int_t user_csrtrsv_choice;
+  data_int_t monitorOrthog;
+  data_int_t restart_max;
+  data_int_t precondition;
+  dataType   parilu_reduction;
+  data_int_t reorder;
+} __attribute__((aligned(DEV_ALIGN))) data_d_gmres_param;
+
+#define LACE_JACOBI_SWEEPS_DEFAULT 5
+
+/* ======================================================================
+ * Drop-in entry points (reference include/sparse.h line cited per entry)
+ * ====================================================================== */
+
+/* Containers and formats (host).  sparse.h:147,165,170,177,181,380 */
+/* major: the reference's C++ default argument is MagmaColMajor (sparse.h:152). */
+int data_zvinit(data_d_matrix *x, int num_rows, int num_cols, dataType value,
+                data_order_t major);
+int data_rowindex(data_d_matrix *A, int **rowidx);
+int data_zmconvert(data_d_matrix A, data_d_matrix *B,
+                   data_storage_t old_format, data_storage_t new_format);
+int data_zmtranspose(data_d_matrix A, data_d_matrix *B);
+int data_zmfree(data_d_matrix *A);
+int data_zmlumerge(data_d_matrix L, data_d_matrix U, data_d_matrix *A);
+
+/* MatrixMarket I/O.  sparse.h:56 (read), control/magma_zmio.cpp:587 (write) */
+int data_z_csr_mtx(data_d_matrix *A, const char *filename);
+int data_zwrite_csr_mtx(data_d_matrix A, data_order_t MajorType,
+                        const char *filename);
+
+/* Norms and the ILU residual check.  sparse.h:410,462-468;
+ * control/norms.cpp:69-92,301-405.  data_zilures evaluates ||A-LU||_F over
+ * LU's pattern (res) and over A's pattern (nonlinres) on the GPU. */
+dataType data_dnrm2(int n, dataType *x, int incx);
+int data_zfrobenius(data_d_matrix A, dataType *res);
+int data_zilures(data_d_matrix A, data_d_matrix L, data_d_matrix U,
+                 data_d_matrix *LU, dataType *res, dataType *nonlinres);
+
+/* Sparse products.  sparse.h:560-572.  data_z_spmv runs on the GPU;
+ * data_z_spmm is the sparse x sparse product used by the residual check. */
+int data_z_spmv(dataType alpha, data_d_matrix *A, data_d_matrix *x,
+                dataType beta, data_d_matrix *y);
+int data_z_spmm(dataType alpha, data_d_matrix A, data_d_matrix B,
+                data_d_matrix *C);
+
+/* Exact ILU(0) of A in place (the reference wraps MKL dcsrilu0,
+ * blas/zcsrilu0.cpp:15-75).  Returns DEV_ERR_NAN on a pivot below 1e-16. */
+int data_dcsrilu0_mkl(data_d_matrix *A);
+
+/* ParILU(0).  sparse.h:774-804.  L is returned as CSRL (unit diagonal last
+ * in each row), U as CSCU (U by columns, diagonal last).  All four run the
+ * sweeps on the GPU in synchronous (Jacobi) mode: every update of sweep k
+ * reads iterate k-1 only, so results are deterministic. */
+void data_PariLU_v0_3(data_d_matrix *A, data_d_matrix *L, data_d_matrix *U,
+                      dataType reduction, data_d_preconditioner_log *log);
+void data_PariLU_v0_3_prescribedSweeps(data_d_matrix *A, data_d_matrix *L,
+                                       data_d_matrix *U, dataType reduction,
+                                       data_d_preconditioner_log *log);
+void data_PariLU_v0_3_gpu(data_d_matrix *A, data_d_matrix *L, data_d_matrix *U,
+                          dataType reduction, data_d_preconditioner_log *log,
+                          int nthreads);
+void data_PariLU_v0_3_gpu_prescribedSweeps(data_d_matrix *A, data_d_matrix *L,
+                                           data_d_matrix *U, dataType reduction,
+                                           data_d_preconditioner_log *log);
+
+/* Triangular solves.  sparse.h:881-924; src/trisolve.cpp:20-119,256-351.
+ * Iterated sweeps until sum((y_new-y_old)^2) <= tol, run synchronously on
+ * the GPU (at most one sweep per level plus one confirming sweep). */
+int data_forward_solve(data_d_matrix *L, data_d_matrix *x, data_d_matrix *rhs,
+                       const dataType tol, int *iter);
+int data_backward_solve(data_d_matrix *U, data_d_matrix *x, data_d_matrix *rhs,
+                        const dataType tol, int *iter);
+int data_parcsrtrsv(const data_uplo_t uplo, const data_storage_t storage,
+                    const data_diagorder_t diag, const int num_rows,
+                    const dataType *Aval, const int *row, const int *col,
+                    const dataType *rhsval, dataType *yval,
+                    const dataType tol, int *iter);
+
+/* FGMRES.  sparse.h:1027-1052; src/fgmres.cpp:64-517,
+ * src/fgmres_householder_restart.cpp:63-788.  L: CSRL, U: CSRU.  x0 is
+ * freed and re-allocated with the solution (src/fgmres.cpp:491-492). */
+int data_fgmres(data_d_matrix *A, data_d_matrix *b, data_d_matrix *x0,
+                data_d_matrix *L, data_d_matrix *U,
+                data_d_gmres_param *gmres_par, data_d_gmres_log *gmres_log);
+int data_fgmres_restart(data_d_matrix *A, data_d_matrix *b, data_d_matrix *x0,
+                        data_d_matrix *L, data_d_matrix *U,
+                        data_d_gmres_param *gmres_par,
+                        data_d_gmres_log *gmres_log);
+int data_fgmres_householder(data_d_matrix *A, data_d_matrix *b,
+                            data_d_matrix *x0, data_d_matrix *L,
+                            data_d_matrix *U, data_d_gmres_param *gmres_par,
+                            data_d_gmres_log *gmres_log);
+int data_fgmres_householder_restart(data_d_matrix *A, data_d_matrix *b,
+                                    data_d_matrix *x0, data_d_matrix *L,
+                                    data_d_matrix *U,
+                                    data_d_gmres_param *gmres_par,
+                                    data_d_gmres_log *gmres_log);
+
+/* Givens generator.  sparse.h:1096; src/givens.cpp:46-67 */
+void givens_rotation(dataType a, dataType b, dataType tol, dataType *c,
+                     dataType *s);
+
+/* ======================================================================
+ * lace_* device-resident API (no reference counterpart; see header note)
+ * ====================================================================== */
+
+typedef enum { LACE_F64 = 0, LACE_F32 = 1, LACE_BF16 = 2 } lace_prec_t;
+
+typedef enum {
+  LACE_TRSV_LEVEL = 0,   /* exact, level-scheduled */
+  LACE_TRSV_JACOBI = 1,  /* k synchronous Jacobi sweeps from y0 = 0 */
+  LACE_TRSV_NONE = 2     /* identity preconditioner */
+} lace_trsv_kind_t;
+
+typedef struct lace_csr lace_csr;         /* device CSR matrix */
+typedef struct lace_parilu lace_parilu;   /* device ParILU factors */
+typedef struct lace_solver lace_solver;   /* device FGMRES solver */
+
+/* Runtime.  lace_init selects the device; lace_last_error returns the text
+ * of the most recent failure on this thread. */
+int lace_init(int device);
+const char *lace_last_error(void);
+int lace_device_synchronize(void);
+
+/* Raw device buffers (bytes) and copies, on the library stream. */
+int lace_dmalloc(void **ptr, size_t bytes);
+int lace_dfree(void *ptr);
+int lace_memcpy_h2d(void *dst, const void *src, size_t bytes);
+int lace_memcpy_d2h(void *dst, const void *src, size_t bytes);
+
+/* Device CSR upload (values converted to prec). */
+int lace_csr_create(const data_d_matrix *A, lace_prec_t prec, lace_csr **out);
+int lace_csr_destroy(lace_csr *A);
+int lace_csr_n(const lace_csr *A);
+/* y = alpha*A*x + beta*y on device vectors of A's precision. */
+int lace_csr_spmv(const lace_csr *A, double alpha, const void *x, double beta,
+                  void *y);
+
+/* ParILU(0) on device.  Factor storage in prec (F64 or F32; BF16 keeps a
+ * per-row scale), iteration arithmetic in F64 for F64 and F32 otherwise. */
+int lace_parilu_create(const data_d_matrix *A, lace_prec_t prec,
+                       lace_parilu **out);
+int lace_parilu_destroy(lace_parilu *P);
+/* Re-initialise the factors from A (L0 = strict-lower(A) + I, U0 = upper(A)). */
+int lace_parilu_reset(lace_parilu *P);
+/* Run nsweeps synchronous sweeps.  If step is non-NULL, the sum of squared
+ * updates of the last sweep divided by ||A||_F is written there (this
+ * synchronizes with the host). */
+int lace_parilu_sweeps(lace_parilu *P, int nsweeps, double *step);
+/* Copy factors to host: L as CSRL (unit diag last), U as CSCU. */
+int lace_parilu_get(lace_parilu *P, data_d_matrix *L, data_d_matrix *U);
+/* ||A - LU||_F restricted to the pattern of A, evaluated on the device. */
+int lace_parilu_nonlinres(lace_parilu *P, double *nonlinres);
+/* Per-sweep HBM bytes by the algorithmic count (see DESIGN). */
+double lace_parilu_sweep_bytes(const lace_parilu *P);
+/* Apply the factors: z = U^{-1} L^{-1} r (trans = 0) or
+ * z = L^{-T} U^{-T} r (trans = 1).  r, z are device vectors in the
+ * factor's arithmetic precision (F64 or F32). */
+int lace_parilu_apply(lace_parilu *P, lace_trsv_kind_t kind, int jacobi_sweeps,
+                      int trans, const void *r, void *z);
+int lace_parilu_levels(const lace_parilu *P, int *nlev_L, int *nlev_U);
+/* Non-zero if a sync-free level solve gave up waiting (a wait exceeded its
+ * poll budget); the result of that solve is then invalid. */
+int lace_syncfree_error(lace_parilu *P, int *err);
+
+/* Solver configuration. */
+typedef struct lace_solver_opts {
+  int    krylov;        /* 0 = Householder FGMRES, 1 = MGS FGMRES */
+  int    search_max;    /* restart length m */
+  int    restart_max;
+  double rtol;          /* relative to ||b - A x0|| */
+  lace_prec_t prec_precond;   /* factor / trsv precision */
+  lace_trsv_kind_t trsv;
+  int    jacobi_sweeps;
+  int    parilu_sweeps;
+  int    inner_iters;   /* >0: nested mixed mode -- each outer FP64 step
+                           preconditions with inner_iters FP32 GMRES steps */
+  int    transpose;     /* 1: solve A^T x = b with transposed factors */
+} lace_solver_opts;
+
+typedef struct lace_solver_stats {
+  int    iterations;      /* outer search directions over all restarts */
+  int    restarts;
+  int    inner_iterations;
+  double initial_residual;
+  double final_residual_est;  /* |g_{j+1}| */
+  double true_residual;       /* ||b - A x||_2 in FP64 */
+  double rel_true_residual;   /* true_residual / ||b||_2 */
+  double setup_ms;
+  double solve_ms;
+  int    converged;
+} lace_solver_stats;
+
+void lace_solver_default_opts(lace_solver_opts *o);
+int lace_solver_create(const data_d_matrix *A, const lace_solver_opts *o,
+                       lace_solver **out);
+int lace_solver_destroy(lace_solver *S);
+/* ParILU setup from A (reset + parilu_sweeps sweeps), on device. */
+int lace_solver_setup(lace_solver *S);
+/* Solve with device vectors b, x (FP64, length n); x is the initial guess
+ * on entry and the solution on exit. */
+int lace_solver_solve(lace_solver *S, const double *b, double *x,
+                      lace_solver_stats *st);
+/* dL/dtheta on the pattern of A: dtheta[k] = -lambda[row(k)] * x[col(k)]
+ * for CSR position k; device vectors. */
+int lace_solver_grad_values(lace_solver *S, const double *lambda,
+                            const double *x, double *dtheta);
+
+/* Kernel timing by class, HIP events on the library stream. */
+int lace_timing_enable(int on);
+int lace_timing_reset(void);
+int lace_timing_get(const char *name, double *total_ms, long *count);
+
+/* Level sets (host): lev(i) = 1 + max lev(k) over the off-diagonal entries
+ * of row i (lower: k < i; upper: k > i), 0 when there are none.  Writes
+ * perm (rows sorted by level, ties by row index) and level_ptr
+ * (nlevels+1 entries) and returns nlevels, or a negative DEV code. */
+int lace_level_sets(int n, const int *row, const int *col, int upper,
+                    int *perm, int *level_ptr);
+
+/* Synthetic problem generators (host). */
+int lace_gen_laplace2d(int nx, int ny, data_d_matrix *A);
+int lace_gen_convdiff3d(int nx, int ny, int nz, double pe_x, double pe_y,
+                        double pe_z, data_d_matrix *A);
+int lace_gen_normal(uint64_t seed, int n, double *out);
+uint32_t *lace_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2],
+                             uint32_t out[4]);
+
+#pragma GCC visibility pop
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* LACE_MI355X_H */

@@ -1,0 +1,529 @@
+// krylov.hip -- restarted Householder FGMRES on the GPU.
+//
+// Reference: src/fgmres_householder_restart.cpp:63-788 (Walker's
+// Householder Arnoldi with right flexible preconditioning), src/givens.cpp.
+//
+// Algorithm per restart cycle (n rows, restart length m):
+//   r = b - A x, first reflector u_0 from r (:239-308), q = -r/dd, g_0 = -dd
+//   for j = 0..m-1:
+//     z_j = M^{-1} q                                  (:320-349)
+//     w   = A z_j                                     (:356-367)
+//     w   = P_j ... P_0 w                             (:375-469)
+//     u_{j+1} from w[j+1:], h_{j+1,j} = -dd           (:476-517)
+//     q   = P_0 ... P_{j+1} e_{j+1}                   (:519-616)
+//     rotate h by the previous Givens rotations, form the new one (:658-701)
+//     stop when |g_{j+1}| < rtol                      (:714)
+//   y = R^{-1} g, x += Z y                            (:716-745)
+//
+// MI355X design: the reflector products are applied in compact-WY form,
+// P_0 P_1 ... P_j = I - U T U^T with T upper triangular
+// (T_{j+1} = [[T_j, -2 T_j U_j^T u_{j+1}], [0, 2]]), so each Arnoldi step
+// streams the reflector block U four times (multi-dot, update, multi-dot of
+// the tail, new column + next basis vector) instead of 2(j+1) dot/axpy
+// passes.  All O(j^2) work (T, Hessenberg, Givens, least squares) stays on
+// the device in one-workgroup kernels; the host only reads the convergence
+// flag.  Every kernel after the first of a cycle returns at once when the
+// device-side done flag is set, so a caller may queue steps ahead.
+#include <cmath>
+#include <type_traits>
+#include <vector>
+
+#include "common.hpp"
+#include "device.hpp"
+#include "krylov.hpp"
+
+namespace lace {
+
+constexpr int kBS = 256;
+
+__device__ __forceinline__ bool is_done(const double* misc) { return misc[MS_DONE] != 0.0; }
+
+// ------------------------------------------------------------ multi-dot
+// partial[b][k] = sum_{i in block b, i >= rowstart} U[i,k] * w[i], k < ncols;
+// with SELF also partial[b][pstride-1] = sum w[i]^2.
+template <typename K, int NC, bool SELF>
+__global__ __launch_bounds__(kBS) void mdot_kernel(int n, int ld, const K* __restrict__ U, int ncols,
+                                                   const K* __restrict__ w, int rowstart, int rpb,
+                                                   double* __restrict__ partial, int pstride,
+                                                   const double* __restrict__ misc) {
+  if (misc && is_done(misc)) return;
+  const int b = xcd_block(blockIdx.x, gridDim.x);
+  const int r0 = max(b * rpb, rowstart), r1 = min((b + 1) * rpb, n);
+  K acc[NC + 1];
+#pragma unroll
+  for (int k = 0; k <= NC; k++) acc[k] = K(0);
+  for (int i = r0 + (int)threadIdx.x; i < r1; i += kBS) {
+    const K wi = w[i];
+#pragma unroll
+    for (int k = 0; k < NC; k++)
+      if (k < ncols) acc[k] += U[(size_t)k * ld + i] * wi;
+    if (SELF) acc[NC] += wi * wi;
+  }
+  __shared__ double red[kBS / 64][NC + 1];
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+  for (int k = 0; k <= NC; k++) {
+    if (k < ncols || (SELF && k == NC)) {
+      double v = wave_sum((double)acc[k]);
+      if (lane == 0) red[wid][k] = v;
+    }
+  }
+  __syncthreads();
+  for (int k = threadIdx.x; k <= NC; k += kBS) {
+    if (k < ncols) partial[(size_t)b * pstride + k] = red[0][k] + red[1][k] + red[2][k] + red[3][k];
+    if (SELF && k == NC)
+      partial[(size_t)b * pstride + pstride - 1] = red[0][NC] + red[1][NC] + red[2][NC] + red[3][NC];
+  }
+}
+
+// Sum of the per-block partials in a fixed order: out[k] for k < cnt and,
+// if self, out[cnt] = the self slot.  One block of 256 threads, 4 segments
+// per column.
+__device__ void finalize_partials(int nb, int cnt, bool self, int pstride,
+                                  const double* __restrict__ partial, double* out) {
+  __shared__ double seg[4][72];
+  const int tot = cnt + (self ? 1 : 0);
+  for (int base = 0; base < tot; base += 64) {
+    const int c = base + (threadIdx.x & 63), s = threadIdx.x >> 6;
+    double v = 0.0;
+    if (c < tot) {
+      const int slot = (self && c == cnt) ? pstride - 1 : c;
+      for (int bb = s; bb < nb; bb += 4) v += partial[(size_t)bb * pstride + slot];
+    }
+    seg[s][threadIdx.x & 63] = v;
+    __syncthreads();
+    if (threadIdx.x < 64 && c < tot)
+      out[c] = seg[0][threadIdx.x] + seg[1][threadIdx.x] + seg[2][threadIdx.x] + seg[3][threadIdx.x];
+    __syncthreads();
+  }
+}
+
+// ------------------------------------------------------------ cycle start
+// After the norm pass over r: rnorm, dd = sgn(r_0)*rnorm, scale of u_0,
+// g_0 = -dd, T_00 = 2; done if rnorm < rtol or not finite.
+template <typename K>
+__global__ __launch_bounds__(256) void hh_start_kernel(int nb, int pstride, const double* partial,
+                                                       const K* __restrict__ r, double* small,
+                                                       SmallLayout lay, int set_rtol, double rtol_rel) {
+  double* misc = small + lay.misc();
+  __shared__ double nu[1];
+  finalize_partials(nb, 0, true, pstride, partial, nu);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const double rnorm = sqrt(nu[0]);
+    misc[MS_RNORM] = rnorm;
+    if (set_rtol) misc[MS_RTOL] = rtol_rel * rnorm;
+    const double r0 = (double)r[0];
+    const double sg = (r0 >= 0.0) ? 1.0 : -1.0;  // mysgn, include/sparse.h:1131
+    const double dd = sg * rnorm;
+    const double nv = sqrt(2.0 * rnorm * (rnorm + fabs(r0)));
+    misc[MS_DD] = dd;
+    misc[MS_SCALE] = nv > 0.0 ? 1.0 / nv : 0.0;
+    misc[MS_NCOLS] = 0.0;
+    misc[MS_BREAK] = 0.0;
+    misc[MS_RES] = rnorm;
+    const bool done = !(rnorm >= misc[MS_RTOL]) || !isfinite(rnorm);
+    misc[MS_DONE] = done ? 1.0 : 0.0;
+    double* g = small + lay.g();
+    for (int t = 0; t < lay.m + 2; t++) g[t] = 0.0;
+    g[0] = -dd;
+    small[lay.T()] = 2.0;
+  }
+}
+
+// u_0 = (r + dd e_0) * scale, q = -r / dd.
+template <typename K>
+__global__ __launch_bounds__(kBS) void hh_start_vec_kernel(int n, const K* __restrict__ r,
+                                                           K* __restrict__ U0, K* __restrict__ q,
+                                                           const double* __restrict__ misc) {
+  if (is_done(misc)) return;
+  const int i = blockIdx.x * kBS + threadIdx.x;
+  if (i >= n) return;
+  const double dd = misc[MS_DD], sc = misc[MS_SCALE];
+  const double ri = (double)r[i];
+  U0[i] = (K)((i == 0 ? ri + dd : ri) * sc);
+  q[i] = (K)(-ri / dd);
+}
+
+// ------------------------------------------------------------ step kernels
+// c = U_j^T w (from partials); d = T_j^T c.
+__global__ __launch_bounds__(256) void hh_dT_kernel(int nb, int pstride, const double* partial, int j,
+                                                    double* small, SmallLayout lay) {
+  double* misc = small + lay.misc();
+  if (is_done(misc)) return;
+  __shared__ double c[72];
+  finalize_partials(nb, j + 1, false, pstride, partial, c);
+  __syncthreads();
+  const double* T = small + lay.T();
+  const int ldT = lay.ldT();
+  double* d = small + lay.d();
+  for (int k = threadIdx.x; k <= j; k += 256) {
+    double v = 0.0;
+    for (int l = 0; l <= k; l++) v += T[l + k * ldT] * c[l];
+    d[k] = v;
+  }
+}
+
+// w <- w - U_j d.
+template <typename K, int NC>
+__global__ __launch_bounds__(kBS) void hh_update_kernel(int n, int ld, const K* __restrict__ U, int ncols,
+                                                        const double* __restrict__ small, SmallLayout lay,
+                                                        K* __restrict__ w) {
+  const double* misc = small + lay.misc();
+  if (is_done(misc)) return;
+  const int b = xcd_block(blockIdx.x, gridDim.x);
+  const int i = b * kBS + threadIdx.x;
+  if (i >= n) return;
+  const double* d = small + lay.d();
+  K v = w[i];
+#pragma unroll
+  for (int k = 0; k < NC; k++)
+    if (k < ncols) v -= U[(size_t)k * ld + i] * (K)d[k];
+  w[i] = v;
+}
+
+// After the tail multi-dot: s = U_j^T w[j+1:], nu = ||w[j+1:]||^2.
+// New reflector u_{j+1} = (w[j+1:] + dd e_{j+1}) * scale; T column j+1;
+// Hessenberg column j rotated by the previous Givens rotations; new
+// rotation (src/givens.cpp:46-67 with tol = the smallest subnormal);
+// residual estimate |g_{j+1}|; y' = T_{j+1} U_{j+1}^T e_{j+1} for q.
+template <typename K>
+__global__ __launch_bounds__(256) void hh_newcol_small_kernel(int nb, int pstride, const double* partial,
+                                                              int j, int n, int ld,
+                                                              const K* __restrict__ U,
+                                                              const K* __restrict__ w, double* small,
+                                                              SmallLayout lay) {
+  double* misc = small + lay.misc();
+  if (is_done(misc)) return;
+  __shared__ double s[72];
+  __shared__ double e[72];
+  __shared__ double sc[4];
+  finalize_partials(nb, j + 1, true, pstride, partial, s);  // s[0..j], s[j+1] = nu
+  __syncthreads();
+  double* T = small + lay.T();
+  const int ldT = lay.ldT();
+  const int j1 = j + 1;
+  if (threadIdx.x == 0) {
+    const double nu = s[j1];
+    const double alpha = j1 < n ? (double)w[j1] : 0.0;
+    const double sq = sqrt(nu);
+    const double dd = (alpha >= 0.0 ? 1.0 : -1.0) * sq;
+    const double nv = sqrt(2.0 * sq * (sq + fabs(alpha)));
+    const double scale = nv > 0.0 ? 1.0 / nv : 0.0;
+    misc[MS_DD] = dd;
+    misc[MS_SCALE] = scale;
+    misc[MS_ALPHA] = alpha;
+    sc[0] = dd;
+    sc[1] = scale;
+    sc[2] = alpha;
+  }
+  __syncthreads();
+  const double dd = sc[0], scale = sc[1], alpha = sc[2];
+  // e = U_{j+1}^T e_{j+1}: row j+1 of the old columns, then u_{j+1}[j+1]
+  for (int k = threadIdx.x; k <= j1; k += 256) {
+    if (k <= j) e[k] = j1 < n ? (double)U[(size_t)k * ld + j1] : 0.0;
+    else e[k] = (alpha + dd) * scale;
+  }
+  __syncthreads();
+  // utu = U_j^T u_{j+1} = scale * (s + dd * e[0..j]); T[0..j, j+1] = -2 T_j utu
+  for (int l = threadIdx.x; l <= j; l += 256) {
+    double v = 0.0;
+    for (int k = l; k <= j; k++) v += T[l + k * ldT] * (scale * (s[k] + dd * e[k]));
+    T[l + j1 * ldT] = -2.0 * v;
+  }
+  if (threadIdx.x == 0) T[j1 + j1 * ldT] = 2.0;
+  __syncthreads();
+  // y' = T_{j+1} e
+  double* yq = small + lay.y();
+  for (int l = threadIdx.x; l <= j1; l += 256) {
+    double v = 0.0;
+    for (int k = l; k <= j1; k++) v += T[l + k * ldT] * e[k];
+    yq[l] = v;
+  }
+  if (threadIdx.x == 0) {
+    double* R = small + lay.R();
+    double* cs = small + lay.cs();
+    double* sn = small + lay.sn();
+    double* g = small + lay.g();
+    const int ldR = lay.m + 1;
+    // Hessenberg column: h_i = w'_i (i <= j), h_{j+1} = -dd
+    for (int i = 0; i <= j; i++) R[i + j * ldR] = (double)w[i];
+    for (int i = 0; i < j; i++) {
+      const double t0 = R[i + j * ldR], t1 = R[i + 1 + j * ldR];
+      R[i + 1 + j * ldR] = -sn[i] * t0 + cs[i] * t1;
+      R[i + j * ldR] = cs[i] * t0 + sn[i] * t1;
+    }
+    const double a = R[j + j * ldR], bb = -dd;
+    const double tol = 4.9406564584124654e-324;  // nextafter(0,1)
+    double c, sn_;
+    if (fabs(bb) <= tol) {
+      c = 1.0;
+      sn_ = 0.0;
+    } else if (fabs(bb) > fabs(a)) {
+      const double t = a / bb;
+      sn_ = 1.0 / sqrt(1.0 + t * t);
+      c = t * sn_;
+    } else {
+      const double t = bb / a;
+      c = 1.0 / sqrt(1.0 + t * t);
+      sn_ = t * c;
+    }
+    cs[j] = c;
+    sn[j] = sn_;
+    R[j + j * ldR] = c * a - sn_ * dd;
+    g[j1] = -sn_ * g[j];
+    g[j] = c * g[j];
+    const double res = fabs(g[j1]);
+    misc[MS_RES] = res;
+    misc[MS_NCOLS] = (double)j1;
+    if (!(res >= misc[MS_RTOL]) || !isfinite(res)) misc[MS_DONE] = 1.0;
+    if (!isfinite(res)) misc[MS_BREAK] = 1.0;
+  }
+}
+
+// u_{j+1} = (w + dd e_{j+1}) * scale on rows >= j+1 (0 above);
+// q = e_{j+1} - U_{j+1} y'.
+template <typename K, int NC>
+__global__ __launch_bounds__(kBS) void hh_newcol_vec_kernel(int n, int ld, K* __restrict__ U, int j,
+                                                            const K* __restrict__ w,
+                                                            const double* __restrict__ small,
+                                                            SmallLayout lay, K* __restrict__ q,
+                                                            int write_q) {
+  const double* misc = small + lay.misc();
+  if (is_done(misc)) return;
+  const int b = xcd_block(blockIdx.x, gridDim.x);
+  const int i = b * kBS + threadIdx.x;
+  if (i >= n) return;
+  const int j1 = j + 1;
+  const double dd = misc[MS_DD], scale = misc[MS_SCALE];
+  K unew = K(0);
+  if (i >= j1) unew = (K)(((double)w[i] + (i == j1 ? dd : 0.0)) * scale);
+  U[(size_t)j1 * ld + i] = unew;
+  if (!write_q) return;
+  const double* yq = small + lay.y();
+  K v = (i == j1) ? K(1) : K(0);
+#pragma unroll
+  for (int k = 0; k < NC; k++)
+    if (k <= j) v -= U[(size_t)k * ld + i] * (K)yq[k];
+  v -= unew * (K)yq[j1];
+  q[i] = v;
+}
+
+// Least squares y = R^{-1} g by the column-oriented back substitution of
+// src/fgmres_householder_restart.cpp:721-729, for the ncols columns done.
+__global__ void hh_lsq_kernel(double* small, SmallLayout lay) {
+  if (threadIdx.x != 0) return;
+  double* misc = small + lay.misc();
+  const int nc = (int)misc[MS_NCOLS];
+  const double* R = small + lay.R();
+  const double* g = small + lay.g();
+  double* y = small + lay.y();
+  const int ldR = lay.m + 1;
+  for (int i = 0; i < nc; i++) y[i] = g[i] / R[i + i * ldR];
+  for (int jj = nc - 1; jj > 0; jj--)
+    for (int i = jj - 1; i >= 0; i--) y[i] = y[i] - R[i + jj * ldR] * y[jj] / R[i + i * ldR];
+}
+
+// x += Z y over the ncols columns done.
+template <typename XT, typename ZT, int NC>
+__global__ __launch_bounds__(kBS) void hh_xupdate_kernel(int n, int ld, const ZT* __restrict__ Z,
+                                                         const double* __restrict__ small,
+                                                         SmallLayout lay, XT* __restrict__ x) {
+  const double* misc = small + lay.misc();
+  const int nc = (int)misc[MS_NCOLS];
+  const int b = xcd_block(blockIdx.x, gridDim.x);
+  const int i = b * kBS + threadIdx.x;
+  if (i >= n) return;
+  const double* y = small + lay.y();
+  using ACC = typename std::conditional<sizeof(XT) == 8, double, float>::type;
+  ACC v = (ACC)x[i];
+#pragma unroll
+  for (int k = 0; k < NC; k++)
+    if (k < nc) v += (ACC)Z[(size_t)k * ld + i] * (ACC)y[k];
+  x[i] = (XT)v;
+}
+
+template <typename To, typename From>
+__global__ void cast_kernel(int n, const From* __restrict__ in, To* __restrict__ out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = (To)in[i];
+}
+
+__global__ void set_misc_kernel(double* small, SmallLayout lay, int slot, double v) {
+  if (threadIdx.x == 0) small[lay.misc() + slot] = v;
+}
+
+// ------------------------------------------------------------ engine
+template <typename K>
+int HHEngine<K>::init(int n_, int m_, bool z_float_) {
+  LACE_REQUIRE(m_ >= 1 && m_ <= 63, DEV_ERR_NOT_SUPPORTED,
+               "restart length %d outside the supported range 1..63", m_);
+  n = n_;
+  m = m_;
+  z_float = z_float_;
+  ld = ((n + 63) / 64) * 64;
+  lay = SmallLayout{m};
+  nb = ceil_div(n, rpb);
+  pstride = m + 3;
+  LACE_TRY(U.alloc(sizeof(K) * (size_t)ld * (m + 1)));
+  LACE_TRY(Z.alloc((z_float ? 4 : sizeof(K)) * (size_t)ld * m));
+  LACE_TRY(w.alloc(sizeof(K) * ld));
+  LACE_TRY(q.alloc(sizeof(K) * ld));
+  LACE_TRY(r.alloc(sizeof(K) * ld));
+  LACE_TRY(small.alloc(sizeof(double) * lay.total()));
+  LACE_TRY(partial.alloc(sizeof(double) * (size_t)nb * pstride));
+  LACE_HIP(hipMemsetAsync(small.p, 0, sizeof(double) * lay.total(), stream()));
+  LACE_HIP(hipMemsetAsync(U.p, 0, sizeof(K) * (size_t)ld * (m + 1), stream()));
+  LACE_HIP(hipHostMalloc((void**)&host_misc, sizeof(double) * 32, hipHostMallocDefault));
+  return DEV_SUCCESS;
+}
+
+template <typename K>
+HHEngine<K>::~HHEngine() {
+  if (host_misc) (void)hipHostFree(host_misc);
+}
+
+template <typename K>
+void* HHEngine<K>::zcol(int j) const {
+  return (char*)Z.p + (size_t)j * ld * (z_float ? 4 : sizeof(K));
+}
+
+template struct HHEngine<double>;
+template struct HHEngine<float>;
+
+// Dispatch on the compile-time column bound.
+#define LACE_NC_DISPATCH(ncols, BODY) \
+  do {                                \
+    if ((ncols) <= 8) {               \
+      constexpr int NCB = 8;          \
+      BODY;                           \
+    } else if ((ncols) <= 16) {       \
+      constexpr int NCB = 16;         \
+      BODY;                           \
+    } else if ((ncols) <= 32) {       \
+      constexpr int NCB = 32;         \
+      BODY;                           \
+    } else {                          \
+      constexpr int NCB = 64;         \
+      BODY;                           \
+    }                                 \
+  } while (0)
+
+// Norm pass over r and the start of a cycle (u_0, q).  set_rtol: store
+// rtol_rel * ||r|| as the absolute tolerance.
+template <typename K>
+int hh_start(HHEngine<K>& E, int set_rtol, double rtol_rel) {
+  TimedRegion tr("hh_start");
+  mdot_kernel<K, 8, true><<<E.nb, kBS, 0, stream()>>>(E.n, E.ld, E.U.template as<K>(), 0,
+                                                     E.r.template as<K>(), 0, E.rpb,
+                                                     E.partial.template as<double>(), E.pstride,
+                                                     nullptr);
+  hh_start_kernel<K><<<1, 256, 0, stream()>>>(E.nb, E.pstride, E.partial.template as<double>(),
+                                              E.r.template as<K>(), E.small.template as<double>(),
+                                              E.lay, set_rtol, rtol_rel);
+  hh_start_vec_kernel<K><<<ceil_div(E.n, kBS), kBS, 0, stream()>>>(
+      E.n, E.r.template as<K>(), E.U.template as<K>(), E.q.template as<K>(),
+      E.small.template as<double>() + E.lay.misc());
+  LACE_HIP(hipGetLastError());
+  return DEV_SUCCESS;
+}
+
+// Householder part of step j (after w = A z_j).
+template <typename K>
+int hh_step(HHEngine<K>& E, int j, bool write_q) {
+  TimedRegion tr("householder");
+  const int nc = j + 1;
+  K* U = E.U.template as<K>();
+  K* w = E.w.template as<K>();
+  double* small = E.small.template as<double>();
+  double* part = E.partial.template as<double>();
+  const double* misc = small + E.lay.misc();
+  const int nbv = ceil_div(E.n, kBS);
+  LACE_NC_DISPATCH(nc, (mdot_kernel<K, NCB, false><<<E.nb, kBS, 0, stream()>>>(
+                           E.n, E.ld, U, nc, w, 0, E.rpb, part, E.pstride, misc)));
+  hh_dT_kernel<<<1, 256, 0, stream()>>>(E.nb, E.pstride, part, j, small, E.lay);
+  LACE_NC_DISPATCH(nc, (hh_update_kernel<K, NCB><<<nbv, kBS, 0, stream()>>>(E.n, E.ld, U, nc, small,
+                                                                            E.lay, w)));
+  LACE_NC_DISPATCH(nc, (mdot_kernel<K, NCB, true><<<E.nb, kBS, 0, stream()>>>(
+                           E.n, E.ld, U, nc, w, j + 1, E.rpb, part, E.pstride, misc)));
+  hh_newcol_small_kernel<K><<<1, 256, 0, stream()>>>(E.nb, E.pstride, part, j, E.n, E.ld, U, w,
+                                                     small, E.lay);
+  LACE_NC_DISPATCH(nc, (hh_newcol_vec_kernel<K, NCB><<<nbv, kBS, 0, stream()>>>(
+                           E.n, E.ld, U, j, w, small, E.lay, E.q.template as<K>(), write_q ? 1 : 0)));
+  LACE_HIP(hipGetLastError());
+  return DEV_SUCCESS;
+}
+
+// y = R^{-1} g and x += Z y.
+template <typename K, typename XT>
+int hh_finish(HHEngine<K>& E, XT* x) {
+  TimedRegion tr("hh_finish");
+  double* small = E.small.template as<double>();
+  hh_lsq_kernel<<<1, 64, 0, stream()>>>(small, E.lay);
+  const int nbv = ceil_div(E.n, kBS);
+  if (E.z_float)
+    LACE_NC_DISPATCH(E.m, (hh_xupdate_kernel<XT, float, NCB><<<nbv, kBS, 0, stream()>>>(
+                              E.n, E.ld, E.Z.template as<float>(), small, E.lay, x)));
+  else
+    LACE_NC_DISPATCH(E.m, (hh_xupdate_kernel<XT, K, NCB><<<nbv, kBS, 0, stream()>>>(
+                              E.n, E.ld, E.Z.template as<K>(), small, E.lay, x)));
+  LACE_HIP(hipGetLastError());
+  return DEV_SUCCESS;
+}
+
+template <typename K>
+int read_misc(HHEngine<K>& E) {
+  LACE_HIP(hipMemcpyAsync(E.host_misc, E.small.template as<double>() + E.lay.misc(),
+                          sizeof(double) * 16, hipMemcpyDeviceToHost, stream()));
+  LACE_HIP(hipStreamSynchronize(stream()));
+  return DEV_SUCCESS;
+}
+
+template int hh_start<double>(HHEngine<double>&, int, double);
+template int hh_start<float>(HHEngine<float>&, int, double);
+template int hh_step<double>(HHEngine<double>&, int, bool);
+template int hh_step<float>(HHEngine<float>&, int, bool);
+template int hh_finish<double, double>(HHEngine<double>&, double*);
+template int hh_finish<float, float>(HHEngine<float>&, float*);
+template int read_misc<double>(HHEngine<double>&);
+template int read_misc<float>(HHEngine<float>&);
+
+// Helpers used by the solver for norms and casts.
+int norm2_sq_partials(int n, int rpb, const double* x, double* partial, int pstride, int nb) {
+  mdot_kernel<double, 8, true><<<nb, kBS, 0, stream()>>>(n, n, x, 0, x, 0, rpb, partial, pstride,
+                                                        nullptr);
+  LACE_HIP(hipGetLastError());
+  return DEV_SUCCESS;
+}
+
+__global__ void sum_slot_kernel(int nb, int pstride, const double* partial, double* out) {
+  __shared__ double v[1];
+  finalize_partials(nb, 0, true, pstride, partial, v);
+  __syncthreads();
+  if (threadIdx.x == 0) *out = v[0];
+}
+
+int sum_slot(int nb, int pstride, const double* partial, double* out) {
+  sum_slot_kernel<<<1, 256, 0, stream()>>>(nb, pstride, partial, out);
+  LACE_HIP(hipGetLastError());
+  return DEV_SUCCESS;
+}
+
+int cast_d2f(int n, const double* in, float* out) {
+  cast_kernel<float, double><<<ceil_div(n, 256), 256, 0, stream()>>>(n, in, out);
+  LACE_HIP(hipGetLastError());
+  return DEV_SUCCESS;
+}
+int cast_f2d(int n, const float* in, double* out) {
+  cast_kernel<double, float><<<ceil_div(n, 256), 256, 0, stream()>>>(n, in, out);
+  LACE_HIP(hipGetLastError());
+  return DEV_SUCCESS;
+}
+
+int set_misc_d(double* small, SmallLayout lay, int slot, double v) {
+  set_misc_kernel<<<1, 64, 0, stream()>>>(small, lay, slot, v);
+  LACE_HIP(hipGetLastError());
+  return DEV_SUCCESS;
+}
+
+}  // namespace lace

@@ -1,0 +1,609 @@
+// parilu.hip -- ParILU(0) fixed-point sweeps on the GPU.
+//
+// Reference: src/parilu_v0_3.cpp:16-132 (the sweep body :70-102),
+// src/parilu_v0_3_prescribedSweeps.cpp:17-132, src/parilu_v0_3_gpu.cpp:17-242.
+//
+// Layout in HBM (one set per handle):
+//   A      CSR (row, col) + COO row index per nonzero + values in the
+//          iteration precision T (double for F64, float for F32/BF16)
+//   L      CSRL: strict-lower entries of A's pattern then the unit diagonal,
+//          two value buffers (Jacobi double buffering)
+//   U      CSCU: U by columns, rows ascending, diagonal last; two buffers
+//   dst    per nonzero of A, its slot in L (>= 0) or U (-1 - slot); used to
+//          re-initialise the factors from A on the device
+//
+// Sweep semantics: synchronous (Jacobi).  Every update of sweep k reads
+// iterate k-1 only, so the result does not depend on scheduling.  The
+// per-nonzero arithmetic is the reference's: the merge subtracts each
+// rounded product l_ik*u_kj in ascending k, adds the last product back
+// (src/parilu_v0_3.cpp:90) and divides by u_jj for L entries.  Contraction
+// is disabled in the kernel so the FP64 and FP32 results equal the CPU
+// restatement bit for bit.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <vector>
+
+#include "common.hpp"
+#include "device.hpp"
+#include "parilu.hpp"
+
+namespace lace {
+
+constexpr int kSweepBS = 256;
+
+// One thread per nonzero of A.  STEP: also accumulate sum (old-new)^2 into
+// a per-block partial.
+template <typename T, bool STEP>
+__global__ __launch_bounds__(kSweepBS) void parilu_sweep_kernel(
+    int nnz, const int* __restrict__ rowidx, const int* __restrict__ acol,
+    const T* __restrict__ aval, const int* __restrict__ lrow, const int* __restrict__ lcol,
+    const T* __restrict__ Lin, T* __restrict__ Lout, const int* __restrict__ ucp,
+    const int* __restrict__ uri, const T* __restrict__ Uin, T* __restrict__ Uout,
+    double* __restrict__ partial) {
+#pragma clang fp contract(off)
+  const int b = xcd_block(blockIdx.x, gridDim.x);
+  const int k = b * kSweepBS + threadIdx.x;
+  double d2 = 0.0;
+  if (k < nnz) {
+    const int i = rowidx[k], j = acol[k];
+    T s = aval[k];
+    T sp = T(0);
+    int il = lrow[i];
+    const int ile = lrow[i + 1];
+    int iu = ucp[j];
+    const int iue = ucp[j + 1];
+    while (il < ile && iu < iue) {
+      sp = T(0);
+      const int jl = lcol[il], ju = uri[iu];
+      if (jl == ju) {
+        sp = Lin[il] * Uin[iu];
+        s = s - sp;
+      }
+      il += (jl <= ju);
+      iu += (jl >= ju);
+    }
+    s = s + sp;
+    if (i > j) {
+      const T t = s / Uin[iue - 1];
+      if (STEP) {
+        double d = (double)Lin[il - 1] - (double)t;
+        d2 = d * d;
+      }
+      Lout[il - 1] = t;
+    } else {
+      if (STEP) {
+        double d = (double)Uin[iu - 1] - (double)s;
+        d2 = d * d;
+      }
+      Uout[iu - 1] = s;
+    }
+  }
+  if (STEP) {
+    __shared__ double sm[kSweepBS / 64];
+    double tot = block_sum<kSweepBS>(d2, sm);
+    if (threadIdx.x == 0) partial[b] = tot;
+  }
+}
+
+// Sum of per-block partials in a fixed order (one block).
+__global__ __launch_bounds__(256) void sum_partials_kernel(int nb, const double* __restrict__ partial,
+                                                           double scale, double* __restrict__ out) {
+  __shared__ double sm[4];
+  double v = 0.0;
+  for (int t = threadIdx.x; t < nb; t += 256) v += partial[t];
+  double tot = block_sum<256>(v, sm);
+  if (threadIdx.x == 0) *out = tot * scale;
+}
+
+// Factors from A: L0 = strict-lower(A) + I, U0 = upper(A).
+template <typename T>
+__global__ void parilu_reset_kernel(int nnz, const int* __restrict__ dst, const T* __restrict__ aval,
+                                    T* __restrict__ L0, T* __restrict__ L1, T* __restrict__ U0,
+                                    T* __restrict__ U1) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= nnz) return;
+  const int d = dst[k];
+  const T v = aval[k];
+  if (d >= 0) {
+    L0[d] = v;
+    L1[d] = v;
+  } else {
+    U0[-1 - d] = v;
+    U1[-1 - d] = v;
+  }
+}
+
+template <typename T>
+__global__ void parilu_unitdiag_kernel(int n, const int* __restrict__ lrow, const int* __restrict__ lcol,
+                                       T* __restrict__ L0, T* __restrict__ L1) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int e = lrow[i + 1] - 1;
+  if (e >= lrow[i] && lcol[e] == i) {
+    L0[e] = T(1);
+    L1[e] = T(1);
+  }
+}
+
+// Residual of the fixed point on A's pattern: (LU)_ij - a_ij with the full
+// merge (no add-back), summed as squares per block.
+template <typename T>
+__global__ __launch_bounds__(kSweepBS) void parilu_nonlinres_kernel(
+    int nnz, const int* __restrict__ rowidx, const int* __restrict__ acol,
+    const T* __restrict__ aval, const int* __restrict__ lrow, const int* __restrict__ lcol,
+    const T* __restrict__ L, const int* __restrict__ ucp, const int* __restrict__ uri,
+    const T* __restrict__ U, double* __restrict__ partial) {
+  const int k = blockIdx.x * kSweepBS + threadIdx.x;
+  double d2 = 0.0;
+  if (k < nnz) {
+    const int i = rowidx[k], j = acol[k];
+    double s = 0.0;
+    int il = lrow[i], iu = ucp[j];
+    const int ile = lrow[i + 1], iue = ucp[j + 1];
+    while (il < ile && iu < iue) {
+      const int jl = lcol[il], ju = uri[iu];
+      if (jl == ju) s += (double)L[il] * (double)U[iu];
+      il += (jl <= ju);
+      iu += (jl >= ju);
+    }
+    double d = s - (double)aval[k];
+    d2 = d * d;
+  }
+  __shared__ double sm[kSweepBS / 64];
+  double tot = block_sum<kSweepBS>(d2, sm);
+  if (threadIdx.x == 0) partial[blockIdx.x] = tot;
+}
+
+template <typename To, typename From>
+__global__ void convert_kernel(long n, const From* __restrict__ in, To* __restrict__ out) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = static_cast<To>(in[i]);
+}
+
+// ---------------------------------------------------------------- host side
+static int build_pattern(const data_d_matrix* A, ParILU* P) {
+  const int n = A->num_rows;
+  LACE_REQUIRE(A->num_rows == A->num_cols, DEV_ERR_ILLEGAL_VALUE, "ParILU needs a square matrix");
+  LACE_REQUIRE(A->row && A->col && A->val, DEV_ERR_INVALID_PTR, "ParILU: A has null arrays");
+  const int nnz = A->row[n];
+  // every row must be sorted and hold its diagonal (the merge relies on the
+  // diagonal closing each L row and U column)
+  for (int i = 0; i < n; i++) {
+    bool diag = false;
+    for (int k = A->row[i]; k < A->row[i + 1]; k++) {
+      if (k > A->row[i]) LACE_REQUIRE(A->col[k] > A->col[k - 1], DEV_ERR_ILLEGAL_VALUE,
+                                      "ParILU: row %d is not sorted / has duplicates", i);
+      diag |= A->col[k] == i;
+    }
+    LACE_REQUIRE(diag, DEV_ERR_ILLEGAL_VALUE, "ParILU: row %d has no diagonal entry", i);
+  }
+  P->n = n;
+  P->nnz = nnz;
+  // L (CSRL): per row, strict-lower then diagonal
+  P->h_lrow.assign(n + 1, 0);
+  P->h_lcol.clear();
+  std::vector<int> dst(nnz);
+  std::vector<int> ucount(n + 1, 0);
+  for (int i = 0; i < n; i++) {
+    P->h_lrow[i] = (int)P->h_lcol.size();
+    for (int k = A->row[i]; k < A->row[i + 1]; k++) {
+      int j = A->col[k];
+      if (j < i) {
+        dst[k] = (int)P->h_lcol.size();
+        P->h_lcol.push_back(j);
+      } else {
+        ucount[j + 1]++;
+      }
+    }
+    P->h_lcol.push_back(i);
+  }
+  P->h_lrow[n] = (int)P->h_lcol.size();
+  P->nnzL = P->h_lrow[n];
+  // U (CSCU): counting sort of upper entries by column, rows ascending
+  for (int j = 0; j < n; j++) ucount[j + 1] += ucount[j];
+  P->h_ucp = ucount;
+  P->nnzU = ucount[n];
+  P->h_uri.assign(P->nnzU, 0);
+  std::vector<int> fill(ucount.begin(), ucount.end() - 1);
+  for (int i = 0; i < n; i++)
+    for (int k = A->row[i]; k < A->row[i + 1]; k++) {
+      int j = A->col[k];
+      if (j >= i) {
+        int p = fill[j]++;
+        P->h_uri[p] = i;
+        dst[k] = -1 - p;
+      }
+    }
+  std::vector<int> rowidx(nnz);
+  for (int i = 0; i < n; i++)
+    for (int k = A->row[i]; k < A->row[i + 1]; k++) rowidx[k] = i;
+  double s = 0.0;
+  for (int k = 0; k < nnz; k++) s += A->val[k] * A->val[k];
+  P->anorm = std::sqrt(s);
+
+  LACE_TRY(upload(P->arow, A->row, sizeof(int) * (n + 1)));
+  LACE_TRY(upload(P->acol, A->col, sizeof(int) * nnz));
+  LACE_TRY(upload(P->arowidx, rowidx.data(), sizeof(int) * nnz));
+  LACE_TRY(upload(P->dst, dst.data(), sizeof(int) * nnz));
+  LACE_TRY(upload(P->lrow, P->h_lrow.data(), sizeof(int) * (n + 1)));
+  LACE_TRY(upload(P->lcol, P->h_lcol.data(), sizeof(int) * P->nnzL));
+  LACE_TRY(upload(P->ucp, P->h_ucp.data(), sizeof(int) * (n + 1)));
+  LACE_TRY(upload(P->uri, P->h_uri.data(), sizeof(int) * P->nnzU));
+  // A values in the iteration precision
+  DevBuf a64;
+  LACE_TRY(upload(a64, A->val, sizeof(double) * nnz));
+  const size_t es = P->elem_size();
+  LACE_TRY(P->aval.alloc(es * nnz));
+  if (P->prec == LACE_F64) {
+    LACE_HIP(hipMemcpyAsync(P->aval.p, a64.p, sizeof(double) * nnz, hipMemcpyDeviceToDevice, stream()));
+  } else {
+    convert_kernel<float, double><<<ceil_div(nnz, 256), 256, 0, stream()>>>(nnz, a64.as<double>(),
+                                                                          P->aval.as<float>());
+    LACE_HIP(hipGetLastError());
+  }
+  for (int b = 0; b < 2; b++) {
+    LACE_TRY(P->lval[b].alloc(es * P->nnzL));
+    LACE_TRY(P->uval[b].alloc(es * P->nnzU));
+  }
+  const int nb = ceil_div(nnz, kSweepBS);
+  LACE_TRY(P->partial.alloc(sizeof(double) * (nb + 1)));
+  LACE_TRY(P->scalar.alloc(sizeof(double) * 4));
+  LACE_HIP(hipStreamSynchronize(stream()));
+  return DEV_SUCCESS;
+}
+
+int ParILU::reset() {
+  const int nb = ceil_div(nnz, 256);
+  if (prec == LACE_F64) {
+    parilu_reset_kernel<double><<<nb, 256, 0, stream()>>>(nnz, dst.as<int>(), aval.as<double>(),
+                                                          lval[0].as<double>(), lval[1].as<double>(),
+                                                          uval[0].as<double>(), uval[1].as<double>());
+    parilu_unitdiag_kernel<double><<<ceil_div(n, 256), 256, 0, stream()>>>(
+        n, lrow.as<int>(), lcol.as<int>(), lval[0].as<double>(), lval[1].as<double>());
+  } else {
+    parilu_reset_kernel<float><<<nb, 256, 0, stream()>>>(nnz, dst.as<int>(), aval.as<float>(),
+                                                         lval[0].as<float>(), lval[1].as<float>(),
+                                                         uval[0].as<float>(), uval[1].as<float>());
+    parilu_unitdiag_kernel<float><<<ceil_div(n, 256), 256, 0, stream()>>>(
+        n, lrow.as<int>(), lcol.as<int>(), lval[0].as<float>(), lval[1].as<float>());
+  }
+  LACE_HIP(hipGetLastError());
+  cur = 0;
+  return DEV_SUCCESS;
+}
+
+template <typename T>
+static int sweep_t(ParILU* P, bool with_step) {
+  const int nb = ceil_div(P->nnz, kSweepBS);
+  const int in = P->cur, out = 1 - P->cur;
+  TimedRegion tr("parilu_sweep");
+  if (with_step)
+    parilu_sweep_kernel<T, true><<<nb, kSweepBS, 0, stream()>>>(
+        P->nnz, P->arowidx.as<int>(), P->acol.as<int>(), P->aval.as<T>(), P->lrow.as<int>(),
+        P->lcol.as<int>(), P->lval[in].as<T>(), P->lval[out].as<T>(), P->ucp.as<int>(),
+        P->uri.as<int>(), P->uval[in].as<T>(), P->uval[out].as<T>(), P->partial.as<double>());
+  else
+    parilu_sweep_kernel<T, false><<<nb, kSweepBS, 0, stream()>>>(
+        P->nnz, P->arowidx.as<int>(), P->acol.as<int>(), P->aval.as<T>(), P->lrow.as<int>(),
+        P->lcol.as<int>(), P->lval[in].as<T>(), P->lval[out].as<T>(), P->ucp.as<int>(),
+        P->uri.as<int>(), P->uval[in].as<T>(), P->uval[out].as<T>(), nullptr);
+  LACE_HIP(hipGetLastError());
+  P->cur = out;
+  return DEV_SUCCESS;
+}
+
+int ParILU::sweep(bool with_step) {
+  return prec == LACE_F64 ? sweep_t<double>(this, with_step) : sweep_t<float>(this, with_step);
+}
+
+int ParILU::read_step(double* step) {
+  const int nb = ceil_div(nnz, kSweepBS);
+  sum_partials_kernel<<<1, 256, 0, stream()>>>(nb, partial.as<double>(), 1.0 / anorm,
+                                               scalar.as<double>());
+  LACE_HIP(hipGetLastError());
+  LACE_HIP(hipMemcpyAsync(step, scalar.p, sizeof(double), hipMemcpyDeviceToHost, stream()));
+  LACE_HIP(hipStreamSynchronize(stream()));
+  return DEV_SUCCESS;
+}
+
+int ParILU::nonlinres(double* out) {
+  const int nb = ceil_div(nnz, kSweepBS);
+  if (prec == LACE_F64)
+    parilu_nonlinres_kernel<double><<<nb, kSweepBS, 0, stream()>>>(
+        nnz, arowidx.as<int>(), acol.as<int>(), aval.as<double>(), lrow.as<int>(), lcol.as<int>(),
+        lval[cur].as<double>(), ucp.as<int>(), uri.as<int>(), uval[cur].as<double>(),
+        partial.as<double>());
+  else
+    parilu_nonlinres_kernel<float><<<nb, kSweepBS, 0, stream()>>>(
+        nnz, arowidx.as<int>(), acol.as<int>(), aval.as<float>(), lrow.as<int>(), lcol.as<int>(),
+        lval[cur].as<float>(), ucp.as<int>(), uri.as<int>(), uval[cur].as<float>(),
+        partial.as<double>());
+  LACE_HIP(hipGetLastError());
+  double s = 0.0;
+  sum_partials_kernel<<<1, 256, 0, stream()>>>(nb, partial.as<double>(), 1.0, scalar.as<double>());
+  LACE_HIP(hipMemcpyAsync(&s, scalar.p, sizeof(double), hipMemcpyDeviceToHost, stream()));
+  LACE_HIP(hipStreamSynchronize(stream()));
+  *out = std::sqrt(s);
+  return DEV_SUCCESS;
+}
+
+double ParILU::sweep_bytes() const {
+  // Algorithmic traffic of one sweep: A value, column and row index per
+  // nonzero, the old and new factor value per nonzero, row pointers.
+  const double es = (double)elem_size();
+  return (double)nnz * (es + 4.0 + 4.0 + 2.0 * es) + 4.0 * (n + 1);
+}
+
+int ParILU::get(data_d_matrix* L, data_d_matrix* U) {
+  // L as CSRL with unit diagonal, U as CSCU (src/parilu_v0_3.cpp:23-27)
+  std::vector<double> lv(nnzL), uv(nnzU);
+  LACE_TRY(values_f64(lv.data(), uv.data()));
+  data_zmfree(L);
+  data_zmfree(U);
+  L->storage_type = Magma_CSRL;
+  L->diagorder_type = Magma_UNITY;
+  L->fill_mode = MagmaLower;
+  L->major = MagmaRowMajor;
+  L->num_rows = L->num_cols = n;
+  L->nnz = L->true_nnz = nnzL;
+  LACE_TRY(host_calloc(&L->row, n + 1));
+  LACE_TRY(host_calloc(&L->col, nnzL));
+  LACE_TRY(host_calloc(&L->val, nnzL));
+  memcpy(L->row, h_lrow.data(), sizeof(int) * (n + 1));
+  memcpy(L->col, h_lcol.data(), sizeof(int) * nnzL);
+  memcpy(L->val, lv.data(), sizeof(double) * nnzL);
+  U->storage_type = Magma_CSCU;
+  U->diagorder_type = Magma_VALUE;
+  U->fill_mode = MagmaUpper;
+  U->major = MagmaColMajor;
+  U->num_rows = U->num_cols = n;
+  U->nnz = U->true_nnz = nnzU;
+  LACE_TRY(host_calloc(&U->row, n + 1));
+  LACE_TRY(host_calloc(&U->col, nnzU));
+  LACE_TRY(host_calloc(&U->val, nnzU));
+  memcpy(U->row, h_ucp.data(), sizeof(int) * (n + 1));
+  memcpy(U->col, h_uri.data(), sizeof(int) * nnzU);
+  memcpy(U->val, uv.data(), sizeof(double) * nnzU);
+  return DEV_SUCCESS;
+}
+
+int ParILU::values_f64(double* lv, double* uv) {
+  if (prec == LACE_F64) {
+    LACE_HIP(hipMemcpyAsync(lv, lval[cur].p, sizeof(double) * nnzL, hipMemcpyDeviceToHost, stream()));
+    LACE_HIP(hipMemcpyAsync(uv, uval[cur].p, sizeof(double) * nnzU, hipMemcpyDeviceToHost, stream()));
+    LACE_HIP(hipStreamSynchronize(stream()));
+  } else {
+    std::vector<float> lf(nnzL), uf(nnzU);
+    LACE_HIP(hipMemcpyAsync(lf.data(), lval[cur].p, sizeof(float) * nnzL, hipMemcpyDeviceToHost, stream()));
+    LACE_HIP(hipMemcpyAsync(uf.data(), uval[cur].p, sizeof(float) * nnzU, hipMemcpyDeviceToHost, stream()));
+    LACE_HIP(hipStreamSynchronize(stream()));
+    for (int k = 0; k < nnzL; k++) lv[k] = lf[k];
+    for (int k = 0; k < nnzU; k++) uv[k] = uf[k];
+  }
+  return DEV_SUCCESS;
+}
+
+int parilu_create(const data_d_matrix* A, lace_prec_t prec, ParILU** out) {
+  LACE_TRY(ensure_init());
+  LACE_REQUIRE(prec == LACE_F64 || prec == LACE_F32, DEV_ERR_NOT_SUPPORTED,
+               "ParILU precision %d not supported", (int)prec);
+  auto* P = new ParILU();
+  P->prec = prec;
+  int rc = build_pattern(A, P);
+  if (rc == DEV_SUCCESS) rc = P->reset();
+  if (rc == DEV_SUCCESS) rc = P->build_trsv();
+  if (rc != DEV_SUCCESS) {
+    delete P;
+    return rc;
+  }
+  *out = P;
+  return DEV_SUCCESS;
+}
+
+}  // namespace lace
+
+using namespace lace;
+
+// ---------------------------------------------------------------- C ABI
+extern "C" {
+
+int lace_parilu_create(const data_d_matrix* A, lace_prec_t prec, lace_parilu** out) {
+  ParILU* P = nullptr;
+  LACE_TRY(parilu_create(A, prec, &P));
+  *out = reinterpret_cast<lace_parilu*>(P);
+  return DEV_SUCCESS;
+}
+
+int lace_parilu_destroy(lace_parilu* P) {
+  delete reinterpret_cast<ParILU*>(P);
+  return DEV_SUCCESS;
+}
+
+int lace_parilu_reset(lace_parilu* P) {
+  LACE_TRY(ensure_init());
+  return reinterpret_cast<ParILU*>(P)->reset();
+}
+
+int lace_parilu_sweeps(lace_parilu* Ph, int nsweeps, double* step) {
+  LACE_TRY(ensure_init());
+  auto* P = reinterpret_cast<ParILU*>(Ph);
+  for (int s = 0; s < nsweeps; s++) LACE_TRY(P->sweep(step != nullptr && s == nsweeps - 1));
+  if (step) {
+    if (nsweeps > 0) LACE_TRY(P->read_step(step));
+    else *step = 0.0;
+  }
+  return DEV_SUCCESS;
+}
+
+int lace_parilu_get(lace_parilu* P, data_d_matrix* L, data_d_matrix* U) {
+  LACE_TRY(ensure_init());
+  return reinterpret_cast<ParILU*>(P)->get(L, U);
+}
+
+int lace_parilu_nonlinres(lace_parilu* P, double* nonlinres) {
+  LACE_TRY(ensure_init());
+  return reinterpret_cast<ParILU*>(P)->nonlinres(nonlinres);
+}
+
+double lace_parilu_sweep_bytes(const lace_parilu* P) {
+  return reinterpret_cast<const ParILU*>(P)->sweep_bytes();
+}
+
+int lace_parilu_levels(const lace_parilu* Ph, int* nlev_L, int* nlev_U) {
+  auto* P = reinterpret_cast<const ParILU*>(Ph);
+  *nlev_L = P->lower.nlev;
+  *nlev_U = P->upper.nlev;
+  return DEV_SUCCESS;
+}
+
+// control/norms.cpp:301-405: res = ||A - LU||_F over LU's pattern,
+// nonlinres = the same restricted to A's pattern.  Host evaluation (a
+// reporting routine called once per factorization).  L: CSRL with unit
+// diagonal (or strictly lower), U: CSR/CSRU or CSC/CSCU.
+int data_zilures(data_d_matrix A, data_d_matrix L, data_d_matrix U, data_d_matrix* LU,
+                 dataType* res, dataType* nonlinres) {
+  data_d_matrix Lr, Ur;
+  memset(&Lr, 0, sizeof(Lr));
+  memset(&Ur, 0, sizeof(Ur));
+  const int n = A.num_rows;
+  // unit-diagonal L in CSR
+  bool strict = true;
+  for (int i = 0; i < n && strict; i++)
+    for (int k = L.row[i]; k < L.row[i + 1]; k++)
+      if (L.col[k] == i) strict = false;
+  if (strict) {
+    const int z = L.row[n] + n;
+    Lr.storage_type = Magma_CSR;
+    Lr.num_rows = Lr.num_cols = n;
+    LACE_TRY(host_calloc(&Lr.row, n + 1));
+    LACE_TRY(host_calloc(&Lr.col, z));
+    LACE_TRY(host_calloc(&Lr.val, z));
+    int p = 0;
+    for (int i = 0; i < n; i++) {
+      Lr.row[i] = p;
+      for (int k = L.row[i]; k < L.row[i + 1]; k++) {
+        Lr.col[p] = L.col[k];
+        Lr.val[p++] = L.val[k];
+      }
+      Lr.col[p] = i;
+      Lr.val[p++] = 1.0;
+    }
+    Lr.row[n] = p;
+    Lr.nnz = p;
+  } else {
+    LACE_TRY(data_zmconvert(L, &Lr, Magma_CSR, Magma_CSR));
+  }
+  if (U.storage_type == Magma_CSC || U.storage_type == Magma_CSCU)
+    LACE_TRY(data_zmconvert(U, &Ur, Magma_CSC, Magma_CSR));
+  else
+    LACE_TRY(data_zmconvert(U, &Ur, Magma_CSR, Magma_CSR));
+  int rc = data_z_spmm(1.0, Lr, Ur, LU);
+  data_zmfree(&Lr);
+  data_zmfree(&Ur);
+  LACE_TRY(rc);
+  double r = 0.0, nl = 0.0;
+  for (int i = 0; i < n; i++) {
+    for (int k = A.row[i]; k < A.row[i + 1]; k++) {
+      // LU rows are sorted: binary search for A's column
+      int lo = LU->row[i], hi = LU->row[i + 1];
+      int c = A.col[k];
+      while (lo < hi) {
+        int mid = (lo + hi) >> 1;
+        if (LU->col[mid] < c) lo = mid + 1;
+        else hi = mid;
+      }
+      if (lo < LU->row[i + 1] && LU->col[lo] == c) {
+        double t = LU->val[lo] - A.val[k];
+        LU->val[lo] = t;
+        nl += t * t;
+      }
+    }
+  }
+  for (int k = 0; k < LU->row[n]; k++) r += LU->val[k] * LU->val[k];
+  *res = std::sqrt(r);
+  *nonlinres = std::sqrt(nl);
+  return DEV_SUCCESS;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------- drop-ins
+namespace {
+
+// Shared body of the four ParILU drop-ins.  mode 0: stop when step <=
+// reduction*Ares or after 20 sweeps (src/parilu_v0_3.cpp:44-45,62);
+// mode 1: stop after log->maxSweeps sweeps or when step <= 4 eps
+// (src/parilu_v0_3_prescribedSweeps.cpp:62).
+int parilu_dropin(data_d_matrix* A, data_d_matrix* L, data_d_matrix* U, double reduction,
+                  data_d_preconditioner_log* log, int mode) {
+  LACE_TRY(ensure_init());
+  L->diagorder_type = Magma_UNITY;
+  LACE_TRY(data_zmconvert(*A, L, Magma_CSR, Magma_CSRL));
+  U->diagorder_type = Magma_VALUE;
+  LACE_TRY(data_zmconvert(*A, U, Magma_CSR, Magma_CSCU));
+  double Ares = 0.0, Anl = 0.0;
+  data_d_matrix LU;
+  memset(&LU, 0, sizeof(LU));
+  LACE_TRY(data_zilures(*A, *L, *U, &LU, &Ares, &Anl));
+  data_zmfree(&LU);
+  double anorm = 0.0;
+  LACE_TRY(data_zfrobenius(*A, &anorm));
+  LACE_TRY(data_rowindex(A, &A->rowidx));
+  ParILU* P = nullptr;
+  auto t0 = std::chrono::steady_clock::now();
+  LACE_TRY(parilu_create(A, LACE_F64, &P));
+  const double tol = reduction * Ares;
+  int iter = 0;
+  double step = 3.402823466e+38;  // FLT_MAX, as the reference starts
+  int rc = DEV_SUCCESS;
+  while (rc == DEV_SUCCESS) {
+    if (mode == 0 && !(step > tol && iter < 20)) break;
+    if (mode == 1 && !(iter < log->maxSweeps && step > 4.0 * 2.220446049250313e-16)) break;
+    rc = P->sweep(true);
+    if (rc == DEV_SUCCESS) rc = P->read_step(&step);
+    iter++;
+  }
+  if (rc == DEV_SUCCESS) rc = P->get(L, U);
+  auto t1 = std::chrono::steady_clock::now();
+  delete P;
+  LACE_TRY(rc);
+  log->sweeps = iter;
+  log->tol = tol;
+  log->finalStep = step;
+  log->A_Frobenius = anorm;
+  log->precond_generation_time = std::chrono::duration<double>(t1 - t0).count();
+  log->initial_residual = Ares;
+  log->initial_nonlinear_residual = Anl;
+  log->omp_num_threads = 1;
+  return DEV_SUCCESS;
+}
+
+}  // namespace
+
+extern "C" {
+
+void data_PariLU_v0_3(data_d_matrix* A, data_d_matrix* L, data_d_matrix* U, dataType reduction,
+                      data_d_preconditioner_log* log) {
+  if (parilu_dropin(A, L, U, reduction, log, 0) != DEV_SUCCESS) log->sweeps = -1;
+}
+
+void data_PariLU_v0_3_prescribedSweeps(data_d_matrix* A, data_d_matrix* L, data_d_matrix* U,
+                                       dataType reduction, data_d_preconditioner_log* log) {
+  if (parilu_dropin(A, L, U, reduction, log, 1) != DEV_SUCCESS) log->sweeps = -1;
+}
+
+// nthreads was the CUDA block size of the single-block reference kernel
+// (src/parilu_v0_3_gpu.cpp:203-209); the grid here is sized by nnz.
+void data_PariLU_v0_3_gpu(data_d_matrix* A, data_d_matrix* L, data_d_matrix* U, dataType reduction,
+                          data_d_preconditioner_log* log, int nthreads) {
+  (void)nthreads;
+  if (parilu_dropin(A, L, U, reduction, log, 0) != DEV_SUCCESS) log->sweeps = -1;
+}
+
+void data_PariLU_v0_3_gpu_prescribedSweeps(data_d_matrix* A, data_d_matrix* L, data_d_matrix* U,
+                                           dataType reduction, data_d_preconditioner_log* log) {
+  if (parilu_dropin(A, L, U, reduction, log, 1) != DEV_SUCCESS) log->sweeps = -1;
+}
+
+}  // extern "C"

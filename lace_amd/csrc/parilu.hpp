@@ -1,0 +1,82 @@
+// parilu.hpp -- device handles shared by the ParILU, triangular-solve and
+// Krylov translation units.
+#pragma once
+
+#include <vector>
+
+#include "common.hpp"
+
+namespace lace {
+
+// Pattern of a triangular factor in CSR with its level schedule.
+//   upper = false: diagonal last in each row (CSRL layout)
+//   upper = true : diagonal first in each row (CSRU layout)
+// Level sets: lev(i) = 1 + max lev(k) over the off-diagonal entries of row
+// i.  `sched` lists rows level by level; each level starts on a multiple of
+// 64 slots (padding slots hold -1) so that one wavefront never holds two
+// levels -- the sync-free kernel relies on it.
+struct TriPlan {
+  int n = 0, nnz = 0;
+  bool upper = false;
+  int nlev = 0;
+  int nslots = 0;
+  DevBuf row, col;
+  DevBuf perm, lptr;        // rows by level, level pointers (unpadded)
+  DevBuf sched;             // padded schedule for the sync-free kernel
+  std::vector<int> h_lptr;  // level pointers on the host
+  int build(int n, const int* row, const int* col, bool upper);
+};
+
+// y = T^{-1} r, exact.  Level-scheduled: one launch per level.
+template <typename T>
+int tri_solve_levels(const TriPlan& P, const T* val, const T* r, T* y);
+// y = T^{-1} r, exact, one launch: rows run in level order and wait on the
+// rows they depend on (value slots are pre-filled with a NaN sentinel).
+template <typename T>
+int tri_solve_syncfree(const TriPlan& P, const T* val, const T* r, T* y, int* err);
+// `sweeps` synchronous Jacobi sweeps from y = 0; tmp is a scratch vector.
+template <typename T>
+int tri_solve_jacobi(const TriPlan& P, const T* val, const T* r, T* y, T* tmp, int sweeps);
+
+// Device ParILU factors (see parilu.hip for the layout).
+struct ParILU {
+  int n = 0, nnz = 0, nnzL = 0, nnzU = 0;
+  lace_prec_t prec = LACE_F64;
+  double anorm = 0.0;
+  DevBuf arow, acol, arowidx, aval, dst;
+  DevBuf lrow, lcol, lval[2];
+  DevBuf ucp, uri, uval[2];
+  int cur = 0;
+  DevBuf partial, scalar;
+  std::vector<int> h_lrow, h_lcol, h_ucp, h_uri;
+
+  // Triangular solves.  lower: L (CSRL arrays, values lval[cur]).
+  // upper: U in CSR (diagonal first), values gathered from the CSC buffer
+  // by u_csr_from_csc after every factor update.  Transposed solves:
+  // lowerT = U^T, whose CSR is U's CSC (diagonal last, values uval[cur]);
+  // upperT = L^T (diagonal first), values gathered through l_t_from_csr.
+  TriPlan lower, upper, lowerT, upperT;
+  DevBuf u_csr_from_csc, uval_csr;
+  DevBuf l_t_from_csr, lval_t;
+  bool csr_dirty = true;
+  DevBuf work, work2;  // scratch vectors for the solves (n elements each)
+  DevBuf syncfree_err;
+
+  size_t elem_size() const { return prec == LACE_F64 ? 8 : 4; }
+  int reset();
+  int sweep(bool with_step);
+  int read_step(double* step);
+  int nonlinres(double* out);
+  double sweep_bytes() const;
+  int get(data_d_matrix* L, data_d_matrix* U);
+  int values_f64(double* lv, double* uv);
+  int build_trsv();
+  int refresh_csr();
+  // z = M^{-1} r (trans = 0) or M^{-T} r (trans = 1); r and z in the
+  // factor precision (double for F64, float otherwise).
+  int apply(lace_trsv_kind_t kind, int sweeps, int trans, const void* r, void* z);
+};
+
+int parilu_create(const data_d_matrix* A, lace_prec_t prec, ParILU** out);
+
+}  // namespace lace

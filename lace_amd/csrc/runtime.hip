@@ -1,0 +1,197 @@
+// runtime.hip -- library initialisation, error text, device memory entry
+// points and the kernel-timing registry.
+#include <cstdarg>
+#include <map>
+#include <mutex>
+
+#include "common.hpp"
+
+namespace lace {
+
+static thread_local char g_err[1024] = {0};
+
+void set_error(const char* fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_err, sizeof(g_err), fmt, ap);
+  va_end(ap);
+}
+const char* get_error() { return g_err; }
+
+static std::mutex g_init_mu;
+static hipStream_t g_stream = nullptr;
+static int g_device = -1;
+static int g_cus = 0;
+
+static int init_device(int dev) {
+  std::lock_guard<std::mutex> lk(g_init_mu);
+  if (g_stream && g_device == dev) return DEV_SUCCESS;
+  int count = 0;
+  hipError_t e = hipGetDeviceCount(&count);
+  if (e != hipSuccess || count == 0) {
+    set_error("no HIP device available (%s)", hipGetErrorString(e));
+    return DEV_ERR_NOT_INITIALIZED;
+  }
+  if (dev < 0 || dev >= count) {
+    set_error("device %d out of range (%d devices)", dev, count);
+    return DEV_ERR_ILLEGAL_VALUE;
+  }
+  LACE_HIP(hipSetDevice(dev));
+  hipDeviceProp_t prop;
+  LACE_HIP(hipGetDeviceProperties(&prop, dev));
+  if (strncmp(prop.gcnArchName, "gfx950", 6) != 0) {
+    set_error("device %d is %s; this library is built for gfx950 only", dev,
+              prop.gcnArchName);
+    return DEV_ERR_NOT_SUPPORTED;
+  }
+  if (g_stream) {
+    (void)hipStreamDestroy(g_stream);
+    g_stream = nullptr;
+  }
+  LACE_HIP(hipStreamCreateWithFlags(&g_stream, hipStreamNonBlocking));
+  g_device = dev;
+  g_cus = prop.multiProcessorCount;
+  return DEV_SUCCESS;
+}
+
+int ensure_init() {
+  if (g_stream) {
+    LACE_HIP(hipSetDevice(g_device));
+    return DEV_SUCCESS;
+  }
+  return init_device(0);
+}
+hipStream_t stream() { return g_stream; }
+int device_id() { return g_device; }
+int num_cus() { return g_cus; }
+
+int upload(DevBuf& b, const void* host, size_t bytes) {
+  LACE_TRY(b.alloc(bytes));
+  if (bytes) {
+    LACE_HIP(hipMemcpyAsync(b.p, host, bytes, hipMemcpyHostToDevice, stream()));
+    LACE_HIP(hipStreamSynchronize(stream()));
+  }
+  return DEV_SUCCESS;
+}
+
+int download(void* host, const DevBuf& b, size_t bytes) {
+  if (bytes) {
+    LACE_HIP(hipMemcpyAsync(host, b.p, bytes, hipMemcpyDeviceToHost, stream()));
+    LACE_HIP(hipStreamSynchronize(stream()));
+  }
+  return DEV_SUCCESS;
+}
+
+// ---------------------------------------------------------------- timing
+struct TimingEntry {
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> pending;
+  double total_ms = 0.0;
+  long count = 0;
+};
+static std::mutex g_tmu;
+static std::map<std::string, TimingEntry> g_timing;
+static bool g_timing_on = false;
+
+bool timing_on() { return g_timing_on; }
+
+TimedRegion::TimedRegion(const char* n) : name(n) {
+  if (!g_timing_on) return;
+  if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) return;
+  on = hipEventRecord(a, stream()) == hipSuccess;
+}
+
+TimedRegion::~TimedRegion() {
+  if (!on) return;
+  (void)hipEventRecord(b, stream());
+  std::lock_guard<std::mutex> lk(g_tmu);
+  g_timing[name].pending.emplace_back(a, b);
+}
+
+static void drain_timing() {
+  std::lock_guard<std::mutex> lk(g_tmu);
+  for (auto& kv : g_timing) {
+    for (auto& ev : kv.second.pending) {
+      (void)hipEventSynchronize(ev.second);
+      float ms = 0.f;
+      if (hipEventElapsedTime(&ms, ev.first, ev.second) == hipSuccess) {
+        kv.second.total_ms += ms;
+        kv.second.count++;
+      }
+      (void)hipEventDestroy(ev.first);
+      (void)hipEventDestroy(ev.second);
+    }
+    kv.second.pending.clear();
+  }
+}
+
+}  // namespace lace
+
+using namespace lace;
+
+extern "C" {
+
+int lace_init(int device) { return init_device(device); }
+
+const char* lace_last_error(void) { return get_error(); }
+
+int lace_device_synchronize(void) {
+  LACE_TRY(ensure_init());
+  LACE_HIP(hipStreamSynchronize(stream()));
+  return DEV_SUCCESS;
+}
+
+int lace_dmalloc(void** ptr, size_t bytes) {
+  LACE_TRY(ensure_init());
+  LACE_REQUIRE(ptr != nullptr, DEV_ERR_INVALID_PTR, "lace_dmalloc: null out pointer");
+  LACE_HIP(hipMalloc(ptr, bytes ? bytes : 256));
+  return DEV_SUCCESS;
+}
+
+int lace_dfree(void* ptr) {
+  LACE_TRY(ensure_init());
+  if (ptr) LACE_HIP(hipFree(ptr));
+  return DEV_SUCCESS;
+}
+
+int lace_memcpy_h2d(void* dst, const void* src, size_t bytes) {
+  LACE_TRY(ensure_init());
+  LACE_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, stream()));
+  LACE_HIP(hipStreamSynchronize(stream()));
+  return DEV_SUCCESS;
+}
+
+int lace_memcpy_d2h(void* dst, const void* src, size_t bytes) {
+  LACE_TRY(ensure_init());
+  LACE_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, stream()));
+  LACE_HIP(hipStreamSynchronize(stream()));
+  return DEV_SUCCESS;
+}
+
+int lace_timing_enable(int on) {
+  LACE_TRY(ensure_init());
+  g_timing_on = on != 0;
+  return DEV_SUCCESS;
+}
+
+int lace_timing_reset(void) {
+  drain_timing();
+  std::lock_guard<std::mutex> lk(g_tmu);
+  g_timing.clear();
+  return DEV_SUCCESS;
+}
+
+int lace_timing_get(const char* name, double* total_ms, long* count) {
+  drain_timing();
+  std::lock_guard<std::mutex> lk(g_tmu);
+  auto it = g_timing.find(name);
+  if (it == g_timing.end()) {
+    *total_ms = 0.0;
+    *count = 0;
+    return DEV_ERR_NOT_FOUND;
+  }
+  *total_ms = it->second.total_ms;
+  *count = it->second.count;
+  return DEV_SUCCESS;
+}
+
+}  // extern "C"

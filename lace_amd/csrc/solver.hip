@@ -1,0 +1,676 @@
+// solver.hip -- ILU-preconditioned FGMRES solver handles (FP64 and mixed
+// precision), the adjoint gradient gather, and the drop-in FGMRES entry
+// points.
+//
+// Precision layouts:
+//   FP64 path   : A, factors, triangular solves, SpMV, reflectors all FP64.
+//   mixed path  : the outer FGMRES (reflectors, Hessenberg/Givens, least
+//                 squares, the Arnoldi/true-residual SpMV with A in FP64) is
+//                 FP64; the preconditioner runs in FP32 -- FP32 factors and
+//                 triangular solves, and with inner_iters > 0 an inner FP32
+//                 Householder GMRES on an FP32 copy of A.  The outer FGMRES
+//                 therefore acts as flexible iterative refinement of the FP32
+//                 solve; preconditioned vectors are stored in FP32 and the
+//                 outer SpMV reads them widened, so the Arnoldi relation holds
+//                 exactly for the stored vectors.
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <vector>
+
+#include "common.hpp"
+#include "device.hpp"
+#include "krylov.hpp"
+
+namespace lace {
+
+template <typename K>
+int hh_start(HHEngine<K>& E, int set_rtol, double rtol_rel);
+template <typename K>
+int hh_step(HHEngine<K>& E, int j, bool write_q);
+template <typename K, typename XT>
+int hh_finish(HHEngine<K>& E, XT* x);
+template <typename K>
+int read_misc(HHEngine<K>& E);
+int norm2_sq_partials(int n, int rpb, const double* x, double* partial, int pstride, int nb);
+int sum_slot(int nb, int pstride, const double* partial, double* out);
+int cast_d2f(int n, const double* in, float* out);
+int cast_f2d(int n, const float* in, double* out);
+int set_misc_d(double* small, SmallLayout lay, int slot, double v);
+
+// ------------------------------------------------------------ preconditioners
+struct PrecILU : Precond {
+  ParILU* P;
+  lace_trsv_kind_t kind;
+  int sweeps, trans;
+  DevBuf qf;
+  PrecILU(ParILU* p, lace_trsv_kind_t k, int s, int t) : P(p), kind(k), sweeps(s), trans(t) {}
+  bool z_is_float() const override { return P->prec != LACE_F64; }
+  int apply_d(const double* q, void* z) override {
+    TimedRegion tr("precond");
+    if (P->prec == LACE_F64) return P->apply(kind, sweeps, trans, q, z);
+    if (qf.bytes < sizeof(float) * P->n) LACE_TRY(qf.alloc(sizeof(float) * P->n));
+    LACE_TRY(cast_d2f(P->n, q, qf.as<float>()));
+    return P->apply(kind, sweeps, trans, qf.as<float>(), z);
+  }
+  int apply_f(const float* q, void* z) override {
+    TimedRegion tr("precond");
+    LACE_REQUIRE(P->prec != LACE_F64, DEV_ERR_NOT_SUPPORTED, "FP32 input to FP64 factors");
+    return P->apply(kind, sweeps, trans, q, z);
+  }
+};
+
+struct PrecIdentity : Precond {
+  int n;
+  explicit PrecIdentity(int n_) : n(n_) {}
+  bool z_is_float() const override { return false; }
+  int apply_d(const double* q, void* z) override {
+    LACE_HIP(hipMemcpyAsync(z, q, sizeof(double) * n, hipMemcpyDeviceToDevice, stream()));
+    return DEV_SUCCESS;
+  }
+};
+
+// Inner FP32 Householder GMRES with a fixed FP32 ILU preconditioner,
+// `iters` steps from x = 0, no host synchronisation.  Output in FP32.
+struct PrecInner : Precond {
+  const DevCSR* A32;
+  PrecILU* M;
+  HHEngine<float> E;
+  int iters;
+  double rtol;
+  PrecInner(const DevCSR* a, PrecILU* m, int it, double rt) : A32(a), M(m), iters(it), rtol(rt) {}
+  int init() { return E.init(A32->n, iters, true); }
+  bool z_is_float() const override { return true; }
+  int apply_d(const double* q, void* z) override {
+    TimedRegion tr("inner_gmres");
+    const int n = A32->n;
+    LACE_TRY(cast_d2f(n, q, E.r.as<float>()));
+    LACE_HIP(hipMemsetAsync(z, 0, sizeof(float) * n, stream()));
+    LACE_TRY(hh_start<float>(E, 1, rtol));
+    for (int j = 0; j < iters; j++) {
+      LACE_TRY(M->apply_f(E.q.as<float>(), E.zcol(j)));
+      LACE_TRY((spmv<float, float>(*A32, 1.0, static_cast<const float*>(E.zcol(j)), 0.0,
+                                   E.w.as<float>())));
+      LACE_TRY(hh_step<float>(E, j, j < iters - 1));
+    }
+    return hh_finish<float, float>(E, static_cast<float*>(z));
+  }
+};
+
+// ------------------------------------------------------------ solver
+struct Solver {
+  lace_solver_opts o;
+  int n = 0;
+  DevCSR A64;          // operator (A, or A^T when transposed)
+  DevCSR A32;          // FP32 copy for the inner solver
+  DevCSR Apat;         // A itself (for the gradient pattern)
+  ParILU* P = nullptr;
+  std::unique_ptr<PrecILU> ilu;
+  std::unique_ptr<PrecIdentity> ident;
+  std::unique_ptr<PrecInner> inner;
+  Precond* M = nullptr;
+  HHEngine<double> E;
+  DevBuf rbuf, partial, scal;
+  int nbn = 0, rpb = 4096, pstride = 4;
+  double* host_scal = nullptr;
+  ~Solver() {
+    delete P;
+    if (host_scal) (void)hipHostFree(host_scal);
+  }
+};
+
+static int transpose_host(const data_d_matrix* A, data_d_matrix* T) {
+  memset(T, 0, sizeof(*T));
+  return data_zmtranspose(*A, T);
+}
+
+static int solver_create(const data_d_matrix* A, const lace_solver_opts* o, Solver** out) {
+  LACE_TRY(ensure_init());
+  auto S = std::make_unique<Solver>();
+  S->o = *o;
+  S->n = A->num_rows;
+  const bool tr = o->transpose != 0;
+  if (tr) {
+    data_d_matrix AT;
+    LACE_TRY(transpose_host(A, &AT));
+    AT.storage_type = Magma_CSR;
+    int rc = S->A64.create(&AT, LACE_F64);
+    if (rc == DEV_SUCCESS && o->inner_iters > 0) rc = S->A32.create(&AT, LACE_F32);
+    data_zmfree(&AT);
+    LACE_TRY(rc);
+  } else {
+    LACE_TRY(S->A64.create(A, LACE_F64));
+    if (o->inner_iters > 0) LACE_TRY(S->A32.create(A, LACE_F32));
+  }
+  LACE_TRY(S->Apat.create(A, LACE_F64));
+  if (o->trsv == LACE_TRSV_NONE && o->inner_iters <= 0) {
+    S->ident = std::make_unique<PrecIdentity>(S->n);
+    S->M = S->ident.get();
+  } else {
+    const lace_prec_t fp = o->inner_iters > 0 ? LACE_F32 : o->prec_precond;
+    LACE_TRY(parilu_create(A, fp, &S->P));
+    S->ilu = std::make_unique<PrecILU>(S->P, o->trsv, o->jacobi_sweeps, tr ? 1 : 0);
+    S->M = S->ilu.get();
+    if (o->inner_iters > 0) {
+      S->inner = std::make_unique<PrecInner>(&S->A32, S->ilu.get(), o->inner_iters, 0.0);
+      LACE_TRY(S->inner->init());
+      S->M = S->inner.get();
+    }
+  }
+  LACE_TRY(S->E.init(S->n, o->search_max, S->M->z_is_float()));
+  LACE_TRY(S->rbuf.alloc(sizeof(double) * S->n));
+  S->nbn = ceil_div(S->n, S->rpb);
+  LACE_TRY(S->partial.alloc(sizeof(double) * S->nbn * S->pstride));
+  LACE_TRY(S->scal.alloc(sizeof(double) * 4));
+  LACE_HIP(hipHostMalloc((void**)&S->host_scal, sizeof(double) * 4, hipHostMallocDefault));
+  LACE_HIP(hipStreamSynchronize(stream()));
+  *out = S.release();
+  return DEV_SUCCESS;
+}
+
+static int solver_setup(Solver* S) {
+  if (!S->P) return DEV_SUCCESS;
+  TimedRegion tr("setup");
+  LACE_TRY(S->P->reset());
+  for (int s = 0; s < S->o.parilu_sweeps; s++) LACE_TRY(S->P->sweep(false));
+  S->P->csr_dirty = true;
+  LACE_TRY(S->P->refresh_csr());
+  return DEV_SUCCESS;
+}
+
+static int norm2(Solver* S, const double* x, double* out) {
+  LACE_TRY(norm2_sq_partials(S->n, S->rpb, x, S->partial.as<double>(), S->pstride, S->nbn));
+  LACE_TRY(sum_slot(S->nbn, S->pstride, S->partial.as<double>(), S->scal.as<double>()));
+  LACE_HIP(hipMemcpyAsync(S->host_scal, S->scal.p, sizeof(double), hipMemcpyDeviceToHost, stream()));
+  LACE_HIP(hipStreamSynchronize(stream()));
+  *out = std::sqrt(S->host_scal[0]);
+  return DEV_SUCCESS;
+}
+
+// One solve to rtol (relative to ||b - A x0||) with restarts.
+// tol_abs >= 0 selects an absolute tolerance instead.
+static int solver_solve(Solver* S, const double* b, double* x, lace_solver_stats* st, double tol_abs) {
+  auto t0 = std::chrono::steady_clock::now();
+  HHEngine<double>& E = S->E;
+  const int m = S->o.search_max;
+  lace_solver_stats local;
+  memset(&local, 0, sizeof(local));
+  bool converged = false;
+  double rtol_abs = 0.0;
+  for (int restart = 0; restart < S->o.restart_max && !converged; restart++) {
+    LACE_TRY((residual<double, double>(S->A64, x, b, E.r.as<double>())));
+    if (restart == 0) {
+      if (tol_abs >= 0.0)
+        LACE_TRY(set_misc_d(E.small.as<double>(), E.lay, MS_RTOL, tol_abs));
+      LACE_TRY(hh_start<double>(E, tol_abs >= 0.0 ? 0 : 1, S->o.rtol));
+    } else {
+      LACE_TRY(hh_start<double>(E, 0, 0.0));
+    }
+    LACE_TRY(read_misc<double>(E));
+    const double rnorm = E.host_misc[MS_RNORM];
+    if (restart == 0) {
+      local.initial_residual = rnorm;
+      rtol_abs = E.host_misc[MS_RTOL];
+    }
+    local.final_residual_est = rnorm;
+    if (!std::isfinite(rnorm)) break;
+    if (E.host_misc[MS_DONE] != 0.0) {
+      converged = rnorm < rtol_abs;
+      break;
+    }
+    for (int j = 0; j < m; j++) {
+      LACE_TRY(S->M->apply_d(E.q.as<double>(), E.zcol(j)));
+      if (E.z_float)
+        LACE_TRY((spmv<float, double>(S->A64, 1.0, static_cast<const float*>(E.zcol(j)), 0.0,
+                                      E.w.as<double>())));
+      else
+        LACE_TRY((spmv<double, double>(S->A64, 1.0, static_cast<const double*>(E.zcol(j)), 0.0,
+                                       E.w.as<double>())));
+      LACE_TRY(hh_step<double>(E, j, j < m - 1));
+      LACE_TRY(read_misc<double>(E));
+      if (E.host_misc[MS_DONE] != 0.0) break;
+    }
+    LACE_TRY((hh_finish<double, double>(E, x)));
+    LACE_TRY(read_misc<double>(E));
+    const int nc = (int)E.host_misc[MS_NCOLS];
+    local.iterations += nc;
+    local.restarts = restart;
+    local.final_residual_est = E.host_misc[MS_RES];
+    if (E.host_misc[MS_BREAK] != 0.0) break;
+    converged = E.host_misc[MS_RES] < rtol_abs;
+  }
+  if (S->inner) local.inner_iterations = local.iterations * S->o.inner_iters;
+  // true residual in FP64 (against the operator in use)
+  LACE_TRY((residual<double, double>(S->A64, x, b, S->rbuf.as<double>())));
+  LACE_TRY(norm2(S, S->rbuf.as<double>(), &local.true_residual));
+  double bn = 0.0;
+  LACE_TRY(norm2(S, b, &bn));
+  local.rel_true_residual = bn > 0.0 ? local.true_residual / bn : local.true_residual;
+  local.converged = converged ? 1 : 0;
+  auto t1 = std::chrono::steady_clock::now();
+  local.solve_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
+  if (st) *st = local;
+  return DEV_SUCCESS;
+}
+
+// dtheta[k] = -lambda[i] * x[col[k]] for the CSR entries k of row i.
+__global__ void grad_gather_kernel(int n, const int* __restrict__ row, const int* __restrict__ col,
+                                   const double* __restrict__ lambda, const double* __restrict__ x,
+                                   double* __restrict__ dtheta) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double li = lambda[i];
+  for (int k = row[i]; k < row[i + 1]; k++) dtheta[k] = -li * x[col[k]];
+}
+
+
+// ------------------------------------------------------------ MGS FGMRES
+// src/fgmres.cpp:64-517 and src/fgmres_restart.cpp: modified Gram-Schmidt
+// Arnoldi with the Brown/Hindmarsh reorthogonalisation test (:314-338),
+// Givens rotations in the form of :385-411, convergence on |g_{j+1}|.
+// Vector passes run on the device; the scalar recurrences run in one-thread
+// kernels on a small device state, so the only host reads are the
+// per-step convergence/reorthogonalisation flags.
+struct MgsState {  // device doubles
+  int m;
+  __host__ __device__ int H() const { return 0; }                        // (m+1) x m
+  __host__ __device__ int cs() const { return (m + 1) * m; }
+  __host__ __device__ int sn() const { return cs() + m; }
+  __host__ __device__ int g() const { return sn() + m; }
+  __host__ __device__ int y() const { return g() + m + 1; }
+  __host__ __device__ int sc() const { return y() + m + 1; }  // scratch scalars
+  __host__ __device__ int total() const { return sc() + 16; }
+};
+enum { MG_H = 0, MG_NORMAV = 1, MG_NORM2 = 2, MG_RES = 3, MG_DONE = 4, MG_REORTH = 5,
+       MG_NC = 6, MG_RTOL = 7, MG_RNORM = 8, MG_ALPHA = 9 };
+
+__global__ __launch_bounds__(256) void dot_partial_kernel(int n, int rpb, const double* __restrict__ a,
+                                                          const double* __restrict__ b,
+                                                          double* __restrict__ partial) {
+  const int blk = xcd_block(blockIdx.x, gridDim.x);
+  const int r0 = blk * rpb, r1 = min(r0 + rpb, n);
+  double v = 0.0;
+  for (int i = r0 + threadIdx.x; i < r1; i += 256) v += a[i] * b[i];
+  __shared__ double sm[4];
+  v = block_sum<256>(v, sm);
+  if (threadIdx.x == 0) partial[blk] = v;
+}
+
+__global__ __launch_bounds__(256) void sum_to_slot_kernel(int nb, const double* partial, double* dst,
+                                                          int take_sqrt) {
+  __shared__ double sm[4];
+  double v = 0.0;
+  for (int t = threadIdx.x; t < nb; t += 256) v += partial[t];
+  v = block_sum<256>(v, sm);
+  if (threadIdx.x == 0) *dst = take_sqrt ? sqrt(v) : v;
+}
+
+// y <- y - (*alpha) * x   or  y <- y + (*alpha) * x (sign = -1 / +1)
+__global__ void axpy_dev_kernel(int n, const double* __restrict__ alpha, double sign,
+                                const double* __restrict__ x, double* __restrict__ y) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) y[i] = y[i] + sign * (*alpha) * x[i];
+}
+
+__global__ void store_scalar_kernel(double* p, double v) { *p = v; }
+
+__global__ void h_accum_kernel(double* h, const double* v, int set) {
+  *h = set ? *v : *h + *v;
+}
+
+__global__ void scale_into_kernel(int n, const double* __restrict__ u, const double* __restrict__ h,
+                                  const double* __restrict__ flag, double* __restrict__ v) {
+  if (*flag == 0.0) return;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) v[i] = u[i] / (*h);
+}
+
+// After the MGS pass of step j: reorthogonalisation decision, breakdown
+// test, Givens rotations, convergence.  sc[MG_H] holds the last dot.
+__global__ void mgs_step_small_kernel(double* st, MgsState L, int j, int last, int stage) {
+  if (threadIdx.x != 0) return;
+  double* H = st + L.H();
+  double* sc = st + L.sc();
+  const int ldH = L.m + 1;
+  if (stage == 0) {  // after the first norm of u: reorthogonalisation test
+    const double normav = sc[MG_NORMAV], normav2 = sc[MG_NORM2];
+    H[j + 1 + j * ldH] = normav2;
+    const double hr = (normav + 0.001 * normav2) - normav;
+    sc[MG_REORTH] = (hr <= 4.9406564584124654e-324) ? 1.0 : 0.0;
+    return;
+  }
+  if (stage == 1) {  // after a reorthogonalisation pass
+    H[j + 1 + j * ldH] = sc[MG_NORM2];
+    return;
+  }
+  // stage 2: breakdown flag, rotations, convergence
+  double* cs = st + L.cs();
+  double* sn = st + L.sn();
+  double* g = st + L.g();
+  const double hj1 = H[j + 1 + j * ldH];
+  sc[MG_ALPHA] = fabs(hj1) > 2.220446049250313e-16 ? 1.0 : 0.0;
+  for (int i = 0; i < j; i++) {
+    const double delta = H[i + j * ldH];
+    H[i + j * ldH] = cs[i] * delta - sn[i] * H[i + 1 + j * ldH];
+    H[i + 1 + j * ldH] = sn[i] * delta + cs[i] * H[i + 1 + j * ldH];
+  }
+  const double a = H[j + j * ldH], bb = H[j + 1 + j * ldH];
+  const double gamma = sqrt(a * a + bb * bb);
+  if (gamma > 0.0) {
+    cs[j] = a / gamma;
+    sn[j] = -bb / gamma;
+    H[j + j * ldH] = cs[j] * a - sn[j] * bb;
+    H[j + 1 + j * ldH] = 0.0;
+    const double delta = g[j];
+    g[j] = cs[j] * delta - sn[j] * g[j + 1];
+    g[j + 1] = sn[j] * delta + cs[j] * g[j + 1];
+  }
+  const double res = fabs(g[j + 1]);
+  sc[MG_RES] = res;
+  sc[MG_NC] = j + 1;
+  sc[MG_DONE] = (res < sc[MG_RTOL] || last || !isfinite(res)) ? 1.0 : 0.0;
+}
+
+__global__ void mgs_lsq_kernel(double* st, MgsState L) {
+  if (threadIdx.x != 0) return;
+  const double* H = st + L.H();
+  const double* g = st + L.g();
+  double* y = st + L.y();
+  const int ldH = L.m + 1, nc = (int)st[L.sc() + MG_NC];
+  for (int i = 0; i < nc; i++) y[i] = g[i] / H[i + i * ldH];
+  for (int jj = nc - 1; jj > 0; jj--)
+    for (int i = jj - 1; i > -1; i--) y[i] = y[i] - H[i + jj * ldH] * y[jj] / H[i + i * ldH];
+}
+
+__global__ void mgs_start_kernel(double* st, MgsState L, int set_rtol, double rtol) {
+  if (threadIdx.x != 0) return;
+  double* sc = st + L.sc();
+  const double rn = sc[MG_RNORM];
+  if (set_rtol) sc[MG_RTOL] = rtol * rn;
+  for (int t = 0; t < L.total() - 16; t++) st[t] = 0.0;
+  st[L.g()] = rn;
+  sc[MG_DONE] = 0.0;
+}
+
+__global__ void mgs_xupdate_kernel(int n, int ld, const double* __restrict__ Z, const double* st,
+                                   MgsState L, double* __restrict__ x) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int nc = (int)st[L.sc() + MG_NC];
+  const double* y = st + L.y();
+  double t = 0.0;
+  for (int k = 0; k < nc; k++) t = t + Z[(size_t)k * ld + i] * y[k];
+  x[i] = x[i] + t;
+}
+
+static int solver_solve_mgs(Solver* S, const double* b, double* x, lace_solver_stats* out,
+                            double tol_abs, int reorth) {
+  auto t0 = std::chrono::steady_clock::now();
+  const int n = S->n, m = S->o.search_max, ld = ((n + 63) / 64) * 64;
+  const int rpb = 4096, nb = ceil_div(n, rpb), nbv = ceil_div(n, 256);
+  MgsState L{m};
+  DevBuf V, Z, u, st, part;
+  LACE_TRY(V.alloc(sizeof(double) * (size_t)ld * (m + 1)));
+  LACE_TRY(Z.alloc(sizeof(double) * (size_t)ld * m));
+  LACE_TRY(u.alloc(sizeof(double) * ld));
+  LACE_TRY(st.alloc(sizeof(double) * L.total()));
+  LACE_TRY(part.alloc(sizeof(double) * nb));
+  LACE_HIP(hipMemsetAsync(st.p, 0, sizeof(double) * L.total(), stream()));
+  double* sd = st.as<double>();
+  double* sc = sd + L.sc();
+  double hostsc[16];
+  auto Vc = [&](int k) { return V.as<double>() + (size_t)k * ld; };
+  auto Zc = [&](int k) { return Z.as<double>() + (size_t)k * ld; };
+  auto nrm_into = [&](const double* a, double* dst) -> int {
+    dot_partial_kernel<<<nb, 256, 0, stream()>>>(n, rpb, a, a, part.as<double>());
+    sum_to_slot_kernel<<<1, 256, 0, stream()>>>(nb, part.as<double>(), dst, 1);
+    LACE_HIP(hipGetLastError());
+    return DEV_SUCCESS;
+  };
+  auto read_sc = [&]() -> int {
+    LACE_HIP(hipMemcpyAsync(hostsc, sc, sizeof(hostsc), hipMemcpyDeviceToHost, stream()));
+    LACE_HIP(hipStreamSynchronize(stream()));
+    return DEV_SUCCESS;
+  };
+  lace_solver_stats local;
+  memset(&local, 0, sizeof(local));
+  bool converged = false;
+  double rtol_abs = 0.0;
+  if (tol_abs >= 0.0) {
+    store_scalar_kernel<<<1, 1, 0, stream()>>>(sc + MG_RTOL, tol_abs);
+    LACE_HIP(hipGetLastError());
+  }
+  for (int restart = 0; restart < S->o.restart_max && !converged; restart++) {
+    double* r = Vc(0);
+    LACE_TRY((residual<double, double>(S->A64, x, b, r)));
+    LACE_TRY(nrm_into(r, sc + MG_RNORM));
+    mgs_start_kernel<<<1, 64, 0, stream()>>>(sd, L, restart == 0 && tol_abs < 0.0, S->o.rtol);
+    LACE_TRY(read_sc());
+    if (restart == 0) {
+      local.initial_residual = hostsc[MG_RNORM];
+      rtol_abs = hostsc[MG_RTOL];
+    }
+    if (!(hostsc[MG_RNORM] >= rtol_abs)) {
+      converged = std::isfinite(hostsc[MG_RNORM]);
+      local.final_residual_est = hostsc[MG_RNORM];
+      break;
+    }
+    // v_0 = r / ||r||
+    scale_into_kernel<<<nbv, 256, 0, stream()>>>(n, r, sc + MG_RNORM, sc + MG_RNORM, Vc(0));
+    for (int j = 0; j < m; j++) {
+      LACE_TRY(S->M->apply_d(Vc(j), Zc(j)));
+      LACE_TRY((spmv<double, double>(S->A64, 1.0, Zc(j), 0.0, u.as<double>())));
+      LACE_TRY(nrm_into(u.as<double>(), sc + MG_NORMAV));
+      const int passes = 1;
+      for (int pass = 0; pass < passes + 1; pass++) {
+        for (int k = 0; k <= j; k++) {
+          dot_partial_kernel<<<nb, 256, 0, stream()>>>(n, rpb, Vc(k), u.as<double>(), part.as<double>());
+          sum_to_slot_kernel<<<1, 256, 0, stream()>>>(nb, part.as<double>(), sc + MG_H, 0);
+          // H[k, j] = dot (pass 0); += dot (reorthogonalisation pass, :327)
+          h_accum_kernel<<<1, 1, 0, stream()>>>(sd + L.H() + k + j * (m + 1), sc + MG_H, pass == 0);
+          axpy_dev_kernel<<<nbv, 256, 0, stream()>>>(n, sc + MG_H, -1.0, Vc(k), u.as<double>());
+        }
+        LACE_TRY(nrm_into(u.as<double>(), sc + MG_NORM2));
+        mgs_step_small_kernel<<<1, 64, 0, stream()>>>(sd, L, j, 0, pass == 0 ? 0 : 1);
+        if (pass == 0) {
+          LACE_TRY(read_sc());
+          const bool again = (reorth == 0 && hostsc[MG_REORTH] != 0.0) || reorth == 2;
+          if (!again) break;
+        }
+      }
+      mgs_step_small_kernel<<<1, 64, 0, stream()>>>(sd, L, j, j == m - 1, 2);
+      // v_{j+1} = u / h_{j+1,j} unless breakdown (:341-356); the H entry was
+      // saved before rotation in sc[MG_NORM2]
+      scale_into_kernel<<<nbv, 256, 0, stream()>>>(n, u.as<double>(), sc + MG_NORM2, sc + MG_ALPHA,
+                                                   Vc(j + 1));
+      LACE_HIP(hipGetLastError());
+      LACE_TRY(read_sc());
+      if (hostsc[MG_DONE] != 0.0) break;
+    }
+    mgs_lsq_kernel<<<1, 64, 0, stream()>>>(sd, L);
+    mgs_xupdate_kernel<<<nbv, 256, 0, stream()>>>(n, ld, Z.as<double>(), sd, L, x);
+    LACE_HIP(hipGetLastError());
+    LACE_TRY(read_sc());
+    local.iterations += (int)hostsc[MG_NC];
+    local.restarts = restart;
+    local.final_residual_est = hostsc[MG_RES];
+    if (!std::isfinite(hostsc[MG_RES])) break;
+    converged = hostsc[MG_RES] < rtol_abs;
+  }
+  LACE_TRY((residual<double, double>(S->A64, x, b, S->rbuf.as<double>())));
+  LACE_TRY(norm2(S, S->rbuf.as<double>(), &local.true_residual));
+  double bn = 0.0;
+  LACE_TRY(norm2(S, b, &bn));
+  local.rel_true_residual = bn > 0.0 ? local.true_residual / bn : local.true_residual;
+  local.converged = converged ? 1 : 0;
+  local.solve_ms =
+      std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  if (out) *out = local;
+  return DEV_SUCCESS;
+}
+
+}  // namespace lace
+
+using namespace lace;
+
+extern "C" {
+
+void lace_solver_default_opts(lace_solver_opts* o) {
+  memset(o, 0, sizeof(*o));
+  o->krylov = 0;
+  o->search_max = 50;
+  o->restart_max = 100;
+  o->rtol = 1e-8;
+  o->prec_precond = LACE_F64;
+  o->trsv = LACE_TRSV_LEVEL;
+  o->jacobi_sweeps = LACE_JACOBI_SWEEPS_DEFAULT;
+  o->parilu_sweeps = 3;
+  o->inner_iters = 0;
+  o->transpose = 0;
+}
+
+int lace_solver_create(const data_d_matrix* A, const lace_solver_opts* o, lace_solver** out) {
+  Solver* S = nullptr;
+  LACE_TRY(solver_create(A, o, &S));
+  *out = reinterpret_cast<lace_solver*>(S);
+  return DEV_SUCCESS;
+}
+
+int lace_solver_destroy(lace_solver* S) {
+  delete reinterpret_cast<Solver*>(S);
+  return DEV_SUCCESS;
+}
+
+int lace_solver_setup(lace_solver* Sh) {
+  LACE_TRY(ensure_init());
+  return solver_setup(reinterpret_cast<Solver*>(Sh));
+}
+
+int lace_solver_solve(lace_solver* Sh, const double* b, double* x, lace_solver_stats* st) {
+  LACE_TRY(ensure_init());
+  return solver_solve(reinterpret_cast<Solver*>(Sh), b, x, st, -1.0);
+}
+
+int lace_solver_grad_values(lace_solver* Sh, const double* lambda, const double* x, double* dtheta) {
+  LACE_TRY(ensure_init());
+  auto* S = reinterpret_cast<Solver*>(Sh);
+  grad_gather_kernel<<<ceil_div(S->n, 256), 256, 0, stream()>>>(
+      S->n, S->Apat.row.as<int>(), S->Apat.col.as<int>(), lambda, x, dtheta);
+  LACE_HIP(hipGetLastError());
+  return DEV_SUCCESS;
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------ drop-ins
+namespace {
+
+// Shared body of data_fgmres_householder[_restart]: L as CSRL, U as CSRU
+// (the test path converts the CSCU factor to CSR, testing/
+// test_linearSolver.cpp:670-672).  The factors are used as given.
+int fgmres_dropin_common(data_d_matrix* A, data_d_matrix* b, data_d_matrix* x0, data_d_matrix* L,
+                         data_d_matrix* U, data_d_gmres_param* par, data_d_gmres_log* log,
+                         bool restarted, bool mgs) {
+  LACE_TRY(ensure_init());
+  auto t0 = std::chrono::steady_clock::now();
+  const int n = A->num_rows;
+  lace_solver_opts o;
+  lace_solver_default_opts(&o);
+  o.search_max = par->search_max;
+  o.restart_max = restarted ? par->restart_max : 1;
+  o.rtol = par->rtol;
+  const int choice = par->user_csrtrsv_choice;
+  // 0: exact solves (mkl_dcsrtrsv in the reference); 1: ParCSRTRSV, which
+  // with the reference's natural row order and one thread is exact forward/
+  // backward substitution plus a confirming sweep (src/trisolve.cpp:281-292),
+  // so it also maps to the exact solve; 2: none; 3: Jacobi-k.
+  o.trsv = choice == 2 ? LACE_TRSV_NONE : (choice == 3 ? LACE_TRSV_JACOBI : LACE_TRSV_LEVEL);
+  o.jacobi_sweeps = LACE_JACOBI_SWEEPS_DEFAULT;
+  o.parilu_sweeps = 0;
+  Solver* S = nullptr;
+  LACE_TRY(solver_create(A, &o, &S));
+  std::unique_ptr<Solver> hold(S);
+  if (S->P) {
+    // load the caller's factors: L (CSRL) into the L buffer, U (CSRU) into
+    // the CSC buffer through the CSR->CSC slot map
+    ParILU* P = S->P;
+    LACE_REQUIRE(L->row && U->row && L->row[n] == P->nnzL && U->row[n] == P->nnzU,
+                 DEV_ERR_ILLEGAL_VALUE, "FGMRES: L/U patterns do not match A's ILU(0) pattern");
+    std::vector<double> uc(P->nnzU);
+    // U CSR slot q of row i col j -> CSC slot
+    std::vector<int> fill(P->h_ucp.begin(), P->h_ucp.end() - 1);
+    for (int i = 0; i < n; i++)
+      for (int k = U->row[i]; k < U->row[i + 1]; k++) uc[fill[U->col[k]]++] = U->val[k];
+    LACE_HIP(hipMemcpyAsync(P->lval[0].p, L->val, sizeof(double) * P->nnzL, hipMemcpyHostToDevice, stream()));
+    LACE_HIP(hipMemcpyAsync(P->uval[0].p, uc.data(), sizeof(double) * P->nnzU, hipMemcpyHostToDevice, stream()));
+    LACE_HIP(hipStreamSynchronize(stream()));
+    P->cur = 0;
+    P->csr_dirty = true;
+    LACE_TRY(P->refresh_csr());
+  }
+  DevBuf db, dx;
+  LACE_TRY(upload(db, b->val, sizeof(double) * n));
+  LACE_TRY(upload(dx, x0->val, sizeof(double) * n));
+  lace_solver_stats st;
+  const double tol_abs = par->tol_type == 1 ? -1.0 : par->rtol;
+  if (mgs)
+    LACE_TRY(solver_solve_mgs(S, db.as<double>(), dx.as<double>(), &st, tol_abs, par->reorth));
+  else
+    LACE_TRY(solver_solve(S, db.as<double>(), dx.as<double>(), &st, tol_abs));
+  // x0 is replaced by the solution (src/fgmres.cpp:491-492)
+  data_d_matrix xs;
+  memset(&xs, 0, sizeof(xs));
+  LACE_TRY(data_zvinit(&xs, n, 1, 0.0, x0->major ? x0->major : MagmaColMajor));
+  LACE_TRY(download(xs.val, dx, sizeof(double) * n));
+  data_zmfree(x0);
+  *x0 = xs;
+  auto t1 = std::chrono::steady_clock::now();
+  log->restarts = st.restarts;
+  log->search_directions = st.iterations;
+  log->solve_time = std::chrono::duration<double>(t1 - t0).count();
+  log->initial_residual = st.initial_residual;
+  log->final_residual = st.final_residual_est;
+  return st.converged ? 0 : DEV_NOTCONVERGED;
+}
+
+}  // namespace
+
+extern "C" {
+
+int data_fgmres_householder(data_d_matrix* A, data_d_matrix* b, data_d_matrix* x0, data_d_matrix* L,
+                            data_d_matrix* U, data_d_gmres_param* par, data_d_gmres_log* log) {
+  return fgmres_dropin_common(A, b, x0, L, U, par, log, false, false);
+}
+
+int data_fgmres_householder_restart(data_d_matrix* A, data_d_matrix* b, data_d_matrix* x0,
+                                    data_d_matrix* L, data_d_matrix* U, data_d_gmres_param* par,
+                                    data_d_gmres_log* log) {
+  return fgmres_dropin_common(A, b, x0, L, U, par, log, true, false);
+}
+
+}  // extern "C"
+
+namespace {
+
+int fgmres_mgs_dropin(data_d_matrix* A, data_d_matrix* b, data_d_matrix* x0, data_d_matrix* L,
+                      data_d_matrix* U, data_d_gmres_param* par, data_d_gmres_log* log,
+                      bool restarted) {
+  return fgmres_dropin_common(A, b, x0, L, U, par, log, restarted, true);
+}
+
+}  // namespace
+
+extern "C" {
+
+int data_fgmres(data_d_matrix* A, data_d_matrix* b, data_d_matrix* x0, data_d_matrix* L,
+                data_d_matrix* U, data_d_gmres_param* par, data_d_gmres_log* log) {
+  return fgmres_mgs_dropin(A, b, x0, L, U, par, log, false);
+}
+
+int data_fgmres_restart(data_d_matrix* A, data_d_matrix* b, data_d_matrix* x0, data_d_matrix* L,
+                        data_d_matrix* U, data_d_gmres_param* par, data_d_gmres_log* log) {
+  return fgmres_mgs_dropin(A, b, x0, L, U, par, log, true);
+}
+
+}  // extern "C"

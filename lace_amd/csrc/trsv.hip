@@ -1,0 +1,466 @@
+// trsv.hip -- sparse triangular solves on the GPU.
+//
+// Reference: src/trisolve.cpp:20-119 (data_forward_solve/backward_solve),
+// :256-351 (data_parcsrtrsv), and the exact mkl_dcsrtrsv solves of
+// src/fgmres.cpp:216-228.  The reference has no level scheduling; its
+// ParCSRTRSV iterates in-place OpenMP sweeps.  Three device variants here:
+//   - level-scheduled, one launch per level (exact);
+//   - sync-free, one launch: rows run in level order, each lane waits on
+//     the values of the rows it depends on (exact);
+//   - synchronous Jacobi sweeps (the update formula of trisolve.cpp:281-290
+//     with all reads from the previous sweep).
+// Every variant forms a row's off-diagonal sum left to right and then
+// divides (r_i - sum) by the diagonal, as the reference loop does, with
+// contraction disabled, so exact solves equal forward/backward
+// substitution on the CPU bit for bit.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "common.hpp"
+#include "device.hpp"
+#include "parilu.hpp"
+
+namespace lace {
+
+// ---------------------------------------------------------------- plans
+int TriPlan::build(int n_, const int* row_, const int* col_, bool upper_) {
+  n = n_;
+  nnz = row_[n_];
+  upper = upper_;
+  std::vector<int> perm_h(n), lptr_h(n + 1);
+  nlev = lace_level_sets(n, row_, col_, upper ? 1 : 0, perm_h.data(), lptr_h.data());
+  h_lptr.assign(lptr_h.begin(), lptr_h.begin() + nlev + 1);
+  std::vector<int> sched_h;
+  sched_h.reserve(n + 64 * (size_t)nlev);
+  for (int l = 0; l < nlev; l++) {
+    for (int t = h_lptr[l]; t < h_lptr[l + 1]; t++) sched_h.push_back(perm_h[t]);
+    while (sched_h.size() % 64) sched_h.push_back(-1);
+  }
+  nslots = (int)sched_h.size();
+  LACE_TRY(upload(row, row_, sizeof(int) * (n + 1)));
+  LACE_TRY(upload(col, col_, sizeof(int) * std::max(nnz, 1)));
+  LACE_TRY(upload(perm, perm_h.data(), sizeof(int) * std::max(n, 1)));
+  LACE_TRY(upload(lptr, h_lptr.data(), sizeof(int) * (nlev + 1)));
+  LACE_TRY(upload(sched, sched_h.data(), sizeof(int) * std::max(nslots, 1)));
+  return DEV_SUCCESS;
+}
+
+// ---------------------------------------------------------------- kernels
+template <typename T, bool UPPER>
+__device__ __forceinline__ void row_range(const int* __restrict__ row, int i, int& k0, int& k1,
+                                          int& kd) {
+  if (UPPER) {
+    kd = row[i];
+    k0 = kd + 1;
+    k1 = row[i + 1];
+  } else {
+    kd = row[i + 1] - 1;
+    k0 = row[i];
+    k1 = kd;
+  }
+}
+
+template <typename T, bool UPPER>
+__global__ __launch_bounds__(256) void tri_level_kernel(int begin, int count,
+                                                        const int* __restrict__ perm,
+                                                        const int* __restrict__ row,
+                                                        const int* __restrict__ col,
+                                                        const T* __restrict__ val,
+                                                        const T* __restrict__ r, T* y) {
+#pragma clang fp contract(off)
+  const int t = blockIdx.x * 256 + threadIdx.x;
+  if (t >= count) return;
+  const int i = perm[begin + t];
+  int k0, k1, kd;
+  row_range<T, UPPER>(row, i, k0, k1, kd);
+  T s = T(0);
+  for (int k = k0; k < k1; k++) s = s + val[k] * y[col[k]];
+  y[i] = (r[i] - s) / val[kd];
+}
+
+template <typename T>
+struct Sentinel;
+template <>
+struct Sentinel<double> {
+  using U = unsigned long long;
+  static constexpr U bits = 0x7FF4DEADBEEF5A5AULL;  // signalling-NaN payload
+};
+template <>
+struct Sentinel<float> {
+  using U = unsigned int;
+  static constexpr U bits = 0x7FA5A5A5U;
+};
+
+template <typename T>
+__global__ void fill_sentinel_kernel(int n, T* y) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) {
+    typename Sentinel<T>::U b = Sentinel<T>::bits;
+    y[i] = __builtin_bit_cast(T, b);
+  }
+}
+
+// Sync-free exact solve.  Slot s of the level-ordered schedule is handled
+// by thread s; a wavefront only ever holds rows of one level, so every row
+// it waits on belongs to an earlier wavefront of this block or of a block
+// with a smaller index (dispatched earlier), which guarantees progress.
+// The solution vector doubles as the ready flag: y[j] holds the sentinel
+// until row j is solved; each value is published with one agent-scope
+// atomic store and read back with agent-scope atomic loads (coherent
+// across the XCD L2s).  Arithmetic results are never the sentinel (it is a
+// signalling NaN; arithmetic yields quiet NaNs).  A wait that exceeds
+// spin_limit polls sets *err and stops waiting, so every wave terminates.
+template <typename T, bool UPPER>
+__global__ __launch_bounds__(256) void tri_syncfree_kernel(int nslots, const int* __restrict__ sched,
+                                                           const int* __restrict__ row,
+                                                           const int* __restrict__ col,
+                                                           const T* __restrict__ val,
+                                                           const T* __restrict__ r, T* y, int* err,
+                                                           int spin_limit) {
+#pragma clang fp contract(off)
+  using UB = typename Sentinel<T>::U;
+  const int slot = blockIdx.x * 256 + threadIdx.x;
+  if (slot >= nslots) return;
+  const int i = sched[slot];
+  if (i < 0) return;
+  int k0, k1, kd;
+  row_range<T, UPPER>(row, i, k0, k1, kd);
+  T s = T(0);
+  for (int k = k0; k < k1; k++) {
+    const int j = col[k];
+    UB* pj = reinterpret_cast<UB*>(y + j);
+    UB b = __hip_atomic_load(pj, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    int spins = 0;
+    while (b == Sentinel<T>::bits) {
+      if (++spins > spin_limit) {
+        __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        break;
+      }
+      __builtin_amdgcn_s_sleep(2);
+      b = __hip_atomic_load(pj, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    s = s + val[k] * __builtin_bit_cast(T, b);
+  }
+  const T yi = (r[i] - s) / val[kd];
+  __hip_atomic_store(reinterpret_cast<UB*>(y + i), __builtin_bit_cast(UB, yi), __ATOMIC_RELAXED,
+                     __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// One synchronous Jacobi sweep: yout_i = (r_i - sum_{k != diag} T_ik yin_k)/T_ii.
+// yin == nullptr means yin = 0.  STEP accumulates (yout - yin)^2 per block.
+template <typename T, bool UPPER, bool STEP>
+__global__ __launch_bounds__(256) void tri_jacobi_kernel(int n, const int* __restrict__ row,
+                                                         const int* __restrict__ col,
+                                                         const T* __restrict__ val,
+                                                         const T* __restrict__ r,
+                                                         const T* __restrict__ yin,
+                                                         T* __restrict__ yout,
+                                                         double* __restrict__ partial) {
+#pragma clang fp contract(off)
+  const int b = xcd_block(blockIdx.x, gridDim.x);
+  const int i = b * 256 + threadIdx.x;
+  double d2 = 0.0;
+  if (i < n) {
+    int k0, k1, kd;
+    row_range<T, UPPER>(row, i, k0, k1, kd);
+    T s = T(0);
+    if (yin)
+      for (int k = k0; k < k1; k++) s = s + val[k] * yin[col[k]];
+    const T v = (r[i] - s) / val[kd];
+    if (STEP) {
+      double d = (double)v - (yin ? (double)yin[i] : 0.0);
+      d2 = d * d;
+    }
+    yout[i] = v;
+  }
+  if (STEP) {
+    __shared__ double sm[4];
+    double tot = block_sum<256>(d2, sm);
+    if (threadIdx.x == 0) partial[b] = tot;
+  }
+}
+
+template <typename T>
+__global__ void gather_kernel(int m, const int* __restrict__ map, const T* __restrict__ in,
+                              T* __restrict__ out) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p < m) out[p] = in[map[p]];
+}
+
+// ---------------------------------------------------------------- host
+template <typename T>
+int tri_solve_levels(const TriPlan& P, const T* val, const T* r, T* y) {
+  TimedRegion tr("trsv_level");
+  for (int l = 0; l < P.nlev; l++) {
+    const int b = P.h_lptr[l], c = P.h_lptr[l + 1] - b;
+    if (P.upper)
+      tri_level_kernel<T, true><<<ceil_div(c, 256), 256, 0, stream()>>>(
+          b, c, P.perm.as<int>(), P.row.as<int>(), P.col.as<int>(), val, r, y);
+    else
+      tri_level_kernel<T, false><<<ceil_div(c, 256), 256, 0, stream()>>>(
+          b, c, P.perm.as<int>(), P.row.as<int>(), P.col.as<int>(), val, r, y);
+  }
+  LACE_HIP(hipGetLastError());
+  return DEV_SUCCESS;
+}
+
+// Poll budget per dependency wait (about 0.1 s); a solve never legitimately
+// waits that long on one row, and the bound keeps every wave finite.
+static int g_spin_limit = 1 << 16;
+
+template <typename T>
+int tri_solve_syncfree(const TriPlan& P, const T* val, const T* r, T* y, int* err) {
+  TimedRegion tr("trsv_syncfree");
+  LACE_REQUIRE((const void*)r != (const void*)y, DEV_ERR_ILLEGAL_VALUE,
+               "sync-free solve needs distinct r and y");
+  fill_sentinel_kernel<T><<<ceil_div(P.n, 256), 256, 0, stream()>>>(P.n, y);
+  if (P.upper)
+    tri_syncfree_kernel<T, true><<<ceil_div(P.nslots, 256), 256, 0, stream()>>>(
+        P.nslots, P.sched.as<int>(), P.row.as<int>(), P.col.as<int>(), val, r, y, err,
+        g_spin_limit);
+  else
+    tri_syncfree_kernel<T, false><<<ceil_div(P.nslots, 256), 256, 0, stream()>>>(
+        P.nslots, P.sched.as<int>(), P.row.as<int>(), P.col.as<int>(), val, r, y, err,
+        g_spin_limit);
+  LACE_HIP(hipGetLastError());
+  return DEV_SUCCESS;
+}
+
+template <typename T>
+static int jacobi_sweep(const TriPlan& P, const T* val, const T* r, const T* yin, T* yout,
+                        double* partial) {
+  const int nb = ceil_div(P.n, 256);
+  if (partial) {
+    if (P.upper)
+      tri_jacobi_kernel<T, true, true><<<nb, 256, 0, stream()>>>(
+          P.n, P.row.as<int>(), P.col.as<int>(), val, r, yin, yout, partial);
+    else
+      tri_jacobi_kernel<T, false, true><<<nb, 256, 0, stream()>>>(
+          P.n, P.row.as<int>(), P.col.as<int>(), val, r, yin, yout, partial);
+  } else {
+    if (P.upper)
+      tri_jacobi_kernel<T, true, false><<<nb, 256, 0, stream()>>>(
+          P.n, P.row.as<int>(), P.col.as<int>(), val, r, yin, yout, nullptr);
+    else
+      tri_jacobi_kernel<T, false, false><<<nb, 256, 0, stream()>>>(
+          P.n, P.row.as<int>(), P.col.as<int>(), val, r, yin, yout, nullptr);
+  }
+  LACE_HIP(hipGetLastError());
+  return DEV_SUCCESS;
+}
+
+template <typename T>
+int tri_solve_jacobi(const TriPlan& P, const T* val, const T* r, T* y, T* tmp, int sweeps) {
+  TimedRegion tr("trsv_jacobi");
+  if (sweeps <= 0) {
+    LACE_HIP(hipMemsetAsync(y, 0, sizeof(T) * P.n, stream()));
+    return DEV_SUCCESS;
+  }
+  // ping-pong so that the last sweep lands in y
+  T* a = (sweeps % 2 == 1) ? y : tmp;
+  T* b = (sweeps % 2 == 1) ? tmp : y;
+  LACE_TRY(jacobi_sweep<T>(P, val, r, nullptr, a, nullptr));
+  for (int s = 1; s < sweeps; s++) {
+    LACE_TRY(jacobi_sweep<T>(P, val, r, a, b, nullptr));
+    std::swap(a, b);
+  }
+  return DEV_SUCCESS;
+}
+
+template int tri_solve_levels<double>(const TriPlan&, const double*, const double*, double*);
+template int tri_solve_levels<float>(const TriPlan&, const float*, const float*, float*);
+template int tri_solve_syncfree<double>(const TriPlan&, const double*, const double*, double*, int*);
+template int tri_solve_syncfree<float>(const TriPlan&, const float*, const float*, float*, int*);
+template int tri_solve_jacobi<double>(const TriPlan&, const double*, const double*, double*, double*, int);
+template int tri_solve_jacobi<float>(const TriPlan&, const float*, const float*, float*, float*, int);
+
+// ---------------------------------------------------------------- ParILU solves
+int ParILU::build_trsv() {
+  LACE_TRY(lower.build(n, h_lrow.data(), h_lcol.data(), false));
+  LACE_TRY(lowerT.build(n, h_ucp.data(), h_uri.data(), false));
+  // U in CSR: transpose of the CSC arrays; map CSR slot -> CSC slot.
+  {
+    std::vector<int> r(n + 1, 0), c(nnzU), map(nnzU);
+    for (int p = 0; p < nnzU; p++) r[h_uri[p] + 1]++;
+    for (int i = 0; i < n; i++) r[i + 1] += r[i];
+    std::vector<int> fill(r.begin(), r.end() - 1);
+    for (int j = 0; j < n; j++)
+      for (int p = h_ucp[j]; p < h_ucp[j + 1]; p++) {
+        int q = fill[h_uri[p]]++;
+        c[q] = j;
+        map[q] = p;
+      }
+    LACE_TRY(upper.build(n, r.data(), c.data(), true));
+    LACE_TRY(upload(u_csr_from_csc, map.data(), sizeof(int) * nnzU));
+  }
+  // L^T in CSR (diagonal first): transpose of L's CSR arrays.
+  {
+    std::vector<int> r(n + 1, 0), c(nnzL), map(nnzL);
+    for (int p = 0; p < nnzL; p++) r[h_lcol[p] + 1]++;
+    for (int i = 0; i < n; i++) r[i + 1] += r[i];
+    std::vector<int> fill(r.begin(), r.end() - 1);
+    for (int i = 0; i < n; i++)
+      for (int p = h_lrow[i]; p < h_lrow[i + 1]; p++) {
+        int q = fill[h_lcol[p]]++;
+        c[q] = i;
+        map[q] = p;
+      }
+    LACE_TRY(upperT.build(n, r.data(), c.data(), true));
+    LACE_TRY(upload(l_t_from_csr, map.data(), sizeof(int) * nnzL));
+  }
+  const size_t es = elem_size();
+  LACE_TRY(uval_csr.alloc(es * nnzU));
+  LACE_TRY(lval_t.alloc(es * nnzL));
+  LACE_TRY(work.alloc(es * n));
+  LACE_TRY(work2.alloc(es * n));
+  LACE_TRY(syncfree_err.alloc(sizeof(int) * 4));
+  LACE_HIP(hipMemsetAsync(syncfree_err.p, 0, sizeof(int) * 4, stream()));
+  csr_dirty = true;
+  return DEV_SUCCESS;
+}
+
+int ParILU::refresh_csr() {
+  if (!csr_dirty) return DEV_SUCCESS;
+  if (prec == LACE_F64) {
+    gather_kernel<double><<<ceil_div(nnzU, 256), 256, 0, stream()>>>(
+        nnzU, u_csr_from_csc.as<int>(), uval[cur].as<double>(), uval_csr.as<double>());
+    gather_kernel<double><<<ceil_div(nnzL, 256), 256, 0, stream()>>>(
+        nnzL, l_t_from_csr.as<int>(), lval[cur].as<double>(), lval_t.as<double>());
+  } else {
+    gather_kernel<float><<<ceil_div(nnzU, 256), 256, 0, stream()>>>(
+        nnzU, u_csr_from_csc.as<int>(), uval[cur].as<float>(), uval_csr.as<float>());
+    gather_kernel<float><<<ceil_div(nnzL, 256), 256, 0, stream()>>>(
+        nnzL, l_t_from_csr.as<int>(), lval[cur].as<float>(), lval_t.as<float>());
+  }
+  LACE_HIP(hipGetLastError());
+  csr_dirty = false;
+  return DEV_SUCCESS;
+}
+
+template <typename T>
+static int apply_t(ParILU* P, lace_trsv_kind_t kind, int sweeps, int trans, const T* r, T* z) {
+  LACE_TRY(P->refresh_csr());
+  T* tmp = P->work.as<T>();
+  const TriPlan& first = trans ? P->lowerT : P->lower;
+  const TriPlan& second = trans ? P->upperT : P->upper;
+  const T* v1 = trans ? P->uval[P->cur].as<T>() : P->lval[P->cur].as<T>();
+  const T* v2 = trans ? P->lval_t.as<T>() : P->uval_csr.as<T>();
+  if (kind == LACE_TRSV_NONE) {
+    LACE_HIP(hipMemcpyAsync(z, r, sizeof(T) * P->n, hipMemcpyDeviceToDevice, stream()));
+    return DEV_SUCCESS;
+  }
+  if (kind == LACE_TRSV_LEVEL) {
+    int* err = P->syncfree_err.as<int>();
+    LACE_TRY(tri_solve_syncfree<T>(first, v1, r, tmp, err));
+    LACE_TRY(tri_solve_syncfree<T>(second, v2, tmp, z, err));
+    return DEV_SUCCESS;
+  }
+  // Jacobi: work holds the intermediate vector, work2 the ping-pong buffer.
+  T* tmp2 = P->work2.as<T>();
+  LACE_TRY(tri_solve_jacobi<T>(first, v1, r, tmp, tmp2, sweeps));
+  LACE_TRY(tri_solve_jacobi<T>(second, v2, tmp, z, tmp2, sweeps));
+  return DEV_SUCCESS;
+}
+
+int ParILU::apply(lace_trsv_kind_t kind, int sweeps, int trans, const void* r, void* z) {
+  if (prec == LACE_F64)
+    return apply_t<double>(this, kind, sweeps, trans, static_cast<const double*>(r),
+                           static_cast<double*>(z));
+  return apply_t<float>(this, kind, sweeps, trans, static_cast<const float*>(r),
+                        static_cast<float*>(z));
+}
+
+}  // namespace lace
+
+using namespace lace;
+
+extern "C" {
+
+int lace_parilu_apply(lace_parilu* Ph, lace_trsv_kind_t kind, int jacobi_sweeps, int trans,
+                      const void* r, void* z) {
+  LACE_TRY(ensure_init());
+  return reinterpret_cast<ParILU*>(Ph)->apply(kind, jacobi_sweeps, trans, r, z);
+}
+
+int lace_syncfree_error(lace_parilu* Ph, int* err) {
+  LACE_TRY(ensure_init());
+  auto* P = reinterpret_cast<ParILU*>(Ph);
+  LACE_HIP(hipMemcpyAsync(err, P->syncfree_err.p, sizeof(int), hipMemcpyDeviceToHost, stream()));
+  LACE_HIP(hipStreamSynchronize(stream()));
+  return DEV_SUCCESS;
+}
+
+// Iterated synchronous sweeps until sum (y_new - y_old)^2 <= tol or the
+// step is not finite (src/trisolve.cpp:274,317), starting from the y
+// passed in.  A triangular Jacobi iteration is exact after nlev sweeps, so
+// at most nlev+1 sweeps run.  storage/uplo must name CSRL+Lower or
+// CSRU+Upper as in the reference (:264-347).
+int data_parcsrtrsv(const data_uplo_t uplo, const data_storage_t storage,
+                    const data_diagorder_t diag, const int num_rows, const dataType* Aval,
+                    const int* row, const int* col, const dataType* rhsval, dataType* yval,
+                    const dataType tol, int* iter) {
+  LACE_TRY(ensure_init());
+  bool upper;
+  if (storage == Magma_CSRL && uplo == MagmaLower && diag != Magma_NODIAG) upper = false;
+  else if (storage == Magma_CSRU && uplo == MagmaUpper && diag != Magma_NODIAG) upper = true;
+  else {
+    set_error("data_parcsrtrsv: storage %d / uplo %d must be CSRL+Lower or CSRU+Upper",
+              (int)storage, (int)uplo);
+    return -1;
+  }
+  const int n = num_rows;
+  TriPlan P;
+  LACE_TRY(P.build(n, row, col, upper));
+  DevBuf val, r, y[2], partial, sum;
+  LACE_TRY(upload(val, Aval, sizeof(double) * std::max(P.nnz, 1)));
+  LACE_TRY(upload(r, rhsval, sizeof(double) * n));
+  LACE_TRY(upload(y[0], yval, sizeof(double) * n));
+  LACE_TRY(y[1].alloc(sizeof(double) * n));
+  const int nb = ceil_div(n, 256);
+  LACE_TRY(partial.alloc(sizeof(double) * nb));
+  std::vector<double> ph(nb);
+  int it = 0, cur = 0;
+  double step = 1.e8;
+  while (step > tol && std::isfinite(step) && it <= P.nlev + 1) {
+    LACE_TRY(jacobi_sweep<double>(P, val.as<double>(), r.as<double>(), y[cur].as<double>(),
+                                  y[1 - cur].as<double>(), partial.as<double>()));
+    LACE_HIP(hipMemcpyAsync(ph.data(), partial.p, sizeof(double) * nb, hipMemcpyDeviceToHost,
+                            stream()));
+    LACE_HIP(hipStreamSynchronize(stream()));
+    step = 0.0;
+    for (int b = 0; b < nb; b++) step += ph[b];
+    cur = 1 - cur;
+    it++;
+  }
+  LACE_TRY(download(yval, y[cur], sizeof(double) * n));
+  *iter = it;
+  return 0;
+}
+
+// src/trisolve.cpp:20-67
+int data_forward_solve(data_d_matrix* L, data_d_matrix* x, data_d_matrix* rhs, const dataType tol,
+                       int* iter) {
+  if (!(L->storage_type == Magma_CSRL && L->fill_mode == MagmaLower &&
+        L->diagorder_type != Magma_NODIAG)) {
+    set_error("data_forward_solve: L must be CSRL / lower with a diagonal");
+    return -1;
+  }
+  return data_parcsrtrsv(MagmaLower, Magma_CSRL, L->diagorder_type, L->num_rows, L->val, L->row,
+                         L->col, rhs->val, x->val, tol, iter);
+}
+
+// src/trisolve.cpp:72-119
+int data_backward_solve(data_d_matrix* U, data_d_matrix* x, data_d_matrix* rhs,
+                        const dataType tol, int* iter) {
+  if (!(U->storage_type == Magma_CSRU && U->fill_mode == MagmaUpper &&
+        U->diagorder_type != Magma_NODIAG)) {
+    set_error("data_backward_solve: U must be CSRU / upper with a diagonal");
+    return -1;
+  }
+  return data_parcsrtrsv(MagmaUpper, Magma_CSRU, U->diagorder_type, U->num_rows, U->val, U->row,
+                         U->col, rhs->val, x->val, tol, iter);
+}
+
+}  // extern "C"

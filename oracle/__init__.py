@@ -1,0 +1,459 @@
+"""CPU oracle for the LACE preconditioned-solve path -- TEST INFRASTRUCTURE.
+
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg import
+this package, and only as the checker (or the timed CPU baseline); the
+shipped library never calls it.  The numeric work lives in lace_oracle.c
+(built by oracle/Makefile); this module wraps it with numpy and adds the
+pure-Python pieces: the MatrixMarket reader (control/magma_zmio.cpp:92-350)
+and independent restatements of the synthetic generators.
+
+Parity status per function is recorded in lace_oracle.c's header; every
+function there is pinned by at least one check in tests/test_oracle_*.py.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+from dataclasses import dataclass
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "_build", "liblace_oracle.so")
+_lib = None
+
+
+def build() -> None:
+    """Compile the oracle with its Makefile (gcc, a few seconds)."""
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            raise RuntimeError(f"oracle not built: {_LIB_PATH} (run make -C oracle)")
+        _lib = C.CDLL(_LIB_PATH)
+        _declare(_lib)
+    return _lib
+
+
+_i = C.c_int
+_d = C.c_double
+_p = C.c_void_p
+
+
+def _declare(L):
+    sig = {
+        "orc_csr_split": (_i, [_i, _p, _p, _p, _i, _i, _p, _p, _p]),
+        "orc_transpose": (None, [_i, _i, _p, _p, _p, _p, _p, _p]),
+        "orc_spgemm": (_i, [_i, _i, _p, _p, _p, _p, _p, _p, _p, _p, _p]),
+        "orc_ilures": (None, [_i, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p]),
+        "orc_parilu_sweep_d": (_d, [_i, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p]),
+        "orc_parilu_sweep_f": (_d, [_i, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p]),
+        "orc_parilu_v03_omp": (_i, [_i, _i, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _i, _d, _p]),
+        "orc_ilu0": (_i, [_i, _p, _p, _p]),
+        "orc_trsv_lower_d": (None, [_i, _p, _p, _p, _p, _p]),
+        "orc_trsv_upper_d": (None, [_i, _p, _p, _p, _p, _p]),
+        "orc_trsv_lower_f": (None, [_i, _p, _p, _p, _p, _p]),
+        "orc_trsv_upper_f": (None, [_i, _p, _p, _p, _p, _p]),
+        "orc_trsv_jacobi_d": (None, [_i, _p, _p, _p, _i, _i, _p, _p]),
+        "orc_trsv_jacobi_f": (None, [_i, _p, _p, _p, _i, _i, _p, _p]),
+        "orc_parcsrtrsv": (_i, [_i, _p, _p, _p, _i, _p, _p, _d]),
+        "orc_levels": (_i, [_i, _p, _p, _i, _p]),
+        "orc_spmv": (None, [_i, _p, _p, _p, _d, _p, _d, _p]),
+        "orc_givens": (None, [_d, _d, _d, _p, _p]),
+        "orc_fgmres_householder": (_i, [_i, _p, _p, _p, _p, _p, _i, _i, _d, _i, _i, _i,
+                                        _p, _p, _p, _p, _p, _p, _p, _p, _p, _i, _p]),
+        "orc_fgmres_mgs": (_i, [_i, _p, _p, _p, _p, _p, _i, _i, _d, _i, _i, _i, _i,
+                                _p, _p, _p, _p, _p, _p, _p, _p, _p, _i, _p]),
+        "orc_housegen": (None, [_i, _p, _p, _p]),
+        "orc_hqrd": (None, [_i, _i, _p, _p, _p]),
+        "orc_philox4x32_10": (None, [_p, _p, _p]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+
+
+def _ptr(a):
+    return a.ctypes.data_as(C.c_void_p) if a is not None else None
+
+
+def _ci(a):
+    return np.ascontiguousarray(a, dtype=np.int32)
+
+
+def _cd(a, dt=np.float64):
+    return np.ascontiguousarray(a, dtype=dt)
+
+
+@dataclass
+class CSR:
+    """Plain CSR (int32 indices).  For CSC views, `row` is the column
+    pointer and `col` the row indices."""
+    n_rows: int
+    n_cols: int
+    row: np.ndarray
+    col: np.ndarray
+    val: np.ndarray
+
+    @property
+    def nnz(self) -> int:
+        return int(self.row[-1])
+
+    def dense(self) -> np.ndarray:
+        D = np.zeros((self.n_rows, self.n_cols))
+        for i in range(self.n_rows):
+            for k in range(self.row[i], self.row[i + 1]):
+                D[i, self.col[k]] += self.val[k]
+        return D
+
+    def rowidx(self) -> np.ndarray:
+        return np.repeat(np.arange(self.n_rows, dtype=np.int32), np.diff(self.row))
+
+    def scipy(self):
+        import scipy.sparse as sp
+        return sp.csr_matrix((self.val, self.col, self.row), shape=(self.n_rows, self.n_cols))
+
+
+# ---------------------------------------------------------------- I/O
+def read_mtx(path: str) -> CSR:
+    """MatrixMarket coordinate reader following control/magma_zmio.cpp:92-350:
+    1-based COO; symmetric/hermitian files duplicate every off-diagonal entry
+    (:221-273); counting sort into CSR; each row sorted by column (:323-336).
+    Pattern files get value 1.0; complex files keep the real part."""
+    with open(path) as f:
+        banner = f.readline().lower().split()
+        if len(banner) < 5 or banner[0] != "%%matrixmarket" or banner[2] != "coordinate":
+            raise ValueError("only coordinate MatrixMarket files are supported")
+        field, symm = banner[3], banner[4]
+        line = f.readline()
+        while line.startswith("%") or not line.strip():
+            line = f.readline()
+        m, n, nz = (int(t) for t in line.split()[:3])
+        rows, cols, vals = [], [], []
+        for _ in range(nz):
+            t = f.readline().split()
+            while not t:
+                t = f.readline().split()
+            rows.append(int(t[0]) - 1)
+            cols.append(int(t[1]) - 1)
+            vals.append(1.0 if field == "pattern" else float(t[2]))
+    r = np.array(rows, dtype=np.int64)
+    c = np.array(cols, dtype=np.int64)
+    v = np.array(vals, dtype=np.float64)
+    if symm in ("symmetric", "hermitian"):
+        rr, cc, vv = [], [], []
+        for a, b, x in zip(r, c, v):
+            rr.append(a); cc.append(b); vv.append(x)
+            if a != b:
+                rr.append(b); cc.append(a); vv.append(x)
+        r, c, v = np.array(rr), np.array(cc), np.array(vv)
+    order = np.lexsort((c, r))
+    r, c, v = r[order], c[order], v[order]
+    row = np.zeros(m + 1, dtype=np.int32)
+    np.add.at(row, r + 1, 1)
+    row = np.cumsum(row).astype(np.int32)
+    return CSR(m, n, row, c.astype(np.int32), v)
+
+
+def read_mtx_dense(path: str) -> np.ndarray:
+    """MatrixMarket `array` (column-major) reader."""
+    with open(path) as f:
+        f.readline()
+        line = f.readline()
+        while line.startswith("%"):
+            line = f.readline()
+        m, n = (int(t) for t in line.split()[:2])
+        vals = [float(t) for t in f.read().split()]
+    return np.array(vals[: m * n]).reshape((n, m)).T.copy()
+
+
+# ---------------------------------------------------------------- generators
+def laplace2d(nx: int, ny: int) -> CSR:
+    """5-point [4,-1,-1,-1,-1] with Dirichlet boundaries, x fastest."""
+    rows, cols, vals = [], [], []
+    for j in range(ny):
+        for i in range(nx):
+            p = i + nx * j
+            ent = []
+            if j > 0: ent.append((p - nx, -1.0))
+            if i > 0: ent.append((p - 1, -1.0))
+            ent.append((p, 4.0))
+            if i < nx - 1: ent.append((p + 1, -1.0))
+            if j < ny - 1: ent.append((p + nx, -1.0))
+            for q, a in ent:
+                rows.append(p); cols.append(q); vals.append(a)
+    return _coo_to_csr(nx * ny, rows, cols, vals)
+
+
+def convdiff3d(nx: int, ny: int, nz: int, pe=(2.0, 1.0, 0.5)) -> CSR:
+    """7-point first-order upwind convection-diffusion, h-scaled:
+    a_ii = 6 + sum(pe), a_{i,i-e_d} = -1 - pe_d, a_{i,i+e_d} = -1,
+    Dirichlet (neighbours outside the box dropped), x fastest."""
+    import scipy.sparse as sp
+
+    def axis(m, pe_d):
+        lo = np.full(m - 1, -1.0 - pe_d)
+        hi = np.full(m - 1, -1.0)
+        return sp.diags([lo, hi], [-1, 1], shape=(m, m))
+
+    Ix, Iy, Iz = sp.identity(nx), sp.identity(ny), sp.identity(nz)
+    A = (sp.kron(Iz, sp.kron(Iy, axis(nx, pe[0])))
+         + sp.kron(Iz, sp.kron(axis(ny, pe[1]), Ix))
+         + sp.kron(axis(nz, pe[2]), sp.kron(Iy, Ix)))
+    A = A + sp.identity(nx * ny * nz) * (6.0 + sum(pe))
+    A = sp.csr_matrix(A)
+    A.sort_indices()
+    return CSR(A.shape[0], A.shape[1], A.indptr.astype(np.int32),
+               A.indices.astype(np.int32), A.data.astype(np.float64))
+
+
+def _coo_to_csr(n, rows, cols, vals) -> CSR:
+    r = np.array(rows); c = np.array(cols); v = np.array(vals, dtype=np.float64)
+    o = np.lexsort((c, r))
+    r, c, v = r[o], c[o], v[o]
+    row = np.zeros(n + 1, dtype=np.int32)
+    np.add.at(row, r + 1, 1)
+    return CSR(n, n, np.cumsum(row).astype(np.int32), c.astype(np.int32), v)
+
+
+def philox4x32_10(ctr, key):
+    out = np.zeros(4, dtype=np.uint32)
+    lib().orc_philox4x32_10(_ptr(np.asarray(ctr, dtype=np.uint32)),
+                            _ptr(np.asarray(key, dtype=np.uint32)), _ptr(out))
+    return out
+
+
+# ---------------------------------------------------------------- formats
+DIAG_OMIT, DIAG_UNIT, DIAG_VALUE, DIAG_ZERO = 0, 1, 2, 3
+
+
+def csr_split(A: CSR, upper: bool, diag_mode: int) -> CSR:
+    """CSRL (upper=False) / CSRU (upper=True): magma_zmconverter.cpp:303-460."""
+    L = lib()
+    row, col, val = _ci(A.row), _ci(A.col), _cd(A.val)
+    nz = L.orc_csr_split(A.n_rows, _ptr(row), _ptr(col), _ptr(val), int(upper), diag_mode,
+                         None, None, None)
+    tr = np.zeros(A.n_rows + 1, np.int32); tc = np.zeros(nz, np.int32); tv = np.zeros(nz)
+    L.orc_csr_split(A.n_rows, _ptr(row), _ptr(col), _ptr(val), int(upper), diag_mode,
+                    _ptr(tr), _ptr(tc), _ptr(tv))
+    return CSR(A.n_rows, A.n_cols, tr, tc, tv)
+
+
+def transpose(A: CSR) -> CSR:
+    tr = np.zeros(A.n_cols + 1, np.int32); tc = np.zeros(A.nnz, np.int32); tv = np.zeros(A.nnz)
+    row, col, val = _ci(A.row), _ci(A.col), _cd(A.val)
+    lib().orc_transpose(A.n_rows, A.n_cols, _ptr(row), _ptr(col), _ptr(val),
+                        _ptr(tr), _ptr(tc), _ptr(tv))
+    return CSR(A.n_cols, A.n_rows, tr, tc, tv)
+
+
+def spgemm(A: CSR, B: CSR) -> CSR:
+    L = lib()
+    a = (_ci(A.row), _ci(A.col), _cd(A.val)); b = (_ci(B.row), _ci(B.col), _cd(B.val))
+    nz = L.orc_spgemm(A.n_rows, B.n_cols, *map(_ptr, a), *map(_ptr, b), None, None, None)
+    cr = np.zeros(A.n_rows + 1, np.int32); cc = np.zeros(nz, np.int32); cv = np.zeros(nz)
+    L.orc_spgemm(A.n_rows, B.n_cols, *map(_ptr, a), *map(_ptr, b), _ptr(cr), _ptr(cc), _ptr(cv))
+    return CSR(A.n_rows, B.n_cols, cr, cc, cv)
+
+
+def ilures(A: CSR, Lf: CSR, Ucsr: CSR):
+    """(res, nonlinres) of control/norms.cpp:301-405; L CSRL, U CSR upper."""
+    res = C.c_double(); nl = C.c_double()
+    arrs = [_ci(A.row), _ci(A.col), _cd(A.val), _ci(Lf.row), _ci(Lf.col), _cd(Lf.val),
+            _ci(Ucsr.row), _ci(Ucsr.col), _cd(Ucsr.val)]
+    lib().orc_ilures(A.n_rows, *map(_ptr, arrs), C.byref(res), C.byref(nl))
+    return res.value, nl.value
+
+
+# ---------------------------------------------------------------- ParILU
+class ParILUState:
+    """L (CSRL, unit diagonal last) and U (CSCU: column pointer in U.row,
+    row indices in U.col, diagonal last per column) initialised from A as
+    src/parilu_v0_3.cpp:23-27 does."""
+
+    def __init__(self, A: CSR, dtype=np.float64):
+        self.A = A
+        self.dtype = dtype
+        self.L = csr_split(A, False, DIAG_UNIT)
+        Ucsr = csr_split(A, True, DIAG_VALUE)
+        self.U = transpose(Ucsr)
+        self.Lval = self.L.val.astype(dtype)
+        self.Uval = self.U.val.astype(dtype)
+        self.Aval = A.val.astype(dtype)
+
+    def sweep(self, mode: str = "jacobi") -> float:
+        A, L, U = self.A, self.L, self.U
+        fn = lib().orc_parilu_sweep_d if self.dtype == np.float64 else lib().orc_parilu_sweep_f
+        a = (_ci(A.row), _ci(A.col), self.Aval)
+        lrc = (_ci(L.row), _ci(L.col)); urc = (_ci(U.row), _ci(U.col))
+        if mode == "jacobi":
+            Lout, Uout = self.Lval.copy(), self.Uval.copy()
+        elif mode == "gs":
+            Lout, Uout = self.Lval, self.Uval
+        else:
+            raise ValueError(mode)
+        step = fn(A.n_rows, *map(_ptr, a), *map(_ptr, lrc), _ptr(self.Lval), _ptr(Lout),
+                  *map(_ptr, urc), _ptr(self.Uval), _ptr(Uout))
+        self.Lval, self.Uval = Lout, Uout
+        return step
+
+    def factors(self):
+        """(L as CSRL, U as CSRU) with the current values (float64)."""
+        Lf = CSR(self.L.n_rows, self.L.n_cols, self.L.row, self.L.col, self.Lval.astype(np.float64))
+        Uc = CSR(self.U.n_rows, self.U.n_cols, self.U.row, self.U.col, self.Uval.astype(np.float64))
+        return Lf, transpose(Uc)
+
+
+def parilu(A: CSR, sweeps: int, mode: str = "jacobi", dtype=np.float64):
+    st = ParILUState(A, dtype)
+    for _ in range(sweeps):
+        st.sweep(mode)
+    return st
+
+
+def parilu_v03_omp(A: CSR, max_sweeps: int):
+    """The reference's OpenMP in-place loop (CPU baseline); returns
+    (state, sweeps_run, final_step, seconds)."""
+    import time
+    st = ParILUState(A)
+    anorm = float(np.sqrt(np.sum(A.val ** 2)))
+    rowidx = _ci(A.rowidx())
+    fs = C.c_double()
+    t0 = time.perf_counter()
+    it = lib().orc_parilu_v03_omp(A.n_rows, A.nnz, _ptr(_ci(A.row)), _ptr(rowidx), _ptr(_ci(A.col)),
+                                  _ptr(_cd(A.val)), _ptr(_ci(st.L.row)), _ptr(_ci(st.L.col)),
+                                  _ptr(st.Lval), _ptr(_ci(st.U.row)), _ptr(_ci(st.U.col)),
+                                  _ptr(st.Uval), max_sweeps, 1.0 / anorm, C.byref(fs))
+    return st, it, fs.value, time.perf_counter() - t0
+
+
+def ilu0(A: CSR):
+    """Exact ILU(0) in A's pattern (merged strict-L / U); returns (CSR, bad_row)."""
+    v = _cd(A.val).copy()
+    bad = lib().orc_ilu0(A.n_rows, _ptr(_ci(A.row)), _ptr(_ci(A.col)), _ptr(v))
+    return CSR(A.n_rows, A.n_cols, A.row.copy(), A.col.copy(), v), bad
+
+
+def split_merged(LU: CSR):
+    """Merged ILU (data_zmlumerge layout) -> (CSRL unit diag, CSRU)."""
+    Lf = csr_split(LU, False, DIAG_UNIT)
+    Uf = csr_split(LU, True, DIAG_VALUE)
+    return Lf, Uf
+
+
+# ---------------------------------------------------------------- trsv
+def trsv(T: CSR, r, upper: bool, dtype=np.float64):
+    L = lib()
+    y = np.zeros(T.n_rows, dtype=dtype)
+    rr = _cd(r, dtype); tv = _cd(T.val, dtype)
+    if dtype == np.float64:
+        fn = L.orc_trsv_upper_d if upper else L.orc_trsv_lower_d
+    else:
+        fn = L.orc_trsv_upper_f if upper else L.orc_trsv_lower_f
+    fn(T.n_rows, _ptr(_ci(T.row)), _ptr(_ci(T.col)), _ptr(tv), _ptr(rr), _ptr(y))
+    return y
+
+
+def trsv_jacobi(T: CSR, r, upper: bool, sweeps: int, dtype=np.float64):
+    L = lib()
+    y = np.zeros(T.n_rows, dtype=dtype)
+    fn = L.orc_trsv_jacobi_d if dtype == np.float64 else L.orc_trsv_jacobi_f
+    fn(T.n_rows, _ptr(_ci(T.row)), _ptr(_ci(T.col)), _ptr(_cd(T.val, dtype)), int(upper), sweeps,
+       _ptr(_cd(r, dtype)), _ptr(y))
+    return y
+
+
+def parcsrtrsv(T: CSR, r, upper: bool, tol: float, y0=None):
+    y = np.zeros(T.n_rows) if y0 is None else _cd(y0).copy()
+    it = lib().orc_parcsrtrsv(T.n_rows, _ptr(_ci(T.row)), _ptr(_ci(T.col)), _ptr(_cd(T.val)),
+                              int(upper), _ptr(_cd(r)), _ptr(y), tol)
+    return y, it
+
+
+def levels(T: CSR, upper: bool):
+    lev = np.zeros(T.n_rows, np.int32)
+    nl = lib().orc_levels(T.n_rows, _ptr(_ci(T.row)), _ptr(_ci(T.col)), int(upper), _ptr(lev))
+    return lev, nl
+
+
+def spmv(A: CSR, x, alpha=1.0, beta=0.0, y=None):
+    yy = np.zeros(A.n_rows) if y is None else _cd(y).copy()
+    lib().orc_spmv(A.n_rows, _ptr(_ci(A.row)), _ptr(_ci(A.col)), _ptr(_cd(A.val)), alpha,
+                   _ptr(_cd(x)), beta, _ptr(yy))
+    return yy
+
+
+def givens(a, b, tol):
+    c = C.c_double(); s = C.c_double()
+    lib().orc_givens(a, b, tol, C.byref(c), C.byref(s))
+    return c.value, s.value
+
+
+# ---------------------------------------------------------------- FGMRES
+@dataclass
+class FgmresResult:
+    x: np.ndarray
+    info: int
+    searches: int
+    restarts: int
+    hist: np.ndarray
+    final_res: float
+
+
+def _fgmres_common(kind, A, b, x0, m, restart_max, rtol, tol_type, choice, jac, Lf, Uf,
+                   reorth=0, hist_len=4096):
+    x = _cd(x0).copy() if x0 is not None else np.zeros(A.n_rows)
+    hist = np.zeros(hist_len)
+    so = C.c_int(); ro = C.c_int(); fr = C.c_double()
+    if Lf is None:
+        Lf = CSR(A.n_rows, A.n_cols, np.zeros(A.n_rows + 1, np.int32), np.zeros(1, np.int32),
+                 np.zeros(1))
+        Uf = Lf
+    fac = [_ci(Lf.row), _ci(Lf.col), _cd(Lf.val), _ci(Uf.row), _ci(Uf.col), _cd(Uf.val)]
+    a = [_ci(A.row), _ci(A.col), _cd(A.val)]
+    if kind == "householder":
+        info = lib().orc_fgmres_householder(
+            A.n_rows, *map(_ptr, a), _ptr(_cd(b)), _ptr(x), m, restart_max, rtol, tol_type,
+            choice, jac, *map(_ptr, fac), C.byref(so), C.byref(ro), _ptr(hist), hist_len,
+            C.byref(fr))
+    else:
+        info = lib().orc_fgmres_mgs(
+            A.n_rows, *map(_ptr, a), _ptr(_cd(b)), _ptr(x), m, restart_max, rtol, tol_type,
+            reorth, choice, jac, *map(_ptr, fac), C.byref(so), C.byref(ro), _ptr(hist),
+            hist_len, C.byref(fr))
+    return FgmresResult(x, info, so.value, ro.value, hist[: min(so.value, hist_len)], fr.value)
+
+
+def fgmres_householder(A, b, x0=None, m=30, restart_max=1, rtol=1e-10, tol_type=1, choice=0,
+                       jac=5, Lf=None, Uf=None):
+    """src/fgmres_householder_restart.cpp:63-788 (sequential restatement)."""
+    return _fgmres_common("householder", A, b, x0, m, restart_max, rtol, tol_type, choice, jac,
+                          Lf, Uf)
+
+
+def fgmres_mgs(A, b, x0=None, m=30, restart_max=1, rtol=1e-10, tol_type=1, choice=0, jac=5,
+               Lf=None, Uf=None, reorth=0):
+    """src/fgmres.cpp:64-517 with restarts (src/fgmres_restart.cpp)."""
+    return _fgmres_common("mgs", A, b, x0, m, restart_max, rtol, tol_type, choice, jac, Lf, Uf,
+                          reorth=reorth)
+
+
+# ---------------------------------------------------------------- Householder QR
+def housegen(x):
+    x = _cd(x)
+    u = np.zeros_like(x); nu = C.c_double()
+    lib().orc_housegen(len(x), _ptr(x), _ptr(u), C.byref(nu))
+    return u, nu.value
+
+
+def hqrd(X):
+    X = np.asfortranarray(np.array(X, dtype=np.float64))
+    m, n = X.shape
+    U = np.zeros((m, n), order="F"); R = np.zeros((n, n), order="F")
+    lib().orc_hqrd(m, n, _ptr(X), _ptr(U), _ptr(R))
+    return U, R

@@ -1,0 +1,359 @@
+"""GPU parity: the HIP path against the CPU oracle on the same inputs.
+
+Bar: bit-exact for patterns, level sets and the contraction-free sweeps and
+substitutions (FP64 and FP32); floating-point reductions (SpMV, FGMRES)
+within the tolerances written in each test.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN
+
+pytestmark = pytest.mark.gpu
+
+
+def problems(orc):
+    return {
+        "laplace64": orc.laplace2d(64, 64),
+        "convdiff16": orc.convdiff3d(16, 16, 16),
+        "trefethen20": orc.read_mtx(os.path.join(GOLDEN, "Trefethen_20.mtx")),
+        "io_test": orc.read_mtx(os.path.join(GOLDEN, "io_test.mtx")),
+    }
+
+
+PROBLEMS = ["laplace64", "convdiff16", "trefethen20", "io_test"]
+
+
+def _csc_of(orc, U):
+    return U  # oracle keeps U as CSC (row = column pointer, col = row index)
+
+
+@pytest.mark.parametrize("name", PROBLEMS)
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+def test_parilu_jacobi_bitexact(gpu, orc, name, prec):
+    A = problems(orc)[name]
+    lp = gpu.LACE_F64 if prec == "f64" else gpu.LACE_F32
+    dt = np.float64 if prec == "f64" else np.float32
+    H = gpu.HostCSR.from_csr(A)
+    P = gpu.ParILU(H, lp)
+    st = orc.ParILUState(A, dt)
+    for k in range(1, 6):
+        P.sweeps(1)
+        st.sweep("jacobi")
+        (lr, lc, lv), (ur, uc, uv) = P.factors()
+        assert np.array_equal(lr, st.L.row) and np.array_equal(lc, st.L.col)
+        assert np.array_equal(ur, st.U.row) and np.array_equal(uc, st.U.col)
+        # FP32 factors come back widened to FP64: compare in the storage type
+        assert np.array_equal(lv.astype(dt), st.Lval), f"L differs after sweep {k}"
+        assert np.array_equal(uv.astype(dt), st.Uval), f"U differs after sweep {k}"
+
+
+def test_parilu_reset_and_nonlinres(gpu, orc):
+    """Device ||A - LU||_F on pattern(A) after every sweep equals the
+    oracle's control/norms.cpp restatement; reset restores L0/U0."""
+    A = orc.convdiff3d(16, 16, 16)
+    P = gpu.ParILU(gpu.HostCSR.from_csr(A), gpu.LACE_F64)
+    st = orc.ParILUState(A)
+    for _ in range(6):
+        _, nl = orc.ilures(A, *st.factors())
+        got = P.nonlinres()
+        assert abs(got - nl) <= 1e-12 * max(1.0, nl)
+        P.sweeps(1)
+        st.sweep("jacobi")
+    P.reset()
+    (lr, lc, lv), (ur, uc, uv) = P.factors()
+    st0 = orc.ParILUState(A)
+    assert np.array_equal(lv, st0.Lval) and np.array_equal(uv, st0.Uval)
+
+
+def test_parilu_finite_termination(gpu, orc):
+    """Synchronous sweeps reach the ILU(0) fixed point after as many sweeps
+    as the dependency depth; one more sweep changes nothing (bitwise)."""
+    A = orc.convdiff3d(6, 6, 6)
+    P = gpu.ParILU(gpu.HostCSR.from_csr(A), gpu.LACE_F64)
+    P.sweeps(40)
+    (_, _, lv1), (_, _, uv1) = P.factors()
+    P.sweeps(1)
+    (_, _, lv2), (_, _, uv2) = P.factors()
+    assert np.array_equal(lv1, lv2) and np.array_equal(uv1, uv2)
+    gs = orc.ParILUState(A)
+    gs.sweep("gs")
+    assert np.array_equal(lv1, gs.Lval) and np.array_equal(uv1, gs.Uval)
+    assert P.nonlinres() <= 1e-12 * np.sqrt(np.sum(A.val ** 2))
+
+
+def test_parilu_step_reduction(gpu, orc):
+    A = orc.laplace2d(64, 64)
+    P = gpu.ParILU(gpu.HostCSR.from_csr(A), gpu.LACE_F64)
+    st = orc.ParILUState(A)
+    anorm = np.sqrt(np.sum(A.val ** 2))
+    for _ in range(3):
+        s_gpu = P.sweeps(1, want_step=True)
+        s_orc = st.sweep("jacobi") / anorm
+        # sums of ~2e4 squares in different orders: relative gap <= n*eps
+        assert abs(s_gpu - s_orc) <= 1e-11 * max(s_orc, 1e-300)
+
+
+def _dropin_parilu(gpu, A, fn, max_sweeps=5, reduction=1e-15):
+    H = gpu.HostCSR.from_csr(A)
+    L = gpu.DMatrix(); U = gpu.DMatrix()
+    log = gpu.PrecondLog()
+    log.maxSweeps = max_sweeps
+    args = [H.ref(), C.byref(L), C.byref(U), reduction, C.byref(log)]
+    if fn == "data_PariLU_v0_3_gpu":
+        args.append(256)  # the reference's block size argument (unused)
+    getattr(gpu.lib(), fn)(*args)
+    assert log.sweeps >= 0, gpu.lib().lace_last_error()
+    Lh = gpu.owned_to_numpy(L)
+    Uh = gpu.owned_to_numpy(U)
+    H.release_rowidx()
+    return Lh, Uh, log
+
+
+@pytest.mark.parametrize("fn", ["data_PariLU_v0_3_prescribedSweeps",
+                                "data_PariLU_v0_3_gpu_prescribedSweeps"])
+def test_parilu_dropin_prescribed(gpu, orc, fn):
+    A = orc.laplace2d(64, 64)
+    (lr, lc, lv), (ur, uc, uv), log = _dropin_parilu(gpu, A, fn, max_sweeps=5)
+    st = orc.parilu(A, 5)
+    assert log.sweeps == 5
+    assert np.array_equal(lv, st.Lval) and np.array_equal(uv, st.Uval)
+    assert abs(log.A_Frobenius - np.sqrt(np.sum(A.val ** 2))) < 1e-12 * log.A_Frobenius
+    # initial residual ||A - L0 U0|| of the A-initialised guess
+    st0 = orc.ParILUState(A)
+    r0, nl0 = orc.ilures(A, *st0.factors())
+    assert abs(log.initial_residual - r0) <= 1e-12 * r0
+    assert abs(log.initial_nonlinear_residual - nl0) <= 1e-12 * max(nl0, 1.0)
+
+
+def test_parilu_reference_bounds(gpu, orc):
+    """testing/test_iLU.cpp:175-176 (Ares, nonlinres <= 1000x exact ILU(0))
+    and testing/test_iLU_gpu.cpp:142-143 (10x) on Trefethen_20.  Synchronous
+    sweeps are exact after as many sweeps as the factor's dependency depth
+    (19 levels here), so 20 sweeps must meet the tighter 10x bound.  (The
+    drop-ins stop earlier by the reference's own step rules, which were
+    tuned for in-place sweeps: see test_parilu_dropin_stop_rule.)"""
+    A = orc.read_mtx(os.path.join(GOLDEN, "Trefethen_20.mtx"))
+    P = gpu.ParILU(gpu.HostCSR.from_csr(A), gpu.LACE_F64)
+    P.sweeps(20)
+    (lr, lc, lv), (ur, uc, uv) = P.factors()
+    Lf = orc.CSR(A.n_rows, A.n_cols, lr, lc, lv)
+    Uc = orc.CSR(A.n_rows, A.n_cols, ur, uc, uv)
+    res, nl = orc.ilures(A, Lf, orc.transpose(Uc))
+    ex, bad = orc.ilu0(A)
+    assert bad < 0
+    rex, nlex = orc.ilures(A, *orc.split_merged(ex))
+    assert res <= 10 * rex
+    assert nl <= 10 * nlex
+
+
+@pytest.mark.parametrize("max_sweeps", [3, 20])
+def test_parilu_dropin_prescribed_stop_rule(gpu, orc, max_sweeps):
+    """src/parilu_v0_3_prescribedSweeps.cpp:62: at most maxSweeps sweeps,
+    stopping once step <= 4 eps."""
+    A = orc.read_mtx(os.path.join(GOLDEN, "Trefethen_20.mtx"))
+    (lr, lc, lv), (ur, uc, uv), log = _dropin_parilu(
+        gpu, A, "data_PariLU_v0_3_prescribedSweeps", max_sweeps=max_sweeps)
+    st = orc.ParILUState(A)
+    anorm = np.sqrt(np.sum(A.val ** 2))
+    step, it = np.finfo(np.float32).max, 0
+    while it < max_sweeps and step > 4 * np.finfo(np.float64).eps:
+        step = st.sweep("jacobi") / anorm
+        it += 1
+    assert log.sweeps == it
+    assert np.array_equal(lv, st.Lval) and np.array_equal(uv, st.Uval)
+
+
+@pytest.mark.parametrize("fn", ["data_PariLU_v0_3", "data_PariLU_v0_3_gpu"])
+@pytest.mark.parametrize("reduction", [1e-10, 1e-15, 1e-20])
+def test_parilu_dropin_stop_rule(gpu, orc, fn, reduction):
+    """src/parilu_v0_3.cpp:44-45,62,104: sweep while step > reduction*Ares
+    (step = sum of squared updates / ||A||_F), at most 20 sweeps.  The sweep
+    count and the factors equal the oracle's synchronous run with the same
+    rule."""
+    A = orc.read_mtx(os.path.join(GOLDEN, "Trefethen_20.mtx"))
+    (lr, lc, lv), (ur, uc, uv), log = _dropin_parilu(gpu, A, fn, reduction=reduction)
+    st = orc.ParILUState(A)
+    ares, _ = orc.ilures(A, *st.factors())
+    anorm = np.sqrt(np.sum(A.val ** 2))
+    tol, step, it = reduction * ares, np.finfo(np.float32).max, 0
+    while step > tol and it < 20:
+        step = st.sweep("jacobi") / anorm
+        it += 1
+    assert log.sweeps == it
+    assert abs(log.tol - tol) <= 1e-12 * tol
+    assert np.array_equal(lv, st.Lval) and np.array_equal(uv, st.Uval)
+
+
+def _factors(orc, A, sweeps=3, dt=np.float64):
+    st = orc.parilu(A, sweeps, dtype=dt)
+    return st.factors()
+
+
+@pytest.mark.parametrize("name", ["laplace64", "convdiff16", "trefethen20"])
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+@pytest.mark.parametrize("kind", ["level", "jacobi"])
+def test_trsv_bitexact(gpu, orc, name, prec, kind):
+    A = problems(orc)[name]
+    lp = gpu.LACE_F64 if prec == "f64" else gpu.LACE_F32
+    dt = np.float64 if prec == "f64" else np.float32
+    P = gpu.ParILU(gpu.HostCSR.from_csr(A), lp)
+    P.sweeps(3)
+    st = orc.parilu(A, 3, dtype=dt)
+    Lf = orc.CSR(A.n_rows, A.n_cols, st.L.row, st.L.col, st.Lval)
+    Uc = orc.CSR(A.n_rows, A.n_cols, st.U.row, st.U.col, st.Uval)
+    Ur = orc.transpose(Uc)
+    rng = np.random.default_rng(7)
+    r = rng.standard_normal(A.n_rows).astype(dt)
+    if kind == "level":
+        y = orc.trsv(Lf, r, upper=False, dtype=dt)
+        z = orc.trsv(Ur, y, upper=True, dtype=dt)
+        got = P.apply(r, gpu.TRSV_LEVEL)
+    else:
+        y = orc.trsv_jacobi(Lf, r, upper=False, sweeps=4, dtype=dt)
+        z = orc.trsv_jacobi(Ur, y, upper=True, sweeps=4, dtype=dt)
+        got = P.apply(r, gpu.TRSV_JACOBI, sweeps=4)
+    assert np.array_equal(got, z)
+
+
+def test_trsv_transposed(gpu, orc):
+    A = orc.convdiff3d(8, 8, 8)
+    P = gpu.ParILU(gpu.HostCSR.from_csr(A), gpu.LACE_F64)
+    P.sweeps(3)
+    Lf, Uf = _factors(orc, A, 3)
+    M = Lf.dense() @ Uf.dense()
+    r = np.random.default_rng(3).standard_normal(A.n_rows)
+    z = P.apply(r, gpu.TRSV_LEVEL, trans=1)
+    ref = np.linalg.solve(M.T, r)
+    assert np.allclose(z, ref, rtol=1e-11, atol=1e-12 * np.abs(ref).max())
+
+
+def test_level_counts(gpu, orc):
+    """Closed form: natural-order 2-D 5-point has nx+ny-1 levels per factor,
+    3-D 7-point nx+ny+nz-2."""
+    P = gpu.ParILU(gpu.HostCSR.from_csr(orc.laplace2d(64, 64)))
+    assert P.levels() == (127, 127)
+    P = gpu.ParILU(gpu.HostCSR.from_csr(orc.convdiff3d(16, 16, 16)))
+    assert P.levels() == (46, 46)
+
+
+def test_spmv_dropin(gpu, orc):
+    A = orc.convdiff3d(20, 20, 20)
+    x = np.random.default_rng(1).standard_normal(A.n_rows)
+    y0 = np.random.default_rng(2).standard_normal(A.n_rows)
+    H = gpu.HostCSR.from_csr(A)
+    X = gpu.HostVec(x); Y = gpu.HostVec(y0)
+    gpu.check(gpu.lib().data_z_spmv(2.0, H.ref(), X.ref(), -0.5, Y.ref()))
+    ref = 2.0 * (A.scipy() @ x) - 0.5 * y0
+    assert np.allclose(Y.v, ref, rtol=1e-14, atol=1e-13)
+
+
+def _solver(gpu, A, **kw):
+    o = gpu.default_opts()
+    for k, v in kw.items():
+        setattr(o, k, v)
+    S = gpu.Solver(gpu.HostCSR.from_csr(A), o)
+    S.setup()
+    return S
+
+
+def test_fgmres_householder_vs_oracle(gpu, orc):
+    """Config 1 shape: 64x64 Laplacian, b = 1, ParILU 5 sweeps, FGMRES(30)
+    to 1e-10 relative.  Iteration count within +-2 of the CPU restatement of
+    src/fgmres_householder_restart.cpp with exact triangular solves."""
+    A = orc.laplace2d(64, 64)
+    b = np.ones(A.n_rows)
+    Lf, Uf = _factors(orc, A, 5)
+    ref = orc.fgmres_householder(A, b, m=30, restart_max=20, rtol=1e-10, tol_type=1, choice=0,
+                                 Lf=Lf, Uf=Uf)
+    assert ref.info == 0
+    S = _solver(gpu, A, search_max=30, restart_max=20, rtol=1e-10, parilu_sweeps=5,
+                trsv=gpu.TRSV_LEVEL, prec_precond=gpu.LACE_F64)
+    x, st = S.solve(b)
+    assert st.converged == 1
+    assert abs(st.iterations - ref.searches) <= 2, (st.iterations, ref.searches)
+    r = b - A.scipy() @ x
+    assert np.linalg.norm(r) <= 1e-10 * np.linalg.norm(b) * 1.01
+    assert np.allclose(x, ref.x, rtol=1e-8, atol=1e-8 * np.abs(ref.x).max())
+
+
+def test_fgmres_householder_history(gpu, orc):
+    """Per-step residual estimates agree with the oracle's (same factors)."""
+    A = orc.convdiff3d(12, 12, 12)
+    b = np.random.default_rng(5).standard_normal(A.n_rows)
+    Lf, Uf = _factors(orc, A, 3)
+    ref = orc.fgmres_householder(A, b, m=40, restart_max=1, rtol=1e-12, tol_type=1, choice=0,
+                                 Lf=Lf, Uf=Uf)
+    S = _solver(gpu, A, search_max=40, restart_max=1, rtol=1e-12, parilu_sweeps=3)
+    x, st = S.solve(b)
+    assert abs(st.iterations - ref.searches) <= 2
+    assert abs(st.final_residual_est - ref.final_res) <= 1e-6 * np.linalg.norm(b) + 0.5 * ref.final_res
+
+
+def _dropin_fgmres(gpu, orc, fn, A, b, tol, choice, search_max=20, restart_max=20, tol_type=0):
+    # factors from the drop-in ParILU, U converted CSC -> CSR
+    # (testing/test_linearSolver.cpp:664-672)
+    (lr, lc, lv), (ur, uc, uv), _ = _dropin_parilu(gpu, A, "data_PariLU_v0_3")
+    Uc = orc.CSR(A.n_rows, A.n_cols, ur, uc, uv)
+    Ur = orc.transpose(Uc)
+    L = gpu.HostCSR(lr, lc, lv, storage=gpu.Magma_CSRL)
+    L.m.diagorder_type = gpu.Magma_UNITY
+    L.m.fill_mode = gpu.MagmaLower
+    U = gpu.HostCSR(Ur.row, Ur.col, Ur.val, storage=gpu.Magma_CSRU)
+    U.m.fill_mode = gpu.MagmaUpper
+    H = gpu.HostCSR.from_csr(A)
+    B = gpu.HostVec(b)
+    X = gpu.DMatrix()
+    gpu.check(gpu.lib().data_zvinit(C.byref(X), A.n_rows, 1, 0.0, gpu.MagmaColMajor))
+    par = gpu.GmresParam()
+    par.tol_type = tol_type; par.rtol = tol; par.search_max = search_max
+    par.restart_max = restart_max; par.reorth = 0; par.precondition = 1
+    par.user_csrtrsv_choice = choice
+    log = gpu.GmresLog()
+    info = getattr(gpu.lib(), fn)(H.ref(), B.ref(), C.byref(X), L.ref(), U.ref(), C.byref(par),
+                                  C.byref(log))
+    x = gpu.owned_to_numpy(X, kind="dense")
+    return x, info, log
+
+
+@pytest.mark.parametrize("fn", ["data_fgmres_householder_restart", "data_fgmres_restart",
+                                "data_fgmres_householder", "data_fgmres"])
+@pytest.mark.parametrize("choice", [0, 1, 2])
+def test_fgmres_dropin_trefethen(gpu, orc, fn, choice):
+    """testing/test_linearSolver.cpp: true residual ||b - A x|| <= tol (1e-3)
+    on Trefethen_20 with b = 1, x0 = 0."""
+    A = orc.read_mtx(os.path.join(GOLDEN, "Trefethen_20.mtx"))
+    b = np.ones(A.n_rows)
+    x, info, log = _dropin_fgmres(gpu, orc, fn, A, b, 1e-3, choice)
+    assert info == 0
+    assert np.linalg.norm(b - A.scipy() @ x) <= 1e-3
+    assert log.search_directions >= 1
+
+
+def test_fgmres_mgs_vs_oracle(gpu, orc):
+    A = orc.convdiff3d(10, 10, 10)
+    b = np.random.default_rng(9).standard_normal(A.n_rows)
+    ref = orc.fgmres_mgs(A, b, m=30, restart_max=10, rtol=1e-10, tol_type=1, choice=2)
+    x, info, log = _dropin_fgmres(gpu, orc, "data_fgmres_restart", A, b, 1e-10, 2, 30, 10, 1)
+    assert info == 0
+    assert abs(log.search_directions - ref.searches) <= 2
+    assert np.linalg.norm(b - A.scipy() @ x) <= 1.01e-10 * np.linalg.norm(b)
+
+
+@pytest.mark.parametrize("mode", ["fp32-ilu", "nested"])
+def test_mixed_precision_reaches_fp64_tolerance(gpu, orc, mode):
+    A = orc.convdiff3d(24, 24, 24)
+    b = np.random.default_rng(0).standard_normal(A.n_rows)
+    kw = dict(search_max=30, restart_max=20, rtol=1e-8, parilu_sweeps=3, trsv=gpu.TRSV_JACOBI,
+              jacobi_sweeps=5, prec_precond=gpu.LACE_F32)
+    if mode == "nested":
+        kw.update(inner_iters=10, search_max=10)
+    S = _solver(gpu, A, **kw)
+    x, st = S.solve(b)
+    assert st.converged == 1
+    rel = np.linalg.norm(b - A.scipy() @ x) / np.linalg.norm(b)
+    assert rel <= 1e-8 * 1.05
+    assert abs(rel - st.rel_true_residual) <= 1e-3 * rel + 1e-15
