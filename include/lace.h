@@ -281,6 +281,9 @@ int lace_dmalloc(void **ptr, size_t bytes);
 int lace_dfree(void *ptr);
 int lace_memcpy_h2d(void *dst, const void *src, size_t bytes);
 int lace_memcpy_d2h(void *dst, const void *src, size_t bytes);
+/* Device-to-device streaming copy kernel (bytes a multiple of 16); used to
+ * calibrate hardware traffic counters against a known byte count. */
+int lace_stream_copy(void *dst, const void *src, size_t bytes);
 
 /* Device CSR upload (values converted to prec). */
 int lace_csr_create(const data_d_matrix *A, lace_prec_t prec, lace_csr **out);
@@ -363,6 +366,8 @@ int lace_solver_grad_values(lace_solver *S, const double *lambda,
 /* Kernel timing by class, HIP events on the library stream. */
 int lace_timing_enable(int on);
 int lace_timing_reset(void);
+/* Restrict timing to one region name (NULL or "" times every region). */
+int lace_timing_only(const char *name);
 int lace_timing_get(const char *name, double *total_ms, long *count);
 
 /* Level sets (host): lev(i) = 1 + max lev(k) over the off-diagonal entries

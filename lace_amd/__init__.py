@@ -133,6 +133,7 @@ SIGNATURES = {
     "lace_dfree": (_i, [_vp]),
     "lace_memcpy_h2d": (_i, [_vp, _vp, C.c_size_t]),
     "lace_memcpy_d2h": (_i, [_vp, _vp, C.c_size_t]),
+    "lace_stream_copy": (_i, [_vp, _vp, C.c_size_t]),
     "lace_csr_create": (_i, [_DM, _i, _P(_vp)]),
     "lace_csr_destroy": (_i, [_vp]),
     "lace_csr_n": (_i, [_vp]),
@@ -155,6 +156,7 @@ SIGNATURES = {
     "lace_solver_grad_values": (_i, [_vp, _vp, _vp, _vp]),
     "lace_timing_enable": (_i, [_i]),
     "lace_timing_reset": (_i, []),
+    "lace_timing_only": (_i, [C.c_char_p]),
     "lace_timing_get": (_i, [C.c_char_p, _P(_d), _P(C.c_long)]),
     "lace_level_sets": (_i, [_i, _P(_i), _P(_i), _i, _P(_i), _P(_i)]),
     "lace_gen_laplace2d": (_i, [_i, _i, _DM]),

@@ -1,6 +1,7 @@
 // runtime.hip -- library initialisation, error text, device memory entry
 // points and the kernel-timing registry.
 #include <cstdarg>
+#include <algorithm>
 #include <map>
 #include <mutex>
 
@@ -91,11 +92,13 @@ struct TimingEntry {
 static std::mutex g_tmu;
 static std::map<std::string, TimingEntry> g_timing;
 static bool g_timing_on = false;
+static std::string g_timing_only;  // empty: time every region
 
 bool timing_on() { return g_timing_on; }
 
 TimedRegion::TimedRegion(const char* n) : name(n) {
   if (!g_timing_on) return;
+  if (!g_timing_only.empty() && g_timing_only != n) return;
   if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) return;
   on = hipEventRecord(a, stream()) == hipSuccess;
 }
@@ -122,6 +125,14 @@ static void drain_timing() {
     }
     kv.second.pending.clear();
   }
+}
+
+// Plain streaming copy, 16 bytes per lane per iteration; the reference
+// kernel for calibrating HBM traffic counters (bytes known exactly).
+__global__ __launch_bounds__(256) void stream_copy_kernel(size_t n16, const uint4* __restrict__ src,
+                                                         uint4* __restrict__ dst) {
+  for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n16; i += (size_t)gridDim.x * 256)
+    dst[i] = src[i];
 }
 
 }  // namespace lace
@@ -167,9 +178,26 @@ int lace_memcpy_d2h(void* dst, const void* src, size_t bytes) {
   return DEV_SUCCESS;
 }
 
+int lace_stream_copy(void* dst, const void* src, size_t bytes) {
+  LACE_TRY(ensure_init());
+  LACE_REQUIRE(bytes % 16 == 0, DEV_ERR_ILLEGAL_VALUE, "lace_stream_copy: bytes must be a multiple of 16");
+  const size_t n16 = bytes / 16;
+  const int grid = (int)std::min<size_t>((n16 + 255) / 256, (size_t)num_cus() * 16);
+  stream_copy_kernel<<<grid > 0 ? grid : 1, 256, 0, stream()>>>(n16, static_cast<const uint4*>(src),
+                                                              static_cast<uint4*>(dst));
+  LACE_HIP(hipGetLastError());
+  return DEV_SUCCESS;
+}
+
 int lace_timing_enable(int on) {
   LACE_TRY(ensure_init());
   g_timing_on = on != 0;
+  return DEV_SUCCESS;
+}
+
+int lace_timing_only(const char* name) {
+  std::lock_guard<std::mutex> lk(g_tmu);
+  g_timing_only = name ? name : "";
   return DEV_SUCCESS;
 }
 

@@ -65,7 +65,7 @@ def _declare(L):
         "orc_spmv": (None, [_i, _p, _p, _p, _d, _p, _d, _p]),
         "orc_givens": (None, [_d, _d, _d, _p, _p]),
         "orc_fgmres_householder": (_i, [_i, _p, _p, _p, _p, _p, _i, _i, _d, _i, _i, _i,
-                                        _p, _p, _p, _p, _p, _p, _p, _p, _p, _i, _p]),
+                                        _p, _p, _p, _p, _p, _p, _p, _p, _p, _i, _p, _i]),
         "orc_fgmres_mgs": (_i, [_i, _p, _p, _p, _p, _p, _i, _i, _d, _i, _i, _i, _i,
                                 _p, _p, _p, _p, _p, _p, _p, _p, _p, _i, _p]),
         "orc_housegen": (None, [_i, _p, _p, _p]),
@@ -406,7 +406,7 @@ class FgmresResult:
 
 
 def _fgmres_common(kind, A, b, x0, m, restart_max, rtol, tol_type, choice, jac, Lf, Uf,
-                   reorth=0, hist_len=4096):
+                   reorth=0, hist_len=4096, nthreads=1):
     x = _cd(x0).copy() if x0 is not None else np.zeros(A.n_rows)
     hist = np.zeros(hist_len)
     so = C.c_int(); ro = C.c_int(); fr = C.c_double()
@@ -420,7 +420,7 @@ def _fgmres_common(kind, A, b, x0, m, restart_max, rtol, tol_type, choice, jac, 
         info = lib().orc_fgmres_householder(
             A.n_rows, *map(_ptr, a), _ptr(_cd(b)), _ptr(x), m, restart_max, rtol, tol_type,
             choice, jac, *map(_ptr, fac), C.byref(so), C.byref(ro), _ptr(hist), hist_len,
-            C.byref(fr))
+            C.byref(fr), nthreads)
     else:
         info = lib().orc_fgmres_mgs(
             A.n_rows, *map(_ptr, a), _ptr(_cd(b)), _ptr(x), m, restart_max, rtol, tol_type,
@@ -430,10 +430,11 @@ def _fgmres_common(kind, A, b, x0, m, restart_max, rtol, tol_type, choice, jac, 
 
 
 def fgmres_householder(A, b, x0=None, m=30, restart_max=1, rtol=1e-10, tol_type=1, choice=0,
-                       jac=5, Lf=None, Uf=None):
-    """src/fgmres_householder_restart.cpp:63-788 (sequential restatement)."""
+                       jac=5, Lf=None, Uf=None, nthreads=1):
+    """src/fgmres_householder_restart.cpp:63-788.  nthreads > 1 runs the
+    n-length loops under OpenMP (CPU baseline); 1 is deterministic."""
     return _fgmres_common("householder", A, b, x0, m, restart_max, rtol, tol_type, choice, jac,
-                          Lf, Uf)
+                          Lf, Uf, nthreads=nthreads)
 
 
 def fgmres_mgs(A, b, x0=None, m=30, restart_max=1, rtol=1e-10, tol_type=1, choice=0, jac=5,

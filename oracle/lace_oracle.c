@@ -538,7 +538,8 @@ ORC_API int orc_fgmres_householder(
     int tol_type, int choice, int jac, const int *lrow, const int *lcol,
     const double *lval, const int *urow, const int *ucol, const double *uval,
     int *searches_out, int *restarts_out, double *hist, int hist_len,
-    double *final_res) {
+    double *final_res, int nthreads) {
+  const int nt = nthreads > 1 ? nthreads : 1;
   double *W = (double *)calloc((size_t)n * (m + 1), sizeof(double));
   double *Z = (double *)calloc((size_t)n * m, sizeof(double));
   double *r = (double *)calloc(n, sizeof(double));
@@ -586,6 +587,7 @@ ORC_API int orc_fgmres_householder(
     for (int s = 0; s < m; ++s) {
       int s1 = s + 1;
       orc_prec_apply(&P, q, &Zc(0, s));                       /* :320-349 */
+#pragma omp parallel for schedule(static) num_threads(nt) if (nt > 1)
       for (int i = 0; i < n; i++) {                           /* :356-367 */
         double t = Wc(i, s1);
         for (int k = arow[i]; k < arow[i + 1]; k++)
@@ -594,8 +596,10 @@ ORC_API int orc_fgmres_householder(
       }
       for (int j = 0; j <= s; ++j) {                          /* :375-469 */
         double sum = 0.0;
+#pragma omp parallel for schedule(static) reduction(+ : sum) num_threads(nt) if (nt > 1)
         for (int i = j; i < n; i++) sum += Wc(i, j) * Wc(i, s1);
         sum = 2.0 * sum;
+#pragma omp parallel for schedule(static) num_threads(nt) if (nt > 1)
         for (int i = j; i < n; i++) Wc(i, s1) = Wc(i, s1) - sum * Wc(i, j);
       }
       if (s < n) {                                            /* :476-517 */
@@ -611,8 +615,10 @@ ORC_API int orc_fgmres_householder(
         if (s1 < n) q[s1] = 1.0;
         for (int j = (s1 < n ? s1 : s); j >= 0; --j) {
           double sum = 0.0;
+#pragma omp parallel for schedule(static) reduction(+ : sum) num_threads(nt) if (nt > 1)
           for (int i = j; i < n; i++) sum += Wc(i, j) * q[i];
           sum = 2.0 * sum;
+#pragma omp parallel for schedule(static) num_threads(nt) if (nt > 1)
           for (int i = j; i < n; i++) q[i] = q[i] - sum * Wc(i, j);
         }
       }
@@ -635,6 +641,7 @@ ORC_API int orc_fgmres_householder(
           for (int i = j - 1; i >= 0; --i)
             y[i] = y[i] - Wc(i, j + 1) * y[j] / Wc(i, i + 1);
         for (int j = 0; j <= s; j++)
+#pragma omp parallel for schedule(static) num_threads(nt) if (nt > 1)
           for (int i = 0; i < n; i++) x[i] = x[i] + Zc(i, j) * y[j];
         fres = fabs(g[s1]);
         restarts = restart;
