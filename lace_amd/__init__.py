@@ -147,6 +147,7 @@ SIGNATURES = {
     "lace_parilu_sweep_bytes": (_d, [_vp]),
     "lace_parilu_apply": (_i, [_vp, _i, _i, _i, _vp, _vp]),
     "lace_parilu_levels": (_i, [_vp, _P(_i), _P(_i)]),
+    "lace_parilu_set_variant": (_i, [_vp, _i]),
     "lace_syncfree_error": (_i, [_vp, _P(_i)]),
     "lace_solver_default_opts": (None, [_P(SolverOpts)]),
     "lace_solver_create": (_i, [_DM, _P(SolverOpts), _P(_vp)]),

@@ -41,56 +41,103 @@ __device__ __forceinline__ bool is_done(const double* misc) { return misc[MS_DON
 // ------------------------------------------------------------ multi-dot
 // partial[b][k] = sum_{i in block b, i >= rowstart} U[i,k] * w[i], k < ncols;
 // with SELF also partial[b][pstride-1] = sum w[i]^2.
+// Sum of NC per-lane values over the 64 lanes of a wave, NC - 1 shuffles
+// instead of NC full reductions: stage s pairs lanes across bit 32>>s and
+// each lane keeps (and completes) half of its remaining values.  Lanes whose
+// low (6 - log2 NC) bits are zero end up holding the total of value index
+// idx(lane); the function stores it to out[idx].  Fixed order: deterministic.
+template <int NC, typename T>
+__device__ __forceinline__ void wave_multi_sum(T (&v)[NC], double* out) {
+  static_assert((NC & (NC - 1)) == 0 && NC <= 64, "NC must be a power of two <= 64");
+  const int lane = threadIdx.x & 63;
+  int idx = 0;
+#pragma unroll
+  for (int half = NC / 2, o = 32; half >= 1; half >>= 1, o >>= 1) {
+    const bool hi = (lane & o) != 0;
+#pragma unroll
+    for (int i = 0; i < half; i++) {
+      const T send = hi ? v[i] : v[i + half];
+      const T keep = hi ? v[i + half] : v[i];
+      v[i] = keep + __shfl_xor(send, o, 64);
+    }
+    if (hi) idx += half;
+  }
+  // the low lane bits not consumed above still split the sum
+  constexpr int used = (NC == 1) ? 0 : (NC == 2 ? 1 : NC == 4 ? 2 : NC == 8 ? 3 : NC == 16 ? 4 : NC == 32 ? 5 : 6);
+#pragma unroll
+  for (int o = 32 >> used; o >= 1; o >>= 1) v[0] += __shfl_xor(v[0], o, 64);
+  if ((lane & ((64 >> used) - 1)) == 0) out[idx] = (double)v[0];
+}
+
+// Multi-dot over one contiguous row slab per block (rows [b*rpb, (b+1)*rpb)
+// intersected with [rowstart, n)).  The four waves of a block split the
+// columns: wave q accumulates columns [q*NC/4, (q+1)*NC/4) for all rows of
+// the slab (lanes stride the rows), so each lane holds only NC/4
+// accumulators and a column's total needs one NC/4-value wave reduction.
+// Totals go to partial[b][0..ncols) and, with SELF, sum w^2 (wave 0) to
+// partial[b][pstride-1].
 template <typename K, int NC, bool SELF>
 __global__ __launch_bounds__(kBS) void mdot_kernel(int n, int ld, const K* __restrict__ U, int ncols,
                                                    const K* __restrict__ w, int rowstart, int rpb,
                                                    double* __restrict__ partial, int pstride,
                                                    const double* __restrict__ misc) {
   if (misc && is_done(misc)) return;
+  constexpr int NW = kBS / 64;
+  constexpr int NCW = NC / NW;  // columns per wave
   const int b = xcd_block(blockIdx.x, gridDim.x);
   const int r0 = max(b * rpb, rowstart), r1 = min((b + 1) * rpb, n);
-  K acc[NC + 1];
+  const int q = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int c0 = q * NCW;
+  const K* Uq = U + (size_t)c0 * ld;
+  K acc[NCW];
+  K self = K(0);
 #pragma unroll
-  for (int k = 0; k <= NC; k++) acc[k] = K(0);
-  for (int i = r0 + (int)threadIdx.x; i < r1; i += kBS) {
-    const K wi = w[i];
+  for (int k = 0; k < NCW; k++) acc[k] = K(0);
+  if (c0 < ncols || (SELF && q == 0)) {
+    for (int i = r0 + lane; i < r1; i += 64) {
+      const K wi = w[i];
 #pragma unroll
-    for (int k = 0; k < NC; k++)
-      if (k < ncols) acc[k] += U[(size_t)k * ld + i] * wi;
-    if (SELF) acc[NC] += wi * wi;
-  }
-  __shared__ double red[kBS / 64][NC + 1];
-  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-#pragma unroll
-  for (int k = 0; k <= NC; k++) {
-    if (k < ncols || (SELF && k == NC)) {
-      double v = wave_sum((double)acc[k]);
-      if (lane == 0) red[wid][k] = v;
+      for (int k = 0; k < NCW; k++)
+        if (c0 + k < ncols) acc[k] += Uq[(size_t)k * ld + i] * wi;
+      if (SELF && q == 0) self += wi * wi;
     }
+  }
+  __shared__ double red[NC + 1];
+  wave_multi_sum<NCW>(acc, red + c0);
+  if (SELF && q == 0) {
+    const double sv = wave_sum((double)self);
+    if (lane == 0) red[NC] = sv;
   }
   __syncthreads();
   for (int k = threadIdx.x; k <= NC; k += kBS) {
-    if (k < ncols) partial[(size_t)b * pstride + k] = red[0][k] + red[1][k] + red[2][k] + red[3][k];
-    if (SELF && k == NC)
-      partial[(size_t)b * pstride + pstride - 1] = red[0][NC] + red[1][NC] + red[2][NC] + red[3][NC];
+    if (k < ncols) partial[(size_t)b * pstride + k] = red[k];
+    if (SELF && k == NC) partial[(size_t)b * pstride + pstride - 1] = red[NC];
   }
 }
 
 // Sum of the per-block partials in a fixed order: out[k] for k < cnt and,
-// if self, out[cnt] = the self slot.  One block of 256 threads, 4 segments
-// per column.
+// if self, out[cnt] = the self slot.  One block of 256 threads: 4 segments
+// per column, each summed with 4 independent accumulators so the loads
+// overlap, then combined in a fixed order.
 __device__ void finalize_partials(int nb, int cnt, bool self, int pstride,
                                   const double* __restrict__ partial, double* out) {
   __shared__ double seg[4][72];
   const int tot = cnt + (self ? 1 : 0);
   for (int base = 0; base < tot; base += 64) {
     const int c = base + (threadIdx.x & 63), s = threadIdx.x >> 6;
-    double v = 0.0;
+    double v0 = 0.0, v1 = 0.0, v2 = 0.0, v3 = 0.0;
     if (c < tot) {
       const int slot = (self && c == cnt) ? pstride - 1 : c;
-      for (int bb = s; bb < nb; bb += 4) v += partial[(size_t)bb * pstride + slot];
+      int bb = s;
+      for (; bb + 12 < nb; bb += 16) {
+        v0 += partial[(size_t)bb * pstride + slot];
+        v1 += partial[(size_t)(bb + 4) * pstride + slot];
+        v2 += partial[(size_t)(bb + 8) * pstride + slot];
+        v3 += partial[(size_t)(bb + 12) * pstride + slot];
+      }
+      for (; bb < nb; bb += 4) v0 += partial[(size_t)bb * pstride + slot];
     }
-    seg[s][threadIdx.x & 63] = v;
+    seg[s][threadIdx.x & 63] = (v0 + v1) + (v2 + v3);
     __syncthreads();
     if (threadIdx.x < 64 && c < tot)
       out[c] = seg[0][threadIdx.x] + seg[1][threadIdx.x] + seg[2][threadIdx.x] + seg[3][threadIdx.x];
@@ -363,6 +410,8 @@ int HHEngine<K>::init(int n_, int m_, bool z_float_) {
   z_float = z_float_;
   ld = ((n + 63) / 64) * 64;
   lay = SmallLayout{m};
+  // about 1024 row slabs (four per CU), at least 1024 rows each
+  rpb = std::max(1024, ((ceil_div(n, 1024) + 63) / 64) * 64);
   nb = ceil_div(n, rpb);
   pstride = m + 3;
   LACE_TRY(U.alloc(sizeof(K) * (size_t)ld * (m + 1)));

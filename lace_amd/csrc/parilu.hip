@@ -155,6 +155,195 @@ __global__ __launch_bounds__(kSweepBS) void parilu_nonlinres_kernel(
   if (threadIdx.x == 0) partial[blockIdx.x] = tot;
 }
 
+// Structurally symmetric sweep: one thread per L position p = (i, j),
+// j <= i, updating l_ij (j < i) and u_ji from one merge, so both writes land
+// at p (coalesced).  With a symmetric pattern the U column j holds rows k of
+// row j's strict-lower pattern plus j, stored at row j's L positions, so
+// every operand sits in row i or row j of the shared pattern:
+//   l_ij = (a_ij - sum_{k in Lr(i) & Lr(j)} l_ik u_kj) / u_jj
+//   u_ji =  a_ji - sum_{k in Lr(i) & Lr(j)} l_jk u_ki
+//   u_ii =  a_ii - sum_{k in Lr(i)} l_ik u_ki                      (j == i)
+// Matches are taken in ascending k, each rounded product subtracted, then
+// the reference's closing term (k = min(i,j)) is subtracted and added back
+// (src/parilu_v0_3.cpp:84-90), so results equal the general kernel bit for
+// bit.  Rows with at most MAXL strict-lower entries keep both rows in
+// registers (all loads issued at once); longer rows fall back to a loop.
+template <typename T, bool STEP, int MAXL>
+__global__ __launch_bounds__(kSweepBS) void parilu_sym_sweep_kernel(
+    int np, const int* __restrict__ prow, const int* __restrict__ pcol,
+    const int* __restrict__ lptr, const T* __restrict__ AL, const T* __restrict__ AU,
+    const T* __restrict__ VLin, const T* __restrict__ VUin, T* __restrict__ VLout,
+    T* __restrict__ VUout, double* __restrict__ partial) {
+#pragma clang fp contract(off)
+  const int b = xcd_block(blockIdx.x, gridDim.x);
+  const int p = b * kSweepBS + threadIdx.x;
+  double d2 = 0.0;
+  if (p < np) {
+    const int i = prow[p], j = pcol[p];
+    const int i0 = lptr[i], i1 = lptr[i + 1] - 1;  // row i strict-lower [i0,i1), diag i1
+    const int j0 = lptr[j], j1 = lptr[j + 1] - 1;
+    const int ni = i1 - i0, nj = j1 - j0;
+    T sL = AL[p];
+    T sU = AU[p];
+    if (ni <= MAXL && nj <= MAXL) {
+      int ci[MAXL], cj[MAXL];
+      T li[MAXL], ui[MAXL], lj[MAXL], uj[MAXL];
+#pragma unroll
+      for (int t = 0; t < MAXL; t++) {
+        const bool oi = t < ni, oj = t < nj;
+        ci[t] = oi ? pcol[i0 + t] : -1;
+        li[t] = oi ? VLin[i0 + t] : T(0);
+        ui[t] = oi ? VUin[i0 + t] : T(0);
+        cj[t] = oj ? pcol[j0 + t] : -2;
+        lj[t] = oj ? VLin[j0 + t] : T(0);
+        uj[t] = oj ? VUin[j0 + t] : T(0);
+      }
+      // row i's columns ascend, so matches come out in ascending k
+#pragma unroll
+      for (int a = 0; a < MAXL; a++) {
+#pragma unroll
+        for (int c = 0; c < MAXL; c++) {
+          if (ci[a] == cj[c]) {
+            sL = sL - li[a] * uj[c];
+            sU = sU - lj[c] * ui[a];
+          }
+        }
+      }
+    } else {
+      int a = i0, c = j0;
+      while (a < i1 && c < j1) {
+        const int ka = pcol[a], kc = pcol[c];
+        if (ka == kc) {
+          sL = sL - VLin[a] * VUin[c];
+          sU = sU - VLin[c] * VUin[a];
+        }
+        a += ka <= kc;
+        c += ka >= kc;
+      }
+    }
+    const T uold = VUin[p];
+    const T pu = T(1) * uold;  // l_jj * u_ji (unit diagonal)
+    sU = sU - pu;
+    sU = sU + pu;
+    VUout[p] = sU;
+    if (STEP) {
+      const double d = (double)uold - (double)sU;
+      d2 = d * d;
+    }
+    if (j < i) {
+      const T ujj = VUin[j1];
+      const T lold = VLin[p];
+      const T pl = lold * ujj;
+      sL = sL - pl;
+      sL = sL + pl;
+      const T t = sL / ujj;
+      VLout[p] = t;
+      if (STEP) {
+        const double d = (double)lold - (double)t;
+        d2 += d * d;
+      }
+    }
+  }
+  if (STEP) {
+    __shared__ double sm[kSweepBS / 64];
+    double tot = block_sum<kSweepBS>(d2, sm);
+    if (threadIdx.x == 0) partial[b] = tot;
+  }
+}
+
+// Same update as parilu_sym_sweep_kernel with one thread per row: row i's
+// positions (strict-lower entries and the diagonal) are loaded once into
+// registers and every position of the row is updated from them, so only
+// the partner rows j are gathered.  Requires every row to have at most MAXL
+// strict-lower entries (checked on the host).  Identical arithmetic.
+template <typename T, bool STEP, int MAXL>
+__global__ __launch_bounds__(kSweepBS) void parilu_symrow_sweep_kernel(
+    int n, const int* __restrict__ pcol, const int* __restrict__ lptr,
+    const T* __restrict__ AL, const T* __restrict__ AU, const T* __restrict__ VLin,
+    const T* __restrict__ VUin, T* __restrict__ VLout, T* __restrict__ VUout,
+    double* __restrict__ partial) {
+#pragma clang fp contract(off)
+  const int b = xcd_block(blockIdx.x, gridDim.x);
+  const int i = b * kSweepBS + threadIdx.x;
+  double d2 = 0.0;
+  if (i < n) {
+    const int i0 = lptr[i], i1 = lptr[i + 1] - 1;  // strict-lower [i0,i1), diagonal at i1
+    const int ni = i1 - i0;
+    int ci[MAXL];
+    T li[MAXL], ui[MAXL];
+#pragma unroll
+    for (int t = 0; t < MAXL; t++) {
+      const bool o = t < ni;
+      ci[t] = o ? pcol[i0 + t] : -1;
+      li[t] = o ? VLin[i0 + t] : T(0);
+      ui[t] = o ? VUin[i0 + t] : T(0);
+    }
+    // diagonal: u_ii = a_ii - sum_k l_ik u_ki, closing term 1*u_ii
+    {
+      T s = AU[i1];
+#pragma unroll
+      for (int t = 0; t < MAXL; t++)
+        if (t < ni) s = s - li[t] * ui[t];
+      const T uold = VUin[i1];
+      const T pu = T(1) * uold;
+      s = s - pu;
+      s = s + pu;
+      VUout[i1] = s;
+      if (STEP) {
+        const double d = (double)uold - (double)s;
+        d2 += d * d;
+      }
+    }
+    // strict-lower positions (i, j): l_ij and u_ji
+#pragma unroll
+    for (int t = 0; t < MAXL; t++) {
+      if (t < ni) {
+        const int p = i0 + t, j = ci[t];
+        const int j0 = lptr[j], j1 = lptr[j + 1] - 1;
+        const int nj = j1 - j0;
+        T sL = AL[p], sU = AU[p];
+        // partner row j: its strict-lower columns are all < j, so only row
+        // i's entries before t can match
+#pragma unroll
+        for (int c = 0; c < MAXL; c++) {
+          if (c < nj) {
+            const int kc = pcol[j0 + c];
+            const T ljc = VLin[j0 + c], ujc = VUin[j0 + c];
+#pragma unroll
+            for (int a = 0; a < MAXL; a++) {
+              if (a < t && ci[a] == kc) {
+                sL = sL - li[a] * ujc;
+                sU = sU - ljc * ui[a];
+              }
+            }
+          }
+        }
+        const T uold = ui[t];
+        const T pu = T(1) * uold;
+        sU = sU - pu;
+        sU = sU + pu;
+        VUout[p] = sU;
+        const T ujj = VUin[j1];
+        const T lold = li[t];
+        const T pl = lold * ujj;
+        sL = sL - pl;
+        sL = sL + pl;
+        const T tv = sL / ujj;
+        VLout[p] = tv;
+        if (STEP) {
+          const double d = (double)uold - (double)sU, e = (double)lold - (double)tv;
+          d2 += d * d + e * e;
+        }
+      }
+    }
+  }
+  if (STEP) {
+    __shared__ double sm[kSweepBS / 64];
+    double tot = block_sum<kSweepBS>(d2, sm);
+    if (threadIdx.x == 0) partial[b] = tot;
+  }
+}
+
 template <typename To, typename From>
 __global__ void convert_kernel(long n, const From* __restrict__ in, To* __restrict__ out) {
   const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -221,6 +410,37 @@ static int build_pattern(const data_d_matrix* A, ParILU* P) {
   double s = 0.0;
   for (int k = 0; k < nnz; k++) s += A->val[k] * A->val[k];
   P->anorm = std::sqrt(s);
+  // symmetric pattern: the CSCU arrays of U equal the CSRL arrays of L
+  P->sym = P->h_lrow == P->h_ucp && P->h_lcol == P->h_uri;
+  if (P->sym) {
+    std::vector<int> prow(P->nnzL);
+    P->maxl = 0;
+    for (int i = 0; i < n; i++) {
+      for (int p = P->h_lrow[i]; p < P->h_lrow[i + 1]; p++) prow[p] = i;
+      P->maxl = std::max(P->maxl, P->h_lrow[i + 1] - P->h_lrow[i] - 1);
+    }
+    std::vector<double> al(P->nnzL), au(P->nnzL);
+    for (int i = 0; i < n; i++)
+      for (int k = A->row[i]; k < A->row[i + 1]; k++) {
+        const int d = dst[k], j = A->col[k];
+        if (d >= 0) {
+          al[d] = A->val[k];                    // a_ij, j < i, at L position (i,j)
+        } else {
+          au[-1 - d] = A->val[k];               // a_ij, j >= i, at L position (j,i)
+          if (j == i) al[-1 - d] = A->val[k];
+        }
+      }
+    LACE_TRY(upload(P->prow, prow.data(), sizeof(int) * P->nnzL));
+    const size_t esz = P->elem_size();
+    if (P->prec == LACE_F64) {
+      LACE_TRY(upload(P->AL, al.data(), sizeof(double) * P->nnzL));
+      LACE_TRY(upload(P->AU, au.data(), sizeof(double) * P->nnzL));
+    } else {
+      std::vector<float> alf(al.begin(), al.end()), auf(au.begin(), au.end());
+      LACE_TRY(upload(P->AL, alf.data(), esz * P->nnzL));
+      LACE_TRY(upload(P->AU, auf.data(), esz * P->nnzL));
+    }
+  }
 
   LACE_TRY(upload(P->arow, A->row, sizeof(int) * (n + 1)));
   LACE_TRY(upload(P->acol, A->col, sizeof(int) * nnz));
@@ -270,14 +490,62 @@ int ParILU::reset() {
   }
   LACE_HIP(hipGetLastError());
   cur = 0;
+  csr_dirty = true;
   return DEV_SUCCESS;
+}
+
+template <typename T, int MAXL>
+static void launch_sym(ParILU* P, bool with_step, int in, int out, int nb) {
+  if (with_step)
+    parilu_sym_sweep_kernel<T, true, MAXL><<<nb, kSweepBS, 0, stream()>>>(
+        P->nnzL, P->prow.as<int>(), P->lcol.as<int>(), P->lrow.as<int>(), P->AL.as<T>(),
+        P->AU.as<T>(), P->lval[in].as<T>(), P->uval[in].as<T>(), P->lval[out].as<T>(),
+        P->uval[out].as<T>(), P->partial.as<double>());
+  else
+    parilu_sym_sweep_kernel<T, false, MAXL><<<nb, kSweepBS, 0, stream()>>>(
+        P->nnzL, P->prow.as<int>(), P->lcol.as<int>(), P->lrow.as<int>(), P->AL.as<T>(),
+        P->AU.as<T>(), P->lval[in].as<T>(), P->uval[in].as<T>(), P->lval[out].as<T>(),
+        P->uval[out].as<T>(), nullptr);
 }
 
 template <typename T>
 static int sweep_t(ParILU* P, bool with_step) {
-  const int nb = ceil_div(P->nnz, kSweepBS);
   const int in = P->cur, out = 1 - P->cur;
   TimedRegion tr("parilu_sweep");
+  // per-row kernel for FP32 (measured faster on the 128^3 stencil); FP64,
+  // whose divisions are long sequences, keeps the per-position kernel's
+  // wider parallelism
+  if (P->sym && P->maxl <= 4 && (P->variant == 3 || (P->variant == 0 && sizeof(T) == 4))) {
+    const int nb = ceil_div(P->n, kSweepBS);
+    if (with_step)
+      parilu_symrow_sweep_kernel<T, true, 4><<<nb, kSweepBS, 0, stream()>>>(
+          P->n, P->lcol.as<int>(), P->lrow.as<int>(), P->AL.as<T>(), P->AU.as<T>(),
+          P->lval[in].as<T>(), P->uval[in].as<T>(), P->lval[out].as<T>(), P->uval[out].as<T>(),
+          P->partial.as<double>());
+    else
+      parilu_symrow_sweep_kernel<T, false, 4><<<nb, kSweepBS, 0, stream()>>>(
+          P->n, P->lcol.as<int>(), P->lrow.as<int>(), P->AL.as<T>(), P->AU.as<T>(),
+          P->lval[in].as<T>(), P->uval[in].as<T>(), P->lval[out].as<T>(), P->uval[out].as<T>(),
+          nullptr);
+    LACE_HIP(hipGetLastError());
+    P->last_nb = nb;
+    P->cur = out;
+    P->csr_dirty = true;
+    return DEV_SUCCESS;
+  }
+  if (P->sym && P->variant != 1) {
+    const int nb = ceil_div(P->nnzL, kSweepBS);
+    if (P->maxl <= 4) launch_sym<T, 4>(P, with_step, in, out, nb);
+    else launch_sym<T, 8>(P, with_step, in, out, nb);
+    LACE_HIP(hipGetLastError());
+    P->last_nb = nb;
+    P->cur = out;
+    P->csr_dirty = true;
+    return DEV_SUCCESS;
+  }
+  const int nb = ceil_div(P->nnz, kSweepBS);
+  P->last_nb = nb;
+  P->csr_dirty = true;
   if (with_step)
     parilu_sweep_kernel<T, true><<<nb, kSweepBS, 0, stream()>>>(
         P->nnz, P->arowidx.as<int>(), P->acol.as<int>(), P->aval.as<T>(), P->lrow.as<int>(),
@@ -298,7 +566,7 @@ int ParILU::sweep(bool with_step) {
 }
 
 int ParILU::read_step(double* step) {
-  const int nb = ceil_div(nnz, kSweepBS);
+  const int nb = last_nb;
   sum_partials_kernel<<<1, 256, 0, stream()>>>(nb, partial.as<double>(), 1.0 / anorm,
                                                scalar.as<double>());
   LACE_HIP(hipGetLastError());
@@ -329,10 +597,11 @@ int ParILU::nonlinres(double* out) {
 }
 
 double ParILU::sweep_bytes() const {
-  // Algorithmic traffic of one sweep: A value, column and row index per
-  // nonzero, the old and new factor value per nonzero, row pointers.
+  // Algorithmic traffic of one sweep: per nonzero of A its value and column
+  // index, the previous and the new factor value; plus the row pointers:
+  // nnz*(s_A + 4 + 2 s_F) + 4(n+1).
   const double es = (double)elem_size();
-  return (double)nnz * (es + 4.0 + 4.0 + 2.0 * es) + 4.0 * (n + 1);
+  return (double)nnz * (es + 4.0 + 2.0 * es) + 4.0 * (n + 1);
 }
 
 int ParILU::get(data_d_matrix* L, data_d_matrix* U) {
@@ -448,6 +717,12 @@ int lace_parilu_nonlinres(lace_parilu* P, double* nonlinres) {
 
 double lace_parilu_sweep_bytes(const lace_parilu* P) {
   return reinterpret_cast<const ParILU*>(P)->sweep_bytes();
+}
+
+int lace_parilu_set_variant(lace_parilu* Ph, int general) {
+  LACE_REQUIRE(general >= 0 && general <= 3, DEV_ERR_ILLEGAL_VALUE, "variant %d", general);
+  reinterpret_cast<ParILU*>(Ph)->variant = general;
+  return DEV_SUCCESS;
 }
 
 int lace_parilu_levels(const lace_parilu* Ph, int* nlev_L, int* nlev_U) {

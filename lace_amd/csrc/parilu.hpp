@@ -20,6 +20,7 @@ struct TriPlan {
   bool upper = false;
   int nlev = 0;
   int nslots = 0;
+  bool unit_diag = false;  // every diagonal value is exactly 1 (ParILU's L, L^T)
   DevBuf row, col;
   DevBuf perm, lptr;        // rows by level, level pointers (unpadded)
   DevBuf sched;             // padded schedule for the sync-free kernel
@@ -49,6 +50,18 @@ struct ParILU {
   int cur = 0;
   DevBuf partial, scalar;
   std::vector<int> h_lrow, h_lcol, h_ucp, h_uri;
+  // Structurally symmetric patterns: L (CSRL) and U (CSCU) then share one
+  // pattern (row i's strict-lower columns, then i), so the sweep runs over
+  // L positions and updates l_ij and u_ji from one merge.  prow: row of
+  // each position; AL/AU: a_ij and a_ji in position order (iteration
+  // precision); maxl: longest strict-lower row.
+  bool sym = false;
+  // sweep kernel: 0 auto, 1 general CSR-L/CSC-U, 2 symmetric per-position,
+  // 3 symmetric per-row
+  int variant = 0;
+  int maxl = 0;
+  int last_nb = 0;  // per-block partials written by the last sweep
+  DevBuf prow, AL, AU;
 
   // Triangular solves.  lower: L (CSRL arrays, values lval[cur]).
   // upper: U in CSR (diagonal first), values gathered from the CSC buffer

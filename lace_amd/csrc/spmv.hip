@@ -20,9 +20,13 @@
 namespace lace {
 
 constexpr int kSpmvBS = 256;
+constexpr int kSpmvRows = 4;  // rows per lane group
 
 // MODE 0: y = alpha*A*x + beta*y  (beta == 0 never reads y)
 // MODE 1: y = b - A*x             (residual form)
+// Each lane group of G lanes handles R rows of the block's slab; the first
+// G nonzeros of all R rows are loaded before any is used, so a wave keeps
+// 3R independent loads per lane in flight (row bounds, column/value, x).
 template <typename TV, typename TX, typename TY, int G, int MODE>
 __global__ __launch_bounds__(kSpmvBS) void spmv_kernel(int n, const int* __restrict__ row,
                                                        const int* __restrict__ col,
@@ -31,23 +35,40 @@ __global__ __launch_bounds__(kSpmvBS) void spmv_kernel(int n, const int* __restr
                                                        const TY* __restrict__ b, double alpha,
                                                        double beta) {
   using ACC = typename std::conditional<sizeof(TV) == 8 || sizeof(TY) == 8, double, float>::type;
+  constexpr int R = kSpmvRows;
+  constexpr int GPB = kSpmvBS / G;  // lane groups per block
   const int blk = xcd_block(blockIdx.x, gridDim.x);
-  const int gid = blk * kSpmvBS + threadIdx.x;
-  const int i = gid / G;
   const int lane = threadIdx.x & (G - 1);
-  ACC s = ACC(0);
-  if (i < n) {
-    const int k1 = row[i + 1];
-    for (int k = row[i] + lane; k < k1; k += G) s += (ACC)val[k] * (ACC)x[col[k]];
+  const int g = threadIdx.x / G;
+  const int base = blk * GPB * R + g;
+  int kb[R], ke[R];
+#pragma unroll
+  for (int r = 0; r < R; r++) {
+    const int i = base + r * GPB;
+    kb[r] = i < n ? row[i] : 0;
+    ke[r] = i < n ? row[i + 1] : 0;
   }
-  s = group_sum<G>(s);
-  if (i < n && lane == 0) {
-    if (MODE == 1) {
-      y[i] = (TY)((ACC)b[i] - s);
-    } else {
-      ACC v = (ACC)alpha * s;
-      if (beta != 0.0) v += (ACC)beta * (ACC)y[i];
-      y[i] = (TY)v;
+  ACC s[R];
+#pragma unroll
+  for (int r = 0; r < R; r++) {
+    const int k = kb[r] + lane;
+    s[r] = k < ke[r] ? (ACC)val[k] * (ACC)x[col[k]] : ACC(0);
+  }
+#pragma unroll
+  for (int r = 0; r < R; r++)
+    for (int k = kb[r] + lane + G; k < ke[r]; k += G) s[r] += (ACC)val[k] * (ACC)x[col[k]];
+#pragma unroll
+  for (int r = 0; r < R; r++) {
+    const int i = base + r * GPB;
+    const ACC t = group_sum<G>(s[r]);
+    if (i < n && lane == 0) {
+      if (MODE == 1) {
+        y[i] = (TY)((ACC)b[i] - t);
+      } else {
+        ACC v = (ACC)alpha * t;
+        if (beta != 0.0) v += (ACC)beta * (ACC)y[i];
+        y[i] = (TY)v;
+      }
     }
   }
 }
@@ -55,8 +76,8 @@ __global__ __launch_bounds__(kSpmvBS) void spmv_kernel(int n, const int* __restr
 template <typename TV, typename TX, typename TY, int MODE>
 static int launch_spmv(const DevCSR& A, const TX* x, TY* y, const TY* b, double alpha, double beta) {
   const int g = A.group;
-  const long threads = (long)A.n * g;
-  const int nb = ceil_div(threads, kSpmvBS);
+  const int rows_per_block = (kSpmvBS / g) * kSpmvRows;
+  const int nb = ceil_div(A.n, rows_per_block);
   const int* row = A.row.as<int>();
   const int* col = A.col.as<int>();
   const TV* val = A.val.as<TV>();

@@ -258,6 +258,23 @@ int tri_solve_jacobi(const TriPlan& P, const T* val, const T* r, T* y, T* tmp, i
     LACE_HIP(hipMemsetAsync(y, 0, sizeof(T) * P.n, stream()));
     return DEV_SUCCESS;
   }
+  if (P.unit_diag) {
+    // the first sweep from y = 0 returns r / 1 = r exactly, so sweep 2 reads
+    // r directly and one launch is saved
+    if (sweeps == 1) {
+      LACE_HIP(hipMemcpyAsync(y, r, sizeof(T) * P.n, hipMemcpyDeviceToDevice, stream()));
+      return DEV_SUCCESS;
+    }
+    const int rest = sweeps - 1;
+    T* a = (rest % 2 == 1) ? y : tmp;
+    T* b = (rest % 2 == 1) ? tmp : y;
+    LACE_TRY(jacobi_sweep<T>(P, val, r, r, a, nullptr));
+    for (int s = 1; s < rest; s++) {
+      LACE_TRY(jacobi_sweep<T>(P, val, r, a, b, nullptr));
+      std::swap(a, b);
+    }
+    return DEV_SUCCESS;
+  }
   // ping-pong so that the last sweep lands in y
   T* a = (sweeps % 2 == 1) ? y : tmp;
   T* b = (sweeps % 2 == 1) ? tmp : y;
@@ -279,6 +296,7 @@ template int tri_solve_jacobi<float>(const TriPlan&, const float*, const float*,
 // ---------------------------------------------------------------- ParILU solves
 int ParILU::build_trsv() {
   LACE_TRY(lower.build(n, h_lrow.data(), h_lcol.data(), false));
+  lower.unit_diag = true;  // CSRL with the unit diagonal (never updated by sweeps)
   LACE_TRY(lowerT.build(n, h_ucp.data(), h_uri.data(), false));
   // U in CSR: transpose of the CSC arrays; map CSR slot -> CSC slot.
   {
@@ -308,6 +326,7 @@ int ParILU::build_trsv() {
         map[q] = p;
       }
     LACE_TRY(upperT.build(n, r.data(), c.data(), true));
+    upperT.unit_diag = true;
     LACE_TRY(upload(l_t_from_csr, map.data(), sizeof(int) * nnzL));
   }
   const size_t es = elem_size();

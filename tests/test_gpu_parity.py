@@ -51,6 +51,31 @@ def test_parilu_jacobi_bitexact(gpu, orc, name, prec):
         assert np.array_equal(uv.astype(dt), st.Uval), f"U differs after sweep {k}"
 
 
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+@pytest.mark.parametrize("variant", [0, 2, 3])
+@pytest.mark.parametrize("name", ["convdiff", "trefethen20"])
+def test_parilu_symmetric_kernels_match_general(gpu, orc, prec, variant, name):
+    """Structurally symmetric patterns run the per-row (variant 0 when rows
+    have <= 4 strict-lower entries) or per-position (2) kernel; both must
+    equal the general CSR-L / CSC-U kernel (1) bit for bit.  Trefethen_20
+    has up to 5 strict-lower entries per row, so variant 0 falls back to the
+    per-position kernel there."""
+    A = orc.convdiff3d(12, 10, 9) if name == "convdiff" else orc.read_mtx(
+        os.path.join(GOLDEN, "Trefethen_20.mtx"))
+    lp = gpu.LACE_F64 if prec == "f64" else gpu.LACE_F32
+    P1 = gpu.ParILU(gpu.HostCSR.from_csr(A), lp)
+    P2 = gpu.ParILU(gpu.HostCSR.from_csr(A), lp)
+    gpu.check(gpu.lib().lace_parilu_set_variant(P1.h, variant))
+    gpu.check(gpu.lib().lace_parilu_set_variant(P2.h, 1))
+    for _ in range(4):
+        s1 = P1.sweeps(1, want_step=True)
+        s2 = P2.sweeps(1, want_step=True)
+        (_, _, l1), (_, _, u1) = P1.factors()
+        (_, _, l2), (_, _, u2) = P2.factors()
+        assert np.array_equal(l1, l2) and np.array_equal(u1, u2)
+        assert abs(s1 - s2) <= 1e-12 * s2
+
+
 def test_parilu_reset_and_nonlinres(gpu, orc):
     """Device ||A - LU||_F on pattern(A) after every sweep equals the
     oracle's control/norms.cpp restatement; reset restores L0/U0."""
