@@ -205,9 +205,11 @@ class HostCSR:
     """data_d_matrix viewing numpy arrays owned by this object (inputs)."""
 
     def __init__(self, row, col, val, n_cols=None, storage=Magma_CSR):
-        self.row = np.ascontiguousarray(row, dtype=np.int32)
-        self.col = np.ascontiguousarray(col, dtype=np.int32)
-        self.val = np.ascontiguousarray(val, dtype=np.float64)
+        # private copies: in-place routines (data_dcsrilu0_mkl) must not
+        # write through to the caller's arrays
+        self.row = np.array(row, dtype=np.int32, copy=True, order="C")
+        self.col = np.array(col, dtype=np.int32, copy=True, order="C")
+        self.val = np.array(val, dtype=np.float64, copy=True, order="C")
         n = len(self.row) - 1
         m = DMatrix()
         m.storage_type = storage

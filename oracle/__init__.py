@@ -71,6 +71,7 @@ def _declare(L):
         "orc_housegen": (None, [_i, _p, _p, _p]),
         "orc_hqrd": (None, [_i, _i, _p, _p, _p]),
         "orc_philox4x32_10": (None, [_p, _p, _p]),
+        "orc_set_threads": (None, [_i]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -219,6 +220,11 @@ def _coo_to_csr(n, rows, cols, vals) -> CSR:
     row = np.zeros(n + 1, dtype=np.int32)
     np.add.at(row, r + 1, 1)
     return CSR(n, n, np.cumsum(row).astype(np.int32), c.astype(np.int32), v)
+
+
+def set_threads(n: int) -> None:
+    """OpenMP thread count of the oracle's parallel loops (0: default)."""
+    lib().orc_set_threads(n)
 
 
 def philox4x32_10(ctr, key):

@@ -848,3 +848,14 @@ ORC_API void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2],
   }
   out[0] = c0; out[1] = c1; out[2] = c2; out[3] = c3;
 }
+
+/* Thread count of the OpenMP loops above (n <= 0 restores the default). */
+ORC_API void orc_set_threads(int n) {
+#ifdef _OPENMP
+  static int dflt = 0;
+  if (dflt == 0) dflt = omp_get_max_threads();
+  omp_set_num_threads(n > 0 ? n : dflt);
+#else
+  (void)n;
+#endif
+}

@@ -1,0 +1,248 @@
+"""The C-ABI library on the CPU: it loads, exports every symbol that
+include/lace.h declares, the header is plain C with the reference's
+container layout, and the host-side routines (reader, format conversions,
+merge, residual check, exact ILU(0), level sets, generators) agree with the
+oracle.  No device calls."""
+import ctypes as C
+import os
+import re
+import subprocess
+import tempfile
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN, ROOT
+
+HEADER = os.path.join(ROOT, "include", "lace.h")
+
+
+def declared_functions():
+    src = open(HEADER).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    names = re.findall(r"\b([A-Za-z_]\w*)\s*\(", src)
+    skip = {"if", "sizeof", "__attribute__", "aligned", "visibility", "push", "pop", "defined"}
+    # function declarations: identifier followed by '(' at statement level
+    decls = set()
+    for m in re.finditer(r"[\w\*\s]+?\b([A-Za-z_]\w*)\s*\([^;{]*\)\s*;", src):
+        decls.add(m.group(1))
+    return sorted(d for d in decls if d not in skip)
+
+
+def test_library_exports_every_declared_symbol(lace):
+    names = declared_functions()
+    assert len(names) > 60
+    out = subprocess.run(["nm", "-D", "--defined-only", lace.LIB_PATH], capture_output=True,
+                         text=True, check=True).stdout
+    exported = {line.split()[-1] for line in out.splitlines() if " T " in line}
+    missing = [n for n in names if n not in exported]
+    assert not missing, missing
+    # and every binding in the Python module resolves
+    for n in lace.SIGNATURES:
+        assert hasattr(lace.lib(), n)
+
+
+def test_header_is_plain_c_with_reference_layout(tmp_path):
+    prog = tmp_path / "probe.c"
+    prog.write_text(r'''
+#include <stdio.h>
+#include <stddef.h>
+#include "lace.h"
+int main(void) {
+  printf("%zu %zu %zu %zu %zu %zu %zu %zu\n", sizeof(data_d_matrix),
+         offsetof(data_d_matrix, val), offsetof(data_d_matrix, row),
+         offsetof(data_d_matrix, col), offsetof(data_d_matrix, major),
+         sizeof(data_d_preconditioner_log), sizeof(data_d_gmres_log),
+         offsetof(data_d_gmres_param, parilu_reduction));
+  return 0;
+}
+''')
+    exe = tmp_path / "probe"
+    subprocess.run(["gcc", "-std=c99", "-Wall", "-Werror", "-I", os.path.dirname(HEADER),
+                    str(prog), "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split()
+    # reference include/sparse_types.h: pointer members aligned to 64 bytes
+    assert [int(v) for v in out] == [384, 64, 192, 320, 360, 128, 64, 40]
+
+
+def _read(lace, path):
+    A = lace.DMatrix()
+    lace.check(lace.lib().data_z_csr_mtx(C.byref(A), path.encode()))
+    return lace.owned_to_numpy(A)
+
+
+@pytest.mark.parametrize("name", ["io_test.mtx", "Trefethen_20.mtx", "bcsr_test.mtx"])
+def test_reader_matches_oracle(lace, orc, name):
+    row, col, val = _read(lace, os.path.join(GOLDEN, name))
+    ref = orc.read_mtx(os.path.join(GOLDEN, name))
+    assert np.array_equal(row, ref.row) and np.array_equal(col, ref.col)
+    assert np.array_equal(val, ref.val)
+
+
+def test_writer_round_trip(lace, orc, tmp_path):
+    A = orc.convdiff3d(4, 3, 2)
+    H = lace.HostCSR.from_csr(A)
+    p = str(tmp_path / "a.mtx")
+    lace.check(lace.lib().data_zwrite_csr_mtx(H.m, lace.MagmaRowMajor, p.encode()))
+    row, col, val = _read(lace, p)
+    assert np.array_equal(row, A.row) and np.array_equal(col, A.col) and np.array_equal(val, A.val)
+    first = open(p).read().splitlines()[:2]
+    assert first[0] == "%%MatrixMarket matrix coordinate real general"
+    assert first[1] == f"{A.n_rows} {A.n_cols} {A.nnz}"
+
+
+def _convert(lace, A, old, new, diag=None):
+    H = lace.HostCSR.from_csr(A)
+    H.m.storage_type = old
+    B = lace.DMatrix()
+    if diag is not None:
+        B.diagorder_type = diag
+    lace.check(lace.lib().data_zmconvert(H.m, C.byref(B), old, new), "zmconvert")
+    return lace.owned_to_numpy(B)
+
+
+@pytest.mark.parametrize("name", ["io_test.mtx", "Trefethen_20.mtx"])
+def test_conversions_match_oracle(lace, orc, name):
+    A = orc.read_mtx(os.path.join(GOLDEN, name))
+    L = _convert(lace, A, lace.Magma_CSR, lace.Magma_CSRL, lace.Magma_UNITY)
+    Lr = orc.csr_split(A, False, orc.DIAG_UNIT)
+    assert all(np.array_equal(a, b) for a, b in zip(L, (Lr.row, Lr.col, Lr.val)))
+    U = _convert(lace, A, lace.Magma_CSR, lace.Magma_CSRU, lace.Magma_VALUE)
+    Ur = orc.csr_split(A, True, orc.DIAG_VALUE)
+    assert all(np.array_equal(a, b) for a, b in zip(U, (Ur.row, Ur.col, Ur.val)))
+    Uc = _convert(lace, A, lace.Magma_CSR, lace.Magma_CSCU, lace.Magma_VALUE)
+    Ucr = orc.transpose(Ur)
+    assert all(np.array_equal(a, b) for a, b in zip(Uc, (Ucr.row, Ucr.col, Ucr.val)))
+    T = _convert(lace, A, lace.Magma_CSR, lace.Magma_CSC)
+    Tr = orc.transpose(A)
+    assert all(np.array_equal(a, b) for a, b in zip(T, (Tr.row, Tr.col, Tr.val)))
+    # CSC -> CSR undoes the transpose (testing/test_linearSolver.cpp:670)
+    back = _convert(lace, orc.CSR(A.n_rows, A.n_cols, *T), lace.Magma_CSC, lace.Magma_CSR)
+    assert all(np.array_equal(a, b) for a, b in zip(back, (A.row, A.col, A.val)))
+
+
+def test_lumerge_matches_split(lace, orc):
+    A = orc.convdiff3d(5, 4, 3)
+    Lr = orc.csr_split(A, False, orc.DIAG_UNIT)
+    Ur = orc.csr_split(A, True, orc.DIAG_VALUE)
+    Lh = lace.HostCSR.from_csr(Lr); Lh.m.storage_type = lace.Magma_CSR
+    Uh = lace.HostCSR.from_csr(Ur); Uh.m.storage_type = lace.Magma_CSR
+    M = lace.DMatrix()
+    lace.check(lace.lib().data_zmlumerge(Lh.m, Uh.m, C.byref(M)))
+    row, col, val = lace.owned_to_numpy(M)
+    # strict-L then U in A's own pattern and order
+    assert np.array_equal(row, A.row) and np.array_equal(col, A.col)
+    assert np.array_equal(val, A.val)
+
+
+def test_ilures_and_spgemm_match_oracle(lace, orc):
+    A = orc.convdiff3d(6, 5, 4)
+    st = orc.parilu(A, 2)
+    Lf, Uf = st.factors()
+    Lh = lace.HostCSR.from_csr(Lf); Lh.m.storage_type = lace.Magma_CSRL
+    Uh = lace.HostCSR.from_csr(Uf); Uh.m.storage_type = lace.Magma_CSRU
+    LU = lace.DMatrix()
+    r = C.c_double(); nl = C.c_double()
+    lace.check(lace.lib().data_zilures(lace.HostCSR.from_csr(A).m, Lh.m, Uh.m, C.byref(LU),
+                                       C.byref(r), C.byref(nl)))
+    lace.owned_to_numpy(LU)
+    rr, nlr = orc.ilures(A, Lf, Uf)
+    assert abs(r.value - rr) <= 1e-13 * rr and abs(nl.value - nlr) <= 1e-13 * nlr
+    Cm = lace.DMatrix()
+    lace.check(lace.lib().data_z_spmm(1.0, Lh.m, Uh.m, C.byref(Cm)))
+    cr, cc, cv = lace.owned_to_numpy(Cm)
+    ref = orc.spgemm(Lf, Uf)
+    assert np.array_equal(cr, ref.row) and np.array_equal(cc, ref.col)
+    assert np.allclose(cv, ref.val, rtol=1e-15, atol=1e-15)
+
+
+def test_exact_ilu0_matches_oracle(lace, orc):
+    A = orc.convdiff3d(6, 6, 5)
+    H = lace.HostCSR.from_csr(A)
+    lace.check(lace.lib().data_dcsrilu0_mkl(H.ref()))
+    ref, bad = orc.ilu0(A)
+    assert bad < 0
+    assert np.array_equal(H.val, ref.val)  # same IKJ operation order: bitwise
+
+
+def test_level_sets_match_oracle(lace, orc):
+    A = orc.laplace2d(11, 9)
+    for upper in (False, True):
+        T = orc.csr_split(A, upper, orc.DIAG_VALUE if upper else orc.DIAG_UNIT)
+        n = T.n_rows
+        perm = np.zeros(n, np.int32); lptr = np.zeros(n + 1, np.int32)
+        nl = lace.lib().lace_level_sets(n, T.row.ctypes.data_as(C.POINTER(C.c_int)),
+                                        T.col.ctypes.data_as(C.POINTER(C.c_int)), int(upper),
+                                        perm.ctypes.data_as(C.POINTER(C.c_int)),
+                                        lptr.ctypes.data_as(C.POINTER(C.c_int)))
+        lev, nlr = orc.levels(T, upper)
+        assert nl == nlr == 11 + 9 - 1
+        # rows sorted by level, ties by index
+        assert np.array_equal(perm, np.lexsort((np.arange(n), lev)))
+        assert np.array_equal(lptr[: nl + 1], np.concatenate([[0], np.cumsum(np.bincount(lev))]))
+
+
+def test_generators_match_oracle(lace, orc):
+    for dims in ((5, 4, 3), (16, 16, 16)):
+        r, c, v = lace.gen_convdiff3d(*dims)
+        ref = orc.convdiff3d(*dims)
+        assert np.array_equal(r, ref.row) and np.array_equal(c, ref.col)
+        assert np.array_equal(v, ref.val)
+    r, c, v = lace.gen_laplace2d(64, 64)
+    ref = orc.laplace2d(64, 64)
+    assert np.array_equal(r, ref.row) and np.array_equal(c, ref.col) and np.array_equal(v, ref.val)
+    assert r[-1] == 20224  # config 1 nnz
+
+
+def test_config2_sizes(lace):
+    r, c, v = lace.gen_convdiff3d(128, 128, 128)
+    assert len(r) - 1 == 2097152 and r[-1] == 14581760
+    d = v[np.arange(len(r) - 1)[:, None] == 0] if False else None  # noqa
+    # row sums: interior rows sum to 0 (h-scaled upwind stencil is conservative)
+    i = 64 + 128 * 64 + 16384 * 64
+    assert abs(v[r[i]:r[i + 1]].sum()) < 1e-14
+
+
+def test_gen_normal_philox_box_muller(lace, orc):
+    """Sample 2t / 2t+1 = Box-Muller pair from Philox4x32-10(counter t,
+    key = seed): two 53-bit uniforms in (0, 1]."""
+    seed = 12345
+    out = lace.gen_normal(seed, 11)
+    key = [seed & 0xFFFFFFFF, seed >> 32]
+    for t in range(6):
+        w = [int(x) for x in orc.philox4x32_10([t, 0, 0, 0], key)]
+        a = ((w[0] << 21) ^ (w[1] >> 11)) & ((1 << 53) - 1)
+        b = ((w[2] << 21) ^ (w[3] >> 11)) & ((1 << 53) - 1)
+        u1 = (a + 1) / 2.0 ** 53
+        u2 = (b + 1) / 2.0 ** 53
+        rad = np.sqrt(-2 * np.log(u1))
+        assert abs(out[2 * t] - rad * np.cos(2 * np.pi * u2)) <= 1e-14 * max(1, abs(out[2 * t]))
+        if 2 * t + 1 < 11:
+            assert abs(out[2 * t + 1] - rad * np.sin(2 * np.pi * u2)) <= 1e-14 * max(1, abs(out[2 * t + 1]))
+    big = lace.gen_normal(0, 200000)
+    assert abs(big.mean()) < 0.01 and abs(big.std() - 1) < 0.01
+
+
+def test_givens_dropin(lace, orc):
+    c = C.c_double(); s = C.c_double()
+    for a, b in [(3.0, 4.0), (-1.0, 2.5), (0.0, 1.0), (2.0, 0.0)]:
+        lace.lib().givens_rotation(a, b, 4.9e-324, C.byref(c), C.byref(s))
+        assert (c.value, s.value) == orc.givens(a, b, 4.9e-324)
+
+
+def test_dnrm2_and_frobenius(lace, orc):
+    x = np.random.default_rng(0).standard_normal(100)
+    assert abs(lace.lib().data_dnrm2(100, x.ctypes.data_as(C.POINTER(C.c_double)), 1)
+               - np.linalg.norm(x)) < 1e-13
+    A = orc.convdiff3d(4, 4, 4)
+    f = C.c_double()
+    lace.check(lace.lib().data_zfrobenius(lace.HostCSR.from_csr(A).m, C.byref(f)))
+    assert abs(f.value - np.linalg.norm(A.val)) < 1e-12
+
+
+def test_device_calls_fail_cleanly_without_gpu(lace):
+    if lace.gpu_available():
+        pytest.skip("a GPU is present")
+    p = C.c_void_p()
+    rc = lace.lib().lace_dmalloc(C.byref(p), 16)
+    assert rc != 0 and b"device" in lace.lib().lace_last_error()
