@@ -143,8 +143,8 @@ def test_ilures_and_spgemm_match_oracle(lace, orc):
     Uh = lace.HostCSR.from_csr(Uf); Uh.m.storage_type = lace.Magma_CSRU
     LU = lace.DMatrix()
     r = C.c_double(); nl = C.c_double()
-    lace.check(lace.lib().data_zilures(lace.HostCSR.from_csr(A).m, Lh.m, Uh.m, C.byref(LU),
-                                       C.byref(r), C.byref(nl)))
+    Ah = lace.HostCSR.from_csr(A)  # keep the arrays alive across the call
+    lace.check(lace.lib().data_zilures(Ah.m, Lh.m, Uh.m, C.byref(LU), C.byref(r), C.byref(nl)))
     lace.owned_to_numpy(LU)
     rr, nlr = orc.ilures(A, Lf, Uf)
     assert abs(r.value - rr) <= 1e-13 * rr and abs(nl.value - nlr) <= 1e-13 * nlr
@@ -236,7 +236,8 @@ def test_dnrm2_and_frobenius(lace, orc):
                - np.linalg.norm(x)) < 1e-13
     A = orc.convdiff3d(4, 4, 4)
     f = C.c_double()
-    lace.check(lace.lib().data_zfrobenius(lace.HostCSR.from_csr(A).m, C.byref(f)))
+    Ah = lace.HostCSR.from_csr(A)
+    lace.check(lace.lib().data_zfrobenius(Ah.m, C.byref(f)))
     assert abs(f.value - np.linalg.norm(A.val)) < 1e-12
 
 
