@@ -184,7 +184,8 @@ int data_zwrite_csr_mtx(data_d_matrix A, data_order_t MajorType,
 
 /* Norms and the ILU residual check.  sparse.h:410,462-468;
  * control/norms.cpp:69-92,301-405.  data_zilures evaluates ||A-LU||_F over
- * LU's pattern (res) and over A's pattern (nonlinres) on the GPU. */
+ * LU's pattern (res) and over A's pattern (nonlinres) on the host (a
+ * once-per-factorization report); lace_parilu_nonlinres is the device form. */
 dataType data_dnrm2(int n, dataType *x, int incx);
 int data_zfrobenius(data_d_matrix A, dataType *res);
 int data_zilures(data_d_matrix A, data_d_matrix L, data_d_matrix U,
