@@ -366,8 +366,18 @@ int lace_solver_setup(lace_solver *S);
  * on entry and the solution on exit. */
 int lace_solver_solve(lace_solver *S, const double *b, double *x,
                       lace_solver_stats *st);
+/* Same solve with A (trans = 0) or A^T (trans = 1).  The transposed solve
+ * reuses the ILU factors of A: M^{-T} = L^{-T} U^{-T} (U^T then L^T sweeps)
+ * inside the same FGMRES; A^T is assembled on the first transposed call. */
+int lace_solver_solve_trans(lace_solver *S, const double *b, double *x, int trans,
+                            lace_solver_stats *st);
+/* Replace the operator's values on A's pattern (host array of nnz values);
+ * the ILU factors stay those of the last lace_solver_setup. */
+int lace_solver_set_values(lace_solver *S, const double *vals);
 /* dL/dtheta on the pattern of A: dtheta[k] = -lambda[row(k)] * x[col(k)]
- * for CSR position k; device vectors. */
+ * for CSR position k; device vectors.  With A x = b and A^T lambda = dL/dx
+ * this is the gradient of L with respect to A's stored values (and
+ * dL/db = lambda). */
 int lace_solver_grad_values(lace_solver *S, const double *lambda,
                             const double *x, double *dtheta);
 

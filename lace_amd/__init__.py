@@ -155,6 +155,8 @@ SIGNATURES = {
     "lace_solver_setup": (_i, [_vp]),
     "lace_solver_solve": (_i, [_vp, _vp, _vp, _P(SolverStats)]),
     "lace_solver_grad_values": (_i, [_vp, _vp, _vp, _vp]),
+    "lace_solver_solve_trans": (_i, [_vp, _vp, _vp, _i, _P(SolverStats)]),
+    "lace_solver_set_values": (_i, [_vp, _P(_d)]),
     "lace_timing_enable": (_i, [_i]),
     "lace_timing_reset": (_i, []),
     "lace_timing_only": (_i, [C.c_char_p]),
@@ -383,6 +385,7 @@ class Solver:
         self.h = C.c_void_p()
         check(lib().lace_solver_create(A.ref(), C.byref(opts), C.byref(self.h)), "solver_create")
         self.n = A.m.num_rows
+        self.nnz = A.m.nnz
         self.opts = opts
 
     def setup(self):
@@ -393,11 +396,21 @@ class Solver:
         check(lib().lace_solver_solve(self.h, db.ptr, dx.ptr, C.byref(st)), "solver_solve")
         return st
 
-    def solve(self, b, x0=None):
+    def solve_dev_trans(self, db: DevVec, dx: DevVec, trans: int) -> SolverStats:
+        st = SolverStats()
+        check(lib().lace_solver_solve_trans(self.h, db.ptr, dx.ptr, trans, C.byref(st)),
+              "solver_solve_trans")
+        return st
+
+    def solve(self, b, x0=None, trans=0):
         db = DevVec(self.n, np.float64, b)
         dx = DevVec(self.n, np.float64, np.zeros(self.n) if x0 is None else x0)
-        st = self.solve_dev(db, dx)
+        st = self.solve_dev_trans(db, dx, trans)
         return dx.download(), st
+
+    def set_values(self, vals):
+        v = np.ascontiguousarray(vals, dtype=np.float64)
+        check(lib().lace_solver_set_values(self.h, _dptr(v)), "solver_set_values")
 
     def grad_values(self, lam: DevVec, x: DevVec, nnz: int) -> np.ndarray:
         g = DevVec(nnz, np.float64)

@@ -101,10 +101,15 @@ struct PrecInner : Precond {
 // ------------------------------------------------------------ solver
 struct Solver {
   lace_solver_opts o;
-  int n = 0;
-  DevCSR A64;          // operator (A, or A^T when transposed)
-  DevCSR A32;          // FP32 copy for the inner solver
-  DevCSR Apat;         // A itself (for the gradient pattern)
+  int n = 0, nnz = 0;
+  DevCSR A64;          // A in FP64 (Arnoldi / true-residual products)
+  DevCSR A32;          // FP32 copy of A for the inner solver
+  DevCSR AT64, AT32;   // A^T, built on the first transposed solve
+  bool have_T = false;
+  std::vector<int> h_row, h_col;  // pattern of A (for A^T)
+  std::vector<int> t_from;        // A^T position -> A position
+  std::vector<double> h_val;      // current operator values
+  const DevCSR* op64 = nullptr;   // operators of the current solve
   ParILU* P = nullptr;
   std::unique_ptr<PrecILU> ilu;
   std::unique_ptr<PrecIdentity> ident;
@@ -120,37 +125,72 @@ struct Solver {
   }
 };
 
-static int transpose_host(const data_d_matrix* A, data_d_matrix* T) {
-  memset(T, 0, sizeof(*T));
-  return data_zmtranspose(*A, T);
+// Host CSR view over a solver's pattern with the given values.
+static data_d_matrix csr_view(int n, std::vector<int>& row, std::vector<int>& col, double* val) {
+  data_d_matrix M;
+  memset(&M, 0, sizeof(M));
+  M.storage_type = Magma_CSR;
+  M.num_rows = M.num_cols = n;
+  M.nnz = row[n];
+  M.row = row.data();
+  M.col = col.data();
+  M.val = val;
+  return M;
+}
+
+// Build A^T (CSR) on the device; keeps the position map for value updates.
+static int ensure_transpose(Solver* S) {
+  if (S->have_T) return DEV_SUCCESS;
+  const int n = S->n, nnz = S->nnz;
+  std::vector<int> tr(n + 1, 0), tc(nnz);
+  S->t_from.assign(nnz, 0);
+  for (int k = 0; k < nnz; k++) tr[S->h_col[k] + 1]++;
+  for (int j = 0; j < n; j++) tr[j + 1] += tr[j];
+  std::vector<int> fill(tr.begin(), tr.end() - 1);
+  for (int i = 0; i < n; i++)
+    for (int k = S->h_row[i]; k < S->h_row[i + 1]; k++) {
+      const int q = fill[S->h_col[k]]++;
+      tc[q] = i;
+      S->t_from[q] = k;
+    }
+  std::vector<double> tv(nnz);
+  for (int q = 0; q < nnz; q++) tv[q] = S->h_val[S->t_from[q]];
+  data_d_matrix T = csr_view(n, tr, tc, tv.data());
+  LACE_TRY(S->AT64.create(&T, LACE_F64));
+  if (S->inner) LACE_TRY(S->AT32.create(&T, LACE_F32));
+  S->have_T = true;
+  return DEV_SUCCESS;
+}
+
+// Select the operator (A or A^T) and the matching preconditioner direction.
+static int select_direction(Solver* S, int trans) {
+  if (trans) LACE_TRY(ensure_transpose(S));
+  S->op64 = trans ? &S->AT64 : &S->A64;
+  if (S->ilu) S->ilu->trans = trans ? 1 : 0;
+  if (S->inner) S->inner->A32 = trans ? &S->AT32 : &S->A32;
+  return DEV_SUCCESS;
 }
 
 static int solver_create(const data_d_matrix* A, const lace_solver_opts* o, Solver** out) {
   LACE_TRY(ensure_init());
+  LACE_REQUIRE(A && A->row && A->col && A->val && A->num_rows == A->num_cols, DEV_ERR_ILLEGAL_VALUE,
+               "solver: A must be a square CSR matrix");
   auto S = std::make_unique<Solver>();
   S->o = *o;
   S->n = A->num_rows;
-  const bool tr = o->transpose != 0;
-  if (tr) {
-    data_d_matrix AT;
-    LACE_TRY(transpose_host(A, &AT));
-    AT.storage_type = Magma_CSR;
-    int rc = S->A64.create(&AT, LACE_F64);
-    if (rc == DEV_SUCCESS && o->inner_iters > 0) rc = S->A32.create(&AT, LACE_F32);
-    data_zmfree(&AT);
-    LACE_TRY(rc);
-  } else {
-    LACE_TRY(S->A64.create(A, LACE_F64));
-    if (o->inner_iters > 0) LACE_TRY(S->A32.create(A, LACE_F32));
-  }
-  LACE_TRY(S->Apat.create(A, LACE_F64));
+  S->nnz = A->row[S->n];
+  S->h_row.assign(A->row, A->row + S->n + 1);
+  S->h_col.assign(A->col, A->col + S->nnz);
+  S->h_val.assign(A->val, A->val + S->nnz);
+  LACE_TRY(S->A64.create(A, LACE_F64));
+  if (o->inner_iters > 0) LACE_TRY(S->A32.create(A, LACE_F32));
   if (o->trsv == LACE_TRSV_NONE && o->inner_iters <= 0) {
     S->ident = std::make_unique<PrecIdentity>(S->n);
     S->M = S->ident.get();
   } else {
     const lace_prec_t fp = o->inner_iters > 0 ? LACE_F32 : o->prec_precond;
     LACE_TRY(parilu_create(A, fp, &S->P));
-    S->ilu = std::make_unique<PrecILU>(S->P, o->trsv, o->jacobi_sweeps, tr ? 1 : 0);
+    S->ilu = std::make_unique<PrecILU>(S->P, o->trsv, o->jacobi_sweeps, 0);
     S->M = S->ilu.get();
     if (o->inner_iters > 0) {
       S->inner = std::make_unique<PrecInner>(&S->A32, S->ilu.get(), o->inner_iters, 0.0);
@@ -158,6 +198,7 @@ static int solver_create(const data_d_matrix* A, const lace_solver_opts* o, Solv
       S->M = S->inner.get();
     }
   }
+  LACE_TRY(select_direction(S.get(), o->transpose));
   LACE_TRY(S->E.init(S->n, o->search_max, S->M->z_is_float()));
   LACE_TRY(S->rbuf.alloc(sizeof(double) * S->n));
   S->nbn = ceil_div(S->n, S->rpb);
@@ -166,6 +207,29 @@ static int solver_create(const data_d_matrix* A, const lace_solver_opts* o, Solv
   LACE_HIP(hipHostMalloc((void**)&S->host_scal, sizeof(double) * 4, hipHostMallocDefault));
   LACE_HIP(hipStreamSynchronize(stream()));
   *out = S.release();
+  return DEV_SUCCESS;
+}
+
+// New values for the operator on A's pattern (the factors are kept until the
+// next setup): updates A, its FP32 copy and A^T.
+static int solver_set_values(Solver* S, const double* vals) {
+  S->h_val.assign(vals, vals + S->nnz);
+  LACE_HIP(hipMemcpyAsync(S->A64.val.p, vals, sizeof(double) * S->nnz, hipMemcpyHostToDevice, stream()));
+  if (S->inner) {
+    std::vector<float> f(vals, vals + S->nnz);
+    LACE_HIP(hipMemcpyAsync(S->A32.val.p, f.data(), sizeof(float) * S->nnz, hipMemcpyHostToDevice, stream()));
+    LACE_HIP(hipStreamSynchronize(stream()));
+  }
+  if (S->have_T) {
+    std::vector<double> tv(S->nnz);
+    for (int q = 0; q < S->nnz; q++) tv[q] = vals[S->t_from[q]];
+    LACE_HIP(hipMemcpyAsync(S->AT64.val.p, tv.data(), sizeof(double) * S->nnz, hipMemcpyHostToDevice, stream()));
+    if (S->inner) {
+      std::vector<float> f(tv.begin(), tv.end());
+      LACE_HIP(hipMemcpyAsync(S->AT32.val.p, f.data(), sizeof(float) * S->nnz, hipMemcpyHostToDevice, stream()));
+    }
+  }
+  LACE_HIP(hipStreamSynchronize(stream()));
   return DEV_SUCCESS;
 }
 
@@ -199,7 +263,7 @@ static int solver_solve(Solver* S, const double* b, double* x, lace_solver_stats
   bool converged = false;
   double rtol_abs = 0.0;
   for (int restart = 0; restart < S->o.restart_max && !converged; restart++) {
-    LACE_TRY((residual<double, double>(S->A64, x, b, E.r.as<double>())));
+    LACE_TRY((residual<double, double>(*S->op64, x, b, E.r.as<double>())));
     if (restart == 0) {
       if (tol_abs >= 0.0)
         LACE_TRY(set_misc_d(E.small.as<double>(), E.lay, MS_RTOL, tol_abs));
@@ -222,10 +286,10 @@ static int solver_solve(Solver* S, const double* b, double* x, lace_solver_stats
     for (int j = 0; j < m; j++) {
       LACE_TRY(S->M->apply_d(E.q.as<double>(), E.zcol(j)));
       if (E.z_float)
-        LACE_TRY((spmv<float, double>(S->A64, 1.0, static_cast<const float*>(E.zcol(j)), 0.0,
+        LACE_TRY((spmv<float, double>(*S->op64, 1.0, static_cast<const float*>(E.zcol(j)), 0.0,
                                       E.w.as<double>())));
       else
-        LACE_TRY((spmv<double, double>(S->A64, 1.0, static_cast<const double*>(E.zcol(j)), 0.0,
+        LACE_TRY((spmv<double, double>(*S->op64, 1.0, static_cast<const double*>(E.zcol(j)), 0.0,
                                        E.w.as<double>())));
       LACE_TRY(hh_step<double>(E, j, j < m - 1));
       LACE_TRY(read_misc<double>(E));
@@ -242,7 +306,7 @@ static int solver_solve(Solver* S, const double* b, double* x, lace_solver_stats
   }
   if (S->inner) local.inner_iterations = local.iterations * S->o.inner_iters;
   // true residual in FP64 (against the operator in use)
-  LACE_TRY((residual<double, double>(S->A64, x, b, S->rbuf.as<double>())));
+  LACE_TRY((residual<double, double>(*S->op64, x, b, S->rbuf.as<double>())));
   LACE_TRY(norm2(S, S->rbuf.as<double>(), &local.true_residual));
   double bn = 0.0;
   LACE_TRY(norm2(S, b, &bn));
@@ -443,7 +507,7 @@ static int solver_solve_mgs(Solver* S, const double* b, double* x, lace_solver_s
   }
   for (int restart = 0; restart < S->o.restart_max && !converged; restart++) {
     double* r = Vc(0);
-    LACE_TRY((residual<double, double>(S->A64, x, b, r)));
+    LACE_TRY((residual<double, double>(*S->op64, x, b, r)));
     LACE_TRY(nrm_into(r, sc + MG_RNORM));
     mgs_start_kernel<<<1, 64, 0, stream()>>>(sd, L, restart == 0 && tol_abs < 0.0, S->o.rtol);
     LACE_TRY(read_sc());
@@ -460,7 +524,7 @@ static int solver_solve_mgs(Solver* S, const double* b, double* x, lace_solver_s
     scale_into_kernel<<<nbv, 256, 0, stream()>>>(n, r, sc + MG_RNORM, sc + MG_RNORM, Vc(0));
     for (int j = 0; j < m; j++) {
       LACE_TRY(S->M->apply_d(Vc(j), Zc(j)));
-      LACE_TRY((spmv<double, double>(S->A64, 1.0, Zc(j), 0.0, u.as<double>())));
+      LACE_TRY((spmv<double, double>(*S->op64, 1.0, Zc(j), 0.0, u.as<double>())));
       LACE_TRY(nrm_into(u.as<double>(), sc + MG_NORMAV));
       const int passes = 1;
       for (int pass = 0; pass < passes + 1; pass++) {
@@ -498,7 +562,7 @@ static int solver_solve_mgs(Solver* S, const double* b, double* x, lace_solver_s
     if (!std::isfinite(hostsc[MG_RES])) break;
     converged = hostsc[MG_RES] < rtol_abs;
   }
-  LACE_TRY((residual<double, double>(S->A64, x, b, S->rbuf.as<double>())));
+  LACE_TRY((residual<double, double>(*S->op64, x, b, S->rbuf.as<double>())));
   LACE_TRY(norm2(S, S->rbuf.as<double>(), &local.true_residual));
   double bn = 0.0;
   LACE_TRY(norm2(S, b, &bn));
@@ -549,14 +613,31 @@ int lace_solver_setup(lace_solver* Sh) {
 
 int lace_solver_solve(lace_solver* Sh, const double* b, double* x, lace_solver_stats* st) {
   LACE_TRY(ensure_init());
-  return solver_solve(reinterpret_cast<Solver*>(Sh), b, x, st, -1.0);
+  auto* S = reinterpret_cast<Solver*>(Sh);
+  LACE_TRY(select_direction(S, S->o.transpose));
+  return solver_solve(S, b, x, st, -1.0);
+}
+
+int lace_solver_solve_trans(lace_solver* Sh, const double* b, double* x, int trans,
+                            lace_solver_stats* st) {
+  LACE_TRY(ensure_init());
+  auto* S = reinterpret_cast<Solver*>(Sh);
+  LACE_TRY(select_direction(S, trans));
+  const int rc = solver_solve(S, b, x, st, -1.0);
+  LACE_TRY(select_direction(S, S->o.transpose));
+  return rc;
+}
+
+int lace_solver_set_values(lace_solver* Sh, const double* vals) {
+  LACE_TRY(ensure_init());
+  return solver_set_values(reinterpret_cast<Solver*>(Sh), vals);
 }
 
 int lace_solver_grad_values(lace_solver* Sh, const double* lambda, const double* x, double* dtheta) {
   LACE_TRY(ensure_init());
   auto* S = reinterpret_cast<Solver*>(Sh);
   grad_gather_kernel<<<ceil_div(S->n, 256), 256, 0, stream()>>>(
-      S->n, S->Apat.row.as<int>(), S->Apat.col.as<int>(), lambda, x, dtheta);
+      S->n, S->A64.row.as<int>(), S->A64.col.as<int>(), lambda, x, dtheta);
   LACE_HIP(hipGetLastError());
   return DEV_SUCCESS;
 }
