@@ -218,6 +218,18 @@ void data_PariLU_v0_3_gpu_prescribedSweeps(data_d_matrix *A, data_d_matrix *L,
                                            data_d_matrix *U, dataType reduction,
                                            data_d_preconditioner_log *log);
 
+/* Block ParILU(0) on a BCSR matrix.  src/parilu_v0_3_bcsr.cpp:16-276, which
+ * defines the symbol data_PariLU_v0_3 a second time (it is in no makefile);
+ * the block form is exported under its own name.  Synchronous FP64 sweeps
+ * on the GPU until sum((new-old)^2) <= reduction (absolute, :61); at most
+ * 1000 sweeps (the reference has no cap), DEV_NOTCONVERGED past it.  L is
+ * returned as BCSRL (identity diagonal blocks last; tagged Magma_BCSR as the
+ * reference's converter does, magma_zmconverter.cpp:1251), U as BCSCU
+ * (block columns, diagonal block last, interiors not transposed).  Every
+ * diagonal block must be present. */
+int data_PariLU_v0_3_bcsr(data_d_matrix *A, data_d_matrix *L, data_d_matrix *U,
+                          dataType reduction, data_d_preconditioner_log *log);
+
 /* Triangular solves.  sparse.h:881-924; src/trisolve.cpp:20-119,256-351.
  * Iterated sweeps until sum((y_new-y_old)^2) <= tol, run synchronously on
  * the GPU (at most one sweep per level plus one confirming sweep). */
@@ -259,7 +271,7 @@ void givens_rotation(dataType a, dataType b, dataType tol, dataType *c,
  * lace_* device-resident API (no reference counterpart; see header note)
  * ====================================================================== */
 
-typedef enum { LACE_F64 = 0, LACE_F32 = 1, LACE_BF16 = 2 } lace_prec_t;
+typedef enum { LACE_F64 = 0, LACE_F32 = 1, LACE_BF16 = 2, LACE_F16 = 3 } lace_prec_t;
 
 typedef enum {
   LACE_TRSV_LEVEL = 0,   /* exact, level-scheduled */
@@ -270,6 +282,8 @@ typedef enum {
 typedef struct lace_csr lace_csr;         /* device CSR matrix */
 typedef struct lace_parilu lace_parilu;   /* device ParILU factors */
 typedef struct lace_solver lace_solver;   /* device FGMRES solver */
+typedef struct lace_bcsr lace_bcsr;       /* device BCSR matrix */
+typedef struct lace_bilu lace_bilu;       /* device block ParILU factors */
 
 /* Runtime.  lace_init selects the device; lace_last_error returns the text
  * of the most recent failure on this thread. */
@@ -328,6 +342,44 @@ int lace_parilu_set_variant(lace_parilu *P, int variant);
  * poll budget); the result of that solve is then invalid. */
 int lace_syncfree_error(lace_parilu *P, int *err);
 
+/* Device BCSR matrix (A->storage_type Magma_BCSR, num_rows = block rows,
+ * blocks row-major, blocksize 1..8); values converted to prec (F64, F32). */
+int lace_bcsr_create(const data_d_matrix *A, lace_prec_t prec, lace_bcsr **out);
+int lace_bcsr_destroy(lace_bcsr *A);
+/* y = A x on device vectors of A's precision (length num_rows*blocksize). */
+int lace_bcsr_spmv(const lace_bcsr *A, const void *x, void *y);
+/* Algorithmic HBM bytes of one lace_bcsr_spmv. */
+double lace_bcsr_spmv_bytes(const lace_bcsr *A);
+
+/* Block ParILU(0) on device (src/parilu_v0_3_bcsr.cpp:118-250 in
+ * synchronous form).  prec is the factor storage: F64 (FP64 iteration), F32,
+ * BF16 or F16 (FP32 iteration; 16-bit storage keeps a power-of-two scale per
+ * scalar row taken from A's row maxima).  Every diagonal block must be
+ * present; block columns sorted within each block row. */
+int lace_bilu_create(const data_d_matrix *A, lace_prec_t prec, lace_bilu **out);
+int lace_bilu_destroy(lace_bilu *B);
+/* L0 = strict-lower blocks of A with identity diagonal blocks, U0 = upper
+ * blocks of A. */
+int lace_bilu_reset(lace_bilu *B);
+/* nsweeps synchronous sweeps; step (optional, synchronizes) receives the
+ * unscaled sum of squared changes of the last sweep. */
+int lace_bilu_sweeps(lace_bilu *B, int nsweeps, double *step);
+/* Host copies: L as BCSRL, U as BCSCU (FP64 values). */
+int lace_bilu_get(lace_bilu *B, data_d_matrix *L, data_d_matrix *U);
+double lace_bilu_sweep_bytes(const lace_bilu *B);
+/* z = U^{-1} L^{-1} r with exact level-scheduled block solves or k Jacobi
+ * sweeps; r, z device vectors in the iteration precision (double for F64
+ * storage, float otherwise). */
+int lace_bilu_apply(lace_bilu *B, lace_trsv_kind_t kind, int jacobi_sweeps,
+                    const void *r, void *z);
+int lace_bilu_levels(const lace_bilu *B, int *nlev_L, int *nlev_U);
+/* Sweep kernel: 0 (default) pairs L_IJ with U_JI in one lane group when A's
+ * block pattern is structurally symmetric, else one group per block of A;
+ * 1 always takes the per-block kernel.  Both give identical factors. */
+int lace_bilu_set_variant(lace_bilu *B, int variant);
+/* 1 if A's block pattern is structurally symmetric. */
+int lace_bilu_is_symmetric(const lace_bilu *B);
+
 /* Solver configuration. */
 typedef struct lace_solver_opts {
   int    krylov;        /* 0 = Householder FGMRES, 1 = MGS FGMRES */
@@ -357,6 +409,9 @@ typedef struct lace_solver_stats {
 } lace_solver_stats;
 
 void lace_solver_default_opts(lace_solver_opts *o);
+/* A is CSR (scalar ParILU factors) or BCSR (Magma_BCSR: block ParILU
+ * factors, block SpMV; prec_precond may then also be BF16 or F16).
+ * Transposed solves, value updates and value gradients take CSR only. */
 int lace_solver_create(const data_d_matrix *A, const lace_solver_opts *o,
                        lace_solver **out);
 int lace_solver_destroy(lace_solver *S);
@@ -399,6 +454,13 @@ int lace_level_sets(int n, const int *row, const int *col, int upper,
 int lace_gen_laplace2d(int nx, int ny, data_d_matrix *A);
 int lace_gen_convdiff3d(int nx, int ny, int nz, double pe_x, double pe_y,
                         double pe_z, data_d_matrix *A);
+/* BCSR with bs x bs blocks on the 7-point block pattern of an nx*ny*nz grid
+ * (x fastest, Dirichlet: blocks outside the box dropped, block columns
+ * ascending).  Entries are sigma * N(0,1) from lace_gen_normal(seed) in
+ * storage order (block by block, row-major inside), and shift is added to
+ * the diagonal entries of the diagonal blocks. */
+int lace_gen_bcsr7(int nx, int ny, int nz, int bs, uint64_t seed, double sigma,
+                   double shift, data_d_matrix *A);
 int lace_gen_normal(uint64_t seed, int n, double *out);
 uint32_t *lace_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2],
                              uint32_t out[4]);

@@ -23,7 +23,8 @@ MagmaUpper, MagmaLower = 121, 122
 Magma_CSR, Magma_DENSE, Magma_CSC = 611, 614, 616
 Magma_CSRL, Magma_CSRU, Magma_CSCU, Magma_CSCL = 627, 628, 635, 634
 Magma_UNITY, Magma_VALUE, Magma_NODIAG = 593, 594, 596
-LACE_F64, LACE_F32, LACE_BF16 = 0, 1, 2
+Magma_BCSR, Magma_BCSRL, Magma_BCSRU, Magma_BCSCU = 615, 641, 642, 646
+LACE_F64, LACE_F32, LACE_BF16, LACE_F16 = 0, 1, 2, 3
 TRSV_LEVEL, TRSV_JACOBI, TRSV_NONE = 0, 1, 2
 DEV_SUCCESS = 0
 
@@ -117,6 +118,7 @@ SIGNATURES = {
     "data_PariLU_v0_3_prescribedSweeps": (None, [_DM, _DM, _DM, _d, _P(PrecondLog)]),
     "data_PariLU_v0_3_gpu": (None, [_DM, _DM, _DM, _d, _P(PrecondLog), _i]),
     "data_PariLU_v0_3_gpu_prescribedSweeps": (None, [_DM, _DM, _DM, _d, _P(PrecondLog)]),
+    "data_PariLU_v0_3_bcsr": (_i, [_DM, _DM, _DM, _d, _P(PrecondLog)]),
     "data_forward_solve": (_i, [_DM, _DM, _DM, _d, _P(_i)]),
     "data_backward_solve": (_i, [_DM, _DM, _DM, _d, _P(_i)]),
     "data_parcsrtrsv": (_i, [_i, _i, _i, _i, _P(_d), _P(_i), _P(_i), _P(_d), _P(_d), _d, _P(_i)]),
@@ -149,6 +151,20 @@ SIGNATURES = {
     "lace_parilu_levels": (_i, [_vp, _P(_i), _P(_i)]),
     "lace_parilu_set_variant": (_i, [_vp, _i]),
     "lace_syncfree_error": (_i, [_vp, _P(_i)]),
+    "lace_bcsr_create": (_i, [_DM, _i, _P(_vp)]),
+    "lace_bcsr_destroy": (_i, [_vp]),
+    "lace_bcsr_spmv": (_i, [_vp, _vp, _vp]),
+    "lace_bcsr_spmv_bytes": (_d, [_vp]),
+    "lace_bilu_create": (_i, [_DM, _i, _P(_vp)]),
+    "lace_bilu_destroy": (_i, [_vp]),
+    "lace_bilu_reset": (_i, [_vp]),
+    "lace_bilu_sweeps": (_i, [_vp, _i, _P(_d)]),
+    "lace_bilu_get": (_i, [_vp, _DM, _DM]),
+    "lace_bilu_sweep_bytes": (_d, [_vp]),
+    "lace_bilu_apply": (_i, [_vp, _i, _i, _vp, _vp]),
+    "lace_bilu_levels": (_i, [_vp, _P(_i), _P(_i)]),
+    "lace_bilu_set_variant": (_i, [_vp, _i]),
+    "lace_bilu_is_symmetric": (_i, [_vp]),
     "lace_solver_default_opts": (None, [_P(SolverOpts)]),
     "lace_solver_create": (_i, [_DM, _P(SolverOpts), _P(_vp)]),
     "lace_solver_destroy": (_i, [_vp]),
@@ -164,6 +180,7 @@ SIGNATURES = {
     "lace_level_sets": (_i, [_i, _P(_i), _P(_i), _i, _P(_i), _P(_i)]),
     "lace_gen_laplace2d": (_i, [_i, _i, _DM]),
     "lace_gen_convdiff3d": (_i, [_i, _i, _i, _d, _d, _d, _DM]),
+    "lace_gen_bcsr7": (_i, [_i, _i, _i, _i, C.c_uint64, _d, _d, _DM]),
     "lace_gen_normal": (_i, [C.c_uint64, _i, _P(_d)]),
     "lace_philox4x32_10": (_P(C.c_uint32), [_P(C.c_uint32), _P(C.c_uint32), _P(C.c_uint32)]),
 }
@@ -241,6 +258,53 @@ class HostCSR:
         if self.m.rowidx:
             C.CDLL(None).free(C.cast(self.m.rowidx, C.c_void_p))
             self.m.rowidx = C.POINTER(C.c_int)()
+
+
+class HostBCSR:
+    """BCSR data_d_matrix over numpy arrays owned by this object: num_rows
+    block rows, blocks of bs x bs stored row-major (val has shape
+    (numblocks, bs, bs) or is flat)."""
+
+    def __init__(self, row, col, val, bs: int):
+        self.row = np.array(row, dtype=np.int32, copy=True, order="C")
+        self.col = np.array(col, dtype=np.int32, copy=True, order="C")
+        self.val = np.array(val, dtype=np.float64, copy=True, order="C").reshape(-1)
+        nbr = len(self.row) - 1
+        nb = int(self.row[-1])
+        assert self.val.size == nb * bs * bs
+        m = DMatrix()
+        m.storage_type = Magma_BCSR
+        m.sym = 581
+        m.diagorder_type = Magma_VALUE
+        m.fill_mode = 123
+        m.num_rows = m.num_cols = nbr
+        m.blocksize = bs
+        m.ldblock = bs * bs
+        m.numblocks = nb
+        m.nnz = m.true_nnz = nb * bs * bs
+        m.major = MagmaRowMajor
+        m.row = _iptr(self.row)
+        m.col = _iptr(self.col)
+        m.val = _dptr(self.val)
+        self.m = m
+        self.bs = bs
+        self.n = nbr * bs
+
+    def ref(self):
+        return C.byref(self.m)
+
+
+def owned_bcsr_to_numpy(m: DMatrix):
+    """(row, col, val[numblocks, bs, bs]) of a library-allocated block
+    matrix (BCSR / BCSRL / BCSCU), freed afterwards."""
+    n, bs = m.num_rows, m.blocksize
+    nb = m.row[n]
+    row = np.ctypeslib.as_array(m.row, shape=(n + 1,)).copy()
+    col = np.ctypeslib.as_array(m.col, shape=(max(nb, 1),))[:nb].copy()
+    val = np.ctypeslib.as_array(m.val, shape=(max(nb, 1) * bs * bs,))[:nb * bs * bs].copy()
+    meta = dict(storage=m.storage_type, bs=bs, numblocks=m.numblocks, major=m.major)
+    lib().data_zmfree(C.byref(m))
+    return row, col, val.reshape(nb, bs, bs), meta
 
 
 class HostVec:
@@ -374,6 +438,85 @@ class ParILU:
             pass
 
 
+class DevBCSR:
+    """Device BCSR matrix (lace_bcsr)."""
+
+    def __init__(self, A: HostBCSR, prec=LACE_F64):
+        self.h = C.c_void_p()
+        check(lib().lace_bcsr_create(A.ref(), prec, C.byref(self.h)), "bcsr_create")
+        self.n = A.n
+        self.prec = prec
+
+    def spmv(self, x: np.ndarray) -> np.ndarray:
+        dt = np.float64 if self.prec == LACE_F64 else np.float32
+        dx = DevVec(self.n, dt, x)
+        dy = DevVec(self.n, dt)
+        check(lib().lace_bcsr_spmv(self.h, dx.ptr, dy.ptr), "bcsr_spmv")
+        return dy.download()
+
+    def bytes(self):
+        return lib().lace_bcsr_spmv_bytes(self.h)
+
+    def __del__(self):
+        try:
+            if self.h:
+                lib().lace_bcsr_destroy(self.h)
+        except Exception:
+            pass
+
+
+class BILU:
+    """Device block ParILU(0) (lace_bilu)."""
+
+    def __init__(self, A: HostBCSR, prec=LACE_F32):
+        self.h = C.c_void_p()
+        check(lib().lace_bilu_create(A.ref(), prec, C.byref(self.h)), "bilu_create")
+        self.prec = prec
+        self.n = A.n
+
+    def reset(self):
+        check(lib().lace_bilu_reset(self.h), "bilu_reset")
+
+    def sweeps(self, k: int, want_step=False):
+        st = C.c_double()
+        check(lib().lace_bilu_sweeps(self.h, k, C.byref(st) if want_step else None), "bilu_sweeps")
+        return st.value if want_step else None
+
+    def factors(self):
+        """(L as BCSRL, U as BCSCU), each (row, col, val[nb, bs, bs], meta)."""
+        L = DMatrix(); U = DMatrix()
+        check(lib().lace_bilu_get(self.h, C.byref(L), C.byref(U)), "bilu_get")
+        return owned_bcsr_to_numpy(L), owned_bcsr_to_numpy(U)
+
+    def sweep_bytes(self):
+        return lib().lace_bilu_sweep_bytes(self.h)
+
+    def set_variant(self, v: int):
+        check(lib().lace_bilu_set_variant(self.h, v), "bilu_set_variant")
+
+    def symmetric(self) -> bool:
+        return bool(lib().lace_bilu_is_symmetric(self.h))
+
+    def levels(self):
+        a = C.c_int(); b = C.c_int()
+        check(lib().lace_bilu_levels(self.h, C.byref(a), C.byref(b)), "bilu_levels")
+        return a.value, b.value
+
+    def apply(self, r: np.ndarray, kind=TRSV_LEVEL, sweeps=5):
+        dt = np.float64 if self.prec == LACE_F64 else np.float32
+        dr = DevVec(self.n, dt, r)
+        dz = DevVec(self.n, dt)
+        check(lib().lace_bilu_apply(self.h, kind, sweeps, dr.ptr, dz.ptr), "bilu_apply")
+        return dz.download()
+
+    def __del__(self):
+        try:
+            if self.h:
+                lib().lace_bilu_destroy(self.h)
+        except Exception:
+            pass
+
+
 def default_opts() -> SolverOpts:
     o = SolverOpts()
     lib().lace_solver_default_opts(C.byref(o))
@@ -381,10 +524,12 @@ def default_opts() -> SolverOpts:
 
 
 class Solver:
-    def __init__(self, A: HostCSR, opts: SolverOpts):
+    """FGMRES solver handle over a HostCSR or a HostBCSR operator."""
+
+    def __init__(self, A, opts: SolverOpts):
         self.h = C.c_void_p()
         check(lib().lace_solver_create(A.ref(), C.byref(opts), C.byref(self.h)), "solver_create")
-        self.n = A.m.num_rows
+        self.n = A.n if isinstance(A, HostBCSR) else A.m.num_rows
         self.nnz = A.m.nnz
         self.opts = opts
 
@@ -436,6 +581,14 @@ def gen_laplace2d(nx, ny):
     m = DMatrix()
     check(lib().lace_gen_laplace2d(nx, ny, C.byref(m)), "gen")
     return owned_to_numpy(m)
+
+
+def gen_bcsr7(nx, ny, nz, bs=5, seed=1, sigma=0.1, shift=5.0):
+    """(row, col, val[nb, bs, bs]) of the 7-point block stencil matrix."""
+    m = DMatrix()
+    check(lib().lace_gen_bcsr7(nx, ny, nz, bs, seed, sigma, shift, C.byref(m)), "gen_bcsr7")
+    row, col, val, _ = owned_bcsr_to_numpy(m)
+    return row, col, val
 
 
 def gen_normal(seed: int, n: int) -> np.ndarray:

@@ -1,0 +1,1230 @@
+// bcsr.hip -- block-sparse (BCSR) SpMV, block ParILU(0) sweeps and block
+// triangular solves on the GPU.
+//
+// Reference: src/parilu_v0_3_bcsr.cpp:16-276 (block ParILU: S = A_IJ minus
+// the merged products L_IK U_KJ, the last product added back, L_IJ = S
+// inv(U_JJ), U_IJ = S; absolute stop tolerance), src/inverse.cpp:17 (block
+// inverse), control/magma_zmconverter.cpp:1248-1445 (BCSRL with identity
+// diagonal blocks, BCSCU = block transpose of BCSRU with block interiors
+// kept as they are).
+//
+// Thread mapping: a group of BS consecutive lanes owns one block row of a
+// block (lane r of the group computes row r), 64/BS groups per wave, groups
+// never straddle a wave.  Full blocks a lane needs (U_KJ, U_JJ) are read by
+// every lane of the group from the same addresses, which the texture path
+// merges.  Arithmetic follows the CPU restatement operation by operation
+// (products accumulated from zero in ascending index order, no fused
+// multiply-add), so FP64 and FP32 factors are bit-identical to it.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <type_traits>
+#include <vector>
+
+#include "bcsr.hpp"
+#include "common.hpp"
+#include "device.hpp"
+
+#pragma clang fp contract(off)
+
+namespace lace {
+
+constexpr int kBBS = 256;  // threads per block for every kernel here
+
+template <typename TS>
+constexpr bool is16() {
+  return sizeof(TS) == 2;
+}
+
+// bfloat16 from float, round to nearest even (NaN kept quiet).
+__device__ __forceinline__ uint16_t f2bf(float v) {
+  uint32_t u = __float_as_uint(v);
+  if ((u & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((u >> 16) | 0x40u);
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+
+// Storage load/store: 16-bit entries carry the scale of their scalar row.
+template <typename TC, typename TS>
+__device__ __forceinline__ TC ldv(const TS* p, size_t i, float s) {
+  if constexpr (std::is_same<TS, uint16_t>::value) {
+    return __uint_as_float((uint32_t)p[i] << 16) * s;
+  } else if constexpr (std::is_same<TS, _Float16>::value) {
+    return (float)p[i] * s;
+  } else {
+    (void)s;
+    return (TC)p[i];
+  }
+}
+template <typename TC, typename TS>
+__device__ __forceinline__ TS quant(TC v, float sinv) {
+  if constexpr (std::is_same<TS, uint16_t>::value) {
+    return f2bf(v * sinv);
+  } else if constexpr (std::is_same<TS, _Float16>::value) {
+    return (_Float16)(v * sinv);
+  } else {
+    (void)sinv;
+    return (TS)v;
+  }
+}
+template <typename TC, typename TS>
+__device__ __forceinline__ TC dequant(TS q, float s) {
+  if constexpr (std::is_same<TS, uint16_t>::value) {
+    return __uint_as_float((uint32_t)q << 16) * s;
+  } else if constexpr (std::is_same<TS, _Float16>::value) {
+    return (float)q * s;
+  } else {
+    (void)s;
+    return (TC)q;
+  }
+}
+
+// Gauss-Jordan inverse without pivoting, in the fixed order of the CPU
+// restatement: pivot p scales its row, then every other row eliminates
+// column p.
+template <int BS, typename T>
+__device__ __forceinline__ void binv(const T (&M)[BS * BS], T (&inv)[BS * BS]) {
+  T a[BS * BS];
+#pragma unroll
+  for (int e = 0; e < BS * BS; e++) {
+    a[e] = M[e];
+    inv[e] = T(0);
+  }
+#pragma unroll
+  for (int r = 0; r < BS; r++) inv[r * BS + r] = T(1);
+#pragma unroll
+  for (int p = 0; p < BS; p++) {
+    const T iv = T(1) / a[p * BS + p];
+#pragma unroll
+    for (int c = 0; c < BS; c++) {
+      a[p * BS + c] = a[p * BS + c] * iv;
+      inv[p * BS + c] = inv[p * BS + c] * iv;
+    }
+#pragma unroll
+    for (int r = 0; r < BS; r++) {
+      if (r == p) continue;
+      const T f = a[r * BS + p];
+#pragma unroll
+      for (int c = 0; c < BS; c++) {
+        a[r * BS + c] = a[r * BS + c] - f * a[p * BS + c];
+        inv[r * BS + c] = inv[r * BS + c] - f * inv[p * BS + c];
+      }
+    }
+  }
+}
+
+// Full block q (row scales rs[kb*BS + m] for 16-bit storage).
+template <int BS, typename TC, typename TS>
+__device__ __forceinline__ void load_block(const TS* __restrict__ V, int q, const float* __restrict__ rs,
+                                           int kb, TC (&out)[BS * BS]) {
+#pragma unroll
+  for (int m = 0; m < BS; m++) {
+    const float s = is16<TS>() ? rs[kb * BS + m] : 1.0f;
+#pragma unroll
+    for (int c = 0; c < BS; c++) out[m * BS + c] = ldv<TC>(V, (size_t)q * BS * BS + m * BS + c, s);
+  }
+}
+
+template <int BS, typename TC, typename TS>
+__device__ __forceinline__ void load_row(const TS* __restrict__ V, int q, int r, float s, TC (&out)[BS]) {
+#pragma unroll
+  for (int c = 0; c < BS; c++) out[c] = ldv<TC>(V, (size_t)q * BS * BS + r * BS + c, s);
+}
+
+// Lane-group coordinates: group index (over the grid) and row in the block.
+template <int BS>
+struct Group {
+  int g, r;
+  bool valid;
+  __device__ __forceinline__ Group(int count) {
+    constexpr int PPW = 64 / BS;
+    const int blk = xcd_block(blockIdx.x, gridDim.x);
+    const int lane = threadIdx.x & 63;
+    const int slot = lane / BS;
+    r = lane - slot * BS;
+    g = (blk * (kBBS / 64) + (int)(threadIdx.x >> 6)) * PPW + slot;
+    valid = slot < PPW && g < count;
+  }
+};
+
+template <int BS>
+inline int group_blocks(long count) {
+  const long per = (long)(kBBS / 64) * (64 / BS);
+  return (int)std::max(1L, (count + per - 1) / per);
+}
+
+// ---------------------------------------------------------------- SpMV
+// MODE 0: y = A x; MODE 1: y = b - A x.  Lane (I, r) sums block row I's
+// row r over blocks in order, entries in column order.
+template <typename TV, typename TX, typename TY, int BS, int MODE>
+__global__ __launch_bounds__(kBBS) void bcsr_spmv_kernel(int nbr, const int* __restrict__ row,
+                                                         const int* __restrict__ col,
+                                                         const TV* __restrict__ val,
+                                                         const TX* __restrict__ x, TY* __restrict__ y,
+                                                         const TY* __restrict__ b) {
+  using ACC = typename std::conditional<sizeof(TV) == 8 || sizeof(TY) == 8, double, float>::type;
+  const Group<BS> G(nbr);
+  if (!G.valid) return;
+  const int I = G.g, r = G.r;
+  const int k0 = row[I], k1 = row[I + 1];
+  ACC acc = ACC(0);
+  for (int k = k0; k < k1; k++) {
+    const int K = col[k];
+    const TV* v = val + (size_t)k * BS * BS + r * BS;
+#pragma unroll
+    for (int c = 0; c < BS; c++) acc = acc + (ACC)v[c] * (ACC)x[(size_t)K * BS + c];
+  }
+  const size_t i = (size_t)I * BS + r;
+  if (MODE == 1) y[i] = (TY)((ACC)b[i] - acc);
+  else y[i] = (TY)acc;
+}
+
+template <typename TV, typename TX, typename TY, int MODE>
+static int launch_bspmv(const DevBCSR& A, const TX* x, TY* y, const TY* b) {
+  const int* row = A.row.as<int>();
+  const int* col = A.col.as<int>();
+  const TV* val = A.val.as<TV>();
+#define LACE_BSPMV(BSV)                                                                    \
+  case BSV:                                                                                \
+    bcsr_spmv_kernel<TV, TX, TY, BSV, MODE>                                                \
+        <<<group_blocks<BSV>(A.nbr), kBBS, 0, stream()>>>(A.nbr, row, col, val, x, y, b); \
+    break;
+  switch (A.bs) {
+    LACE_BSPMV(1) LACE_BSPMV(2) LACE_BSPMV(3) LACE_BSPMV(4)
+    LACE_BSPMV(5) LACE_BSPMV(6) LACE_BSPMV(7) LACE_BSPMV(8)
+    default: LACE_REQUIRE(false, DEV_ERR_NOT_SUPPORTED, "BCSR block size %d", A.bs);
+  }
+#undef LACE_BSPMV
+  LACE_HIP(hipGetLastError());
+  return DEV_SUCCESS;
+}
+
+int DevBCSR::create(const data_d_matrix* A, lace_prec_t p) {
+  LACE_TRY(ensure_init());
+  LACE_REQUIRE(A && A->row && A->col && A->val, DEV_ERR_INVALID_PTR, "DevBCSR: null arrays");
+  LACE_REQUIRE(A->storage_type == Magma_BCSR, DEV_ERR_NOT_SUPPORTED, "DevBCSR: storage %d",
+               (int)A->storage_type);
+  LACE_REQUIRE(A->blocksize >= 1 && A->blocksize <= kMaxBlock, DEV_ERR_NOT_SUPPORTED,
+               "DevBCSR: block size %d (1..%d)", A->blocksize, kMaxBlock);
+  LACE_REQUIRE(p == LACE_F64 || p == LACE_F32, DEV_ERR_NOT_SUPPORTED, "DevBCSR: precision %d", (int)p);
+  nbr = A->num_rows;
+  bs = A->blocksize;
+  nblocks = A->row[nbr];
+  prec = p;
+  const size_t lb = (size_t)bs * bs, nv = (size_t)std::max(nblocks, 1) * lb;
+  LACE_TRY(upload(row, A->row, sizeof(int) * (nbr + 1)));
+  LACE_TRY(upload(col, A->col, sizeof(int) * std::max(nblocks, 1)));
+  if (p == LACE_F64) {
+    LACE_TRY(upload(val, A->val, sizeof(double) * nv));
+  } else {
+    std::vector<float> f(nv, 0.0f);
+    for (size_t e = 0; e < (size_t)nblocks * lb; e++) f[e] = (float)A->val[e];
+    LACE_TRY(upload(val, f.data(), sizeof(float) * nv));
+  }
+  return DEV_SUCCESS;
+}
+
+double DevBCSR::bytes(int sx, int sy, bool residual) const {
+  const double sv = prec == LACE_F64 ? 8.0 : 4.0;
+  const double n = (double)nbr * bs;
+  return (double)nblocks * bs * bs * sv + 4.0 * nblocks + 4.0 * (nbr + 1) + (double)sx * n +
+         (double)sy * n + (residual ? (double)sy * n : 0.0);
+}
+
+template <typename TX, typename TY>
+int bcsr_spmv(const DevBCSR& A, const TX* x, TY* y) {
+  TimedRegion tr("bcsr_spmv");
+  if (A.prec == LACE_F64) return launch_bspmv<double, TX, TY, 0>(A, x, y, nullptr);
+  return launch_bspmv<float, TX, TY, 0>(A, x, y, nullptr);
+}
+
+template <typename TX, typename TY>
+int bcsr_residual(const DevBCSR& A, const TX* x, const TY* b, TY* r) {
+  TimedRegion tr("bcsr_spmv_residual");
+  if (A.prec == LACE_F64) return launch_bspmv<double, TX, TY, 1>(A, x, r, b);
+  return launch_bspmv<float, TX, TY, 1>(A, x, r, b);
+}
+
+template int bcsr_spmv<double, double>(const DevBCSR&, const double*, double*);
+template int bcsr_spmv<float, double>(const DevBCSR&, const float*, double*);
+template int bcsr_spmv<float, float>(const DevBCSR&, const float*, float*);
+template int bcsr_residual<double, double>(const DevBCSR&, const double*, const double*, double*);
+template int bcsr_residual<float, float>(const DevBCSR&, const float*, const float*, float*);
+
+// ---------------------------------------------------------------- ParILU
+// Factors from A: L = strict-lower blocks of A (the identity diagonal
+// blocks are implicit and stored as zeros), U = upper blocks of A.  One
+// lane per stored entry; lfrom/ufrom name the source block in src_l/src_u.
+template <typename TC, typename TS>
+__global__ __launch_bounds__(kBBS) void bilu_reset_kernel(long countL, long countU, int bs,
+                                                          const int* __restrict__ lrowv,
+                                                          const int* __restrict__ uri,
+                                                          const int* __restrict__ lfrom,
+                                                          const int* __restrict__ ufrom,
+                                                          const TC* __restrict__ src_l,
+                                                          const TC* __restrict__ src_u,
+                                                          const float* __restrict__ rs,
+                                                          TS* __restrict__ L0, TS* __restrict__ L1,
+                                                          TS* __restrict__ U0, TS* __restrict__ U1) {
+  const long e = (long)blockIdx.x * kBBS + threadIdx.x;
+  const int lb = bs * bs;
+  if (e < countL) {
+    const int q = (int)(e / lb), w = (int)(e - (long)q * lb), r = w / bs;
+    const int f = lfrom[q];
+    const float s = is16<TS>() ? rs[lrowv[q] * bs + r] : 1.0f;
+    const TS v = f >= 0 ? quant<TC, TS>(src_l[(size_t)f * lb + w], 1.0f / s) : TS(0);
+    L0[e] = v;
+    L1[e] = v;
+  }
+  if (e < countU) {
+    const int q = (int)(e / lb), w = (int)(e - (long)q * lb), r = w / bs;
+    const float s = is16<TS>() ? rs[uri[q] * bs + r] : 1.0f;
+    const TS v = quant<TC, TS>(src_u[(size_t)ufrom[q] * lb + w], 1.0f / s);
+    U0[e] = v;
+    U1[e] = v;
+  }
+}
+
+// Symmetric-pattern sweep: group (q, r) updates row r of L_IJ and of U_JI
+// at position q = (I, J), reading (Lin, Uin) and writing (Lout, Uout).
+template <int BS, typename TC, typename TS, bool STEP>
+__global__ __launch_bounds__(kBBS) void bilu_sym_sweep_kernel(
+    int np, const int* __restrict__ prow, const int* __restrict__ pcol, const int* __restrict__ lptr,
+    const TC* __restrict__ AL, const TC* __restrict__ AU, const TS* __restrict__ Lin,
+    const TS* __restrict__ Uin, TS* __restrict__ Lout, TS* __restrict__ Uout,
+    const float* __restrict__ rs, double* __restrict__ partial) {
+  constexpr int LB = BS * BS;
+  const Group<BS> G(np);
+  double d2 = 0.0;
+  if (G.valid) {
+    const int p = G.g, r = G.r;
+    const int I = prow[p], J = pcol[p];
+    const bool strict = I != J;
+    const float sI = is16<TS>() ? rs[I * BS + r] : 1.0f;
+    const float sJ = is16<TS>() ? rs[J * BS + r] : 1.0f;
+    TC sL[BS], sU[BS];
+#pragma unroll
+    for (int c = 0; c < BS; c++) {
+      sU[c] = AU[(size_t)p * LB + r * BS + c];
+      sL[c] = strict ? AL[(size_t)p * LB + r * BS + c] : TC(0);
+    }
+    // products over K < J: row I positions before p, row J positions
+    // before its diagonal, merged by block column
+    int qi = lptr[I], qj = lptr[J];
+    const int qj_end = lptr[J + 1] - 1;
+    while (qi < p && qj < qj_end) {
+      const int ki = pcol[qi], kj = pcol[qj];
+      if (ki == kj) {
+        TC ub[LB], lr[BS];
+        if (strict) {
+          // L_IJ row r -= L_IK(r,:) U_KJ;  U_KJ sits at position (J, K)
+          load_block<BS, TC, TS>(Uin, qj, rs, ki, ub);
+          load_row<BS, TC, TS>(Lin, qi, r, sI, lr);
+#pragma unroll
+          for (int c = 0; c < BS; c++) {
+            TC acc = TC(0);
+#pragma unroll
+            for (int m = 0; m < BS; m++) acc = acc + lr[m] * ub[m * BS + c];
+            sL[c] = sL[c] - acc;
+          }
+        }
+        // U_JI row r -= L_JK(r,:) U_KI;  U_KI sits at position (I, K)
+        load_block<BS, TC, TS>(Uin, qi, rs, ki, ub);
+        load_row<BS, TC, TS>(Lin, qj, r, sJ, lr);
+#pragma unroll
+        for (int c = 0; c < BS; c++) {
+          TC acc = TC(0);
+#pragma unroll
+          for (int m = 0; m < BS; m++) acc = acc + lr[m] * ub[m * BS + c];
+          sU[c] = sU[c] - acc;
+        }
+      }
+      qi += ki <= kj;
+      qj += ki >= kj;
+    }
+    // the closing product K = J, subtracted and added back
+    if (strict) {
+      TC ujj[LB], lold[BS], inv[LB];
+      load_block<BS, TC, TS>(Uin, qj_end, rs, J, ujj);
+      load_row<BS, TC, TS>(Lin, p, r, sI, lold);
+#pragma unroll
+      for (int c = 0; c < BS; c++) {
+        TC acc = TC(0);
+#pragma unroll
+        for (int m = 0; m < BS; m++) acc = acc + lold[m] * ujj[m * BS + c];
+        sL[c] = sL[c] - acc;
+        sL[c] = sL[c] + acc;
+      }
+      binv<BS, TC>(ujj, inv);
+      const float sinv = 1.0f / sI;
+#pragma unroll
+      for (int c = 0; c < BS; c++) {
+        TC acc = TC(0);
+#pragma unroll
+        for (int m = 0; m < BS; m++) acc = acc + sL[m] * inv[m * BS + c];
+        const TS q = quant<TC, TS>(acc, sinv);
+        Lout[(size_t)p * LB + r * BS + c] = q;
+        if (STEP) {
+          const double d = (double)lold[c] - (double)dequant<TC, TS>(q, sI);
+          d2 += d * d;
+        }
+      }
+    }
+    {
+      TC uold[BS];
+      load_row<BS, TC, TS>(Uin, p, r, sJ, uold);
+      const float sinv = 1.0f / sJ;
+#pragma unroll
+      for (int c = 0; c < BS; c++) {
+        // L_JJ is the identity: the closing product is U_JI's own row
+        TC v = sU[c] - uold[c];
+        v = v + uold[c];
+        const TS q = quant<TC, TS>(v, sinv);
+        Uout[(size_t)p * LB + r * BS + c] = q;
+        if (STEP) {
+          const double d = (double)uold[c] - (double)dequant<TC, TS>(q, sJ);
+          d2 += d * d;
+        }
+      }
+    }
+  }
+  if (STEP) {
+    __shared__ double sm[kBBS / 64];
+    const double tot = block_sum<kBBS>(d2, sm);
+    if (threadIdx.x == 0) partial[xcd_block(blockIdx.x, gridDim.x)] = tot;
+  }
+}
+
+// General sweep: group (k, r) takes block k = (I, J) of A in BCSR order,
+// merges L row I (BCSRL) with U column J (BCSCU) and writes row r of L_IJ
+// (I > J) or U_IJ (I <= J), as src/parilu_v0_3_bcsr.cpp:130-247 does.  The
+// merge always ends on the match K = min(I, J); that product is
+// subtracted and added back.
+template <int BS, typename TC, typename TS, bool STEP>
+__global__ __launch_bounds__(kBBS) void bilu_gen_sweep_kernel(
+    int nblk, const int* __restrict__ arowidx, const int* __restrict__ acol,
+    const TC* __restrict__ AV, const int* __restrict__ lptr, const int* __restrict__ lcol,
+    const int* __restrict__ ucp, const int* __restrict__ uri, const TS* __restrict__ Lin,
+    const TS* __restrict__ Uin, TS* __restrict__ Lout, TS* __restrict__ Uout,
+    const float* __restrict__ rs, double* __restrict__ partial) {
+  constexpr int LB = BS * BS;
+  const Group<BS> G(nblk);
+  double d2 = 0.0;
+  if (G.valid) {
+    const int k = G.g, r = G.r;
+    const int I = arowidx[k], J = acol[k];
+    const float sI = is16<TS>() ? rs[I * BS + r] : 1.0f;
+    TC s[BS], last[BS];
+#pragma unroll
+    for (int c = 0; c < BS; c++) {
+      s[c] = AV[(size_t)k * LB + r * BS + c];
+      last[c] = TC(0);
+    }
+    int il = lptr[I], iu = ucp[J];
+    const int il_end = lptr[I + 1], iu_end = ucp[J + 1];
+    while (il < il_end && iu < iu_end) {
+      const int jl = lcol[il], ju = uri[iu];
+      if (jl == ju) {
+        if (jl == I) {
+          // L_II is the identity: the product is U_IJ's row r
+          load_row<BS, TC, TS>(Uin, iu, r, sI, last);
+        } else {
+          TC ub[LB], lr[BS];
+          load_block<BS, TC, TS>(Uin, iu, rs, ju, ub);
+          load_row<BS, TC, TS>(Lin, il, r, sI, lr);
+#pragma unroll
+          for (int c = 0; c < BS; c++) {
+            TC acc = TC(0);
+#pragma unroll
+            for (int m = 0; m < BS; m++) acc = acc + lr[m] * ub[m * BS + c];
+            last[c] = acc;
+          }
+        }
+#pragma unroll
+        for (int c = 0; c < BS; c++) s[c] = s[c] - last[c];
+      }
+      il += jl <= ju;
+      iu += jl >= ju;
+    }
+#pragma unroll
+    for (int c = 0; c < BS; c++) s[c] = s[c] + last[c];
+    const float sinv = 1.0f / sI;
+    if (I > J) {
+      TC ujj[LB], inv[LB], lold[BS];
+      load_block<BS, TC, TS>(Uin, iu_end - 1, rs, J, ujj);
+      load_row<BS, TC, TS>(Lin, il - 1, r, sI, lold);
+      binv<BS, TC>(ujj, inv);
+#pragma unroll
+      for (int c = 0; c < BS; c++) {
+        TC acc = TC(0);
+#pragma unroll
+        for (int m = 0; m < BS; m++) acc = acc + s[m] * inv[m * BS + c];
+        const TS q = quant<TC, TS>(acc, sinv);
+        Lout[(size_t)(il - 1) * LB + r * BS + c] = q;
+        if (STEP) {
+          const double d = (double)lold[c] - (double)dequant<TC, TS>(q, sI);
+          d2 += d * d;
+        }
+      }
+    } else {
+      TC uold[BS];
+      load_row<BS, TC, TS>(Uin, iu - 1, r, sI, uold);
+#pragma unroll
+      for (int c = 0; c < BS; c++) {
+        const TS q = quant<TC, TS>(s[c], sinv);
+        Uout[(size_t)(iu - 1) * LB + r * BS + c] = q;
+        if (STEP) {
+          const double d = (double)uold[c] - (double)dequant<TC, TS>(q, sI);
+          d2 += d * d;
+        }
+      }
+    }
+  }
+  if (STEP) {
+    __shared__ double sm[kBBS / 64];
+    const double tot = block_sum<kBBS>(d2, sm);
+    if (threadIdx.x == 0) partial[xcd_block(blockIdx.x, gridDim.x)] = tot;
+  }
+}
+
+__global__ __launch_bounds__(256) void bilu_sum_kernel(int nb, const double* __restrict__ partial,
+                                                       double* __restrict__ out) {
+  __shared__ double sm[4];
+  double v = 0.0;
+  for (int t = threadIdx.x; t < nb; t += 256) v += partial[t];
+  const double tot = block_sum<256>(v, sm);
+  if (threadIdx.x == 0) *out = tot;
+}
+
+// Inverses of U's diagonal blocks (last block of each U column), one lane
+// per block row.
+template <int BS, typename TC, typename TS>
+__global__ __launch_bounds__(kBBS) void bilu_dinv_kernel(int nbr, const int* __restrict__ ucp,
+                                                         const TS* __restrict__ Uv,
+                                                         const float* __restrict__ rs,
+                                                         TC* __restrict__ Dinv) {
+  const int I = blockIdx.x * kBBS + threadIdx.x;
+  if (I >= nbr) return;
+  TC u[BS * BS], inv[BS * BS];
+  load_block<BS, TC, TS>(Uv, ucp[I + 1] - 1, rs, I, u);
+  binv<BS, TC>(u, inv);
+#pragma unroll
+  for (int e = 0; e < BS * BS; e++) Dinv[(size_t)I * BS * BS + e] = inv[e];
+}
+
+// ---------------------------------------------------------------- solves
+// Lower (identity diagonal blocks): y_I = r_I - sum_K L_IK yin_K over the
+// strict entries of L row I.  LEVEL: the group list is rows[0..count).
+template <int BS, typename TC, typename TS, bool LEVEL>
+__global__ __launch_bounds__(kBBS) void btri_lower_kernel(int count, const int* __restrict__ rows,
+                                                          const int* __restrict__ lptr,
+                                                          const int* __restrict__ lcol,
+                                                          const TS* __restrict__ Lv,
+                                                          const float* __restrict__ rs,
+                                                          const TC* __restrict__ rhs,
+                                                          const TC* yin, TC* yout) {
+  const Group<BS> G(count);
+  if (!G.valid) return;
+  const int I = LEVEL ? rows[G.g] : G.g, r = G.r;
+  const float s = is16<TS>() ? rs[I * BS + r] : 1.0f;
+  TC acc = TC(0);
+  const int q1 = lptr[I + 1] - 1;
+  for (int q = lptr[I]; q < q1; q++) {
+    const int K = lcol[q];
+#pragma unroll
+    for (int c = 0; c < BS; c++)
+      acc = acc + ldv<TC>(Lv, (size_t)q * BS * BS + r * BS + c, s) * yin[(size_t)K * BS + c];
+  }
+  yout[(size_t)I * BS + r] = rhs[(size_t)I * BS + r] - acc;
+}
+
+// Upper: y_I = inv(U_II) (r_I - sum_{K>I} U_IK yin_K); U row I lists
+// (index into U's storage, block column K).  FIRST skips the sum (yin = 0).
+template <int BS, typename TC, typename TS, bool LEVEL, bool FIRST>
+__global__ __launch_bounds__(kBBS) void btri_upper_kernel(int count, const int* __restrict__ rows,
+                                                          const int* __restrict__ urp,
+                                                          const int* __restrict__ uidx,
+                                                          const int* __restrict__ ucolk,
+                                                          const TS* __restrict__ Uv,
+                                                          const float* __restrict__ rs,
+                                                          const TC* __restrict__ Dinv,
+                                                          const TC* __restrict__ rhs, const TC* yin,
+                                                          TC* yout) {
+  const Group<BS> G(count);
+  // every lane of a group takes part in the exchange below
+  const bool valid = G.valid;
+  const int I = valid ? (LEVEL ? rows[G.g] : G.g) : 0, r = G.r;
+  TC t = TC(0);
+  if (valid) {
+    TC acc = TC(0);
+    if (!FIRST) {
+      const float s = is16<TS>() ? rs[I * BS + r] : 1.0f;
+      for (int k = urp[I]; k < urp[I + 1]; k++) {
+        const int q = uidx[k], K = ucolk[k];
+#pragma unroll
+        for (int c = 0; c < BS; c++)
+          acc = acc + ldv<TC>(Uv, (size_t)q * BS * BS + r * BS + c, s) * yin[(size_t)K * BS + c];
+      }
+    }
+    t = rhs[(size_t)I * BS + r] - acc;
+  }
+  const int base = (threadIdx.x & 63) - r;
+  TC y = TC(0);
+#pragma unroll
+  for (int c = 0; c < BS; c++) {
+    const TC tc = __shfl(t, base + c, 64);
+    if (valid) y = y + Dinv[(size_t)I * BS * BS + r * BS + c] * tc;
+  }
+  if (valid) yout[(size_t)I * BS + r] = y;
+}
+
+// ---------------------------------------------------------------- host
+static void host_scale_rows(const data_d_matrix* A, std::vector<float>& rs) {
+  const int bs = A->blocksize, nbr = A->num_rows, lb = bs * bs;
+  rs.assign((size_t)nbr * bs, 1.0f);
+  for (int I = 0; I < nbr; I++)
+    for (int a = 0; a < bs; a++) {
+      double m = 0.0;
+      for (int k = A->row[I]; k < A->row[I + 1]; k++)
+        for (int c = 0; c < bs; c++) m = std::max(m, std::fabs(A->val[(size_t)k * lb + a * bs + c]));
+      if (m > 0.0 && std::isfinite(m)) {
+        int e = 0;
+        (void)std::frexp(m, &e);  // m = f 2^e, f in [0.5, 1)
+        e = std::max(-126, std::min(127, e - 1));
+        rs[(size_t)I * bs + a] = std::ldexp(1.0f, e);
+      }
+    }
+}
+
+template <typename TC>
+static int upload_as(DevBuf& dst, const std::vector<double>& v) {
+  std::vector<TC> t(v.begin(), v.end());
+  if (t.empty()) t.push_back(TC(0));
+  return upload(dst, t.data(), sizeof(TC) * t.size());
+}
+
+static int upload_ints(DevBuf& dst, const std::vector<int>& v) {
+  if (v.empty()) {
+    const int z = 0;
+    return upload(dst, &z, sizeof(int));
+  }
+  return upload(dst, v.data(), sizeof(int) * v.size());
+}
+
+int BILU::create(const data_d_matrix* A, lace_prec_t storage) {
+  LACE_TRY(ensure_init());
+  LACE_REQUIRE(A && A->row && A->col && A->val, DEV_ERR_INVALID_PTR, "block ParILU: null arrays");
+  LACE_REQUIRE(A->storage_type == Magma_BCSR, DEV_ERR_NOT_SUPPORTED,
+               "block ParILU: A must be BCSR (storage %d)", (int)A->storage_type);
+  LACE_REQUIRE(A->blocksize >= 1 && A->blocksize <= kMaxBlock, DEV_ERR_NOT_SUPPORTED,
+               "block ParILU: block size %d (1..%d)", A->blocksize, kMaxBlock);
+  LACE_REQUIRE(storage == LACE_F64 || storage == LACE_F32 || storage == LACE_BF16 || storage == LACE_F16,
+               DEV_ERR_NOT_SUPPORTED, "block ParILU: storage precision %d", (int)storage);
+  LACE_REQUIRE(A->num_rows == A->num_cols, DEV_ERR_ILLEGAL_VALUE, "block ParILU: A not square");
+  nbr = A->num_rows;
+  bs = A->blocksize;
+  store = storage;
+  nblocksA = A->row[nbr];
+  const int lb = bs * bs;
+  // pattern checks: sorted block columns, diagonal block present
+  std::vector<int> dpos(nbr, -1);
+  for (int I = 0; I < nbr; I++)
+    for (int k = A->row[I]; k < A->row[I + 1]; k++) {
+      const int J = A->col[k];
+      LACE_REQUIRE(J >= 0 && J < nbr, DEV_ERR_ILLEGAL_VALUE, "block ParILU: column %d out of range", J);
+      LACE_REQUIRE(k == A->row[I] || A->col[k - 1] < J, DEV_ERR_ILLEGAL_VALUE,
+                   "block ParILU: block row %d not sorted", I);
+      if (J == I) dpos[I] = k;
+    }
+  for (int I = 0; I < nbr; I++)
+    LACE_REQUIRE(dpos[I] >= 0, DEV_ERR_ILLEGAL_VALUE, "block ParILU: diagonal block %d missing", I);
+  // L (BCSRL): strict-lower blocks, then the diagonal
+  std::vector<int> lf;
+  h_lptr.assign(nbr + 1, 0);
+  h_lcol.clear();
+  h_lrow.clear();
+  for (int I = 0; I < nbr; I++) {
+    h_lptr[I] = (int)h_lcol.size();
+    for (int k = A->row[I]; k < A->row[I + 1] && A->col[k] <= I; k++) {
+      h_lcol.push_back(A->col[k]);
+      h_lrow.push_back(I);
+      lf.push_back(A->col[k] == I ? -1 : k);
+    }
+  }
+  h_lptr[nbr] = (int)h_lcol.size();
+  nnzL = h_lptr[nbr];
+  nstrict = nnzL - nbr;
+  // U (BCSCU): column J holds rows K <= J ascending, diagonal last
+  std::vector<int> uf;
+  h_ucp.assign(nbr + 1, 0);
+  for (int I = 0; I < nbr; I++)
+    for (int k = A->row[I]; k < A->row[I + 1]; k++)
+      if (A->col[k] >= I) h_ucp[A->col[k] + 1]++;
+  for (int J = 0; J < nbr; J++) h_ucp[J + 1] += h_ucp[J];
+  nnzU = h_ucp[nbr];
+  h_uri.assign(nnzU, 0);
+  uf.assign(nnzU, 0);
+  {
+    std::vector<int> fill(h_ucp.begin(), h_ucp.end() - 1);
+    for (int I = 0; I < nbr; I++)
+      for (int k = A->row[I]; k < A->row[I + 1]; k++)
+        if (A->col[k] >= I) {
+          const int q = fill[A->col[k]]++;
+          h_uri[q] = I;
+          uf[q] = k;
+        }
+  }
+  sym = (h_ucp == h_lptr) && (h_uri == h_lcol);
+  // U by rows for the solves: (U index, column K > I), columns ascending
+  h_urp.assign(nbr + 1, 0);
+  for (int J = 0; J < nbr; J++)
+    for (int q = h_ucp[J]; q < h_ucp[J + 1]; q++)
+      if (h_uri[q] != J) h_urp[h_uri[q] + 1]++;
+  for (int I = 0; I < nbr; I++) h_urp[I + 1] += h_urp[I];
+  h_uidx.assign(h_urp[nbr], 0);
+  h_ucolk.assign(h_urp[nbr], 0);
+  {
+    std::vector<int> fill(h_urp.begin(), h_urp.end() - 1);
+    for (int J = 0; J < nbr; J++)
+      for (int q = h_ucp[J]; q < h_ucp[J + 1]; q++)
+        if (h_uri[q] != J) {
+          const int t = fill[h_uri[q]]++;
+          h_uidx[t] = q;
+          h_ucolk[t] = J;
+        }
+  }
+  double fro = 0.0;
+  for (size_t e = 0; e < (size_t)nblocksA * lb; e++) fro += A->val[e] * A->val[e];
+  anorm = std::sqrt(fro);
+  // values in the iteration precision: A at L positions for the symmetric
+  // kernel (then L / U initial values come from AL / AU), A itself for the
+  // general kernel
+  std::vector<int> lfrom_h(nnzL), ufrom_h(nnzU);
+  if (sym) {
+    std::vector<double> al((size_t)nnzL * lb, 0.0), au((size_t)nnzL * lb, 0.0);
+    for (int q = 0; q < nnzL; q++) {
+      if (lf[q] >= 0)
+        memcpy(&al[(size_t)q * lb], A->val + (size_t)lf[q] * lb, sizeof(double) * lb);
+      memcpy(&au[(size_t)q * lb], A->val + (size_t)uf[q] * lb, sizeof(double) * lb);
+      lfrom_h[q] = lf[q] >= 0 ? q : -1;
+      ufrom_h[q] = q;
+    }
+    if (c64()) {
+      LACE_TRY(upload_as<double>(AL, al));
+      LACE_TRY(upload_as<double>(AU, au));
+    } else {
+      LACE_TRY(upload_as<float>(AL, al));
+      LACE_TRY(upload_as<float>(AU, au));
+    }
+  }
+  // host copy of A for the general kernel (now, or after a variant switch)
+  h_arow.assign(A->row, A->row + nbr + 1);
+  h_acol.assign(A->col, A->col + nblocksA);
+  h_aval.assign(A->val, A->val + (size_t)nblocksA * lb);
+  if (!sym) {
+    LACE_TRY(upload_general());
+    lfrom_h = lf;
+    ufrom_h = uf;
+  }
+  LACE_TRY(upload_ints(lfrom, lfrom_h));
+  LACE_TRY(upload_ints(ufrom, ufrom_h));
+  std::vector<float> h_rs;
+  host_scale_rows(A, h_rs);
+  LACE_TRY(upload(rs, h_rs.data(), sizeof(float) * h_rs.size()));
+  LACE_TRY(upload_ints(lptr, h_lptr));
+  LACE_TRY(upload_ints(lcol, h_lcol));
+  LACE_TRY(upload_ints(lrow, h_lrow));
+  LACE_TRY(upload_ints(ucp, h_ucp));
+  LACE_TRY(upload_ints(uri, h_uri));
+  LACE_TRY(upload_ints(urp, h_urp));
+  LACE_TRY(upload_ints(uidx, h_uidx));
+  LACE_TRY(upload_ints(ucolk, h_ucolk));
+  for (int s = 0; s < 2; s++) {
+    LACE_TRY(Lv[s].alloc((size_t)nnzL * lb * selem()));
+    LACE_TRY(Uv[s].alloc((size_t)nnzU * lb * selem()));
+  }
+  LACE_TRY(Dinv.alloc((size_t)nbr * lb * celem()));
+  LACE_TRY(work.alloc((size_t)2 * n() * celem()));
+  const int nbmax = group_blocks<kMaxBlock>(std::max(nnzL, nblocksA));
+  LACE_TRY(partial.alloc(sizeof(double) * nbmax));
+  LACE_TRY(scalar.alloc(sizeof(double) * 2));
+  // level schedules of the block triangles
+  {
+    std::vector<int> lrs(nbr + 1, 0), lcs;
+    for (int I = 0; I < nbr; I++) {
+      lrs[I] = (int)lcs.size();
+      for (int q = h_lptr[I]; q < h_lptr[I + 1] - 1; q++) lcs.push_back(h_lcol[q]);
+    }
+    lrs[nbr] = (int)lcs.size();
+    if (lcs.empty()) lcs.push_back(0);
+    h_lperm.assign(nbr, 0);
+    h_llev.assign(nbr + 1, 0);
+    const int nl = lace_level_sets(nbr, lrs.data(), lcs.data(), 0, h_lperm.data(), h_llev.data());
+    LACE_REQUIRE(nl >= 0, DEV_ERR_ILLEGAL_VALUE, "block ParILU: level sets of L failed");
+    h_llev.resize(nl + 1);
+    std::vector<int> uc(h_ucolk);
+    if (uc.empty()) uc.push_back(0);
+    h_uperm.assign(nbr, 0);
+    h_ulev.assign(nbr + 1, 0);
+    const int nu = lace_level_sets(nbr, h_urp.data(), uc.data(), 1, h_uperm.data(), h_ulev.data());
+    LACE_REQUIRE(nu >= 0, DEV_ERR_ILLEGAL_VALUE, "block ParILU: level sets of U failed");
+    h_ulev.resize(nu + 1);
+    LACE_TRY(upload_ints(lperm, h_lperm));
+    LACE_TRY(upload_ints(uperm, h_uperm));
+  }
+  return reset();
+}
+
+// A's coordinates and values for the general kernel (uploaded on demand).
+int BILU::upload_general() {
+  if (have_general) return DEV_SUCCESS;
+  const int lb = bs * bs;
+  std::vector<int> ri(nblocksA);
+  for (int I = 0; I < nbr; I++)
+    for (int k = h_arow[I]; k < h_arow[I + 1]; k++) ri[k] = I;
+  LACE_TRY(upload_ints(arowidx, ri));
+  LACE_TRY(upload_ints(acol, h_acol));
+  std::vector<double> v(h_aval.begin(), h_aval.begin() + (size_t)nblocksA * lb);
+  if (c64()) LACE_TRY(upload_as<double>(AV, v));
+  else LACE_TRY(upload_as<float>(AV, v));
+  have_general = true;
+  return DEV_SUCCESS;
+}
+
+int BILU::set_variant(int v) {
+  LACE_REQUIRE(v == 0 || v == 1, DEV_ERR_ILLEGAL_VALUE, "block ParILU: variant %d", v);
+  variant = v;
+  if (v == 1) LACE_TRY(upload_general());
+  return DEV_SUCCESS;
+}
+
+template <typename TC, typename TS>
+static int reset_t(BILU* B) {
+  const int lb = B->bs * B->bs;
+  const long cl = (long)B->nnzL * lb, cu = (long)B->nnzU * lb;
+  const long count = std::max(cl, cu);
+  const TC* src_l = B->sym ? B->AL.as<TC>() : B->AV.as<TC>();
+  const TC* src_u = B->sym ? B->AU.as<TC>() : B->AV.as<TC>();
+  bilu_reset_kernel<TC, TS><<<(int)((count + kBBS - 1) / kBBS), kBBS, 0, stream()>>>(
+      cl, cu, B->bs, B->lrow.as<int>(), B->uri.as<int>(), B->lfrom.as<int>(), B->ufrom.as<int>(),
+      src_l, src_u, B->rs.as<float>(), B->Lv[0].as<TS>(), B->Lv[1].as<TS>(), B->Uv[0].as<TS>(),
+      B->Uv[1].as<TS>());
+  LACE_HIP(hipGetLastError());
+  return DEV_SUCCESS;
+}
+
+int BILU::reset() {
+  cur = 0;
+  dirty = true;
+  switch (store) {
+    case LACE_F64: LACE_TRY((reset_t<double, double>(this))); break;
+    case LACE_F32: LACE_TRY((reset_t<float, float>(this))); break;
+    case LACE_BF16: LACE_TRY((reset_t<float, uint16_t>(this))); break;
+    default: LACE_TRY((reset_t<float, _Float16>(this))); break;
+  }
+  return DEV_SUCCESS;
+}
+
+template <int BS, typename TC, typename TS>
+static int sweep_bs(BILU* B, bool with_step) {
+  const int c = B->cur, o = 1 - B->cur;
+  int nb = 0;
+  if (B->use_sym()) {
+    nb = group_blocks<BS>(B->nnzL);
+    auto go = [&](auto kern) {
+      kern<<<nb, kBBS, 0, stream()>>>(B->nnzL, B->lrow.as<int>(), B->lcol.as<int>(), B->lptr.as<int>(),
+                                       B->AL.as<TC>(), B->AU.as<TC>(), B->Lv[c].as<TS>(),
+                                       B->Uv[c].as<TS>(), B->Lv[o].as<TS>(), B->Uv[o].as<TS>(),
+                                       B->rs.as<float>(), B->partial.as<double>());
+    };
+    if (with_step) go(bilu_sym_sweep_kernel<BS, TC, TS, true>);
+    else go(bilu_sym_sweep_kernel<BS, TC, TS, false>);
+  } else {
+    nb = group_blocks<BS>(B->nblocksA);
+    auto go = [&](auto kern) {
+      kern<<<nb, kBBS, 0, stream()>>>(B->nblocksA, B->arowidx.as<int>(), B->acol.as<int>(),
+                                       B->AV.as<TC>(), B->lptr.as<int>(), B->lcol.as<int>(),
+                                       B->ucp.as<int>(), B->uri.as<int>(), B->Lv[c].as<TS>(),
+                                       B->Uv[c].as<TS>(), B->Lv[o].as<TS>(), B->Uv[o].as<TS>(),
+                                       B->rs.as<float>(), B->partial.as<double>());
+    };
+    if (with_step) go(bilu_gen_sweep_kernel<BS, TC, TS, true>);
+    else go(bilu_gen_sweep_kernel<BS, TC, TS, false>);
+  }
+  LACE_HIP(hipGetLastError());
+  B->last_nb = nb;
+  B->cur = o;
+  B->dirty = true;
+  return DEV_SUCCESS;
+}
+
+template <typename TC, typename TS>
+static int sweep_t(BILU* B, bool with_step) {
+  switch (B->bs) {
+    case 1: return sweep_bs<1, TC, TS>(B, with_step);
+    case 2: return sweep_bs<2, TC, TS>(B, with_step);
+    case 3: return sweep_bs<3, TC, TS>(B, with_step);
+    case 4: return sweep_bs<4, TC, TS>(B, with_step);
+    case 5: return sweep_bs<5, TC, TS>(B, with_step);
+    case 6: return sweep_bs<6, TC, TS>(B, with_step);
+    case 7: return sweep_bs<7, TC, TS>(B, with_step);
+    default: return sweep_bs<8, TC, TS>(B, with_step);
+  }
+}
+
+int BILU::sweep(bool with_step) {
+  TimedRegion tr("bilu_sweep");
+  switch (store) {
+    case LACE_F64: return sweep_t<double, double>(this, with_step);
+    case LACE_F32: return sweep_t<float, float>(this, with_step);
+    case LACE_BF16: return sweep_t<float, uint16_t>(this, with_step);
+    default: return sweep_t<float, _Float16>(this, with_step);
+  }
+}
+
+int BILU::read_step(double* step) {
+  bilu_sum_kernel<<<1, 256, 0, stream()>>>(last_nb, partial.as<double>(), scalar.as<double>());
+  LACE_HIP(hipGetLastError());
+  LACE_HIP(hipMemcpyAsync(step, scalar.p, sizeof(double), hipMemcpyDeviceToHost, stream()));
+  LACE_HIP(hipStreamSynchronize(stream()));
+  return DEV_SUCCESS;
+}
+
+double BILU::sweep_bytes() const {
+  // A's blocks once, old L (strict) and U read once, new ones written once,
+  // the pattern (two ints per entry updated), and row scales for 16-bit
+  // storage.
+  const double lb = (double)bs * bs;
+  const double a = (double)nblocksA * lb * celem();
+  const double f = 2.0 * ((double)nstrict + nnzU) * lb * selem();
+  return a + f + 8.0 * nblocksA + 4.0 * (nbr + 1) + (is16s() ? 4.0 * n() : 0.0);
+}
+
+template <int BS, typename TC, typename TS>
+static int dinv_bs(BILU* B) {
+  bilu_dinv_kernel<BS, TC, TS><<<ceil_div(B->nbr, kBBS), kBBS, 0, stream()>>>(
+      B->nbr, B->ucp.as<int>(), B->Uv[B->cur].as<TS>(), B->rs.as<float>(), B->Dinv.as<TC>());
+  LACE_HIP(hipGetLastError());
+  return DEV_SUCCESS;
+}
+
+template <typename TC, typename TS>
+static int dinv_t(BILU* B) {
+  switch (B->bs) {
+    case 1: return dinv_bs<1, TC, TS>(B);
+    case 2: return dinv_bs<2, TC, TS>(B);
+    case 3: return dinv_bs<3, TC, TS>(B);
+    case 4: return dinv_bs<4, TC, TS>(B);
+    case 5: return dinv_bs<5, TC, TS>(B);
+    case 6: return dinv_bs<6, TC, TS>(B);
+    case 7: return dinv_bs<7, TC, TS>(B);
+    default: return dinv_bs<8, TC, TS>(B);
+  }
+}
+
+int BILU::refresh() {
+  if (!dirty) return DEV_SUCCESS;
+  switch (store) {
+    case LACE_F64: LACE_TRY((dinv_t<double, double>(this))); break;
+    case LACE_F32: LACE_TRY((dinv_t<float, float>(this))); break;
+    case LACE_BF16: LACE_TRY((dinv_t<float, uint16_t>(this))); break;
+    default: LACE_TRY((dinv_t<float, _Float16>(this))); break;
+  }
+  dirty = false;
+  return DEV_SUCCESS;
+}
+
+template <int BS, typename TC, typename TS>
+static int apply_bs(BILU* B, lace_trsv_kind_t kind, int sweeps, const TC* r, TC* z) {
+  const int nbr = B->nbr;
+  const int* lptr = B->lptr.as<int>();
+  const int* lcol = B->lcol.as<int>();
+  const int* urp = B->urp.as<int>();
+  const int* uidx = B->uidx.as<int>();
+  const int* ucolk = B->ucolk.as<int>();
+  const float* rs = B->rs.as<float>();
+  const TS* Lv = B->Lv[B->cur].as<TS>();
+  const TS* Uv = B->Uv[B->cur].as<TS>();
+  const TC* Dinv = B->Dinv.as<TC>();
+  TC* w0 = B->work.as<TC>();
+  TC* w1 = w0 + (size_t)B->n();
+  if (kind == LACE_TRSV_LEVEL) {
+    {
+      TimedRegion tr("btrsv_lower");
+      for (size_t l = 0; l + 1 < B->h_llev.size(); l++) {
+        const int cnt = B->h_llev[l + 1] - B->h_llev[l];
+        btri_lower_kernel<BS, TC, TS, true><<<group_blocks<BS>(cnt), kBBS, 0, stream()>>>(
+            cnt, B->lperm.as<int>() + B->h_llev[l], lptr, lcol, Lv, rs, r, w0, w0);
+      }
+      LACE_HIP(hipGetLastError());
+    }
+    TimedRegion tr("btrsv_upper");
+    for (size_t l = 0; l + 1 < B->h_ulev.size(); l++) {
+      const int cnt = B->h_ulev[l + 1] - B->h_ulev[l];
+      btri_upper_kernel<BS, TC, TS, true, false><<<group_blocks<BS>(cnt), kBBS, 0, stream()>>>(
+          cnt, B->uperm.as<int>() + B->h_ulev[l], urp, uidx, ucolk, Uv, rs, Dinv, w0, z, z);
+    }
+    LACE_HIP(hipGetLastError());
+    return DEV_SUCCESS;
+  }
+  LACE_REQUIRE(kind == LACE_TRSV_JACOBI && sweeps >= 1, DEV_ERR_ILLEGAL_VALUE,
+               "block trsv: kind %d with %d sweeps", (int)kind, sweeps);
+  const int nb = group_blocks<BS>(nbr);
+  // lower: y1 = r, then sweeps-1 Jacobi updates alternating w0/w1
+  TC* y = w0;
+  {
+    TimedRegion tr("btrsv_jacobi_lower");
+    LACE_HIP(hipMemcpyAsync(w0, r, sizeof(TC) * B->n(), hipMemcpyDeviceToDevice, stream()));
+    for (int s = 1; s < sweeps; s++) {
+      TC* nxt = (y == w0) ? w1 : w0;
+      btri_lower_kernel<BS, TC, TS, false><<<nb, kBBS, 0, stream()>>>(nbr, nullptr, lptr, lcol, Lv, rs,
+                                                                      r, y, nxt);
+      y = nxt;
+    }
+    LACE_HIP(hipGetLastError());
+  }
+  // upper on rhs y: iterates u_1..u_k with u_k landing in z
+  TimedRegion tr("btrsv_jacobi_upper");
+  TC* other = (y == w0) ? w1 : w0;
+  for (int s = 1; s <= sweeps; s++) {
+    TC* dst = ((sweeps - s) % 2 == 0) ? z : other;
+    TC* src = (dst == z) ? other : z;
+    if (s == 1)
+      btri_upper_kernel<BS, TC, TS, false, true><<<nb, kBBS, 0, stream()>>>(
+          nbr, nullptr, urp, uidx, ucolk, Uv, rs, Dinv, y, nullptr, dst);
+    else
+      btri_upper_kernel<BS, TC, TS, false, false><<<nb, kBBS, 0, stream()>>>(
+          nbr, nullptr, urp, uidx, ucolk, Uv, rs, Dinv, y, src, dst);
+  }
+  LACE_HIP(hipGetLastError());
+  return DEV_SUCCESS;
+}
+
+template <typename TC, typename TS>
+static int apply_t(BILU* B, lace_trsv_kind_t kind, int sweeps, const void* r, void* z) {
+  const TC* rr = static_cast<const TC*>(r);
+  TC* zz = static_cast<TC*>(z);
+  switch (B->bs) {
+    case 1: return apply_bs<1, TC, TS>(B, kind, sweeps, rr, zz);
+    case 2: return apply_bs<2, TC, TS>(B, kind, sweeps, rr, zz);
+    case 3: return apply_bs<3, TC, TS>(B, kind, sweeps, rr, zz);
+    case 4: return apply_bs<4, TC, TS>(B, kind, sweeps, rr, zz);
+    case 5: return apply_bs<5, TC, TS>(B, kind, sweeps, rr, zz);
+    case 6: return apply_bs<6, TC, TS>(B, kind, sweeps, rr, zz);
+    case 7: return apply_bs<7, TC, TS>(B, kind, sweeps, rr, zz);
+    default: return apply_bs<8, TC, TS>(B, kind, sweeps, rr, zz);
+  }
+}
+
+int BILU::apply(lace_trsv_kind_t kind, int sweeps, const void* r, void* z) {
+  LACE_TRY(refresh());
+  switch (store) {
+    case LACE_F64: return apply_t<double, double>(this, kind, sweeps, r, z);
+    case LACE_F32: return apply_t<float, float>(this, kind, sweeps, r, z);
+    case LACE_BF16: return apply_t<float, uint16_t>(this, kind, sweeps, r, z);
+    default: return apply_t<float, _Float16>(this, kind, sweeps, r, z);
+  }
+}
+
+// Host copies of the factors in the reference's containers: L as BCSRL
+// (identity diagonal blocks written out), U as BCSCU.  Values are the
+// stored ones, widened to double.
+int BILU::get(data_d_matrix* L, data_d_matrix* U) {
+  const size_t lb = (size_t)bs * bs;
+  std::vector<float> h_rs((size_t)n());
+  LACE_TRY(download(h_rs.data(), rs, sizeof(float) * h_rs.size()));
+  auto widen = [&](const DevBuf& src, int count, const std::vector<int>& rowblk,
+                   std::vector<double>& dst) -> int {
+    const size_t nv = (size_t)count * lb;
+    dst.assign(nv, 0.0);
+    if (store == LACE_F64) return download(dst.data(), src, sizeof(double) * nv);
+    if (store == LACE_F32) {
+      std::vector<float> f(nv);
+      LACE_TRY(download(f.data(), src, sizeof(float) * nv));
+      for (size_t e = 0; e < nv; e++) dst[e] = f[e];
+      return DEV_SUCCESS;
+    }
+    std::vector<uint16_t> h(nv);
+    LACE_TRY(download(h.data(), src, sizeof(uint16_t) * nv));
+    for (int q = 0; q < count; q++)
+      for (size_t e = 0; e < lb; e++) {
+        const uint16_t bits = h[(size_t)q * lb + e];
+        float v;
+        if (store == LACE_BF16) {
+          const uint32_t u = (uint32_t)bits << 16;
+          memcpy(&v, &u, 4);
+        } else {
+          _Float16 hv;
+          memcpy(&hv, &bits, 2);
+          v = (float)hv;
+        }
+        dst[(size_t)q * lb + e] = (double)(v * h_rs[(size_t)rowblk[q] * bs + e / bs]);
+      }
+    return DEV_SUCCESS;
+  };
+  std::vector<double> lv, uv;
+  LACE_TRY(widen(Lv[cur], nnzL, h_lrow, lv));
+  LACE_TRY(widen(Uv[cur], nnzU, h_uri, uv));
+  data_zmfree(L);
+  data_zmfree(U);
+  auto fill = [&](data_d_matrix* M, data_storage_t st, data_uplo_t fm, data_order_t mj,
+                  data_diagorder_t dg, const std::vector<int>& ptr, const std::vector<int>& idx) -> int {
+    const int cnt = ptr[nbr];
+    M->storage_type = st;
+    M->fill_mode = fm;
+    M->major = mj;
+    M->diagorder_type = dg;
+    M->num_rows = M->num_cols = nbr;
+    M->blocksize = bs;
+    M->ldblock = (int)lb;
+    M->numblocks = cnt;
+    M->nnz = M->true_nnz = (int)(cnt * lb);
+    LACE_TRY(host_calloc(&M->row, nbr + 1));
+    LACE_TRY(host_calloc(&M->col, std::max(cnt, 1)));
+    LACE_TRY(host_calloc(&M->val, (size_t)std::max(cnt, 1) * lb));
+    memcpy(M->row, ptr.data(), sizeof(int) * (nbr + 1));
+    memcpy(M->col, idx.data(), sizeof(int) * cnt);
+    return DEV_SUCCESS;
+  };
+  // the reference tags its BCSRL result Magma_BCSR (magma_zmconverter.cpp:1251)
+  LACE_TRY(fill(L, Magma_BCSR, MagmaLower, MagmaRowMajor, Magma_UNITY, h_lptr, h_lcol));
+  LACE_TRY(fill(U, Magma_BCSCU, MagmaUpper, MagmaColMajor, Magma_VALUE, h_ucp, h_uri));
+  for (int q = 0; q < nnzL; q++) {
+    const bool diag = h_lrow[q] == h_lcol[q];
+    for (size_t e = 0; e < lb; e++)
+      L->val[(size_t)q * lb + e] = diag ? ((e / bs == e % bs) ? 1.0 : 0.0) : lv[(size_t)q * lb + e];
+  }
+  memcpy(U->val, uv.data(), sizeof(double) * uv.size());
+  return DEV_SUCCESS;
+}
+
+}  // namespace lace
+
+using namespace lace;
+
+extern "C" {
+
+int lace_bcsr_create(const data_d_matrix* A, lace_prec_t prec, lace_bcsr** out) {
+  auto* M = new DevBCSR();
+  const int rc = M->create(A, prec);
+  if (rc != DEV_SUCCESS) {
+    delete M;
+    return rc;
+  }
+  *out = reinterpret_cast<lace_bcsr*>(M);
+  return DEV_SUCCESS;
+}
+
+int lace_bcsr_destroy(lace_bcsr* A) {
+  delete reinterpret_cast<DevBCSR*>(A);
+  return DEV_SUCCESS;
+}
+
+int lace_bcsr_spmv(const lace_bcsr* Ah, const void* x, void* y) {
+  LACE_TRY(ensure_init());
+  const DevBCSR& A = *reinterpret_cast<const DevBCSR*>(Ah);
+  if (A.prec == LACE_F64)
+    return bcsr_spmv<double, double>(A, static_cast<const double*>(x), static_cast<double*>(y));
+  return bcsr_spmv<float, float>(A, static_cast<const float*>(x), static_cast<float*>(y));
+}
+
+double lace_bcsr_spmv_bytes(const lace_bcsr* Ah) {
+  const DevBCSR& A = *reinterpret_cast<const DevBCSR*>(Ah);
+  const int s = A.prec == LACE_F64 ? 8 : 4;
+  return A.bytes(s, s, false);
+}
+
+int lace_bilu_create(const data_d_matrix* A, lace_prec_t prec, lace_bilu** out) {
+  auto* B = new BILU();
+  const int rc = B->create(A, prec);
+  if (rc != DEV_SUCCESS) {
+    delete B;
+    return rc;
+  }
+  *out = reinterpret_cast<lace_bilu*>(B);
+  return DEV_SUCCESS;
+}
+
+int lace_bilu_destroy(lace_bilu* B) {
+  delete reinterpret_cast<BILU*>(B);
+  return DEV_SUCCESS;
+}
+
+int lace_bilu_reset(lace_bilu* B) {
+  LACE_TRY(ensure_init());
+  return reinterpret_cast<BILU*>(B)->reset();
+}
+
+int lace_bilu_sweeps(lace_bilu* Bh, int nsweeps, double* step) {
+  LACE_TRY(ensure_init());
+  auto* B = reinterpret_cast<BILU*>(Bh);
+  for (int s = 0; s < nsweeps; s++) LACE_TRY(B->sweep(step != nullptr && s == nsweeps - 1));
+  if (step) {
+    if (nsweeps > 0) LACE_TRY(B->read_step(step));
+    else *step = 0.0;
+  }
+  return DEV_SUCCESS;
+}
+
+int lace_bilu_get(lace_bilu* B, data_d_matrix* L, data_d_matrix* U) {
+  LACE_TRY(ensure_init());
+  return reinterpret_cast<BILU*>(B)->get(L, U);
+}
+
+double lace_bilu_sweep_bytes(const lace_bilu* B) {
+  return reinterpret_cast<const BILU*>(B)->sweep_bytes();
+}
+
+int lace_bilu_apply(lace_bilu* B, lace_trsv_kind_t kind, int jacobi_sweeps, const void* r, void* z) {
+  LACE_TRY(ensure_init());
+  return reinterpret_cast<BILU*>(B)->apply(kind, jacobi_sweeps, r, z);
+}
+
+int lace_bilu_levels(const lace_bilu* Bh, int* nlev_L, int* nlev_U) {
+  const auto* B = reinterpret_cast<const BILU*>(Bh);
+  if (nlev_L) *nlev_L = (int)B->h_llev.size() - 1;
+  if (nlev_U) *nlev_U = (int)B->h_ulev.size() - 1;
+  return DEV_SUCCESS;
+}
+
+int lace_bilu_set_variant(lace_bilu* B, int variant) {
+  LACE_TRY(ensure_init());
+  return reinterpret_cast<BILU*>(B)->set_variant(variant);
+}
+
+int lace_bilu_is_symmetric(const lace_bilu* B) { return reinterpret_cast<const BILU*>(B)->sym ? 1 : 0; }
+
+// src/parilu_v0_3_bcsr.cpp:16-276 (which defines the symbol
+// data_PariLU_v0_3 a second time; here it has its own name).  Sweeps run
+// on the device in FP64 until the unscaled step sum_(new-old)^2 is at most
+// `reduction` (:61, :118); the reference has no sweep cap, this one stops
+// at 1000 sweeps and returns DEV_NOTCONVERGED.  L is returned as BCSRL
+// (identity diagonal blocks), U as BCSCU.
+int data_PariLU_v0_3_bcsr(data_d_matrix* A, data_d_matrix* L, data_d_matrix* U, dataType reduction,
+                          data_d_preconditioner_log* log) {
+  LACE_TRY(ensure_init());
+  auto t0 = std::chrono::steady_clock::now();
+  BILU B;
+  LACE_TRY(B.create(A, LACE_F64));
+  double step = 1e300;
+  int iter = 0;
+  const int cap = 1000;
+  while (step > reduction && iter < cap) {
+    LACE_TRY(B.sweep(true));
+    LACE_TRY(B.read_step(&step));
+    iter++;
+  }
+  LACE_TRY(B.get(L, U));
+  auto t1 = std::chrono::steady_clock::now();
+  log->sweeps = iter;
+  log->tol = reduction;
+  log->finalStep = step;
+  log->A_Frobenius = B.anorm;
+  log->precond_generation_time = std::chrono::duration<double>(t1 - t0).count();
+  log->initial_residual = 0.0;
+  log->initial_nonlinear_residual = 0.0;
+  return step <= reduction ? DEV_SUCCESS : DEV_NOTCONVERGED;
+}
+
+}  // extern "C"

@@ -117,6 +117,168 @@ int transpose_csr(const data_d_matrix& A, data_d_matrix* B) {
   return DEV_SUCCESS;
 }
 
+bool is_bcsr_like(data_storage_t s) {
+  return s == Magma_BCSR || s == Magma_BCSRD || s == Magma_BCSRL || s == Magma_BCSRU ||
+         s == Magma_BCSC || s == Magma_BCSCD || s == Magma_BCSCL || s == Magma_BCSCU;
+}
+
+void copy_block_header(const data_d_matrix& A, data_d_matrix* B) {
+  B->num_rows = A.num_rows;
+  B->num_cols = A.num_cols;
+  B->pad_rows = A.pad_rows;
+  B->pad_cols = A.pad_cols;
+  B->diameter = A.diameter;
+  B->blocksize = A.blocksize;
+  B->ldblock = A.ldblock;
+}
+
+// CSR -> BCSR with bs = B->blocksize (the reference calls mkl_dcsrbsr with
+// zero-based indexing, which keeps blocks row-major,
+// control/magma_zmconverter.cpp:639-677): a block is stored when any entry
+// of A falls in it; block columns ascending; absent entries are zero.
+int csr_to_bcsr(const data_d_matrix& A, data_d_matrix* B) {
+  const int bs = B->blocksize;
+  LACE_REQUIRE(bs > 0, DEV_ERR_NOT_SUPPORTED,
+               "data_zmconvert: conversion to BCSR requires blocksize > 0 (have %d)", bs);
+  const int nbr = (A.num_rows + bs - 1) / bs, nbc = (A.num_cols + bs - 1) / bs;
+  const int lb = bs * bs;
+  std::vector<int> brow(nbr + 1, 0), bcol, mark(nbc, -1);
+  for (int I = 0; I < nbr; I++) {
+    brow[I] = (int)bcol.size();
+    const size_t start = bcol.size();
+    for (int i = I * bs; i < std::min(A.num_rows, (I + 1) * bs); i++)
+      for (int k = A.row[i]; k < A.row[i + 1]; k++) {
+        const int J = A.col[k] / bs;
+        if (mark[J] != I) {
+          mark[J] = I;
+          bcol.push_back(J);
+        }
+      }
+    std::sort(bcol.begin() + start, bcol.end());
+  }
+  brow[nbr] = (int)bcol.size();
+  const int nb = brow[nbr];
+  B->storage_type = Magma_BCSR;
+  B->major = MagmaRowMajor;
+  B->fill_mode = A.fill_mode;
+  B->num_rows = nbr;
+  B->num_cols = nbc;
+  B->ldblock = lb;
+  B->numblocks = nb;
+  B->nnz = nb * lb;
+  B->true_nnz = A.nnz;
+  LACE_TRY(host_calloc(&B->row, nbr + 1));
+  LACE_TRY(host_calloc(&B->col, std::max(nb, 1)));
+  LACE_TRY(host_calloc(&B->val, (size_t)std::max(nb, 1) * lb));
+  memcpy(B->row, brow.data(), sizeof(int) * (nbr + 1));
+  if (nb) memcpy(B->col, bcol.data(), sizeof(int) * nb);
+  for (int I = 0; I < nbr; I++)
+    for (int i = I * bs; i < std::min(A.num_rows, (I + 1) * bs); i++)
+      for (int k = A.row[i]; k < A.row[i + 1]; k++) {
+        const int J = A.col[k] / bs;
+        const int* it = std::lower_bound(B->col + brow[I], B->col + brow[I + 1], J);
+        const size_t q = (size_t)(it - B->col);
+        B->val[q * lb + (i - I * bs) * bs + (A.col[k] - J * bs)] += A.val[k];
+      }
+  return DEV_SUCCESS;
+}
+
+// BCSR -> BCSRL / BCSRU (control/magma_zmconverter.cpp:1248-1436): strict
+// lower (upper) blocks and the diagonal block per B->diagorder_type --
+// UNITY: identity, ZERO: zeros, NODIAG: omitted, any other tag: A's block.
+// BCSRL comes back tagged Magma_BCSR as in the reference (:1251).
+int bcsr_split(const data_d_matrix& A, data_d_matrix* B, bool upper) {
+  const int nbr = A.num_rows, bs = A.blocksize, lb = bs * bs;
+  const data_diagorder_t dmode = B->diagorder_type;
+  int cnt = 0;
+  for (int I = 0; I < nbr; I++)
+    for (int k = A.row[I]; k < A.row[I + 1]; k++) {
+      const int J = A.col[k];
+      if ((upper && J > I) || (!upper && J < I) || (J == I && dmode != Magma_NODIAG)) cnt++;
+    }
+  copy_block_header(A, B);
+  B->storage_type = upper ? Magma_BCSRU : Magma_BCSR;
+  B->major = MagmaRowMajor;
+  B->fill_mode = upper ? MagmaUpper : MagmaLower;
+  B->ldblock = lb;
+  B->numblocks = cnt;
+  B->nnz = cnt * lb;
+  LACE_TRY(host_calloc(&B->row, nbr + 1));
+  LACE_TRY(host_calloc(&B->col, std::max(cnt, 1)));
+  LACE_TRY(host_calloc(&B->val, (size_t)std::max(cnt, 1) * lb));
+  int z = 0;
+  for (int I = 0; I < nbr; I++) {
+    B->row[I] = z;
+    for (int k = A.row[I]; k < A.row[I + 1]; k++) {
+      const int J = A.col[k];
+      const bool off = (upper && J > I) || (!upper && J < I);
+      if (!off && !(J == I && dmode != Magma_NODIAG)) continue;
+      double* dst = B->val + (size_t)z * lb;
+      if (off || (dmode != Magma_UNITY && dmode != Magma_ZERO)) {
+        memcpy(dst, A.val + (size_t)k * lb, sizeof(double) * lb);
+      } else if (dmode == Magma_UNITY) {
+        for (int a = 0; a < bs; a++) dst[a * bs + a] = 1.0;
+      }
+      B->col[z++] = J;
+    }
+  }
+  B->row[nbr] = z;
+  return DEV_SUCCESS;
+}
+
+// Block transpose by counting sort: blocks move, their interiors are kept
+// as stored (control/magma_zmtranspose.cpp:237-380).
+int transpose_bcsr(const data_d_matrix& A, data_d_matrix* B) {
+  const int m = A.num_rows, n = A.num_cols, nb = A.row[m], lb = A.blocksize * A.blocksize;
+  data_int_t *row = nullptr, *col = nullptr;
+  dataType* val = nullptr;
+  LACE_TRY(host_calloc(&row, n + 1));
+  LACE_TRY(host_calloc(&col, std::max(nb, 1)));
+  LACE_TRY(host_calloc(&val, (size_t)std::max(nb, 1) * lb));
+  for (int k = 0; k < nb; k++) row[A.col[k] + 1]++;
+  for (int j = 0; j < n; j++) row[j + 1] += row[j];
+  std::vector<int> fill(row, row + n);
+  for (int i = 0; i < m; i++)
+    for (int k = A.row[i]; k < A.row[i + 1]; k++) {
+      const int p = fill[A.col[k]]++;
+      col[p] = i;
+      memcpy(val + (size_t)p * lb, A.val + (size_t)k * lb, sizeof(double) * lb);
+    }
+  copy_block_header(A, B);
+  B->num_rows = n;
+  B->num_cols = m;
+  B->pad_rows = A.pad_cols;
+  B->pad_cols = A.pad_rows;
+  B->row = row;
+  B->col = col;
+  B->val = val;
+  B->numblocks = nb;
+  B->nnz = A.nnz;
+  B->true_nnz = A.true_nnz;
+  B->fill_mode = A.fill_mode;
+  B->storage_type = A.storage_type;
+  return DEV_SUCCESS;
+}
+
+int copy_bcsr(const data_d_matrix& A, data_d_matrix* B) {
+  const int nbr = A.num_rows, nb = A.row[nbr], lb = A.blocksize * A.blocksize;
+  copy_block_header(A, B);
+  B->storage_type = Magma_BCSR;
+  B->major = MagmaRowMajor;
+  B->fill_mode = A.fill_mode;
+  B->ldblock = lb;
+  B->numblocks = nb;
+  B->nnz = nb * lb;
+  B->true_nnz = A.true_nnz;
+  LACE_TRY(host_calloc(&B->row, nbr + 1));
+  LACE_TRY(host_calloc(&B->col, std::max(nb, 1)));
+  LACE_TRY(host_calloc(&B->val, (size_t)std::max(nb, 1) * lb));
+  memcpy(B->row, A.row, sizeof(int) * (nbr + 1));
+  memcpy(B->col, A.col, sizeof(int) * nb);
+  memcpy(B->val, A.val, sizeof(double) * (size_t)nb * lb);
+  return DEV_SUCCESS;
+}
+
 int copy_csr(const data_d_matrix& A, data_d_matrix* B) {
   const int n = A.num_rows, nnz = A.row[n];
   B->storage_type = Magma_CSR;
@@ -238,6 +400,7 @@ int data_rowindex(data_d_matrix* A, int** rowidx) {
 
 int data_zmtranspose(data_d_matrix A, data_d_matrix* B) {
   if (is_csr_like(A.storage_type) || is_csc_like(A.storage_type)) return transpose_csr(A, B);
+  if (is_bcsr_like(A.storage_type)) return transpose_bcsr(A, B);
   if (is_dense(A.storage_type)) {
     LACE_TRY(dense_copy(A, B));
     // transpose in place of the copy
@@ -271,6 +434,7 @@ int data_zmconvert(data_d_matrix A, data_d_matrix* B, data_storage_t old_format,
         return data_rowindex(B, &B->rowidx);
       }
       case Magma_DENSE: return csr_to_dense(A, B);
+      case Magma_BCSR: return csr_to_bcsr(A, B);
       case Magma_CSC: {
         LACE_TRY(transpose_csr(A, B));
         B->storage_type = Magma_CSC;
@@ -309,6 +473,26 @@ int data_zmconvert(data_d_matrix A, data_d_matrix* B, data_storage_t old_format,
       case Magma_CSCU: {
         LACE_TRY(split(A, B, new_format == Magma_CSCU));
         B->storage_type = new_format;
+        B->major = MagmaColMajor;
+        return DEV_SUCCESS;
+      }
+      default: break;
+    }
+  } else if (old_format == Magma_BCSR) {
+    // control/magma_zmconverter.cpp:1248-1445
+    switch (new_format) {
+      case Magma_BCSR: return copy_bcsr(A, B);
+      case Magma_BCSRL: return bcsr_split(A, B, false);
+      case Magma_BCSRU: return bcsr_split(A, B, true);
+      case Magma_BCSCU: {
+        data_d_matrix C;
+        memset(&C, 0, sizeof(C));
+        C.diagorder_type = B->diagorder_type;
+        LACE_TRY(bcsr_split(A, &C, true));
+        const int r = transpose_bcsr(C, B);
+        data_zmfree(&C);
+        LACE_TRY(r);
+        B->storage_type = Magma_BCSCU;
         B->major = MagmaColMajor;
         return DEV_SUCCESS;
       }
@@ -748,6 +932,55 @@ int lace_gen_convdiff3d(int nx, int ny, int nz, double pe_x, double pe_y, double
         if (k < nz - 1) { A->col[z] = p + sz; A->val[z++] = -1.0; }
       }
   A->row[n] = z;
+  return DEV_SUCCESS;
+}
+
+int lace_gen_bcsr7(int nx, int ny, int nz, int bs, uint64_t seed, double sigma, double shift,
+                   data_d_matrix* A) {
+  const long nl = (long)nx * ny * nz;
+  LACE_REQUIRE(nx > 0 && ny > 0 && nz > 0 && bs > 0, DEV_ERR_ILLEGAL_VALUE, "lace_gen_bcsr7: sizes");
+  LACE_REQUIRE(nl * bs < (1L << 31), DEV_ERR_ILLEGAL_VALUE, "grid too large for int32 indices");
+  const int nbr = (int)nl, lb = bs * bs;
+  long nb = 0;
+  for (int k = 0; k < nz; k++)
+    for (int j = 0; j < ny; j++)
+      for (int i = 0; i < nx; i++)
+        nb += 1 + (i > 0) + (i < nx - 1) + (j > 0) + (j < ny - 1) + (k > 0) + (k < nz - 1);
+  LACE_REQUIRE(nb * lb < (1L << 31), DEV_ERR_ILLEGAL_VALUE, "too many entries for int32 counts");
+  data_zmfree(A);
+  A->storage_type = Magma_BCSR;
+  A->major = MagmaRowMajor;
+  A->fill_mode = MagmaFull;
+  A->sym = Magma_GENERAL;
+  A->diagorder_type = Magma_VALUE;
+  A->num_rows = A->num_cols = nbr;
+  A->blocksize = bs;
+  A->ldblock = lb;
+  A->numblocks = (int)nb;
+  A->nnz = A->true_nnz = (int)(nb * lb);
+  A->max_nnz_row = 7;
+  LACE_TRY(host_calloc(&A->row, nbr + 1));
+  LACE_TRY(host_calloc(&A->col, nb));
+  LACE_TRY(host_calloc(&A->val, (size_t)nb * lb));
+  LACE_TRY(lace_gen_normal(seed, (int)(nb * lb), A->val));
+  for (size_t e = 0; e < (size_t)nb * lb; e++) A->val[e] *= sigma;
+  const int sy = nx, sz = nx * ny;
+  int z = 0;
+  for (int k = 0; k < nz; k++)
+    for (int j = 0; j < ny; j++)
+      for (int i = 0; i < nx; i++) {
+        const int p = i + sy * j + sz * k;
+        A->row[p] = z;
+        if (k > 0) A->col[z++] = p - sz;
+        if (j > 0) A->col[z++] = p - sy;
+        if (i > 0) A->col[z++] = p - 1;
+        for (int a = 0; a < bs; a++) A->val[(size_t)z * lb + a * bs + a] += shift;
+        A->col[z++] = p;
+        if (i < nx - 1) A->col[z++] = p + 1;
+        if (j < ny - 1) A->col[z++] = p + sy;
+        if (k < nz - 1) A->col[z++] = p + sz;
+      }
+  A->row[nbr] = z;
   return DEV_SUCCESS;
 }
 

@@ -7,6 +7,17 @@
 
 namespace lace {
 
+// The operator of a solve: y = A x with the vector precisions the engines
+// need (FP64 Krylov vectors, FP32 preconditioned vectors widened, and the
+// FP32 operator of the inner solver), and the FP64 residual r = b - A x.
+struct LinOp {
+  virtual ~LinOp() = default;
+  virtual int apply_dd(const double* x, double* y) const = 0;
+  virtual int apply_fd(const float* x, double* y) const = 0;
+  virtual int apply_ff(const float* x, float* y) const = 0;
+  virtual int residual_dd(const double* x, const double* b, double* r) const = 0;
+};
+
 // z = M^{-1} q.  q has the Krylov precision of the calling engine; z is a
 // column of the engine's Z store (precision ZT of the engine).
 struct Precond {

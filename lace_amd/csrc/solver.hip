@@ -19,11 +19,35 @@
 #include <memory>
 #include <vector>
 
+#include "bcsr.hpp"
 #include "common.hpp"
 #include "device.hpp"
 #include "krylov.hpp"
 
 namespace lace {
+
+// ------------------------------------------------------------ operators
+struct CsrOp : LinOp {
+  const DevCSR* A;
+  explicit CsrOp(const DevCSR* a) : A(a) {}
+  int apply_dd(const double* x, double* y) const override { return spmv<double, double>(*A, 1.0, x, 0.0, y); }
+  int apply_fd(const float* x, double* y) const override { return spmv<float, double>(*A, 1.0, x, 0.0, y); }
+  int apply_ff(const float* x, float* y) const override { return spmv<float, float>(*A, 1.0, x, 0.0, y); }
+  int residual_dd(const double* x, const double* b, double* r) const override {
+    return residual<double, double>(*A, x, b, r);
+  }
+};
+
+struct BcsrOp : LinOp {
+  const DevBCSR* A;
+  explicit BcsrOp(const DevBCSR* a) : A(a) {}
+  int apply_dd(const double* x, double* y) const override { return bcsr_spmv<double, double>(*A, x, y); }
+  int apply_fd(const float* x, double* y) const override { return bcsr_spmv<float, double>(*A, x, y); }
+  int apply_ff(const float* x, float* y) const override { return bcsr_spmv<float, float>(*A, x, y); }
+  int residual_dd(const double* x, const double* b, double* r) const override {
+    return bcsr_residual<double, double>(*A, x, b, r);
+  }
+};
 
 template <typename K>
 int hh_start(HHEngine<K>& E, int set_rtol, double rtol_rel);
@@ -61,6 +85,29 @@ struct PrecILU : Precond {
   }
 };
 
+// Block ILU factors: FP64 storage takes FP64 vectors; F32/BF16/F16 storage
+// iterates in FP32, so FP64 input is narrowed first and z is FP32.
+struct PrecBILU : Precond {
+  BILU* B;
+  lace_trsv_kind_t kind;
+  int sweeps;
+  DevBuf qf;
+  PrecBILU(BILU* b, lace_trsv_kind_t k, int s) : B(b), kind(k), sweeps(s) {}
+  bool z_is_float() const override { return B->store != LACE_F64; }
+  int apply_d(const double* q, void* z) override {
+    TimedRegion tr("precond");
+    if (B->store == LACE_F64) return B->apply(kind, sweeps, q, z);
+    if (qf.bytes < sizeof(float) * B->n()) LACE_TRY(qf.alloc(sizeof(float) * B->n()));
+    LACE_TRY(cast_d2f(B->n(), q, qf.as<float>()));
+    return B->apply(kind, sweeps, qf.as<float>(), z);
+  }
+  int apply_f(const float* q, void* z) override {
+    TimedRegion tr("precond");
+    LACE_REQUIRE(B->store != LACE_F64, DEV_ERR_NOT_SUPPORTED, "FP32 input to FP64 block factors");
+    return B->apply(kind, sweeps, q, z);
+  }
+};
+
 struct PrecIdentity : Precond {
   int n;
   explicit PrecIdentity(int n_) : n(n_) {}
@@ -74,24 +121,23 @@ struct PrecIdentity : Precond {
 // Inner FP32 Householder GMRES with a fixed FP32 ILU preconditioner,
 // `iters` steps from x = 0, no host synchronisation.  Output in FP32.
 struct PrecInner : Precond {
-  const DevCSR* A32;
-  PrecILU* M;
+  const LinOp* A32;
+  Precond* M;
   HHEngine<float> E;
-  int iters;
+  int n, iters;
   double rtol;
-  PrecInner(const DevCSR* a, PrecILU* m, int it, double rt) : A32(a), M(m), iters(it), rtol(rt) {}
-  int init() { return E.init(A32->n, iters, true); }
+  PrecInner(const LinOp* a, Precond* m, int n_, int it, double rt)
+      : A32(a), M(m), n(n_), iters(it), rtol(rt) {}
+  int init() { return E.init(n, iters, true); }
   bool z_is_float() const override { return true; }
   int apply_d(const double* q, void* z) override {
     TimedRegion tr("inner_gmres");
-    const int n = A32->n;
     LACE_TRY(cast_d2f(n, q, E.r.as<float>()));
     LACE_HIP(hipMemsetAsync(z, 0, sizeof(float) * n, stream()));
     LACE_TRY(hh_start<float>(E, 1, rtol));
     for (int j = 0; j < iters; j++) {
       LACE_TRY(M->apply_f(E.q.as<float>(), E.zcol(j)));
-      LACE_TRY((spmv<float, float>(*A32, 1.0, static_cast<const float*>(E.zcol(j)), 0.0,
-                                   E.w.as<float>())));
+      LACE_TRY(A32->apply_ff(static_cast<const float*>(E.zcol(j)), E.w.as<float>()));
       LACE_TRY(hh_step<float>(E, j, j < iters - 1));
     }
     return hh_finish<float, float>(E, static_cast<float*>(z));
@@ -109,7 +155,14 @@ struct Solver {
   std::vector<int> h_row, h_col;  // pattern of A (for A^T)
   std::vector<int> t_from;        // A^T position -> A position
   std::vector<double> h_val;      // current operator values
-  const DevCSR* op64 = nullptr;   // operators of the current solve
+  // block (BCSR) operator and block ILU factors
+  bool block = false;
+  DevBCSR B64, B32;
+  BILU* BP = nullptr;
+  std::unique_ptr<PrecBILU> bilu;
+  // operator wrappers; op64 is the FP64 operator of the current solve
+  std::unique_ptr<LinOp> opA64, opA32, opAT64, opAT32;
+  const LinOp* op64 = nullptr;
   ParILU* P = nullptr;
   std::unique_ptr<PrecILU> ilu;
   std::unique_ptr<PrecIdentity> ident;
@@ -121,6 +174,7 @@ struct Solver {
   double* host_scal = nullptr;
   ~Solver() {
     delete P;
+    delete BP;
     if (host_scal) (void)hipHostFree(host_scal);
   }
 };
@@ -157,43 +211,79 @@ static int ensure_transpose(Solver* S) {
   for (int q = 0; q < nnz; q++) tv[q] = S->h_val[S->t_from[q]];
   data_d_matrix T = csr_view(n, tr, tc, tv.data());
   LACE_TRY(S->AT64.create(&T, LACE_F64));
-  if (S->inner) LACE_TRY(S->AT32.create(&T, LACE_F32));
+  S->opAT64 = std::make_unique<CsrOp>(&S->AT64);
+  if (S->inner) {
+    LACE_TRY(S->AT32.create(&T, LACE_F32));
+    S->opAT32 = std::make_unique<CsrOp>(&S->AT32);
+  }
   S->have_T = true;
   return DEV_SUCCESS;
 }
 
 // Select the operator (A or A^T) and the matching preconditioner direction.
 static int select_direction(Solver* S, int trans) {
+  LACE_REQUIRE(!(trans && S->block), DEV_ERR_NOT_SUPPORTED,
+               "solver: transposed solves are available for CSR operators only");
   if (trans) LACE_TRY(ensure_transpose(S));
-  S->op64 = trans ? &S->AT64 : &S->A64;
+  S->op64 = trans ? S->opAT64.get() : S->opA64.get();
   if (S->ilu) S->ilu->trans = trans ? 1 : 0;
-  if (S->inner) S->inner->A32 = trans ? &S->AT32 : &S->A32;
+  if (S->inner) S->inner->A32 = trans ? S->opAT32.get() : S->opA32.get();
   return DEV_SUCCESS;
 }
 
 static int solver_create(const data_d_matrix* A, const lace_solver_opts* o, Solver** out) {
   LACE_TRY(ensure_init());
   LACE_REQUIRE(A && A->row && A->col && A->val && A->num_rows == A->num_cols, DEV_ERR_ILLEGAL_VALUE,
-               "solver: A must be a square CSR matrix");
+               "solver: A must be a square CSR or BCSR matrix");
   auto S = std::make_unique<Solver>();
   S->o = *o;
-  S->n = A->num_rows;
-  S->nnz = A->row[S->n];
-  S->h_row.assign(A->row, A->row + S->n + 1);
-  S->h_col.assign(A->col, A->col + S->nnz);
-  S->h_val.assign(A->val, A->val + S->nnz);
-  LACE_TRY(S->A64.create(A, LACE_F64));
-  if (o->inner_iters > 0) LACE_TRY(S->A32.create(A, LACE_F32));
+  S->block = A->storage_type == Magma_BCSR;
+  if (S->block) {
+    // block operator: FP64 BCSR for the outer products, block ILU factors
+    LACE_TRY(S->B64.create(A, LACE_F64));
+    S->n = S->B64.n();
+    S->nnz = S->B64.nblocks * S->B64.bs * S->B64.bs;
+    S->opA64 = std::make_unique<BcsrOp>(&S->B64);
+    if (o->inner_iters > 0) {
+      LACE_TRY(S->B32.create(A, LACE_F32));
+      S->opA32 = std::make_unique<BcsrOp>(&S->B32);
+    }
+  } else {
+    S->n = A->num_rows;
+    S->nnz = A->row[S->n];
+    S->h_row.assign(A->row, A->row + S->n + 1);
+    S->h_col.assign(A->col, A->col + S->nnz);
+    S->h_val.assign(A->val, A->val + S->nnz);
+    LACE_TRY(S->A64.create(A, LACE_F64));
+    S->opA64 = std::make_unique<CsrOp>(&S->A64);
+    if (o->inner_iters > 0) {
+      LACE_TRY(S->A32.create(A, LACE_F32));
+      S->opA32 = std::make_unique<CsrOp>(&S->A32);
+    }
+  }
   if (o->trsv == LACE_TRSV_NONE && o->inner_iters <= 0) {
     S->ident = std::make_unique<PrecIdentity>(S->n);
     S->M = S->ident.get();
   } else {
-    const lace_prec_t fp = o->inner_iters > 0 ? LACE_F32 : o->prec_precond;
-    LACE_TRY(parilu_create(A, fp, &S->P));
-    S->ilu = std::make_unique<PrecILU>(S->P, o->trsv, o->jacobi_sweeps, 0);
-    S->M = S->ilu.get();
+    // the inner solver iterates in FP32, so its factors are at most FP32
+    lace_prec_t fp = o->prec_precond;
+    if (o->inner_iters > 0 && fp == LACE_F64) fp = LACE_F32;
+    Precond* base = nullptr;
+    if (S->block) {
+      S->BP = new BILU();
+      LACE_TRY(S->BP->create(A, fp));
+      S->bilu = std::make_unique<PrecBILU>(S->BP, o->trsv, o->jacobi_sweeps);
+      base = S->bilu.get();
+    } else {
+      LACE_REQUIRE(fp == LACE_F64 || fp == LACE_F32, DEV_ERR_NOT_SUPPORTED,
+                   "solver: CSR factors are stored in F64 or F32 (have %d)", (int)fp);
+      LACE_TRY(parilu_create(A, fp, &S->P));
+      S->ilu = std::make_unique<PrecILU>(S->P, o->trsv, o->jacobi_sweeps, 0);
+      base = S->ilu.get();
+    }
+    S->M = base;
     if (o->inner_iters > 0) {
-      S->inner = std::make_unique<PrecInner>(&S->A32, S->ilu.get(), o->inner_iters, 0.0);
+      S->inner = std::make_unique<PrecInner>(S->opA32.get(), base, S->n, o->inner_iters, 0.0);
       LACE_TRY(S->inner->init());
       S->M = S->inner.get();
     }
@@ -213,6 +303,7 @@ static int solver_create(const data_d_matrix* A, const lace_solver_opts* o, Solv
 // New values for the operator on A's pattern (the factors are kept until the
 // next setup): updates A, its FP32 copy and A^T.
 static int solver_set_values(Solver* S, const double* vals) {
+  LACE_REQUIRE(!S->block, DEV_ERR_NOT_SUPPORTED, "solver: value updates are for CSR operators");
   S->h_val.assign(vals, vals + S->nnz);
   LACE_HIP(hipMemcpyAsync(S->A64.val.p, vals, sizeof(double) * S->nnz, hipMemcpyHostToDevice, stream()));
   if (S->inner) {
@@ -234,6 +325,12 @@ static int solver_set_values(Solver* S, const double* vals) {
 }
 
 static int solver_setup(Solver* S) {
+  if (S->BP) {
+    TimedRegion tr("setup");
+    LACE_TRY(S->BP->reset());
+    for (int s = 0; s < S->o.parilu_sweeps; s++) LACE_TRY(S->BP->sweep(false));
+    return S->BP->refresh();
+  }
   if (!S->P) return DEV_SUCCESS;
   TimedRegion tr("setup");
   LACE_TRY(S->P->reset());
@@ -263,7 +360,7 @@ static int solver_solve(Solver* S, const double* b, double* x, lace_solver_stats
   bool converged = false;
   double rtol_abs = 0.0;
   for (int restart = 0; restart < S->o.restart_max && !converged; restart++) {
-    LACE_TRY((residual<double, double>(*S->op64, x, b, E.r.as<double>())));
+    LACE_TRY(S->op64->residual_dd(x, b, E.r.as<double>()));
     if (restart == 0) {
       if (tol_abs >= 0.0)
         LACE_TRY(set_misc_d(E.small.as<double>(), E.lay, MS_RTOL, tol_abs));
@@ -286,11 +383,9 @@ static int solver_solve(Solver* S, const double* b, double* x, lace_solver_stats
     for (int j = 0; j < m; j++) {
       LACE_TRY(S->M->apply_d(E.q.as<double>(), E.zcol(j)));
       if (E.z_float)
-        LACE_TRY((spmv<float, double>(*S->op64, 1.0, static_cast<const float*>(E.zcol(j)), 0.0,
-                                      E.w.as<double>())));
+        LACE_TRY(S->op64->apply_fd(static_cast<const float*>(E.zcol(j)), E.w.as<double>()));
       else
-        LACE_TRY((spmv<double, double>(*S->op64, 1.0, static_cast<const double*>(E.zcol(j)), 0.0,
-                                       E.w.as<double>())));
+        LACE_TRY(S->op64->apply_dd(static_cast<const double*>(E.zcol(j)), E.w.as<double>()));
       LACE_TRY(hh_step<double>(E, j, j < m - 1));
       LACE_TRY(read_misc<double>(E));
       if (E.host_misc[MS_DONE] != 0.0) break;
@@ -306,7 +401,7 @@ static int solver_solve(Solver* S, const double* b, double* x, lace_solver_stats
   }
   if (S->inner) local.inner_iterations = local.iterations * S->o.inner_iters;
   // true residual in FP64 (against the operator in use)
-  LACE_TRY((residual<double, double>(*S->op64, x, b, S->rbuf.as<double>())));
+  LACE_TRY(S->op64->residual_dd(x, b, S->rbuf.as<double>()));
   LACE_TRY(norm2(S, S->rbuf.as<double>(), &local.true_residual));
   double bn = 0.0;
   LACE_TRY(norm2(S, b, &bn));
@@ -507,7 +602,7 @@ static int solver_solve_mgs(Solver* S, const double* b, double* x, lace_solver_s
   }
   for (int restart = 0; restart < S->o.restart_max && !converged; restart++) {
     double* r = Vc(0);
-    LACE_TRY((residual<double, double>(*S->op64, x, b, r)));
+    LACE_TRY(S->op64->residual_dd(x, b, r));
     LACE_TRY(nrm_into(r, sc + MG_RNORM));
     mgs_start_kernel<<<1, 64, 0, stream()>>>(sd, L, restart == 0 && tol_abs < 0.0, S->o.rtol);
     LACE_TRY(read_sc());
@@ -524,7 +619,7 @@ static int solver_solve_mgs(Solver* S, const double* b, double* x, lace_solver_s
     scale_into_kernel<<<nbv, 256, 0, stream()>>>(n, r, sc + MG_RNORM, sc + MG_RNORM, Vc(0));
     for (int j = 0; j < m; j++) {
       LACE_TRY(S->M->apply_d(Vc(j), Zc(j)));
-      LACE_TRY((spmv<double, double>(*S->op64, 1.0, Zc(j), 0.0, u.as<double>())));
+      LACE_TRY(S->op64->apply_dd(Zc(j), u.as<double>()));
       LACE_TRY(nrm_into(u.as<double>(), sc + MG_NORMAV));
       const int passes = 1;
       for (int pass = 0; pass < passes + 1; pass++) {
@@ -562,7 +657,7 @@ static int solver_solve_mgs(Solver* S, const double* b, double* x, lace_solver_s
     if (!std::isfinite(hostsc[MG_RES])) break;
     converged = hostsc[MG_RES] < rtol_abs;
   }
-  LACE_TRY((residual<double, double>(*S->op64, x, b, S->rbuf.as<double>())));
+  LACE_TRY(S->op64->residual_dd(x, b, S->rbuf.as<double>()));
   LACE_TRY(norm2(S, S->rbuf.as<double>(), &local.true_residual));
   double bn = 0.0;
   LACE_TRY(norm2(S, b, &bn));
@@ -636,6 +731,7 @@ int lace_solver_set_values(lace_solver* Sh, const double* vals) {
 int lace_solver_grad_values(lace_solver* Sh, const double* lambda, const double* x, double* dtheta) {
   LACE_TRY(ensure_init());
   auto* S = reinterpret_cast<Solver*>(Sh);
+  LACE_REQUIRE(!S->block, DEV_ERR_NOT_SUPPORTED, "solver: value gradients are for CSR operators");
   grad_gather_kernel<<<ceil_div(S->n, 256), 256, 0, stream()>>>(
       S->n, S->A64.row.as<int>(), S->A64.col.as<int>(), lambda, x, dtheta);
   LACE_HIP(hipGetLastError());

@@ -72,6 +72,11 @@ def _declare(L):
         "orc_hqrd": (None, [_i, _i, _p, _p, _p]),
         "orc_philox4x32_10": (None, [_p, _p, _p]),
         "orc_set_threads": (None, [_i]),
+        "orc_bilu_sweep_d": (_d, [_i, _i, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p]),
+        "orc_bilu_sweep_f": (_d, [_i, _i, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p]),
+        "orc_binv_double": (None, [_i, _p, _p]),
+        "orc_bcsr_spmv": (None, [_i, _i, _p, _p, _p, _p, _p]),
+        "orc_btrsv": (None, [_i, _i, _p, _p, _p, _i, _i, _p, _p]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -464,3 +469,239 @@ def hqrd(X):
     U = np.zeros((m, n), order="F"); R = np.zeros((n, n), order="F")
     lib().orc_hqrd(m, n, _ptr(X), _ptr(U), _ptr(R))
     return U, R
+
+
+# ---------------------------------------------------------------- BCSR
+@dataclass
+class BCSR:
+    """Block CSR: nbr block rows, bs x bs blocks stored row-major in
+    val[numblocks, bs, bs].  For block-column (BCSC) views, `row` is the
+    block column pointer and `col` the block row indices; block interiors
+    are never transposed (control/magma_zmtranspose.cpp:237-380)."""
+    nbr: int
+    bs: int
+    row: np.ndarray
+    col: np.ndarray
+    val: np.ndarray
+
+    @property
+    def nblocks(self) -> int:
+        return int(self.row[-1])
+
+    def dense(self) -> np.ndarray:
+        bs = self.bs
+        D = np.zeros((self.nbr * bs, self.nbr * bs))
+        for I in range(self.nbr):
+            for k in range(self.row[I], self.row[I + 1]):
+                J = self.col[k]
+                D[I * bs:(I + 1) * bs, J * bs:(J + 1) * bs] += self.val[k]
+        return D
+
+
+def bcsr_from_csr(A: CSR, bs: int) -> BCSR:
+    """CSR -> BCSR (control/magma_zmconverter.cpp:639-677 via
+    mkl_dcsrbsr, zero-based so blocks are row-major): a block is kept when
+    any stored entry falls in it; block columns ascending."""
+    nbr = (A.n_rows + bs - 1) // bs
+    row = [0]; col = []; blocks = []
+    for I in range(nbr):
+        cols = sorted({int(A.col[k]) // bs for i in range(I * bs, min(A.n_rows, (I + 1) * bs))
+                       for k in range(A.row[i], A.row[i + 1])})
+        for J in cols:
+            blk = np.zeros((bs, bs))
+            for i in range(I * bs, min(A.n_rows, (I + 1) * bs)):
+                for k in range(A.row[i], A.row[i + 1]):
+                    if A.col[k] // bs == J:
+                        blk[i - I * bs, A.col[k] - J * bs] += A.val[k]
+            col.append(J); blocks.append(blk)
+        row.append(len(col))
+    val = np.array(blocks) if blocks else np.zeros((0, bs, bs))
+    return BCSR(nbr, bs, np.array(row, np.int32), np.array(col, np.int32), val)
+
+
+def bcsr_split(B: BCSR, upper: bool, diag: str) -> BCSR:
+    """BCSRL / BCSRU (control/magma_zmconverter.cpp:1248-1436): strict
+    lower (upper) blocks and the diagonal block -- 'unit' identity,
+    'value' A's block, 'omit'.  Rows sorted, so BCSRL keeps the diagonal
+    block last and BCSRU first."""
+    row = [0]; col = []; blocks = []
+    for I in range(B.nbr):
+        for k in range(B.row[I], B.row[I + 1]):
+            J = int(B.col[k])
+            if (upper and J > I) or (not upper and J < I):
+                col.append(J); blocks.append(B.val[k].copy())
+            elif J == I and diag != "omit":
+                col.append(J)
+                blocks.append(np.eye(B.bs) if diag == "unit" else B.val[k].copy())
+        row.append(len(col))
+    return BCSR(B.nbr, B.bs, np.array(row, np.int32), np.array(col, np.int32), np.array(blocks))
+
+
+def bcsr_transpose(B: BCSR) -> BCSR:
+    """Block transpose by counting sort; blocks move, interiors stay."""
+    nb = B.nblocks
+    cnt = np.zeros(B.nbr + 1, np.int64)
+    for k in range(nb):
+        cnt[B.col[k] + 1] += 1
+    ptr = np.cumsum(cnt).astype(np.int32)
+    fill = ptr[:-1].copy()
+    col = np.zeros(nb, np.int32); val = np.zeros_like(B.val)
+    for I in range(B.nbr):
+        for k in range(B.row[I], B.row[I + 1]):
+            q = fill[B.col[k]]; fill[B.col[k]] += 1
+            col[q] = I; val[q] = B.val[k]
+    return BCSR(B.nbr, B.bs, ptr, col, val)
+
+
+def gen_bcsr7_pattern(nx: int, ny: int, nz: int):
+    """(row, col) of the 7-point block stencil, x fastest, Dirichlet,
+    block columns ascending."""
+    row = [0]; col = []
+    sy, sz = nx, nx * ny
+    for k in range(nz):
+        for j in range(ny):
+            for i in range(nx):
+                p = i + sy * j + sz * k
+                nb = []
+                if k > 0: nb.append(p - sz)
+                if j > 0: nb.append(p - sy)
+                if i > 0: nb.append(p - 1)
+                nb.append(p)
+                if i < nx - 1: nb.append(p + 1)
+                if j < ny - 1: nb.append(p + sy)
+                if k < nz - 1: nb.append(p + sz)
+                col.extend(nb); row.append(len(col))
+    return np.array(row, np.int32), np.array(col, np.int32)
+
+
+def bf16_round(x) -> np.ndarray:
+    """float32 -> bfloat16 (round to nearest even), returned as float32."""
+    f = np.ascontiguousarray(x, dtype=np.float32)
+    u = f.view(np.uint32).astype(np.uint64)
+    nan = (u & 0x7fffffff) > 0x7f800000
+    r = ((u + 0x7fff + ((u >> 16) & 1)) >> 16) & 0xffff
+    r = np.where(nan, (u >> 16) | 0x40, r).astype(np.uint32) << 16
+    return r.astype(np.uint32).view(np.float32)
+
+
+def row_scales(B: BCSR) -> np.ndarray:
+    """Power-of-two scale per scalar row: 2^e with 2^e <= max|row| < 2^(e+1)."""
+    bs = B.bs
+    rs = np.ones(B.nbr * bs, np.float32)
+    for I in range(B.nbr):
+        blk = B.val[B.row[I]:B.row[I + 1]]
+        if len(blk) == 0:
+            continue
+        m = np.abs(blk).max(axis=(0, 2))
+        for a in range(bs):
+            if m[a] > 0 and np.isfinite(m[a]):
+                _, e = np.frexp(m[a])
+                rs[I * bs + a] = np.float32(2.0) ** int(np.clip(e - 1, -126, 127))
+    return rs
+
+
+class BILUState:
+    """Block ParILU(0) of src/parilu_v0_3_bcsr.cpp:16-276: L as BCSRL
+    (identity diagonal block last), U as BCSCU (block columns, diagonal
+    block last, interiors as stored).  dtype float64 or float32 iterates
+    in that precision; storage 'bf16' / 'f16' keeps the factors rounded to
+    16 bits with a power-of-two scale per scalar row (FP32 iteration) and
+    runs Jacobi sweeps only."""
+
+    def __init__(self, B: BCSR, dtype=np.float64, storage=None):
+        self.B = B
+        self.bs = B.bs
+        self.dtype = np.float32 if storage else dtype
+        self.storage = storage
+        self.L = bcsr_split(B, False, "unit")
+        self.U = bcsr_transpose(bcsr_split(B, True, "value"))
+        self.Aval = np.ascontiguousarray(B.val.reshape(-1), dtype=np.float64)
+        self.Lval = self.L.val.reshape(-1).astype(self.dtype)
+        self.Uval = self.U.val.reshape(-1).astype(self.dtype)
+        if storage:
+            self.rs = row_scales(B)
+            self.Lval = self._q(self.Lval, self._lrows(), diag_unit=True)
+            self.Uval = self._q(self.Uval, self._urows())
+
+    # scalar row of each stored entry
+    def _lrows(self):
+        bs = self.bs
+        I = np.repeat(np.arange(self.B.nbr), np.diff(self.L.row))
+        return (I[:, None] * bs + np.arange(bs)[None, :]).repeat(bs, axis=1).reshape(-1)
+
+    def _urows(self):
+        bs = self.bs
+        K = self.U.col
+        return (K[:, None] * bs + np.arange(bs)[None, :]).repeat(bs, axis=1).reshape(-1)
+
+    def _q(self, v, rows, diag_unit=False):
+        s = self.rs[rows]
+        t = (v.astype(np.float32) * (np.float32(1.0) / s)).astype(np.float32)
+        q = bf16_round(t) if self.storage == "bf16" else t.astype(np.float16).astype(np.float32)
+        out = (q * s).astype(np.float32)
+        if diag_unit:  # the identity blocks are never stored or rounded
+            bs = self.bs
+            I = np.repeat(np.arange(self.B.nbr), np.diff(self.L.row))
+            d = (self.L.col == I).repeat(bs * bs)
+            out[d] = v[d]
+        return out
+
+    def sweep(self, mode: str = "jacobi") -> float:
+        L, U, B = self.L, self.U, self.B
+        fn = lib().orc_bilu_sweep_d if self.dtype == np.float64 else lib().orc_bilu_sweep_f
+        if mode == "jacobi":
+            Lout, Uout = self.Lval.copy(), self.Uval.copy()
+        elif mode == "gs" and not self.storage:
+            Lout, Uout = self.Lval, self.Uval
+        else:
+            raise ValueError(mode)
+        step = fn(B.nbr, B.bs, _ptr(_ci(B.row)), _ptr(_ci(B.col)), _ptr(self.Aval),
+                  _ptr(_ci(L.row)), _ptr(_ci(L.col)), _ptr(self.Lval), _ptr(Lout),
+                  _ptr(_ci(U.row)), _ptr(_ci(U.col)), _ptr(self.Uval), _ptr(Uout))
+        if self.storage:
+            Lout = self._q(Lout, self._lrows(), diag_unit=True)
+            Uout = self._q(Uout, self._urows())
+            step = float(np.sum((self.Lval.astype(np.float64) - Lout) ** 2) +
+                         np.sum((self.Uval.astype(np.float64) - Uout) ** 2))
+        self.Lval, self.Uval = Lout, Uout
+        return step
+
+    def factors(self):
+        """(L as BCSRL, U as BCSCU) with the current values (float64)."""
+        bs = self.bs
+        Lf = BCSR(self.B.nbr, bs, self.L.row, self.L.col, self.Lval.astype(np.float64).reshape(-1, bs, bs))
+        Uf = BCSR(self.B.nbr, bs, self.U.row, self.U.col, self.Uval.astype(np.float64).reshape(-1, bs, bs))
+        return Lf, Uf
+
+
+def bilu(B: BCSR, sweeps: int, mode: str = "jacobi", dtype=np.float64, storage=None):
+    st = BILUState(B, dtype, storage)
+    for _ in range(sweeps):
+        st.sweep(mode)
+    return st
+
+
+def binv(M) -> np.ndarray:
+    M = np.ascontiguousarray(M, dtype=np.float64)
+    out = np.zeros_like(M)
+    lib().orc_binv_double(M.shape[0], _ptr(M), _ptr(out))
+    return out
+
+
+def bcsr_spmv(B: BCSR, x) -> np.ndarray:
+    x = _cd(x); y = np.zeros(B.nbr * B.bs)
+    lib().orc_bcsr_spmv(B.nbr, B.bs, _ptr(_ci(B.row)), _ptr(_ci(B.col)),
+                        _ptr(_cd(B.val.reshape(-1))), _ptr(x), _ptr(y))
+    return y
+
+
+def btrsv(T: BCSR, r, upper: bool, jac: int = 0) -> np.ndarray:
+    """FP32 block triangular solve: T lower with the diagonal block last
+    (BCSRL) or upper with it first (BCSRU); jac > 0 runs that many Jacobi
+    sweeps from zero instead of exact substitution."""
+    r = np.ascontiguousarray(r, dtype=np.float32)
+    y = np.zeros(T.nbr * T.bs, np.float32)
+    lib().orc_btrsv(T.nbr, T.bs, _ptr(_ci(T.row)), _ptr(_ci(T.col)),
+                    _ptr(np.ascontiguousarray(T.val.reshape(-1), dtype=np.float32)),
+                    1 if upper else 0, jac, _ptr(r), _ptr(y))
+    return y

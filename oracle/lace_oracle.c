@@ -859,3 +859,165 @@ ORC_API void orc_set_threads(int n) {
   (void)n;
 #endif
 }
+
+/* ------------------------------------------------------------------ */
+/* Block (BCSR) routines.  Blocks are bs x bs, row-major, ldblock = bs*bs
+ * (the layout the reference's block ParILU multiplies with row-major
+ * dgemm, src/parilu_v0_3_bcsr.cpp:167-171).  Block inverse: Gauss-Jordan
+ * without pivoting on [M | I] in a fixed order (the reference iterates
+ * dense ParLU + triangular sweeps to 1e-15, src/inverse.cpp:17-160; the
+ * fixed elimination is the exact inverse that iteration approximates). */
+#define ORC_MAXBS 8
+
+#define ORC_BLOCK_FUNCS(T, SUF)                                               \
+  static void orc_binv_##SUF(int bs, const T *M, T *Inv) {                    \
+    T a[ORC_MAXBS * ORC_MAXBS];                                               \
+    for (int e = 0; e < bs * bs; e++) {                                       \
+      a[e] = M[e];                                                            \
+      Inv[e] = (T)0;                                                          \
+    }                                                                         \
+    for (int r = 0; r < bs; r++) Inv[r * bs + r] = (T)1;                      \
+    for (int p = 0; p < bs; p++) {                                            \
+      const T inv = (T)1 / a[p * bs + p];                                     \
+      for (int c = 0; c < bs; c++) {                                          \
+        a[p * bs + c] = a[p * bs + c] * inv;                                  \
+        Inv[p * bs + c] = Inv[p * bs + c] * inv;                              \
+      }                                                                       \
+      for (int r = 0; r < bs; r++) {                                          \
+        if (r == p) continue;                                                 \
+        const T f = a[r * bs + p];                                            \
+        for (int c = 0; c < bs; c++) {                                        \
+          a[r * bs + c] = a[r * bs + c] - f * a[p * bs + c];                  \
+          Inv[r * bs + c] = Inv[r * bs + c] - f * Inv[p * bs + c];            \
+        }                                                                     \
+      }                                                                       \
+    }                                                                         \
+  }                                                                           \
+  /* C = A * B, each entry summed over m in ascending order from zero */      \
+  static void orc_bmul_##SUF(int bs, const T *A, const T *B, T *Cm) {         \
+    for (int r = 0; r < bs; r++)                                              \
+      for (int c = 0; c < bs; c++) {                                          \
+        T acc = (T)0;                                                         \
+        for (int m = 0; m < bs; m++) acc = acc + A[r * bs + m] * B[m * bs + c]; \
+        Cm[r * bs + c] = acc;                                                 \
+      }                                                                       \
+  }
+
+ORC_BLOCK_FUNCS(double, d)
+ORC_BLOCK_FUNCS(float, f)
+
+/* One synchronous block ParILU sweep in the reference's form
+ * (src/parilu_v0_3_bcsr.cpp:118-250): for each block (i,j) of A in BCSR
+ * order, S = A_ij; merge L block row i (BCSRL, identity diagonal block
+ * last) with U block column j (BCSCU, diagonal block last); every match
+ * subtracts SP = L_ik U_kj; the last SP is added back; i > j: L_ij =
+ * S inv(U_jj), else U_ij = S.  Reads (Lin, Uin), writes (Lout, Uout).
+ * Returns sum of squared changes (no ||A|| scaling, :61). */
+#define ORC_BSWEEP_BODY(T, SUF)                                               \
+  const int lb = bs * bs;                                                     \
+  T s[ORC_MAXBS * ORC_MAXBS], sp[ORC_MAXBS * ORC_MAXBS];                      \
+  T inv[ORC_MAXBS * ORC_MAXBS], t[ORC_MAXBS * ORC_MAXBS];                     \
+  double step = 0.0;                                                          \
+  for (int i = 0; i < nbr; i++) {                                             \
+    for (int k = arow[i]; k < arow[i + 1]; k++) {                             \
+      const int j = acol[k];                                                  \
+      for (int e = 0; e < lb; e++) s[e] = (T)aval[(size_t)k * lb + e];        \
+      for (int e = 0; e < lb; e++) sp[e] = (T)0;                              \
+      int il = lrow[i], iu = ucolptr[j];                                      \
+      while (il < lrow[i + 1] && iu < ucolptr[j + 1]) {                       \
+        for (int e = 0; e < lb; e++) sp[e] = (T)0;                            \
+        const int jl = lcol[il], ju = urowind[iu];                            \
+        if (jl == ju) {                                                       \
+          orc_bmul_##SUF(bs, &Lin[(size_t)il * lb], &Uin[(size_t)iu * lb], sp); \
+          for (int e = 0; e < lb; e++) s[e] = s[e] - sp[e];                   \
+        }                                                                     \
+        il = (jl <= ju) ? il + 1 : il;                                        \
+        iu = (jl >= ju) ? iu + 1 : iu;                                        \
+      }                                                                       \
+      for (int e = 0; e < lb; e++) s[e] = s[e] + sp[e];                       \
+      if (i > j) {                                                            \
+        orc_binv_##SUF(bs, &Uin[(size_t)(ucolptr[j + 1] - 1) * lb], inv);     \
+        orc_bmul_##SUF(bs, s, inv, t);                                        \
+        for (int e = 0; e < lb; e++) {                                        \
+          const double d = (double)Lin[(size_t)(il - 1) * lb + e] - (double)t[e]; \
+          step += d * d;                                                      \
+          Lout[(size_t)(il - 1) * lb + e] = t[e];                             \
+        }                                                                     \
+      } else {                                                                \
+        for (int e = 0; e < lb; e++) {                                        \
+          const double d = (double)Uin[(size_t)(iu - 1) * lb + e] - (double)s[e]; \
+          step += d * d;                                                      \
+          Uout[(size_t)(iu - 1) * lb + e] = s[e];                             \
+        }                                                                     \
+      }                                                                       \
+    }                                                                         \
+  }                                                                           \
+  return step;
+
+ORC_API double orc_bilu_sweep_d(int nbr, int bs, const int *arow, const int *acol,
+                                const double *aval, const int *lrow, const int *lcol,
+                                const double *Lin, double *Lout, const int *ucolptr,
+                                const int *urowind, const double *Uin, double *Uout) {
+  ORC_BSWEEP_BODY(double, d)
+}
+
+ORC_API double orc_bilu_sweep_f(int nbr, int bs, const int *arow, const int *acol,
+                                const double *aval, const int *lrow, const int *lcol,
+                                const float *Lin, float *Lout, const int *ucolptr,
+                                const int *urowind, const float *Uin, float *Uout) {
+  ORC_BSWEEP_BODY(float, f)
+}
+
+ORC_API void orc_binv_double(int bs, const double *M, double *Inv) { orc_binv_d(bs, M, Inv); }
+
+/* y = A x for BCSR A (bs x bs row-major blocks); y_{I,a} sums blocks in
+ * order, within a block over c ascending. */
+ORC_API void orc_bcsr_spmv(int nbr, int bs, const int *row, const int *col,
+                           const double *val, const double *x, double *y) {
+  const int lb = bs * bs;
+  for (int I = 0; I < nbr; I++)
+    for (int a = 0; a < bs; a++) {
+      double t = 0.0;
+      for (int k = row[I]; k < row[I + 1]; k++)
+        for (int c = 0; c < bs; c++) t += val[(size_t)k * lb + a * bs + c] * x[(size_t)col[k] * bs + c];
+      y[(size_t)I * bs + a] = t;
+    }
+}
+
+/* Block triangular solves on L (BCSRL: diagonal block last in each block
+ * row) or U in block-CSR form with the diagonal block first.  Exact:
+ * t = r_I - sum_K T_IK y_K (blocks in order, entries c ascending), then
+ * y_I = inv(T_II) t (Gauss-Jordan inverse above); jac > 0: that many
+ * synchronous Jacobi sweeps from y = 0 instead. */
+ORC_API void orc_btrsv(int nbr, int bs, const int *row, const int *col, const float *val,
+                       int upper, int jac, const float *r, float *y) {
+  const int lb = bs * bs;
+  float inv[ORC_MAXBS * ORC_MAXBS], tt[ORC_MAXBS];
+  float *prev = (float *)calloc((size_t)nbr * bs, sizeof(float));
+  const int sweeps = jac > 0 ? jac : 1;
+  for (int sw = 0; sw < sweeps; sw++) {
+    for (int ii = 0; ii < nbr; ii++) {
+      const int I = (upper && jac <= 0) ? nbr - 1 - ii : ii;
+      const int kd = upper ? row[I] : row[I + 1] - 1;
+      const int k0 = upper ? row[I] + 1 : row[I];
+      const int k1 = upper ? row[I + 1] : row[I + 1] - 1;
+      const float *yy = jac > 0 ? prev : y;
+      for (int a = 0; a < bs; a++) {
+        float t = 0.0f;
+        if (!(jac > 0 && sw == 0))
+          for (int k = k0; k < k1; k++)
+            for (int c = 0; c < bs; c++)
+              t = t + val[(size_t)k * lb + a * bs + c] * yy[(size_t)col[k] * bs + c];
+        tt[a] = r[(size_t)I * bs + a] - t;
+      }
+      orc_binv_f(bs, &val[(size_t)kd * lb], inv);
+      for (int a = 0; a < bs; a++) {
+        float acc = 0.0f;
+        for (int c = 0; c < bs; c++) acc = acc + inv[a * bs + c] * tt[c];
+        y[(size_t)I * bs + a] = acc;
+      }
+    }
+    if (jac > 0) memcpy(prev, y, sizeof(float) * (size_t)nbr * bs);
+  }
+  free(prev);
+}

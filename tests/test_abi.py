@@ -247,3 +247,82 @@ def test_device_calls_fail_cleanly_without_gpu(lace):
     p = C.c_void_p()
     rc = lace.lib().lace_dmalloc(C.byref(p), 16)
     assert rc != 0 and b"device" in lace.lib().lace_last_error()
+
+
+# ------------------------------------------------------------ block formats
+def _bcsr_equal(got, ref):
+    row, col, val = got[:3]
+    return (np.array_equal(row, ref.row) and np.array_equal(col, ref.col)
+            and np.array_equal(val.reshape(ref.val.shape), ref.val))
+
+
+def test_gen_bcsr7_matches_oracle_pattern(lace, orc):
+    """7-point block pattern (Dirichlet, x fastest, columns ascending); the
+    values are sigma * gen_normal(seed) in storage order plus the shift on
+    the diagonal entries of the diagonal blocks."""
+    nx, ny, nz, bs = 5, 4, 3, 5
+    row, col, val = lace.gen_bcsr7(nx, ny, nz, bs, seed=1, sigma=0.1, shift=5.0)
+    prow, pcol = orc.gen_bcsr7_pattern(nx, ny, nz)
+    assert np.array_equal(row, prow) and np.array_equal(col, pcol)
+    want = 0.1 * lace.gen_normal(1, len(col) * bs * bs).reshape(-1, bs, bs)
+    for I in range(len(row) - 1):
+        for k in range(row[I], row[I + 1]):
+            if col[k] == I:
+                want[k] += 5.0 * np.eye(bs)
+    assert np.array_equal(val, want)
+
+
+def test_config3_block_counts(lace, orc):
+    """64^3 7-point block grid: 1,810,432 blocks (Dirichlet), 5x5 blocks."""
+    prow, _ = orc.gen_bcsr7_pattern(64, 64, 64)
+    assert prow[-1] == 1810432 and len(prow) - 1 == 262144
+
+
+def _to_bcsr(lace, A, bs):
+    H = lace.HostCSR(A.row, A.col, A.val, A.n_cols)
+    B = lace.DMatrix()
+    B.blocksize = bs
+    lace.check(lace.lib().data_zmconvert(H.m, C.byref(B), lace.Magma_CSR, lace.Magma_BCSR))
+    return lace.owned_bcsr_to_numpy(B)
+
+
+@pytest.mark.parametrize("name,bs", [("bcsr_test.mtx", 5), ("io_test.mtx", 2), ("Trefethen_20.mtx", 3)])
+def test_csr_to_bcsr_matches_oracle(lace, orc, name, bs):
+    A = orc.read_mtx(os.path.join(GOLDEN, name))
+    got = _to_bcsr(lace, A, bs)
+    assert got[3]["storage"] == lace.Magma_BCSR and got[3]["bs"] == bs
+    assert _bcsr_equal(got, orc.bcsr_from_csr(A, bs))
+
+
+def test_csr_to_bcsr_needs_blocksize(lace, orc):
+    A = orc.read_mtx(os.path.join(GOLDEN, "io_test.mtx"))
+    H = lace.HostCSR(A.row, A.col, A.val, A.n_cols)
+    B = lace.DMatrix()
+    rc = lace.lib().data_zmconvert(H.m, C.byref(B), lace.Magma_CSR, lace.Magma_BCSR)
+    assert rc != 0 and b"blocksize" in lace.lib().lace_last_error()
+
+
+def test_bcsr_splits_and_block_transpose(lace, orc):
+    """BCSR -> BCSRL (unit) / BCSRU / BCSCU and data_zmtranspose on a BCSR
+    matrix against the oracle (blocks move, interiors stay)."""
+    A = orc.read_mtx(os.path.join(GOLDEN, "bcsr_test.mtx"))
+    Bo = orc.bcsr_from_csr(A, 5)
+    H = lace.HostBCSR(Bo.row, Bo.col, Bo.val, 5)
+
+    def conv(new, diag):
+        M = lace.DMatrix()
+        M.diagorder_type = diag
+        lace.check(lace.lib().data_zmconvert(H.m, C.byref(M), lace.Magma_BCSR, new))
+        return lace.owned_bcsr_to_numpy(M)
+
+    L = conv(lace.Magma_BCSRL, lace.Magma_UNITY)
+    assert L[3]["storage"] == lace.Magma_BCSR  # the reference's tag for BCSRL
+    assert _bcsr_equal(L, orc.bcsr_split(Bo, False, "unit"))
+    U = conv(lace.Magma_BCSRU, lace.Magma_VALUE)
+    assert _bcsr_equal(U, orc.bcsr_split(Bo, True, "value"))
+    Uc = conv(lace.Magma_BCSCU, lace.Magma_VALUE)
+    assert Uc[3]["storage"] == lace.Magma_BCSCU and Uc[3]["major"] == lace.MagmaColMajor
+    assert _bcsr_equal(Uc, orc.bcsr_transpose(orc.bcsr_split(Bo, True, "value")))
+    T = lace.DMatrix()
+    lace.check(lace.lib().data_zmtranspose(H.m, C.byref(T)))
+    assert _bcsr_equal(lace.owned_bcsr_to_numpy(T), orc.bcsr_transpose(Bo))
