@@ -94,6 +94,82 @@ def pmc_traffic():
         return json.load(f).get("bytes_per_launch")
 
 
+def _region_rate(L, name, nbytes, fn, reps):
+    """Average duration of the named region over reps calls of fn (HIP
+    events on the library stream) and the rate nbytes / duration."""
+    lib = L.lib()
+    L.check(lib.lace_timing_reset())
+    L.check(lib.lace_timing_only(name.encode()))
+    L.check(lib.lace_timing_enable(1))
+    for _ in range(reps):
+        fn()
+    L.check(lib.lace_device_synchronize())
+    L.check(lib.lace_timing_enable(0))
+    L.check(lib.lace_timing_only(None))
+    ms, cnt = L.timing(name)
+    avg = ms / cnt * 1e-3
+    return avg, nbytes / avg / 1e9
+
+
+def bench_bcsr(args) -> int:
+    """Config 3: BCSR 5x5 blocks on the 64^3 7-point block grid
+    (n = 1,310,720; 1,810,432 blocks = 45,260,800 entries); entries
+    0.1 N(0,1) (seed 1), diagonal blocks + 5 I, b ~ N(0,1) (seed 2).
+    FP64 FGMRES(40) to 1e-8 with block ParILU(0) factors (3 sweeps) stored
+    in FP32 or BF16 (FP32 iteration, Jacobi-5 block solves); the error of
+    each solution against an FP64 solve to 1e-13 is reported."""
+    import lace_amd as L
+    lib = L.lib()
+    L.check(lib.lace_init(0), "lace_init")
+    N = args.size if args.size != 128 else 64
+    row, col, val = L.gen_bcsr7(N, N, N, 5, seed=1, sigma=0.1, shift=5.0)
+    H = L.HostBCSR(row, col, val, 5)
+    n = H.n
+    b = L.gen_normal(2, n)
+    db = L.DevVec(n, np.float64, b)
+    dx = L.DevVec(n, np.float64)
+    zeros = np.zeros(n)
+    out = {"mode": "bcsr", "workload": f"config 3: BCSR 5x5, 7-point block grid {N}^3",
+           "n": n, "blocks": int(row[-1])}
+    # sweeps and SpMV bandwidth
+    for name, prec in (("f32", L.LACE_F32), ("bf16", L.LACE_BF16), ("f64", L.LACE_F64)):
+        P = L.BILU(H, prec)
+        P.sweeps(2)
+        avg, gbs = _region_rate(L, "bilu_sweep", P.sweep_bytes(), lambda: P.sweeps(1), 10)
+        out[f"bilu_sweep_{name}"] = {"us": round(avg * 1e6, 2), "GBps": round(gbs, 1),
+                                     "bytes": P.sweep_bytes()}
+        del P
+    A = L.DevBCSR(H, L.LACE_F64)
+    x = L.DevVec(n, np.float64, b)
+    y = L.DevVec(n, np.float64)
+    avg, gbs = _region_rate(L, "bcsr_spmv", A.bytes(),
+                            lambda: L.check(lib.lace_bcsr_spmv(A.h, x.ptr, y.ptr)), 20)
+    out["bcsr_spmv_f64"] = {"us": round(avg * 1e6, 2), "GBps": round(gbs, 1), "bytes": A.bytes()}
+
+    def solver(prec, rtol, m=40):
+        o = make_opts(L, search_max=m, trsv=L.TRSV_JACOBI, jacobi_sweeps=5, prec_precond=prec)
+        o.rtol = rtol
+        return L.Solver(H, o)
+
+    ref = solver(L.LACE_F64, 1e-13)
+    timed_solves(L, ref, db, dx, zeros, 1)
+    xref = dx.download()
+    for name, prec in (("f32", L.LACE_F32), ("bf16", L.LACE_BF16), ("f64", L.LACE_F64)):
+        S = solver(prec, 1e-8)
+        timed_solves(L, S, db, dx, zeros, args.warmup)
+        res = timed_solves(L, S, db, dx, zeros, args.steps)
+        t = float(np.mean([r[0] for r in res]))
+        st = res[-1][1]
+        xs = dx.download()
+        out[f"solve_{name}"] = {
+            "s": round(t, 5), "iterations": st.iterations, "converged": st.converged,
+            "rel_true_residual": st.rel_true_residual,
+            "err_vs_fp64_solution": float(np.linalg.norm(xs - xref) / np.linalg.norm(xref))}
+        del S
+    print(json.dumps(out))
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -102,15 +178,18 @@ def main() -> int:
     ap.add_argument("--size", type=int, default=128)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
-    ap.add_argument("--mode", choices=["solve", "sweeps"], default="solve",
+    ap.add_argument("--mode", choices=["solve", "sweeps", "bcsr"], default="solve",
                     help="sweeps: only FP32 ParILU sweeps plus one 256 MiB copy kernel "
-                         "(the workload profiled for HBM traffic counters)")
+                         "(the workload profiled for HBM traffic counters); bcsr: the "
+                         "block-sparse 5x5 workload on the 64^3 block grid")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: --gpus {args.gpus} is not supported; this implementation runs on one "
               "GPU (multi-GPU sharding of the path is described in DESIGN.md, not built)",
               file=sys.stderr)
         return 2
+    if args.mode == "bcsr":
+        return bench_bcsr(args)
 
     import lace_amd as L
     lib = L.lib()

@@ -116,31 +116,34 @@ __global__ __launch_bounds__(kBS) void mdot_kernel(int n, int ld, const K* __res
 }
 
 // Sum of the per-block partials in a fixed order: out[k] for k < cnt and,
-// if self, out[cnt] = the self slot.  One block of 256 threads: 4 segments
-// per column, each summed with 4 independent accumulators so the loads
-// overlap, then combined in a fixed order.
+// if self, out[cnt] = the self slot.  One block of 256 threads.  Thread t
+// owns the partial rows t, t+256, ... and accumulates 16 columns at a time,
+// so all of its loads for a chunk are independent and in flight together;
+// the 16 per-thread sums then meet in a butterfly per wave and a fixed
+// 4-wave combination.
 __device__ void finalize_partials(int nb, int cnt, bool self, int pstride,
                                   const double* __restrict__ partial, double* out) {
-  __shared__ double seg[4][72];
+  constexpr int CH = 16;
+  __shared__ double seg[kBS / 64][CH];
   const int tot = cnt + (self ? 1 : 0);
-  for (int base = 0; base < tot; base += 64) {
-    const int c = base + (threadIdx.x & 63), s = threadIdx.x >> 6;
-    double v0 = 0.0, v1 = 0.0, v2 = 0.0, v3 = 0.0;
-    if (c < tot) {
-      const int slot = (self && c == cnt) ? pstride - 1 : c;
-      int bb = s;
-      for (; bb + 12 < nb; bb += 16) {
-        v0 += partial[(size_t)bb * pstride + slot];
-        v1 += partial[(size_t)(bb + 4) * pstride + slot];
-        v2 += partial[(size_t)(bb + 8) * pstride + slot];
-        v3 += partial[(size_t)(bb + 12) * pstride + slot];
+  const int wv = threadIdx.x >> 6;
+  for (int base = 0; base < tot; base += CH) {
+    double acc[CH];
+#pragma unroll
+    for (int k = 0; k < CH; k++) acc[k] = 0.0;
+    for (int bb = threadIdx.x; bb < nb; bb += kBS) {
+      const double* pr = partial + (size_t)bb * pstride;
+#pragma unroll
+      for (int k = 0; k < CH; k++) {
+        const int c = base + k;
+        if (c < tot) acc[k] += pr[(self && c == cnt) ? pstride - 1 : c];
       }
-      for (; bb < nb; bb += 4) v0 += partial[(size_t)bb * pstride + slot];
     }
-    seg[s][threadIdx.x & 63] = (v0 + v1) + (v2 + v3);
+    wave_multi_sum<CH>(acc, seg[wv]);
     __syncthreads();
-    if (threadIdx.x < 64 && c < tot)
-      out[c] = seg[0][threadIdx.x] + seg[1][threadIdx.x] + seg[2][threadIdx.x] + seg[3][threadIdx.x];
+    if (threadIdx.x < CH && base + (int)threadIdx.x < tot)
+      out[base + threadIdx.x] = (seg[0][threadIdx.x] + seg[1][threadIdx.x]) +
+                                (seg[2][threadIdx.x] + seg[3][threadIdx.x]);
     __syncthreads();
   }
 }
@@ -227,6 +230,55 @@ __global__ __launch_bounds__(kBS) void hh_update_kernel(int n, int ld, const K* 
   for (int k = 0; k < NC; k++)
     if (k < ncols) v -= U[(size_t)k * ld + i] * (K)d[k];
   w[i] = v;
+}
+
+// w <- w - U_j d and, from the updated rows >= rowstart, the tail
+// multi-dot U_j^T w[rowstart:] and ||w[rowstart:]||^2 as per-block partials
+// (the slabs of mdot_kernel): one pass over U_j instead of two.  Each
+// thread keeps its row's NC reflector entries in registers for both uses.
+template <typename K, int NC>
+__global__ __launch_bounds__(kBS) void hh_update_mdot_kernel(int n, int ld, const K* __restrict__ U,
+                                                             int ncols, K* __restrict__ w, int rowstart,
+                                                             int rpb, const double* __restrict__ small,
+                                                             SmallLayout lay, double* __restrict__ partial,
+                                                             int pstride) {
+  const double* misc = small + lay.misc();
+  if (is_done(misc)) return;
+  const int b = xcd_block(blockIdx.x, gridDim.x);
+  const int r0 = b * rpb, r1 = min(r0 + rpb, n);
+  const double* d = small + lay.d();
+  K dk[NC], acc[NC];
+#pragma unroll
+  for (int k = 0; k < NC; k++) {
+    dk[k] = k < ncols ? (K)d[k] : K(0);
+    acc[k] = K(0);
+  }
+  K self = K(0);
+  for (int i = r0 + threadIdx.x; i < r1; i += kBS) {
+    K u[NC];
+#pragma unroll
+    for (int k = 0; k < NC; k++) u[k] = k < ncols ? U[(size_t)k * ld + i] : K(0);
+    K v = w[i];
+#pragma unroll
+    for (int k = 0; k < NC; k++) v -= u[k] * dk[k];
+    w[i] = v;
+    if (i >= rowstart) {
+#pragma unroll
+      for (int k = 0; k < NC; k++) acc[k] += u[k] * v;
+      self += v * v;
+    }
+  }
+  __shared__ double red[kBS / 64][NC + 1];
+  const int wv = threadIdx.x >> 6;
+  wave_multi_sum<NC>(acc, red[wv]);
+  const double sv = wave_sum((double)self);
+  if ((threadIdx.x & 63) == 0) red[wv][NC] = sv;
+  __syncthreads();
+  for (int k = threadIdx.x; k <= NC; k += kBS) {
+    const double t = (red[0][k] + red[1][k]) + (red[2][k] + red[3][k]);
+    if (k < ncols) partial[(size_t)b * pstride + k] = t;
+    if (k == NC) partial[(size_t)b * pstride + pstride - 1] = t;
+  }
 }
 
 // After the tail multi-dot: s = U_j^T w[j+1:], nu = ||w[j+1:]||^2.
@@ -491,10 +543,24 @@ int hh_step(HHEngine<K>& E, int j, bool write_q) {
   LACE_NC_DISPATCH(nc, (mdot_kernel<K, NCB, false><<<E.nb, kBS, 0, stream()>>>(
                            E.n, E.ld, U, nc, w, 0, E.rpb, part, E.pstride, misc)));
   hh_dT_kernel<<<1, 256, 0, stream()>>>(E.nb, E.pstride, part, j, small, E.lay);
-  LACE_NC_DISPATCH(nc, (hh_update_kernel<K, NCB><<<nbv, kBS, 0, stream()>>>(E.n, E.ld, U, nc, small,
-                                                                            E.lay, w)));
-  LACE_NC_DISPATCH(nc, (mdot_kernel<K, NCB, true><<<E.nb, kBS, 0, stream()>>>(
-                           E.n, E.ld, U, nc, w, j + 1, E.rpb, part, E.pstride, misc)));
+  // fused update + tail multi-dot while the per-thread reflector row fits
+  // in registers (up to 16 columns; 32 in FP32)
+  if (nc <= 16 || (sizeof(K) == 4 && nc <= 32)) {
+    if (nc <= 8)
+      hh_update_mdot_kernel<K, 8><<<E.nb, kBS, 0, stream()>>>(E.n, E.ld, U, nc, w, j + 1, E.rpb,
+                                                             small, E.lay, part, E.pstride);
+    else if (nc <= 16)
+      hh_update_mdot_kernel<K, 16><<<E.nb, kBS, 0, stream()>>>(E.n, E.ld, U, nc, w, j + 1, E.rpb,
+                                                              small, E.lay, part, E.pstride);
+    else
+      hh_update_mdot_kernel<K, 32><<<E.nb, kBS, 0, stream()>>>(E.n, E.ld, U, nc, w, j + 1, E.rpb,
+                                                              small, E.lay, part, E.pstride);
+  } else {
+    LACE_NC_DISPATCH(nc, (hh_update_kernel<K, NCB><<<nbv, kBS, 0, stream()>>>(E.n, E.ld, U, nc, small,
+                                                                              E.lay, w)));
+    LACE_NC_DISPATCH(nc, (mdot_kernel<K, NCB, true><<<E.nb, kBS, 0, stream()>>>(
+                             E.n, E.ld, U, nc, w, j + 1, E.rpb, part, E.pstride, misc)));
+  }
   hh_newcol_small_kernel<K><<<1, 256, 0, stream()>>>(E.nb, E.pstride, part, j, E.n, E.ld, U, w,
                                                      small, E.lay);
   LACE_NC_DISPATCH(nc, (hh_newcol_vec_kernel<K, NCB><<<nbv, kBS, 0, stream()>>>(

@@ -86,6 +86,71 @@ __global__ __launch_bounds__(kSweepBS) void parilu_sweep_kernel(
   }
 }
 
+// Symmetric-pattern sweep as a stream over factor positions: thread p reads
+// a_ij, a_ji, l_ij, u_ji at its own position (consecutive threads,
+// consecutive addresses) and the merge partners recorded at set-up -- u_jj
+// for a strict position; the earlier positions of its own row for a
+// diagonal one (u_ii = a_ii - sum_k l_ik u_ki); and, when the pattern has
+// any, the strict position's matches k < j.  Same arithmetic and order as
+// the per-position kernel above.
+template <typename T, bool STEP, bool MATCH>
+__global__ __launch_bounds__(kSweepBS) void parilu_stream_sweep_kernel(
+    int np, const int* __restrict__ pmeta, const int* __restrict__ mptr,
+    const int* __restrict__ mpairs, const T* __restrict__ AL, const T* __restrict__ AU,
+    const T* __restrict__ VLin, const T* __restrict__ VUin, T* __restrict__ VLout,
+    T* __restrict__ VUout, double* __restrict__ partial) {
+#pragma clang fp contract(off)
+  const int b = xcd_block(blockIdx.x, gridDim.x);
+  const int p = b * kSweepBS + threadIdx.x;
+  double d2 = 0.0;
+  if (p < np) {
+    const int m = pmeta[p];
+    const T uold = VUin[p];
+    const T pu = T(1) * uold;  // l_jj * u_ji (unit diagonal)
+    if (m < 0) {
+      // diagonal: u_ii over the row's strict positions [i0, p)
+      T s = AU[p];
+      for (int q = -m - 1; q < p; q++) s = s - VLin[q] * VUin[q];
+      s = s - pu;
+      s = s + pu;
+      VUout[p] = s;
+      if (STEP) {
+        const double d = (double)uold - (double)s;
+        d2 = d * d;
+      }
+    } else {
+      T sL = AL[p], sU = AU[p];
+      const T lold = VLin[p];
+      const T ujj = VUin[m];
+      if (MATCH) {
+        for (int t = mptr[p]; t < mptr[p + 1]; t++) {
+          const int qa = mpairs[2 * t], qc = mpairs[2 * t + 1];
+          sL = sL - VLin[qa] * VUin[qc];
+          sU = sU - VLin[qc] * VUin[qa];
+        }
+      }
+      sU = sU - pu;
+      sU = sU + pu;
+      VUout[p] = sU;
+      const T pl = lold * ujj;
+      sL = sL - pl;
+      sL = sL + pl;
+      const T t = sL / ujj;
+      VLout[p] = t;
+      if (STEP) {
+        const double d1 = (double)uold - (double)sU;
+        const double d = (double)lold - (double)t;
+        d2 = d1 * d1 + d * d;
+      }
+    }
+  }
+  if (STEP) {
+    __shared__ double sm[kSweepBS / 64];
+    double tot = block_sum<kSweepBS>(d2, sm);
+    if (threadIdx.x == 0) partial[b] = tot;
+  }
+}
+
 // Sum of per-block partials in a fixed order (one block).
 __global__ __launch_bounds__(256) void sum_partials_kernel(int nb, const double* __restrict__ partial,
                                                            double scale, double* __restrict__ out) {
@@ -431,6 +496,42 @@ static int build_pattern(const data_d_matrix* A, ParILU* P) {
         }
       }
     LACE_TRY(upload(P->prow, prow.data(), sizeof(int) * P->nnzL));
+    // static merge data for the streaming kernel: per position the index of
+    // u_jj (strict) or -(row start + 1) (diagonal); the strict positions'
+    // own matches k < j in rows i and j, ascending k, as (q_i, q_j) pairs
+    {
+      std::vector<int> meta(P->nnzL), mptr(P->nnzL + 1, 0), pairs;
+      for (int i = 0; i < n; i++) {
+        const int i0 = P->h_lrow[i], i1 = P->h_lrow[i + 1] - 1;
+        for (int p = i0; p <= i1; p++) {
+          mptr[p] = (int)(pairs.size() / 2);
+          if (p == i1) {
+            meta[p] = -(i0 + 1);
+            continue;
+          }
+          const int j = P->h_lcol[p];
+          const int j0 = P->h_lrow[j], j1 = P->h_lrow[j + 1] - 1;
+          meta[p] = j1;
+          int a = i0, c = j0;
+          while (a < p && c < j1) {
+            const int ka = P->h_lcol[a], kc = P->h_lcol[c];
+            if (ka == kc) {
+              pairs.push_back(a);
+              pairs.push_back(c);
+            }
+            a += ka <= kc;
+            c += ka >= kc;
+          }
+        }
+      }
+      mptr[P->nnzL] = (int)(pairs.size() / 2);
+      LACE_TRY(upload(P->pmeta, meta.data(), sizeof(int) * P->nnzL));
+      P->strict_matches = !pairs.empty();
+      if (P->strict_matches) {
+        LACE_TRY(upload(P->mptr, mptr.data(), sizeof(int) * (P->nnzL + 1)));
+        LACE_TRY(upload(P->mpairs, pairs.data(), sizeof(int) * pairs.size()));
+      }
+    }
     const size_t esz = P->elem_size();
     if (P->prec == LACE_F64) {
       LACE_TRY(upload(P->AL, al.data(), sizeof(double) * P->nnzL));
@@ -512,10 +613,30 @@ template <typename T>
 static int sweep_t(ParILU* P, bool with_step) {
   const int in = P->cur, out = 1 - P->cur;
   TimedRegion tr("parilu_sweep");
-  // per-row kernel for FP32 (measured faster on the 128^3 stencil); FP64,
-  // whose divisions are long sequences, keeps the per-position kernel's
-  // wider parallelism
-  if (P->sym && P->maxl <= 4 && (P->variant == 3 || (P->variant == 0 && sizeof(T) == 4))) {
+  // symmetric patterns default to the streaming kernel
+  if (P->sym && (P->variant == 0 || P->variant == 4)) {
+    const int nb = ceil_div(P->nnzL, kSweepBS);
+    auto go = [&](auto kern) {
+      kern<<<nb, kSweepBS, 0, stream()>>>(P->nnzL, P->pmeta.as<int>(), P->mptr.as<int>(),
+                                          P->mpairs.as<int>(), P->AL.as<T>(), P->AU.as<T>(),
+                                          P->lval[in].as<T>(), P->uval[in].as<T>(),
+                                          P->lval[out].as<T>(), P->uval[out].as<T>(),
+                                          P->partial.as<double>());
+    };
+    if (P->strict_matches) {
+      if (with_step) go(parilu_stream_sweep_kernel<T, true, true>);
+      else go(parilu_stream_sweep_kernel<T, false, true>);
+    } else {
+      if (with_step) go(parilu_stream_sweep_kernel<T, true, false>);
+      else go(parilu_stream_sweep_kernel<T, false, false>);
+    }
+    LACE_HIP(hipGetLastError());
+    P->last_nb = nb;
+    P->cur = out;
+    P->csr_dirty = true;
+    return DEV_SUCCESS;
+  }
+  if (P->sym && P->maxl <= 4 && P->variant == 3) {
     const int nb = ceil_div(P->n, kSweepBS);
     if (with_step)
       parilu_symrow_sweep_kernel<T, true, 4><<<nb, kSweepBS, 0, stream()>>>(
@@ -720,7 +841,7 @@ double lace_parilu_sweep_bytes(const lace_parilu* P) {
 }
 
 int lace_parilu_set_variant(lace_parilu* Ph, int general) {
-  LACE_REQUIRE(general >= 0 && general <= 3, DEV_ERR_ILLEGAL_VALUE, "variant %d", general);
+  LACE_REQUIRE(general >= 0 && general <= 4, DEV_ERR_ILLEGAL_VALUE, "variant %d", general);
   reinterpret_cast<ParILU*>(Ph)->variant = general;
   return DEV_SUCCESS;
 }

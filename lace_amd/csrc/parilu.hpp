@@ -57,11 +57,16 @@ struct ParILU {
   // precision); maxl: longest strict-lower row.
   bool sym = false;
   // sweep kernel: 0 auto, 1 general CSR-L/CSC-U, 2 symmetric per-position,
-  // 3 symmetric per-row
+  // 3 symmetric per-row, 4 symmetric streaming (precomputed merge data)
   int variant = 0;
   int maxl = 0;
   int last_nb = 0;  // per-block partials written by the last sweep
   DevBuf prow, AL, AU;
+  // streaming kernel: pmeta[p] = index of u_jj for a strict position,
+  // -(row start + 1) for a diagonal one; strict positions' own matches
+  // (k < j) as index pairs, present only when some position has one
+  DevBuf pmeta, mptr, mpairs;
+  bool strict_matches = false;
 
   // Triangular solves.  lower: L (CSRL arrays, values lval[cur]).
   // upper: U in CSR (diagonal first), values gathered from the CSC buffer

@@ -73,8 +73,63 @@ __global__ __launch_bounds__(kSpmvBS) void spmv_kernel(int n, const int* __restr
   }
 }
 
+// CSR-stream form for short rows: a block owns kStreamRows consecutive
+// rows; their nonzeros form one contiguous range of col/val, which the
+// block reads with consecutive threads on consecutive entries (chunks of
+// kStreamCap), forming val*x[col] into LDS.  Thread t then sums row
+// r0+t's products in column order, so each row is summed sequentially
+// from zero exactly as a serial CSR loop does.
+constexpr int kStreamRows = 256;
+constexpr int kStreamCap = 2048;
+
+template <typename TV, typename TX, typename TY, int MODE>
+__global__ __launch_bounds__(kStreamRows) void spmv_stream_kernel(
+    int n, const int* __restrict__ row, const int* __restrict__ col, const TV* __restrict__ val,
+    const TX* __restrict__ x, TY* __restrict__ y, const TY* __restrict__ b, double alpha, double beta) {
+#pragma clang fp contract(off)
+  using ACC = typename std::conditional<sizeof(TV) == 8 || sizeof(TY) == 8, double, float>::type;
+  __shared__ ACC prod[kStreamCap];
+  const int blk = xcd_block(blockIdx.x, gridDim.x);
+  const int r0 = blk * kStreamRows;
+  const int r1 = min(r0 + kStreamRows, n);
+  const int i = r0 + (int)threadIdx.x;
+  const int k0 = row[r0], k1 = row[r1];
+  int rs = 0, re = 0;
+  if (i < r1) {
+    rs = row[i];
+    re = row[i + 1];
+  }
+  ACC acc = ACC(0);
+  for (int c0 = k0; c0 < k1; c0 += kStreamCap) {
+    const int cn = min(kStreamCap, k1 - c0);
+#pragma unroll 4
+    for (int t = threadIdx.x; t < cn; t += kStreamRows)
+      prod[t] = (ACC)val[c0 + t] * (ACC)x[col[c0 + t]];
+    __syncthreads();
+    const int lo = max(rs, c0), hi = min(re, c0 + cn);
+    for (int k = lo; k < hi; k++) acc = acc + prod[k - c0];
+    __syncthreads();
+  }
+  if (i < r1) {
+    if (MODE == 1) {
+      y[i] = (TY)((ACC)b[i] - acc);
+    } else {
+      ACC v = (ACC)alpha * acc;
+      if (beta != 0.0) v = v + (ACC)beta * (ACC)y[i];
+      y[i] = (TY)v;
+    }
+  }
+}
+
 template <typename TV, typename TX, typename TY, int MODE>
 static int launch_spmv(const DevCSR& A, const TX* x, TY* y, const TY* b, double alpha, double beta) {
+  if (A.n > 0 && A.nnz <= 16LL * A.n) {
+    // short rows (at most 16 per row on average): the stream kernel
+    spmv_stream_kernel<TV, TX, TY, MODE><<<ceil_div(A.n, kStreamRows), kStreamRows, 0, stream()>>>(
+        A.n, A.row.as<int>(), A.col.as<int>(), A.val.as<TV>(), x, y, b, alpha, beta);
+    LACE_HIP(hipGetLastError());
+    return DEV_SUCCESS;
+  }
   const int g = A.group;
   const int rows_per_block = (kSpmvBS / g) * kSpmvRows;
   const int nb = ceil_div(A.n, rows_per_block);

@@ -182,6 +182,40 @@ __global__ __launch_bounds__(256) void tri_jacobi_kernel(int n, const int* __res
   }
 }
 
+// Jacobi sweep in CSR-stream form (yin != null, no step): the block's
+// rows own one contiguous range of col/val, read by consecutive threads
+// into LDS as val*yin[col]; thread t then sums row r0+t's off-diagonal
+// products in column order and divides by the diagonal -- the same
+// sequence as tri_jacobi_kernel.
+constexpr int kJacRows = 256;
+constexpr int kJacCap = 2048;
+
+template <typename T, bool UPPER>
+__global__ __launch_bounds__(kJacRows) void tri_jacobi_stream_kernel(
+    int n, const int* __restrict__ row, const int* __restrict__ col, const T* __restrict__ val,
+    const T* __restrict__ r, const T* __restrict__ yin, T* __restrict__ yout) {
+#pragma clang fp contract(off)
+  __shared__ T prod[kJacCap];
+  const int b = xcd_block(blockIdx.x, gridDim.x);
+  const int r0 = b * kJacRows;
+  const int r1 = min(r0 + kJacRows, n);
+  const int i = r0 + (int)threadIdx.x;
+  const int c_begin = row[r0], c_end = row[r1];
+  int k0 = 0, k1 = 0, kd = 0;
+  if (i < r1) row_range<T, UPPER>(row, i, k0, k1, kd);
+  T s = T(0);
+  for (int c0 = c_begin; c0 < c_end; c0 += kJacCap) {
+    const int cn = min(kJacCap, c_end - c0);
+#pragma unroll 4
+    for (int t = threadIdx.x; t < cn; t += kJacRows) prod[t] = val[c0 + t] * yin[col[c0 + t]];
+    __syncthreads();
+    const int lo = max(k0, c0), hi = min(k1, c0 + cn);
+    for (int k = lo; k < hi; k++) s = s + prod[k - c0];
+    __syncthreads();
+  }
+  if (i < r1) yout[i] = (r[i] - s) / val[kd];
+}
+
 template <typename T>
 __global__ void gather_kernel(int m, const int* __restrict__ map, const T* __restrict__ in,
                               T* __restrict__ out) {
@@ -232,6 +266,16 @@ template <typename T>
 static int jacobi_sweep(const TriPlan& P, const T* val, const T* r, const T* yin, T* yout,
                         double* partial) {
   const int nb = ceil_div(P.n, 256);
+  if (!partial && yin && P.n > 0 && P.nnz <= 16LL * P.n) {
+    if (P.upper)
+      tri_jacobi_stream_kernel<T, true><<<ceil_div(P.n, kJacRows), kJacRows, 0, stream()>>>(
+          P.n, P.row.as<int>(), P.col.as<int>(), val, r, yin, yout);
+    else
+      tri_jacobi_stream_kernel<T, false><<<ceil_div(P.n, kJacRows), kJacRows, 0, stream()>>>(
+          P.n, P.row.as<int>(), P.col.as<int>(), val, r, yin, yout);
+    LACE_HIP(hipGetLastError());
+    return DEV_SUCCESS;
+  }
   if (partial) {
     if (P.upper)
       tri_jacobi_kernel<T, true, true><<<nb, 256, 0, stream()>>>(

@@ -33,6 +33,14 @@
  *   orc_levels                    : closed form lev = i+j (2-D) and
  *       i+j+k (3-D) on natural-order stencils; brute-force longest path.
  *   orc_housegen / orc_hqrd       : testing/matrices/magic7_Householder_{U,R}.mtx.
+ *   orc_bilu_sweep_d/f (blocks)   : block size 1 equals orc_parilu_sweep to
+ *       1e-14 (s*(1/u) vs s/u); sweep-1 dense closed form; the fixed point
+ *       and one in-place sweep equal a dense block ILU(0) (IKJ, numpy) on
+ *       testing/matrices/bcsr_test.mtx and on 7-point block grids.
+ *   orc_binv (block inverse)      : numpy.linalg.inv, block sizes 1..8.
+ *   orc_bcsr_spmv                 : dense numpy product.
+ *   orc_btrsv                     : numpy.linalg.solve (exact); the Neumann
+ *       closed form of block Jacobi-k.
  */
 #include <float.h>
 #include <math.h>

@@ -52,14 +52,14 @@ def test_parilu_jacobi_bitexact(gpu, orc, name, prec):
 
 
 @pytest.mark.parametrize("prec", ["f64", "f32"])
-@pytest.mark.parametrize("variant", [0, 2, 3])
+@pytest.mark.parametrize("variant", [0, 2, 3, 4])
 @pytest.mark.parametrize("name", ["convdiff", "trefethen20"])
 def test_parilu_symmetric_kernels_match_general(gpu, orc, prec, variant, name):
-    """Structurally symmetric patterns run the per-row (variant 0 when rows
-    have <= 4 strict-lower entries) or per-position (2) kernel; both must
-    equal the general CSR-L / CSC-U kernel (1) bit for bit.  Trefethen_20
-    has up to 5 strict-lower entries per row, so variant 0 falls back to the
-    per-position kernel there."""
+    """Structurally symmetric patterns run the streaming (0 = 4), per-row
+    (3) or per-position (2) kernel; each must equal the general CSR-L /
+    CSC-U kernel (1) bit for bit.  Trefethen_20 has strict positions with
+    merge partners k < j (triangles in its graph), the 7-point stencil has
+    none, so both branches of the streaming kernel run."""
     A = orc.convdiff3d(12, 10, 9) if name == "convdiff" else orc.read_mtx(
         os.path.join(GOLDEN, "Trefethen_20.mtx"))
     lp = gpu.LACE_F64 if prec == "f64" else gpu.LACE_F32
