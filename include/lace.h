@@ -437,6 +437,11 @@ int lace_solver_set_values(lace_solver *S, const double *vals);
 int lace_solver_grad_values(lace_solver *S, const double *lambda,
                             const double *x, double *dtheta);
 
+/* Kernel-shape options for tuning runs.  "stream_rows_per_thread" (1, 2 or
+ * 4): rows each thread owns in the CSR-stream SpMV and Jacobi sweeps
+ * (default: 2 for FP32 values, 1 for FP64). */
+int lace_set_option(const char *name, int value);
+
 /* Kernel timing by class, HIP events on the library stream. */
 int lace_timing_enable(int on);
 int lace_timing_reset(void);

@@ -173,6 +173,7 @@ SIGNATURES = {
     "lace_solver_grad_values": (_i, [_vp, _vp, _vp, _vp]),
     "lace_solver_solve_trans": (_i, [_vp, _vp, _vp, _i, _P(SolverStats)]),
     "lace_solver_set_values": (_i, [_vp, _P(_d)]),
+    "lace_set_option": (_i, [C.c_char_p, _i]),
     "lace_timing_enable": (_i, [_i]),
     "lace_timing_reset": (_i, []),
     "lace_timing_only": (_i, [C.c_char_p]),

@@ -52,6 +52,8 @@ int ensure_init();
 hipStream_t stream();
 int device_id();
 int num_cus();
+// Kernel-shape option set through lace_set_option (dflt when unset).
+int get_option(const char* name, int dflt);
 
 // ---------------------------------------------------------------- memory
 // Host arrays in the reference convention (include/dense_types.h:430-450):

@@ -42,6 +42,46 @@ __device__ __forceinline__ T block_sum(T v, T* smem) {
   return r;
 }
 
+// Four consecutive values from a 4-element-aligned address.
+__device__ __forceinline__ void load4(const float* p, float (&v)[4]) {
+  const float4 t = *reinterpret_cast<const float4*>(p);
+  v[0] = t.x; v[1] = t.y; v[2] = t.z; v[3] = t.w;
+}
+__device__ __forceinline__ void load4(const double* p, double (&v)[4]) {
+  const double2 a = *reinterpret_cast<const double2*>(p);
+  const double2 b = *reinterpret_cast<const double2*>(p + 2);
+  v[0] = a.x; v[1] = a.y; v[2] = b.x; v[3] = b.y;
+}
+
+// CSR-stream staging: prod[k - c0] = val[k] * x[col[k]] for k in
+// [c0, c0 + cn), by NT threads.  Each thread takes aligned groups of four
+// entries (16-byte column and value loads; col and val must start on a
+// 16-byte boundary, as device allocations do); entries before c0 in the
+// first group are skipped, and the last, partial group is read entry by
+// entry so nothing past c0 + cn is touched.
+template <int NT, typename ACC, typename TV, typename TX>
+__device__ __forceinline__ void stage_products(const int* __restrict__ col, const TV* __restrict__ val,
+                                               const TX* __restrict__ x, int c0, int cn, ACC* prod) {
+  const int end = c0 + cn;
+  for (int idx = (c0 & ~3) + 4 * (int)threadIdx.x; idx < end; idx += 4 * NT) {
+    if (idx + 3 < end) {
+      const int4 c4 = *reinterpret_cast<const int4*>(col + idx);
+      TV v[4];
+      load4(val + idx, v);
+      const int cc[4] = {c4.x, c4.y, c4.z, c4.w};
+#pragma unroll
+      for (int e = 0; e < 4; e++)
+        if (idx + e >= c0) prod[idx + e - c0] = (ACC)v[e] * (ACC)x[cc[e]];
+    } else {
+#pragma unroll
+      for (int e = 0; e < 4; e++) {
+        const int k = idx + e;
+        if (k >= c0 && k < end) prod[k - c0] = (ACC)val[k] * (ACC)x[col[k]];
+      }
+    }
+  }
+}
+
 template <typename T>
 struct Prec;
 template <>

@@ -135,6 +135,17 @@ __global__ __launch_bounds__(256) void stream_copy_kernel(size_t n16, const uint
     dst[i] = src[i];
 }
 
+// ---------------------------------------------------------------- options
+// Kernel-shape options (benchmarking and tuning); read by the launchers.
+static std::mutex g_omu;
+static std::map<std::string, int> g_options;
+
+int get_option(const char* name, int dflt) {
+  std::lock_guard<std::mutex> lk(g_omu);
+  auto it = g_options.find(name);
+  return it == g_options.end() ? dflt : it->second;
+}
+
 }  // namespace lace
 
 using namespace lace;
@@ -223,3 +234,14 @@ int lace_timing_get(const char* name, double* total_ms, long* count) {
 }
 
 }  // extern "C"
+
+extern "C" int lace_set_option(const char* name, int value) {
+  LACE_REQUIRE(name != nullptr, DEV_ERR_INVALID_PTR, "lace_set_option: null name");
+  const std::string key(name);
+  LACE_REQUIRE(key == "stream_rows_per_thread", DEV_ERR_ILLEGAL_VALUE, "unknown option %s", name);
+  LACE_REQUIRE(value == 1 || value == 2 || value == 4, DEV_ERR_ILLEGAL_VALUE,
+               "stream_rows_per_thread must be 1, 2 or 4 (have %d)", value);
+  std::lock_guard<std::mutex> lk(g_omu);
+  g_options[key] = value;
+  return DEV_SUCCESS;
+}

@@ -4,11 +4,13 @@
 // Arnoldi/residual loops of src/fgmres.cpp:259-268 and
 // src/fgmres_householder_restart.cpp:210-221,357-367.
 //
-// Kernel: G lanes per row (G = 4, 8, 16 or 32 chosen from the mean row
-// length), each lane strides the row's nonzeros, partial sums meet in a
-// shuffle tree inside the lane group.  A wave covers 64/G consecutive rows,
-// so each load instruction touches one contiguous run of col/val.  Blocks
-// are remapped so each XCD streams one contiguous slab of rows.
+// Kernels: matrices with at most 16 nonzeros per row on average take the
+// CSR-stream kernel (the block's nonzeros staged through LDS with
+// coalesced loads, each row summed in CSR order by one thread).  Longer
+// rows take the lane-group kernel: G lanes per row (G = 4, 8, 16 or 32
+// from the mean row length), each lane strides the row's nonzeros, partial
+// sums meet in a shuffle tree inside the lane group.  Blocks are remapped
+// so each XCD streams one contiguous slab of rows.
 #include <algorithm>
 #include <type_traits>
 #include <vector>
@@ -73,60 +75,91 @@ __global__ __launch_bounds__(kSpmvBS) void spmv_kernel(int n, const int* __restr
   }
 }
 
-// CSR-stream form for short rows: a block owns kStreamRows consecutive
-// rows; their nonzeros form one contiguous range of col/val, which the
-// block reads with consecutive threads on consecutive entries (chunks of
-// kStreamCap), forming val*x[col] into LDS.  Thread t then sums row
-// r0+t's products in column order, so each row is summed sequentially
-// from zero exactly as a serial CSR loop does.
-constexpr int kStreamRows = 256;
-constexpr int kStreamCap = 2048;
+// CSR-stream form for short rows: a block owns 256*R consecutive rows
+// (thread t: rows r0 + t + 256 q, q < R); their nonzeros form one
+// contiguous range of col/val, which the block reads with consecutive
+// threads on consecutive entries (chunks of CAP), forming val*x[col] into
+// LDS.  Each thread then sums its rows' products in column order, so each
+// row is summed sequentially from zero exactly as a serial CSR loop does.
+// The per-row operands (row bounds, b or y) are loaded before the staging
+// so their latency overlaps it.
+constexpr int kStreamThreads = 256;
 
-template <typename TV, typename TX, typename TY, int MODE>
-__global__ __launch_bounds__(kStreamRows) void spmv_stream_kernel(
+template <typename ACC, int R>
+struct StreamCap {
+  static constexpr int rows = kStreamThreads * R;
+  static constexpr int cap = (rows * 8 * (int)sizeof(ACC) <= 32768) ? rows * 8 : 32768 / (int)sizeof(ACC);
+};
+
+template <typename TV, typename TX, typename TY, int MODE, int R>
+__global__ __launch_bounds__(kStreamThreads) void spmv_stream_kernel(
     int n, const int* __restrict__ row, const int* __restrict__ col, const TV* __restrict__ val,
     const TX* __restrict__ x, TY* __restrict__ y, const TY* __restrict__ b, double alpha, double beta) {
 #pragma clang fp contract(off)
   using ACC = typename std::conditional<sizeof(TV) == 8 || sizeof(TY) == 8, double, float>::type;
-  __shared__ ACC prod[kStreamCap];
+  constexpr int ROWS = StreamCap<ACC, R>::rows, CAP = StreamCap<ACC, R>::cap;
+  __shared__ ACC prod[CAP];
   const int blk = xcd_block(blockIdx.x, gridDim.x);
-  const int r0 = blk * kStreamRows;
-  const int r1 = min(r0 + kStreamRows, n);
-  const int i = r0 + (int)threadIdx.x;
-  const int k0 = row[r0], k1 = row[r1];
-  int rs = 0, re = 0;
-  if (i < r1) {
-    rs = row[i];
-    re = row[i + 1];
-  }
-  ACC acc = ACC(0);
-  for (int c0 = k0; c0 < k1; c0 += kStreamCap) {
-    const int cn = min(kStreamCap, k1 - c0);
-#pragma unroll 4
-    for (int t = threadIdx.x; t < cn; t += kStreamRows)
-      prod[t] = (ACC)val[c0 + t] * (ACC)x[col[c0 + t]];
-    __syncthreads();
-    const int lo = max(rs, c0), hi = min(re, c0 + cn);
-    for (int k = lo; k < hi; k++) acc = acc + prod[k - c0];
-    __syncthreads();
-  }
-  if (i < r1) {
-    if (MODE == 1) {
-      y[i] = (TY)((ACC)b[i] - acc);
-    } else {
-      ACC v = (ACC)alpha * acc;
-      if (beta != 0.0) v = v + (ACC)beta * (ACC)y[i];
-      y[i] = (TY)v;
+  const int r0 = blk * ROWS;
+  const int r1 = min(r0 + ROWS, n);
+  int rs[R], re[R];
+  ACC add[R], acc[R];
+#pragma unroll
+  for (int q = 0; q < R; q++) {
+    const int i = r0 + (int)threadIdx.x + q * kStreamThreads;
+    rs[q] = re[q] = 0;
+    add[q] = acc[q] = ACC(0);
+    if (i < r1) {
+      rs[q] = row[i];
+      re[q] = row[i + 1];
+      if (MODE == 1) add[q] = (ACC)b[i];
+      else if (beta != 0.0) add[q] = (ACC)beta * (ACC)y[i];
     }
   }
+  const int k0 = row[r0], k1 = row[r1];
+  for (int c0 = k0; c0 < k1; c0 += CAP) {
+    const int cn = min(CAP, k1 - c0);
+    stage_products<kStreamThreads, ACC>(col, val, x, c0, cn, prod);
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < R; q++) {
+      const int lo = max(rs[q], c0), hi = min(re[q], c0 + cn);
+      for (int k = lo; k < hi; k++) acc[q] = acc[q] + prod[k - c0];
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int q = 0; q < R; q++) {
+    const int i = r0 + (int)threadIdx.x + q * kStreamThreads;
+    if (i < r1) {
+      if (MODE == 1) {
+        y[i] = (TY)(add[q] - acc[q]);
+      } else {
+        ACC v = (ACC)alpha * acc[q];
+        if (beta != 0.0) v = v + add[q];
+        y[i] = (TY)v;
+      }
+    }
+  }
+}
+
+template <typename TV, typename TX, typename TY, int MODE, int R>
+static void launch_stream(const DevCSR& A, const TX* x, TY* y, const TY* b, double alpha, double beta) {
+  using ACC = typename std::conditional<sizeof(TV) == 8 || sizeof(TY) == 8, double, float>::type;
+  spmv_stream_kernel<TV, TX, TY, MODE, R>
+      <<<ceil_div(A.n, StreamCap<ACC, R>::rows), kStreamThreads, 0, stream()>>>(
+          A.n, A.row.as<int>(), A.col.as<int>(), A.val.as<TV>(), x, y, b, alpha, beta);
 }
 
 template <typename TV, typename TX, typename TY, int MODE>
 static int launch_spmv(const DevCSR& A, const TX* x, TY* y, const TY* b, double alpha, double beta) {
   if (A.n > 0 && A.nnz <= 16LL * A.n) {
     // short rows (at most 16 per row on average): the stream kernel
-    spmv_stream_kernel<TV, TX, TY, MODE><<<ceil_div(A.n, kStreamRows), kStreamRows, 0, stream()>>>(
-        A.n, A.row.as<int>(), A.col.as<int>(), A.val.as<TV>(), x, y, b, alpha, beta);
+    // measured at 128^3: two rows per thread for FP32 values, one for FP64
+    const int R = get_option("stream_rows_per_thread", sizeof(TV) == 8 ? 1 : 2);
+    if (R == 1) launch_stream<TV, TX, TY, MODE, 1>(A, x, y, b, alpha, beta);
+    else if (R == 4) launch_stream<TV, TX, TY, MODE, 4>(A, x, y, b, alpha, beta);
+    else launch_stream<TV, TX, TY, MODE, 2>(A, x, y, b, alpha, beta);
     LACE_HIP(hipGetLastError());
     return DEV_SUCCESS;
   }

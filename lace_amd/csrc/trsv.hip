@@ -182,38 +182,76 @@ __global__ __launch_bounds__(256) void tri_jacobi_kernel(int n, const int* __res
   }
 }
 
-// Jacobi sweep in CSR-stream form (yin != null, no step): the block's
-// rows own one contiguous range of col/val, read by consecutive threads
-// into LDS as val*yin[col]; thread t then sums row r0+t's off-diagonal
+// Jacobi sweep in CSR-stream form (yin != null, no step): a block owns
+// 256*R consecutive rows (thread t: rows r0 + t + 256 q); their nonzeros
+// form one contiguous range of col/val, read by consecutive threads into
+// LDS as val*yin[col]; each thread then sums its rows' off-diagonal
 // products in column order and divides by the diagonal -- the same
-// sequence as tri_jacobi_kernel.
-constexpr int kJacRows = 256;
-constexpr int kJacCap = 2048;
+// sequence as tri_jacobi_kernel.  Row bounds, r and the diagonal are
+// loaded before the staging so their latency overlaps it.
+template <typename T, int R>
+struct JacCap {
+  static constexpr int rows = 256 * R;
+  static constexpr int cap = (rows * 8 * (int)sizeof(T) <= 32768) ? rows * 8 : 32768 / (int)sizeof(T);
+};
 
-template <typename T, bool UPPER>
-__global__ __launch_bounds__(kJacRows) void tri_jacobi_stream_kernel(
+template <typename T, bool UPPER, int R>
+__global__ __launch_bounds__(256) void tri_jacobi_stream_kernel(
     int n, const int* __restrict__ row, const int* __restrict__ col, const T* __restrict__ val,
     const T* __restrict__ r, const T* __restrict__ yin, T* __restrict__ yout) {
 #pragma clang fp contract(off)
-  __shared__ T prod[kJacCap];
+  constexpr int ROWS = JacCap<T, R>::rows, CAP = JacCap<T, R>::cap;
+  __shared__ T prod[CAP];
   const int b = xcd_block(blockIdx.x, gridDim.x);
-  const int r0 = b * kJacRows;
-  const int r1 = min(r0 + kJacRows, n);
-  const int i = r0 + (int)threadIdx.x;
+  const int r0 = b * ROWS;
+  const int r1 = min(r0 + ROWS, n);
+  int k0[R], k1[R];
+  T rr[R], dg[R], s[R];
+#pragma unroll
+  for (int q = 0; q < R; q++) {
+    const int i = r0 + (int)threadIdx.x + 256 * q;
+    k0[q] = k1[q] = 0;
+    rr[q] = s[q] = T(0);
+    dg[q] = T(1);
+    if (i < r1) {
+      int kd;
+      row_range<T, UPPER>(row, i, k0[q], k1[q], kd);
+      rr[q] = r[i];
+      dg[q] = val[kd];
+    }
+  }
   const int c_begin = row[r0], c_end = row[r1];
-  int k0 = 0, k1 = 0, kd = 0;
-  if (i < r1) row_range<T, UPPER>(row, i, k0, k1, kd);
-  T s = T(0);
-  for (int c0 = c_begin; c0 < c_end; c0 += kJacCap) {
-    const int cn = min(kJacCap, c_end - c0);
-#pragma unroll 4
-    for (int t = threadIdx.x; t < cn; t += kJacRows) prod[t] = val[c0 + t] * yin[col[c0 + t]];
+  for (int c0 = c_begin; c0 < c_end; c0 += CAP) {
+    const int cn = min(CAP, c_end - c0);
+    stage_products<256, T>(col, val, yin, c0, cn, prod);
     __syncthreads();
-    const int lo = max(k0, c0), hi = min(k1, c0 + cn);
-    for (int k = lo; k < hi; k++) s = s + prod[k - c0];
+#pragma unroll
+    for (int q = 0; q < R; q++) {
+      const int lo = max(k0[q], c0), hi = min(k1[q], c0 + cn);
+      for (int k = lo; k < hi; k++) s[q] = s[q] + prod[k - c0];
+    }
     __syncthreads();
   }
-  if (i < r1) yout[i] = (r[i] - s) / val[kd];
+#pragma unroll
+  for (int q = 0; q < R; q++) {
+    const int i = r0 + (int)threadIdx.x + 256 * q;
+    if (i < r1) yout[i] = (rr[q] - s[q]) / dg[q];
+  }
+}
+
+template <typename T, bool UPPER, int R>
+static void launch_jacobi_stream(const TriPlan& P, const T* val, const T* r, const T* yin, T* yout) {
+  tri_jacobi_stream_kernel<T, UPPER, R><<<ceil_div(P.n, JacCap<T, R>::rows), 256, 0, stream()>>>(
+      P.n, P.row.as<int>(), P.col.as<int>(), val, r, yin, yout);
+}
+
+template <typename T, bool UPPER>
+static void jacobi_stream(const TriPlan& P, const T* val, const T* r, const T* yin, T* yout) {
+  // measured at 128^3: two rows per thread for FP32 values, one for FP64
+  const int R = get_option("stream_rows_per_thread", sizeof(T) == 8 ? 1 : 2);
+  if (R == 1) launch_jacobi_stream<T, UPPER, 1>(P, val, r, yin, yout);
+  else if (R == 4) launch_jacobi_stream<T, UPPER, 4>(P, val, r, yin, yout);
+  else launch_jacobi_stream<T, UPPER, 2>(P, val, r, yin, yout);
 }
 
 template <typename T>
@@ -267,12 +305,8 @@ static int jacobi_sweep(const TriPlan& P, const T* val, const T* r, const T* yin
                         double* partial) {
   const int nb = ceil_div(P.n, 256);
   if (!partial && yin && P.n > 0 && P.nnz <= 16LL * P.n) {
-    if (P.upper)
-      tri_jacobi_stream_kernel<T, true><<<ceil_div(P.n, kJacRows), kJacRows, 0, stream()>>>(
-          P.n, P.row.as<int>(), P.col.as<int>(), val, r, yin, yout);
-    else
-      tri_jacobi_stream_kernel<T, false><<<ceil_div(P.n, kJacRows), kJacRows, 0, stream()>>>(
-          P.n, P.row.as<int>(), P.col.as<int>(), val, r, yin, yout);
+    if (P.upper) jacobi_stream<T, true>(P, val, r, yin, yout);
+    else jacobi_stream<T, false>(P, val, r, yin, yout);
     LACE_HIP(hipGetLastError());
     return DEV_SUCCESS;
   }
