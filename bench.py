@@ -86,12 +86,46 @@ def cpu_baseline(row, col, val, b, threads):
                        f"substitution, {r.searches} iterations to rel. residual {rel:.2e}")}
 
 
+SWEEP_KERNEL = "parilu_stream_sweep_kernel<float>"
+JACOBI_KERNEL = "tri_jacobi_stream_kernel<float>"
+SPMV64_KERNEL = "spmv_stream_kernel<double, float, double>"
+SPMV32_KERNEL = "spmv_stream_kernel<float, float, float>"
+# kernel class -> per-launch timing regions of the library
+KERNEL_CLASSES = {
+    SWEEP_KERNEL: ["parilu_sweep"],
+    JACOBI_KERNEL: ["trsv_jacobi_sweep_lower", "trsv_jacobi_sweep_upper"],
+    SPMV64_KERNEL: ["spmv"],
+    SPMV32_KERNEL: ["spmv_f32a"],
+}
+HOT_REGIONS = [r for rs in KERNEL_CLASSES.values() for r in rs]
+
+
+def kernel_bytes(n, nnz, nnz_l, nnz_u, sweep_bytes):
+    """Algorithmic HBM bytes per launch of each timed region (int32
+    indices): ParILU sweep nnz*(4 + 4 + 2*4) + 4(n+1); Jacobi sweep on the
+    strict-lower L (unit diagonal) or on U: nnz_T*(4+4) + 4(n+1) + 3*4n
+    (r, previous iterate, new iterate); outer SpMV with A in FP64 and the
+    FP32 preconditioned vector: 12 nnz + 4(n+1) + 4n + 8n; inner FP32 SpMV:
+    8 nnz + 4(n+1) + 8n."""
+    return {
+        "parilu_sweep": sweep_bytes,
+        "trsv_jacobi_sweep_lower": 8.0 * nnz_l + 4.0 * (n + 1) + 12.0 * n,
+        "trsv_jacobi_sweep_upper": 8.0 * nnz_u + 4.0 * (n + 1) + 12.0 * n,
+        "spmv": 12.0 * nnz + 4.0 * (n + 1) + 12.0 * n,
+        "spmv_f32a": 8.0 * nnz + 4.0 * (n + 1) + 8.0 * n,
+    }
+
+
 def pmc_traffic():
-    p = os.path.join(ROOT, "profiles", "parilu_sweep_traffic.json")
+    """HBM bytes per launch of each kernel class from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE summary (calibrated against a copy
+    kernel of known size), or None."""
+    p = os.path.join(ROOT, "profiles", "hbm_traffic_convdiff128.json")
     if not os.path.exists(p):
         return None
     with open(p) as f:
-        return json.load(f).get("bytes_per_launch")
+        data = json.load(f)
+    return {k: v.get("bytes_per_launch") for k, v in data.get("kernels", {}).items()}
 
 
 def _region_rate(L, name, nbytes, fn, reps):
@@ -178,8 +212,8 @@ def main() -> int:
     ap.add_argument("--size", type=int, default=128)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
-    ap.add_argument("--mode", choices=["solve", "sweeps", "bcsr"], default="solve",
-                    help="sweeps: only FP32 ParILU sweeps plus one 256 MiB copy kernel "
+    ap.add_argument("--mode", choices=["solve", "kernels", "bcsr"], default="solve",
+                    help="kernels: one mixed-path step plus one 256 MiB copy kernel "
                          "(the workload profiled for HBM traffic counters); bcsr: the "
                          "block-sparse 5x5 workload on the 64^3 block grid")
     args = ap.parse_args()
@@ -199,15 +233,19 @@ def main() -> int:
     n, nnz = len(row) - 1, int(row[-1])
     b = L.gen_normal(0, n)
     H = L.HostCSR(row, col, val)
-    if args.mode == "sweeps":
-        P = L.ParILU(H, L.LACE_F32)
-        P.sweeps(args.warmup + args.steps)
+    if args.mode == "kernels":
+        # one mixed-path step (every hot kernel as in the timed step) plus
+        # one 256 MiB copy, the traffic-counter calibration
+        S = L.Solver(H, make_opts(L, **MIXED))
+        dbk = L.DevVec(n, np.float64, b)
+        dxk = L.DevVec(n, np.float64)
+        res = timed_solves(L, S, dbk, dxk, np.zeros(n), 1)
         src = L.DevVec(1 << 25, np.float64)
         dst = L.DevVec(1 << 25, np.float64)
         L.check(lib.lace_stream_copy(dst.ptr, src.ptr, 1 << 28))
         L.check(lib.lace_device_synchronize())
-        print(json.dumps({"mode": "sweeps", "sweeps": args.warmup + args.steps,
-                          "sweep_bytes": P.sweep_bytes(), "copy_bytes": 1 << 28}))
+        print(json.dumps({"mode": "kernels", "step_s": res[0][0],
+                          "iterations": res[0][1].iterations, "copy_bytes": 1 << 28}))
         return 0
     db = L.DevVec(n, np.float64, b)
     dx = L.DevVec(n, np.float64)
@@ -218,14 +256,14 @@ def main() -> int:
     for S in (mixed, fp64):
         timed_solves(L, S, db, dx, zeros, args.warmup)
 
-    # timed region: mixed steps with the ParILU sweep timed by HIP events on
-    # the library stream (only that region carries events)
+    # timed region: mixed steps with the hot kernels timed per launch by HIP
+    # events on the library stream (only these regions carry events)
     L.check(lib.lace_timing_reset())
-    L.check(lib.lace_timing_only(b"parilu_sweep"))
+    L.check(lib.lace_timing_only(",".join(HOT_REGIONS).encode()))
     L.check(lib.lace_timing_enable(1))
     res_mixed = timed_solves(L, mixed, db, dx, zeros, args.steps)
     L.check(lib.lace_timing_enable(0))
-    sweep_ms, sweep_cnt = L.timing("parilu_sweep")
+    live = {r: L.timing(r) for r in HOT_REGIONS}
     res_fp64 = timed_solves(L, fp64, db, dx, zeros, args.steps)
 
     t_mixed = float(np.mean([t for t, _ in res_mixed]))
@@ -233,11 +271,29 @@ def main() -> int:
     st_m, st_f = res_mixed[-1][1], res_fp64[-1][1]
     ok = all(s.converged == 1 and s.rel_true_residual <= 1e-8 for _, s in res_mixed + res_fp64)
 
-    # dominant named kernel: the FP32 ParILU sweep of the mixed path
+    # per-kernel rates from the live timings, algorithmic bytes per launch
     P = L.ParILU(H, L.LACE_F32)
-    sweep_bytes = P.sweep_bytes()
-    avg_s = (sweep_ms / sweep_cnt) * 1e-3 if sweep_cnt else float("nan")
-    achieved = sweep_bytes / avg_s / 1e9
+    nnz_l = (nnz - n) // 2  # strict-lower entries (structurally symmetric pattern)
+    nnz_u = nnz - nnz_l
+    kb = kernel_bytes(n, nnz, nnz_l, nnz_u, P.sweep_bytes())
+    traffic = pmc_traffic()
+    kernels = {}
+    for name, regions in KERNEL_CLASSES.items():
+        ms = sum(live[r][0] for r in regions)
+        cnt = sum(live[r][1] for r in regions)
+        by = sum(kb[r] * live[r][1] for r in regions)
+        if cnt == 0:
+            continue
+        avg_s = ms * 1e-3 / cnt
+        ach = by / (ms * 1e-3) / 1e9
+        kernels[name] = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS,
+                         "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
+                         "traffic": (traffic or {}).get(name),
+                         "bytes_per_launch": round(by / cnt), "avg_launch_us": round(avg_s * 1e6, 2),
+                         "launches_timed": cnt, "total_ms_in_timed_steps": round(ms, 3)}
+    dominant = max(kernels, key=lambda k: kernels[k]["total_ms_in_timed_steps"])
+    sw = kernels.get(SWEEP_KERNEL, {})
+    achieved = sw.get("achieved", float("nan"))
     P64 = L.ParILU(H, L.LACE_F64)
     L.check(lib.lace_timing_reset())
     L.check(lib.lace_timing_enable(1))
@@ -281,18 +337,9 @@ def main() -> int:
         "fp64_rel_true_residual": st_f.rel_true_residual,
         "all_converged": ok,
         "parilu_sweep_GBps": {"f32": round(achieved, 1), "f64": round(ach64, 1)},
-        "roofline": {
-            "bound": "hbm",
-            "kernel": "parilu_sweep_kernel<float> (mixed path setup)",
-            "achieved": round(achieved, 1),
-            "peak": HBM_PEAK_GBS,
-            "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": pmc_traffic(),
-            "bytes_per_launch": sweep_bytes,
-            "avg_launch_us": round(avg_s * 1e6, 2),
-            "launches_timed": sweep_cnt,
-        },
+        # the kernel with the most time in the timed mixed steps
+        "roofline": dict(kernel=dominant, **kernels[dominant]),
+        "kernels": kernels,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

@@ -445,7 +445,10 @@ int lace_set_option(const char *name, int value);
 /* Kernel timing by class, HIP events on the library stream. */
 int lace_timing_enable(int on);
 int lace_timing_reset(void);
-/* Restrict timing to one region name (NULL or "" times every region). */
+/* Restrict timing to a comma-separated list of region names (NULL or ""
+ * times every region).  Per-launch regions of the hot kernels:
+ * "parilu_sweep", "spmv" (FP64 matrix), "spmv_f32a" (FP32 matrix),
+ * "trsv_jacobi_sweep_lower" / "_upper" (one stream Jacobi sweep). */
 int lace_timing_only(const char *name);
 int lace_timing_get(const char *name, double *total_ms, long *count);
 

@@ -96,9 +96,23 @@ static std::string g_timing_only;  // empty: time every region
 
 bool timing_on() { return g_timing_on; }
 
+// g_timing_only: a comma-separated list of region names (empty: all).
+static bool timing_selected(const char* n) {
+  if (g_timing_only.empty()) return true;
+  const size_t len = strlen(n);
+  size_t pos = 0;
+  while (pos <= g_timing_only.size()) {
+    size_t end = g_timing_only.find(',', pos);
+    if (end == std::string::npos) end = g_timing_only.size();
+    if (end - pos == len && g_timing_only.compare(pos, len, n) == 0) return true;
+    pos = end + 1;
+  }
+  return false;
+}
+
 TimedRegion::TimedRegion(const char* n) : name(n) {
   if (!g_timing_on) return;
-  if (!g_timing_only.empty() && g_timing_only != n) return;
+  if (!timing_selected(n)) return;
   if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) return;
   on = hipEventRecord(a, stream()) == hipSuccess;
 }

@@ -210,7 +210,7 @@ double DevCSR::bytes(int sx, int sy, bool residual) const {
 // y = alpha*A*x + beta*y; vector precisions follow the template arguments.
 template <typename TX, typename TY>
 int spmv(const DevCSR& A, double alpha, const TX* x, double beta, TY* y) {
-  TimedRegion tr("spmv");
+  TimedRegion tr(A.prec == LACE_F64 ? "spmv" : "spmv_f32a");
   if (A.prec == LACE_F64) return launch_spmv<double, TX, TY, 0>(A, x, y, nullptr, alpha, beta);
   return launch_spmv<float, TX, TY, 0>(A, x, y, nullptr, alpha, beta);
 }

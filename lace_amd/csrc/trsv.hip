@@ -305,6 +305,7 @@ static int jacobi_sweep(const TriPlan& P, const T* val, const T* r, const T* yin
                         double* partial) {
   const int nb = ceil_div(P.n, 256);
   if (!partial && yin && P.n > 0 && P.nnz <= 16LL * P.n) {
+    TimedRegion tr(P.upper ? "trsv_jacobi_sweep_upper" : "trsv_jacobi_sweep_lower");
     if (P.upper) jacobi_stream<T, true>(P, val, r, yin, yout);
     else jacobi_stream<T, false>(P, val, r, yin, yout);
     LACE_HIP(hipGetLastError());
