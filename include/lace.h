@@ -263,6 +263,21 @@ int data_fgmres_householder_restart(data_d_matrix *A, data_d_matrix *b,
                                     data_d_gmres_param *gmres_par,
                                     data_d_gmres_log *gmres_log);
 
+/* Orderings.  sparse.h:398-402,662-665; src/cuthill_mckee.cpp:53-265,
+ * src/reorder.cpp:167-253.  nn_map[i] = new number of row i; every quirk of
+ * the reference ordering is kept (documented in lace_amd/csrc/reorder.cpp);
+ * where the reference reads uninitialised memory this returns DEV_ERR.
+ * data_sparse_reorder: reorder 1 = Cuthill-McKee, 2 = reversed, 0/other =
+ * identity (3 = AMD is not provided: DEV_ERR_NOT_SUPPORTED); A becomes
+ * A(P, P) with rows sorted by column.  lace_bool is C++ bool / C _Bool. */
+#ifdef __cplusplus
+typedef bool lace_bool;
+#else
+typedef _Bool lace_bool;
+#endif
+int data_reorder_cuthill_mckee(data_d_matrix *A, int *nn_map, lace_bool reverse);
+int data_sparse_reorder(data_d_matrix *A, int *P, int *Pinv, int reorder);
+
 /* Givens generator.  sparse.h:1096; src/givens.cpp:46-67 */
 void givens_rotation(dataType a, dataType b, dataType tol, dataType *c,
                      dataType *s);

@@ -128,6 +128,8 @@ SIGNATURES = {
     "data_fgmres_householder_restart": (_i, [_DM, _DM, _DM, _DM, _DM, _P(GmresParam),
                                              _P(GmresLog)]),
     "givens_rotation": (None, [_d, _d, _d, _P(_d), _P(_d)]),
+    "data_reorder_cuthill_mckee": (_i, [_DM, _P(_i), C.c_bool]),
+    "data_sparse_reorder": (_i, [_DM, _P(_i), _P(_i), _i]),
     "lace_init": (_i, [_i]),
     "lace_last_error": (C.c_char_p, []),
     "lace_device_synchronize": (_i, []),

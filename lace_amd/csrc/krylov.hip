@@ -18,9 +18,9 @@
 // MI355X design: the reflector products are applied in compact-WY form,
 // P_0 P_1 ... P_j = I - U T U^T with T upper triangular
 // (T_{j+1} = [[T_j, -2 T_j U_j^T u_{j+1}], [0, 2]]), so each Arnoldi step
-// streams the reflector block U four times (multi-dot, update, multi-dot of
-// the tail, new column + next basis vector) instead of 2(j+1) dot/axpy
-// passes.  All O(j^2) work (T, Hessenberg, Givens, least squares) stays on
+// streams the reflector block U three times (multi-dot; the update fused
+// with the multi-dot of the tail; new column + next basis vector) instead
+// of 2(j+1) dot/axpy passes.  All O(j^2) work (T, Hessenberg, Givens, least squares) stays on
 // the device in one-workgroup kernels; the host only reads the convergence
 // flag.  Every kernel after the first of a cycle returns at once when the
 // device-side done flag is set, so a caller may queue steps ahead.
@@ -196,20 +196,30 @@ __global__ __launch_bounds__(kBS) void hh_start_vec_kernel(int n, const K* __res
 }
 
 // ------------------------------------------------------------ step kernels
+// The one-workgroup kernels below stage the small dense state they loop
+// over (T, the Hessenberg column, the rotations) in LDS with parallel
+// loads first, so their O(j) loops never wait on a global load.
+constexpr int kMaxT = 65 * 65;  // T for restart lengths up to 63
+
+__device__ __forceinline__ void stage(const double* __restrict__ src, int count, double* dst) {
+  for (int e = threadIdx.x; e < count; e += blockDim.x) dst[e] = src[e];
+}
+
 // c = U_j^T w (from partials); d = T_j^T c.
 __global__ __launch_bounds__(256) void hh_dT_kernel(int nb, int pstride, const double* partial, int j,
                                                     double* small, SmallLayout lay) {
   double* misc = small + lay.misc();
   if (is_done(misc)) return;
   __shared__ double c[72];
+  __shared__ double Ts[kMaxT];
+  const int ldT = lay.ldT();
+  stage(small + lay.T(), (j + 1) * ldT, Ts);
   finalize_partials(nb, j + 1, false, pstride, partial, c);
   __syncthreads();
-  const double* T = small + lay.T();
-  const int ldT = lay.ldT();
   double* d = small + lay.d();
   for (int k = threadIdx.x; k <= j; k += 256) {
     double v = 0.0;
-    for (int l = 0; l <= k; l++) v += T[l + k * ldT] * c[l];
+    for (int l = 0; l <= k; l++) v += Ts[l + k * ldT] * c[l];
     d[k] = v;
   }
 }
@@ -296,12 +306,29 @@ __global__ __launch_bounds__(256) void hh_newcol_small_kernel(int nb, int pstrid
   if (is_done(misc)) return;
   __shared__ double s[72];
   __shared__ double e[72];
+  __shared__ double h[72];   // Hessenberg column j
+  __shared__ double cz[72];  // previous rotations
+  __shared__ double sz[72];
   __shared__ double sc[4];
-  finalize_partials(nb, j + 1, true, pstride, partial, s);  // s[0..j], s[j+1] = nu
-  __syncthreads();
+  __shared__ double Ts[kMaxT];
   double* T = small + lay.T();
   const int ldT = lay.ldT();
   const int j1 = j + 1;
+  const double* cs = small + lay.cs();
+  const double* sn = small + lay.sn();
+  // parallel staging: T's first j+1 columns, w'[0..j], the rotations, and
+  // row j+1 of the reflectors
+  stage(T, j1 * ldT, Ts);
+  for (int i = threadIdx.x; i <= j; i += 256) {
+    h[i] = (double)w[i];
+    e[i] = j1 < n ? (double)U[(size_t)i * ld + j1] : 0.0;
+    if (i < j) {
+      cz[i] = cs[i];
+      sz[i] = sn[i];
+    }
+  }
+  finalize_partials(nb, j + 1, true, pstride, partial, s);  // s[0..j], s[j+1] = nu
+  __syncthreads();
   if (threadIdx.x == 0) {
     const double nu = s[j1];
     const double alpha = j1 < n ? (double)w[j1] : 0.0;
@@ -314,45 +341,42 @@ __global__ __launch_bounds__(256) void hh_newcol_small_kernel(int nb, int pstrid
     misc[MS_ALPHA] = alpha;
     sc[0] = dd;
     sc[1] = scale;
-    sc[2] = alpha;
+    // e = U_{j+1}^T e_{j+1}: row j+1 of the old columns, then u_{j+1}[j+1]
+    e[j1] = (alpha + dd) * scale;
   }
   __syncthreads();
-  const double dd = sc[0], scale = sc[1], alpha = sc[2];
-  // e = U_{j+1}^T e_{j+1}: row j+1 of the old columns, then u_{j+1}[j+1]
-  for (int k = threadIdx.x; k <= j1; k += 256) {
-    if (k <= j) e[k] = j1 < n ? (double)U[(size_t)k * ld + j1] : 0.0;
-    else e[k] = (alpha + dd) * scale;
-  }
-  __syncthreads();
+  const double dd = sc[0], scale = sc[1];
   // utu = U_j^T u_{j+1} = scale * (s + dd * e[0..j]); T[0..j, j+1] = -2 T_j utu
-  for (int l = threadIdx.x; l <= j; l += 256) {
-    double v = 0.0;
-    for (int k = l; k <= j; k++) v += T[l + k * ldT] * (scale * (s[k] + dd * e[k]));
-    T[l + j1 * ldT] = -2.0 * v;
+  for (int l = threadIdx.x; l <= j1; l += 256) {
+    double v = 2.0;
+    if (l <= j) {
+      v = 0.0;
+      for (int k = l; k <= j; k++) v += Ts[l + k * ldT] * (scale * (s[k] + dd * e[k]));
+      v = -2.0 * v;
+    }
+    Ts[l + j1 * ldT] = v;
+    T[l + j1 * ldT] = v;
   }
-  if (threadIdx.x == 0) T[j1 + j1 * ldT] = 2.0;
   __syncthreads();
   // y' = T_{j+1} e
   double* yq = small + lay.y();
   for (int l = threadIdx.x; l <= j1; l += 256) {
     double v = 0.0;
-    for (int k = l; k <= j1; k++) v += T[l + k * ldT] * e[k];
+    for (int k = l; k <= j1; k++) v += Ts[l + k * ldT] * e[k];
     yq[l] = v;
   }
   if (threadIdx.x == 0) {
-    double* R = small + lay.R();
-    double* cs = small + lay.cs();
-    double* sn = small + lay.sn();
+    double* csw = small + lay.cs();
+    double* snw = small + lay.sn();
     double* g = small + lay.g();
-    const int ldR = lay.m + 1;
-    // Hessenberg column: h_i = w'_i (i <= j), h_{j+1} = -dd
-    for (int i = 0; i <= j; i++) R[i + j * ldR] = (double)w[i];
+    // Hessenberg column: h_i = w'_i (i <= j), h_{j+1} = -dd; apply the
+    // previous rotations
     for (int i = 0; i < j; i++) {
-      const double t0 = R[i + j * ldR], t1 = R[i + 1 + j * ldR];
-      R[i + 1 + j * ldR] = -sn[i] * t0 + cs[i] * t1;
-      R[i + j * ldR] = cs[i] * t0 + sn[i] * t1;
+      const double t0 = h[i], t1 = h[i + 1];
+      h[i + 1] = -sz[i] * t0 + cz[i] * t1;
+      h[i] = cz[i] * t0 + sz[i] * t1;
     }
-    const double a = R[j + j * ldR], bb = -dd;
+    const double a = h[j], bb = -dd;
     const double tol = 4.9406564584124654e-324;  // nextafter(0,1)
     double c, sn_;
     if (fabs(bb) <= tol) {
@@ -367,17 +391,23 @@ __global__ __launch_bounds__(256) void hh_newcol_small_kernel(int nb, int pstrid
       c = 1.0 / sqrt(1.0 + t * t);
       sn_ = t * c;
     }
-    cs[j] = c;
-    sn[j] = sn_;
-    R[j + j * ldR] = c * a - sn_ * dd;
-    g[j1] = -sn_ * g[j];
-    g[j] = c * g[j];
+    csw[j] = c;
+    snw[j] = sn_;
+    h[j] = c * a - sn_ * dd;
+    const double gj = g[j];
+    g[j1] = -sn_ * gj;
+    g[j] = c * gj;
     const double res = fabs(g[j1]);
     misc[MS_RES] = res;
     misc[MS_NCOLS] = (double)j1;
     if (!(res >= misc[MS_RTOL]) || !isfinite(res)) misc[MS_DONE] = 1.0;
     if (!isfinite(res)) misc[MS_BREAK] = 1.0;
   }
+  __syncthreads();
+  // rotated Hessenberg column -> R
+  double* R = small + lay.R();
+  const int ldR = lay.m + 1;
+  for (int i = threadIdx.x; i <= j; i += 256) R[i + j * ldR] = h[i];
 }
 
 // u_{j+1} = (w + dd e_{j+1}) * scale on rows >= j+1 (0 above);

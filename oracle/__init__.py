@@ -705,3 +705,57 @@ def btrsv(T: BCSR, r, upper: bool, jac: int = 0) -> np.ndarray:
                     _ptr(np.ascontiguousarray(T.val.reshape(-1), dtype=np.float32)),
                     1 if upper else 0, jac, _ptr(r), _ptr(y))
     return y
+
+
+# ---------------------------------------------------------------- orderings
+def cuthill_mckee(A: CSR, reverse: bool = False):
+    """Restatement of src/cuthill_mckee.cpp:53-265 written from the reference
+    text with its indexing as-is: 1-offset neighbour store (:100-115), the
+    window [nnsn[seed-1], nnsn[seed]) (:155-157, nnsn[-1] taken as 0), seed
+    scan starting from row 1's degree (:130-142), queue sorted stably by the
+    degree of node v+1 descending (:183-189), position-swap reverse
+    (:241-252).  Returns nn_map, or None where the reference would read an
+    uninitialised queue entry."""
+    nn = A.n_rows
+    nnsn = [0] * (nn + 1)
+    nsn = [0] * (max(A.nnz, 1) + 1)
+    for i in range(nn):
+        offset = nnsn[i - 1] if i > 0 else 0
+        for j in range(A.row[i], A.row[i + 1]):
+            if A.col[j] != i:
+                nnsn[i] += 1
+                nsn[offset + nnsn[i]] = int(A.col[j])
+        if i > 0:
+            nnsn[i] += nnsn[i - 1]
+    seed, maxdegree, degree = 0, nnsn[1] - nnsn[0], nnsn[0]
+    for i in range(nn):
+        if i > 0:
+            degree = nnsn[i] - nnsn[i - 1]
+        if degree > maxdegree:
+            maxdegree, seed = degree, i
+    nn_map = [-1] * nn
+    queue = [0] * nn
+    inode = 0
+    nn_map[seed] = inode; inode += 1; queue[seed] = 1
+    S = []
+    start_s = 0
+    while inode < nn:
+        lo = nnsn[seed - 1] if seed > 0 else 0
+        R = []
+        for i in range(lo, nnsn[seed]):
+            if queue[nsn[i]] == 0:
+                R.append(nsn[i]); queue[nsn[i]] = 1
+        R.sort(key=lambda v: -(nnsn[v + 1] - nnsn[v]))  # stable
+        for v in R:
+            nn_map[v] = inode; inode += 1; S.append(v)
+        if inode >= nn:
+            break
+        if start_s >= len(S):
+            return None
+        seed = S[start_s]; start_s += 1
+    if reverse:
+        counter = nn - 1
+        for i in range(nn // 2 + 1):
+            nn_map[counter], nn_map[i] = nn_map[i], nn_map[counter]
+            counter -= 1
+    return np.array(nn_map, dtype=np.int32)
