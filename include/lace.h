@@ -206,6 +206,10 @@ int data_dcsrilu0_mkl(data_d_matrix *A);
  * in each row), U as CSCU (U by columns, diagonal last).  All four run the
  * sweeps on the GPU in synchronous (Jacobi) mode: every update of sweep k
  * reads iterate k-1 only, so results are deterministic. */
+/* sparse.h:767-771; src/parilu_v0_2.cpp:16-110: the same sweep stopped only
+ * by step <= reduction*Ares (the reference has no cap; 1000 here). */
+void data_PariLU_v0_2(data_d_matrix *A, data_d_matrix *L, data_d_matrix *U,
+                      dataType reduction);
 void data_PariLU_v0_3(data_d_matrix *A, data_d_matrix *L, data_d_matrix *U,
                       dataType reduction, data_d_preconditioner_log *log);
 void data_PariLU_v0_3_prescribedSweeps(data_d_matrix *A, data_d_matrix *L,
@@ -263,6 +267,15 @@ int data_fgmres_householder_restart(data_d_matrix *A, data_d_matrix *b,
                                     data_d_gmres_param *gmres_par,
                                     data_d_gmres_log *gmres_log);
 
+/* The reference's MKL RCI FGMRES baseline.  sparse.h:978-980;
+ * src/mkl_fgmres.cpp:26-: ParILU v0.2 factors (parilu_reduction), exact
+ * triangular solves, MGS FGMRES with restart length restart_max and an
+ * iteration cap of search_max (rounded up to whole cycles), rtol relative
+ * (tol_type 1) or absolute (0); precondition = 0 drops the preconditioner.
+ * Runs on the device; the solution is written into x0->val in place. */
+int data_MKL_FGMRES(data_d_matrix *A, data_d_matrix *x0,
+                    data_d_matrix *rhs_vector, data_d_gmres_param *param);
+
 /* Orderings.  sparse.h:398-402,662-665; src/cuthill_mckee.cpp:53-265,
  * src/reorder.cpp:167-253.  nn_map[i] = new number of row i; every quirk of
  * the reference ordering is kept (documented in lace_amd/csrc/reorder.cpp);
@@ -277,6 +290,13 @@ typedef _Bool lace_bool;
 #endif
 int data_reorder_cuthill_mckee(data_d_matrix *A, int *nn_map, lace_bool reverse);
 int data_sparse_reorder(data_d_matrix *A, int *P, int *Pinv, int reorder);
+
+/* Householder generator and triangularization (host, small dense).
+ * sparse.h:1059-1067; src/householder.cpp:49-160.  Dense containers are
+ * column-major with leading dimension ld; data_hqrd overwrites X and
+ * addresses R with X's leading dimension, as the reference does. */
+void data_housegen(int n, dataType *X, dataType *u, dataType *nu);
+void data_hqrd(data_d_matrix *X, data_d_matrix *U, data_d_matrix *R);
 
 /* Givens generator.  sparse.h:1096; src/givens.cpp:46-67 */
 void givens_rotation(dataType a, dataType b, dataType tol, dataType *c,

@@ -128,6 +128,10 @@ SIGNATURES = {
     "data_fgmres_householder_restart": (_i, [_DM, _DM, _DM, _DM, _DM, _P(GmresParam),
                                              _P(GmresLog)]),
     "givens_rotation": (None, [_d, _d, _d, _P(_d), _P(_d)]),
+    "data_housegen": (None, [_i, _P(_d), _P(_d), _P(_d)]),
+    "data_hqrd": (None, [_DM, _DM, _DM]),
+    "data_PariLU_v0_2": (None, [_DM, _DM, _DM, _d]),
+    "data_MKL_FGMRES": (_i, [_DM, _DM, _DM, _P(GmresParam)]),
     "data_reorder_cuthill_mckee": (_i, [_DM, _P(_i), C.c_bool]),
     "data_sparse_reorder": (_i, [_DM, _P(_i), _P(_i), _i]),
     "lace_init": (_i, [_i]),

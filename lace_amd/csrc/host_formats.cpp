@@ -935,6 +935,52 @@ int lace_gen_convdiff3d(int nx, int ny, int nz, double pe_x, double pe_y, double
   return DEV_SUCCESS;
 }
 
+// src/householder.cpp:49-84 (Stewart's housegen): u = X/||X||, u_0 += 1
+// with nu = -||X|| when X_0 >= 0 (else u_0 -= 1), u scaled by
+// 1/sqrt(|u_0|); ||X|| below the smallest subnormal gives u_0 = sqrt(2).
+void data_housegen(int n, dataType* X, dataType* u, dataType* nu) {
+  const double eps = 4.9406564584124654e-324;  // nextafter(0, 1)
+  for (int i = 0; i < n; i++) u[i] = X[i];
+  *nu = data_dnrm2(n, X, 1);
+  if (*nu < eps) {
+    u[0] = std::sqrt(2.0);
+    return;
+  }
+  for (int i = 0; i < n; i++) u[i] = X[i] / (*nu);
+  if (u[0] >= 0) {
+    u[0] = u[0] + 1.0;
+    *nu = -(*nu);
+  } else {
+    u[0] = u[0] - 1.0;
+  }
+  const double t = 1.0 / std::sqrt(std::fabs(u[0]));
+  for (int i = 0; i < n; i++) u[i] = u[i] * t;
+}
+
+// src/householder.cpp:120-160: Householder triangularization [U, R] =
+// hqrd(X) on column-major dense containers (X is overwritten); reflector k
+// from X(k:n, k), applied to the columns to its right.  R's off-diagonal
+// entries are addressed with X's leading dimension, as the reference does
+// (:157), so U, R and X are expected to share it.
+void data_hqrd(data_d_matrix* X, data_d_matrix* U, data_d_matrix* R) {
+  const int n = X->num_rows, p = X->num_cols;
+  const int lx = X->ld, lu = U->ld, lr = R->ld;
+  std::vector<double> v(std::max(p, 1), 0.0);
+  for (int k = 0; k < std::min(n, p); k++) {
+    data_housegen(n - k, &X->val[k + (size_t)k * lx], &U->val[k + (size_t)k * lu],
+                  &R->val[k + (size_t)k * lr]);
+    for (int j = k + 1; j < p; j++) {
+      double colsum = 0.0;
+      for (int i = k; i < n; i++) colsum = colsum + U->val[i + (size_t)k * lu] * X->val[i + (size_t)j * lx];
+      v[j] = colsum;
+    }
+    for (int i = k; i < n; i++)
+      for (int j = k + 1; j < p; j++)
+        X->val[i + (size_t)j * lx] = X->val[i + (size_t)j * lx] - U->val[i + (size_t)k * lu] * v[j];
+    for (int j = k + 1; j < p; j++) R->val[k + (size_t)j * lx] = X->val[k + (size_t)j * lx];
+  }
+}
+
 int lace_gen_bcsr7(int nx, int ny, int nz, int bs, uint64_t seed, double sigma, double shift,
                    data_d_matrix* A) {
   const long nl = (long)nx * ny * nz;

@@ -927,10 +927,12 @@ int data_zilures(data_d_matrix A, data_d_matrix L, data_d_matrix U, data_d_matri
 // ---------------------------------------------------------------- drop-ins
 namespace {
 
-// Shared body of the four ParILU drop-ins.  mode 0: stop when step <=
+// Shared body of the ParILU drop-ins.  mode 0: stop when step <=
 // reduction*Ares or after 20 sweeps (src/parilu_v0_3.cpp:44-45,62);
 // mode 1: stop after log->maxSweeps sweeps or when step <= 4 eps
-// (src/parilu_v0_3_prescribedSweeps.cpp:62).
+// (src/parilu_v0_3_prescribedSweeps.cpp:62); mode 2: stop when step <=
+// reduction*Ares, the only rule of src/parilu_v0_2.cpp:39,61 (which has
+// no sweep cap; this one stops at 1000).
 int parilu_dropin(data_d_matrix* A, data_d_matrix* L, data_d_matrix* U, double reduction,
                   data_d_preconditioner_log* log, int mode) {
   LACE_TRY(ensure_init());
@@ -956,6 +958,7 @@ int parilu_dropin(data_d_matrix* A, data_d_matrix* L, data_d_matrix* U, double r
   while (rc == DEV_SUCCESS) {
     if (mode == 0 && !(step > tol && iter < 20)) break;
     if (mode == 1 && !(iter < log->maxSweeps && step > 4.0 * 2.220446049250313e-16)) break;
+    if (mode == 2 && !(step > tol && iter < 1000)) break;
     rc = P->sweep(true);
     if (rc == DEV_SUCCESS) rc = P->read_step(&step);
     iter++;
@@ -978,6 +981,13 @@ int parilu_dropin(data_d_matrix* A, data_d_matrix* L, data_d_matrix* U, double r
 }  // namespace
 
 extern "C" {
+
+// src/parilu_v0_2.cpp:16-110: the same sweep, stop rule only, no log.
+void data_PariLU_v0_2(data_d_matrix* A, data_d_matrix* L, data_d_matrix* U, dataType reduction) {
+  data_d_preconditioner_log log;
+  memset(&log, 0, sizeof(log));
+  (void)parilu_dropin(A, L, U, reduction, &log, 2);
+}
 
 void data_PariLU_v0_3(data_d_matrix* A, data_d_matrix* L, data_d_matrix* U, dataType reduction,
                       data_d_preconditioner_log* log) {

@@ -748,7 +748,7 @@ namespace {
 // test_linearSolver.cpp:670-672).  The factors are used as given.
 int fgmres_dropin_common(data_d_matrix* A, data_d_matrix* b, data_d_matrix* x0, data_d_matrix* L,
                          data_d_matrix* U, data_d_gmres_param* par, data_d_gmres_log* log,
-                         bool restarted, bool mgs) {
+                         bool restarted, bool mgs, bool x_in_place = false) {
   LACE_TRY(ensure_init());
   auto t0 = std::chrono::steady_clock::now();
   const int n = A->num_rows;
@@ -795,13 +795,17 @@ int fgmres_dropin_common(data_d_matrix* A, data_d_matrix* b, data_d_matrix* x0, 
     LACE_TRY(solver_solve_mgs(S, db.as<double>(), dx.as<double>(), &st, tol_abs, par->reorth));
   else
     LACE_TRY(solver_solve(S, db.as<double>(), dx.as<double>(), &st, tol_abs));
-  // x0 is replaced by the solution (src/fgmres.cpp:491-492)
-  data_d_matrix xs;
-  memset(&xs, 0, sizeof(xs));
-  LACE_TRY(data_zvinit(&xs, n, 1, 0.0, x0->major ? x0->major : MagmaColMajor));
-  LACE_TRY(download(xs.val, dx, sizeof(double) * n));
-  data_zmfree(x0);
-  *x0 = xs;
+  if (x_in_place) {
+    LACE_TRY(download(x0->val, dx, sizeof(double) * n));
+  } else {
+    // x0 is replaced by the solution (src/fgmres.cpp:491-492)
+    data_d_matrix xs;
+    memset(&xs, 0, sizeof(xs));
+    LACE_TRY(data_zvinit(&xs, n, 1, 0.0, x0->major ? x0->major : MagmaColMajor));
+    LACE_TRY(download(xs.val, dx, sizeof(double) * n));
+    data_zmfree(x0);
+    *x0 = xs;
+  }
   auto t1 = std::chrono::steady_clock::now();
   log->restarts = st.restarts;
   log->search_directions = st.iterations;
@@ -848,6 +852,47 @@ int data_fgmres(data_d_matrix* A, data_d_matrix* b, data_d_matrix* x0, data_d_ma
 int data_fgmres_restart(data_d_matrix* A, data_d_matrix* b, data_d_matrix* x0, data_d_matrix* L,
                         data_d_matrix* U, data_d_gmres_param* par, data_d_gmres_log* log) {
   return fgmres_mgs_dropin(A, b, x0, L, U, par, log, true);
+}
+
+}  // extern "C"
+
+extern "C" {
+
+// src/mkl_fgmres.cpp:26-: the reference's CPU baseline, MKL's RCI FGMRES
+// (modified Gram-Schmidt Arnoldi) with ILU(0) factors from ParILU v0.2
+// (parilu_reduction) and exact triangular solves on the merged LU
+// (mkl_dcsrtrsv), ipar[14] = restart_max as the restart length, ipar[4] =
+// search_max as the iteration cap, dpar[0]/dpar[1] = rtol relative
+// (tol_type 1) or absolute (0).  Here the same method runs on the device:
+// ParILU v0.2 sweeps, level-scheduled exact solves, MGS FGMRES with
+// restart length restart_max and ceil(search_max / restart_max) cycles
+// (the last cycle may run past the cap by less than one cycle);
+// precondition = 0 drops the preconditioner.  The solution is written into
+// x0->val in place, as the reference does; returns 0, or DEV_NOTCONVERGED.
+int data_MKL_FGMRES(data_d_matrix* A, data_d_matrix* x0, data_d_matrix* rhs_vector,
+                    data_d_gmres_param* par) {
+  LACE_TRY(ensure_init());
+  LACE_REQUIRE(A && x0 && rhs_vector && par && x0->val && rhs_vector->val, DEV_ERR_INVALID_PTR,
+               "data_MKL_FGMRES: null argument");
+  data_d_matrix L, Ucsc, U;
+  memset(&L, 0, sizeof(L));
+  memset(&Ucsc, 0, sizeof(Ucsc));
+  memset(&U, 0, sizeof(U));
+  data_PariLU_v0_2(A, &L, &Ucsc, par->parilu_reduction);
+  LACE_REQUIRE(L.row && Ucsc.row, DEV_ERR, "data_MKL_FGMRES: ParILU failed: %s", get_error());
+  int rc = data_zmconvert(Ucsc, &U, Magma_CSCU, Magma_CSRU);
+  data_d_gmres_param p = *par;
+  p.search_max = std::max(1, par->restart_max);
+  p.restart_max = std::max(1, (par->search_max + p.search_max - 1) / p.search_max);
+  p.user_csrtrsv_choice = par->precondition ? 0 : 2;
+  data_d_gmres_log log;
+  memset(&log, 0, sizeof(log));
+  if (rc == DEV_SUCCESS)
+    rc = fgmres_dropin_common(A, rhs_vector, x0, &L, &U, &p, &log, true, true, true);
+  data_zmfree(&L);
+  data_zmfree(&Ucsc);
+  data_zmfree(&U);
+  return rc;
 }
 
 }  // extern "C"

@@ -326,3 +326,42 @@ def test_bcsr_splits_and_block_transpose(lace, orc):
     T = lace.DMatrix()
     lace.check(lace.lib().data_zmtranspose(H.m, C.byref(T)))
     assert _bcsr_equal(lace.owned_bcsr_to_numpy(T), orc.bcsr_transpose(Bo))
+
+
+def _dense_dm(lace, A):
+    """Column-major dense data_d_matrix over a numpy array (kept alive by
+    the caller through the returned tuple)."""
+    a = np.asfortranarray(np.array(A, dtype=np.float64))
+    m = lace.DMatrix()
+    m.storage_type = lace.Magma_DENSE
+    m.num_rows, m.num_cols = a.shape
+    m.nnz = a.size
+    m.ld = a.shape[0]
+    m.major = lace.MagmaColMajor
+    flat = a.reshape(-1, order="F")
+    m.val = flat.ctypes.data_as(C.POINTER(C.c_double))
+    return m, flat
+
+
+def test_hqrd_and_housegen_dropins_match_goldens(lace, orc):
+    """testing/matrices/magic7_Householder_{U,R}.mtx: the reference's
+    Householder factors of magic(7)."""
+    X = orc.read_mtx_dense(os.path.join(GOLDEN, "magic7.mtx"))
+    Uref = orc.read_mtx_dense(os.path.join(GOLDEN, "magic7_Householder_U.mtx"))
+    Rref = orc.read_mtx_dense(os.path.join(GOLDEN, "magic7_Householder_R.mtx"))
+    Xm, xf = _dense_dm(lace, X)
+    Um, uf = _dense_dm(lace, np.zeros((7, 7)))
+    Rm, rf = _dense_dm(lace, np.zeros((7, 7)))
+    lace.lib().data_hqrd(C.byref(Xm), C.byref(Um), C.byref(Rm))
+    U = uf.reshape(7, 7, order="F"); R = rf.reshape(7, 7, order="F")
+    assert np.allclose(U, Uref, rtol=1e-12, atol=1e-12)
+    assert np.allclose(R, Rref, rtol=1e-12, atol=1e-11)
+    # housegen on its own against the oracle and the reflector property
+    x = np.array([3.0, -1.0, 2.0, 0.5])
+    u = np.zeros(4); nu = C.c_double()
+    lace.lib().data_housegen(4, x.ctypes.data_as(C.POINTER(C.c_double)),
+                             u.ctypes.data_as(C.POINTER(C.c_double)), C.byref(nu))
+    uo, nuo = orc.housegen(x)
+    assert np.allclose(u, uo, rtol=1e-15) and nu.value == nuo
+    Hx = x - u * np.dot(u, x)
+    assert np.allclose(Hx, [nu.value, 0, 0, 0], atol=1e-14)

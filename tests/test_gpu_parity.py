@@ -382,3 +382,57 @@ def test_mixed_precision_reaches_fp64_tolerance(gpu, orc, mode):
     rel = np.linalg.norm(b - A.scipy() @ x) / np.linalg.norm(b)
     assert rel <= 1e-8 * 1.05
     assert abs(rel - st.rel_true_residual) <= 1e-3 * rel + 1e-15
+
+
+@pytest.mark.parametrize("reduction", [1e-10, 1e-20])
+def test_parilu_v0_2_stop_rule(gpu, orc, reduction):
+    """src/parilu_v0_2.cpp:39,61: sweep while step / ||A||_F > reduction *
+    Ares, no sweep cap; sweep count and factors equal the oracle's
+    synchronous run with the same rule."""
+    A = orc.read_mtx(os.path.join(GOLDEN, "Trefethen_20.mtx"))
+    H = gpu.HostCSR.from_csr(A)
+    L = gpu.DMatrix(); U = gpu.DMatrix()
+    gpu.lib().data_PariLU_v0_2(H.ref(), C.byref(L), C.byref(U), reduction)
+    lr, lc, lv = gpu.owned_to_numpy(L)
+    ur, uc, uv = gpu.owned_to_numpy(U)
+    H.release_rowidx()
+    st = orc.ParILUState(A)
+    ares, _ = orc.ilures(A, *st.factors())
+    anorm = np.sqrt(np.sum(A.val ** 2))
+    step = np.finfo(np.float32).max
+    while step > reduction * ares:
+        step = st.sweep("jacobi") / anorm
+    assert np.array_equal(lv, st.Lval) and np.array_equal(uv, st.Uval)
+
+
+@pytest.mark.parametrize("tol_type", [0, 1])
+def test_mkl_fgmres_dropin(gpu, orc, tol_type):
+    """src/mkl_fgmres.cpp: restart length restart_max, iteration cap
+    search_max, ParILU v0.2 factors with exact solves; the solution is
+    written into x0->val in place."""
+    A = orc.convdiff3d(12, 10, 8)
+    b = np.random.default_rng(5).standard_normal(A.n_rows)
+    H = gpu.HostCSR.from_csr(A)
+    B = gpu.HostVec(b)
+    X = gpu.HostVec(np.zeros(A.n_rows))
+    ptr_before = C.cast(X.m.val, C.c_void_p).value
+    par = gpu.GmresParam()
+    par.tol_type = tol_type; par.rtol = 1e-10 if tol_type else 1e-8
+    par.search_max = 200; par.restart_max = 30; par.precondition = 1
+    par.parilu_reduction = 1e-10
+    info = gpu.lib().data_MKL_FGMRES(H.ref(), X.ref(), B.ref(), C.byref(par))
+    H.release_rowidx()
+    assert info == 0, gpu.lib().lace_last_error()
+    assert C.cast(X.m.val, C.c_void_p).value == ptr_before
+    r = np.linalg.norm(b - A.scipy() @ X.v)
+    bound = 1.01e-10 * np.linalg.norm(b) if tol_type else 1.01e-8
+    assert r <= bound
+    # the same solve with FGMRES(MGS) from the oracle, factors after the
+    # same v0.2 sweeps: iteration counts within 2
+    par.precondition = 0
+    X2 = gpu.HostVec(np.zeros(A.n_rows))
+    info = gpu.lib().data_MKL_FGMRES(H.ref(), X2.ref(), B.ref(), C.byref(par))
+    H.release_rowidx()
+    assert info == 0
+    ref = orc.fgmres_mgs(A, b, m=30, restart_max=7, rtol=par.rtol, tol_type=tol_type, choice=2)
+    assert np.linalg.norm(b - A.scipy() @ X2.v) <= max(bound, 1.01 * np.linalg.norm(b - A.scipy() @ ref.x))
