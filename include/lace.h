@@ -416,6 +416,18 @@ int lace_bilu_set_variant(lace_bilu *B, int variant);
 /* 1 if A's block pattern is structurally symmetric. */
 int lace_bilu_is_symmetric(const lace_bilu *B);
 
+/* Tall-skinny FP64 products on MFMA (n x 32 row-major device blocks; the
+ * multi-RHS layout).  blocks: host array of nb device pointers (nb <= 40).
+ * tn: C[b] (32 x 32 row-major) = X_b^T W over rows [rs_b, n) with
+ *     rs_b = rs_main for b < nb-1 and rs_last for the last block.
+ * nn: out = base - sum_b X_b D_b on rows [rlo, n), base on rows below;
+ *     D holds nb consecutive 32 x 32 row-major blocks; base_mode 0 zero,
+ *     1 the block base, 2 the identity block column starting at row e0. */
+int lace_tsgemm_tn(int n, int nb, const double *const *blocks, const double *W,
+                   int rs_main, int rs_last, double *C);
+int lace_tsgemm_nn(int n, int nb, const double *const *blocks, const double *D,
+                   const double *base, int base_mode, int e0, int rlo, double *out);
+
 /* Solver configuration. */
 typedef struct lace_solver_opts {
   int    krylov;        /* 0 = Householder FGMRES, 1 = MGS FGMRES */

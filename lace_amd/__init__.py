@@ -180,6 +180,8 @@ SIGNATURES = {
     "lace_solver_solve_trans": (_i, [_vp, _vp, _vp, _i, _P(SolverStats)]),
     "lace_solver_set_values": (_i, [_vp, _P(_d)]),
     "lace_set_option": (_i, [C.c_char_p, _i]),
+    "lace_tsgemm_tn": (_i, [_i, _i, _P(_vp), _vp, _i, _i, _vp]),
+    "lace_tsgemm_nn": (_i, [_i, _i, _P(_vp), _vp, _vp, _i, _i, _i, _vp]),
     "lace_timing_enable": (_i, [_i]),
     "lace_timing_reset": (_i, []),
     "lace_timing_only": (_i, [C.c_char_p]),
