@@ -484,6 +484,58 @@ int lace_solver_set_values(lace_solver *S, const double *vals);
 int lace_solver_grad_values(lace_solver *S, const double *lambda,
                             const double *x, double *dtheta);
 
+/* ---- 32 right-hand sides at once (no reference counterpart: the
+ * reference solves one right-hand side per call, src/fgmres.cpp:64 and
+ * src/fgmres_householder_restart.cpp:63).  Blocks are n x 32 device arrays,
+ * row-major: row i holds entry i of the 32 right-hand sides. */
+#define LACE_NRHS 32
+
+typedef struct lace_block_stats {
+  int    iterations;        /* block Arnoldi steps over all restarts */
+  int    restarts;          /* restart cycles started */
+  int    panel_passes;      /* CholeskyQR passes over all panels */
+  int    shifted_panels;    /* panels that needed a shifted first pass */
+  int    converged;         /* every column reached rtol */
+  double max_rel_true_residual;
+  double rel_true_residual[LACE_NRHS];  /* ||b_c - A x_c|| / ||b_c|| */
+  double initial_residual[LACE_NRHS];   /* ||b_c - A x0_c|| */
+  double solve_ms;
+} lace_block_stats;
+
+/* Block Householder FGMRES(search_max) on the operator of S (trans = 0: A;
+ * 1: A^T with M^{-T}): 32 right-hand sides share one block Krylov space.
+ * Per block step: preconditioned block Z_k = M^{-1} V_k (Jacobi sweeps of
+ * the ILU factors in their precision, Z stored in it), W = A Z_k (FP64 A),
+ * the FP64 block reflectors applied to W as one compact-WY product (MFMA
+ * Gram X^T W and update W - X D), the new panel's Householder QR by
+ * CholeskyQR2 (three passes after a shifted first one) and reconstruction,
+ * and the block Hessenberg least squares by 64 x 32 Householder windows on
+ * device.  Column c converges when its residual estimate is <= rtol *
+ * ||b_c - A x0_c||; the solve stops when all have.  X is the initial guess
+ * on entry and the solution on exit.  Needs trsv = Jacobi (or none), no
+ * inner iterations, n >= 32 (search_max + 2), search_max <= 39. */
+int lace_solver_solve_block(lace_solver *S, const double *B, double *X, int trans,
+                            lace_block_stats *st);
+/* dtheta[k] = -sum_c Lambda[row(k)][c] * X[col(k)][c]: the gradient of
+ * sum_c L_c with respect to A's stored values, for A X = B and
+ * A^T Lambda = [dL_c/dx_c]. */
+int lace_solver_grad_values_block(lace_solver *S, const double *Lambda, const double *X,
+                                  double *dtheta);
+/* Y = A X (B == NULL) or Y = B - A X on n x 32 blocks; A FP64; X FP64
+ * (x_prec LACE_F64) or FP32 (LACE_F32). */
+int lace_csr_spmm32(const lace_csr *A, const void *X, lace_prec_t x_prec, const double *B,
+                    double *Y);
+/* Z = M^{-1} R (trans 0) or M^{-T} R (trans 1) on n x 32 blocks in the
+ * factor precision (Jacobi sweeps or kind NONE). */
+int lace_parilu_apply_block(lace_parilu *P, lace_trsv_kind_t kind, int jacobi_sweeps, int trans,
+                            const void *R, void *Z);
+/* Householder QR of rows [r0, n) of the n x 32 block W (device): writes
+ * Y (n x 32: zero above r0, unit lower triangle at rows r0..r0+31), Tb
+ * and Rt (32 x 32 row-major) with (I - Y Tb Y^T) [Rt; 0] = W[r0:, :];
+ * passes = CholeskyQR passes used. */
+int lace_block_panel_qr(int n, int r0, const double *W, double *Y, double *Tb, double *Rt,
+                        int *passes);
+
 /* Kernel-shape options for tuning runs.  "stream_rows_per_thread" (1, 2 or
  * 4): rows each thread owns in the CSR-stream SpMV and Jacobi sweeps
  * (default: 2 for FP32 values, 1 for FP64). */

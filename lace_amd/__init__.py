@@ -90,8 +90,20 @@ class SolverStats(C.Structure):
                 ("setup_ms", C.c_double), ("solve_ms", C.c_double), ("converged", C.c_int)]
 
 
+NRHS = 32
+
+
+class BlockStats(C.Structure):
+    _fields_ = [("iterations", C.c_int), ("restarts", C.c_int), ("panel_passes", C.c_int),
+                ("shifted_panels", C.c_int), ("converged", C.c_int),
+                ("max_rel_true_residual", C.c_double),
+                ("rel_true_residual", C.c_double * NRHS),
+                ("initial_residual", C.c_double * NRHS), ("solve_ms", C.c_double)]
+
+
 assert C.sizeof(PrecondLog) == 128 and C.sizeof(GmresLog) == 64 and C.sizeof(GmresParam) == 64
 assert C.sizeof(SolverOpts) == 48 and C.sizeof(SolverStats) == 72
+assert C.sizeof(BlockStats) == 24 + 8 + 2 * 8 * NRHS + 8
 
 _P = C.POINTER
 _DM = _P(DMatrix)
@@ -180,6 +192,11 @@ SIGNATURES = {
     "lace_solver_solve_trans": (_i, [_vp, _vp, _vp, _i, _P(SolverStats)]),
     "lace_solver_set_values": (_i, [_vp, _P(_d)]),
     "lace_set_option": (_i, [C.c_char_p, _i]),
+    "lace_solver_solve_block": (_i, [_vp, _vp, _vp, _i, _P(BlockStats)]),
+    "lace_solver_grad_values_block": (_i, [_vp, _vp, _vp, _vp]),
+    "lace_csr_spmm32": (_i, [_vp, _vp, _i, _vp, _vp]),
+    "lace_parilu_apply_block": (_i, [_vp, _i, _i, _i, _vp, _vp]),
+    "lace_block_panel_qr": (_i, [_i, _i, _vp, _vp, _vp, _vp, _P(_i)]),
     "lace_tsgemm_tn": (_i, [_i, _i, _P(_vp), _vp, _i, _i, _vp]),
     "lace_tsgemm_nn": (_i, [_i, _i, _P(_vp), _vp, _vp, _i, _i, _i, _vp]),
     "lace_timing_enable": (_i, [_i]),
@@ -439,12 +456,75 @@ class ParILU:
             raise LaceError("sync-free solve exceeded its poll budget")
         return dz.download()
 
+    def apply_block(self, R: np.ndarray, kind=TRSV_JACOBI, sweeps=5, trans=0):
+        """Z = M^{-1} R (or M^{-T} R) for an (n, 32) block in the factor
+        precision."""
+        dt = np.float64 if self.prec == LACE_F64 else np.float32
+        R = np.ascontiguousarray(R, dtype=dt)
+        dr = DevVec(R.size, dt, R.reshape(-1))
+        dz = DevVec(R.size, dt)
+        check(lib().lace_parilu_apply_block(self.h, kind, sweeps, trans, dr.ptr, dz.ptr),
+              "apply_block")
+        return dz.download().reshape(R.shape)
+
     def __del__(self):
         try:
             if self.h:
                 lib().lace_parilu_destroy(self.h)
         except Exception:
             pass
+
+
+class DevCSRMatrix:
+    """Device CSR matrix (lace_csr) for the product kernels."""
+
+    def __init__(self, A: HostCSR, prec=LACE_F64):
+        self.h = C.c_void_p()
+        check(lib().lace_csr_create(A.ref(), prec, C.byref(self.h)), "csr_create")
+        self.n = A.m.num_rows
+        self.prec = prec
+
+    def spmm32(self, X: np.ndarray, B=None) -> np.ndarray:
+        """A X, or B - A X, for (n, 32) blocks; X FP64 or FP32."""
+        xp = LACE_F64 if X.dtype == np.float64 else LACE_F32
+        X = np.ascontiguousarray(X)
+        dx = DevVec(X.size, X.dtype, X.reshape(-1))
+        dy = DevVec(self.n * NRHS, np.float64)
+        db = None
+        if B is not None:
+            db = DevVec(self.n * NRHS, np.float64, np.ascontiguousarray(B, np.float64).reshape(-1))
+        check(lib().lace_csr_spmm32(self.h, dx.ptr, xp, db.ptr if db is not None else None,
+                                    dy.ptr), "spmm32")
+        return dy.download().reshape(self.n, NRHS)
+
+    def spmv(self, x: np.ndarray) -> np.ndarray:
+        dt = np.float64 if self.prec == LACE_F64 else np.float32
+        dx = DevVec(self.n, dt, x)
+        dy = DevVec(self.n, dt)
+        check(lib().lace_csr_spmv(self.h, 1.0, dx.ptr, 0.0, dy.ptr), "spmv")
+        return dy.download()
+
+    def __del__(self):
+        try:
+            if self.h:
+                lib().lace_csr_destroy(self.h)
+        except Exception:
+            pass
+
+
+def block_panel_qr(W: np.ndarray, r0: int):
+    """Householder QR of rows r0.. of an (n, 32) FP64 block on the device:
+    returns (Y, Tb, Rt, passes)."""
+    n = W.shape[0]
+    W = np.ascontiguousarray(W, np.float64)
+    dW = DevVec(W.size, np.float64, W.reshape(-1))
+    dY = DevVec(W.size, np.float64)
+    dT = DevVec(1024, np.float64)
+    dR = DevVec(1024, np.float64)
+    p = C.c_int()
+    check(lib().lace_block_panel_qr(n, r0, dW.ptr, dY.ptr, dT.ptr, dR.ptr, C.byref(p)), "panel_qr")
+    return (dY.download().reshape(n, NRHS), dT.download().reshape(32, 32),
+            dR.download().reshape(32, 32), p.value)
 
 
 class DevBCSR:
@@ -569,6 +649,28 @@ class Solver:
     def grad_values(self, lam: DevVec, x: DevVec, nnz: int) -> np.ndarray:
         g = DevVec(nnz, np.float64)
         check(lib().lace_solver_grad_values(self.h, lam.ptr, x.ptr, g.ptr), "grad_values")
+        return g.download()
+
+    # 32 right-hand sides at once: blocks are (n, 32) arrays, row-major
+    def solve_block_dev(self, dB: DevVec, dX: DevVec, trans=0) -> BlockStats:
+        st = BlockStats()
+        check(lib().lace_solver_solve_block(self.h, dB.ptr, dX.ptr, trans, C.byref(st)),
+              "solver_solve_block")
+        return st
+
+    def solve_block(self, B, X0=None, trans=0):
+        B = np.ascontiguousarray(B, dtype=np.float64)
+        assert B.shape == (self.n, NRHS)
+        dB = DevVec(B.size, np.float64, B.reshape(-1))
+        X0 = np.zeros_like(B) if X0 is None else np.ascontiguousarray(X0, dtype=np.float64)
+        dX = DevVec(B.size, np.float64, X0.reshape(-1))
+        st = self.solve_block_dev(dB, dX, trans)
+        return dX.download().reshape(self.n, NRHS), st
+
+    def grad_values_block(self, dLam: DevVec, dX: DevVec, nnz: int) -> np.ndarray:
+        g = DevVec(nnz, np.float64)
+        check(lib().lace_solver_grad_values_block(self.h, dLam.ptr, dX.ptr, g.ptr),
+              "grad_values_block")
         return g.download()
 
     def __del__(self):

@@ -1,0 +1,253 @@
+// mrhs.hip -- multi-right-hand-side kernels for the block Krylov path.
+//
+// Blocks are n x 32 and row-major, so one row of a block is 32 contiguous
+// values (256 bytes in FP64, 128 in FP32).  A wave covers two matrix rows
+// with one lane per right-hand side: each half-wave reads the row's column
+// indices and values once (the 32 lanes share the addresses) and gathers
+// whole 256- or 128-byte rows of the right-hand block, so every access to
+// the dense blocks is a full, aligned row.  The matrix is read once per
+// product for all 32 right-hand sides.
+//
+// Arithmetic per column is the single-vector arithmetic of the CSR-stream
+// kernels (products val*x in column order summed from zero, then b - s or
+// (r - s) / diag, contraction off), so each column of a block result is the
+// single-vector result of that column bit for bit.  The sparse triangular
+// sweep formula is src/trisolve.cpp:281-290 (data_parcsrtrsv) with all reads
+// from the previous sweep; the product is the row loop of
+// blas/zspmv.cpp:48-78 applied to 32 vectors at once (the reference has no
+// sparse-times-dense product: blas/zspmm.cpp is sparse times sparse).
+#include <algorithm>
+
+#include "common.hpp"
+#include "mrhs.hpp"
+
+namespace lace {
+
+constexpr int kChunk = 8;  // nonzeros whose operands are loaded ahead
+
+template <typename TX, int MODE>
+__global__ __launch_bounds__(256) void spmm32_kernel(int n, const int* __restrict__ row,
+                                                     const int* __restrict__ col,
+                                                     const double* __restrict__ val,
+                                                     const TX* __restrict__ X,
+                                                     const double* __restrict__ B,
+                                                     double* __restrict__ Y) {
+#pragma clang fp contract(off)
+  const int i = xcd_block(blockIdx.x, gridDim.x) * 8 + (threadIdx.x >> 5);
+  const int c = threadIdx.x & 31;
+  if (i >= n) return;
+  const int k0 = row[i], k1 = row[i + 1];
+  const double add = MODE == 1 ? B[(size_t)i * 32 + c] : 0.0;
+  double s = 0.0;
+  for (int kb = k0; kb < k1; kb += kChunk) {
+    int cc[kChunk];
+    double vv[kChunk], xv[kChunk];
+#pragma unroll
+    for (int e = 0; e < kChunk; e++) {
+      const bool in = kb + e < k1;
+      cc[e] = in ? col[kb + e] : 0;
+      vv[e] = in ? val[kb + e] : 0.0;
+    }
+#pragma unroll
+    for (int e = 0; e < kChunk; e++) xv[e] = kb + e < k1 ? (double)X[(size_t)cc[e] * 32 + c] : 0.0;
+#pragma unroll
+    for (int e = 0; e < kChunk; e++)
+      if (kb + e < k1) s = s + vv[e] * xv[e];
+  }
+  Y[(size_t)i * 32 + c] = MODE == 1 ? add - s : s;
+}
+
+template <typename TX>
+int spmm32(const DevCSR& A, const TX* X, const double* B, double* Y, bool residual) {
+  LACE_REQUIRE(A.prec == LACE_F64, DEV_ERR_NOT_SUPPORTED, "spmm32: FP64 matrix values expected");
+  if (A.n == 0) return DEV_SUCCESS;
+  const int nb = ceil_div(A.n, 8);
+  if (residual) {
+    TimedRegion tr("spmm32_residual");
+    spmm32_kernel<TX, 1><<<nb, 256, 0, stream()>>>(A.n, A.row.as<int>(), A.col.as<int>(),
+                                                   A.val.as<double>(), X, B, Y);
+  } else {
+    TimedRegion tr(sizeof(TX) == 4 ? "spmm32_f32x" : "spmm32");
+    spmm32_kernel<TX, 0><<<nb, 256, 0, stream()>>>(A.n, A.row.as<int>(), A.col.as<int>(),
+                                                   A.val.as<double>(), X, nullptr, Y);
+  }
+  LACE_HIP(hipGetLastError());
+  return DEV_SUCCESS;
+}
+
+template int spmm32<double>(const DevCSR&, const double*, const double*, double*, bool);
+template int spmm32<float>(const DevCSR&, const float*, const double*, double*, bool);
+
+// One synchronous Jacobi sweep on 32 right-hand sides:
+//   yout[i][c] = (r[i][c] - sum_{k != diag} T_ik yin[k][c]) / T_ii,
+// yin == nullptr meaning yin = 0.  Upper triangles store the diagonal first
+// in each row, lower ones last (the TriPlan layouts).
+template <typename T, bool UPPER>
+__global__ __launch_bounds__(256) void jacobi32_kernel(int n, const int* __restrict__ row,
+                                                       const int* __restrict__ col,
+                                                       const T* __restrict__ val,
+                                                       const T* __restrict__ r,
+                                                       const T* __restrict__ yin,
+                                                       T* __restrict__ yout) {
+#pragma clang fp contract(off)
+  const int i = xcd_block(blockIdx.x, gridDim.x) * 8 + (threadIdx.x >> 5);
+  const int c = threadIdx.x & 31;
+  if (i >= n) return;
+  int k0, k1, kd;
+  if (UPPER) {
+    kd = row[i];
+    k0 = kd + 1;
+    k1 = row[i + 1];
+  } else {
+    k0 = row[i];
+    kd = row[i + 1] - 1;
+    k1 = kd;
+  }
+  const T ri = r[(size_t)i * 32 + c];
+  const T di = val[kd];
+  T s = T(0);
+  if (yin) {
+    for (int kb = k0; kb < k1; kb += kChunk) {
+      int cc[kChunk];
+      T vv[kChunk], yv[kChunk];
+#pragma unroll
+      for (int e = 0; e < kChunk; e++) {
+        const bool in = kb + e < k1;
+        cc[e] = in ? col[kb + e] : 0;
+        vv[e] = in ? val[kb + e] : T(0);
+      }
+#pragma unroll
+      for (int e = 0; e < kChunk; e++) yv[e] = kb + e < k1 ? yin[(size_t)cc[e] * 32 + c] : T(0);
+#pragma unroll
+      for (int e = 0; e < kChunk; e++)
+        if (kb + e < k1) s = s + vv[e] * yv[e];
+    }
+  }
+  yout[(size_t)i * 32 + c] = (ri - s) / di;
+}
+
+template <typename T>
+static int jacobi32_sweep(const TriPlan& P, const T* val, const T* r, const T* yin, T* yout) {
+  const int nb = ceil_div(P.n, 8);
+  TimedRegion tr(P.upper ? "block_jacobi_sweep_upper" : "block_jacobi_sweep_lower");
+  if (P.upper)
+    jacobi32_kernel<T, true><<<nb, 256, 0, stream()>>>(P.n, P.row.as<int>(), P.col.as<int>(), val, r,
+                                                       yin, yout);
+  else
+    jacobi32_kernel<T, false><<<nb, 256, 0, stream()>>>(P.n, P.row.as<int>(), P.col.as<int>(), val, r,
+                                                        yin, yout);
+  LACE_HIP(hipGetLastError());
+  return DEV_SUCCESS;
+}
+
+// `sweeps` Jacobi sweeps from y = 0 -- the same sequence as the
+// single-vector solve, including the unit-diagonal shortcut (the first sweep
+// from zero returns r exactly, so the second reads r).
+template <typename T>
+static int jacobi32_solve(const TriPlan& P, const T* val, const T* r, T* y, T* tmp, int sweeps) {
+  const size_t bytes = sizeof(T) * (size_t)P.n * kNrhs;
+  if (sweeps <= 0) {
+    LACE_HIP(hipMemsetAsync(y, 0, bytes, stream()));
+    return DEV_SUCCESS;
+  }
+  if (P.unit_diag) {
+    if (sweeps == 1) {
+      LACE_HIP(hipMemcpyAsync(y, r, bytes, hipMemcpyDeviceToDevice, stream()));
+      return DEV_SUCCESS;
+    }
+    const int rest = sweeps - 1;
+    T* a = (rest % 2 == 1) ? y : tmp;
+    T* b = (rest % 2 == 1) ? tmp : y;
+    LACE_TRY(jacobi32_sweep<T>(P, val, r, r, a));
+    for (int s = 1; s < rest; s++) {
+      LACE_TRY(jacobi32_sweep<T>(P, val, r, a, b));
+      std::swap(a, b);
+    }
+    return DEV_SUCCESS;
+  }
+  T* a = (sweeps % 2 == 1) ? y : tmp;
+  T* b = (sweeps % 2 == 1) ? tmp : y;
+  LACE_TRY(jacobi32_sweep<T>(P, val, r, nullptr, a));
+  for (int s = 1; s < sweeps; s++) {
+    LACE_TRY(jacobi32_sweep<T>(P, val, r, a, b));
+    std::swap(a, b);
+  }
+  return DEV_SUCCESS;
+}
+
+template <typename T>
+int block_precond(ParILU* P, lace_trsv_kind_t kind, int sweeps, int trans, const T* R, T* Z, T* tmp,
+                  T* tmp2) {
+  LACE_REQUIRE((P->prec == LACE_F64) == (sizeof(T) == 8), DEV_ERR_ILLEGAL_VALUE,
+               "block_precond: vector precision differs from the factors'");
+  if (kind == LACE_TRSV_NONE) {
+    LACE_HIP(hipMemcpyAsync(Z, R, sizeof(T) * (size_t)P->n * kNrhs, hipMemcpyDeviceToDevice, stream()));
+    return DEV_SUCCESS;
+  }
+  LACE_REQUIRE(kind == LACE_TRSV_JACOBI, DEV_ERR_NOT_SUPPORTED,
+               "block solves take Jacobi (or no) triangular solves");
+  LACE_TRY(P->refresh_csr());
+  const TriPlan& first = trans ? P->lowerT : P->lower;
+  const TriPlan& second = trans ? P->upperT : P->upper;
+  const T* v1 = trans ? P->uval[P->cur].as<T>() : P->lval[P->cur].as<T>();
+  const T* v2 = trans ? P->lval_t.as<T>() : P->uval_csr.as<T>();
+  TimedRegion tr("block_precond");
+  LACE_TRY(jacobi32_solve<T>(first, v1, R, tmp, tmp2, sweeps));
+  LACE_TRY(jacobi32_solve<T>(second, v2, tmp, Z, tmp2, sweeps));
+  return DEV_SUCCESS;
+}
+
+template int block_precond<double>(ParILU*, lace_trsv_kind_t, int, int, const double*, double*,
+                                   double*, double*);
+template int block_precond<float>(ParILU*, lace_trsv_kind_t, int, int, const float*, float*, float*,
+                                  float*);
+
+__global__ void block_cast_d2f_kernel(size_t m, const double* __restrict__ in, float* __restrict__ out) {
+  const size_t e = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (e < m) out[e] = (float)in[e];
+}
+
+int block_cast_d2f(int n, const double* in, float* out) {
+  const size_t m = (size_t)n * kNrhs;
+  if (m == 0) return DEV_SUCCESS;
+  block_cast_d2f_kernel<<<ceil_div((long)m, 256), 256, 0, stream()>>>(m, in, out);
+  LACE_HIP(hipGetLastError());
+  return DEV_SUCCESS;
+}
+
+}  // namespace lace
+
+using namespace lace;
+
+extern "C" {
+
+int lace_csr_spmm32(const lace_csr* Ah, const void* X, lace_prec_t x_prec, const double* B, double* Y) {
+  LACE_TRY(ensure_init());
+  const DevCSR& A = *reinterpret_cast<const DevCSR*>(Ah);
+  if (x_prec == LACE_F64) return spmm32<double>(A, static_cast<const double*>(X), B, Y, B != nullptr);
+  LACE_REQUIRE(x_prec == LACE_F32, DEV_ERR_ILLEGAL_VALUE, "spmm32: X precision %d", (int)x_prec);
+  return spmm32<float>(A, static_cast<const float*>(X), B, Y, B != nullptr);
+}
+
+int lace_parilu_apply_block(lace_parilu* Ph, lace_trsv_kind_t kind, int jacobi_sweeps, int trans,
+                            const void* R, void* Z) {
+  LACE_TRY(ensure_init());
+  auto* P = reinterpret_cast<ParILU*>(Ph);
+  const size_t elems = (size_t)P->n * kNrhs;
+  DevBuf t1, t2;
+  const size_t es = P->prec == LACE_F64 ? sizeof(double) : sizeof(float);
+  LACE_TRY(t1.alloc(elems * es));
+  LACE_TRY(t2.alloc(elems * es));
+  int rc;
+  if (P->prec == LACE_F64)
+    rc = block_precond<double>(P, kind, jacobi_sweeps, trans, static_cast<const double*>(R),
+                               static_cast<double*>(Z), t1.as<double>(), t2.as<double>());
+  else
+    rc = block_precond<float>(P, kind, jacobi_sweeps, trans, static_cast<const float*>(R),
+                              static_cast<float*>(Z), t1.as<float>(), t2.as<float>());
+  LACE_TRY(rc);
+  LACE_HIP(hipStreamSynchronize(stream()));
+  return DEV_SUCCESS;
+}
+
+}  // extern "C"

@@ -20,6 +20,7 @@
 #include <vector>
 
 #include "bcsr.hpp"
+#include "bkrylov.hpp"
 #include "common.hpp"
 #include "device.hpp"
 #include "krylov.hpp"
@@ -169,6 +170,7 @@ struct Solver {
   std::unique_ptr<PrecInner> inner;
   Precond* M = nullptr;
   HHEngine<double> E;
+  std::unique_ptr<BlockEngine> beng;  // 32-right-hand-side solver, on first use
   DevBuf rbuf, partial, scal;
   int nbn = 0, rpb = 4096, pstride = 4;
   double* host_scal = nullptr;
@@ -421,6 +423,24 @@ __global__ void grad_gather_kernel(int n, const int* __restrict__ row, const int
   if (i >= n) return;
   const double li = lambda[i];
   for (int k = row[i]; k < row[i + 1]; k++) dtheta[k] = -li * x[col[k]];
+}
+
+// Block form over 32 right-hand sides: dtheta[k] = -sum_c Lam[i][c] X[j][c]
+// for position k = (i, j); 32 lanes per row, lane c one right-hand side,
+// the sum over c in a fixed butterfly order.
+__global__ __launch_bounds__(256) void grad_gather_block_kernel(int n, const int* __restrict__ row,
+                                                                const int* __restrict__ col,
+                                                                const double* __restrict__ Lam,
+                                                                const double* __restrict__ X,
+                                                                double* __restrict__ dtheta) {
+  const int i = xcd_block(blockIdx.x, gridDim.x) * 8 + (threadIdx.x >> 5);
+  const int c = threadIdx.x & 31;
+  if (i >= n) return;
+  const double li = Lam[(size_t)i * 32 + c];
+  for (int k = row[i]; k < row[i + 1]; k++) {
+    const double s = group_sum<32>(li * X[(size_t)col[k] * 32 + c]);
+    if (c == 0) dtheta[k] = -s;
+  }
 }
 
 
@@ -734,6 +754,49 @@ int lace_solver_grad_values(lace_solver* Sh, const double* lambda, const double*
   LACE_REQUIRE(!S->block, DEV_ERR_NOT_SUPPORTED, "solver: value gradients are for CSR operators");
   grad_gather_kernel<<<ceil_div(S->n, 256), 256, 0, stream()>>>(
       S->n, S->A64.row.as<int>(), S->A64.col.as<int>(), lambda, x, dtheta);
+  LACE_HIP(hipGetLastError());
+  return DEV_SUCCESS;
+}
+
+int lace_solver_solve_block(lace_solver* Sh, const double* B, double* X, int trans, lace_block_stats* st) {
+  LACE_TRY(ensure_init());
+  auto* S = reinterpret_cast<Solver*>(Sh);
+  const int m = S->o.search_max;
+  LACE_REQUIRE(!S->block && S->P, DEV_ERR_NOT_SUPPORTED, "block solve: CSR operator with ILU factors");
+  LACE_REQUIRE(S->o.inner_iters == 0, DEV_ERR_NOT_SUPPORTED, "block solve: no inner iterations");
+  LACE_REQUIRE(S->o.trsv == LACE_TRSV_JACOBI || S->o.trsv == LACE_TRSV_NONE, DEV_ERR_NOT_SUPPORTED,
+               "block solve: Jacobi (or no) triangular solves");
+  LACE_REQUIRE(m >= 1 && m + 1 <= kMaxBlocks, DEV_ERR_ILLEGAL_VALUE, "block solve: search_max %d", m);
+  LACE_REQUIRE((long)S->n >= 32L * (m + 2), DEV_ERR_ILLEGAL_VALUE,
+               "block solve: n = %d below 32 (search_max + 2)", S->n);
+  if (trans) LACE_TRY(ensure_transpose(S));
+  const bool zf = S->P->prec != LACE_F64;
+  if (!S->beng || S->beng->m != m || S->beng->zf != zf) {
+    S->beng.reset();
+    auto E = std::make_unique<BlockEngine>();
+    LACE_TRY(E->init(S->n, m, zf));
+    S->beng = std::move(E);
+  }
+  BlockSolveArgs args;
+  args.A = trans ? &S->AT64 : &S->A64;
+  args.P = S->P;
+  args.kind = S->o.trsv;
+  args.sweeps = S->o.jacobi_sweeps;
+  args.trans = trans ? 1 : 0;
+  args.restart_max = S->o.restart_max;
+  args.rtol = S->o.rtol;
+  lace_block_stats local;
+  LACE_TRY(block_fgmres(*S->beng, args, B, X, &local));
+  if (st) *st = local;
+  return DEV_SUCCESS;
+}
+
+int lace_solver_grad_values_block(lace_solver* Sh, const double* Lambda, const double* X, double* dtheta) {
+  LACE_TRY(ensure_init());
+  auto* S = reinterpret_cast<Solver*>(Sh);
+  LACE_REQUIRE(!S->block, DEV_ERR_NOT_SUPPORTED, "solver: value gradients are for CSR operators");
+  grad_gather_block_kernel<<<ceil_div(S->n, 8), 256, 0, stream()>>>(
+      S->n, S->A64.row.as<int>(), S->A64.col.as<int>(), Lambda, X, dtheta);
   LACE_HIP(hipGetLastError());
   return DEV_SUCCESS;
 }

@@ -48,7 +48,7 @@ __global__ __launch_bounds__(256) void tn_partial_kernel(int n, int rpb, BlockLi
   for (int t = 0; t < kTnBlocksPerGroup; t++) {
     acc[t] = d4{0.0, 0.0, 0.0, 0.0};
     const int b = g * kTnBlocksPerGroup + t;
-    xb[t] = b < nb ? X.p[b] : nullptr;
+    xb[t] = b < nb ? static_cast<const double*>(X.p[b]) : nullptr;
     rsb[t] = b == nb - 1 ? rs_last : rs_main;
   }
   const int col_a = ah * 16 + (lane & 15), col_b = bh * 16 + (lane & 15);
@@ -96,9 +96,10 @@ __global__ __launch_bounds__(256) void tn_reduce_kernel(int nslab, int count, co
 
 // A workgroup owns a band of 32 rows; wave w computes the 16 x 16 tile of
 // rows band + 16*(w/2) .. +15 and columns 16*(w%2) .. +15.
+template <typename TX, typename TO>
 __global__ __launch_bounds__(256) void nn_kernel(int n, BlockList X,
                                                  const double* __restrict__ D, const double* base,
-                                                 int base_mode, int e0, int rlo, double* out) {
+                                                 int base_mode, int e0, int rlo, TO* out) {
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int band = xcd_block(blockIdx.x, gridDim.x) * 32;  // 32 rows per workgroup
   const int rt = band + 16 * (w >> 1);                      // tile's first row
@@ -108,12 +109,12 @@ __global__ __launch_bounds__(256) void nn_kernel(int n, BlockList X,
   d4 acc = d4{0.0, 0.0, 0.0, 0.0};
   const bool arow_in = arow < n && arow >= rlo;
   for (int b = 0; b < X.nb; b++) {
-    const double* Xb = X.p[b];
+    const TX* Xb = static_cast<const TX*>(X.p[b]);
     const double* Db = D + (size_t)b * 1024;
 #pragma unroll
     for (int k0 = 0; k0 < 32; k0 += 4) {
       const int kk = k0 + (lane >> 4);
-      const double av = arow_in ? Xb[(size_t)arow * 32 + kk] : 0.0;
+      const double av = arow_in ? (double)Xb[(size_t)arow * 32 + kk] : 0.0;
       const double bv = Db[kk * 32 + ch * 16 + (lane & 15)];
       acc = mfma4(av, bv, acc);
     }
@@ -127,7 +128,7 @@ __global__ __launch_bounds__(256) void nn_kernel(int n, BlockList X,
     double bval = 0.0;
     if (base_mode == 1) bval = base[(size_t)row * 32 + col];
     else if (base_mode == 2) bval = (row - e0 == col) ? 1.0 : 0.0;
-    out[(size_t)row * 32 + col] = row >= rlo ? bval - acc[v] : bval;
+    out[(size_t)row * 32 + col] = (TO)(row >= rlo ? bval - acc[v] : bval);
   }
 }
 
@@ -151,14 +152,22 @@ int TsGemm::tn(const BlockList& L, const double* W, int rs_main, int rs_last, do
   return DEV_SUCCESS;
 }
 
+template <typename TX, typename TO>
 int TsGemm::nn(const BlockList& L, const double* D, const double* base, int base_mode, int e0, int rlo,
-               double* out) {
+               TO* out) {
   LACE_REQUIRE(L.nb >= 0 && L.nb <= kMaxBlocks, DEV_ERR_ILLEGAL_VALUE, "tsgemm: %d blocks", L.nb);
   TimedRegion tr("tsgemm_nn");
-  nn_kernel<<<ceil_div(n, 32), 256, 0, stream()>>>(n, L, D, base, base_mode, e0, rlo, out);
+  nn_kernel<TX, TO><<<ceil_div(n, 32), 256, 0, stream()>>>(n, L, D, base, base_mode, e0, rlo, out);
   LACE_HIP(hipGetLastError());
   return DEV_SUCCESS;
 }
+
+template int TsGemm::nn<double, double>(const BlockList&, const double*, const double*, int, int, int,
+                                        double*);
+template int TsGemm::nn<double, float>(const BlockList&, const double*, const double*, int, int, int,
+                                       float*);
+template int TsGemm::nn<float, double>(const BlockList&, const double*, const double*, int, int, int,
+                                       double*);
 
 }  // namespace lace
 
@@ -191,7 +200,7 @@ int lace_tsgemm_nn(int n, int nb, const double* const* blocks, const double* D, 
   BlockList L;
   L.nb = nb;
   for (int b = 0; b < nb; b++) L.p[b] = blocks[b];
-  LACE_TRY(T.nn(L, D, base, base_mode, e0, rlo, out));
+  LACE_TRY((T.nn<double, double>(L, D, base, base_mode, e0, rlo, out)));
   LACE_HIP(hipStreamSynchronize(stream()));
   return DEV_SUCCESS;
 }

@@ -7,9 +7,10 @@ namespace lace {
 
 constexpr int kMaxBlocks = 40;
 
-// A list of n x 32 row-major blocks, passed to kernels by value.
+// A list of n x 32 row-major blocks (FP64, or FP32 where a product says
+// so), passed to kernels by value.
 struct BlockList {
-  const double* p[kMaxBlocks];
+  const void* p[kMaxBlocks];
   int nb;
 };
 
@@ -23,9 +24,11 @@ struct TsGemm {
   // out = base - sum_b X_b D_b on rows [rlo, n) (D: nb consecutive 32 x 32
   // row-major blocks); rows below rlo get base.  base_mode 0: zero, 1: the
   // block `base`, 2: identity block column (1 at row e0 + c, column c).
-  // out may be base; it must not be one of the X_b.
+  // out may be base; it must not be one of the X_b.  TX: element type of
+  // the X_b (double or float, widened exactly); TO: element type of out.
+  template <typename TX, typename TO>
   int nn(const BlockList& L, const double* D, const double* base, int base_mode, int e0, int rlo,
-         double* out);
+         TO* out);
 };
 
 }  // namespace lace
