@@ -204,6 +204,123 @@ def bench_bcsr(args) -> int:
     return 0
 
 
+MFMA_F64_PEAK_TFS = 78.6
+BLOCK_REGIONS = ["tsgemm_tn", "tsgemm_nn", "spmm32_f32x", "spmm32_residual",
+                 "block_jacobi_sweep_lower", "block_jacobi_sweep_upper"]
+
+
+def bench_block(args) -> int:
+    """Config 5: 32 N(0,1) right-hand sides (Philox seeds 0-31) on the 128^3
+    convection-diffusion matrix.  One step = FP32 ParILU setup (3 sweeps) +
+    block Householder FGMRES(30) for the 32 right-hand sides to 1e-8 per
+    column (FP32 factors, Jacobi-5, FP64 reflectors with the MFMA Gram and
+    panel products) + one adjoint block solve A^T Lambda = X - X_t for the
+    32 quadratic losses 0.5 ||x_c - x_t,c||^2 (x_t,c ~ N(0,1), seeds
+    1000-1031) + the gradient gather of their sum."""
+    import lace_amd as L
+    lib = L.lib()
+    L.check(lib.lace_init(0), "lace_init")
+    N = args.size
+    row, col, val = L.gen_convdiff3d(N, N, N)
+    n, nnz = len(row) - 1, int(row[-1])
+    H = L.HostCSR(row, col, val)
+    B = np.stack([L.gen_normal(c, n) for c in range(32)], axis=1)
+    Xt = np.stack([L.gen_normal(1000 + c, n) for c in range(32)], axis=1)
+    o = make_opts(L, search_max=30, trsv=L.TRSV_JACOBI, jacobi_sweeps=5, prec_precond=L.LACE_F32)
+    o.restart_max = 20
+    S = L.Solver(H, o)
+    m = n * 32
+    dB = L.DevVec(m, np.float64, B.reshape(-1))
+    dXt = L.DevVec(m, np.float64, Xt.reshape(-1))
+    dX = L.DevVec(m, np.float64)
+    dG = L.DevVec(m, np.float64)
+    dLam = L.DevVec(m, np.float64)
+    dth = L.DevVec(nnz, np.float64)
+    sync = lambda: L.check(lib.lace_device_synchronize())
+
+    def step():
+        L.check(lib.lace_daxpby(m, 0.0, None, 0.0, None, dX.ptr))
+        L.check(lib.lace_daxpby(m, 0.0, None, 0.0, None, dLam.ptr))
+        sync()
+        t0 = time.perf_counter()
+        S.setup()
+        sync()
+        t1 = time.perf_counter()
+        stf = S.solve_block_dev(dB, dX, 0)
+        t2 = time.perf_counter()
+        L.check(lib.lace_daxpby(m, 1.0, dX.ptr, -1.0, dXt.ptr, dG.ptr))
+        sta = S.solve_block_dev(dG, dLam, 1)
+        L.check(lib.lace_solver_grad_values_block(S.h, dLam.ptr, dX.ptr, dth.ptr))
+        sync()
+        t3 = time.perf_counter()
+        return t3 - t0, t1 - t0, t2 - t1, t3 - t2, stf, sta
+
+    for _ in range(args.warmup):
+        step()
+    L.check(lib.lace_timing_reset())
+    L.check(lib.lace_timing_only(",".join(BLOCK_REGIONS).encode()))
+    L.check(lib.lace_timing_enable(1))
+    res = [step() for _ in range(args.steps)]
+    L.check(lib.lace_timing_enable(0))
+    L.check(lib.lace_timing_only(None))
+    kernels = {}
+    for r in BLOCK_REGIONS:
+        ms, cnt = L.timing(r)
+        if cnt == 0:
+            continue
+        fl, by = L.timing_work(r)
+        sec = ms * 1e-3
+        kernels[r] = {"launches": cnt, "total_ms": round(ms, 3), "avg_us": round(ms / cnt * 1e3, 2),
+                      "GBps": round(by / sec / 1e9, 1), "hbm_frac": round(by / sec / 1e9 / HBM_PEAK_GBS, 4),
+                      "TFLOPs": round(fl / sec / 1e12, 2)}
+        if r.startswith("tsgemm"):
+            kernels[r]["mfma_frac"] = round(fl / sec / 1e12 / MFMA_F64_PEAK_TFS, 4)
+    t = float(np.mean([r[0] for r in res]))
+    stf, sta = res[-1][4], res[-1][5]
+    ok = all(r[4].converged == 1 and r[5].converged == 1 and r[4].max_rel_true_residual <= 1e-8
+             and r[5].max_rel_true_residual <= 1e-8 for r in res)
+    # FP64 oracle error of two columns: FP64 single-vector solves to 1e-12
+    X = dX.download().reshape(n, 32)
+    errs = {}
+    o64 = make_opts(L, search_max=50, trsv=L.TRSV_JACOBI, jacobi_sweeps=5, prec_precond=L.LACE_F64)
+    o64.rtol = 1e-12
+    ref = L.Solver(H, o64)
+    ref.setup()
+    for c in (0, 31):
+        xr, st = ref.solve(B[:, c])
+        errs[str(c)] = float(np.linalg.norm(X[:, c] - xr) / np.linalg.norm(xr))
+    tn = kernels.get("tsgemm_tn", {})
+    out = {
+        "mode": "block",
+        "metric": "32-RHS forward + adjoint block ILU-FGMRES(30) to 1e-8, s per step",
+        "value": round(t, 4), "unit": "s", "steps": args.steps, "warmup": args.warmup,
+        "higher_is_better": False, "dtype": "f32 preconditioner / f64 block FGMRES",
+        "config": {"workload": f"config 5: 32 right-hand sides, convection-diffusion {N}^3",
+                   "n": n, "nnz": nnz, "nrhs": 32, "restart": 30, "parilu_sweeps": 3,
+                   "trsv": "Jacobi-5 (FP32 factors)", "rhs_seeds": "0-31", "x_t_seeds": "1000-1031"},
+        "solves_per_s": round(64 / t, 2),
+        "setup_s": round(float(np.mean([r[1] for r in res])), 4),
+        "forward_s": round(float(np.mean([r[2] for r in res])), 4),
+        "adjoint_and_gradient_s": round(float(np.mean([r[3] for r in res])), 4),
+        "forward": {"block_steps": stf.iterations, "restarts": stf.restarts,
+                    "panel_passes": stf.panel_passes, "shifted_panels": stf.shifted_panels,
+                    "max_rel_true_residual": stf.max_rel_true_residual},
+        "adjoint": {"block_steps": sta.iterations, "restarts": sta.restarts,
+                    "panel_passes": sta.panel_passes, "shifted_panels": sta.shifted_panels,
+                    "max_rel_true_residual": sta.max_rel_true_residual},
+        "err_vs_fp64_solution": errs,
+        "all_converged": ok,
+        "roofline": {"kernel": "tsgemm_tn (MFMA FP64 Gram)", "bound": "hbm",
+                     "achieved": tn.get("GBps"), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": tn.get("hbm_frac"), "traffic": None,
+                     "mfma_TFLOPs": tn.get("TFLOPs"), "mfma_peak_TFLOPs": MFMA_F64_PEAK_TFS,
+                     "mfma_frac": tn.get("mfma_frac")},
+        "kernels": kernels,
+    }
+    print(json.dumps(out))
+    return 0 if ok else 1
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -212,10 +329,11 @@ def main() -> int:
     ap.add_argument("--size", type=int, default=128)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
-    ap.add_argument("--mode", choices=["solve", "kernels", "bcsr"], default="solve",
+    ap.add_argument("--mode", choices=["solve", "kernels", "bcsr", "block"], default="solve",
                     help="kernels: one mixed-path step plus one 256 MiB copy kernel "
                          "(the workload profiled for HBM traffic counters); bcsr: the "
-                         "block-sparse 5x5 workload on the 64^3 block grid")
+                         "block-sparse 5x5 workload on the 64^3 block grid; block: 32 "
+                         "right-hand sides, forward and adjoint block FGMRES")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: --gpus {args.gpus} is not supported; this implementation runs on one "
@@ -224,6 +342,8 @@ def main() -> int:
         return 2
     if args.mode == "bcsr":
         return bench_bcsr(args)
+    if args.mode == "block":
+        return bench_block(args)
 
     import lace_amd as L
     lib = L.lib()

@@ -529,6 +529,9 @@ int lace_csr_spmm32(const lace_csr *A, const void *X, lace_prec_t x_prec, const 
  * factor precision (Jacobi sweeps or kind NONE). */
 int lace_parilu_apply_block(lace_parilu *P, lace_trsv_kind_t kind, int jacobi_sweeps, int trans,
                             const void *R, void *Z);
+/* out = a x + b y on device arrays of m doubles (a term with a zero
+ * coefficient is not read), e.g. the adjoint right-hand sides x - x_t. */
+int lace_daxpby(size_t m, double a, const double *x, double b, const double *y, double *out);
 /* Householder QR of rows [r0, n) of the n x 32 block W (device): writes
  * Y (n x 32: zero above r0, unit lower triangle at rows r0..r0+31), Tb
  * and Rt (32 x 32 row-major) with (I - Y Tb Y^T) [Rt; 0] = W[r0:, :];
@@ -550,6 +553,11 @@ int lace_timing_reset(void);
  * "trsv_jacobi_sweep_lower" / "_upper" (one stream Jacobi sweep). */
 int lace_timing_only(const char *name);
 int lace_timing_get(const char *name, double *total_ms, long *count);
+/* Algorithmic work summed over the timed launches of a region: flops and
+ * HBM bytes (recorded by the regions whose shape varies between launches:
+ * "tsgemm_tn", "tsgemm_nn", and the block kernels "spmm32", "spmm32_f32x",
+ * "spmm32_residual", "block_jacobi_sweep_lower" / "_upper"). */
+int lace_timing_get_work(const char *name, double *flops, double *bytes);
 
 /* Level sets (host): lev(i) = 1 + max lev(k) over the off-diagonal entries
  * of row i (lower: k < i; upper: k > i), 0 when there are none.  Writes

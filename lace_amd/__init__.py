@@ -197,12 +197,14 @@ SIGNATURES = {
     "lace_csr_spmm32": (_i, [_vp, _vp, _i, _vp, _vp]),
     "lace_parilu_apply_block": (_i, [_vp, _i, _i, _i, _vp, _vp]),
     "lace_block_panel_qr": (_i, [_i, _i, _vp, _vp, _vp, _vp, _P(_i)]),
+    "lace_daxpby": (_i, [C.c_size_t, _d, _vp, _d, _vp, _vp]),
     "lace_tsgemm_tn": (_i, [_i, _i, _P(_vp), _vp, _i, _i, _vp]),
     "lace_tsgemm_nn": (_i, [_i, _i, _P(_vp), _vp, _vp, _i, _i, _i, _vp]),
     "lace_timing_enable": (_i, [_i]),
     "lace_timing_reset": (_i, []),
     "lace_timing_only": (_i, [C.c_char_p]),
     "lace_timing_get": (_i, [C.c_char_p, _P(_d), _P(C.c_long)]),
+    "lace_timing_get_work": (_i, [C.c_char_p, _P(_d), _P(_d)]),
     "lace_level_sets": (_i, [_i, _P(_i), _P(_i), _i, _P(_i), _P(_i)]),
     "lace_gen_laplace2d": (_i, [_i, _i, _DM]),
     "lace_gen_convdiff3d": (_i, [_i, _i, _i, _d, _d, _d, _DM]),
@@ -712,3 +714,10 @@ def timing(name: str):
     t = C.c_double(); c = C.c_long()
     lib().lace_timing_get(name.encode(), C.byref(t), C.byref(c))
     return t.value, c.value
+
+
+def timing_work(name: str):
+    """(flops, bytes) summed over the timed launches of a region."""
+    f = C.c_double(); b = C.c_double()
+    lib().lace_timing_get_work(name.encode(), C.byref(f), C.byref(b))
+    return f.value, b.value

@@ -109,11 +109,15 @@ int download(void* host, const DevBuf& b, size_t bytes);
 // ---------------------------------------------------------------- timing
 // Named kernel classes timed with HIP events on the library stream when
 // timing is enabled; the averages feed the roofline report.
+// A region may carry its algorithmic work (flops and HBM bytes of that
+// launch), summed per region for launches whose shape varies.
 struct TimedRegion {
   const char* name;
   hipEvent_t a = nullptr, b = nullptr;
   bool on = false;
+  double flops = 0.0, bytes = 0.0;
   explicit TimedRegion(const char* n);
+  TimedRegion(const char* n, double flops_, double bytes_);
   ~TimedRegion();
 };
 bool timing_on();

@@ -25,54 +25,116 @@ namespace lace {
 
 constexpr int kChunk = 8;  // nonzeros whose operands are loaded ahead
 
-template <typename TX, int MODE>
+// W consecutive elements at a 16-byte aligned address, moved as 16-byte
+// words (W * sizeof(T) is a multiple of 16).
+template <typename T, int W>
+__device__ __forceinline__ void vload(const T* __restrict__ p, T (&v)[W]) {
+  constexpr int NQ = W * (int)sizeof(T) / 16;
+#pragma unroll
+  for (int q = 0; q < NQ; q++) {
+    const uint4 u = reinterpret_cast<const uint4*>(p)[q];
+    __builtin_memcpy(&v[q * 16 / (int)sizeof(T)], &u, 16);
+  }
+}
+template <typename T, int W>
+__device__ __forceinline__ void vstore(T* __restrict__ p, const T (&v)[W]) {
+  constexpr int NQ = W * (int)sizeof(T) / 16;
+#pragma unroll
+  for (int q = 0; q < NQ; q++) {
+    uint4 u;
+    __builtin_memcpy(&u, &v[q * 16 / (int)sizeof(T)], 16);
+    reinterpret_cast<uint4*>(p)[q] = u;
+  }
+}
+
+// Lane geometry: each lane owns W = 16 / sizeof(TX) consecutive right-hand
+// sides of one row (one 16-byte load of X per nonzero), 32 / W lanes per
+// row, 256 W / 32 rows per workgroup.
+template <typename TX>
+struct Lanes {
+  static constexpr int W = 16 / (int)sizeof(TX);
+  static constexpr int LPR = 32 / W;
+  static constexpr int RPB = 256 / LPR;
+};
+
+// Y = A X (MODE 0) or Y = B - A X (MODE 1).  Accumulation in double when the
+// values or the output are double (the single-vector rule).
+template <typename TV, typename TX, typename TY, int MODE>
 __global__ __launch_bounds__(256) void spmm32_kernel(int n, const int* __restrict__ row,
                                                      const int* __restrict__ col,
-                                                     const double* __restrict__ val,
+                                                     const TV* __restrict__ val,
                                                      const TX* __restrict__ X,
-                                                     const double* __restrict__ B,
-                                                     double* __restrict__ Y) {
+                                                     const TY* __restrict__ B,
+                                                     TY* __restrict__ Y) {
 #pragma clang fp contract(off)
-  const int i = xcd_block(blockIdx.x, gridDim.x) * 8 + (threadIdx.x >> 5);
-  const int c = threadIdx.x & 31;
+  using ACC = typename std::conditional<sizeof(TV) == 8 || sizeof(TY) == 8, double, float>::type;
+  constexpr int W = Lanes<TX>::W, LPR = Lanes<TX>::LPR, RPB = Lanes<TX>::RPB;
+  const int i = xcd_block(blockIdx.x, gridDim.x) * RPB + (int)threadIdx.x / LPR;
+  const int c0 = ((int)threadIdx.x % LPR) * W;
   if (i >= n) return;
   const int k0 = row[i], k1 = row[i + 1];
-  const double add = MODE == 1 ? B[(size_t)i * 32 + c] : 0.0;
-  double s = 0.0;
+  TY add[W];
+  if (MODE == 1) vload<TY, W>(B + (size_t)i * 32 + c0, add);
+  ACC s[W];
+#pragma unroll
+  for (int w = 0; w < W; w++) s[w] = ACC(0);
   for (int kb = k0; kb < k1; kb += kChunk) {
     int cc[kChunk];
-    double vv[kChunk], xv[kChunk];
+    TV vv[kChunk];
+    TX xv[kChunk][W];
 #pragma unroll
     for (int e = 0; e < kChunk; e++) {
       const bool in = kb + e < k1;
       cc[e] = in ? col[kb + e] : 0;
-      vv[e] = in ? val[kb + e] : 0.0;
+      vv[e] = in ? val[kb + e] : TV(0);
     }
 #pragma unroll
-    for (int e = 0; e < kChunk; e++) xv[e] = kb + e < k1 ? (double)X[(size_t)cc[e] * 32 + c] : 0.0;
+    for (int e = 0; e < kChunk; e++)
+      if (kb + e < k1) vload<TX, W>(X + (size_t)cc[e] * 32 + c0, xv[e]);
 #pragma unroll
     for (int e = 0; e < kChunk; e++)
-      if (kb + e < k1) s = s + vv[e] * xv[e];
+      if (kb + e < k1) {
+#pragma unroll
+        for (int w = 0; w < W; w++) s[w] = s[w] + (ACC)vv[e] * (ACC)xv[e][w];
+      }
   }
-  Y[(size_t)i * 32 + c] = MODE == 1 ? add - s : s;
+  TY out[W];
+#pragma unroll
+  for (int w = 0; w < W; w++) out[w] = MODE == 1 ? (TY)((ACC)add[w] - s[w]) : (TY)s[w];
+  vstore<TY, W>(Y + (size_t)i * 32 + c0, out);
+}
+
+template <typename TV, typename TX, typename TY>
+static int spmm32_launch(const DevCSR& A, const TX* X, const TY* B, TY* Y, bool residual) {
+  if (A.n == 0) return DEV_SUCCESS;
+  const int nb = ceil_div(A.n, Lanes<TX>::RPB);
+  // algorithmic work: 2 flops per nonzero and right-hand side; values and
+  // columns, row pointers, X, Y (and B) once
+  const double flops = 64.0 * A.nnz;
+  const double bytes = (4.0 + sizeof(TV)) * A.nnz + 4.0 * (A.n + 1) +
+                       32.0 * A.n * (sizeof(TX) + sizeof(TY)) + (residual ? 32.0 * sizeof(TY) * A.n : 0.0);
+  const char* name = residual ? "spmm32_residual"
+                              : (sizeof(TV) == 4 ? "spmm32_f32a" : (sizeof(TX) == 4 ? "spmm32_f32x" : "spmm32"));
+  TimedRegion tr(name, flops, bytes);
+  if (residual)
+    spmm32_kernel<TV, TX, TY, 1><<<nb, 256, 0, stream()>>>(A.n, A.row.as<int>(), A.col.as<int>(),
+                                                           A.val.as<TV>(), X, B, Y);
+  else
+    spmm32_kernel<TV, TX, TY, 0><<<nb, 256, 0, stream()>>>(A.n, A.row.as<int>(), A.col.as<int>(),
+                                                           A.val.as<TV>(), X, nullptr, Y);
+  LACE_HIP(hipGetLastError());
+  return DEV_SUCCESS;
 }
 
 template <typename TX>
 int spmm32(const DevCSR& A, const TX* X, const double* B, double* Y, bool residual) {
   LACE_REQUIRE(A.prec == LACE_F64, DEV_ERR_NOT_SUPPORTED, "spmm32: FP64 matrix values expected");
-  if (A.n == 0) return DEV_SUCCESS;
-  const int nb = ceil_div(A.n, 8);
-  if (residual) {
-    TimedRegion tr("spmm32_residual");
-    spmm32_kernel<TX, 1><<<nb, 256, 0, stream()>>>(A.n, A.row.as<int>(), A.col.as<int>(),
-                                                   A.val.as<double>(), X, B, Y);
-  } else {
-    TimedRegion tr(sizeof(TX) == 4 ? "spmm32_f32x" : "spmm32");
-    spmm32_kernel<TX, 0><<<nb, 256, 0, stream()>>>(A.n, A.row.as<int>(), A.col.as<int>(),
-                                                   A.val.as<double>(), X, nullptr, Y);
-  }
-  LACE_HIP(hipGetLastError());
-  return DEV_SUCCESS;
+  return spmm32_launch<double, TX, double>(A, X, B, Y, residual);
+}
+
+int spmm32_f32(const DevCSR& A, const float* X, float* Y) {
+  LACE_REQUIRE(A.prec == LACE_F32, DEV_ERR_NOT_SUPPORTED, "spmm32_f32: FP32 matrix values expected");
+  return spmm32_launch<float, float, float>(A, X, nullptr, Y, false);
 }
 
 template int spmm32<double>(const DevCSR&, const double*, const double*, double*, bool);
@@ -90,8 +152,9 @@ __global__ __launch_bounds__(256) void jacobi32_kernel(int n, const int* __restr
                                                        const T* __restrict__ yin,
                                                        T* __restrict__ yout) {
 #pragma clang fp contract(off)
-  const int i = xcd_block(blockIdx.x, gridDim.x) * 8 + (threadIdx.x >> 5);
-  const int c = threadIdx.x & 31;
+  constexpr int W = Lanes<T>::W, LPR = Lanes<T>::LPR, RPB = Lanes<T>::RPB;
+  const int i = xcd_block(blockIdx.x, gridDim.x) * RPB + (int)threadIdx.x / LPR;
+  const int c0 = ((int)threadIdx.x % LPR) * W;
   if (i >= n) return;
   int k0, k1, kd;
   if (UPPER) {
@@ -103,13 +166,15 @@ __global__ __launch_bounds__(256) void jacobi32_kernel(int n, const int* __restr
     kd = row[i + 1] - 1;
     k1 = kd;
   }
-  const T ri = r[(size_t)i * 32 + c];
+  T ri[W], s[W];
+  vload<T, W>(r + (size_t)i * 32 + c0, ri);
   const T di = val[kd];
-  T s = T(0);
+#pragma unroll
+  for (int w = 0; w < W; w++) s[w] = T(0);
   if (yin) {
     for (int kb = k0; kb < k1; kb += kChunk) {
       int cc[kChunk];
-      T vv[kChunk], yv[kChunk];
+      T vv[kChunk], yv[kChunk][W];
 #pragma unroll
       for (int e = 0; e < kChunk; e++) {
         const bool in = kb + e < k1;
@@ -117,19 +182,32 @@ __global__ __launch_bounds__(256) void jacobi32_kernel(int n, const int* __restr
         vv[e] = in ? val[kb + e] : T(0);
       }
 #pragma unroll
-      for (int e = 0; e < kChunk; e++) yv[e] = kb + e < k1 ? yin[(size_t)cc[e] * 32 + c] : T(0);
+      for (int e = 0; e < kChunk; e++)
+        if (kb + e < k1) vload<T, W>(yin + (size_t)cc[e] * 32 + c0, yv[e]);
 #pragma unroll
       for (int e = 0; e < kChunk; e++)
-        if (kb + e < k1) s = s + vv[e] * yv[e];
+        if (kb + e < k1) {
+#pragma unroll
+          for (int w = 0; w < W; w++) s[w] = s[w] + vv[e] * yv[e][w];
+        }
     }
   }
-  yout[(size_t)i * 32 + c] = (ri - s) / di;
+  T out[W];
+#pragma unroll
+  for (int w = 0; w < W; w++) out[w] = (ri[w] - s[w]) / di;
+  vstore<T, W>(yout + (size_t)i * 32 + c0, out);
 }
 
 template <typename T>
 static int jacobi32_sweep(const TriPlan& P, const T* val, const T* r, const T* yin, T* yout) {
-  const int nb = ceil_div(P.n, 8);
-  TimedRegion tr(P.upper ? "block_jacobi_sweep_upper" : "block_jacobi_sweep_lower");
+  const int nb = ceil_div(P.n, Lanes<T>::RPB);
+  // algorithmic work: off-diagonal values and columns, the diagonal, row
+  // pointers, r, the previous iterate (when read) and the new one
+  const double offd = (double)P.nnz - P.n;
+  const double bytes = (4.0 + sizeof(T)) * (yin ? offd : 0.0) + sizeof(T) * (double)P.n +
+                       4.0 * (P.n + 1) + 32.0 * sizeof(T) * P.n * (yin ? 3.0 : 2.0);
+  TimedRegion tr(P.upper ? "block_jacobi_sweep_upper" : "block_jacobi_sweep_lower",
+                 64.0 * (yin ? offd : 0.0) + 32.0 * P.n, bytes);
   if (P.upper)
     jacobi32_kernel<T, true><<<nb, 256, 0, stream()>>>(P.n, P.row.as<int>(), P.col.as<int>(), val, r,
                                                        yin, yout);
@@ -247,6 +325,32 @@ int lace_parilu_apply_block(lace_parilu* Ph, lace_trsv_kind_t kind, int jacobi_s
                               static_cast<float*>(Z), t1.as<float>(), t2.as<float>());
   LACE_TRY(rc);
   LACE_HIP(hipStreamSynchronize(stream()));
+  return DEV_SUCCESS;
+}
+
+}  // extern "C"
+
+namespace lace {
+
+__global__ void daxpby_kernel(size_t m, double a, const double* __restrict__ x, double b,
+                              const double* __restrict__ y, double* __restrict__ out) {
+  const size_t e = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (e >= m) return;
+  double v = 0.0;
+  if (a != 0.0) v = a * x[e];
+  if (b != 0.0) v = v + b * y[e];
+  out[e] = v;
+}
+
+}  // namespace lace
+
+extern "C" {
+
+int lace_daxpby(size_t m, double a, const double* x, double b, const double* y, double* out) {
+  LACE_TRY(ensure_init());
+  if (m == 0) return DEV_SUCCESS;
+  daxpby_kernel<<<ceil_div((long)m, 256), 256, 0, stream()>>>(m, a, x, b, y, out);
+  LACE_HIP(hipGetLastError());
   return DEV_SUCCESS;
 }
 

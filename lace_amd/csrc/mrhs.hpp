@@ -16,6 +16,8 @@ constexpr int kNrhs = 32;
 // bit (row sums in column order from zero, no contraction).
 template <typename TX>
 int spmm32(const DevCSR& A, const TX* X, const double* B, double* Y, bool residual);
+// Y = A X with FP32 values, X and Y (FP32 accumulation).
+int spmm32_f32(const DevCSR& A, const float* X, float* Y);
 
 // Z = M^{-1} R (trans = 0) or M^{-T} R (trans = 1) with `sweeps` Jacobi
 // sweeps per triangle, in the factor precision T; tmp and tmp2 are n x 32

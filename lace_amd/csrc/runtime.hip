@@ -88,6 +88,7 @@ struct TimingEntry {
   std::vector<std::pair<hipEvent_t, hipEvent_t>> pending;
   double total_ms = 0.0;
   long count = 0;
+  double flops = 0.0, bytes = 0.0;
 };
 static std::mutex g_tmu;
 static std::map<std::string, TimingEntry> g_timing;
@@ -117,11 +118,19 @@ TimedRegion::TimedRegion(const char* n) : name(n) {
   on = hipEventRecord(a, stream()) == hipSuccess;
 }
 
+TimedRegion::TimedRegion(const char* n, double flops_, double bytes_) : TimedRegion(n) {
+  flops = flops_;
+  bytes = bytes_;
+}
+
 TimedRegion::~TimedRegion() {
   if (!on) return;
   (void)hipEventRecord(b, stream());
   std::lock_guard<std::mutex> lk(g_tmu);
-  g_timing[name].pending.emplace_back(a, b);
+  TimingEntry& e = g_timing[name];
+  e.pending.emplace_back(a, b);
+  e.flops += flops;
+  e.bytes += bytes;
 }
 
 static void drain_timing() {
@@ -245,6 +254,15 @@ int lace_timing_get(const char* name, double* total_ms, long* count) {
   *total_ms = it->second.total_ms;
   *count = it->second.count;
   return DEV_SUCCESS;
+}
+
+int lace_timing_get_work(const char* name, double* flops, double* bytes) {
+  drain_timing();
+  std::lock_guard<std::mutex> lk(g_tmu);
+  auto it = g_timing.find(name);
+  *flops = it == g_timing.end() ? 0.0 : it->second.flops;
+  *bytes = it == g_timing.end() ? 0.0 : it->second.bytes;
+  return it == g_timing.end() ? DEV_ERR_NOT_FOUND : DEV_SUCCESS;
 }
 
 }  // extern "C"

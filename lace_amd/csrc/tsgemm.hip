@@ -9,8 +9,9 @@
 //   nn: out = base - sum_b X_b D_b over rows [rlo, n) (D_b 32 x 32); rows
 //       below rlo keep base.  base is a block, zero, or an identity block
 //       column (1 at row e0 + c, column c).
-// Both run on v_mfma_f64_16x16x4f64: a wave owns one 16 x 16 output tile
-// and steps the contraction 4 at a time.  Operand layout (CDNA3/4): A
+// Both run on v_mfma_f64_16x16x4f64: a wave owns 16 x 16 output tiles and
+// steps the contraction 4 at a time, with operands read from LDS tiles that
+// the workgroup stages by coalesced 16-byte loads (whole 256-byte rows).  Operand layout (CDNA3/4): A
 // (16 x 4): lane l holds A[l % 16][l / 16]; B (4 x 16): lane l holds
 // B[l / 16][l % 16]; C (16 x 16): lane l holds rows 4*v + l/16, v < 4,
 // of column l % 16.
@@ -28,51 +29,76 @@ __device__ __forceinline__ d4 mfma4(double a, double b, d4 c) {
   return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
 }
 
-constexpr int kTnBlocksPerGroup = 4;  // X blocks per workgroup (4 waves = 4 tiles each)
+constexpr int kPitch = 34;   // LDS row pitch in doubles: 16-byte aligned rows, banks spread
+constexpr int kTnG = 8;      // X blocks per workgroup
+constexpr int kTnRows = 16;  // rows per LDS chunk
+
+// 16-byte word e of a 32-column FP64 row chunk: row e / 16, columns
+// 2 (e % 16) and 2 (e % 16) + 1.
+__device__ __forceinline__ double2 ld2(const double* p) { return *reinterpret_cast<const double2*>(p); }
+__device__ __forceinline__ void st2(double* p, double2 v) { *reinterpret_cast<double2*>(p) = v; }
 
 // grid (nslab, ngroups); workgroup (slab s, group g) handles rows
-// [s*rpb, (s+1)*rpb) and blocks [4g, 4g+4).  Wave w owns tile (w/2, w%2)
-// of every block of its group.
+// [s*rpb, (s+1)*rpb) and blocks [8g, 8g+8).  Each 16-row chunk of W and of
+// the group's X blocks is staged in LDS with coalesced 16-byte loads (a
+// half-wave reads one whole 256-byte row); wave w then owns tile (w/2, w%2)
+// of every block: A = X_b^T (lane l: X_b[row k0 + l/16][16 ah + l%16]),
+// B = W (lane l: W[row k0 + l/16][16 bh + l%16]).
 __global__ __launch_bounds__(256) void tn_partial_kernel(int n, int rpb, BlockList X,
                                                          const double* __restrict__ W, int rs_main,
                                                          int rs_last, double* __restrict__ partial) {
+  __shared__ double Ws[kTnRows][kPitch];
+  __shared__ double Xs[kTnG][kTnRows][kPitch];
   const int nb = X.nb;
   const int slab = blockIdx.x, g = blockIdx.y;
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
   const int ah = w >> 1, bh = w & 1;
   const int r_begin = slab * rpb, r_end = min(r_begin + rpb, n);
-  d4 acc[kTnBlocksPerGroup];
-  const double* xb[kTnBlocksPerGroup];
-  int rsb[kTnBlocksPerGroup];
+  const int b0 = g * kTnG, gb = min(kTnG, nb - b0);
+  d4 acc[kTnG];
+  const double* xb[kTnG];
+  int rsb[kTnG];
 #pragma unroll
-  for (int t = 0; t < kTnBlocksPerGroup; t++) {
-    acc[t] = d4{0.0, 0.0, 0.0, 0.0};
-    const int b = g * kTnBlocksPerGroup + t;
-    xb[t] = b < nb ? static_cast<const double*>(X.p[b]) : nullptr;
-    rsb[t] = b == nb - 1 ? rs_last : rs_main;
+  for (int q = 0; q < kTnG; q++) {
+    acc[q] = d4{0.0, 0.0, 0.0, 0.0};
+    const int b = b0 + q;
+    xb[q] = q < gb ? static_cast<const double*>(X.p[b]) : nullptr;
+    rsb[q] = b == nb - 1 ? rs_last : rs_main;
   }
-  const int col_a = ah * 16 + (lane & 15), col_b = bh * 16 + (lane & 15);
-  for (int r = r_begin; r < r_end; r += 4) {
-    const int row = r + (lane >> 4);
+  // staging: 16 rows x 16 words = 256 words per operand, one per thread
+  const int srow = t >> 4, scol = (t & 15) * 2;
+  for (int r0 = r_begin; r0 < r_end; r0 += kTnRows) {
+    const int row = r0 + srow;
     const bool in = row < r_end;
-    const double bv = in ? W[(size_t)row * 32 + col_b] : 0.0;
+    const double2 z2 = make_double2(0.0, 0.0);
+    double2 wv = in ? ld2(W + (size_t)row * 32 + scol) : z2;
+    double2 xv[kTnG];
 #pragma unroll
-    for (int t = 0; t < kTnBlocksPerGroup; t++) {
-      if (!xb[t]) continue;
-      const double av = (in && row >= rsb[t]) ? xb[t][(size_t)row * 32 + col_a] : 0.0;
-      acc[t] = mfma4(av, bv, acc[t]);
+    for (int q = 0; q < kTnG; q++)
+      xv[q] = (q < gb && in && row >= rsb[q]) ? ld2(xb[q] + (size_t)row * 32 + scol) : z2;
+    st2(&Ws[srow][scol], wv);
+#pragma unroll
+    for (int q = 0; q < kTnG; q++) st2(&Xs[q][srow][scol], xv[q]);
+    __syncthreads();
+#pragma unroll
+    for (int k0 = 0; k0 < kTnRows; k0 += 4) {
+      const int kr = k0 + (lane >> 4);
+      const double bv = Ws[kr][bh * 16 + (lane & 15)];
+#pragma unroll
+      for (int q = 0; q < kTnG; q++)
+        if (q < gb) acc[q] = mfma4(Xs[q][kr][ah * 16 + (lane & 15)], bv, acc[q]);
     }
+    __syncthreads();
   }
   // partial[slab][b][32][32], tile (ah, bh): rows ah*16 + 4*v + l/16, col bh*16 + l%16
 #pragma unroll
-  for (int t = 0; t < kTnBlocksPerGroup; t++) {
-    const int b = g * kTnBlocksPerGroup + t;
-    if (b >= nb) continue;
-    double* P = partial + ((size_t)slab * nb + b) * 1024;
+  for (int q = 0; q < kTnG; q++) {
+    if (q >= gb) continue;
+    double* P = partial + ((size_t)slab * nb + b0 + q) * 1024;
 #pragma unroll
     for (int v = 0; v < 4; v++) {
       const int rr = ah * 16 + 4 * v + (lane >> 4), cc = bh * 16 + (lane & 15);
-      P[rr * 32 + cc] = acc[t][v];
+      P[rr * 32 + cc] = acc[q][v];
     }
   }
 }
@@ -94,48 +120,84 @@ __global__ __launch_bounds__(256) void tn_reduce_kernel(int nslab, int count, co
   C[e] = (s0 + s1) + (s2 + s3);
 }
 
-// A workgroup owns a band of 32 rows; wave w computes the 16 x 16 tile of
-// rows band + 16*(w/2) .. +15 and columns 16*(w%2) .. +15.
+constexpr int kNnRows = 64;  // rows per workgroup
+
+// Element-wise widening of a staged 16-byte word.
+template <typename TX>
+struct Stage;
+template <>
+struct Stage<double> {
+  static constexpr int WORDS = 16;  // 16-byte words per 32-column row
+  __device__ __forceinline__ static void put(const double* src, double* dst) { st2(dst, ld2(src)); }
+};
+template <>
+struct Stage<float> {
+  static constexpr int WORDS = 8;
+  __device__ __forceinline__ static void put(const float* src, double* dst) {
+    const float4 v = *reinterpret_cast<const float4*>(src);
+    st2(dst, make_double2(v.x, v.y));
+    st2(dst + 2, make_double2(v.z, v.w));
+  }
+};
+
+// A workgroup owns a band of 64 rows; for each block the band of X_b and
+// D_b are staged in LDS (coalesced 16-byte loads), then wave w computes
+// rows band + 16 w .. +15 in both 16-column halves: A = X_b (lane l:
+// X_b[16 w + l%16][k0 + l/16]), B = D_b (lane l: D_b[k0 + l/16][16 ch + l%16]).
 template <typename TX, typename TO>
 __global__ __launch_bounds__(256) void nn_kernel(int n, BlockList X,
                                                  const double* __restrict__ D, const double* base,
                                                  int base_mode, int e0, int rlo, TO* out) {
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  const int band = xcd_block(blockIdx.x, gridDim.x) * 32;  // 32 rows per workgroup
-  const int rt = band + 16 * (w >> 1);                      // tile's first row
-  const int ch = w & 1;
-  // A operand: X[rt + l%16][k0 + l/16]; B operand: D[k0 + l/16][ch*16 + l%16]
-  const int arow = rt + (lane & 15);
-  d4 acc = d4{0.0, 0.0, 0.0, 0.0};
-  const bool arow_in = arow < n && arow >= rlo;
+  __shared__ double Xs[kNnRows][kPitch];
+  __shared__ double Ds[32][kPitch];
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  const int band = xcd_block(blockIdx.x, gridDim.x) * kNnRows;
+  constexpr int WPR = Stage<TX>::WORDS, EPW = 32 / WPR;  // words per row, elements per word
+  d4 acc0 = d4{0.0, 0.0, 0.0, 0.0}, acc1 = d4{0.0, 0.0, 0.0, 0.0};
   for (int b = 0; b < X.nb; b++) {
     const TX* Xb = static_cast<const TX*>(X.p[b]);
     const double* Db = D + (size_t)b * 1024;
+    for (int e = t; e < kNnRows * WPR; e += 256) {
+      const int rr = e / WPR, cw = (e % WPR) * EPW;
+      const int row = band + rr;
+      if (row < n && row >= rlo) {
+        Stage<TX>::put(Xb + (size_t)row * 32 + cw, &Xs[rr][cw]);
+      } else {
+        for (int q = 0; q < EPW; q++) Xs[rr][cw + q] = 0.0;
+      }
+    }
+    for (int e = t; e < 512; e += 256) st2(&Ds[e >> 4][(e & 15) * 2], ld2(Db + (e >> 4) * 32 + (e & 15) * 2));
+    __syncthreads();
 #pragma unroll
     for (int k0 = 0; k0 < 32; k0 += 4) {
       const int kk = k0 + (lane >> 4);
-      const double av = arow_in ? (double)Xb[(size_t)arow * 32 + kk] : 0.0;
-      const double bv = Db[kk * 32 + ch * 16 + (lane & 15)];
-      acc = mfma4(av, bv, acc);
+      const double av = Xs[16 * w + (lane & 15)][kk];
+      acc0 = mfma4(av, Ds[kk][lane & 15], acc0);
+      acc1 = mfma4(av, Ds[kk][16 + (lane & 15)], acc1);
     }
+    __syncthreads();
   }
-  // C layout: rows rt + 4*v + l/16, column ch*16 + l%16
-  const int col = ch * 16 + (lane & 15);
+  // C layout: rows band + 16 w + 4 v + l/16, columns 16 ch + l%16
 #pragma unroll
-  for (int v = 0; v < 4; v++) {
-    const int row = rt + 4 * v + (lane >> 4);
-    if (row >= n) continue;
-    double bval = 0.0;
-    if (base_mode == 1) bval = base[(size_t)row * 32 + col];
-    else if (base_mode == 2) bval = (row - e0 == col) ? 1.0 : 0.0;
-    out[(size_t)row * 32 + col] = (TO)(row >= rlo ? bval - acc[v] : bval);
+  for (int ch = 0; ch < 2; ch++) {
+    const int col = ch * 16 + (lane & 15);
+#pragma unroll
+    for (int v = 0; v < 4; v++) {
+      const int row = band + 16 * w + 4 * v + (lane >> 4);
+      if (row >= n) continue;
+      double bval = 0.0;
+      if (base_mode == 1) bval = base[(size_t)row * 32 + col];
+      else if (base_mode == 2) bval = (row - e0 == col) ? 1.0 : 0.0;
+      const double a = ch == 0 ? acc0[v] : acc1[v];
+      out[(size_t)row * 32 + col] = (TO)(row >= rlo ? bval - a : bval);
+    }
   }
 }
 
 int TsGemm::init(int n_) {
   n = n_;
-  // about 1024 row slabs, a multiple of 4 rows each
-  rpb = std::max(256, ((n + 1023) / 1024 + 3) / 4 * 4);
+  // about 1024 row slabs, a multiple of the 16-row staging chunk each
+  rpb = std::max(256, ((n + 1023) / 1024 + 15) / 16 * 16);
   nslab = ceil_div(n, rpb);
   LACE_TRY(partial.alloc(sizeof(double) * 1024 * (size_t)nslab * kMaxBlocks));
   return DEV_SUCCESS;
@@ -143,8 +205,12 @@ int TsGemm::init(int n_) {
 
 int TsGemm::tn(const BlockList& L, const double* W, int rs_main, int rs_last, double* C) {
   LACE_REQUIRE(L.nb >= 1 && L.nb <= kMaxBlocks, DEV_ERR_ILLEGAL_VALUE, "tsgemm: %d blocks", L.nb);
-  TimedRegion tr("tsgemm_tn");
-  dim3 grid(nslab, ceil_div(L.nb, kTnBlocksPerGroup));
+  // algorithmic work: 2*32*32 flops per block row; each X_b row and each W
+  // row read once, the 32 x 32 results written once
+  const double rows_x = (double)(L.nb - 1) * (n - rs_main) + (double)(n - rs_last);
+  TimedRegion tr("tsgemm_tn", 2048.0 * rows_x,
+                 256.0 * (rows_x + (n - std::min(rs_main, rs_last))) + 8192.0 * L.nb);
+  dim3 grid(nslab, ceil_div(L.nb, kTnG));
   tn_partial_kernel<<<grid, 256, 0, stream()>>>(n, rpb, L, W, rs_main, rs_last, partial.as<double>());
   const int count = L.nb * 1024;
   tn_reduce_kernel<<<ceil_div(count, 256), 256, 0, stream()>>>(nslab, count, partial.as<double>(), C);
@@ -156,8 +222,13 @@ template <typename TX, typename TO>
 int TsGemm::nn(const BlockList& L, const double* D, const double* base, int base_mode, int e0, int rlo,
                TO* out) {
   LACE_REQUIRE(L.nb >= 0 && L.nb <= kMaxBlocks, DEV_ERR_ILLEGAL_VALUE, "tsgemm: %d blocks", L.nb);
-  TimedRegion tr("tsgemm_nn");
-  nn_kernel<TX, TO><<<ceil_div(n, 32), 256, 0, stream()>>>(n, L, D, base, base_mode, e0, rlo, out);
+  // algorithmic work: 2*32*32 flops per block row below rlo; X_b rows from
+  // rlo read once, D once, out written once, base read once (mode 1)
+  const double rows = (double)std::max(0, n - rlo);
+  TimedRegion tr("tsgemm_nn", 2048.0 * L.nb * rows,
+                 32.0 * sizeof(TX) * L.nb * rows + 8192.0 * L.nb + 32.0 * sizeof(TO) * n +
+                     (base_mode == 1 ? 256.0 * n : 0.0));
+  nn_kernel<TX, TO><<<ceil_div(n, kNnRows), 256, 0, stream()>>>(n, L, D, base, base_mode, e0, rlo, out);
   LACE_HIP(hipGetLastError());
   return DEV_SUCCESS;
 }
