@@ -505,15 +505,17 @@ typedef struct lace_block_stats {
 /* Block Householder FGMRES(search_max) on the operator of S (trans = 0: A;
  * 1: A^T with M^{-T}): 32 right-hand sides share one block Krylov space.
  * Per block step: preconditioned block Z_k = M^{-1} V_k (Jacobi sweeps of
- * the ILU factors in their precision, Z stored in it), W = A Z_k (FP64 A),
+ * the ILU factors in their precision, Z stored in it; with inner_iters > 0
+ * and FP32 factors, inner_iters steps of batched FP32 GMRES per column,
+ * each preconditioned by the FP32 ILU), W = A Z_k (FP64 A),
  * the FP64 block reflectors applied to W as one compact-WY product (MFMA
  * Gram X^T W and update W - X D), the new panel's Householder QR by
  * CholeskyQR2 (three passes after a shifted first one) and reconstruction,
  * and the block Hessenberg least squares by 64 x 32 Householder windows on
  * device.  Column c converges when its residual estimate is <= rtol *
  * ||b_c - A x0_c||; the solve stops when all have.  X is the initial guess
- * on entry and the solution on exit.  Needs trsv = Jacobi (or none), no
- * inner iterations, n >= 32 (search_max + 2), search_max <= 39. */
+ * on entry and the solution on exit.  Needs trsv = Jacobi (or none),
+ * inner_iters <= 12, n >= 32 (search_max + 2), search_max <= 39. */
 int lace_solver_solve_block(lace_solver *S, const double *B, double *X, int trans,
                             lace_block_stats *st);
 /* dtheta[k] = -sum_c Lambda[row(k)][c] * X[col(k)][c]: the gradient of

@@ -1,0 +1,280 @@
+// binner.hip -- batched inner FP32 GMRES: 32 independent right-preconditioned
+// GMRES(m) iterations in lockstep on n x 32 row-major FP32 blocks.
+//
+// This is the inner solver of the nested mixed-precision scheme (an FP64
+// flexible outer iteration preconditioned by a fixed number of FP32 Krylov
+// steps), here for the 32 columns of the block solver at once: the ILU
+// sweeps and the FP32 product read the factors and A once for all columns,
+// and every vector access is a whole 128-byte row.  Per step j (column c):
+//   p_j = M^{-1} q_j, w = A p_j, h_ij = q_i . w (i <= j, classical
+//   Gram-Schmidt, FP64 sums), w <- w - sum_i h_ij q_i, h_{j+1,j} = ||w||,
+//   q_{j+1} = w / h_{j+1,j}, Givens rotations on the Hessenberg column;
+// after m steps z = sum_j p_j y_j with y from the triangular least squares.
+// Dot products and norms are reduced per 2048-row slab and then over the
+// slabs in a fixed order, so results do not depend on scheduling.
+#include <algorithm>
+#include <cmath>
+
+#include "binner.hpp"
+#include "mrhs.hpp"
+
+namespace lace {
+
+// FP64 state per column c at [x * 32 + c]: g (m+1), cs (m), sn (m),
+// R (m x m, R[i][j] at i*m + j), y (m), h (m+1), nrm2 (1).
+struct InnerLayout {
+  int m;
+  __host__ __device__ int g() const { return 0; }
+  __host__ __device__ int cs() const { return (m + 1) * 32; }
+  __host__ __device__ int sn() const { return cs() + m * 32; }
+  __host__ __device__ int R() const { return sn() + m * 32; }
+  __host__ __device__ int y() const { return R() + m * m * 32; }
+  __host__ __device__ int h() const { return y() + m * 32; }
+  __host__ __device__ int nrm2() const { return h() + (m + 1) * 32; }
+  __host__ __device__ int total() const { return nrm2() + 32; }
+};
+
+__device__ __forceinline__ void load4f(const float* p, double (&v)[4]) {
+  const float4 a = *reinterpret_cast<const float4*>(p);
+  v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w;
+}
+
+// partial[slab][i][c] = sum over the slab's rows of q_i[row][c] w[row][c].
+// Thread t: column group 4 (t % 8), rows t / 8 + 32 k of the slab.
+__global__ __launch_bounds__(256) void bi_dots_kernel(int n, int rpb, int nq, const float* __restrict__ Q,
+                                                      size_t bstride, const float* __restrict__ w,
+                                                      double* __restrict__ partial) {
+  __shared__ double red[4][kMaxInner + 1][32];
+  const int t = threadIdx.x, lane = t & 63, wv = t >> 6, cg = t & 7, rg = t >> 3;
+  const int slab = blockIdx.x;
+  const int r0 = slab * rpb, r1 = min(n, r0 + rpb);
+  double acc[kMaxInner + 1][4];
+#pragma unroll
+  for (int i = 0; i <= kMaxInner; i++)
+#pragma unroll
+    for (int e = 0; e < 4; e++) acc[i][e] = 0.0;
+  for (int row = r0 + rg; row < r1; row += 32) {
+    double wv4[4];
+    load4f(w + (size_t)row * 32 + 4 * cg, wv4);
+#pragma unroll
+    for (int i = 0; i <= kMaxInner; i++) {
+      if (i < nq) {
+        double q[4];
+        load4f(Q + i * bstride + (size_t)row * 32 + 4 * cg, q);
+#pragma unroll
+        for (int e = 0; e < 4; e++) acc[i][e] += q[e] * wv4[e];
+      }
+    }
+  }
+#pragma unroll
+  for (int i = 0; i <= kMaxInner; i++) {
+    if (i >= nq) continue;
+#pragma unroll
+    for (int e = 0; e < 4; e++) {
+      double v = acc[i][e];
+      v += __shfl_xor(v, 8, 64);
+      v += __shfl_xor(v, 16, 64);
+      v += __shfl_xor(v, 32, 64);
+      if (lane < 8) red[wv][i][4 * lane + e] = v;
+    }
+  }
+  __syncthreads();
+  for (int idx = t; idx < nq * 32; idx += 256) {
+    const int i = idx >> 5, c = idx & 31;
+    partial[((size_t)slab * nq + i) * 32 + c] = (red[0][i][c] + red[1][i][c]) + (red[2][i][c] + red[3][i][c]);
+  }
+}
+
+// out[e] = sum over slabs of partial[s][e] in slab order.
+__global__ __launch_bounds__(256) void bi_reduce_kernel(int nslab, int count, const double* __restrict__ partial,
+                                                        double* __restrict__ out) {
+  const int e = blockIdx.x * 256 + threadIdx.x;
+  if (e >= count) return;
+  double s0 = 0.0, s1 = 0.0;
+  int s = 0;
+  for (; s + 1 < nslab; s += 2) {
+    s0 += partial[(size_t)s * count + e];
+    s1 += partial[(size_t)(s + 1) * count + e];
+  }
+  if (s < nslab) s0 += partial[(size_t)s * count + e];
+  out[e] = s0 + s1;
+}
+
+// w <- w - sum_{i < nq} q_i h_i (per column); partial[slab][c] = sum of the
+// squares of the new (FP32-rounded) w.
+__global__ __launch_bounds__(256) void bi_update_kernel(int n, int rpb, int nq, const float* __restrict__ Q,
+                                                        size_t bstride, const double* __restrict__ h,
+                                                        float* __restrict__ w, double* __restrict__ partial) {
+  __shared__ double red[4][32];
+  const int t = threadIdx.x, lane = t & 63, wv = t >> 6, cg = t & 7, rg = t >> 3;
+  const int slab = blockIdx.x;
+  const int r0 = slab * rpb, r1 = min(n, r0 + rpb);
+  double hc[kMaxInner + 1][4];
+#pragma unroll
+  for (int i = 0; i <= kMaxInner; i++)
+#pragma unroll
+    for (int e = 0; e < 4; e++) hc[i][e] = i < nq ? h[i * 32 + 4 * cg + e] : 0.0;
+  double s2[4] = {0.0, 0.0, 0.0, 0.0};
+  for (int row = r0 + rg; row < r1; row += 32) {
+    float* wp = w + (size_t)row * 32 + 4 * cg;
+    double x[4];
+    load4f(wp, x);
+#pragma unroll
+    for (int i = 0; i <= kMaxInner; i++) {
+      if (i < nq) {
+        double q[4];
+        load4f(Q + i * bstride + (size_t)row * 32 + 4 * cg, q);
+#pragma unroll
+        for (int e = 0; e < 4; e++) x[e] -= q[e] * hc[i][e];
+      }
+    }
+    const float4 o = make_float4((float)x[0], (float)x[1], (float)x[2], (float)x[3]);
+    *reinterpret_cast<float4*>(wp) = o;
+    s2[0] += (double)o.x * o.x;
+    s2[1] += (double)o.y * o.y;
+    s2[2] += (double)o.z * o.z;
+    s2[3] += (double)o.w * o.w;
+  }
+#pragma unroll
+  for (int e = 0; e < 4; e++) {
+    double v = s2[e];
+    v += __shfl_xor(v, 8, 64);
+    v += __shfl_xor(v, 16, 64);
+    v += __shfl_xor(v, 32, 64);
+    if (lane < 8) red[wv][4 * lane + e] = v;
+  }
+  __syncthreads();
+  if (t < 32) partial[(size_t)slab * 32 + t] = (red[0][t] + red[1][t]) + (red[2][t] + red[3][t]);
+}
+
+// q = w / sqrt(nrm2[c]) per column (zero where the norm is zero).
+__global__ __launch_bounds__(256) void bi_scale_kernel(size_t m, const float* __restrict__ w,
+                                                       const double* __restrict__ nrm2, float* __restrict__ q) {
+  const size_t e = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (e >= m) return;
+  const double s2 = nrm2[e & 31];
+  q[e] = s2 > 0.0 ? (float)((double)w[e] / sqrt(s2)) : 0.0f;
+}
+
+__global__ void bi_init_kernel(InnerLayout L, double* __restrict__ st) {
+  const int c = threadIdx.x;
+  if (c >= 32) return;
+  st[L.g() + c] = sqrt(st[L.nrm2() + c]);
+  for (int i = 1; i <= L.m; i++) st[L.g() + i * 32 + c] = 0.0;
+}
+
+// Givens step j on each column's Hessenberg column h[0..j], ||w||.
+__global__ void bi_givens_kernel(InnerLayout L, int j, double* __restrict__ st) {
+  const int c = threadIdx.x;
+  if (c >= 32) return;
+  double h[kMaxInner + 1];
+  for (int i = 0; i <= j; i++) h[i] = st[L.h() + i * 32 + c];
+  h[j + 1] = sqrt(st[L.nrm2() + c]);
+  for (int i = 0; i < j; i++) {
+    const double cs = st[L.cs() + i * 32 + c], sn = st[L.sn() + i * 32 + c];
+    const double a = cs * h[i] + sn * h[i + 1];
+    h[i + 1] = -sn * h[i] + cs * h[i + 1];
+    h[i] = a;
+  }
+  const double r = hypot(h[j], h[j + 1]);
+  double cs = 1.0, sn = 0.0;
+  if (r > 0.0) {
+    cs = h[j] / r;
+    sn = h[j + 1] / r;
+  }
+  h[j] = r;
+  st[L.cs() + j * 32 + c] = cs;
+  st[L.sn() + j * 32 + c] = sn;
+  const double gj = st[L.g() + j * 32 + c];
+  st[L.g() + (j + 1) * 32 + c] = -sn * gj;
+  st[L.g() + j * 32 + c] = cs * gj;
+  for (int i = 0; i <= j; i++) st[L.R() + (i * L.m + j) * 32 + c] = h[i];
+}
+
+// y = R^{-1} g on the first mu steps (a zero pivot gives y_j = 0).
+__global__ void bi_solve_kernel(InnerLayout L, int mu, double* __restrict__ st) {
+  const int c = threadIdx.x;
+  if (c >= 32) return;
+  for (int j = mu - 1; j >= 0; j--) {
+    double s = st[L.g() + j * 32 + c];
+    for (int i = j + 1; i < mu; i++) s -= st[L.R() + (j * L.m + i) * 32 + c] * st[L.y() + i * 32 + c];
+    const double d = st[L.R() + (j * L.m + j) * 32 + c];
+    st[L.y() + j * 32 + c] = d != 0.0 ? s / d : 0.0;
+  }
+}
+
+// Z[row][c] = sum_{j < mu} P_j[row][c] y_j[c].
+__global__ __launch_bounds__(256) void bi_combine_kernel(int n, int mu, const float* __restrict__ P, size_t bstride,
+                                                         const double* __restrict__ y, float* __restrict__ Z) {
+  const size_t e4 = (size_t)blockIdx.x * 256 + threadIdx.x;  // index of a 4-column group
+  if (e4 >= (size_t)n * 8) return;
+  const int cg = (int)(e4 & 7);
+  double acc[4] = {0.0, 0.0, 0.0, 0.0};
+  for (int j = 0; j < mu; j++) {
+    double p[4];
+    load4f(P + j * bstride + e4 * 4, p);
+#pragma unroll
+    for (int e = 0; e < 4; e++) acc[e] += p[e] * y[j * 32 + 4 * cg + e];
+  }
+  *reinterpret_cast<float4*>(Z + e4 * 4) = make_float4((float)acc[0], (float)acc[1], (float)acc[2], (float)acc[3]);
+}
+
+int InnerBatch::init(int n_, int m_) {
+  LACE_REQUIRE(m_ >= 1 && m_ <= kMaxInner, DEV_ERR_ILLEGAL_VALUE, "inner steps %d (1..%d)", m_, kMaxInner);
+  n = n_;
+  m = m_;
+  nslab = ceil_div(n, rpb);
+  const size_t blk = (size_t)n * 32 * sizeof(float);
+  LACE_TRY(Q.alloc(blk * (m + 1)));
+  LACE_TRY(P.alloc(blk * m));
+  LACE_TRY(w.alloc(blk));
+  LACE_TRY(t1.alloc(blk));
+  LACE_TRY(t2.alloc(blk));
+  InnerLayout L{m};
+  LACE_TRY(state.alloc(sizeof(double) * L.total()));
+  LACE_TRY(partial.alloc(sizeof(double) * (size_t)nslab * (kMaxInner + 1) * 32));
+  return DEV_SUCCESS;
+}
+
+int InnerBatch::solve(const DevCSR& A32, ParILU* F, lace_trsv_kind_t kind, int sweeps, int trans,
+                      const float* V, float* Z) {
+  TimedRegion tr("block_inner");
+  const size_t bs = (size_t)n * 32;
+  float* Qb = Q.as<float>();
+  float* Pb = P.as<float>();
+  float* wb = w.as<float>();
+  double* st = state.as<double>();
+  double* part = partial.as<double>();
+  InnerLayout L{m};
+  const int eb = ceil_div((long)bs, 256);
+  // beta_c = ||v_c||, q_0 = v / beta
+  bi_dots_kernel<<<nslab, 256, 0, stream()>>>(n, rpb, 1, V, bs, V, part);
+  bi_reduce_kernel<<<1, 256, 0, stream()>>>(nslab, 32, part, st + L.nrm2());
+  bi_init_kernel<<<1, 32, 0, stream()>>>(L, st);
+  bi_scale_kernel<<<eb, 256, 0, stream()>>>(bs, V, st + L.nrm2(), Qb);
+  for (int j = 0; j < m; j++) {
+    float* pj = Pb + j * bs;
+    LACE_TRY(block_precond<float>(F, kind, sweeps, trans, Qb + j * bs, pj, t1.as<float>(), t2.as<float>()));
+    LACE_TRY(spmm32_f32(A32, pj, wb));
+    {
+      // algorithmic bytes: q_0..q_j and w read once
+      TimedRegion td("inner_dots", 2.0 * (j + 1) * bs, 4.0 * (j + 2) * bs);
+      bi_dots_kernel<<<nslab, 256, 0, stream()>>>(n, rpb, j + 1, Qb, bs, wb, part);
+    }
+    bi_reduce_kernel<<<ceil_div((j + 1) * 32, 256), 256, 0, stream()>>>(nslab, (j + 1) * 32, part, st + L.h());
+    {
+      // q_0..q_j and w read once, w written once
+      TimedRegion tu("inner_update", 2.0 * (j + 1) * bs, 4.0 * (j + 3) * bs);
+      bi_update_kernel<<<nslab, 256, 0, stream()>>>(n, rpb, j + 1, Qb, bs, st + L.h(), wb, part);
+    }
+    bi_reduce_kernel<<<1, 256, 0, stream()>>>(nslab, 32, part, st + L.nrm2());
+    bi_givens_kernel<<<1, 32, 0, stream()>>>(L, j, st);
+    if (j + 1 < m) bi_scale_kernel<<<eb, 256, 0, stream()>>>(bs, wb, st + L.nrm2(), Qb + (j + 1) * bs);
+  }
+  bi_solve_kernel<<<1, 32, 0, stream()>>>(L, m, st);
+  bi_combine_kernel<<<ceil_div((long)n * 8, 256), 256, 0, stream()>>>(n, m, Pb, bs, st + L.y(), Z);
+  LACE_HIP(hipGetLastError());
+  return DEV_SUCCESS;
+}
+
+}  // namespace lace

@@ -289,8 +289,17 @@ static int fgmres_t(BlockEngine& E, const BlockSolveArgs& g, const double* B, do
       LACE_TRY((E.tg.nn<double, T>(Yl, E.D, nullptr, 2, k * 32, 0, E.Vt.as<T>())));
       // Z_k = M^{-1} V_k; W = A Z_k
       T* zk = E.Z[k]->as<T>();
-      LACE_TRY(block_precond<T>(g.P, g.kind, g.sweeps, g.trans, E.Vt.as<T>(), zk, E.t1.as<T>(),
-                                E.t2.as<T>()));
+      if constexpr (std::is_same<T, float>::value) {
+        if (g.inner_iters > 0) {
+          LACE_TRY(E.inner->solve(*g.A32, g.P, g.kind, g.sweeps, g.trans, E.Vt.as<float>(), zk));
+        } else {
+          LACE_TRY(block_precond<T>(g.P, g.kind, g.sweeps, g.trans, E.Vt.as<T>(), zk, E.t1.as<T>(),
+                                    E.t2.as<T>()));
+        }
+      } else {
+        LACE_TRY(block_precond<T>(g.P, g.kind, g.sweeps, g.trans, E.Vt.as<T>(), zk, E.t1.as<T>(),
+                                  E.t2.as<T>()));
+      }
       LACE_TRY(spmm32<T>(A, zk, nullptr, W, false));
       // W <- Q^T W
       LACE_TRY(E.tg.tn(Yl, W, 0, 0, E.Gk));
@@ -342,6 +351,15 @@ static int fgmres_t(BlockEngine& E, const BlockSolveArgs& g, const double* B, do
 int block_fgmres(BlockEngine& E, const BlockSolveArgs& g, const double* B, double* X,
                  lace_block_stats* st) {
   memset(st, 0, sizeof(*st));
+  if (g.inner_iters > 0) {
+    LACE_REQUIRE(E.zf && g.A32, DEV_ERR_NOT_SUPPORTED, "inner iterations need FP32 factors and A");
+    if (!E.inner || E.inner->m != g.inner_iters) {
+      E.inner.reset();
+      auto I = std::make_unique<InnerBatch>();
+      LACE_TRY(I->init(E.n, g.inner_iters));
+      E.inner = std::move(I);
+    }
+  }
   const auto t0 = std::chrono::steady_clock::now();
   int rc;
   if (E.zf) rc = fgmres_t<float>(E, g, B, X, st);

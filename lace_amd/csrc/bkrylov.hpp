@@ -4,6 +4,7 @@
 #include <memory>
 #include <vector>
 
+#include "binner.hpp"
 #include "common.hpp"
 #include "mrhs.hpp"
 #include "parilu.hpp"
@@ -25,6 +26,7 @@ struct BlockEngine {
   DevBuf small, ints, partial;
   TsGemm tg;
   HessState H;
+  std::unique_ptr<InnerBatch> inner;  // nested FP32 inner iterations, when used
   int ldT = 0;  // leading dimension of the accumulated T, (m + 1) * 32
   double *Tall = nullptr, *a = nullptr, *D = nullptr, *Gk = nullptr, *Gp = nullptr, *Km = nullptr,
          *tmpM = nullptr, *Dneg = nullptr, *G1 = nullptr, *Rl = nullptr, *Rlinv = nullptr,
@@ -39,8 +41,11 @@ struct BlockEngine {
 };
 
 struct BlockSolveArgs {
-  const DevCSR* A;  // operator (A or A^T), FP64 values
-  ParILU* P;        // ILU factors (precision decides T)
+  const DevCSR* A;    // operator (A or A^T), FP64 values
+  const DevCSR* A32;  // its FP32 copy (inner iterations only)
+  ParILU* P;          // ILU factors (precision decides T)
+  int inner_iters;    // > 0: each outer step preconditions with this many
+                      // batched inner FP32 GMRES steps (FP32 factors)
   lace_trsv_kind_t kind;
   int sweeps, trans, restart_max;
   double rtol;

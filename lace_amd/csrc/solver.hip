@@ -763,7 +763,8 @@ int lace_solver_solve_block(lace_solver* Sh, const double* B, double* X, int tra
   auto* S = reinterpret_cast<Solver*>(Sh);
   const int m = S->o.search_max;
   LACE_REQUIRE(!S->block && S->P, DEV_ERR_NOT_SUPPORTED, "block solve: CSR operator with ILU factors");
-  LACE_REQUIRE(S->o.inner_iters == 0, DEV_ERR_NOT_SUPPORTED, "block solve: no inner iterations");
+  LACE_REQUIRE(S->o.inner_iters >= 0 && S->o.inner_iters <= kMaxInner, DEV_ERR_NOT_SUPPORTED,
+               "block solve: inner iterations %d outside 0..%d", S->o.inner_iters, kMaxInner);
   LACE_REQUIRE(S->o.trsv == LACE_TRSV_JACOBI || S->o.trsv == LACE_TRSV_NONE, DEV_ERR_NOT_SUPPORTED,
                "block solve: Jacobi (or no) triangular solves");
   LACE_REQUIRE(m >= 1 && m + 1 <= kMaxBlocks, DEV_ERR_ILLEGAL_VALUE, "block solve: search_max %d", m);
@@ -779,6 +780,8 @@ int lace_solver_solve_block(lace_solver* Sh, const double* B, double* X, int tra
   }
   BlockSolveArgs args;
   args.A = trans ? &S->AT64 : &S->A64;
+  args.A32 = S->o.inner_iters > 0 ? (trans ? &S->AT32 : &S->A32) : nullptr;
+  args.inner_iters = S->o.inner_iters;
   args.P = S->P;
   args.kind = S->o.trsv;
   args.sweeps = S->o.jacobi_sweeps;
