@@ -17,6 +17,7 @@
 
 #include "binner.hpp"
 #include "mrhs.hpp"
+#include "smalldense.hpp"
 
 namespace lace {
 
@@ -83,21 +84,6 @@ __global__ __launch_bounds__(256) void bi_dots_kernel(int n, int rpb, int nq, co
     const int i = idx >> 5, c = idx & 31;
     partial[((size_t)slab * nq + i) * 32 + c] = (red[0][i][c] + red[1][i][c]) + (red[2][i][c] + red[3][i][c]);
   }
-}
-
-// out[e] = sum over slabs of partial[s][e] in slab order.
-__global__ __launch_bounds__(256) void bi_reduce_kernel(int nslab, int count, const double* __restrict__ partial,
-                                                        double* __restrict__ out) {
-  const int e = blockIdx.x * 256 + threadIdx.x;
-  if (e >= count) return;
-  double s0 = 0.0, s1 = 0.0;
-  int s = 0;
-  for (; s + 1 < nslab; s += 2) {
-    s0 += partial[(size_t)s * count + e];
-    s1 += partial[(size_t)(s + 1) * count + e];
-  }
-  if (s < nslab) s0 += partial[(size_t)s * count + e];
-  out[e] = s0 + s1;
 }
 
 // w <- w - sum_{i < nq} q_i h_i (per column); partial[slab][c] = sum of the
@@ -249,7 +235,7 @@ int InnerBatch::solve(const DevCSR& A32, ParILU* F, lace_trsv_kind_t kind, int s
   const int eb = ceil_div((long)bs, 256);
   // beta_c = ||v_c||, q_0 = v / beta
   bi_dots_kernel<<<nslab, 256, 0, stream()>>>(n, rpb, 1, V, bs, V, part);
-  bi_reduce_kernel<<<1, 256, 0, stream()>>>(nslab, 32, part, st + L.nrm2());
+  LACE_TRY(sd_slab_reduce(nslab, 32, part, st + L.nrm2(), false));
   bi_init_kernel<<<1, 32, 0, stream()>>>(L, st);
   bi_scale_kernel<<<eb, 256, 0, stream()>>>(bs, V, st + L.nrm2(), Qb);
   for (int j = 0; j < m; j++) {
@@ -261,13 +247,13 @@ int InnerBatch::solve(const DevCSR& A32, ParILU* F, lace_trsv_kind_t kind, int s
       TimedRegion td("inner_dots", 2.0 * (j + 1) * bs, 4.0 * (j + 2) * bs);
       bi_dots_kernel<<<nslab, 256, 0, stream()>>>(n, rpb, j + 1, Qb, bs, wb, part);
     }
-    bi_reduce_kernel<<<ceil_div((j + 1) * 32, 256), 256, 0, stream()>>>(nslab, (j + 1) * 32, part, st + L.h());
+    LACE_TRY(sd_slab_reduce(nslab, (j + 1) * 32, part, st + L.h(), false));
     {
       // q_0..q_j and w read once, w written once
       TimedRegion tu("inner_update", 2.0 * (j + 1) * bs, 4.0 * (j + 3) * bs);
       bi_update_kernel<<<nslab, 256, 0, stream()>>>(n, rpb, j + 1, Qb, bs, st + L.h(), wb, part);
     }
-    bi_reduce_kernel<<<1, 256, 0, stream()>>>(nslab, 32, part, st + L.nrm2());
+    LACE_TRY(sd_slab_reduce(nslab, 32, part, st + L.nrm2(), false));
     bi_givens_kernel<<<1, 32, 0, stream()>>>(L, j, st);
     if (j + 1 < m) bi_scale_kernel<<<eb, 256, 0, stream()>>>(bs, wb, st + L.nrm2(), Qb + (j + 1) * bs);
   }

@@ -529,12 +529,26 @@ __global__ __launch_bounds__(256) void colnorm_partial_kernel(int n, const doubl
   }
 }
 
-__global__ void colnorm_final_kernel(int nslab, const double* __restrict__ partial, double* __restrict__ out) {
-  const int c = threadIdx.x;
-  if (c >= 32) return;
+// out[e] = sum_s partial[s * count + e] (sqrt of it with SQRT): one
+// workgroup per output, thread t summing slabs t, t + 256, ... in order, then
+// the fixed-order block tree -- deterministic.
+template <bool SQRT>
+__global__ __launch_bounds__(256) void slab_reduce_kernel(int nslab, int count, const double* __restrict__ partial,
+                                                          double* __restrict__ out) {
+  __shared__ double sm[4];
+  const int e = blockIdx.x;
   double s = 0.0;
-  for (int b = 0; b < nslab; b++) s += partial[(size_t)b * 32 + c];
-  out[c] = sqrt(s);
+  for (int q = threadIdx.x; q < nslab; q += 256) s += partial[(size_t)q * count + e];
+  s = block_sum<256>(s, sm);
+  if (threadIdx.x == 0) out[e] = SQRT ? sqrt(s) : s;
+}
+
+int sd_slab_reduce(int nslab, int count, const double* partial, double* out, bool take_sqrt) {
+  if (count <= 0) return DEV_SUCCESS;
+  if (take_sqrt) slab_reduce_kernel<true><<<count, 256, 0, stream()>>>(nslab, count, partial, out);
+  else slab_reduce_kernel<false><<<count, 256, 0, stream()>>>(nslab, count, partial, out);
+  LACE_HIP(hipGetLastError());
+  return DEV_SUCCESS;
 }
 
 size_t sd_colnorms_partial_elems(int n) { return (size_t)std::max(1, ceil_div(n, kNormRows)) * 32; }
@@ -542,9 +556,8 @@ size_t sd_colnorms_partial_elems(int n) { return (size_t)std::max(1, ceil_div(n,
 int sd_colnorms(int n, const double* X, double* partial, double* out) {
   const int nslab = std::max(1, ceil_div(n, kNormRows));
   colnorm_partial_kernel<<<nslab, 256, 0, stream()>>>(n, X, partial);
-  colnorm_final_kernel<<<1, 64, 0, stream()>>>(nslab, partial, out);
   LACE_HIP(hipGetLastError());
-  return DEV_SUCCESS;
+  return sd_slab_reduce(nslab, 32, partial, out, true);
 }
 
 }  // namespace lace

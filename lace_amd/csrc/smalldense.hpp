@@ -74,6 +74,10 @@ int sd_hess_step(const HessState& H, int k, const double* W, const double* Rt, d
 // diagonal-block inverses).
 int sd_backsolve(const HessState& H, int nblk, double* Dneg);
 
+// out[e] = sum over slabs s of partial[s * count + e] (its square root with
+// take_sqrt), e < count; fixed summation order.
+int sd_slab_reduce(int nslab, int count, const double* partial, double* out, bool take_sqrt);
+
 // out[c] = ||X[:, c]||_2 for an n x 32 block (fixed-order reduction);
 // partial holds ceil(n / 4096) * 32 doubles.
 int sd_colnorms(int n, const double* X, double* partial, double* out);

@@ -103,22 +103,38 @@ __global__ __launch_bounds__(256) void tn_partial_kernel(int n, int rpb, BlockLi
   }
 }
 
-// C[b][e] = sum over slabs in order (four independent accumulators).
-__global__ __launch_bounds__(256) void tn_reduce_kernel(int nslab, int count, const double* __restrict__ partial,
-                                                        double* __restrict__ C) {
+// Two-stage fixed-order reduction over the slabs: stage 1 sums slab group
+// g (blockIdx.y) with four independent accumulators, stage 2 adds the group
+// sums in group order.
+__global__ __launch_bounds__(256) void tn_reduce1_kernel(int nslab, int count, int per,
+                                                         const double* __restrict__ partial,
+                                                         double* __restrict__ mid) {
   const int e = blockIdx.x * 256 + threadIdx.x;
   if (e >= count) return;
+  const int g = blockIdx.y;
+  const int s0i = g * per, s1i = min(nslab, s0i + per);
   double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
-  int s = 0;
-  for (; s + 3 < nslab; s += 4) {
+  int s = s0i;
+  for (; s + 3 < s1i; s += 4) {
     s0 += partial[(size_t)s * count + e];
     s1 += partial[(size_t)(s + 1) * count + e];
     s2 += partial[(size_t)(s + 2) * count + e];
     s3 += partial[(size_t)(s + 3) * count + e];
   }
-  for (; s < nslab; s++) s0 += partial[(size_t)s * count + e];
-  C[e] = (s0 + s1) + (s2 + s3);
+  for (; s < s1i; s++) s0 += partial[(size_t)s * count + e];
+  mid[(size_t)g * count + e] = (s0 + s1) + (s2 + s3);
 }
+
+__global__ __launch_bounds__(256) void tn_reduce2_kernel(int ng, int count, const double* __restrict__ mid,
+                                                         double* __restrict__ C) {
+  const int e = blockIdx.x * 256 + threadIdx.x;
+  if (e >= count) return;
+  double s = 0.0;
+  for (int g = 0; g < ng; g++) s += mid[(size_t)g * count + e];
+  C[e] = s;
+}
+
+constexpr int kRedGroups = 16;
 
 constexpr int kNnRows = 64;  // rows per workgroup
 
@@ -200,6 +216,7 @@ int TsGemm::init(int n_) {
   rpb = std::max(256, ((n + 1023) / 1024 + 15) / 16 * 16);
   nslab = ceil_div(n, rpb);
   LACE_TRY(partial.alloc(sizeof(double) * 1024 * (size_t)nslab * kMaxBlocks));
+  LACE_TRY(mid.alloc(sizeof(double) * 1024 * (size_t)kRedGroups * kMaxBlocks));
   return DEV_SUCCESS;
 }
 
@@ -213,7 +230,10 @@ int TsGemm::tn(const BlockList& L, const double* W, int rs_main, int rs_last, do
   dim3 grid(nslab, ceil_div(L.nb, kTnG));
   tn_partial_kernel<<<grid, 256, 0, stream()>>>(n, rpb, L, W, rs_main, rs_last, partial.as<double>());
   const int count = L.nb * 1024;
-  tn_reduce_kernel<<<ceil_div(count, 256), 256, 0, stream()>>>(nslab, count, partial.as<double>(), C);
+  const int per = ceil_div(nslab, kRedGroups), ng = ceil_div(nslab, per);
+  tn_reduce1_kernel<<<dim3(ceil_div(count, 256), ng), 256, 0, stream()>>>(nslab, count, per,
+                                                                         partial.as<double>(), mid.as<double>());
+  tn_reduce2_kernel<<<ceil_div(count, 256), 256, 0, stream()>>>(ng, count, mid.as<double>(), C);
   LACE_HIP(hipGetLastError());
   return DEV_SUCCESS;
 }

@@ -16,7 +16,7 @@ struct BlockList {
 
 struct TsGemm {
   int n = 0, rpb = 0, nslab = 0;
-  DevBuf partial;
+  DevBuf partial, mid;  // per-slab partial products; per-slab-group sums
   int init(int n_);
   // C[b] (32 x 32, row-major, b < L.nb) = X_b^T W over rows [rs_b, n):
   // rs_b = rs_main for b < nb-1, rs_last for the last block.
