@@ -86,7 +86,7 @@ def cpu_baseline(row, col, val, b, threads):
                        f"substitution, {r.searches} iterations to rel. residual {rel:.2e}")}
 
 
-SWEEP_KERNEL = "parilu_stream_sweep_kernel<float>"
+SWEEP_KERNEL = "parilu_stream4_sweep_kernel<float>"
 JACOBI_KERNEL = "tri_jacobi_stream_kernel<float>"
 SPMV64_KERNEL = "spmv_stream_kernel<double, float, double>"
 SPMV32_KERNEL = "spmv_stream_kernel<float, float, float>"

@@ -366,12 +366,14 @@ double lace_parilu_sweep_bytes(const lace_parilu *P);
 int lace_parilu_apply(lace_parilu *P, lace_trsv_kind_t kind, int jacobi_sweeps,
                       int trans, const void *r, void *z);
 int lace_parilu_levels(const lace_parilu *P, int *nlev_L, int *nlev_U);
-/* Sweep kernel selection: 0 (default) runs the streaming symmetric kernel
- * (4) when A's pattern is structurally symmetric, else the general one;
+/* Sweep kernel selection: 0 (default) runs the four-position streaming
+ * symmetric kernel (5) when A's pattern is structurally symmetric, else the
+ * general one;
  * 1 always runs the general CSR-L / CSC-U kernel; 2 the per-position
  * symmetric kernel that merges rows i and j on every sweep; 3 the per-row
  * symmetric kernel (rows with at most 4 strict-lower entries); 4 the
- * streaming symmetric kernel (merge partners recorded once at set-up).
+ * streaming symmetric kernel (merge partners recorded once at set-up); 5 the
+ * streaming kernel with four positions per thread (16-byte accesses).
  * All give identical factors. */
 int lace_parilu_set_variant(lace_parilu *P, int variant);
 /* Non-zero if a sync-free level solve gave up waiting (a wait exceeded its

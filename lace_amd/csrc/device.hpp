@@ -53,6 +53,28 @@ __device__ __forceinline__ void load4(const double* p, double (&v)[4]) {
   v[0] = a.x; v[1] = a.y; v[2] = b.x; v[3] = b.y;
 }
 
+// W consecutive elements at a 16-byte aligned address, moved as 16-byte
+// words (W * sizeof(T) is a multiple of 16).
+template <typename T, int W>
+__device__ __forceinline__ void vload(const T* __restrict__ p, T (&v)[W]) {
+  constexpr int NQ = W * (int)sizeof(T) / 16;
+#pragma unroll
+  for (int q = 0; q < NQ; q++) {
+    const uint4 u = reinterpret_cast<const uint4*>(p)[q];
+    __builtin_memcpy(&v[q * 16 / (int)sizeof(T)], &u, 16);
+  }
+}
+template <typename T, int W>
+__device__ __forceinline__ void vstore(T* __restrict__ p, const T (&v)[W]) {
+  constexpr int NQ = W * (int)sizeof(T) / 16;
+#pragma unroll
+  for (int q = 0; q < NQ; q++) {
+    uint4 u;
+    __builtin_memcpy(&u, &v[q * 16 / (int)sizeof(T)], 16);
+    reinterpret_cast<uint4*>(p)[q] = u;
+  }
+}
+
 // CSR-stream staging: prod[k - c0] = val[k] * x[col[k]] for k in
 // [c0, c0 + cn), by NT threads.  Each thread takes aligned groups of four
 // entries (16-byte column and value loads; col and val must start on a

@@ -19,33 +19,12 @@
 #include <algorithm>
 
 #include "common.hpp"
+#include "device.hpp"
 #include "mrhs.hpp"
 
 namespace lace {
 
 constexpr int kChunk = 8;  // nonzeros whose operands are loaded ahead
-
-// W consecutive elements at a 16-byte aligned address, moved as 16-byte
-// words (W * sizeof(T) is a multiple of 16).
-template <typename T, int W>
-__device__ __forceinline__ void vload(const T* __restrict__ p, T (&v)[W]) {
-  constexpr int NQ = W * (int)sizeof(T) / 16;
-#pragma unroll
-  for (int q = 0; q < NQ; q++) {
-    const uint4 u = reinterpret_cast<const uint4*>(p)[q];
-    __builtin_memcpy(&v[q * 16 / (int)sizeof(T)], &u, 16);
-  }
-}
-template <typename T, int W>
-__device__ __forceinline__ void vstore(T* __restrict__ p, const T (&v)[W]) {
-  constexpr int NQ = W * (int)sizeof(T) / 16;
-#pragma unroll
-  for (int q = 0; q < NQ; q++) {
-    uint4 u;
-    __builtin_memcpy(&u, &v[q * 16 / (int)sizeof(T)], 16);
-    reinterpret_cast<uint4*>(p)[q] = u;
-  }
-}
 
 // Lane geometry: each lane owns W = 16 / sizeof(TX) consecutive right-hand
 // sides of one row (one 16-byte load of X per nonzero), 32 / W lanes per

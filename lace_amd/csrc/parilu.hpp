@@ -25,7 +25,17 @@ struct TriPlan {
   DevBuf perm, lptr;        // rows by level, level pointers (unpadded)
   DevBuf sched;             // padded schedule for the sync-free kernel
   std::vector<int> h_lptr;  // level pointers on the host
+  // Off-diagonal form for the Jacobi sweeps: CSR of the off-diagonal
+  // entries in row order (srow, scol), each one's position in the full CSR
+  // (smap) and each row's diagonal position (dpos); sval / dval hold the
+  // off-diagonal and diagonal values gathered from the value array
+  // strict_src by refresh_strict (the sweeps use them when called with
+  // that array).
+  int snnz = 0;
+  DevBuf srow, scol, smap, dpos, sval, dval;
+  const void* strict_src = nullptr;
   int build(int n, const int* row, const int* col, bool upper);
+  int refresh_strict(const void* val, size_t elem_size);
 };
 
 // y = T^{-1} r, exact.  Level-scheduled: one launch per level.

@@ -44,6 +44,54 @@ int TriPlan::build(int n_, const int* row_, const int* col_, bool upper_) {
   LACE_TRY(upload(perm, perm_h.data(), sizeof(int) * std::max(n, 1)));
   LACE_TRY(upload(lptr, h_lptr.data(), sizeof(int) * (nlev + 1)));
   LACE_TRY(upload(sched, sched_h.data(), sizeof(int) * std::max(nslots, 1)));
+  // off-diagonal form: every entry but the row's diagonal slot (first for
+  // upper, last for lower), in row order
+  std::vector<int> sr(n + 1, 0), sc, sm, dp(n);
+  sc.reserve(std::max(0, nnz - n));
+  sm.reserve(std::max(0, nnz - n));
+  for (int i = 0; i < n; i++) {
+    const int kd = upper ? row_[i] : row_[i + 1] - 1;
+    dp[i] = kd;
+    for (int k = row_[i]; k < row_[i + 1]; k++)
+      if (k != kd) {
+        sc.push_back(col_[k]);
+        sm.push_back(k);
+      }
+    sr[i + 1] = (int)sc.size();
+  }
+  snnz = (int)sc.size();
+  LACE_TRY(upload(srow, sr.data(), sizeof(int) * (n + 1)));
+  LACE_TRY(upload(scol, sc.data(), sizeof(int) * std::max(snnz, 1)));
+  LACE_TRY(upload(smap, sm.data(), sizeof(int) * std::max(snnz, 1)));
+  LACE_TRY(upload(dpos, dp.data(), sizeof(int) * std::max(n, 1)));
+  strict_src = nullptr;
+  return DEV_SUCCESS;
+}
+
+template <typename T>
+__global__ void strict_gather_kernel(int snnz, int n, const int* __restrict__ smap, const int* __restrict__ dpos,
+                                     const T* __restrict__ val, T* __restrict__ sval, T* __restrict__ dval) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e < snnz) sval[e] = val[smap[e]];
+  if (e < n) dval[e] = val[dpos[e]];
+}
+
+int TriPlan::refresh_strict(const void* val, size_t es) {
+  if (sval.bytes < es * std::max(snnz, 1)) LACE_TRY(sval.alloc(es * std::max(snnz, 1)));
+  if (dval.bytes < es * std::max(n, 1)) LACE_TRY(dval.alloc(es * std::max(n, 1)));
+  const int m = std::max(snnz, n);
+  if (m > 0) {
+    if (es == 8)
+      strict_gather_kernel<double><<<ceil_div(m, 256), 256, 0, stream()>>>(
+          snnz, n, smap.as<int>(), dpos.as<int>(), static_cast<const double*>(val), sval.as<double>(),
+          dval.as<double>());
+    else
+      strict_gather_kernel<float><<<ceil_div(m, 256), 256, 0, stream()>>>(
+          snnz, n, smap.as<int>(), dpos.as<int>(), static_cast<const float*>(val), sval.as<float>(),
+          dval.as<float>());
+    LACE_HIP(hipGetLastError());
+  }
+  strict_src = val;
   return DEV_SUCCESS;
 }
 
@@ -239,6 +287,77 @@ __global__ __launch_bounds__(256) void tri_jacobi_stream_kernel(
   }
 }
 
+// The same sweep on the plan's off-diagonal form: only off-diagonal values
+// and columns are streamed and the diagonal comes from its own array, so a
+// row no longer costs a diagonal value, column and gather; with a unit
+// diagonal the division by 1 (exact) is dropped.  Same sums in the same
+// order, so the result equals the CSR-stream kernel bit for bit.
+template <typename T, bool UNIT, int R>
+__global__ __launch_bounds__(256) void tri_jacobi_strict_kernel(
+    int n, const int* __restrict__ srow, const int* __restrict__ scol, const T* __restrict__ sval,
+    const T* __restrict__ dval, const T* __restrict__ r, const T* __restrict__ yin, T* __restrict__ yout) {
+#pragma clang fp contract(off)
+  constexpr int ROWS = JacCap<T, R>::rows, CAP = JacCap<T, R>::cap;
+  __shared__ T prod[CAP];
+  const int b = xcd_block(blockIdx.x, gridDim.x);
+  const int r0 = b * ROWS;
+  const int r1 = min(r0 + ROWS, n);
+  int k0[R], k1[R];
+  T rr[R], dg[R], s[R];
+#pragma unroll
+  for (int q = 0; q < R; q++) {
+    const int i = r0 + (int)threadIdx.x + 256 * q;
+    k0[q] = k1[q] = 0;
+    rr[q] = s[q] = T(0);
+    dg[q] = T(1);
+    if (i < r1) {
+      k0[q] = srow[i];
+      k1[q] = srow[i + 1];
+      rr[q] = r[i];
+      if (!UNIT) dg[q] = dval[i];
+    }
+  }
+  const int c_begin = srow[r0], c_end = srow[r1];
+  for (int c0 = c_begin; c0 < c_end; c0 += CAP) {
+    const int cn = min(CAP, c_end - c0);
+    stage_products<256, T>(scol, sval, yin, c0, cn, prod);
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < R; q++) {
+      const int lo = max(k0[q], c0), hi = min(k1[q], c0 + cn);
+      for (int k = lo; k < hi; k++) s[q] = s[q] + prod[k - c0];
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int q = 0; q < R; q++) {
+    const int i = r0 + (int)threadIdx.x + 256 * q;
+    if (i < r1) yout[i] = UNIT ? (rr[q] - s[q]) : (rr[q] - s[q]) / dg[q];
+  }
+}
+
+// First sweep from zero: y = r / diag.
+template <typename T>
+__global__ void tri_diag_div_kernel(int n, const T* __restrict__ dval, const T* __restrict__ r,
+                                    T* __restrict__ y) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) y[i] = r[i] / dval[i];
+}
+
+template <typename T, bool UNIT, int R>
+static void launch_jacobi_strict(const TriPlan& P, const T* r, const T* yin, T* yout) {
+  tri_jacobi_strict_kernel<T, UNIT, R><<<ceil_div(P.n, JacCap<T, R>::rows), 256, 0, stream()>>>(
+      P.n, P.srow.as<int>(), P.scol.as<int>(), P.sval.as<T>(), P.dval.as<T>(), r, yin, yout);
+}
+
+template <typename T, bool UNIT>
+static void jacobi_strict(const TriPlan& P, const T* r, const T* yin, T* yout) {
+  const int R = get_option("stream_rows_per_thread", sizeof(T) == 8 ? 1 : 2);
+  if (R == 1) launch_jacobi_strict<T, UNIT, 1>(P, r, yin, yout);
+  else if (R == 4) launch_jacobi_strict<T, UNIT, 4>(P, r, yin, yout);
+  else launch_jacobi_strict<T, UNIT, 2>(P, r, yin, yout);
+}
+
 template <typename T, bool UPPER, int R>
 static void launch_jacobi_stream(const TriPlan& P, const T* val, const T* r, const T* yin, T* yout) {
   tri_jacobi_stream_kernel<T, UPPER, R><<<ceil_div(P.n, JacCap<T, R>::rows), 256, 0, stream()>>>(
@@ -304,10 +423,18 @@ template <typename T>
 static int jacobi_sweep(const TriPlan& P, const T* val, const T* r, const T* yin, T* yout,
                         double* partial) {
   const int nb = ceil_div(P.n, 256);
+  const bool strict = P.strict_src == (const void*)val;  // off-diagonal form is current
   if (!partial && yin && P.n > 0 && P.nnz <= 16LL * P.n) {
     TimedRegion tr(P.upper ? "trsv_jacobi_sweep_upper" : "trsv_jacobi_sweep_lower");
-    if (P.upper) jacobi_stream<T, true>(P, val, r, yin, yout);
+    if (strict && P.unit_diag) jacobi_strict<T, true>(P, r, yin, yout);
+    else if (strict) jacobi_strict<T, false>(P, r, yin, yout);
+    else if (P.upper) jacobi_stream<T, true>(P, val, r, yin, yout);
     else jacobi_stream<T, false>(P, val, r, yin, yout);
+    LACE_HIP(hipGetLastError());
+    return DEV_SUCCESS;
+  }
+  if (!partial && !yin && strict && P.n > 0) {
+    tri_diag_div_kernel<T><<<nb, 256, 0, stream()>>>(P.n, P.dval.as<T>(), r, yout);
     LACE_HIP(hipGetLastError());
     return DEV_SUCCESS;
   }
@@ -433,6 +560,12 @@ int ParILU::refresh_csr() {
         nnzL, l_t_from_csr.as<int>(), lval[cur].as<float>(), lval_t.as<float>());
   }
   LACE_HIP(hipGetLastError());
+  // off-diagonal forms for the Jacobi sweeps, from the arrays the solves pass
+  const size_t es = elem_size();
+  LACE_TRY(lower.refresh_strict(lval[cur].p, es));
+  LACE_TRY(upper.refresh_strict(uval_csr.p, es));
+  LACE_TRY(lowerT.refresh_strict(uval[cur].p, es));
+  LACE_TRY(upperT.refresh_strict(lval_t.p, es));
   csr_dirty = false;
   return DEV_SUCCESS;
 }

@@ -52,7 +52,7 @@ def test_parilu_jacobi_bitexact(gpu, orc, name, prec):
 
 
 @pytest.mark.parametrize("prec", ["f64", "f32"])
-@pytest.mark.parametrize("variant", [0, 2, 3, 4])
+@pytest.mark.parametrize("variant", [0, 2, 3, 4, 5])
 @pytest.mark.parametrize("name", ["convdiff", "trefethen20"])
 def test_parilu_symmetric_kernels_match_general(gpu, orc, prec, variant, name):
     """Structurally symmetric patterns run the streaming (0 = 4), per-row
