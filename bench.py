@@ -87,9 +87,9 @@ def cpu_baseline(row, col, val, b, threads):
 
 
 SWEEP_KERNEL = "parilu_stream4_sweep_kernel<float>"
-JACOBI_KERNEL = "tri_jacobi_stream_kernel<float>"
-SPMV64_KERNEL = "spmv_stream_kernel<double, float, double>"
-SPMV32_KERNEL = "spmv_stream_kernel<float, float, float>"
+JACOBI_KERNEL = "tri_jacobi_ell_kernel<float>"
+SPMV64_KERNEL = "spmv_ell_kernel<double, float, double>"
+SPMV32_KERNEL = "spmv_ell_kernel<float, float, float>"
 # kernel class -> per-launch timing regions of the library
 KERNEL_CLASSES = {
     SWEEP_KERNEL: ["parilu_sweep"],
@@ -103,14 +103,14 @@ HOT_REGIONS = [r for rs in KERNEL_CLASSES.values() for r in rs]
 def kernel_bytes(n, nnz, nnz_l, nnz_u, sweep_bytes):
     """Algorithmic HBM bytes per launch of each timed region (int32
     indices): ParILU sweep nnz*(4 + 4 + 2*4) + 4(n+1); Jacobi sweep on the
-    strict-lower L (unit diagonal) or on U: nnz_T*(4+4) + 4(n+1) + 3*4n
-    (r, previous iterate, new iterate); outer SpMV with A in FP64 and the
-    FP32 preconditioned vector: 12 nnz + 4(n+1) + 4n + 8n; inner FP32 SpMV:
-    8 nnz + 4(n+1) + 8n."""
+    off-diagonal entries of L (unit diagonal) or of U (plus U's diagonal):
+    nnz_off*(4+4) [+ 4n] + 4(n+1) + 3*4n (r, previous iterate, new iterate);
+    outer SpMV with A in FP64 and the FP32 preconditioned vector:
+    12 nnz + 4(n+1) + 4n + 8n; inner FP32 SpMV: 8 nnz + 4(n+1) + 8n."""
     return {
         "parilu_sweep": sweep_bytes,
         "trsv_jacobi_sweep_lower": 8.0 * nnz_l + 4.0 * (n + 1) + 12.0 * n,
-        "trsv_jacobi_sweep_upper": 8.0 * nnz_u + 4.0 * (n + 1) + 12.0 * n,
+        "trsv_jacobi_sweep_upper": 8.0 * (nnz_u - n) + 4.0 * n + 4.0 * (n + 1) + 12.0 * n,
         "spmv": 12.0 * nnz + 4.0 * (n + 1) + 12.0 * n,
         "spmv_f32a": 8.0 * nnz + 4.0 * (n + 1) + 8.0 * n,
     }

@@ -545,7 +545,12 @@ int lace_block_panel_qr(int n, int r0, const double *W, double *Y, double *Tb, d
 
 /* Kernel-shape options for tuning runs.  "stream_rows_per_thread" (1, 2 or
  * 4): rows each thread owns in the CSR-stream SpMV and Jacobi sweeps
- * (default: 2 for FP32 values, 1 for FP64). */
+ * (default: 2 for FP32 values, 1 for FP64).  "jacobi_ell" (0 or 1, default
+ * 1): Jacobi sweeps of ILU factors with short rows use the ELLPACK form of
+ * the off-diagonal entries (0: the off-diagonal CSR-stream form).
+ * "spmv_ell" (0 or 1, default 1): SpMV of a matrix with at most 8 entries
+ * per row uses its ELLPACK copy (0: the CSR-stream kernel).  Every choice
+ * gives bit-identical results. */
 int lace_set_option(const char *name, int value);
 
 /* Kernel timing by class, HIP events on the library stream. */

@@ -34,6 +34,12 @@ struct TriPlan {
   int snnz = 0;
   DevBuf srow, scol, smap, dpos, sval, dval;
   const void* strict_src = nullptr;
+  // The same off-diagonal entries in ELLPACK form when every row has at most
+  // 8 of them and padding stays under 25 %: ew slots per row, slot-major
+  // (slot q of row i at q*n + i), column -1 marking padding; emap gives each
+  // slot's position in the full CSR, eval its gathered value.
+  int ew = 0;
+  DevBuf ecol, emap, eval;
   int build(int n, const int* row, const int* col, bool upper);
   int refresh_strict(const void* val, size_t elem_size);
 };

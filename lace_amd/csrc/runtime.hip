@@ -270,9 +270,14 @@ int lace_timing_get_work(const char* name, double* flops, double* bytes) {
 extern "C" int lace_set_option(const char* name, int value) {
   LACE_REQUIRE(name != nullptr, DEV_ERR_INVALID_PTR, "lace_set_option: null name");
   const std::string key(name);
-  LACE_REQUIRE(key == "stream_rows_per_thread", DEV_ERR_ILLEGAL_VALUE, "unknown option %s", name);
-  LACE_REQUIRE(value == 1 || value == 2 || value == 4, DEV_ERR_ILLEGAL_VALUE,
-               "stream_rows_per_thread must be 1, 2 or 4 (have %d)", value);
+  if (key == "stream_rows_per_thread") {
+    LACE_REQUIRE(value == 1 || value == 2 || value == 4, DEV_ERR_ILLEGAL_VALUE,
+                 "stream_rows_per_thread must be 1, 2 or 4 (have %d)", value);
+  } else if (key == "jacobi_ell" || key == "spmv_ell") {
+    LACE_REQUIRE(value == 0 || value == 1, DEV_ERR_ILLEGAL_VALUE, "%s must be 0 or 1 (have %d)", name, value);
+  } else {
+    LACE_REQUIRE(false, DEV_ERR_ILLEGAL_VALUE, "unknown option %s", name);
+  }
   std::lock_guard<std::mutex> lk(g_omu);
   g_options[key] = value;
   return DEV_SUCCESS;

@@ -322,6 +322,12 @@ static int solver_set_values(Solver* S, const double* vals) {
       LACE_HIP(hipMemcpyAsync(S->AT32.val.p, f.data(), sizeof(float) * S->nnz, hipMemcpyHostToDevice, stream()));
     }
   }
+  LACE_TRY(S->A64.refresh_ell());
+  if (S->inner) LACE_TRY(S->A32.refresh_ell());
+  if (S->have_T) {
+    LACE_TRY(S->AT64.refresh_ell());
+    if (S->inner) LACE_TRY(S->AT32.refresh_ell());
+  }
   LACE_HIP(hipStreamSynchronize(stream()));
   return DEV_SUCCESS;
 }

@@ -151,8 +151,70 @@ static void launch_stream(const DevCSR& A, const TX* x, TY* y, const TY* b, doub
           A.n, A.row.as<int>(), A.col.as<int>(), A.val.as<TV>(), x, y, b, alpha, beta);
 }
 
+// ELLPACK form: thread per row, the row's slots read slot-major (coalesced),
+// padding skipped, products summed in CSR order from zero -- the same
+// arithmetic as the stream kernel, so the results are bit-identical.
+template <typename TV, typename TX, typename TY, int MODE, int W>
+__global__ __launch_bounds__(256) void spmv_ell_kernel(int n, const int* __restrict__ ecol,
+                                                       const TV* __restrict__ eval,
+                                                       const TX* __restrict__ x, TY* __restrict__ y,
+                                                       const TY* __restrict__ b, double alpha, double beta) {
+#pragma clang fp contract(off)
+  using ACC = typename std::conditional<sizeof(TV) == 8 || sizeof(TY) == 8, double, float>::type;
+  const int i = xcd_block(blockIdx.x, gridDim.x) * 256 + (int)threadIdx.x;
+  if (i >= n) return;
+  int c[W];
+  TV v[W];
+  TX xv[W];
+#pragma unroll
+  for (int q = 0; q < W; q++) {
+    c[q] = ecol[(size_t)q * n + i];
+    v[q] = eval[(size_t)q * n + i];
+  }
+#pragma unroll
+  for (int q = 0; q < W; q++) xv[q] = c[q] >= 0 ? x[c[q]] : TX(0);
+  ACC add = ACC(0);
+  if (MODE == 1) add = (ACC)b[i];
+  else if (beta != 0.0) add = (ACC)beta * (ACC)y[i];
+  ACC s = ACC(0);
+#pragma unroll
+  for (int q = 0; q < W; q++)
+    if (c[q] >= 0) s = s + (ACC)v[q] * (ACC)xv[q];
+  if (MODE == 1) {
+    y[i] = (TY)(add - s);
+  } else {
+    ACC o = (ACC)alpha * s;
+    if (beta != 0.0) o = o + add;
+    y[i] = (TY)o;
+  }
+}
+
 template <typename TV, typename TX, typename TY, int MODE>
 static int launch_spmv(const DevCSR& A, const TX* x, TY* y, const TY* b, double alpha, double beta) {
+  if (A.ew > 0 && get_option("spmv_ell", 1) != 0) {
+    const int nb = ceil_div(A.n, 256);
+    const int* ec = A.ecol.as<int>();
+    const TV* ev = A.eval.as<TV>();
+    switch (A.ew) {
+#define LACE_SPMV_ELL(W)                                                                               \
+  case W:                                                                                              \
+    spmv_ell_kernel<TV, TX, TY, MODE, W><<<nb, 256, 0, stream()>>>(A.n, ec, ev, x, y, b, alpha, beta); \
+    break;
+      LACE_SPMV_ELL(1)
+      LACE_SPMV_ELL(2)
+      LACE_SPMV_ELL(3)
+      LACE_SPMV_ELL(4)
+      LACE_SPMV_ELL(5)
+      LACE_SPMV_ELL(6)
+      LACE_SPMV_ELL(7)
+      LACE_SPMV_ELL(8)
+#undef LACE_SPMV_ELL
+      default:
+        break;
+    }
+    LACE_HIP(hipGetLastError());
+    return DEV_SUCCESS;
+  }
   if (A.n > 0 && A.nnz <= 16LL * A.n) {
     // short rows (at most 16 per row on average): the stream kernel
     // measured at 128^3: two rows per thread for FP32 values, one for FP64
@@ -198,6 +260,43 @@ int DevCSR::create(const data_d_matrix* A, lace_prec_t p) {
     for (int k = 0; k < nnz; k++) f[k] = (float)A->val[k];
     LACE_TRY(upload(val, f.data(), sizeof(float) * f.size()));
   }
+  int wmax = 0;
+  for (int i = 0; i < n; i++) wmax = std::max(wmax, A->row[i + 1] - A->row[i]);
+  ew = 0;
+  if (n > 0 && wmax >= 1 && wmax <= 8 && (double)wmax * n <= 1.25 * nnz) {
+    ew = wmax;
+    std::vector<int> ec((size_t)ew * n, -1), em((size_t)ew * n, -1);
+    for (int i = 0; i < n; i++)
+      for (int k = A->row[i]; k < A->row[i + 1]; k++) {
+        ec[(size_t)(k - A->row[i]) * n + i] = A->col[k];
+        em[(size_t)(k - A->row[i]) * n + i] = k;
+      }
+    LACE_TRY(upload(ecol, ec.data(), sizeof(int) * ec.size()));
+    LACE_TRY(upload(emap, em.data(), sizeof(int) * em.size()));
+    LACE_TRY(refresh_ell());
+  }
+  return DEV_SUCCESS;
+}
+
+template <typename T>
+__global__ void csr_ell_gather_kernel(size_t m, const int* __restrict__ emap, const T* __restrict__ val,
+                                      T* __restrict__ eval) {
+  const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e < m) eval[e] = emap[e] >= 0 ? val[emap[e]] : T(0);
+}
+
+int DevCSR::refresh_ell() {
+  if (ew == 0) return DEV_SUCCESS;
+  const size_t m = (size_t)ew * n;
+  const size_t es = prec == LACE_F64 ? sizeof(double) : sizeof(float);
+  if (eval.bytes < es * m) LACE_TRY(eval.alloc(es * m));
+  if (prec == LACE_F64)
+    csr_ell_gather_kernel<double><<<ceil_div((long)m, 256), 256, 0, stream()>>>(
+        m, emap.as<int>(), val.as<double>(), eval.as<double>());
+  else
+    csr_ell_gather_kernel<float><<<ceil_div((long)m, 256), 256, 0, stream()>>>(
+        m, emap.as<int>(), val.as<float>(), eval.as<float>());
+  LACE_HIP(hipGetLastError());
   return DEV_SUCCESS;
 }
 

@@ -10,7 +10,13 @@ struct DevCSR {
   int group = 8;  // lanes per row
   lace_prec_t prec = LACE_F64;
   DevBuf row, col, val;
+  // ELLPACK copy for short rows (at most 8 per row, padding under 25 %):
+  // ew slots per row, slot-major, column -1 for padding; eval is gathered
+  // from val through emap by refresh_ell (call it after changing val).
+  int ew = 0;
+  DevBuf ecol, emap, eval;
   int create(const data_d_matrix* A, lace_prec_t p);
+  int refresh_ell();
   // Algorithmic bytes of one product: values+columns, row pointers, x, y
   // (and b for the residual form); sx, sy are vector element sizes.
   double bytes(int sx, int sy, bool residual) const;

@@ -64,8 +64,30 @@ int TriPlan::build(int n_, const int* row_, const int* col_, bool upper_) {
   LACE_TRY(upload(scol, sc.data(), sizeof(int) * std::max(snnz, 1)));
   LACE_TRY(upload(smap, sm.data(), sizeof(int) * std::max(snnz, 1)));
   LACE_TRY(upload(dpos, dp.data(), sizeof(int) * std::max(n, 1)));
+  // ELLPACK form for short rows
+  int wmax = 0;
+  for (int i = 0; i < n; i++) wmax = std::max(wmax, sr[i + 1] - sr[i]);
+  ew = 0;
+  if (n > 0 && wmax >= 1 && wmax <= 8 && (double)wmax * n <= 1.25 * snnz) {
+    ew = wmax;
+    std::vector<int> ec((size_t)ew * n, -1), em((size_t)ew * n, -1);
+    for (int i = 0; i < n; i++)
+      for (int k = sr[i]; k < sr[i + 1]; k++) {
+        ec[(size_t)(k - sr[i]) * n + i] = sc[k];
+        em[(size_t)(k - sr[i]) * n + i] = sm[k];
+      }
+    LACE_TRY(upload(ecol, ec.data(), sizeof(int) * ec.size()));
+    LACE_TRY(upload(emap, em.data(), sizeof(int) * em.size()));
+  }
   strict_src = nullptr;
   return DEV_SUCCESS;
+}
+
+template <typename T>
+__global__ void ell_gather_kernel(size_t m, const int* __restrict__ emap, const T* __restrict__ val,
+                                  T* __restrict__ eval) {
+  const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e < m) eval[e] = emap[e] >= 0 ? val[emap[e]] : T(0);
 }
 
 template <typename T>
@@ -89,6 +111,17 @@ int TriPlan::refresh_strict(const void* val, size_t es) {
       strict_gather_kernel<float><<<ceil_div(m, 256), 256, 0, stream()>>>(
           snnz, n, smap.as<int>(), dpos.as<int>(), static_cast<const float*>(val), sval.as<float>(),
           dval.as<float>());
+    LACE_HIP(hipGetLastError());
+  }
+  if (ew > 0) {
+    const size_t m2 = (size_t)ew * n;
+    if (eval.bytes < es * m2) LACE_TRY(eval.alloc(es * m2));
+    if (es == 8)
+      ell_gather_kernel<double><<<ceil_div((long)m2, 256), 256, 0, stream()>>>(
+          m2, emap.as<int>(), static_cast<const double*>(val), eval.as<double>());
+    else
+      ell_gather_kernel<float><<<ceil_div((long)m2, 256), 256, 0, stream()>>>(
+          m2, emap.as<int>(), static_cast<const float*>(val), eval.as<float>());
     LACE_HIP(hipGetLastError());
   }
   strict_src = val;
@@ -336,6 +369,61 @@ __global__ __launch_bounds__(256) void tri_jacobi_strict_kernel(
   }
 }
 
+// The same sweep on the ELLPACK form: thread per row, the W slots of the row
+// read as coalesced slot-major words, padding (column -1) skipped; the sum
+// visits the row's entries in CSR order, so results are bit-identical.
+template <typename T, bool UNIT, int W>
+__global__ __launch_bounds__(256) void tri_jacobi_ell_kernel(int n, const int* __restrict__ ecol,
+                                                             const T* __restrict__ eval,
+                                                             const T* __restrict__ dval,
+                                                             const T* __restrict__ r,
+                                                             const T* __restrict__ yin,
+                                                             T* __restrict__ yout) {
+#pragma clang fp contract(off)
+  const int i = xcd_block(blockIdx.x, gridDim.x) * 256 + (int)threadIdx.x;
+  if (i >= n) return;
+  int c[W];
+  T v[W], y[W];
+#pragma unroll
+  for (int q = 0; q < W; q++) {
+    c[q] = ecol[(size_t)q * n + i];
+    v[q] = eval[(size_t)q * n + i];
+  }
+#pragma unroll
+  for (int q = 0; q < W; q++) y[q] = c[q] >= 0 ? yin[c[q]] : T(0);
+  const T ri = r[i];
+  T s = T(0);
+#pragma unroll
+  for (int q = 0; q < W; q++)
+    if (c[q] >= 0) s = s + v[q] * y[q];
+  yout[i] = UNIT ? ri - s : (ri - s) / dval[i];
+}
+
+template <typename T, bool UNIT>
+static void jacobi_ell(const TriPlan& P, const T* r, const T* yin, T* yout) {
+  const int nb = ceil_div(P.n, 256);
+  const int* ec = P.ecol.as<int>();
+  const T* ev = P.eval.as<T>();
+  const T* dv = P.dval.as<T>();
+  switch (P.ew) {
+#define LACE_ELL_CASE(W)                                                                              \
+  case W:                                                                                             \
+    tri_jacobi_ell_kernel<T, UNIT, W><<<nb, 256, 0, stream()>>>(P.n, ec, ev, dv, r, yin, yout); \
+    break;
+    LACE_ELL_CASE(1)
+    LACE_ELL_CASE(2)
+    LACE_ELL_CASE(3)
+    LACE_ELL_CASE(4)
+    LACE_ELL_CASE(5)
+    LACE_ELL_CASE(6)
+    LACE_ELL_CASE(7)
+    LACE_ELL_CASE(8)
+#undef LACE_ELL_CASE
+    default:
+      break;
+  }
+}
+
 // First sweep from zero: y = r / diag.
 template <typename T>
 __global__ void tri_diag_div_kernel(int n, const T* __restrict__ dval, const T* __restrict__ r,
@@ -426,7 +514,10 @@ static int jacobi_sweep(const TriPlan& P, const T* val, const T* r, const T* yin
   const bool strict = P.strict_src == (const void*)val;  // off-diagonal form is current
   if (!partial && yin && P.n > 0 && P.nnz <= 16LL * P.n) {
     TimedRegion tr(P.upper ? "trsv_jacobi_sweep_upper" : "trsv_jacobi_sweep_lower");
-    if (strict && P.unit_diag) jacobi_strict<T, true>(P, r, yin, yout);
+    const bool ell = strict && P.ew > 0 && get_option("jacobi_ell", 1) != 0;
+    if (ell && P.unit_diag) jacobi_ell<T, true>(P, r, yin, yout);
+    else if (ell) jacobi_ell<T, false>(P, r, yin, yout);
+    else if (strict && P.unit_diag) jacobi_strict<T, true>(P, r, yin, yout);
     else if (strict) jacobi_strict<T, false>(P, r, yin, yout);
     else if (P.upper) jacobi_stream<T, true>(P, val, r, yin, yout);
     else jacobi_stream<T, false>(P, val, r, yin, yout);
