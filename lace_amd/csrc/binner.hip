@@ -10,7 +10,7 @@
 //   Gram-Schmidt, FP64 sums), w <- w - sum_i h_ij q_i, h_{j+1,j} = ||w||,
 //   q_{j+1} = w / h_{j+1,j}, Givens rotations on the Hessenberg column;
 // after m steps z = sum_j p_j y_j with y from the triangular least squares.
-// Dot products and norms are reduced per 2048-row slab and then over the
+// Dot products and norms are reduced per 1024-row slab and then over the
 // slabs in a fixed order, so results do not depend on scheduling.
 #include <algorithm>
 #include <cmath>
@@ -41,35 +41,53 @@ __device__ __forceinline__ void load4f(const float* p, double (&v)[4]) {
 }
 
 // partial[slab][i][c] = sum over the slab's rows of q_i[row][c] w[row][c].
-// Thread t: column group 4 (t % 8), rows t / 8 + 32 k of the slab.
-__global__ __launch_bounds__(256) void bi_dots_kernel(int n, int rpb, int nq, const float* __restrict__ Q,
+// Thread t: column group 4 (t % 8), rows t / 8 + 32 k of the slab, two rows
+// per iteration with every load of both issued first.  NQ = nq.
+template <int NQ>
+__global__ __launch_bounds__(256) void bi_dots_kernel(int n, int rpb, const float* __restrict__ Q,
                                                       size_t bstride, const float* __restrict__ w,
                                                       double* __restrict__ partial) {
-  __shared__ double red[4][kMaxInner + 1][32];
+  __shared__ double red[4][NQ][32];
   const int t = threadIdx.x, lane = t & 63, wv = t >> 6, cg = t & 7, rg = t >> 3;
   const int slab = blockIdx.x;
   const int r0 = slab * rpb, r1 = min(n, r0 + rpb);
-  double acc[kMaxInner + 1][4];
+  double acc[NQ][4];
 #pragma unroll
-  for (int i = 0; i <= kMaxInner; i++)
+  for (int i = 0; i < NQ; i++)
 #pragma unroll
     for (int e = 0; e < 4; e++) acc[i][e] = 0.0;
-  for (int row = r0 + rg; row < r1; row += 32) {
-    double wv4[4];
-    load4f(w + (size_t)row * 32 + 4 * cg, wv4);
+  const float4 z4 = make_float4(0.f, 0.f, 0.f, 0.f);
+  for (int row = r0 + rg; row < r1; row += 64) {
+    const int row2 = row + 32;
+    const bool has2 = row2 < r1;
+    const float4 wa = *reinterpret_cast<const float4*>(w + (size_t)row * 32 + 4 * cg);
+    const float4 wb = has2 ? *reinterpret_cast<const float4*>(w + (size_t)row2 * 32 + 4 * cg) : z4;
+    float4 qa[NQ], qb[NQ];
 #pragma unroll
-    for (int i = 0; i <= kMaxInner; i++) {
-      if (i < nq) {
-        double q[4];
-        load4f(Q + i * bstride + (size_t)row * 32 + 4 * cg, q);
+    for (int i = 0; i < NQ; i++) {
+      qa[i] = *reinterpret_cast<const float4*>(Q + i * bstride + (size_t)row * 32 + 4 * cg);
+      qb[i] = has2 ? *reinterpret_cast<const float4*>(Q + i * bstride + (size_t)row2 * 32 + 4 * cg) : z4;
+    }
 #pragma unroll
-        for (int e = 0; e < 4; e++) acc[i][e] += q[e] * wv4[e];
+    for (int i = 0; i < NQ; i++) {
+      acc[i][0] += (double)qa[i].x * (double)wa.x;
+      acc[i][1] += (double)qa[i].y * (double)wa.y;
+      acc[i][2] += (double)qa[i].z * (double)wa.z;
+      acc[i][3] += (double)qa[i].w * (double)wa.w;
+    }
+    if (has2) {
+#pragma unroll
+      for (int i = 0; i < NQ; i++) {
+        acc[i][0] += (double)qb[i].x * (double)wb.x;
+        acc[i][1] += (double)qb[i].y * (double)wb.y;
+        acc[i][2] += (double)qb[i].z * (double)wb.z;
+        acc[i][3] += (double)qb[i].w * (double)wb.w;
       }
     }
   }
+  constexpr int nq = NQ;
 #pragma unroll
-  for (int i = 0; i <= kMaxInner; i++) {
-    if (i >= nq) continue;
+  for (int i = 0; i < NQ; i++) {
 #pragma unroll
     for (int e = 0; e < 4; e++) {
       double v = acc[i][e];
@@ -88,38 +106,51 @@ __global__ __launch_bounds__(256) void bi_dots_kernel(int n, int rpb, int nq, co
 
 // w <- w - sum_{i < nq} q_i h_i (per column); partial[slab][c] = sum of the
 // squares of the new (FP32-rounded) w.
-__global__ __launch_bounds__(256) void bi_update_kernel(int n, int rpb, int nq, const float* __restrict__ Q,
+template <int NQ>
+__global__ __launch_bounds__(256) void bi_update_kernel(int n, int rpb, const float* __restrict__ Q,
                                                         size_t bstride, const double* __restrict__ h,
                                                         float* __restrict__ w, double* __restrict__ partial) {
   __shared__ double red[4][32];
   const int t = threadIdx.x, lane = t & 63, wv = t >> 6, cg = t & 7, rg = t >> 3;
   const int slab = blockIdx.x;
   const int r0 = slab * rpb, r1 = min(n, r0 + rpb);
-  double hc[kMaxInner + 1][4];
+  double hc[NQ][4];
 #pragma unroll
-  for (int i = 0; i <= kMaxInner; i++)
+  for (int i = 0; i < NQ; i++)
 #pragma unroll
-    for (int e = 0; e < 4; e++) hc[i][e] = i < nq ? h[i * 32 + 4 * cg + e] : 0.0;
+    for (int e = 0; e < 4; e++) hc[i][e] = h[i * 32 + 4 * cg + e];
   double s2[4] = {0.0, 0.0, 0.0, 0.0};
-  for (int row = r0 + rg; row < r1; row += 32) {
-    float* wp = w + (size_t)row * 32 + 4 * cg;
-    double x[4];
-    load4f(wp, x);
+  const float4 z4 = make_float4(0.f, 0.f, 0.f, 0.f);
+  for (int row0 = r0 + rg; row0 < r1; row0 += 64) {
+    const int rows[2] = {row0, row0 + 32};
+    const bool has2 = rows[1] < r1;
+    float4 wx[2], qx[2][NQ];
 #pragma unroll
-    for (int i = 0; i <= kMaxInner; i++) {
-      if (i < nq) {
-        double q[4];
-        load4f(Q + i * bstride + (size_t)row * 32 + 4 * cg, q);
+    for (int u = 0; u < 2; u++) {
+      const bool in = u == 0 || has2;
+      wx[u] = in ? *reinterpret_cast<const float4*>(w + (size_t)rows[u] * 32 + 4 * cg) : z4;
 #pragma unroll
-        for (int e = 0; e < 4; e++) x[e] -= q[e] * hc[i][e];
-      }
+      for (int i = 0; i < NQ; i++)
+        qx[u][i] = in ? *reinterpret_cast<const float4*>(Q + i * bstride + (size_t)rows[u] * 32 + 4 * cg) : z4;
     }
-    const float4 o = make_float4((float)x[0], (float)x[1], (float)x[2], (float)x[3]);
-    *reinterpret_cast<float4*>(wp) = o;
-    s2[0] += (double)o.x * o.x;
-    s2[1] += (double)o.y * o.y;
-    s2[2] += (double)o.z * o.z;
-    s2[3] += (double)o.w * o.w;
+#pragma unroll
+    for (int u = 0; u < 2; u++) {
+      if (u == 1 && !has2) break;
+      double x[4] = {wx[u].x, wx[u].y, wx[u].z, wx[u].w};
+#pragma unroll
+      for (int i = 0; i < NQ; i++) {
+        x[0] -= (double)qx[u][i].x * hc[i][0];
+        x[1] -= (double)qx[u][i].y * hc[i][1];
+        x[2] -= (double)qx[u][i].z * hc[i][2];
+        x[3] -= (double)qx[u][i].w * hc[i][3];
+      }
+      const float4 o = make_float4((float)x[0], (float)x[1], (float)x[2], (float)x[3]);
+      *reinterpret_cast<float4*>(w + (size_t)rows[u] * 32 + 4 * cg) = o;
+      s2[0] += (double)o.x * o.x;
+      s2[1] += (double)o.y * o.y;
+      s2[2] += (double)o.z * o.z;
+      s2[3] += (double)o.w * o.w;
+    }
   }
 #pragma unroll
   for (int e = 0; e < 4; e++) {
@@ -131,6 +162,38 @@ __global__ __launch_bounds__(256) void bi_update_kernel(int n, int rpb, int nq, 
   }
   __syncthreads();
   if (t < 32) partial[(size_t)slab * 32 + t] = (red[0][t] + red[1][t]) + (red[2][t] + red[3][t]);
+}
+
+// Launch the NQ instance for a runtime nq (1 .. kMaxInner + 1).
+#define LACE_NQ_SWITCH(nq, CALL) \
+  switch (nq) {                  \
+    case 1: CALL(1); break;      \
+    case 2: CALL(2); break;      \
+    case 3: CALL(3); break;      \
+    case 4: CALL(4); break;      \
+    case 5: CALL(5); break;      \
+    case 6: CALL(6); break;      \
+    case 7: CALL(7); break;      \
+    case 8: CALL(8); break;      \
+    case 9: CALL(9); break;      \
+    case 10: CALL(10); break;    \
+    case 11: CALL(11); break;    \
+    case 12: CALL(12); break;    \
+    default: CALL(13); break;    \
+  }
+
+static void launch_dots(int nq, int nslab, int n, int rpb, const float* Q, size_t bs, const float* w,
+                        double* part) {
+#define LACE_DOTS(K) bi_dots_kernel<K><<<nslab, 256, 0, stream()>>>(n, rpb, Q, bs, w, part)
+  LACE_NQ_SWITCH(nq, LACE_DOTS)
+#undef LACE_DOTS
+}
+
+static void launch_update(int nq, int nslab, int n, int rpb, const float* Q, size_t bs, const double* h,
+                          float* w, double* part) {
+#define LACE_UPD(K) bi_update_kernel<K><<<nslab, 256, 0, stream()>>>(n, rpb, Q, bs, h, w, part)
+  LACE_NQ_SWITCH(nq, LACE_UPD)
+#undef LACE_UPD
 }
 
 // q = w / sqrt(nrm2[c]) per column (zero where the norm is zero).
@@ -234,7 +297,7 @@ int InnerBatch::solve(const DevCSR& A32, ParILU* F, lace_trsv_kind_t kind, int s
   InnerLayout L{m};
   const int eb = ceil_div((long)bs, 256);
   // beta_c = ||v_c||, q_0 = v / beta
-  bi_dots_kernel<<<nslab, 256, 0, stream()>>>(n, rpb, 1, V, bs, V, part);
+  launch_dots(1, nslab, n, rpb, V, bs, V, part);
   LACE_TRY(sd_slab_reduce(nslab, 32, part, st + L.nrm2(), false));
   bi_init_kernel<<<1, 32, 0, stream()>>>(L, st);
   bi_scale_kernel<<<eb, 256, 0, stream()>>>(bs, V, st + L.nrm2(), Qb);
@@ -245,13 +308,13 @@ int InnerBatch::solve(const DevCSR& A32, ParILU* F, lace_trsv_kind_t kind, int s
     {
       // algorithmic bytes: q_0..q_j and w read once
       TimedRegion td("inner_dots", 2.0 * (j + 1) * bs, 4.0 * (j + 2) * bs);
-      bi_dots_kernel<<<nslab, 256, 0, stream()>>>(n, rpb, j + 1, Qb, bs, wb, part);
+      launch_dots(j + 1, nslab, n, rpb, Qb, bs, wb, part);
     }
     LACE_TRY(sd_slab_reduce(nslab, (j + 1) * 32, part, st + L.h(), false));
     {
       // q_0..q_j and w read once, w written once
       TimedRegion tu("inner_update", 2.0 * (j + 1) * bs, 4.0 * (j + 3) * bs);
-      bi_update_kernel<<<nslab, 256, 0, stream()>>>(n, rpb, j + 1, Qb, bs, st + L.h(), wb, part);
+      launch_update(j + 1, nslab, n, rpb, Qb, bs, st + L.h(), wb, part);
     }
     LACE_TRY(sd_slab_reduce(nslab, 32, part, st + L.nrm2(), false));
     bi_givens_kernel<<<1, 32, 0, stream()>>>(L, j, st);

@@ -16,7 +16,7 @@ constexpr int kMaxInner = 12;
 // Used as the preconditioner of the outer FP64 block FGMRES (flexible in
 // the preconditioner, so the inner solve may change from step to step).
 struct InnerBatch {
-  int n = 0, m = 0, rpb = 2048, nslab = 0;
+  int n = 0, m = 0, rpb = 1024, nslab = 0;
   DevBuf Q, P, w, t1, t2;  // FP32 n x 32 blocks: Q (m+1), P (m), w, Jacobi scratch
   DevBuf state, partial, red;
   int init(int n_, int m_);

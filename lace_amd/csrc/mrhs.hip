@@ -83,6 +83,69 @@ __global__ __launch_bounds__(256) void spmm32_kernel(int n, const int* __restric
   vstore<TY, W>(Y + (size_t)i * 32 + c0, out);
 }
 
+// The product on A's ELLPACK copy (slot-major, column -1 for padding).
+template <typename TV, typename TX, typename TY, int MODE, int WS>
+__global__ __launch_bounds__(256) void spmm32_ell_kernel(int n, const int* __restrict__ ecol,
+                                                         const TV* __restrict__ eval,
+                                                         const TX* __restrict__ X,
+                                                         const TY* __restrict__ B, TY* __restrict__ Y) {
+#pragma clang fp contract(off)
+  using ACC = typename std::conditional<sizeof(TV) == 8 || sizeof(TY) == 8, double, float>::type;
+  constexpr int W = Lanes<TX>::W, LPR = Lanes<TX>::LPR, RPB = Lanes<TX>::RPB;
+  const int i = xcd_block(blockIdx.x, gridDim.x) * RPB + (int)threadIdx.x / LPR;
+  const int c0 = ((int)threadIdx.x % LPR) * W;
+  if (i >= n) return;
+  int cc[WS];
+  TV vv[WS];
+  TX xv[WS][W];
+#pragma unroll
+  for (int q = 0; q < WS; q++) {
+    cc[q] = ecol[(size_t)q * n + i];
+    vv[q] = eval[(size_t)q * n + i];
+  }
+#pragma unroll
+  for (int q = 0; q < WS; q++)
+    if (cc[q] >= 0) vload<TX, W>(X + (size_t)cc[q] * 32 + c0, xv[q]);
+  TY add[W];
+  if (MODE == 1) vload<TY, W>(B + (size_t)i * 32 + c0, add);
+  ACC s[W];
+#pragma unroll
+  for (int w = 0; w < W; w++) s[w] = ACC(0);
+#pragma unroll
+  for (int q = 0; q < WS; q++)
+    if (cc[q] >= 0) {
+#pragma unroll
+      for (int w = 0; w < W; w++) s[w] = s[w] + (ACC)vv[q] * (ACC)xv[q][w];
+    }
+  TY out[W];
+#pragma unroll
+  for (int w = 0; w < W; w++) out[w] = MODE == 1 ? (TY)((ACC)add[w] - s[w]) : (TY)s[w];
+  vstore<TY, W>(Y + (size_t)i * 32 + c0, out);
+}
+
+template <typename TV, typename TX, typename TY, int MODE>
+static bool spmm32_ell(const DevCSR& A, const TX* X, const TY* B, TY* Y, int nb) {
+  const int* ec = A.ecol.as<int>();
+  const TV* ev = A.eval.as<TV>();
+  switch (A.ew) {
+#define LACE_SPMM_ELL(WS)                                                                                \
+  case WS:                                                                                               \
+    spmm32_ell_kernel<TV, TX, TY, MODE, WS><<<nb, 256, 0, stream()>>>(A.n, ec, ev, X, B, Y);             \
+    return true;
+    LACE_SPMM_ELL(1)
+    LACE_SPMM_ELL(2)
+    LACE_SPMM_ELL(3)
+    LACE_SPMM_ELL(4)
+    LACE_SPMM_ELL(5)
+    LACE_SPMM_ELL(6)
+    LACE_SPMM_ELL(7)
+    LACE_SPMM_ELL(8)
+#undef LACE_SPMM_ELL
+    default:
+      return false;
+  }
+}
+
 template <typename TV, typename TX, typename TY>
 static int spmm32_launch(const DevCSR& A, const TX* X, const TY* B, TY* Y, bool residual) {
   if (A.n == 0) return DEV_SUCCESS;
@@ -95,6 +158,11 @@ static int spmm32_launch(const DevCSR& A, const TX* X, const TY* B, TY* Y, bool 
   const char* name = residual ? "spmm32_residual"
                               : (sizeof(TV) == 4 ? "spmm32_f32a" : (sizeof(TX) == 4 ? "spmm32_f32x" : "spmm32"));
   TimedRegion tr(name, flops, bytes);
+  const bool ell = A.ew > 0 && get_option("spmv_ell", 1) != 0;
+  if (ell && (residual ? spmm32_ell<TV, TX, TY, 1>(A, X, B, Y, nb) : spmm32_ell<TV, TX, TY, 0>(A, X, B, Y, nb))) {
+    LACE_HIP(hipGetLastError());
+    return DEV_SUCCESS;
+  }
   if (residual)
     spmm32_kernel<TV, TX, TY, 1><<<nb, 256, 0, stream()>>>(A.n, A.row.as<int>(), A.col.as<int>(),
                                                            A.val.as<TV>(), X, B, Y);
@@ -177,6 +245,47 @@ __global__ __launch_bounds__(256) void jacobi32_kernel(int n, const int* __restr
   vstore<T, W>(yout + (size_t)i * 32 + c0, out);
 }
 
+// The sweep on the factor's off-diagonal ELLPACK form (slot-major, column -1
+// for padding) with the diagonal in its own array: the row's slots are read
+// once per lane group, no row pointers; same sums in the same order.
+template <typename T, bool UNIT, int WS>
+__global__ __launch_bounds__(256) void jacobi32_ell_kernel(int n, const int* __restrict__ ecol,
+                                                           const T* __restrict__ eval,
+                                                           const T* __restrict__ dval,
+                                                           const T* __restrict__ r,
+                                                           const T* __restrict__ yin,
+                                                           T* __restrict__ yout) {
+#pragma clang fp contract(off)
+  constexpr int W = Lanes<T>::W, LPR = Lanes<T>::LPR, RPB = Lanes<T>::RPB;
+  const int i = xcd_block(blockIdx.x, gridDim.x) * RPB + (int)threadIdx.x / LPR;
+  const int c0 = ((int)threadIdx.x % LPR) * W;
+  if (i >= n) return;
+  int cc[WS];
+  T vv[WS], yv[WS][W], ri[W], s[W];
+#pragma unroll
+  for (int q = 0; q < WS; q++) {
+    cc[q] = ecol[(size_t)q * n + i];
+    vv[q] = eval[(size_t)q * n + i];
+  }
+#pragma unroll
+  for (int q = 0; q < WS; q++)
+    if (cc[q] >= 0) vload<T, W>(yin + (size_t)cc[q] * 32 + c0, yv[q]);
+  vload<T, W>(r + (size_t)i * 32 + c0, ri);
+  const T di = UNIT ? T(1) : dval[i];
+#pragma unroll
+  for (int w = 0; w < W; w++) s[w] = T(0);
+#pragma unroll
+  for (int q = 0; q < WS; q++)
+    if (cc[q] >= 0) {
+#pragma unroll
+      for (int w = 0; w < W; w++) s[w] = s[w] + vv[q] * yv[q][w];
+    }
+  T out[W];
+#pragma unroll
+  for (int w = 0; w < W; w++) out[w] = UNIT ? ri[w] - s[w] : (ri[w] - s[w]) / di;
+  vstore<T, W>(yout + (size_t)i * 32 + c0, out);
+}
+
 template <typename T>
 static int jacobi32_sweep(const TriPlan& P, const T* val, const T* r, const T* yin, T* yout) {
   const int nb = ceil_div(P.n, Lanes<T>::RPB);
@@ -187,6 +296,33 @@ static int jacobi32_sweep(const TriPlan& P, const T* val, const T* r, const T* y
                        4.0 * (P.n + 1) + 32.0 * sizeof(T) * P.n * (yin ? 3.0 : 2.0);
   TimedRegion tr(P.upper ? "block_jacobi_sweep_upper" : "block_jacobi_sweep_lower",
                  64.0 * (yin ? offd : 0.0) + 32.0 * P.n, bytes);
+  if (yin && P.ew > 0 && P.strict_src == (const void*)val && get_option("jacobi_ell", 1) != 0) {
+    const int* ec = P.ecol.as<int>();
+    const T* ev = P.eval.as<T>();
+    const T* dv = P.dval.as<T>();
+    switch (P.ew) {
+#define LACE_J32_ELL(W)                                                                                      \
+  case W:                                                                                                    \
+    if (P.unit_diag)                                                                                         \
+      jacobi32_ell_kernel<T, true, W><<<nb, 256, 0, stream()>>>(P.n, ec, ev, dv, r, yin, yout);              \
+    else                                                                                                     \
+      jacobi32_ell_kernel<T, false, W><<<nb, 256, 0, stream()>>>(P.n, ec, ev, dv, r, yin, yout);             \
+    break;
+      LACE_J32_ELL(1)
+      LACE_J32_ELL(2)
+      LACE_J32_ELL(3)
+      LACE_J32_ELL(4)
+      LACE_J32_ELL(5)
+      LACE_J32_ELL(6)
+      LACE_J32_ELL(7)
+      LACE_J32_ELL(8)
+#undef LACE_J32_ELL
+      default:
+        break;
+    }
+    LACE_HIP(hipGetLastError());
+    return DEV_SUCCESS;
+  }
   if (P.upper)
     jacobi32_kernel<T, true><<<nb, 256, 0, stream()>>>(P.n, P.row.as<int>(), P.col.as<int>(), val, r,
                                                        yin, yout);
