@@ -181,6 +181,13 @@ int data_zmlumerge(data_d_matrix L, data_d_matrix U, data_d_matrix *A);
 int data_z_csr_mtx(data_d_matrix *A, const char *filename);
 int data_zwrite_csr_mtx(data_d_matrix A, data_order_t MajorType,
                         const char *filename);
+/* Dense MatrixMarket ("array") I/O: control/magma_zmio.cpp:1221-1318. */
+int data_z_dense_mtx(data_d_matrix *A, data_order_t major, const char *filename);
+int data_zwrite_dense_mtx(int n_row, int n_col, int nnz, data_order_t major,
+                          dataType **val, const char *filename);
+int data_zwrite_dense(data_d_matrix A, const char *filename);
+/* dy <- alpha dx + dy on host arrays: blas/zaxpy.cpp:52-59. */
+void data_zaxpy(int n, dataType alpha, const dataType *dx, int incx, dataType *dy, int incy);
 
 /* Norms and the ILU residual check.  sparse.h:410,462-468;
  * control/norms.cpp:69-92,301-405.  data_zilures evaluates ||A-LU||_F over
@@ -249,7 +256,10 @@ int data_parcsrtrsv(const data_uplo_t uplo, const data_storage_t storage,
 
 /* FGMRES.  sparse.h:1027-1052; src/fgmres.cpp:64-517,
  * src/fgmres_householder_restart.cpp:63-788.  L: CSRL, U: CSRU.  x0 is
- * freed and re-allocated with the solution (src/fgmres.cpp:491-492). */
+ * freed and re-allocated with the solution (src/fgmres.cpp:491-492).  The
+ * Householder forms keep at most 63 reflectors per cycle: a larger
+ * search_max runs as cycles of 63 with the same total number of search
+ * directions.  Progress lines go to stdout unless "matlab_log" is 0. */
 int data_fgmres(data_d_matrix *A, data_d_matrix *b, data_d_matrix *x0,
                 data_d_matrix *L, data_d_matrix *U,
                 data_d_gmres_param *gmres_par, data_d_gmres_log *gmres_log);
@@ -550,7 +560,11 @@ int lace_block_panel_qr(int n, int r0, const double *W, double *Y, double *Tb, d
  * the off-diagonal entries (0: the off-diagonal CSR-stream form).
  * "spmv_ell" (0 or 1, default 1): SpMV of a matrix with at most 8 entries
  * per row uses its ELLPACK copy (0: the CSR-stream kernel).  Every choice
- * gives bit-identical results. */
+ * gives bit-identical results.  "matlab_log" (0 or 1, default 1): the
+ * drop-in FGMRES entry points print the reference's MATLAB-syntax progress
+ * lines to stdout (src/fgmres.cpp:71,178,430; src/fgmres_householder_restart.cpp:
+ * 70,81,196,231,707) -- "FGMRES_Householders_partrsv_search(j) = |g_j|;" etc.,
+ * the log format its post-processing scripts read; 0 silences them. */
 int lace_set_option(const char *name, int value);
 
 /* Kernel timing by class, HIP events on the library stream. */

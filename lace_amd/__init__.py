@@ -121,6 +121,10 @@ SIGNATURES = {
     "data_z_csr_mtx": (_i, [_DM, C.c_char_p]),
     "data_zwrite_csr_mtx": (_i, [DMatrix, _i, C.c_char_p]),
     "data_dnrm2": (_d, [_i, _P(_d), _i]),
+    "data_z_dense_mtx": (_i, [_DM, _i, C.c_char_p]),
+    "data_zwrite_dense_mtx": (_i, [_i, _i, _i, _i, _P(_P(_d)), C.c_char_p]),
+    "data_zwrite_dense": (_i, [DMatrix, C.c_char_p]),
+    "data_zaxpy": (None, [_i, _d, _P(_d), _i, _P(_d), _i]),
     "data_zfrobenius": (_i, [DMatrix, _P(_d)]),
     "data_zilures": (_i, [DMatrix, DMatrix, DMatrix, _DM, _P(_d), _P(_d)]),
     "data_z_spmv": (_i, [_d, _DM, _DM, _d, _DM]),

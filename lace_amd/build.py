@@ -41,6 +41,32 @@ def _compile(src: str) -> str:
     return obj
 
 
+DRIVERS = os.path.join(HERE, "drivers")
+DRIVER_BIN = os.path.join(DRIVERS, "bin")
+# executable -> (source, extra defines): the reference's driver names
+DRIVER_TARGETS = {
+    "generate_iLU": ("generate_iLU.cpp", []),
+    "generate_iLU_gpu": ("generate_iLU.cpp", ["-DLACE_DRIVER_GPU"]),
+    "solver": ("solver_interface.cpp", []),
+}
+
+
+def build_drivers() -> list:
+    """Host-only command-line drivers linked against the library (g++)."""
+    os.makedirs(DRIVER_BIN, exist_ok=True)
+    inc = os.path.join(os.path.dirname(HERE), "include")
+    out = []
+    for exe, (src, defs) in DRIVER_TARGETS.items():
+        target = os.path.join(DRIVER_BIN, exe)
+        cmd = ["g++", "-O2", "-std=c++17", "-Wall", *defs, "-I", inc, os.path.join(DRIVERS, src),
+               "-L", HERE, "-llace_mi355x", "-Wl,-rpath,$ORIGIN/../..", "-o", target]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"driver build failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+        out.append(target)
+    return out
+
+
 def build(jobs: int = 8) -> str:
     os.makedirs(OUT, exist_ok=True)
     srcs = sorted(glob.glob(os.path.join(CSRC, "*.hip")) + glob.glob(os.path.join(CSRC, "*.cpp")))
@@ -52,6 +78,7 @@ def build(jobs: int = 8) -> str:
     if r.returncode != 0:
         raise RuntimeError(f"link failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
     os.replace(tmp, LIB)
+    build_drivers()
     return LIB
 
 

@@ -680,6 +680,113 @@ int data_zwrite_csr_mtx(data_d_matrix A, data_order_t MajorType, const char* fil
   return DEV_SUCCESS;
 }
 
+// control/magma_zmio.cpp:1056-1190 (read_z_dense_from_mtx) and :1265-1282
+// (data_z_dense_mtx): a MatrixMarket "array" file read in file order (real,
+// integer, pattern = 1, complex = real part); ld follows `major`.  The
+// reference prints its progress as MATLAB comment lines; so does this.
+int data_z_dense_mtx(data_d_matrix* A, data_order_t major, const char* filename) {
+  FILE* f = fopen(filename, "r");
+  if (!f) {
+    printf("%% Unable to open file %s\n", filename);
+    set_error("cannot open %s", filename);
+    return DEV_ERR_NOT_FOUND;
+  }
+  printf("%% Reading dense matrix from file (%s):", filename);
+  fflush(stdout);
+  char line[1024];
+  if (!fgets(line, sizeof(line), f)) {
+    fclose(f);
+    set_error("%s: empty file", filename);
+    return DEV_ERR_NOT_SUPPORTED;
+  }
+  char b0[64] = {0}, b1[64] = {0}, b2[64] = {0}, b3[64] = {0};
+  sscanf(line, "%63s %63s %63s %63s", b0, b1, b2, b3);
+  for (char* p = b2; *p; ++p) *p = (char)tolower(*p);
+  for (char* p = b3; *p; ++p) *p = (char)tolower(*p);
+  if (strcmp(b2, "array") != 0) {
+    fclose(f);
+    printf("\n%% Sorry, only dense (array) MatrixMarket files are read here.\n");
+    set_error("%s: not a MatrixMarket array file", filename);
+    return DEV_ERR_NOT_SUPPORTED;
+  }
+  const bool pattern = strcmp(b3, "pattern") == 0;
+  const bool complex_ = strcmp(b3, "complex") == 0;
+  int m = 0, n = 0;
+  while (fgets(line, sizeof(line), f)) {
+    if (line[0] == '%' || line[0] == '\n') continue;
+    if (sscanf(line, "%d %d", &m, &n) == 2) break;
+  }
+  const int nnz = m * n;
+  double* val = nullptr;
+  LACE_TRY(host_calloc(&val, std::max(nnz, 1)));
+  for (int e = 0; e < nnz; e++) {
+    double x = 1.0, xi = 0.0;
+    int got = 1;
+    if (pattern) {
+      int dummy;
+      got = fscanf(f, " %d", &dummy);
+    } else if (complex_) {
+      got = fscanf(f, " %lf %lf", &x, &xi);
+    } else {
+      got = fscanf(f, " %lf", &x);
+    }
+    if (got < 1) {
+      fclose(f);
+      free(val);
+      set_error("%s: truncated value list", filename);
+      return DEV_ERR_NOT_SUPPORTED;
+    }
+    val[e] = x;
+  }
+  fclose(f);
+  printf(" done.  done.\n");
+  free(A->val);
+  A->val = val;
+  A->num_rows = m;
+  A->num_cols = n;
+  A->nnz = nnz;
+  A->true_nnz = nnz;
+  A->ld = major == MagmaRowMajor ? n : m;
+  A->major = major;
+  return DEV_SUCCESS;
+}
+
+// control/magma_zmio.cpp:1221-1261: MatrixMarket array, one "%.16e" value
+// per line in storage order.
+int data_zwrite_dense_mtx(int n_row, int n_col, int nnz, data_order_t major, dataType** val,
+                          const char* filename) {
+  (void)nnz;
+  (void)major;
+  printf("%% Writing dense matrix to file (%s):", filename);
+  fflush(stdout);
+  FILE* f = fopen(filename, "w");
+  if (!f) {
+    printf("\n%% error writing matrix: file exists or missing write permission\n");
+    set_error("cannot write %s", filename);
+    return -1;
+  }
+  fprintf(f, "%%%%MatrixMarket matrix array real general\n");
+  fprintf(f, "%d %d\n", n_row, n_col);
+  for (long i = 0; i < (long)n_row * n_col; i++) fprintf(f, "%.16e\n", (*val)[i]);
+  if (fclose(f) != 0) printf("\n%% error: writing matrix failed\n");
+  else printf(" done\n");
+  fflush(stdout);
+  return DEV_SUCCESS;
+}
+
+// control/magma_zmio.cpp:1304-1318
+int data_zwrite_dense(data_d_matrix A, const char* filename) {
+  if (A.storage_type == Magma_DENSED) return data_zwrite_dense_mtx(A.nnz, 1, A.nnz, A.major, &A.val, filename);
+  if (A.pad_rows > 0 && A.pad_cols > 0)
+    return data_zwrite_dense_mtx(A.pad_rows, A.pad_cols, A.nnz, A.major, &A.val, filename);
+  return data_zwrite_dense_mtx(A.num_rows, A.num_cols, A.nnz, A.major, &A.val, filename);
+}
+
+// blas/zaxpy.cpp:52-59: dy <- alpha dx + dy (host, strided).
+void data_zaxpy(int n, dataType alpha, const dataType* dx, int incx, dataType* dy, int incy) {
+  for (int i = 0; i < n; i++) dy[(size_t)i * incy] += alpha * dx[(size_t)i * incx];
+}
+
 // control/norms.cpp:7-13
 dataType data_dnrm2(int n, dataType* x, int incx) {
   double s = 0.0;

@@ -1130,9 +1130,14 @@ void data_PariLU_v0_3_gpu(data_d_matrix* A, data_d_matrix* L, data_d_matrix* U, 
   if (parilu_dropin(A, L, U, reduction, log, 0) != DEV_SUCCESS) log->sweeps = -1;
 }
 
+// The thread count is an input here (src/parilu_v0_3_gpu_prescribedSweeps.cpp:
+// 140 reads log->omp_num_threads as the CUDA block size and leaves it), and
+// drivers name their factor files after it, so it is kept.
 void data_PariLU_v0_3_gpu_prescribedSweeps(data_d_matrix* A, data_d_matrix* L, data_d_matrix* U,
                                            dataType reduction, data_d_preconditioner_log* log) {
+  const int threads = log->omp_num_threads;
   if (parilu_dropin(A, L, U, reduction, log, 1) != DEV_SUCCESS) log->sweeps = -1;
+  log->omp_num_threads = threads;
 }
 
 }  // extern "C"

@@ -171,6 +171,11 @@ struct Solver {
   Precond* M = nullptr;
   HHEngine<double> E;
   std::unique_ptr<BlockEngine> beng;  // 32-right-hand-side solver, on first use
+  // MATLAB-syntax progress lines of the drop-in FGMRES entry points (the
+  // reference prints them unconditionally): 0 none, 1 Householder names,
+  // 2 MGS names; mlog_mkl selects the "mkltrsv" spelling (exact solves).
+  int mlog = 0;
+  bool mlog_mkl = false;
   DevBuf rbuf, partial, scal;
   int nbn = 0, rpb = 4096, pstride = 4;
   double* host_scal = nullptr;
@@ -383,6 +388,11 @@ static int solver_solve(Solver* S, const double* b, double* x, lace_solver_stats
       rtol_abs = E.host_misc[MS_RTOL];
     }
     local.final_residual_est = rnorm;
+    if (S->mlog) {
+      // src/fgmres_householder_restart.cpp:196,231
+      printf("restart = %d\n", restart);
+      printf("rnorm2 = %e; rtol = %e;\n", rnorm, rtol_abs);
+    }
     if (!std::isfinite(rnorm)) break;
     if (E.host_misc[MS_DONE] != 0.0) {
       converged = rnorm < rtol_abs;
@@ -396,8 +406,12 @@ static int solver_solve(Solver* S, const double* b, double* x, lace_solver_stats
         LACE_TRY(S->op64->apply_dd(static_cast<const double*>(E.zcol(j)), E.w.as<double>()));
       LACE_TRY(hh_step<double>(E, j, j < m - 1));
       LACE_TRY(read_misc<double>(E));
+      if (S->mlog)  // src/fgmres_householder_restart.cpp:707,710
+        printf("FGMRES_Householders_%s_search(%d) = %.16e;\n", S->mlog_mkl ? "mkltrsv" : "partrsv", j + 1,
+               E.host_misc[MS_RES]);
       if (E.host_misc[MS_DONE] != 0.0) break;
     }
+    if (S->mlog) fflush(stdout);
     LACE_TRY((hh_finish<double, double>(E, x)));
     LACE_TRY(read_misc<double>(E));
     const int nc = (int)E.host_misc[MS_NCOLS];
@@ -635,6 +649,8 @@ static int solver_solve_mgs(Solver* S, const double* b, double* x, lace_solver_s
     if (restart == 0) {
       local.initial_residual = hostsc[MG_RNORM];
       rtol_abs = hostsc[MG_RTOL];
+      if (S->mlog)  // src/fgmres.cpp:178
+        printf("rnorm2 = %e; tol = %e; rtol = %e;\n", hostsc[MG_RNORM], S->o.rtol, rtol_abs);
     }
     if (!(hostsc[MG_RNORM] >= rtol_abs)) {
       converged = std::isfinite(hostsc[MG_RNORM]);
@@ -671,6 +687,10 @@ static int solver_solve_mgs(Solver* S, const double* b, double* x, lace_solver_s
                                                    Vc(j + 1));
       LACE_HIP(hipGetLastError());
       LACE_TRY(read_sc());
+      if (S->mlog) {  // src/fgmres.cpp:430,433
+        printf("FGMRES_%s_search(%d) = %.16e;\n", S->mlog_mkl ? "mkltrsv" : "partrsv", j + 1, hostsc[MG_RES]);
+        fflush(stdout);
+      }
       if (hostsc[MG_DONE] != 0.0) break;
     }
     mgs_lsq_kernel<<<1, 64, 0, stream()>>>(sd, L);
@@ -837,9 +857,27 @@ int fgmres_dropin_common(data_d_matrix* A, data_d_matrix* b, data_d_matrix* x0, 
   o.trsv = choice == 2 ? LACE_TRSV_NONE : (choice == 3 ? LACE_TRSV_JACOBI : LACE_TRSV_LEVEL);
   o.jacobi_sweeps = LACE_JACOBI_SWEEPS_DEFAULT;
   o.parilu_sweeps = 0;
+  if (!mgs && o.search_max > 63) {
+    // the device Householder engine holds at most 63 reflectors per cycle:
+    // the same budget of search directions as cycles of 63
+    const long total = (long)o.search_max * std::max(1, o.restart_max);
+    o.search_max = 63;
+    o.restart_max = (int)((total + 62) / 63);
+  }
   Solver* S = nullptr;
   LACE_TRY(solver_create(A, &o, &S));
   std::unique_ptr<Solver> hold(S);
+  if (get_option("matlab_log", 1) != 0) {
+    S->mlog = mgs ? 2 : 1;
+    S->mlog_mkl = choice == 0;
+    if (mgs) {
+      printf("%% data_fgmres begin\n");  // src/fgmres.cpp:71
+    } else {
+      // src/fgmres_householder_restart.cpp:70,81
+      printf("%% data_fgmres_householder_restart begin\n");
+      printf("%% data_fgmres_householder restart_max = %d search_max =%d\n", o.restart_max, o.search_max);
+    }
+  }
   if (S->P) {
     // load the caller's factors: L (CSRL) into the L buffer, U (CSRU) into
     // the CSC buffer through the CSR->CSC slot map
