@@ -257,9 +257,10 @@ int data_parcsrtrsv(const data_uplo_t uplo, const data_storage_t storage,
 /* FGMRES.  sparse.h:1027-1052; src/fgmres.cpp:64-517,
  * src/fgmres_householder_restart.cpp:63-788.  L: CSRL, U: CSRU.  x0 is
  * freed and re-allocated with the solution (src/fgmres.cpp:491-492).  The
- * Householder forms keep at most 63 reflectors per cycle: a larger
- * search_max runs as cycles of 63 with the same total number of search
- * directions.  Progress lines go to stdout unless "matlab_log" is 0. */
+ * device Krylov bases hold at most 63 vectors (MGS) or reflectors
+ * (Householder) per cycle: a larger search_max runs as cycles of 63 with the
+ * same total number of search directions.  Progress lines go to stdout
+ * unless "matlab_log" is 0. */
 int data_fgmres(data_d_matrix *A, data_d_matrix *b, data_d_matrix *x0,
                 data_d_matrix *L, data_d_matrix *U,
                 data_d_gmres_param *gmres_par, data_d_gmres_log *gmres_log);

@@ -857,8 +857,8 @@ int fgmres_dropin_common(data_d_matrix* A, data_d_matrix* b, data_d_matrix* x0, 
   o.trsv = choice == 2 ? LACE_TRSV_NONE : (choice == 3 ? LACE_TRSV_JACOBI : LACE_TRSV_LEVEL);
   o.jacobi_sweeps = LACE_JACOBI_SWEEPS_DEFAULT;
   o.parilu_sweeps = 0;
-  if (!mgs && o.search_max > 63) {
-    // the device Householder engine holds at most 63 reflectors per cycle:
+  if (o.search_max > 63) {
+    // the device Krylov engines hold at most 63 basis vectors per cycle:
     // the same budget of search directions as cycles of 63
     const long total = (long)o.search_max * std::max(1, o.restart_max);
     o.search_max = 63;

@@ -19,6 +19,7 @@ TREF = os.path.join(HERE, "golden", "Trefethen_20.mtx")
 def _run(args, cwd):
     r = subprocess.run(args, cwd=cwd, capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stdout + r.stderr
+    assert "failed" not in r.stderr, r.stdout + r.stderr
     return r.stdout
 
 
@@ -55,7 +56,7 @@ def test_generate_and_solve_pipeline(gpu, orc, tmp_path):
                      "--U", fu, "--RHS", "ONES", "--outDir", out, "--tolType", "0", "--tol", "1.0e-10",
                      "--searchMax", "1000", "--csrtrsvType", "0", "--monitorOrthog", "1"], out)
         hist = re.findall(r"^FGMRES_%s_search\((\d+)\) = ([-+0-9.eE]+);$" % kind, slog, re.M)
-        assert len(hist) >= 2
+        assert len(hist) >= 2, slog
         res = [float(v) for _, v in hist]
         assert res[-1] < 1e-10
         chk = float(re.search(r"external check of rnorm2 = ([-+0-9.eE]+);", slog).group(1))
