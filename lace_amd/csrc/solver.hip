@@ -13,9 +13,11 @@
 //                 solve; preconditioned vectors are stored in FP32 and the
 //                 outer SpMV reads them widened, so the Arnoldi relation holds
 //                 exactly for the stored vectors.
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <limits>
 #include <memory>
 #include <vector>
 
@@ -145,6 +147,116 @@ struct PrecInner : Precond {
   }
 };
 
+// ------------------------------------------------------------ MATLAB log
+// dots[i] = sum_r Q[r, i] * Q[r, j] for i <= j; one workgroup per column,
+// fixed reduction order.
+__global__ __launch_bounds__(256) void ortho_dots_kernel(int n, const double* __restrict__ Q, size_t ld,
+                                                         int j, double* __restrict__ dots) {
+  __shared__ double red[256];
+  const double* a = Q + (size_t)blockIdx.x * ld;
+  const double* b = Q + (size_t)j * ld;
+  double s = 0.0;
+  for (int r = threadIdx.x; r < n; r += 256) s = fma(a[r], b[r], s);
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) dots[blockIdx.x] = red[0];
+}
+
+// MATLAB-syntax progress lines of the drop-in FGMRES entry points, in the
+// four forms of the reference: src/fgmres.cpp (MGS), src/fgmres_restart.cpp
+// (MGS, restarted), src/fgmres_householder.cpp and
+// src/fgmres_householder_restart.cpp.  The single-cycle forms number their
+// search directions across the cycles of 63 that a larger search_max runs
+// as.  With monitorOrthog the orthogonality error ||I - Q_k^T Q_k||_inf of
+// the cycle's first k basis vectors (src/orthogonality_error.cpp:46-82;
+// Householder: the preconditioned vectors q_j, src/
+// fgmres_householder_restart.cpp:628-656) precedes each search line.
+struct MatlabLog {
+  bool on = false, mgs = false, restarted = false, orthog = false;
+  int choice = 0;  // user_csrtrsv_choice
+  int offset = 0;  // search index before the current cycle (single-cycle forms)
+  DevBuf Q, dots;  // Householder basis copies, dot products
+  std::vector<double> G, hd;
+  int k = 0;
+
+  const char* tag() const { return choice == 0 ? "mkltrsv" : "partrsv"; }
+  const char* fam() const { return mgs ? "" : "Householders_"; }
+  int idx(int j) const { return restarted ? j + 1 : offset + j + 1; }
+
+  void cycle_begin(int restart, int m, double rnorm, double rtol_param, double rtol_abs) {
+    if (!on) return;
+    k = 0;
+    offset = restart * m;
+    if (restarted) {
+      printf("restart = %d\n", restart);           // src/fgmres_householder_restart.cpp:196
+      printf("rnorm2 = %e; rtol = %e;\n", rnorm, rtol_abs);  // :231
+    } else if (restart == 0) {
+      printf("rnorm2 = %e; tol = %e; rtol = %e;\n", rnorm, rtol_param, rtol_param * rnorm);  // src/fgmres.cpp:178
+    }
+  }
+  // src/fgmres.cpp:234,240: sweep counts of the two ParCSRTRSV solves.  The
+  // solves here are exact, which is what the in-place natural-order sweep
+  // of src/trisolve.cpp:281-292 reaches after one substituting sweep and one
+  // confirming sweep.
+  void trsv_lines(int j) const {
+    if (!on || choice != 1) return;
+    printf("ParCSRTRSV_L(%d) = %d;\n", idx(j), 2);
+    printf("ParCSRTRSV_U(%d) = %d;\n", idx(j), 2);
+  }
+  void reorth_line(int j) const {  // src/fgmres.cpp:316, src/fgmres_restart.cpp:338
+    if (!on) return;
+    if (restarted) printf("Reorthogonalize %d\n", j);
+    else printf("Reorthogonalize(%d) = 1;\n", offset + j);
+  }
+  void breakdown_line() const {  // src/fgmres.cpp:355
+    if (on) printf("%%\t******* happy breakdown **********\n");
+  }
+  // Column j of the basis Qb (leading dimension ld) joins the monitored set.
+  int ortho_line(int n, const double* Qb, size_t ld, int j) {
+    if (!on || !orthog) return DEV_SUCCESS;
+    if (!dots.p) {
+      LACE_TRY(dots.alloc(sizeof(double) * 64));
+      hd.resize(64);
+      G.resize(64 * 64);
+    }
+    LACE_REQUIRE(j < 64, DEV_ERR_NOT_SUPPORTED, "orthogonality monitor: column %d beyond 63", j);
+    ortho_dots_kernel<<<j + 1, 256, 0, stream()>>>(n, Qb, ld, j, dots.as<double>());
+    LACE_HIP(hipGetLastError());
+    LACE_HIP(hipMemcpyAsync(hd.data(), dots.p, sizeof(double) * (j + 1), hipMemcpyDeviceToHost, stream()));
+    LACE_HIP(hipStreamSynchronize(stream()));
+    for (int i = 0; i <= j; i++) G[i * 64 + j] = G[j * 64 + i] = hd[i];
+    k = j + 1;
+    // infinity norm: largest absolute row sum of I - G
+    double err = 0.0;
+    for (int i = 0; i < k; i++) {
+      double s = 0.0;
+      for (int l = 0; l < k; l++) s += std::fabs((i == l ? 1.0 : 0.0) - G[i * 64 + l]);
+      err = std::max(err, s);
+    }
+    printf("FGMRES_%s%s_ortherr(%d) = %.16e;\n", fam(), tag(), idx(j), err);
+    return DEV_SUCCESS;
+  }
+  // Householder: the preconditioned vector q_j is copied aside first.
+  int ortho_line_copy(int n, const double* q, int j) {
+    if (!on || !orthog) return DEV_SUCCESS;
+    const size_t ld = ((size_t)n + 63) / 64 * 64;
+    if (!Q.p) LACE_TRY(Q.alloc(sizeof(double) * ld * 64));
+    LACE_REQUIRE(j < 64, DEV_ERR_NOT_SUPPORTED, "orthogonality monitor: column %d beyond 63", j);
+    LACE_HIP(hipMemcpyAsync(Q.as<double>() + (size_t)j * ld, q, sizeof(double) * n, hipMemcpyDeviceToDevice,
+                            stream()));
+    return ortho_line(n, Q.as<double>(), ld, j);
+  }
+  void search_line(int j, double res) const {  // src/fgmres.cpp:430,433; ..._restart.cpp:707,710
+    if (!on) return;
+    printf("FGMRES_%s%s_search(%d) = %.16e;\n", fam(), tag(), idx(j), res);
+    fflush(stdout);
+  }
+};
+
 // ------------------------------------------------------------ solver
 struct Solver {
   lace_solver_opts o;
@@ -171,11 +283,7 @@ struct Solver {
   Precond* M = nullptr;
   HHEngine<double> E;
   std::unique_ptr<BlockEngine> beng;  // 32-right-hand-side solver, on first use
-  // MATLAB-syntax progress lines of the drop-in FGMRES entry points (the
-  // reference prints them unconditionally): 0 none, 1 Householder names,
-  // 2 MGS names; mlog_mkl selects the "mkltrsv" spelling (exact solves).
-  int mlog = 0;
-  bool mlog_mkl = false;
+  MatlabLog mlog;  // progress lines of the drop-in FGMRES entry points
   DevBuf rbuf, partial, scal;
   int nbn = 0, rpb = 4096, pstride = 4;
   double* host_scal = nullptr;
@@ -388,11 +496,7 @@ static int solver_solve(Solver* S, const double* b, double* x, lace_solver_stats
       rtol_abs = E.host_misc[MS_RTOL];
     }
     local.final_residual_est = rnorm;
-    if (S->mlog) {
-      // src/fgmres_householder_restart.cpp:196,231
-      printf("restart = %d\n", restart);
-      printf("rnorm2 = %e; rtol = %e;\n", rnorm, rtol_abs);
-    }
+    S->mlog.cycle_begin(restart, m, rnorm, S->o.rtol, rtol_abs);
     if (!std::isfinite(rnorm)) break;
     if (E.host_misc[MS_DONE] != 0.0) {
       converged = rnorm < rtol_abs;
@@ -400,18 +504,17 @@ static int solver_solve(Solver* S, const double* b, double* x, lace_solver_stats
     }
     for (int j = 0; j < m; j++) {
       LACE_TRY(S->M->apply_d(E.q.as<double>(), E.zcol(j)));
+      S->mlog.trsv_lines(j);
+      LACE_TRY(S->mlog.ortho_line_copy(S->n, E.q.as<double>(), j));
       if (E.z_float)
         LACE_TRY(S->op64->apply_fd(static_cast<const float*>(E.zcol(j)), E.w.as<double>()));
       else
         LACE_TRY(S->op64->apply_dd(static_cast<const double*>(E.zcol(j)), E.w.as<double>()));
       LACE_TRY(hh_step<double>(E, j, j < m - 1));
       LACE_TRY(read_misc<double>(E));
-      if (S->mlog)  // src/fgmres_householder_restart.cpp:707,710
-        printf("FGMRES_Householders_%s_search(%d) = %.16e;\n", S->mlog_mkl ? "mkltrsv" : "partrsv", j + 1,
-               E.host_misc[MS_RES]);
+      S->mlog.search_line(j, E.host_misc[MS_RES]);
       if (E.host_misc[MS_DONE] != 0.0) break;
     }
-    if (S->mlog) fflush(stdout);
     LACE_TRY((hh_finish<double, double>(E, x)));
     LACE_TRY(read_misc<double>(E));
     const int nc = (int)E.host_misc[MS_NCOLS];
@@ -649,9 +752,8 @@ static int solver_solve_mgs(Solver* S, const double* b, double* x, lace_solver_s
     if (restart == 0) {
       local.initial_residual = hostsc[MG_RNORM];
       rtol_abs = hostsc[MG_RTOL];
-      if (S->mlog)  // src/fgmres.cpp:178
-        printf("rnorm2 = %e; tol = %e; rtol = %e;\n", hostsc[MG_RNORM], S->o.rtol, rtol_abs);
     }
+    S->mlog.cycle_begin(restart, m, hostsc[MG_RNORM], S->o.rtol, rtol_abs);
     if (!(hostsc[MG_RNORM] >= rtol_abs)) {
       converged = std::isfinite(hostsc[MG_RNORM]);
       local.final_residual_est = hostsc[MG_RNORM];
@@ -661,6 +763,7 @@ static int solver_solve_mgs(Solver* S, const double* b, double* x, lace_solver_s
     scale_into_kernel<<<nbv, 256, 0, stream()>>>(n, r, sc + MG_RNORM, sc + MG_RNORM, Vc(0));
     for (int j = 0; j < m; j++) {
       LACE_TRY(S->M->apply_d(Vc(j), Zc(j)));
+      S->mlog.trsv_lines(j);
       LACE_TRY(S->op64->apply_dd(Zc(j), u.as<double>()));
       LACE_TRY(nrm_into(u.as<double>(), sc + MG_NORMAV));
       const int passes = 1;
@@ -678,6 +781,7 @@ static int solver_solve_mgs(Solver* S, const double* b, double* x, lace_solver_s
           LACE_TRY(read_sc());
           const bool again = (reorth == 0 && hostsc[MG_REORTH] != 0.0) || reorth == 2;
           if (!again) break;
+          S->mlog.reorth_line(j);
         }
       }
       mgs_step_small_kernel<<<1, 64, 0, stream()>>>(sd, L, j, j == m - 1, 2);
@@ -687,10 +791,9 @@ static int solver_solve_mgs(Solver* S, const double* b, double* x, lace_solver_s
                                                    Vc(j + 1));
       LACE_HIP(hipGetLastError());
       LACE_TRY(read_sc());
-      if (S->mlog) {  // src/fgmres.cpp:430,433
-        printf("FGMRES_%s_search(%d) = %.16e;\n", S->mlog_mkl ? "mkltrsv" : "partrsv", j + 1, hostsc[MG_RES]);
-        fflush(stdout);
-      }
+      if (!(std::fabs(hostsc[MG_NORM2]) > std::numeric_limits<double>::epsilon())) S->mlog.breakdown_line();
+      LACE_TRY(S->mlog.ortho_line(n, V.as<double>(), ld, j));
+      S->mlog.search_line(j, hostsc[MG_RES]);
       if (hostsc[MG_DONE] != 0.0) break;
     }
     mgs_lsq_kernel<<<1, 64, 0, stream()>>>(sd, L);
@@ -868,15 +971,18 @@ int fgmres_dropin_common(data_d_matrix* A, data_d_matrix* b, data_d_matrix* x0, 
   LACE_TRY(solver_create(A, &o, &S));
   std::unique_ptr<Solver> hold(S);
   if (get_option("matlab_log", 1) != 0) {
-    S->mlog = mgs ? 2 : 1;
-    S->mlog_mkl = choice == 0;
-    if (mgs) {
-      printf("%% data_fgmres begin\n");  // src/fgmres.cpp:71
-    } else {
-      // src/fgmres_householder_restart.cpp:70,81
-      printf("%% data_fgmres_householder_restart begin\n");
-      printf("%% data_fgmres_householder restart_max = %d search_max =%d\n", o.restart_max, o.search_max);
-    }
+    MatlabLog& ml = S->mlog;
+    ml.on = true;
+    ml.mgs = mgs;
+    ml.restarted = restarted;
+    ml.orthog = par->monitorOrthog == 1;
+    ml.choice = choice;
+    // src/fgmres.cpp:71, src/fgmres_restart.cpp:70,81,
+    // src/fgmres_householder.cpp:70, src/fgmres_householder_restart.cpp:70,81
+    const char* name = mgs ? "data_fgmres" : "data_fgmres_householder";
+    printf("%% %s%s begin\n", name, restarted && !mgs ? "_restart" : "");
+    if (restarted)
+      printf("%% %s restart_max = %d search_max =%d\n", name, par->restart_max, par->search_max);
   }
   if (S->P) {
     // load the caller's factors: L (CSRL) into the L buffer, U (CSRU) into

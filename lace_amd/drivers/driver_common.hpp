@@ -37,3 +37,14 @@ inline void check(int rc, const char* what) {
     std::exit(1);
   }
 }
+
+// Thread count named in the CPU driver's output: the first entry of
+// OMP_NUM_THREADS, else 1.  The reference's OpenMP sweeps report
+// omp_get_num_threads() (src/parilu_v0_3.cpp:114,130) and its scripts
+// (drivers/PariLU_testMatrix.sh) vary OMP_NUM_THREADS between runs; the
+// sweeps here run on the GPU, and the label keeps those runs' files apart.
+inline int omp_threads_label() {
+  const char* s = std::getenv("OMP_NUM_THREADS");
+  const int t = s ? std::atoi(s) : 0;
+  return t > 0 ? t : 1;
+}

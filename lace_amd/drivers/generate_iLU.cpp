@@ -59,6 +59,9 @@ int main(int argc, char** argv) {
     std::fprintf(stderr, "ParILU failed: %s\n", lace_last_error());
     return 1;
   }
+#ifndef LACE_DRIVER_GPU
+  plog.omp_num_threads = omp_threads_label();
+#endif
   // ||A - LU||_F on LU's pattern and on A's
   dataType ares = 0.0, anl = 0.0;
   data_d_matrix LU;
