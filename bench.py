@@ -50,11 +50,13 @@ MIXED = dict(search_max=10, inner_iters=10, trsv=1, jacobi_sweeps=5, prec_precon
 FP64 = dict(search_max=50, trsv=1, jacobi_sweeps=5, prec_precond=0)
 
 
-def timed_solves(L, S, db, dx, zeros, k):
+def timed_solves(L, S, db, dx, zeros, k, flush=None):
     lib = L.lib()
     out = []
     for _ in range(k):
         dx.upload(zeros)
+        if flush:
+            flush()
         L.check(lib.lace_device_synchronize())
         t0 = time.perf_counter()
         S.setup()
@@ -86,11 +88,15 @@ def cpu_baseline(row, col, val, b, threads):
                        f"substitution, {r.searches} iterations to rel. residual {rel:.2e}")}
 
 
+# kernel classes by the kernel each launch runs on the 128^3 operator (the
+# FP32 streaming sweep; Jacobi sweeps and products on the diagonal form)
 SWEEP_KERNEL = "parilu_stream4_sweep_kernel<float>"
-JACOBI_KERNEL = "tri_jacobi_ell_kernel<float>"
-SPMV64_KERNEL = "spmv_ell_kernel<double, float, double>"
-SPMV32_KERNEL = "spmv_ell_kernel<float, float, float>"
-# kernel class -> per-launch timing regions of the library
+JACOBI_KERNEL = "tri_jacobi_dia_kernel<float>"
+SPMV64_KERNEL = "spmv_dia_kernel<double, float, double>"
+SPMV32_KERNEL = "spmv_dia_kernel<float, float, float>"
+# kernel class -> per-launch timing regions of the library; each launch
+# records its algorithmic bytes (every element of the storage form it reads,
+# and its vectors, once)
 KERNEL_CLASSES = {
     SWEEP_KERNEL: ["parilu_sweep"],
     JACOBI_KERNEL: ["trsv_jacobi_sweep_lower", "trsv_jacobi_sweep_upper"],
@@ -98,22 +104,6 @@ KERNEL_CLASSES = {
     SPMV32_KERNEL: ["spmv_f32a"],
 }
 HOT_REGIONS = [r for rs in KERNEL_CLASSES.values() for r in rs]
-
-
-def kernel_bytes(n, nnz, nnz_l, nnz_u, sweep_bytes):
-    """Algorithmic HBM bytes per launch of each timed region (int32
-    indices): ParILU sweep nnz*(4 + 4 + 2*4) + 4(n+1); Jacobi sweep on the
-    off-diagonal entries of L (unit diagonal) or of U (plus U's diagonal):
-    nnz_off*(4+4) [+ 4n] + 4(n+1) + 3*4n (r, previous iterate, new iterate);
-    outer SpMV with A in FP64 and the FP32 preconditioned vector:
-    12 nnz + 4(n+1) + 4n + 8n; inner FP32 SpMV: 8 nnz + 4(n+1) + 8n."""
-    return {
-        "parilu_sweep": sweep_bytes,
-        "trsv_jacobi_sweep_lower": 8.0 * nnz_l + 4.0 * (n + 1) + 12.0 * n,
-        "trsv_jacobi_sweep_upper": 8.0 * (nnz_u - n) + 4.0 * n + 4.0 * (n + 1) + 12.0 * n,
-        "spmv": 12.0 * nnz + 4.0 * (n + 1) + 12.0 * n,
-        "spmv_f32a": 8.0 * nnz + 4.0 * (n + 1) + 8.0 * n,
-    }
 
 
 def pmc_traffic():
@@ -126,6 +116,89 @@ def pmc_traffic():
     with open(p) as f:
         data = json.load(f)
     return {k: v.get("bytes_per_launch") for k, v in data.get("kernels", {}).items()}
+
+
+FLUSH_BYTES = 1 << 30
+# events around every k-th launch of each hot region in the timed steps
+TIMING_SAMPLE = 8
+# bytes read (and written) by the reference copy: 2 x 120 MiB ~ one FP32 sweep
+COPY_BYTES = 120 << 20
+
+
+def make_flush(L):
+    """A copy of 512 MiB into another 512 MiB buffer: it touches 1 GiB of
+    HBM, four times the 256 MB Infinity Cache, so the kernel launched next
+    finds none of its data in the Infinity Cache or L2."""
+    lib = L.lib()
+    half = FLUSH_BYTES // 2
+    src = L.DevVec(half // 8, np.float64)
+    dst = L.DevVec(half // 8, np.float64)
+
+    def flush():
+        L.check(lib.lace_stream_copy(dst.ptr, src.ptr, half))
+        L.check(lib.lace_device_synchronize())
+    flush.buffers = (src, dst)
+    return flush
+
+
+def cache_rates(L, H, n, flush, reps=10):
+    """Per-launch duration and rate of each hot kernel class run on its own:
+    warm (reps launches back to back, data left in the caches by the
+    previous launch) and cold (each launch after a flush).  Bytes per call
+    are those the launches recorded for the storage form they read."""
+    lib = L.lib()
+    rng = np.random.default_rng(0)
+    Ps = L.ParILU(H, L.LACE_F32)
+    Pj = L.ParILU(H, L.LACE_F32)
+    Pj.sweeps(3)
+    r = L.DevVec(n, np.float32, rng.standard_normal(n).astype(np.float32))
+    z = L.DevVec(n, np.float32)
+    A64 = L.DevCSRMatrix(H, L.LACE_F64)
+    A32 = L.DevCSRMatrix(H, L.LACE_F32)
+    x64 = L.DevVec(n, np.float64, rng.standard_normal(n))
+    y64 = L.DevVec(n, np.float64)
+    x32 = L.DevVec(n, np.float32, rng.standard_normal(n).astype(np.float32))
+    y32 = L.DevVec(n, np.float32)
+    csrc = L.DevVec(COPY_BYTES // 8, np.float64)
+    cdst = L.DevVec(COPY_BYTES // 8, np.float64)
+    cases = [
+        (SWEEP_KERNEL, ["parilu_sweep"], lambda: Ps.sweeps(1)),
+        # Jacobi-2: one timed off-diagonal sweep each for L and U
+        (JACOBI_KERNEL, ["trsv_jacobi_sweep_lower", "trsv_jacobi_sweep_upper"],
+         lambda: L.check(lib.lace_parilu_apply(Pj.h, L.TRSV_JACOBI, 2, 0, r.ptr, z.ptr))),
+        # FP64 A, x and y (the solve's outer SpMV reads an FP32 x)
+        (SPMV64_KERNEL, ["spmv"], lambda: L.check(lib.lace_csr_spmv(A64.h, 1.0, x64.ptr, 0.0, y64.ptr))),
+        (SPMV32_KERNEL, ["spmv_f32a"], lambda: L.check(lib.lace_csr_spmv(A32.h, 1.0, x32.ptr, 0.0, y32.ptr))),
+        # a plain copy moving as many bytes as the FP32 sweep: what a
+        # streaming kernel of this length reaches
+        ("stream_copy_kernel", ["stream_copy"],
+         lambda: L.check(lib.lace_stream_copy(cdst.ptr, csrc.ptr, COPY_BYTES))),
+    ]
+    out = {}
+    for name, regions, fn in cases:
+        fn()  # first call: one-time conversions outside the timings
+        L.check(lib.lace_device_synchronize())
+        per = {}
+        for label in ("warm", "cold"):
+            L.check(lib.lace_timing_reset())
+            L.check(lib.lace_timing_only(",".join(regions).encode()))
+            for _ in range(reps):
+                if label == "cold":
+                    flush()
+                L.check(lib.lace_timing_enable(1))
+                fn()
+                L.check(lib.lace_timing_enable(0))
+            L.check(lib.lace_device_synchronize())
+            L.check(lib.lace_timing_only(None))
+            # one launch per region per call: the call's kernel time and
+            # bytes are the sums of the regions' per-launch averages
+            us = sum(L.timing(rg)[0] / L.timing(rg)[1] for rg in regions) * 1e3
+            nbytes = sum(L.timing_work(rg)[1] / L.timing(rg)[1] for rg in regions)
+            per[f"{label}_us"] = round(us, 2)
+            per[f"{label}_GBps"] = round(nbytes / (us * 1e-6) / 1e9, 1)
+        per["bytes_per_call"] = round(nbytes)
+        out[name] = per
+    return out
 
 
 def _region_rate(L, name, nbytes, fn, reps):
@@ -385,41 +458,56 @@ def main() -> int:
     for S in (mixed, fp64):
         timed_solves(L, S, db, dx, zeros, args.warmup)
 
-    # timed region: mixed steps with the hot kernels timed per launch by HIP
-    # events on the library stream (only these regions carry events)
+    # timed region: mixed steps with the hot kernels timed by HIP events on
+    # the library stream (only these regions carry events), every
+    # TIMING_SAMPLE-th launch of each region
     L.check(lib.lace_timing_reset())
+    L.check(lib.lace_set_option(b"timing_sample", TIMING_SAMPLE))
     L.check(lib.lace_timing_only(",".join(HOT_REGIONS).encode()))
     L.check(lib.lace_timing_enable(1))
     res_mixed = timed_solves(L, mixed, db, dx, zeros, args.steps)
     L.check(lib.lace_timing_enable(0))
+    L.check(lib.lace_set_option(b"timing_sample", 1))
     live = {r: L.timing(r) for r in HOT_REGIONS}
+    calls = {r: L.timing_calls(r) for r in HOT_REGIONS}
+    work = {r: L.timing_work(r) for r in HOT_REGIONS}
     res_fp64 = timed_solves(L, fp64, db, dx, zeros, args.steps)
+    # the same mixed steps without events (the events' cost), and each
+    # started with nothing of its data in the caches
+    res_plain = timed_solves(L, mixed, db, dx, zeros, args.steps)
+    flush = make_flush(L)
+    res_cold = timed_solves(L, mixed, db, dx, zeros, args.steps, flush=flush)
+    t_cold = float(np.mean([t for t, _ in res_cold]))
+    t_plain = float(np.mean([t for t, _ in res_plain]))
 
     t_mixed = float(np.mean([t for t, _ in res_mixed]))
     t_fp64 = float(np.mean([t for t, _ in res_fp64]))
     st_m, st_f = res_mixed[-1][1], res_fp64[-1][1]
-    ok = all(s.converged == 1 and s.rel_true_residual <= 1e-8 for _, s in res_mixed + res_fp64)
+    ok = all(s.converged == 1 and s.rel_true_residual <= 1e-8
+             for _, s in res_mixed + res_fp64 + res_plain + res_cold)
 
     # per-kernel rates from the live timings, algorithmic bytes per launch
-    P = L.ParILU(H, L.LACE_F32)
-    nnz_l = (nnz - n) // 2  # strict-lower entries (structurally symmetric pattern)
-    nnz_u = nnz - nnz_l
-    kb = kernel_bytes(n, nnz, nnz_l, nnz_u, P.sweep_bytes())
     traffic = pmc_traffic()
     kernels = {}
     for name, regions in KERNEL_CLASSES.items():
         ms = sum(live[r][0] for r in regions)
         cnt = sum(live[r][1] for r in regions)
-        by = sum(kb[r] * live[r][1] for r in regions)
+        # algorithmic bytes of the timed launches, as each launch recorded
+        # them for the storage form it read
+        by = sum(work[r][1] for r in regions)
         if cnt == 0:
             continue
         avg_s = ms * 1e-3 / cnt
         ach = by / (ms * 1e-3) / 1e9
+        # all launches of the class in the timed steps, at the sampled
+        # launches' average duration per region
+        est_ms = sum(live[r][0] / live[r][1] * calls[r] for r in regions if live[r][1])
         kernels[name] = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS,
                          "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
                          "traffic": (traffic or {}).get(name),
                          "bytes_per_launch": round(by / cnt), "avg_launch_us": round(avg_s * 1e6, 2),
-                         "launches_timed": cnt, "total_ms_in_timed_steps": round(ms, 3)}
+                         "launches_timed": cnt, "launches": sum(calls[r] for r in regions),
+                         "total_ms_in_timed_steps": round(est_ms, 3)}
     dominant = max(kernels, key=lambda k: kernels[k]["total_ms_in_timed_steps"])
     sw = kernels.get(SWEEP_KERNEL, {})
     achieved = sw.get("achieved", float("nan"))
@@ -432,6 +520,8 @@ def main() -> int:
     s64_ms, s64_cnt = L.timing("parilu_sweep")
     ach64 = P64.sweep_bytes() / (s64_ms / s64_cnt * 1e-3) / 1e9
     L.check(lib.lace_timing_only(None))
+    del P64
+    cache = cache_rates(L, H, n, flush)
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -469,6 +559,15 @@ def main() -> int:
         # the kernel with the most time in the timed mixed steps
         "roofline": dict(kernel=dominant, **kernels[dominant]),
         "kernels": kernels,
+        # value and ms_per_step: steps run back to back (each finds what the
+        # previous one left in the 256 MB Infinity Cache); cold: each step
+        # and each isolated launch preceded by a copy touching 1 GiB
+        "cache": {"flush_bytes": FLUSH_BYTES, "ms_per_step_warm": round(t_mixed * 1e3, 3),
+                  "ms_per_step_cold": round(t_cold * 1e3, 3), "isolated_launches": cache},
+        # the timed steps carry events around every TIMING_SAMPLE-th hot
+        # launch; the same steps without any events:
+        "timing_sample": TIMING_SAMPLE,
+        "ms_per_step_without_events": round(t_plain * 1e3, 3),
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

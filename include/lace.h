@@ -353,6 +353,11 @@ int lace_csr_n(const lace_csr *A);
 /* y = alpha*A*x + beta*y on device vectors of A's precision. */
 int lace_csr_spmv(const lace_csr *A, double alpha, const void *x, double beta,
                   void *y);
+/* Storage forms held besides CSR: ndiag > 0 when the entries lie on ndiag
+ * (at most 8) diagonals and the diagonal form applies; ell_width > 0 when
+ * the ELLPACK form (that many slots per row) applies.  The products use the
+ * diagonal form, else ELLPACK, else CSR (options "spmv_dia", "spmv_ell"). */
+int lace_csr_forms(const lace_csr *A, int *ndiag, int *ell_width);
 
 /* ParILU(0) on device.  Factor storage in prec (F64 or F32; BF16 keeps a
  * per-row scale), iteration arithmetic in F64 for F64 and F32 otherwise. */
@@ -377,6 +382,10 @@ double lace_parilu_sweep_bytes(const lace_parilu *P);
 int lace_parilu_apply(lace_parilu *P, lace_trsv_kind_t kind, int jacobi_sweeps,
                       int trans, const void *r, void *z);
 int lace_parilu_levels(const lace_parilu *P, int *nlev_L, int *nlev_U);
+/* Storage forms of the off-diagonal entries of L (upper = 0) or U
+ * (upper = 1) for the Jacobi sweeps, as for lace_csr_forms (options
+ * "jacobi_dia", "jacobi_ell"). */
+int lace_parilu_forms(const lace_parilu *P, int upper, int *ndiag, int *ell_width);
 /* Sweep kernel selection: 0 (default) runs the four-position streaming
  * symmetric kernel (5) when A's pattern is structurally symmetric, else the
  * general one;
@@ -565,7 +574,9 @@ int lace_block_panel_qr(int n, int r0, const double *W, double *Y, double *Tb, d
  * drop-in FGMRES entry points print the reference's MATLAB-syntax progress
  * lines to stdout (src/fgmres.cpp:71,178,430; src/fgmres_householder_restart.cpp:
  * 70,81,196,231,707) -- "FGMRES_Householders_partrsv_search(j) = |g_j|;" etc.,
- * the log format its post-processing scripts read; 0 silences them. */
+ * the log format its post-processing scripts read; 0 silences them.
+ * "timing_sample" (>= 1, default 1): the timing regions below record events
+ * around every k-th entry of each region only. */
 int lace_set_option(const char *name, int value);
 
 /* Kernel timing by class, HIP events on the library stream. */
@@ -576,7 +587,10 @@ int lace_timing_reset(void);
  * "parilu_sweep", "spmv" (FP64 matrix), "spmv_f32a" (FP32 matrix),
  * "trsv_jacobi_sweep_lower" / "_upper" (one stream Jacobi sweep). */
 int lace_timing_only(const char *name);
+/* Summed duration and number of the timed entries of a region. */
 int lace_timing_get(const char *name, double *total_ms, long *count);
+/* Entries of a region while timing was on, timed or skipped by sampling. */
+int lace_timing_get_calls(const char *name, long *calls);
 /* Algorithmic work summed over the timed launches of a region: flops and
  * HBM bytes (recorded by the regions whose shape varies between launches:
  * "tsgemm_tn", "tsgemm_nn", and the block kernels "spmm32", "spmm32_f32x",

@@ -162,6 +162,7 @@ SIGNATURES = {
     "lace_csr_destroy": (_i, [_vp]),
     "lace_csr_n": (_i, [_vp]),
     "lace_csr_spmv": (_i, [_vp, _d, _vp, _d, _vp]),
+    "lace_csr_forms": (_i, [_vp, _P(_i), _P(_i)]),
     "lace_parilu_create": (_i, [_DM, _i, _P(_vp)]),
     "lace_parilu_destroy": (_i, [_vp]),
     "lace_parilu_reset": (_i, [_vp]),
@@ -171,6 +172,7 @@ SIGNATURES = {
     "lace_parilu_sweep_bytes": (_d, [_vp]),
     "lace_parilu_apply": (_i, [_vp, _i, _i, _i, _vp, _vp]),
     "lace_parilu_levels": (_i, [_vp, _P(_i), _P(_i)]),
+    "lace_parilu_forms": (_i, [_vp, _i, _P(_i), _P(_i)]),
     "lace_parilu_set_variant": (_i, [_vp, _i]),
     "lace_syncfree_error": (_i, [_vp, _P(_i)]),
     "lace_bcsr_create": (_i, [_DM, _i, _P(_vp)]),
@@ -209,6 +211,7 @@ SIGNATURES = {
     "lace_timing_only": (_i, [C.c_char_p]),
     "lace_timing_get": (_i, [C.c_char_p, _P(_d), _P(C.c_long)]),
     "lace_timing_get_work": (_i, [C.c_char_p, _P(_d), _P(_d)]),
+    "lace_timing_get_calls": (_i, [C.c_char_p, _P(C.c_long)]),
     "lace_level_sets": (_i, [_i, _P(_i), _P(_i), _i, _P(_i), _P(_i)]),
     "lace_gen_laplace2d": (_i, [_i, _i, _DM]),
     "lace_gen_convdiff3d": (_i, [_i, _i, _i, _d, _d, _d, _DM]),
@@ -451,6 +454,13 @@ class ParILU:
         check(lib().lace_parilu_levels(self.h, C.byref(a), C.byref(b)), "levels")
         return a.value, b.value
 
+    def forms(self, upper: bool):
+        """(diagonals, ELLPACK width) of L's or U's off-diagonal entries
+        (0 where that form does not apply)."""
+        d = C.c_int(); w = C.c_int()
+        check(lib().lace_parilu_forms(self.h, int(upper), C.byref(d), C.byref(w)), "forms")
+        return d.value, w.value
+
     def apply(self, r: np.ndarray, kind=TRSV_LEVEL, sweeps=5, trans=0):
         dt = np.float64 if self.prec == LACE_F64 else np.float32
         dr = DevVec(self.n, dt, r)
@@ -490,6 +500,20 @@ class DevCSRMatrix:
         self.n = A.m.num_rows
         self.prec = prec
 
+    def forms(self):
+        """(diagonals, ELLPACK width) held besides CSR (0: not applicable)."""
+        d = C.c_int(); w = C.c_int()
+        check(lib().lace_csr_forms(self.h, C.byref(d), C.byref(w)), "csr_forms")
+        return d.value, w.value
+
+    def spmv(self, x: np.ndarray, alpha=1.0, beta=0.0, y=None) -> np.ndarray:
+        """alpha A x + beta y on device vectors of the matrix precision."""
+        dt = np.float64 if self.prec == LACE_F64 else np.float32
+        dx = DevVec(self.n, dt, np.asarray(x, dt))
+        dy = DevVec(self.n, dt, np.zeros(self.n, dt) if y is None else np.asarray(y, dt))
+        check(lib().lace_csr_spmv(self.h, alpha, dx.ptr, beta, dy.ptr), "csr_spmv")
+        return dy.download()
+
     def spmm32(self, X: np.ndarray, B=None) -> np.ndarray:
         """A X, or B - A X, for (n, 32) blocks; X FP64 or FP32."""
         xp = LACE_F64 if X.dtype == np.float64 else LACE_F32
@@ -502,13 +526,6 @@ class DevCSRMatrix:
         check(lib().lace_csr_spmm32(self.h, dx.ptr, xp, db.ptr if db is not None else None,
                                     dy.ptr), "spmm32")
         return dy.download().reshape(self.n, NRHS)
-
-    def spmv(self, x: np.ndarray) -> np.ndarray:
-        dt = np.float64 if self.prec == LACE_F64 else np.float32
-        dx = DevVec(self.n, dt, x)
-        dy = DevVec(self.n, dt)
-        check(lib().lace_csr_spmv(self.h, 1.0, dx.ptr, 0.0, dy.ptr), "spmv")
-        return dy.download()
 
     def __del__(self):
         try:
@@ -718,6 +735,14 @@ def timing(name: str):
     t = C.c_double(); c = C.c_long()
     lib().lace_timing_get(name.encode(), C.byref(t), C.byref(c))
     return t.value, c.value
+
+
+def timing_calls(name: str) -> int:
+    """Entries of a region while timing was on (timed or skipped by the
+    "timing_sample" option)."""
+    c = C.c_long()
+    lib().lace_timing_get_calls(name.encode(), C.byref(c))
+    return c.value
 
 
 def timing_work(name: str):

@@ -712,7 +712,7 @@ static void launch_sym(ParILU* P, bool with_step, int in, int out, int nb) {
 template <typename T>
 static int sweep_t(ParILU* P, bool with_step) {
   const int in = P->cur, out = 1 - P->cur;
-  TimedRegion tr("parilu_sweep");
+  TimedRegion tr("parilu_sweep", 0.0, P->sweep_bytes());
   // symmetric patterns default to the four-position streaming kernel
   // (128^3: FP32 38.8 us vs 43.7 us for one position per thread)
   if (P->sym && (P->variant == 0 || P->variant == 5)) {
@@ -973,6 +973,15 @@ int lace_parilu_levels(const lace_parilu* Ph, int* nlev_L, int* nlev_U) {
   auto* P = reinterpret_cast<const ParILU*>(Ph);
   *nlev_L = P->lower.nlev;
   *nlev_U = P->upper.nlev;
+  return DEV_SUCCESS;
+}
+
+int lace_parilu_forms(const lace_parilu* Ph, int upper, int* ndiag, int* ell_width) {
+  LACE_REQUIRE(Ph && ndiag && ell_width, DEV_ERR_INVALID_PTR, "lace_parilu_forms: null argument");
+  auto* P = reinterpret_cast<const ParILU*>(Ph);
+  const TriPlan& T = upper ? P->upper : P->lower;
+  *ndiag = T.dia.nd;
+  *ell_width = T.ew;
   return DEV_SUCCESS;
 }
 

@@ -5,6 +5,7 @@
 #include <vector>
 
 #include "common.hpp"
+#include "dia.hpp"
 
 namespace lace {
 
@@ -40,6 +41,9 @@ struct TriPlan {
   // slot's position in the full CSR, eval its gathered value.
   int ew = 0;
   DevBuf ecol, emap, eval;
+  // The same off-diagonal entries in diagonal form when they lie on at most
+  // 8 diagonals (stencil factors); gathered with the other forms.
+  DiaForm dia;
   int build(int n, const int* row, const int* col, bool upper);
   int refresh_strict(const void* val, size_t elem_size);
 };

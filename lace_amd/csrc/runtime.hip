@@ -87,13 +87,29 @@ int download(void* host, const DevBuf& b, size_t bytes) {
 struct TimingEntry {
   std::vector<std::pair<hipEvent_t, hipEvent_t>> pending;
   double total_ms = 0.0;
-  long count = 0;
+  long count = 0;  // timed entries
+  long calls = 0;  // all entries while timing was on (timed or skipped by sampling)
   double flops = 0.0, bytes = 0.0;
 };
 static std::mutex g_tmu;
 static std::map<std::string, TimingEntry> g_timing;
 static bool g_timing_on = false;
 static std::string g_timing_only;  // empty: time every region
+static int g_timing_sample = 1;    // time every k-th entry of each region
+// Events are reused: creating two per timed region costs several
+// microseconds of host time per launch, which a launch-bound solve loop
+// would otherwise carry into its wall time.
+static std::vector<hipEvent_t> g_event_pool;
+
+static hipEvent_t pooled_event() {  // caller holds g_tmu
+  if (!g_event_pool.empty()) {
+    hipEvent_t e = g_event_pool.back();
+    g_event_pool.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  return hipEventCreate(&e) == hipSuccess ? e : nullptr;
+}
 
 bool timing_on() { return g_timing_on; }
 
@@ -113,9 +129,23 @@ static bool timing_selected(const char* n) {
 
 TimedRegion::TimedRegion(const char* n) : name(n) {
   if (!g_timing_on) return;
+  std::lock_guard<std::mutex> lk(g_tmu);
   if (!timing_selected(n)) return;
-  if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) return;
+  TimingEntry& e = g_timing[name];
+  if (e.calls++ % g_timing_sample != 0) return;
+  a = pooled_event();
+  b = pooled_event();
+  if (!a || !b) {
+    if (a) g_event_pool.push_back(a);
+    if (b) g_event_pool.push_back(b);
+    a = b = nullptr;
+    return;
+  }
   on = hipEventRecord(a, stream()) == hipSuccess;
+  if (!on) {
+    g_event_pool.push_back(a);
+    g_event_pool.push_back(b);
+  }
 }
 
 TimedRegion::TimedRegion(const char* n, double flops_, double bytes_) : TimedRegion(n) {
@@ -143,8 +173,8 @@ static void drain_timing() {
         kv.second.total_ms += ms;
         kv.second.count++;
       }
-      (void)hipEventDestroy(ev.first);
-      (void)hipEventDestroy(ev.second);
+      g_event_pool.push_back(ev.first);
+      g_event_pool.push_back(ev.second);
     }
     kv.second.pending.clear();
   }
@@ -217,6 +247,7 @@ int lace_stream_copy(void* dst, const void* src, size_t bytes) {
   LACE_REQUIRE(bytes % 16 == 0, DEV_ERR_ILLEGAL_VALUE, "lace_stream_copy: bytes must be a multiple of 16");
   const size_t n16 = bytes / 16;
   const int grid = (int)std::min<size_t>((n16 + 255) / 256, (size_t)num_cus() * 16);
+  TimedRegion tr("stream_copy", 0.0, 2.0 * (double)bytes);
   stream_copy_kernel<<<grid > 0 ? grid : 1, 256, 0, stream()>>>(n16, static_cast<const uint4*>(src),
                                                               static_cast<uint4*>(dst));
   LACE_HIP(hipGetLastError());
@@ -256,6 +287,13 @@ int lace_timing_get(const char* name, double* total_ms, long* count) {
   return DEV_SUCCESS;
 }
 
+int lace_timing_get_calls(const char* name, long* calls) {
+  std::lock_guard<std::mutex> lk(g_tmu);
+  auto it = g_timing.find(name);
+  *calls = it == g_timing.end() ? 0 : it->second.calls;
+  return it == g_timing.end() ? DEV_ERR_NOT_FOUND : DEV_SUCCESS;
+}
+
 int lace_timing_get_work(const char* name, double* flops, double* bytes) {
   drain_timing();
   std::lock_guard<std::mutex> lk(g_tmu);
@@ -273,8 +311,13 @@ extern "C" int lace_set_option(const char* name, int value) {
   if (key == "stream_rows_per_thread") {
     LACE_REQUIRE(value == 1 || value == 2 || value == 4, DEV_ERR_ILLEGAL_VALUE,
                  "stream_rows_per_thread must be 1, 2 or 4 (have %d)", value);
-  } else if (key == "jacobi_ell" || key == "spmv_ell" || key == "matlab_log") {
+  } else if (key == "jacobi_ell" || key == "spmv_ell" || key == "jacobi_dia" || key == "spmv_dia" ||
+             key == "matlab_log") {
     LACE_REQUIRE(value == 0 || value == 1, DEV_ERR_ILLEGAL_VALUE, "%s must be 0 or 1 (have %d)", name, value);
+  } else if (key == "timing_sample") {
+    LACE_REQUIRE(value >= 1, DEV_ERR_ILLEGAL_VALUE, "timing_sample must be >= 1 (have %d)", value);
+    std::lock_guard<std::mutex> lk(g_tmu);
+    g_timing_sample = value;
   } else {
     LACE_REQUIRE(false, DEV_ERR_ILLEGAL_VALUE, "unknown option %s", name);
   }

@@ -189,9 +189,77 @@ __global__ __launch_bounds__(256) void spmv_ell_kernel(int n, const int* __restr
   }
 }
 
+// Diagonal form: thread per row, the row's value on diagonal d read at
+// d*n + i and x at i + off[d] (contiguous across the wavefront; rows without
+// an entry there read x[i] and skip it); the products are summed in
+// ascending offset = CSR order, the same arithmetic as the other forms.
+template <typename TV, typename TX, typename TY, int MODE, int D>
+__global__ __launch_bounds__(256) void spmv_dia_kernel(int n, DiaOffsets off, const uint8_t* __restrict__ mask,
+                                                       const TV* __restrict__ dia,
+                                                       const TX* __restrict__ x, TY* __restrict__ y,
+                                                       const TY* __restrict__ b, double alpha, double beta) {
+#pragma clang fp contract(off)
+  using ACC = typename std::conditional<sizeof(TV) == 8 || sizeof(TY) == 8, double, float>::type;
+  const int i = xcd_block(blockIdx.x, gridDim.x) * 256 + (int)threadIdx.x;
+  if (i >= n) return;
+  const unsigned m = mask[i];
+  TV v[D];
+  TX xv[D];
+#pragma unroll
+  for (int q = 0; q < D; q++) v[q] = dia[(size_t)q * n + i];
+#pragma unroll
+  for (int q = 0; q < D; q++) xv[q] = x[((m >> q) & 1u) ? i + off.o[q] : i];
+  ACC add = ACC(0);
+  if (MODE == 1) add = (ACC)b[i];
+  else if (beta != 0.0) add = (ACC)beta * (ACC)y[i];
+  ACC s = ACC(0);
+#pragma unroll
+  for (int q = 0; q < D; q++)
+    if ((m >> q) & 1u) s = s + (ACC)v[q] * (ACC)xv[q];
+  if (MODE == 1) {
+    y[i] = (TY)(add - s);
+  } else {
+    ACC o = (ACC)alpha * s;
+    if (beta != 0.0) o = o + add;
+    y[i] = (TY)o;
+  }
+}
+
+// Which form launch_spmv uses: 2 diagonal, 1 ELLPACK, 0 CSR.
+static int spmv_form(const DevCSR& A) {
+  if (A.dia.nd > 0 && get_option("spmv_dia", 1) != 0) return 2;
+  if (A.ew > 0 && get_option("spmv_ell", 1) != 0) return 1;
+  return 0;
+}
+
 template <typename TV, typename TX, typename TY, int MODE>
 static int launch_spmv(const DevCSR& A, const TX* x, TY* y, const TY* b, double alpha, double beta) {
-  if (A.ew > 0 && get_option("spmv_ell", 1) != 0) {
+  const int form = spmv_form(A);
+  if (form == 2) {
+    const int nb = ceil_div(A.n, 256);
+    const uint8_t* mk = A.dia.mask.as<uint8_t>();
+    const TV* dv = A.dia.val.as<TV>();
+    switch (A.dia.nd) {
+#define LACE_SPMV_DIA(D)                                                                                        \
+  case D:                                                                                                       \
+    spmv_dia_kernel<TV, TX, TY, MODE, D><<<nb, 256, 0, stream()>>>(A.n, A.dia.off, mk, dv, x, y, b, alpha, beta); \
+    break;
+      LACE_SPMV_DIA(1)
+      LACE_SPMV_DIA(2)
+      LACE_SPMV_DIA(3)
+      LACE_SPMV_DIA(4)
+      LACE_SPMV_DIA(5)
+      LACE_SPMV_DIA(6)
+      LACE_SPMV_DIA(7)
+      LACE_SPMV_DIA(8)
+#undef LACE_SPMV_DIA
+      default:
+        break;
+    }
+    LACE_HIP(hipGetLastError());
+    return DEV_SUCCESS;
+  }
+  if (form == 1) {
     const int nb = ceil_div(A.n, 256);
     const int* ec = A.ecol.as<int>();
     const TV* ev = A.eval.as<TV>();
@@ -273,9 +341,9 @@ int DevCSR::create(const data_d_matrix* A, lace_prec_t p) {
       }
     LACE_TRY(upload(ecol, ec.data(), sizeof(int) * ec.size()));
     LACE_TRY(upload(emap, em.data(), sizeof(int) * em.size()));
-    LACE_TRY(refresh_ell());
   }
-  return DEV_SUCCESS;
+  if (n == ncols) LACE_TRY(dia.build(n, A->row, A->col, [](int, int) { return true; }));
+  return refresh_ell();
 }
 
 template <typename T>
@@ -286,9 +354,10 @@ __global__ void csr_ell_gather_kernel(size_t m, const int* __restrict__ emap, co
 }
 
 int DevCSR::refresh_ell() {
+  const size_t es = prec == LACE_F64 ? sizeof(double) : sizeof(float);
+  LACE_TRY(dia.gather(val.p, es));
   if (ew == 0) return DEV_SUCCESS;
   const size_t m = (size_t)ew * n;
-  const size_t es = prec == LACE_F64 ? sizeof(double) : sizeof(float);
   if (eval.bytes < es * m) LACE_TRY(eval.alloc(es * m));
   if (prec == LACE_F64)
     csr_ell_gather_kernel<double><<<ceil_div((long)m, 256), 256, 0, stream()>>>(
@@ -300,23 +369,30 @@ int DevCSR::refresh_ell() {
   return DEV_SUCCESS;
 }
 
+// Bytes of the form launch_spmv uses, each element once: the matrix
+// (values and masks, values and columns, or CSR), x, y, and b for the
+// residual form.
 double DevCSR::bytes(int sx, int sy, bool residual) const {
   const double sv = prec == LACE_F64 ? 8.0 : 4.0;
-  return (double)nnz * (sv + 4.0) + 4.0 * (n + 1) + (double)sx * ncols + (double)sy * n +
-         (residual ? (double)sy * n : 0.0);
+  const int form = spmv_form(*this);
+  const double mat = form == 2 ? dia.bytes((size_t)sv)
+                     : form == 1 ? (double)ew * n * (sv + 4.0)
+                                 : (double)nnz * (sv + 4.0) + 4.0 * (n + 1);
+  return mat + (double)sx * ncols + (double)sy * n + (residual ? (double)sy * n : 0.0);
 }
 
 // y = alpha*A*x + beta*y; vector precisions follow the template arguments.
 template <typename TX, typename TY>
 int spmv(const DevCSR& A, double alpha, const TX* x, double beta, TY* y) {
-  TimedRegion tr(A.prec == LACE_F64 ? "spmv" : "spmv_f32a");
+  TimedRegion tr(A.prec == LACE_F64 ? "spmv" : "spmv_f32a", 0.0,
+                 A.bytes((int)sizeof(TX), (int)sizeof(TY), false) + (beta != 0.0 ? sizeof(TY) * (double)A.n : 0.0));
   if (A.prec == LACE_F64) return launch_spmv<double, TX, TY, 0>(A, x, y, nullptr, alpha, beta);
   return launch_spmv<float, TX, TY, 0>(A, x, y, nullptr, alpha, beta);
 }
 
 template <typename TX, typename TY>
 int residual(const DevCSR& A, const TX* x, const TY* b, TY* r) {
-  TimedRegion tr("spmv_residual");
+  TimedRegion tr("spmv_residual", 0.0, A.bytes((int)sizeof(TX), (int)sizeof(TY), true));
   if (A.prec == LACE_F64) return launch_spmv<double, TX, TY, 1>(A, x, r, b, 1.0, 0.0);
   return launch_spmv<float, TX, TY, 1>(A, x, r, b, 1.0, 0.0);
 }
@@ -350,6 +426,14 @@ int lace_csr_destroy(lace_csr* A) {
 }
 
 int lace_csr_n(const lace_csr* A) { return reinterpret_cast<const DevCSR*>(A)->n; }
+
+int lace_csr_forms(const lace_csr* Ah, int* ndiag, int* ell_width) {
+  LACE_REQUIRE(Ah && ndiag && ell_width, DEV_ERR_INVALID_PTR, "lace_csr_forms: null argument");
+  const DevCSR* A = reinterpret_cast<const DevCSR*>(Ah);
+  *ndiag = A->dia.nd;
+  *ell_width = A->ew;
+  return DEV_SUCCESS;
+}
 
 int lace_csr_spmv(const lace_csr* Ah, double alpha, const void* x, double beta, void* y) {
   LACE_TRY(ensure_init());

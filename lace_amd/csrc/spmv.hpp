@@ -2,6 +2,7 @@
 #pragma once
 
 #include "common.hpp"
+#include "dia.hpp"
 
 namespace lace {
 
@@ -15,6 +16,9 @@ struct DevCSR {
   // from val through emap by refresh_ell (call it after changing val).
   int ew = 0;
   DevBuf ecol, emap, eval;
+  // Diagonal form of a square matrix with at most 8 diagonals (stencils),
+  // refreshed with the ELLPACK copy.
+  DiaForm dia;
   int create(const data_d_matrix* A, lace_prec_t p);
   int refresh_ell();
   // Algorithmic bytes of one product: values+columns, row pointers, x, y

@@ -79,6 +79,8 @@ int TriPlan::build(int n_, const int* row_, const int* col_, bool upper_) {
     LACE_TRY(upload(ecol, ec.data(), sizeof(int) * ec.size()));
     LACE_TRY(upload(emap, em.data(), sizeof(int) * em.size()));
   }
+  // diagonal form of the same entries
+  LACE_TRY(dia.build(n, row_, col_, [&](int i, int k) { return k != dp[i]; }));
   strict_src = nullptr;
   return DEV_SUCCESS;
 }
@@ -124,6 +126,7 @@ int TriPlan::refresh_strict(const void* val, size_t es) {
           m2, emap.as<int>(), static_cast<const float*>(val), eval.as<float>());
     LACE_HIP(hipGetLastError());
   }
+  LACE_TRY(dia.gather(val, es));
   strict_src = val;
   return DEV_SUCCESS;
 }
@@ -424,6 +427,60 @@ static void jacobi_ell(const TriPlan& P, const T* r, const T* yin, T* yout) {
   }
 }
 
+// The same sweep on the diagonal form: thread per row, the row's value on
+// diagonal d read at d*n + i and y at i + off[d] (contiguous across the
+// wavefront; rows without an entry there read y[i] and skip it), summed in
+// ascending offset = CSR order, so results are bit-identical.
+template <typename T, bool UNIT, int D>
+__global__ __launch_bounds__(256) void tri_jacobi_dia_kernel(int n, DiaOffsets off,
+                                                             const uint8_t* __restrict__ mask,
+                                                             const T* __restrict__ dia,
+                                                             const T* __restrict__ dval,
+                                                             const T* __restrict__ r,
+                                                             const T* __restrict__ yin,
+                                                             T* __restrict__ yout) {
+#pragma clang fp contract(off)
+  const int i = xcd_block(blockIdx.x, gridDim.x) * 256 + (int)threadIdx.x;
+  if (i >= n) return;
+  const unsigned m = mask[i];
+  T v[D], y[D];
+#pragma unroll
+  for (int q = 0; q < D; q++) v[q] = dia[(size_t)q * n + i];
+#pragma unroll
+  for (int q = 0; q < D; q++) y[q] = yin[((m >> q) & 1u) ? i + off.o[q] : i];
+  const T ri = r[i];
+  T s = T(0);
+#pragma unroll
+  for (int q = 0; q < D; q++)
+    if ((m >> q) & 1u) s = s + v[q] * y[q];
+  yout[i] = UNIT ? ri - s : (ri - s) / dval[i];
+}
+
+template <typename T, bool UNIT>
+static void jacobi_dia(const TriPlan& P, const T* r, const T* yin, T* yout) {
+  const int nb = ceil_div(P.n, 256);
+  const uint8_t* mk = P.dia.mask.as<uint8_t>();
+  const T* dv = P.dia.val.as<T>();
+  const T* dg = P.dval.as<T>();
+  switch (P.dia.nd) {
+#define LACE_DIA_CASE(D)                                                                                         \
+  case D:                                                                                                        \
+    tri_jacobi_dia_kernel<T, UNIT, D><<<nb, 256, 0, stream()>>>(P.n, P.dia.off, mk, dv, dg, r, yin, yout); \
+    break;
+    LACE_DIA_CASE(1)
+    LACE_DIA_CASE(2)
+    LACE_DIA_CASE(3)
+    LACE_DIA_CASE(4)
+    LACE_DIA_CASE(5)
+    LACE_DIA_CASE(6)
+    LACE_DIA_CASE(7)
+    LACE_DIA_CASE(8)
+#undef LACE_DIA_CASE
+    default:
+      break;
+  }
+}
+
 // First sweep from zero: y = r / diag.
 template <typename T>
 __global__ void tri_diag_div_kernel(int n, const T* __restrict__ dval, const T* __restrict__ r,
@@ -513,9 +570,21 @@ static int jacobi_sweep(const TriPlan& P, const T* val, const T* r, const T* yin
   const int nb = ceil_div(P.n, 256);
   const bool strict = P.strict_src == (const void*)val;  // off-diagonal form is current
   if (!partial && yin && P.n > 0 && P.nnz <= 16LL * P.n) {
-    TimedRegion tr(P.upper ? "trsv_jacobi_sweep_upper" : "trsv_jacobi_sweep_lower");
-    const bool ell = strict && P.ew > 0 && get_option("jacobi_ell", 1) != 0;
-    if (ell && P.unit_diag) jacobi_ell<T, true>(P, r, yin, yout);
+    const bool dia = strict && P.dia.nd > 0 && get_option("jacobi_dia", 1) != 0;
+    const bool ell = !dia && strict && P.ew > 0 && get_option("jacobi_ell", 1) != 0;
+    // bytes the chosen form reads once each, plus r, the previous iterate
+    // and the new one (and the diagonal unless it is a unit one)
+    const double es = sizeof(T), n = P.n;
+    const double diag = P.unit_diag ? 0.0 : es * n;
+    double bytes = 3.0 * es * n;
+    if (dia) bytes += P.dia.bytes(sizeof(T)) + diag;
+    else if (ell) bytes += (double)P.ew * n * (es + 4.0) + diag;
+    else if (strict) bytes += (double)P.snnz * (es + 4.0) + 4.0 * (n + 1) + diag;
+    else bytes += (double)P.nnz * (es + 4.0) + 4.0 * (n + 1);
+    TimedRegion tr(P.upper ? "trsv_jacobi_sweep_upper" : "trsv_jacobi_sweep_lower", 0.0, bytes);
+    if (dia && P.unit_diag) jacobi_dia<T, true>(P, r, yin, yout);
+    else if (dia) jacobi_dia<T, false>(P, r, yin, yout);
+    else if (ell && P.unit_diag) jacobi_ell<T, true>(P, r, yin, yout);
     else if (ell) jacobi_ell<T, false>(P, r, yin, yout);
     else if (strict && P.unit_diag) jacobi_strict<T, true>(P, r, yin, yout);
     else if (strict) jacobi_strict<T, false>(P, r, yin, yout);
