@@ -569,8 +569,12 @@ int lace_block_panel_qr(int n, int r0, const double *W, double *Y, double *Tb, d
  * 1): Jacobi sweeps of ILU factors with short rows use the ELLPACK form of
  * the off-diagonal entries (0: the off-diagonal CSR-stream form).
  * "spmv_ell" (0 or 1, default 1): SpMV of a matrix with at most 8 entries
- * per row uses its ELLPACK copy (0: the CSR-stream kernel).  Every choice
- * gives bit-identical results.  "matlab_log" (0 or 1, default 1): the
+ * per row uses its ELLPACK copy (0: the CSR-stream kernel).  "jacobi_dia",
+ * "spmv_dia" (0 or 1, default 1): the same with the diagonal form, which
+ * takes precedence over ELLPACK where both apply.  "jacobi_fuse" (0 or 1,
+ * default 1): with an odd sweep count, the first Jacobi sweep of U rides on
+ * the last sweep of L (diagonal form only).  Every choice gives
+ * bit-identical results.  "matlab_log" (0 or 1, default 1): the
  * drop-in FGMRES entry points print the reference's MATLAB-syntax progress
  * lines to stdout (src/fgmres.cpp:71,178,430; src/fgmres_householder_restart.cpp:
  * 70,81,196,231,707) -- "FGMRES_Householders_partrsv_search(j) = |g_j|;" etc.,

@@ -56,8 +56,15 @@ int tri_solve_levels(const TriPlan& P, const T* val, const T* r, T* y);
 template <typename T>
 int tri_solve_syncfree(const TriPlan& P, const T* val, const T* r, T* y, int* err);
 // `sweeps` synchronous Jacobi sweeps from y = 0; tmp is a scratch vector.
+// epi_y / epi_d: the last sweep may also write epi_y = y / epi_d (the first
+// sweep of a following solve with diagonal epi_d) and then sets *epi_done;
+// it does so only on the diagonal form, and epi_y must not be r, y or tmp.
+// first_done: the first sweep's result r / diag is already in the buffer
+// the first sweep would write (y for odd sweeps, tmp for even).
 template <typename T>
-int tri_solve_jacobi(const TriPlan& P, const T* val, const T* r, T* y, T* tmp, int sweeps);
+int tri_solve_jacobi(const TriPlan& P, const T* val, const T* r, T* y, T* tmp, int sweeps,
+                     T* epi_y = nullptr, const T* epi_d = nullptr, bool* epi_done = nullptr,
+                     bool first_done = false);
 
 // Device ParILU factors (see parilu.hip for the layout).
 struct ParILU {

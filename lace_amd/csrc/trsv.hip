@@ -430,15 +430,18 @@ static void jacobi_ell(const TriPlan& P, const T* r, const T* yin, T* yout) {
 // The same sweep on the diagonal form: thread per row, the row's value on
 // diagonal d read at d*n + i and y at i + off[d] (contiguous across the
 // wavefront; rows without an entry there read y[i] and skip it), summed in
-// ascending offset = CSR order, so results are bit-identical.
-template <typename T, bool UNIT, int D>
+// ascending offset = CSR order, so results are bit-identical.  EPI: the
+// sweep also writes y2 = yout / d2, the first Jacobi sweep (from zero) of
+// the next triangular solve, whose diagonal is d2.
+template <typename T, bool UNIT, int D, bool EPI>
 __global__ __launch_bounds__(256) void tri_jacobi_dia_kernel(int n, DiaOffsets off,
                                                              const uint8_t* __restrict__ mask,
                                                              const T* __restrict__ dia,
                                                              const T* __restrict__ dval,
                                                              const T* __restrict__ r,
                                                              const T* __restrict__ yin,
-                                                             T* __restrict__ yout) {
+                                                             T* __restrict__ yout, T* __restrict__ y2,
+                                                             const T* __restrict__ d2) {
 #pragma clang fp contract(off)
   const int i = xcd_block(blockIdx.x, gridDim.x) * 256 + (int)threadIdx.x;
   if (i >= n) return;
@@ -453,19 +456,26 @@ __global__ __launch_bounds__(256) void tri_jacobi_dia_kernel(int n, DiaOffsets o
 #pragma unroll
   for (int q = 0; q < D; q++)
     if ((m >> q) & 1u) s = s + v[q] * y[q];
-  yout[i] = UNIT ? ri - s : (ri - s) / dval[i];
+  const T o = UNIT ? ri - s : (ri - s) / dval[i];
+  yout[i] = o;
+  if (EPI) y2[i] = o / d2[i];
 }
 
 template <typename T, bool UNIT>
-static void jacobi_dia(const TriPlan& P, const T* r, const T* yin, T* yout) {
+static void jacobi_dia(const TriPlan& P, const T* r, const T* yin, T* yout, T* y2, const T* d2) {
   const int nb = ceil_div(P.n, 256);
   const uint8_t* mk = P.dia.mask.as<uint8_t>();
   const T* dv = P.dia.val.as<T>();
   const T* dg = P.dval.as<T>();
   switch (P.dia.nd) {
-#define LACE_DIA_CASE(D)                                                                                         \
-  case D:                                                                                                        \
-    tri_jacobi_dia_kernel<T, UNIT, D><<<nb, 256, 0, stream()>>>(P.n, P.dia.off, mk, dv, dg, r, yin, yout); \
+#define LACE_DIA_CASE(D)                                                                                   \
+  case D:                                                                                                  \
+    if (y2)                                                                                                \
+      tri_jacobi_dia_kernel<T, UNIT, D, true>                                                              \
+          <<<nb, 256, 0, stream()>>>(P.n, P.dia.off, mk, dv, dg, r, yin, yout, y2, d2);                    \
+    else                                                                                                   \
+      tri_jacobi_dia_kernel<T, UNIT, D, false>                                                             \
+          <<<nb, 256, 0, stream()>>>(P.n, P.dia.off, mk, dv, dg, r, yin, yout, nullptr, nullptr);          \
     break;
     LACE_DIA_CASE(1)
     LACE_DIA_CASE(2)
@@ -564,26 +574,29 @@ int tri_solve_syncfree(const TriPlan& P, const T* val, const T* r, T* y, int* er
   return DEV_SUCCESS;
 }
 
+// One sweep yout = D^{-1}(r - N yin).  With y2 and d2 given, the sweep on
+// the diagonal form also writes y2 = yout / d2 and sets *epi.
 template <typename T>
 static int jacobi_sweep(const TriPlan& P, const T* val, const T* r, const T* yin, T* yout,
-                        double* partial) {
+                        double* partial, T* y2 = nullptr, const T* d2 = nullptr, bool* epi = nullptr) {
   const int nb = ceil_div(P.n, 256);
   const bool strict = P.strict_src == (const void*)val;  // off-diagonal form is current
   if (!partial && yin && P.n > 0 && P.nnz <= 16LL * P.n) {
     const bool dia = strict && P.dia.nd > 0 && get_option("jacobi_dia", 1) != 0;
     const bool ell = !dia && strict && P.ew > 0 && get_option("jacobi_ell", 1) != 0;
+    const bool fuse = dia && y2 && d2;
     // bytes the chosen form reads once each, plus r, the previous iterate
     // and the new one (and the diagonal unless it is a unit one)
     const double es = sizeof(T), n = P.n;
     const double diag = P.unit_diag ? 0.0 : es * n;
-    double bytes = 3.0 * es * n;
+    double bytes = 3.0 * es * n + (fuse ? 2.0 * es * n : 0.0);
     if (dia) bytes += P.dia.bytes(sizeof(T)) + diag;
     else if (ell) bytes += (double)P.ew * n * (es + 4.0) + diag;
     else if (strict) bytes += (double)P.snnz * (es + 4.0) + 4.0 * (n + 1) + diag;
     else bytes += (double)P.nnz * (es + 4.0) + 4.0 * (n + 1);
     TimedRegion tr(P.upper ? "trsv_jacobi_sweep_upper" : "trsv_jacobi_sweep_lower", 0.0, bytes);
-    if (dia && P.unit_diag) jacobi_dia<T, true>(P, r, yin, yout);
-    else if (dia) jacobi_dia<T, false>(P, r, yin, yout);
+    if (dia && P.unit_diag) jacobi_dia<T, true>(P, r, yin, yout, fuse ? y2 : nullptr, d2);
+    else if (dia) jacobi_dia<T, false>(P, r, yin, yout, fuse ? y2 : nullptr, d2);
     else if (ell && P.unit_diag) jacobi_ell<T, true>(P, r, yin, yout);
     else if (ell) jacobi_ell<T, false>(P, r, yin, yout);
     else if (strict && P.unit_diag) jacobi_strict<T, true>(P, r, yin, yout);
@@ -591,6 +604,7 @@ static int jacobi_sweep(const TriPlan& P, const T* val, const T* r, const T* yin
     else if (P.upper) jacobi_stream<T, true>(P, val, r, yin, yout);
     else jacobi_stream<T, false>(P, val, r, yin, yout);
     LACE_HIP(hipGetLastError());
+    if (epi) *epi = fuse;
     return DEV_SUCCESS;
   }
   if (!partial && !yin && strict && P.n > 0) {
@@ -618,8 +632,10 @@ static int jacobi_sweep(const TriPlan& P, const T* val, const T* r, const T* yin
 }
 
 template <typename T>
-int tri_solve_jacobi(const TriPlan& P, const T* val, const T* r, T* y, T* tmp, int sweeps) {
+int tri_solve_jacobi(const TriPlan& P, const T* val, const T* r, T* y, T* tmp, int sweeps, T* epi_y,
+                     const T* epi_d, bool* epi_done, bool first_done) {
   TimedRegion tr("trsv_jacobi");
+  if (epi_done) *epi_done = false;
   if (sweeps <= 0) {
     LACE_HIP(hipMemsetAsync(y, 0, sizeof(T) * P.n, stream()));
     return DEV_SUCCESS;
@@ -634,17 +650,23 @@ int tri_solve_jacobi(const TriPlan& P, const T* val, const T* r, T* y, T* tmp, i
     const int rest = sweeps - 1;
     T* a = (rest % 2 == 1) ? y : tmp;
     T* b = (rest % 2 == 1) ? tmp : y;
-    LACE_TRY(jacobi_sweep<T>(P, val, r, r, a, nullptr));
+    // the last sweep may carry the next solve's first sweep
+    auto sweep = [&](const T* yin, T* yout, bool last) {
+      return last ? jacobi_sweep<T>(P, val, r, yin, yout, nullptr, epi_y, epi_d, epi_done)
+                  : jacobi_sweep<T>(P, val, r, yin, yout, nullptr);
+    };
+    LACE_TRY(sweep(r, a, rest == 1));
     for (int s = 1; s < rest; s++) {
-      LACE_TRY(jacobi_sweep<T>(P, val, r, a, b, nullptr));
+      LACE_TRY(sweep(a, b, s == rest - 1));
       std::swap(a, b);
     }
     return DEV_SUCCESS;
   }
-  // ping-pong so that the last sweep lands in y
+  // ping-pong so that the last sweep lands in y; with first_done the first
+  // sweep's result (r / diag) is already in its buffer
   T* a = (sweeps % 2 == 1) ? y : tmp;
   T* b = (sweeps % 2 == 1) ? tmp : y;
-  LACE_TRY(jacobi_sweep<T>(P, val, r, nullptr, a, nullptr));
+  if (!first_done) LACE_TRY(jacobi_sweep<T>(P, val, r, nullptr, a, nullptr));
   for (int s = 1; s < sweeps; s++) {
     LACE_TRY(jacobi_sweep<T>(P, val, r, a, b, nullptr));
     std::swap(a, b);
@@ -656,8 +678,10 @@ template int tri_solve_levels<double>(const TriPlan&, const double*, const doubl
 template int tri_solve_levels<float>(const TriPlan&, const float*, const float*, float*);
 template int tri_solve_syncfree<double>(const TriPlan&, const double*, const double*, double*, int*);
 template int tri_solve_syncfree<float>(const TriPlan&, const float*, const float*, float*, int*);
-template int tri_solve_jacobi<double>(const TriPlan&, const double*, const double*, double*, double*, int);
-template int tri_solve_jacobi<float>(const TriPlan&, const float*, const float*, float*, float*, int);
+template int tri_solve_jacobi<double>(const TriPlan&, const double*, const double*, double*, double*, int, double*,
+                                       const double*, bool*, bool);
+template int tri_solve_jacobi<float>(const TriPlan&, const float*, const float*, float*, float*, int, float*,
+                                      const float*, bool*, bool);
 
 // ---------------------------------------------------------------- ParILU solves
 int ParILU::build_trsv() {
@@ -749,9 +773,17 @@ static int apply_t(ParILU* P, lace_trsv_kind_t kind, int sweeps, int trans, cons
     return DEV_SUCCESS;
   }
   // Jacobi: work holds the intermediate vector, work2 the ping-pong buffer.
+  // For an odd sweep count the second solve's first sweep (tmp / diag,
+  // written to z) rides on the first solve's last sweep when that runs on
+  // the diagonal form; z is then distinct from every buffer that sweep reads.
   T* tmp2 = P->work2.as<T>();
-  LACE_TRY(tri_solve_jacobi<T>(first, v1, r, tmp, tmp2, sweeps));
-  LACE_TRY(tri_solve_jacobi<T>(second, v2, tmp, z, tmp2, sweeps));
+  const bool fuse = first.unit_diag && !second.unit_diag && sweeps % 2 == 1 &&
+                    second.strict_src == (const void*)v2 && (const void*)z != (const void*)r &&
+                    get_option("jacobi_fuse", 1) != 0;
+  bool fused = false;
+  LACE_TRY(tri_solve_jacobi<T>(first, v1, r, tmp, tmp2, sweeps, fuse ? z : nullptr,
+                               fuse ? second.dval.as<T>() : nullptr, &fused));
+  LACE_TRY(tri_solve_jacobi<T>(second, v2, tmp, z, tmp2, sweeps, nullptr, nullptr, nullptr, fused));
   return DEV_SUCCESS;
 }
 

@@ -22,6 +22,7 @@ def forms(gpu):
     yield set_forms
     for kind in ("spmv", "jacobi"):
         set_forms(kind, "dia")
+    gpu.check(gpu.lib().lace_set_option(b"jacobi_fuse", 1))
 
 
 def _holes(orc, A, frac, seed):
@@ -78,12 +79,16 @@ def test_jacobi_forms_bitexact(gpu, orc, forms, name, prec):
     Lf = orc.CSR(A.n_rows, A.n_cols, st.L.row, st.L.col, st.Lval)
     Ur = orc.transpose(orc.CSR(A.n_rows, A.n_cols, st.U.row, st.U.col, st.Uval))
     r = np.random.default_rng(11).standard_normal(A.n_rows).astype(dt)
-    y = orc.trsv_jacobi(Lf, r, upper=False, sweeps=5, dtype=dt)
-    ref = orc.trsv_jacobi(Ur, y, upper=True, sweeps=5, dtype=dt)
-    for form in FORMS:
-        forms("jacobi", form)
-        got = P.apply(r, gpu.TRSV_JACOBI, sweeps=5)
-        assert np.array_equal(got, ref), form
+    # odd sweep counts let U's first sweep ride on L's last (diagonal form)
+    for sweeps in (5, 4, 3, 2):
+        y = orc.trsv_jacobi(Lf, r, upper=False, sweeps=sweeps, dtype=dt)
+        ref = orc.trsv_jacobi(Ur, y, upper=True, sweeps=sweeps, dtype=dt)
+        for form in FORMS:
+            forms("jacobi", form)
+            for fuse in (1, 0):
+                gpu.check(gpu.lib().lace_set_option(b"jacobi_fuse", fuse))
+                got = P.apply(r, gpu.TRSV_JACOBI, sweeps=sweeps)
+                assert np.array_equal(got, ref), (form, fuse, sweeps)
 
 
 def _csr_order_sum(A, x, dt):
