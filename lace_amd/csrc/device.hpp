@@ -75,6 +75,39 @@ __device__ __forceinline__ void vstore(T* __restrict__ p, const T (&v)[W]) {
   }
 }
 
+// W consecutive elements as one word of W * sizeof(T) = 4, 8 or 16 bytes
+// (p aligned to that size).
+template <typename T, int W>
+__device__ __forceinline__ void wload(const T* __restrict__ p, T (&v)[W]) {
+  constexpr int B = W * (int)sizeof(T);
+  static_assert(B == 4 || B == 8 || B == 16, "word of 4, 8 or 16 bytes");
+  if constexpr (B == 16) {
+    vload<T, W>(p, v);
+  } else if constexpr (B == 8) {
+    const uint2 u = *reinterpret_cast<const uint2*>(p);
+    __builtin_memcpy(&v[0], &u, 8);
+  } else {
+    const unsigned u = *reinterpret_cast<const unsigned*>(p);
+    __builtin_memcpy(&v[0], &u, 4);
+  }
+}
+template <typename T, int W>
+__device__ __forceinline__ void wstore(T* __restrict__ p, const T (&v)[W]) {
+  constexpr int B = W * (int)sizeof(T);
+  static_assert(B == 4 || B == 8 || B == 16, "word of 4, 8 or 16 bytes");
+  if constexpr (B == 16) {
+    vstore<T, W>(p, v);
+  } else if constexpr (B == 8) {
+    uint2 u;
+    __builtin_memcpy(&u, &v[0], 8);
+    *reinterpret_cast<uint2*>(p) = u;
+  } else {
+    unsigned u;
+    __builtin_memcpy(&u, &v[0], 4);
+    *reinterpret_cast<unsigned*>(p) = u;
+  }
+}
+
 // CSR-stream staging: prod[k - c0] = val[k] * x[col[k]] for k in
 // [c0, c0 + cn), by NT threads.  Each thread takes aligned groups of four
 // entries (16-byte column and value loads; col and val must start on a

@@ -115,6 +115,74 @@ __global__ __launch_bounds__(kBS) void mdot_kernel(int n, int ld, const K* __res
   }
 }
 
+// The same multi-dot with the slab's rows split over the block's threads
+// and every column in each thread's registers (NC <= 16): thread t takes
+// the groups of V = 16 / sizeof(K) consecutive rows t, t + 256, ..., and
+// loads each column's V rows and w's as one 16-byte word, so a wave keeps
+// ncols + 1 kilobyte-sized requests in flight.  U, w and ld must be 16-byte
+// aligned (the engine's buffers are); groups cut by rowstart or the slab
+// end load element by element.
+template <typename K, int NC, bool SELF>
+__global__ __launch_bounds__(kBS) void mdot_rows_kernel(int n, int ld, const K* __restrict__ U, int ncols,
+                                                        const K* __restrict__ w, int rowstart, int rpb,
+                                                        double* __restrict__ partial, int pstride,
+                                                        const double* __restrict__ misc) {
+  if (misc && is_done(misc)) return;
+  constexpr int V = 16 / (int)sizeof(K);
+  const int b = xcd_block(blockIdx.x, gridDim.x);
+  const int r0 = max(b * rpb, rowstart), r1 = min((b + 1) * rpb, n);
+  K acc[NC];
+  K self = K(0);
+#pragma unroll
+  for (int k = 0; k < NC; k++) acc[k] = K(0);
+  if (r0 < r1) {
+    const int g1 = (r1 + V - 1) / V;
+    for (int g = r0 / V + (int)threadIdx.x; g < g1; g += kBS) {
+      const int i0 = g * V;
+      const bool full = i0 >= r0 && i0 + V <= r1;
+      K wv[V];
+      if (full) {
+        vload<K, V>(w + i0, wv);
+      } else {
+#pragma unroll
+        for (int e = 0; e < V; e++) wv[e] = (i0 + e >= r0 && i0 + e < r1) ? w[i0 + e] : K(0);
+      }
+#pragma unroll
+      for (int k = 0; k < NC; k++) {
+        if (k < ncols) {
+          const K* col = U + (size_t)k * ld;
+          K uv[V];
+          if (full) {
+            vload<K, V>(col + i0, uv);
+          } else {
+#pragma unroll
+            for (int e = 0; e < V; e++) uv[e] = (i0 + e >= r0 && i0 + e < r1) ? col[i0 + e] : K(0);
+          }
+#pragma unroll
+          for (int e = 0; e < V; e++) acc[k] += uv[e] * wv[e];
+        }
+      }
+      if (SELF) {
+#pragma unroll
+        for (int e = 0; e < V; e++) self += wv[e] * wv[e];
+      }
+    }
+  }
+  __shared__ double red[kBS / 64][NC + 1];
+  const int wave = threadIdx.x >> 6;
+  wave_multi_sum<NC>(acc, red[wave]);
+  if (SELF) {
+    const double sv = wave_sum((double)self);
+    if ((threadIdx.x & 63) == 0) red[wave][NC] = sv;
+  }
+  __syncthreads();
+  for (int k = threadIdx.x; k <= NC; k += kBS) {
+    if (k < ncols) partial[(size_t)b * pstride + k] = (red[0][k] + red[1][k]) + (red[2][k] + red[3][k]);
+    if (SELF && k == NC)
+      partial[(size_t)b * pstride + pstride - 1] = (red[0][NC] + red[1][NC]) + (red[2][NC] + red[3][NC]);
+  }
+}
+
 // Sum of the per-block partials in a fixed order: out[k] for k < cnt and,
 // if self, out[cnt] = the self slot.  One block of 256 threads.  Thread t
 // owns the partial rows t, t+256, ... and accumulates 16 columns at a time,
@@ -264,18 +332,51 @@ __global__ __launch_bounds__(kBS) void hh_update_mdot_kernel(int n, int ld, cons
     acc[k] = K(0);
   }
   K self = K(0);
-  for (int i = r0 + threadIdx.x; i < r1; i += kBS) {
-    K u[NC];
+  // groups of V consecutive rows: 16-byte words up to 8 columns, single
+  // rows beyond (a thread holds NC * V reflector entries for both uses;
+  // measured at 2M rows, FP32: 16 columns take 24 us with single rows,
+  // 26 us with 8-byte and 29 us with 16-byte words, which cost occupancy);
+  // slabs start on multiples of 64 rows, and a group cut by the end of the
+  // matrix goes element by element
+  constexpr int V = NC <= 8 ? 16 / (int)sizeof(K) : 1;
+  for (int i0 = r0 + (int)threadIdx.x * V; i0 < r1; i0 += kBS * V) {
+    const bool full = i0 + V <= r1;
+    K v[V];
+    if (full) {
+      wload<K, V>(w + i0, v);
+    } else {
 #pragma unroll
-    for (int k = 0; k < NC; k++) u[k] = k < ncols ? U[(size_t)k * ld + i] : K(0);
-    K v = w[i];
+      for (int e = 0; e < V; e++) v[e] = i0 + e < r1 ? w[i0 + e] : K(0);
+    }
+    K u[NC][V];
 #pragma unroll
-    for (int k = 0; k < NC; k++) v -= u[k] * dk[k];
-    w[i] = v;
-    if (i >= rowstart) {
+    for (int k = 0; k < NC; k++) {
+      if (k < ncols && full) {
+        wload<K, V>(U + (size_t)k * ld + i0, u[k]);
+      } else {
 #pragma unroll
-      for (int k = 0; k < NC; k++) acc[k] += u[k] * v;
-      self += v * v;
+        for (int e = 0; e < V; e++) u[k][e] = (k < ncols && i0 + e < r1) ? U[(size_t)k * ld + i0 + e] : K(0);
+      }
+    }
+#pragma unroll
+    for (int e = 0; e < V; e++) {
+#pragma unroll
+      for (int k = 0; k < NC; k++) v[e] -= u[k][e] * dk[k];
+    }
+    if (full) {
+      wstore<K, V>(w + i0, v);
+    } else {
+#pragma unroll
+      for (int e = 0; e < V; e++)
+        if (i0 + e < r1) w[i0 + e] = v[e];
+    }
+#pragma unroll
+    for (int e = 0; e < V; e++) {
+      if (i0 + e >= rowstart && i0 + e < r1) {
+#pragma unroll
+        for (int k = 0; k < NC; k++) acc[k] += u[k][e] * v[e];
+        self += v[e] * v[e];
+      }
     }
   }
   __shared__ double red[kBS / 64][NC + 1];
@@ -570,8 +671,15 @@ int hh_step(HHEngine<K>& E, int j, bool write_q) {
   double* part = E.partial.template as<double>();
   const double* misc = small + E.lay.misc();
   const int nbv = ceil_div(E.n, kBS);
-  LACE_NC_DISPATCH(nc, (mdot_kernel<K, NCB, false><<<E.nb, kBS, 0, stream()>>>(
-                           E.n, E.ld, U, nc, w, 0, E.rpb, part, E.pstride, misc)));
+  if (nc <= 8)
+    mdot_rows_kernel<K, 8, false><<<E.nb, kBS, 0, stream()>>>(E.n, E.ld, U, nc, w, 0, E.rpb, part,
+                                                            E.pstride, misc);
+  else if (nc <= 16)
+    mdot_rows_kernel<K, 16, false><<<E.nb, kBS, 0, stream()>>>(E.n, E.ld, U, nc, w, 0, E.rpb, part,
+                                                             E.pstride, misc);
+  else
+    LACE_NC_DISPATCH(nc, (mdot_kernel<K, NCB, false><<<E.nb, kBS, 0, stream()>>>(
+                             E.n, E.ld, U, nc, w, 0, E.rpb, part, E.pstride, misc)));
   hh_dT_kernel<<<1, 256, 0, stream()>>>(E.nb, E.pstride, part, j, small, E.lay);
   // fused update + tail multi-dot while the per-thread reflector row fits
   // in registers (up to 16 columns; 32 in FP32)

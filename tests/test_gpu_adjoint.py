@@ -52,9 +52,9 @@ def test_transposed_mixed_precision(gpu, orc, mixed):
     assert np.linalg.norm(g - A.scipy().T @ lam) <= 1.05e-10 * np.linalg.norm(g)
 
 
-def _solve(S, A_vals, b):
+def _solve(S, A_vals, b, x0=None):
     S.set_values(A_vals)
-    x, st = S.solve(b)
+    x, st = S.solve(b, x0=x0)
     assert st.converged == 1
     return x
 
@@ -69,7 +69,11 @@ def _central_difference(xp, xm, xt, h):
 @pytest.mark.parametrize("size", [12, 64])
 def test_gradient_vs_central_differences(gpu, orc, size):
     """Config 4 at 64^3: x_t ~ N(0,1) seed 3, b ~ N(0,1) seed 4, directions
-    seeds 100-119 (the 12^3 case is a quick check of the same code)."""
+    seeds 100-119 (the 12^3 case is a quick check of the same code).  The
+    perturbed solves start from the unperturbed solution x, so their
+    relative tolerance applies to the O(h) correction: the difference
+    quotient then carries no solve error of the size of x (rtol * |x| / h
+    would be 1e-7 relative here, at the level of the 1e-6 bar)."""
     import lace_amd as L
     row, col, val = L.gen_convdiff3d(size, size, size)
     A = orc.CSR(len(row) - 1, len(row) - 1, row, col, val)
@@ -90,14 +94,14 @@ def test_gradient_vs_central_differences(gpu, orc, size):
     ndir = 20 if size == 64 else 6
     for seed in range(100, 100 + ndir):
         d = L.gen_normal(seed, nnz)
-        fd = _central_difference(_solve(S, val + h * d, b), _solve(S, val - h * d, b), xt, h)
+        fd = _central_difference(_solve(S, val + h * d, b, x), _solve(S, val - h * d, b, x), xt, h)
         an = float(np.dot(g, d))
         assert abs(fd - an) <= 1e-6 * abs(an), (seed, fd, an)
     # dL/db = lambda: directional check in b
     S.set_values(val)
     for seed in (200, 201):
         e = L.gen_normal(seed, n)
-        fd = _central_difference(_solve(S, val, b + h * e), _solve(S, val, b - h * e), xt, h)
+        fd = _central_difference(_solve(S, val, b + h * e, x), _solve(S, val, b - h * e, x), xt, h)
         an = float(np.dot(lam, e))
         assert abs(fd - an) <= 1e-6 * abs(an)
 
