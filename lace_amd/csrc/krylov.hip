@@ -115,72 +115,102 @@ __global__ __launch_bounds__(kBS) void mdot_kernel(int n, int ld, const K* __res
   }
 }
 
-// The same multi-dot with the slab's rows split over the block's threads
-// and every column in each thread's registers (NC <= 16): thread t takes
-// the groups of V = 16 / sizeof(K) consecutive rows t, t + 256, ..., and
-// loads each column's V rows and w's as one 16-byte word, so a wave keeps
-// ncols + 1 kilobyte-sized requests in flight.  U, w and ld must be 16-byte
-// aligned (the engine's buffers are); groups cut by rowstart or the slab
-// end load element by element.
+// The same multi-dot with the slab's rows split over the block's threads:
+// thread t takes the groups of V = 16 / sizeof(K) consecutive rows t,
+// t + 256, ... and loads each column's V rows and w's as one 16-byte word,
+// so a wave keeps a kilobyte-sized request per column in flight.  The
+// columns go in chunks of NC held in registers (w's slab is read again per
+// chunk, from cache).  U, w and ld must be 16-byte aligned (the engine's
+// buffers are); groups cut by rowstart or the slab end load element by
+// element.
 template <typename K, int NC, bool SELF>
-__global__ __launch_bounds__(kBS) void mdot_rows_kernel(int n, int ld, const K* __restrict__ U, int ncols,
+__global__ __launch_bounds__(kBS, 4) void mdot_rows_kernel(int n, int ld, const K* __restrict__ U, int ncols,
                                                         const K* __restrict__ w, int rowstart, int rpb,
                                                         double* __restrict__ partial, int pstride,
                                                         const double* __restrict__ misc) {
   if (misc && is_done(misc)) return;
-  constexpr int V = 16 / (int)sizeof(K);
+  // FP64 chunks of 16 columns take single rows: two rows each would hold
+  // 32 doubles of U per thread and halve the occupancy
+  constexpr int V = (sizeof(K) == 8 && NC > 8) ? 1 : 16 / (int)sizeof(K);
   const int b = xcd_block(blockIdx.x, gridDim.x);
   const int r0 = max(b * rpb, rowstart), r1 = min((b + 1) * rpb, n);
-  K acc[NC];
-  K self = K(0);
+  const int wave = threadIdx.x >> 6;
+  __shared__ double red[kBS / 64][NC + 1];
+  // at least one pass, for the SELF sum when there are no columns
+  for (int c0 = 0; c0 < ncols || (c0 == 0 && SELF); c0 += NC) {
+    const int nck = min(NC, ncols - c0);
+    const bool self_pass = SELF && c0 == 0;
+    K acc[NC];
+    K self = K(0);
 #pragma unroll
-  for (int k = 0; k < NC; k++) acc[k] = K(0);
-  if (r0 < r1) {
-    const int g1 = (r1 + V - 1) / V;
-    for (int g = r0 / V + (int)threadIdx.x; g < g1; g += kBS) {
-      const int i0 = g * V;
-      const bool full = i0 >= r0 && i0 + V <= r1;
-      K wv[V];
-      if (full) {
-        vload<K, V>(w + i0, wv);
-      } else {
+    for (int k = 0; k < NC; k++) acc[k] = K(0);
+    if (r0 < r1) {
+      const int g1 = (r1 + V - 1) / V;
+      for (int g = r0 / V + (int)threadIdx.x; g < g1; g += kBS) {
+        const int i0 = g * V;
+        const bool full = i0 >= r0 && i0 + V <= r1;
+        K wv[V];
+        if (full) {
+          wload<K, V>(w + i0, wv);
+        } else {
 #pragma unroll
-        for (int e = 0; e < V; e++) wv[e] = (i0 + e >= r0 && i0 + e < r1) ? w[i0 + e] : K(0);
-      }
+          for (int e = 0; e < V; e++) wv[e] = (i0 + e >= r0 && i0 + e < r1) ? w[i0 + e] : K(0);
+        }
 #pragma unroll
-      for (int k = 0; k < NC; k++) {
-        if (k < ncols) {
-          const K* col = U + (size_t)k * ld;
-          K uv[V];
-          if (full) {
-            vload<K, V>(col + i0, uv);
-          } else {
+        for (int k = 0; k < NC; k++) {
+          if (k < nck) {
+            const K* col = U + (size_t)(c0 + k) * ld;
+            K uv[V];
+            if (full) {
+              wload<K, V>(col + i0, uv);
+            } else {
 #pragma unroll
-            for (int e = 0; e < V; e++) uv[e] = (i0 + e >= r0 && i0 + e < r1) ? col[i0 + e] : K(0);
+              for (int e = 0; e < V; e++) uv[e] = (i0 + e >= r0 && i0 + e < r1) ? col[i0 + e] : K(0);
+            }
+#pragma unroll
+            for (int e = 0; e < V; e++) acc[k] += uv[e] * wv[e];
           }
+        }
+        if (self_pass) {
 #pragma unroll
-          for (int e = 0; e < V; e++) acc[k] += uv[e] * wv[e];
+          for (int e = 0; e < V; e++) self += wv[e] * wv[e];
         }
       }
-      if (SELF) {
-#pragma unroll
-        for (int e = 0; e < V; e++) self += wv[e] * wv[e];
-      }
     }
+    wave_multi_sum<NC>(acc, red[wave]);
+    if (self_pass) {
+      const double sv = wave_sum((double)self);
+      if ((threadIdx.x & 63) == 0) red[wave][NC] = sv;
+    }
+    __syncthreads();
+    for (int k = threadIdx.x; k <= NC; k += kBS) {
+      if (k < nck)
+        partial[(size_t)b * pstride + c0 + k] = (red[0][k] + red[1][k]) + (red[2][k] + red[3][k]);
+      if (self_pass && k == NC)
+        partial[(size_t)b * pstride + pstride - 1] = (red[0][NC] + red[1][NC]) + (red[2][NC] + red[3][NC]);
+    }
+    __syncthreads();
   }
-  __shared__ double red[kBS / 64][NC + 1];
-  const int wave = threadIdx.x >> 6;
-  wave_multi_sum<NC>(acc, red[wave]);
-  if (SELF) {
-    const double sv = wave_sum((double)self);
-    if ((threadIdx.x & 63) == 0) red[wave][NC] = sv;
-  }
-  __syncthreads();
-  for (int k = threadIdx.x; k <= NC; k += kBS) {
-    if (k < ncols) partial[(size_t)b * pstride + k] = (red[0][k] + red[1][k]) + (red[2][k] + red[3][k]);
-    if (SELF && k == NC)
-      partial[(size_t)b * pstride + pstride - 1] = (red[0][NC] + red[1][NC]) + (red[2][NC] + red[3][NC]);
-  }
+}
+
+// Column totals of the per-block partials when there are many columns: one
+// workgroup per column c < cnt (and one more for the self slot when self),
+// each summing partial[b][c] over the blocks in a fixed order.  The totals
+// land in tot with the partial row layout (self at pstride - 1), so the
+// one-workgroup kernels read them as the partials of a single block instead
+// of reducing nb rows of many columns on one workgroup.
+__global__ __launch_bounds__(256) void reduce_cols_kernel(int nb, int cnt, int self, int pstride,
+                                                          const double* __restrict__ partial,
+                                                          double* __restrict__ tot,
+                                                          const double* __restrict__ misc) {
+  if (misc && is_done(misc)) return;
+  const int c = blockIdx.x;
+  const int col = (self && c == cnt) ? pstride - 1 : c;
+  double s = 0.0;
+  for (int b = threadIdx.x; b < nb; b += 256) s += partial[(size_t)b * pstride + col];
+  __shared__ double sm[256 / 64];
+  const double t = block_sum<256>(s, sm);
+  if (threadIdx.x == 0) tot[col] = t;
 }
 
 // Sum of the per-block partials in a fixed order: out[k] for k < cnt and,
@@ -604,6 +634,7 @@ int HHEngine<K>::init(int n_, int m_, bool z_float_) {
   LACE_TRY(r.alloc(sizeof(K) * ld));
   LACE_TRY(small.alloc(sizeof(double) * lay.total()));
   LACE_TRY(partial.alloc(sizeof(double) * (size_t)nb * pstride));
+  LACE_TRY(ptot.alloc(sizeof(double) * pstride));
   LACE_HIP(hipMemsetAsync(small.p, 0, sizeof(double) * lay.total(), stream()));
   LACE_HIP(hipMemsetAsync(U.p, 0, sizeof(K) * (size_t)ld * (m + 1), stream()));
   LACE_HIP(hipHostMalloc((void**)&host_misc, sizeof(double) * 32, hipHostMallocDefault));
@@ -646,10 +677,10 @@ template struct HHEngine<float>;
 template <typename K>
 int hh_start(HHEngine<K>& E, int set_rtol, double rtol_rel) {
   TimedRegion tr("hh_start");
-  mdot_kernel<K, 8, true><<<E.nb, kBS, 0, stream()>>>(E.n, E.ld, E.U.template as<K>(), 0,
-                                                     E.r.template as<K>(), 0, E.rpb,
-                                                     E.partial.template as<double>(), E.pstride,
-                                                     nullptr);
+  mdot_rows_kernel<K, 8, true><<<E.nb, kBS, 0, stream()>>>(E.n, E.ld, E.U.template as<K>(), 0,
+                                                          E.r.template as<K>(), 0, E.rpb,
+                                                          E.partial.template as<double>(), E.pstride,
+                                                          nullptr);
   hh_start_kernel<K><<<1, 256, 0, stream()>>>(E.nb, E.pstride, E.partial.template as<double>(),
                                               E.r.template as<K>(), E.small.template as<double>(),
                                               E.lay, set_rtol, rtol_rel);
@@ -660,7 +691,9 @@ int hh_start(HHEngine<K>& E, int set_rtol, double rtol_rel) {
   return DEV_SUCCESS;
 }
 
-// Householder part of step j (after w = A z_j).
+// Householder part of step j (after w = A z_j).  Beyond 16 columns the
+// per-block partials are first reduced column by column on many
+// workgroups, and the one-workgroup kernels read the totals.
 template <typename K>
 int hh_step(HHEngine<K>& E, int j, bool write_q) {
   TimedRegion tr("householder");
@@ -669,18 +702,18 @@ int hh_step(HHEngine<K>& E, int j, bool write_q) {
   K* w = E.w.template as<K>();
   double* small = E.small.template as<double>();
   double* part = E.partial.template as<double>();
+  double* ptot = E.ptot.template as<double>();
   const double* misc = small + E.lay.misc();
   const int nbv = ceil_div(E.n, kBS);
+  const bool wide = nc > 16;
   if (nc <= 8)
     mdot_rows_kernel<K, 8, false><<<E.nb, kBS, 0, stream()>>>(E.n, E.ld, U, nc, w, 0, E.rpb, part,
                                                             E.pstride, misc);
-  else if (nc <= 16)
+  else
     mdot_rows_kernel<K, 16, false><<<E.nb, kBS, 0, stream()>>>(E.n, E.ld, U, nc, w, 0, E.rpb, part,
                                                              E.pstride, misc);
-  else
-    LACE_NC_DISPATCH(nc, (mdot_kernel<K, NCB, false><<<E.nb, kBS, 0, stream()>>>(
-                             E.n, E.ld, U, nc, w, 0, E.rpb, part, E.pstride, misc)));
-  hh_dT_kernel<<<1, 256, 0, stream()>>>(E.nb, E.pstride, part, j, small, E.lay);
+  if (wide) reduce_cols_kernel<<<nc, 256, 0, stream()>>>(E.nb, nc, 0, E.pstride, part, ptot, misc);
+  hh_dT_kernel<<<1, 256, 0, stream()>>>(wide ? 1 : E.nb, E.pstride, wide ? ptot : part, j, small, E.lay);
   // fused update + tail multi-dot while the per-thread reflector row fits
   // in registers (up to 16 columns; 32 in FP32)
   if (nc <= 16 || (sizeof(K) == 4 && nc <= 32)) {
@@ -696,11 +729,12 @@ int hh_step(HHEngine<K>& E, int j, bool write_q) {
   } else {
     LACE_NC_DISPATCH(nc, (hh_update_kernel<K, NCB><<<nbv, kBS, 0, stream()>>>(E.n, E.ld, U, nc, small,
                                                                               E.lay, w)));
-    LACE_NC_DISPATCH(nc, (mdot_kernel<K, NCB, true><<<E.nb, kBS, 0, stream()>>>(
-                             E.n, E.ld, U, nc, w, j + 1, E.rpb, part, E.pstride, misc)));
+    mdot_rows_kernel<K, 16, true><<<E.nb, kBS, 0, stream()>>>(E.n, E.ld, U, nc, w, j + 1, E.rpb, part,
+                                                            E.pstride, misc);
   }
-  hh_newcol_small_kernel<K><<<1, 256, 0, stream()>>>(E.nb, E.pstride, part, j, E.n, E.ld, U, w,
-                                                     small, E.lay);
+  if (wide) reduce_cols_kernel<<<nc + 1, 256, 0, stream()>>>(E.nb, nc, 1, E.pstride, part, ptot, misc);
+  hh_newcol_small_kernel<K><<<1, 256, 0, stream()>>>(wide ? 1 : E.nb, E.pstride, wide ? ptot : part, j,
+                                                     E.n, E.ld, U, w, small, E.lay);
   LACE_NC_DISPATCH(nc, (hh_newcol_vec_kernel<K, NCB><<<nbv, kBS, 0, stream()>>>(
                            E.n, E.ld, U, j, w, small, E.lay, E.q.template as<K>(), write_q ? 1 : 0)));
   LACE_HIP(hipGetLastError());

@@ -55,6 +55,7 @@ struct HHEngine {
   int n = 0, m = 0, ld = 0, nb = 0, rpb = 4096, pstride = 0;
   bool z_float = false;
   DevBuf U, Z, w, q, r, small, partial, hostflag_dev;
+  DevBuf ptot;  // column totals of partial (one row, pstride doubles)
   SmallLayout lay{0};
   double* host_misc = nullptr;  // pinned copy of the misc slots
   int init(int n_, int m_, bool z_float_);
