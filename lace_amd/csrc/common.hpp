@@ -4,6 +4,7 @@
 #pragma once
 
 #include <hip/hip_runtime.h>
+#include <hip/hip_ext.h>
 
 #include <cstddef>
 #include <cstdint>
@@ -111,16 +112,30 @@ int download(void* host, const DevBuf& b, size_t bytes);
 // timing is enabled; the averages feed the roofline report.
 // A region may carry its algorithmic work (flops and HBM bytes of that
 // launch), summed per region for launches whose shape varies.
+// A region made with kernel_events = true times one kernel: launched
+// through launch(), that kernel's own start and end set the two events
+// (hipExtLaunchKernel), so the duration excludes the dispatch latency that
+// marker events around a short kernel would add.
 struct TimedRegion {
   const char* name;
   hipEvent_t a = nullptr, b = nullptr;
-  bool on = false;
+  bool on = false, ext = false, used = false;
   double flops = 0.0, bytes = 0.0;
   explicit TimedRegion(const char* n);
-  TimedRegion(const char* n, double flops_, double bytes_);
+  TimedRegion(const char* n, double flops_, double bytes_, bool kernel_events = false);
   ~TimedRegion();
+  template <typename F, typename... Args>
+  void launch(F kernel, dim3 grid, dim3 block, Args... args);
 };
 bool timing_on();
+
+template <typename F, typename... Args>
+void TimedRegion::launch(F kernel, dim3 grid, dim3 block, Args... args) {
+  const bool timed = on && ext && !used;
+  hipExtLaunchKernelGGL(kernel, grid, block, 0, stream(), timed ? a : nullptr, timed ? b : nullptr, 0,
+                        args...);
+  used = used || timed;
+}
 
 // ---------------------------------------------------------------- launch
 constexpr int kWave = 64;

@@ -712,17 +712,17 @@ static void launch_sym(ParILU* P, bool with_step, int in, int out, int nb) {
 template <typename T>
 static int sweep_t(ParILU* P, bool with_step) {
   const int in = P->cur, out = 1 - P->cur;
-  TimedRegion tr("parilu_sweep", 0.0, P->sweep_bytes());
+  const bool stream4 = P->sym && (P->variant == 0 || P->variant == 5);
+  // the four-position kernel's launch carries the events itself
+  TimedRegion tr("parilu_sweep", 0.0, P->sweep_bytes(), stream4);
   // symmetric patterns default to the four-position streaming kernel
   // (128^3: FP32 38.8 us vs 43.7 us for one position per thread)
-  if (P->sym && (P->variant == 0 || P->variant == 5)) {
+  if (stream4) {
     const int nb = ceil_div(P->nnzL, kSweepBS * 4);
     auto go = [&](auto kern) {
-      kern<<<nb, kSweepBS, 0, stream()>>>(P->nnzL, P->pmeta.as<int>(), P->mptr.as<int>(),
-                                          P->mpairs.as<int>(), P->AL.as<T>(), P->AU.as<T>(),
-                                          P->lval[in].as<T>(), P->uval[in].as<T>(),
-                                          P->lval[out].as<T>(), P->uval[out].as<T>(),
-                                          P->partial.as<double>());
+      tr.launch(kern, dim3(nb), dim3(kSweepBS), P->nnzL, P->pmeta.as<int>(), P->mptr.as<int>(),
+                P->mpairs.as<int>(), P->AL.as<T>(), P->AU.as<T>(), P->lval[in].as<T>(),
+                P->uval[in].as<T>(), P->lval[out].as<T>(), P->uval[out].as<T>(), P->partial.as<double>());
     };
     if (P->strict_matches) {
       if (with_step) go(parilu_stream4_sweep_kernel<T, true, true>);

@@ -148,14 +148,45 @@ TimedRegion::TimedRegion(const char* n) : name(n) {
   }
 }
 
-TimedRegion::TimedRegion(const char* n, double flops_, double bytes_) : TimedRegion(n) {
+TimedRegion::TimedRegion(const char* n, double flops_, double bytes_, bool kernel_events) : name(n) {
   flops = flops_;
   bytes = bytes_;
+  if (!g_timing_on) return;
+  std::lock_guard<std::mutex> lk(g_tmu);
+  if (!timing_selected(n)) return;
+  TimingEntry& e = g_timing[name];
+  if (e.calls++ % g_timing_sample != 0) return;
+  a = pooled_event();
+  b = pooled_event();
+  if (!a || !b) {
+    if (a) g_event_pool.push_back(a);
+    if (b) g_event_pool.push_back(b);
+    a = b = nullptr;
+    return;
+  }
+  if (kernel_events) {
+    // recorded by the kernel launched through launch()
+    ext = true;
+    on = true;
+    return;
+  }
+  on = hipEventRecord(a, stream()) == hipSuccess;
+  if (!on) {
+    g_event_pool.push_back(a);
+    g_event_pool.push_back(b);
+  }
 }
 
 TimedRegion::~TimedRegion() {
   if (!on) return;
-  (void)hipEventRecord(b, stream());
+  if (ext && !used) {
+    // no kernel took the events: nothing was timed
+    std::lock_guard<std::mutex> lk(g_tmu);
+    g_event_pool.push_back(a);
+    g_event_pool.push_back(b);
+    return;
+  }
+  if (!ext) (void)hipEventRecord(b, stream());
   std::lock_guard<std::mutex> lk(g_tmu);
   TimingEntry& e = g_timing[name];
   e.pending.emplace_back(a, b);

@@ -233,16 +233,18 @@ static int spmv_form(const DevCSR& A) {
 }
 
 template <typename TV, typename TX, typename TY, int MODE>
-static int launch_spmv(const DevCSR& A, const TX* x, TY* y, const TY* b, double alpha, double beta) {
+static int launch_spmv(const DevCSR& A, const TX* x, TY* y, const TY* b, double alpha, double beta,
+                       TimedRegion& tr) {
   const int form = spmv_form(A);
   if (form == 2) {
     const int nb = ceil_div(A.n, 256);
     const uint8_t* mk = A.dia.mask.as<uint8_t>();
     const TV* dv = A.dia.val.as<TV>();
     switch (A.dia.nd) {
-#define LACE_SPMV_DIA(D)                                                                                        \
-  case D:                                                                                                       \
-    spmv_dia_kernel<TV, TX, TY, MODE, D><<<nb, 256, 0, stream()>>>(A.n, A.dia.off, mk, dv, x, y, b, alpha, beta); \
+#define LACE_SPMV_DIA(D)                                                                                     \
+  case D:                                                                                                    \
+    tr.launch(spmv_dia_kernel<TV, TX, TY, MODE, D>, dim3(nb), dim3(256), A.n, A.dia.off, mk, dv, x, y, b, \
+              alpha, beta);                                                                                 \
     break;
       LACE_SPMV_DIA(1)
       LACE_SPMV_DIA(2)
@@ -384,17 +386,19 @@ double DevCSR::bytes(int sx, int sy, bool residual) const {
 // y = alpha*A*x + beta*y; vector precisions follow the template arguments.
 template <typename TX, typename TY>
 int spmv(const DevCSR& A, double alpha, const TX* x, double beta, TY* y) {
+  // the diagonal-form launch carries the events itself
   TimedRegion tr(A.prec == LACE_F64 ? "spmv" : "spmv_f32a", 0.0,
-                 A.bytes((int)sizeof(TX), (int)sizeof(TY), false) + (beta != 0.0 ? sizeof(TY) * (double)A.n : 0.0));
-  if (A.prec == LACE_F64) return launch_spmv<double, TX, TY, 0>(A, x, y, nullptr, alpha, beta);
-  return launch_spmv<float, TX, TY, 0>(A, x, y, nullptr, alpha, beta);
+                 A.bytes((int)sizeof(TX), (int)sizeof(TY), false) + (beta != 0.0 ? sizeof(TY) * (double)A.n : 0.0),
+                 spmv_form(A) == 2);
+  if (A.prec == LACE_F64) return launch_spmv<double, TX, TY, 0>(A, x, y, nullptr, alpha, beta, tr);
+  return launch_spmv<float, TX, TY, 0>(A, x, y, nullptr, alpha, beta, tr);
 }
 
 template <typename TX, typename TY>
 int residual(const DevCSR& A, const TX* x, const TY* b, TY* r) {
-  TimedRegion tr("spmv_residual", 0.0, A.bytes((int)sizeof(TX), (int)sizeof(TY), true));
-  if (A.prec == LACE_F64) return launch_spmv<double, TX, TY, 1>(A, x, r, b, 1.0, 0.0);
-  return launch_spmv<float, TX, TY, 1>(A, x, r, b, 1.0, 0.0);
+  TimedRegion tr("spmv_residual", 0.0, A.bytes((int)sizeof(TX), (int)sizeof(TY), true), spmv_form(A) == 2);
+  if (A.prec == LACE_F64) return launch_spmv<double, TX, TY, 1>(A, x, r, b, 1.0, 0.0, tr);
+  return launch_spmv<float, TX, TY, 1>(A, x, r, b, 1.0, 0.0, tr);
 }
 
 template int spmv<double, double>(const DevCSR&, double, const double*, double, double*);

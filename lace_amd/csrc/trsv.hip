@@ -462,20 +462,23 @@ __global__ __launch_bounds__(256) void tri_jacobi_dia_kernel(int n, DiaOffsets o
 }
 
 template <typename T, bool UNIT>
-static void jacobi_dia(const TriPlan& P, const T* r, const T* yin, T* yout, T* y2, const T* d2) {
+static void jacobi_dia(const TriPlan& P, const T* r, const T* yin, T* yout, T* y2, const T* d2,
+                       TimedRegion& tr) {
   const int nb = ceil_div(P.n, 256);
   const uint8_t* mk = P.dia.mask.as<uint8_t>();
   const T* dv = P.dia.val.as<T>();
   const T* dg = P.dval.as<T>();
+  T* no_y2 = nullptr;
+  const T* no_d2 = nullptr;
   switch (P.dia.nd) {
-#define LACE_DIA_CASE(D)                                                                                   \
-  case D:                                                                                                  \
-    if (y2)                                                                                                \
-      tri_jacobi_dia_kernel<T, UNIT, D, true>                                                              \
-          <<<nb, 256, 0, stream()>>>(P.n, P.dia.off, mk, dv, dg, r, yin, yout, y2, d2);                    \
-    else                                                                                                   \
-      tri_jacobi_dia_kernel<T, UNIT, D, false>                                                             \
-          <<<nb, 256, 0, stream()>>>(P.n, P.dia.off, mk, dv, dg, r, yin, yout, nullptr, nullptr);          \
+#define LACE_DIA_CASE(D)                                                                                  \
+  case D:                                                                                                 \
+    if (y2)                                                                                               \
+      tr.launch(tri_jacobi_dia_kernel<T, UNIT, D, true>, dim3(nb), dim3(256), P.n, P.dia.off, mk, dv, dg, \
+                r, yin, yout, y2, d2);                                                                    \
+    else                                                                                                  \
+      tr.launch(tri_jacobi_dia_kernel<T, UNIT, D, false>, dim3(nb), dim3(256), P.n, P.dia.off, mk, dv,    \
+                dg, r, yin, yout, no_y2, no_d2);                                                          \
     break;
     LACE_DIA_CASE(1)
     LACE_DIA_CASE(2)
@@ -594,9 +597,10 @@ static int jacobi_sweep(const TriPlan& P, const T* val, const T* r, const T* yin
     else if (ell) bytes += (double)P.ew * n * (es + 4.0) + diag;
     else if (strict) bytes += (double)P.snnz * (es + 4.0) + 4.0 * (n + 1) + diag;
     else bytes += (double)P.nnz * (es + 4.0) + 4.0 * (n + 1);
-    TimedRegion tr(P.upper ? "trsv_jacobi_sweep_upper" : "trsv_jacobi_sweep_lower", 0.0, bytes);
-    if (dia && P.unit_diag) jacobi_dia<T, true>(P, r, yin, yout, fuse ? y2 : nullptr, d2);
-    else if (dia) jacobi_dia<T, false>(P, r, yin, yout, fuse ? y2 : nullptr, d2);
+    // the diagonal-form launch carries the events itself
+    TimedRegion tr(P.upper ? "trsv_jacobi_sweep_upper" : "trsv_jacobi_sweep_lower", 0.0, bytes, dia);
+    if (dia && P.unit_diag) jacobi_dia<T, true>(P, r, yin, yout, fuse ? y2 : nullptr, d2, tr);
+    else if (dia) jacobi_dia<T, false>(P, r, yin, yout, fuse ? y2 : nullptr, d2, tr);
     else if (ell && P.unit_diag) jacobi_ell<T, true>(P, r, yin, yout);
     else if (ell) jacobi_ell<T, false>(P, r, yin, yout);
     else if (strict && P.unit_diag) jacobi_strict<T, true>(P, r, yin, yout);
