@@ -292,8 +292,9 @@ int data_MKL_FGMRES(data_d_matrix *A, data_d_matrix *x0,
  * the reference ordering is kept (documented in lace_amd/csrc/reorder.cpp);
  * where the reference reads uninitialised memory this returns DEV_ERR.
  * data_sparse_reorder: reorder 1 = Cuthill-McKee, 2 = reversed, 0/other =
- * identity (3 = AMD is not provided: DEV_ERR_NOT_SUPPORTED); A becomes
- * A(P, P) with rows sorted by column.  lace_bool is C++ bool / C _Bool. */
+ * identity (3 = AMD is not provided: DEV_ERR_NOT_SUPPORTED; 4, an addition,
+ * = lace_reorder_rcm); A becomes A(P, P) with rows sorted by column.
+ * lace_bool is C++ bool / C _Bool. */
 #ifdef __cplusplus
 typedef bool lace_bool;
 #else
@@ -301,6 +302,13 @@ typedef _Bool lace_bool;
 #endif
 int data_reorder_cuthill_mckee(data_d_matrix *A, int *nn_map, lace_bool reverse);
 int data_sparse_reorder(data_d_matrix *A, int *P, int *Pinv, int reorder);
+/* Reverse Cuthill-McKee in the textbook form (George & Liu 1981): on the
+ * symmetrised graph without self loops, each component from a
+ * pseudo-peripheral node, neighbours in increasing (degree, number), the
+ * whole order reversed.  P[k] = old number of the row placed at k.  No
+ * counterpart in the reference, whose ordering keeps its quirks (above);
+ * this one is for callers that want RCM's bandwidth reduction. */
+int lace_reorder_rcm(const data_d_matrix *A, int *P);
 
 /* Householder generator and triangularization (host, small dense).
  * sparse.h:1059-1067; src/householder.cpp:49-160.  Dense containers are

@@ -150,6 +150,7 @@ SIGNATURES = {
     "data_MKL_FGMRES": (_i, [_DM, _DM, _DM, _P(GmresParam)]),
     "data_reorder_cuthill_mckee": (_i, [_DM, _P(_i), C.c_bool]),
     "data_sparse_reorder": (_i, [_DM, _P(_i), _P(_i), _i]),
+    "lace_reorder_rcm": (_i, [_DM, _P(_i)]),
     "lace_init": (_i, [_i]),
     "lace_last_error": (C.c_char_p, []),
     "lace_device_synchronize": (_i, []),

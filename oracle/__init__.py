@@ -759,3 +759,65 @@ def cuthill_mckee(A: CSR, reverse: bool = False):
             nn_map[counter], nn_map[i] = nn_map[i], nn_map[counter]
             counter -= 1
     return np.array(nn_map, dtype=np.int32)
+
+
+def rcm_textbook(A: CSR):
+    """Reverse Cuthill-McKee as George & Liu (1981, ch. 4) state it, the
+    ordering lace_reorder_rcm computes (an addition: not the reference's):
+    symmetrised graph without self loops; per component a pseudo-peripheral
+    start (from the least (degree, number) node, move to the last BFS
+    level's least node while the level count grows); breadth-first
+    numbering with unnumbered neighbours by increasing (degree, number);
+    the order reversed.  Returns P with P[k] = old number of new row k."""
+    n = A.n_rows
+    nb = [set() for _ in range(n)]
+    for i in range(n):
+        for k in range(A.row[i], A.row[i + 1]):
+            j = int(A.col[k])
+            if j != i:
+                nb[i].add(j)
+                nb[j].add(i)
+    adj = [sorted(s) for s in nb]
+    deg = [len(a) for a in adj]
+    key = lambda v: (deg[v], v)  # noqa: E731
+    numbered = [False] * n
+
+    def levels_from(r):
+        seen = {r}
+        level = [r]
+        depth = 0
+        last = level
+        while level:
+            last = level
+            depth += 1
+            nxt = []
+            for v in level:
+                for u in adj[v]:
+                    if not numbered[u] and u not in seen:
+                        seen.add(u)
+                        nxt.append(u)
+            level = nxt
+        return depth, min(last, key=key)
+
+    order = []
+    for start in sorted(range(n), key=key):
+        if numbered[start]:
+            continue
+        depth, far = levels_from(start)
+        while True:
+            d2, far2 = levels_from(far)
+            root = far
+            if d2 <= depth:
+                break
+            depth, far = d2, far2
+        numbered[root] = True
+        comp = [root]
+        t = 0
+        while t < len(comp):
+            cand = sorted((u for u in adj[comp[t]] if not numbered[u]), key=key)
+            for u in cand:
+                numbered[u] = True
+                comp.append(u)
+            t += 1
+        order.extend(comp)
+    return np.array(order[::-1], dtype=np.int32)

@@ -113,3 +113,131 @@ def test_amd_not_provided(lace, orc):
     ip = C.POINTER(C.c_int)
     # rejected before A's arrays are touched
     assert lace.lib().data_sparse_reorder(H.ref(), P.ctypes.data_as(ip), Pinv.ctypes.data_as(ip), 3) != 0
+
+
+def _rcm_lib(lace, A):
+    H = lace.HostCSR(A.row, A.col, A.val)
+    P = np.zeros(A.n_rows, np.int32)
+    lace.check(lace.lib().lace_reorder_rcm(H.ref(), P.ctypes.data_as(C.POINTER(C.c_int))))
+    return P
+
+
+def _bandwidth(A, P):
+    pos = np.empty_like(P)
+    pos[P] = np.arange(len(P))
+    ri = A.rowidx()
+    return int(np.abs(pos[ri] - pos[A.col]).max())
+
+
+def _graphs(orc):
+    import os
+    from conftest import GOLDEN
+    rng = np.random.default_rng(3)
+    # a scrambled path and a second component (a scrambled star): RCM must
+    # lay the path out contiguously
+    n = 40
+    lab = rng.permutation(n)
+    rows, cols = list(range(n)), list(range(n))
+    for a, b in zip(lab[:29], lab[1:30]):
+        rows += [a, b]; cols += [b, a]
+    for leaf in lab[31:]:
+        rows += [lab[30], leaf]; cols += [leaf, lab[30]]
+    M = orc._coo_to_csr(n, np.array(rows), np.array(cols), np.ones(len(rows)))
+    return {"lap2d": orc.laplace2d(9, 7), "conv3d": orc.convdiff3d(6, 5, 4),
+            "trefethen": orc.read_mtx(os.path.join(GOLDEN, "Trefethen_20.mtx")), "path_star": M}
+
+
+def test_textbook_rcm_matches_oracle(lace, orc):
+    for name, A in _graphs(orc).items():
+        P = _rcm_lib(lace, A)
+        assert np.array_equal(P, orc.rcm_textbook(A)), name
+        assert sorted(P) == list(range(A.n_rows))
+
+
+def test_textbook_rcm_properties(orc):
+    """The restatement against what RCM fixes: a path comes out contiguous
+    (bandwidth 1 within it), the Cuthill-McKee order (before reversal) runs
+    through each component by non-decreasing BFS distance from its start,
+    and on the grid and path graphs the bandwidth is no worse than scipy's
+    RCM (RCM is a heuristic: on the dense Trefethen graph scipy's
+    minimum-degree start happens to do better)."""
+    import scipy.sparse.csgraph as cg
+    for name, A in _graphs(orc).items():
+        P = orc.rcm_textbook(A)
+        S = A.scipy()
+        G = ((S + S.T) != 0).astype(np.int8).tocsr()
+        cm = P[::-1]
+        # components in CM order: distances from each component's first node
+        seen = np.zeros(A.n_rows, bool)
+        k = 0
+        while k < A.n_rows:
+            d = cg.shortest_path(G, unweighted=True, indices=cm[k])
+            comp = np.isfinite(d)
+            size = int(comp.sum())
+            block = cm[k:k + size]
+            assert comp[block].all() and not seen[block].any(), name
+            assert np.all(np.diff(d[block]) >= 0), name
+            seen[block] = True
+            k += size
+        if name != "trefethen":
+            ref = cg.reverse_cuthill_mckee(S.tocsr(), symmetric_mode=False)
+            assert _bandwidth(A, P) <= _bandwidth(A, np.asarray(ref, np.int32)), name
+    A = _graphs(orc)["path_star"]
+    P = orc.rcm_textbook(A)
+    pos = np.empty_like(P)
+    pos[P] = np.arange(len(P))
+    ri = A.rowidx()
+    _, comp = cg.connected_components(A.scipy(), directed=False)
+    path = np.bincount(comp).argmax()  # the 30-node path outnumbers the star
+    path_edges = [(r, c) for r, c in zip(ri, A.col) if r != c and comp[r] == path]
+    assert len(path_edges) == 58
+    assert max(abs(pos[r] - pos[c]) for r, c in path_edges) == 1
+
+
+def test_sparse_reorder_textbook_rcm(lace, orc):
+    A = orc.convdiff3d(6, 5, 4)
+    H = lace.HostCSR(A.row, A.col, A.val)
+    M = lace.DMatrix()
+    lace.check(lace.lib().data_zmconvert(H.m, C.byref(M), lace.Magma_CSR, lace.Magma_CSR))
+    P = np.zeros(A.n_rows, np.int32); Pinv = np.zeros(A.n_rows, np.int32)
+    ip = C.POINTER(C.c_int)
+    lace.check(lace.lib().data_sparse_reorder(C.byref(M), P.ctypes.data_as(ip), Pinv.ctypes.data_as(ip), 4))
+    assert np.array_equal(P, orc.rcm_textbook(A))
+    row, col, val = lace.owned_to_numpy(M)
+    B = orc.CSR(A.n_rows, A.n_rows, row, col, val)
+    assert np.array_equal(B.dense(), A.dense()[np.ix_(P, P)])
+    # the 3-D grid: textbook RCM narrows the band, the reference's ordering widens it
+    assert _bandwidth(A, P) < _bandwidth(A, np.arange(A.n_rows, dtype=np.int32))
+
+
+def test_reference_ordering_breaks_synchronous_sweeps(lace, orc):
+    """A finding about the RCM pre-step: on the reference's ordering
+    (data_sparse_reorder 2, with its quirks) the synchronous ParILU sweeps of
+    the 3-D convection-diffusion operator overflow within three sweeps,
+    although exact ILU(0) of the same matrix has no zero pivot; on the
+    textbook RCM ordering they converge like the natural order."""
+    A = orc.convdiff3d(16, 14, 12)
+    out = {}
+    for name, reorder in (("reference", 2), ("textbook", 4)):
+        H = lace.HostCSR(A.row, A.col, A.val)
+        M = lace.DMatrix()
+        lace.check(lace.lib().data_zmconvert(H.m, C.byref(M), lace.Magma_CSR, lace.Magma_CSR))
+        P = np.zeros(A.n_rows, np.int32); Pinv = np.zeros(A.n_rows, np.int32)
+        ip = C.POINTER(C.c_int)
+        lace.check(lace.lib().data_sparse_reorder(C.byref(M), P.ctypes.data_as(ip), Pinv.ctypes.data_as(ip),
+                                                  reorder))
+        row, col, val = lace.owned_to_numpy(M)
+        B = orc.CSR(A.n_rows, A.n_rows, row, col, val)
+        st = orc.ParILUState(B)
+        nl = []
+        with np.errstate(all="ignore"):
+            for _ in range(3):
+                st.sweep("jacobi")
+                nl.append(orc.ilures(B, *st.factors())[1])
+        ex, bad = orc.ilu0(B)
+        out[name] = (nl, bad, _bandwidth(A, P))
+    assert out["reference"][1] < 0 and not np.isfinite(out["reference"][0][-1])
+    assert out["textbook"][1] < 0 and np.all(np.isfinite(out["textbook"][0]))
+    assert out["textbook"][0][-1] < out["textbook"][0][0]
+    natural = _bandwidth(A, np.arange(A.n_rows, dtype=np.int32))
+    assert out["textbook"][2] < natural < out["reference"][2]
