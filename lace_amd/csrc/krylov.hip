@@ -270,7 +270,8 @@ __global__ __launch_bounds__(256) void hh_start_kernel(int nb, int pstride, cons
     misc[MS_NCOLS] = 0.0;
     misc[MS_BREAK] = 0.0;
     misc[MS_RES] = rnorm;
-    const bool done = !(rnorm >= misc[MS_RTOL]) || !isfinite(rnorm);
+    // a zero residual is solved (and has no reflector: dd would be 0)
+    const bool done = !(rnorm >= misc[MS_RTOL]) || !isfinite(rnorm) || rnorm == 0.0;
     misc[MS_DONE] = done ? 1.0 : 0.0;
     double* g = small + lay.g();
     for (int t = 0; t < lay.m + 2; t++) g[t] = 0.0;

@@ -69,7 +69,10 @@ int num_cus() { return g_cus; }
 int upload(DevBuf& b, const void* host, size_t bytes) {
   LACE_TRY(b.alloc(bytes));
   if (bytes) {
-    LACE_HIP(hipMemcpyAsync(b.p, host, bytes, hipMemcpyHostToDevice, stream()));
+    // an empty host array (a matrix without off-diagonal entries, say)
+    // still gets its placeholder allocation, zeroed
+    if (host) LACE_HIP(hipMemcpyAsync(b.p, host, bytes, hipMemcpyHostToDevice, stream()));
+    else LACE_HIP(hipMemsetAsync(b.p, 0, bytes, stream()));
     LACE_HIP(hipStreamSynchronize(stream()));
   }
   return DEV_SUCCESS;

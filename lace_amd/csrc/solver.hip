@@ -499,7 +499,7 @@ static int solver_solve(Solver* S, const double* b, double* x, lace_solver_stats
     S->mlog.cycle_begin(restart, m, rnorm, S->o.rtol, rtol_abs);
     if (!std::isfinite(rnorm)) break;
     if (E.host_misc[MS_DONE] != 0.0) {
-      converged = rnorm < rtol_abs;
+      converged = rnorm < rtol_abs || rnorm == 0.0;
       break;
     }
     for (int j = 0; j < m; j++) {
@@ -754,7 +754,7 @@ static int solver_solve_mgs(Solver* S, const double* b, double* x, lace_solver_s
       rtol_abs = hostsc[MG_RTOL];
     }
     S->mlog.cycle_begin(restart, m, hostsc[MG_RNORM], S->o.rtol, rtol_abs);
-    if (!(hostsc[MG_RNORM] >= rtol_abs)) {
+    if (!(hostsc[MG_RNORM] >= rtol_abs) || hostsc[MG_RNORM] == 0.0) {
       converged = std::isfinite(hostsc[MG_RNORM]);
       local.final_residual_est = hostsc[MG_RNORM];
       break;
