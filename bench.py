@@ -343,12 +343,17 @@ def bench_bcsr(args) -> int:
            "n": n, "blocks": int(row[-1])}
     # sweeps and SpMV bandwidth
     for name, prec in (("f32", L.LACE_F32), ("bf16", L.LACE_BF16), ("f64", L.LACE_F64)):
-        P = L.BILU(H, prec)
-        P.sweeps(2)
-        avg, gbs = _region_rate(L, "bilu_sweep", P.sweep_bytes(), lambda: P.sweeps(1), 10)
-        out[f"bilu_sweep_{name}"] = {"us": round(avg * 1e6, 2), "GBps": round(gbs, 1),
-                                     "bytes": P.sweep_bytes()}
-        del P
+        # forms of the paired sweep: 0 the default, 2 the device-side merge of
+        # the two block rows, 3 set-up tables, 4 tables + LDS staging
+        for variant, tag in ((0, ""), (2, "_merge"), (3, "_tables"), (4, "_tables_lds")):
+            P = L.BILU(H, prec)
+            if variant:
+                P.set_variant(variant)
+            P.sweeps(2)
+            avg, gbs = _region_rate(L, "bilu_sweep", P.sweep_bytes(), lambda: P.sweeps(1), 10)
+            out[f"bilu_sweep_{name}{tag}"] = {"us": round(avg * 1e6, 2), "GBps": round(gbs, 1),
+                                              "bytes": P.sweep_bytes()}
+            del P
     A = L.DevBCSR(H, L.LACE_F64)
     x = L.DevVec(n, np.float64, b)
     y = L.DevVec(n, np.float64)

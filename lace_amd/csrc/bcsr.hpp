@@ -49,7 +49,8 @@ int bcsr_residual(const DevBCSR& A, const TX* x, const TY* b, TY* r);
 struct BILU {
   int nbr = 0, bs = 0, nnzL = 0, nnzU = 0, nstrict = 0, nblocksA = 0;
   bool sym = false;
-  int variant = 0;  // 0: symmetric-pattern kernel when possible, 1: general
+  int variant = 0;  // 0: symmetric-pattern kernel when possible, 1: general;
+                    // 2-4: symmetric kernel forms (merge, tables, tables + LDS)
   lace_prec_t store = LACE_F32;
   double anorm = 0.0;
   // L (BCSRL) and U (BCSCU) patterns; lrow = block row of each L entry
@@ -62,6 +63,7 @@ struct BILU {
   bool have_general = false;
   DevBuf arowidx, acol;  // A's block coordinates (general kernel)
   DevBuf AL, AU;         // A_IJ and A_JI at each L position (symmetric kernel)
+  DevBuf smeta, spend, spairs;  // symmetric kernel's per-position tables
   DevBuf AV;             // A's blocks in its own order (general kernel)
   DevBuf lfrom, ufrom;   // A block feeding each L / U entry (-1: none)
   DevBuf Lv[2], Uv[2];

@@ -1,5 +1,5 @@
 """Block (BCSR) path on the GPU against the oracle: block ParILU sweeps
-(symmetric-pair and per-block kernels, FP64/FP32 bit-exact, BF16/F16
+(symmetric-pair kernels with and without LDS staging, per-block kernel, FP64/FP32 bit-exact, BF16/F16
 storage bit-exact against the oracle's rounding), block triangular solves
 (exact level-scheduled and Jacobi-k, FP32 bit-exact), block SpMV (FP64
 bit-exact), the data_PariLU_v0_3_bcsr drop-in, and block-preconditioned
@@ -87,6 +87,64 @@ def test_16bit_storage_bit_exact(gpu, orc, prec):
         assert np.array_equal(uv, uw), (prec, k)
     lv, lw, uv, uw = _compare(gpu, orc, _bcsr_test(orc), prec, 2)
     assert np.array_equal(lv, lw) and np.array_equal(uv, uw)
+
+
+def _block_sym_random(orc, nbr, bs, seed, most=6, shift=8.0):
+    """Random structurally symmetric block pattern (each row its diagonal and
+    up to `most` partners, each pair in both rows): unlike a grid, it holds
+    triangles, so positions (I, J) have block columns K < J common to rows I
+    and J (products in the sweep)."""
+    rng = np.random.default_rng(seed)
+    pairs = set()
+    for i in range(nbr):
+        for j in rng.choice(nbr, size=rng.integers(0, most + 1), replace=False):
+            if j != i:
+                pairs.add((i, int(j)))
+                pairs.add((int(j), i))
+    pairs |= {(i, i) for i in range(nbr)}
+    ent = sorted(pairs)
+    row = np.zeros(nbr + 1, np.int32)
+    for i, _ in ent:
+        row[i + 1] += 1
+    row = np.cumsum(row).astype(np.int32)
+    col = np.array([j for _, j in ent], np.int32)
+    val = rng.normal(0.0, 0.1, size=(len(col), bs, bs))
+    for k, (i, j) in enumerate(ent):
+        if i == j:
+            val[k] += shift * np.eye(bs)
+    return orc.BCSR(nbr, bs, row, col, val)
+
+
+@pytest.mark.parametrize("prec", ["f64", "f32", "bf16", "f16"])
+@pytest.mark.parametrize("bs", [5, 4, 3, 2])
+@pytest.mark.parametrize("pattern", ["grid", "random"])
+def test_paired_sweep_forms_identical(gpu, orc, prec, bs, pattern):
+    """The forms of the paired sweep: the device-side merge of the two block
+    rows (variant 2) against the set-up tables without staging (3), with
+    each workgroup's blocks staged through LDS wherever that applies (4), and
+    the default choice (0): identical factors and step norms over many
+    workgroups with a partial last one.  A 7-point grid has no products
+    over K < J (no triangles); the random pattern has many."""
+    if pattern == "grid":
+        B = _block7(orc, 7, 6, 5, bs, seed=30 + bs)
+    else:
+        B = _block_sym_random(orc, 300, bs, seed=40 + bs)
+    code = _PREC[prec][0]
+    Ps = {v: gpu.BILU(_host(gpu, B), code) for v in (2, 3, 4, 0)}
+    for v, P in Ps.items():
+        assert P.symmetric()
+        if v:
+            P.set_variant(v)
+    for k in range(3):
+        steps = {v: P.sweeps(1, want_step=True) for v, P in Ps.items()}
+        (_, _, l2, _), (_, _, u2, _) = Ps[2].factors()
+        for v in (3, 4, 0):
+            assert steps[v] == steps[2], (v, k)
+            (_, _, lv, _), (_, _, uv, _) = Ps[v].factors()
+            assert np.array_equal(lv, l2) and np.array_equal(uv, u2), (v, k)
+    if pattern == "random" and prec in ("f64", "f32"):
+        lv, lw, uv, uw = _compare(gpu, orc, B, prec, 3)
+        assert np.array_equal(lv, lw) and np.array_equal(uv, uw)
 
 
 def test_step_matches_oracle(gpu, orc):
