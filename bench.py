@@ -124,7 +124,7 @@ def _cpu_model():
 # kernel classes by the kernel each launch runs on the 128^3 operator (the
 # FP32 streaming sweep; Jacobi sweeps and products on the diagonal form)
 SWEEP_KERNEL = "parilu_stream4_sweep_kernel<float>"
-JACOBI_KERNEL = "tri_jacobi_dia_kernel<float>"
+JACOBI_KERNEL = "tri_jacobi_dia_vec_kernel<float>"
 SPMV64_KERNEL = "spmv_dia_kernel<double, float, double>"
 SPMV32_KERNEL = "spmv_dia_kernel<float, float, float>"
 # kernel class -> per-launch timing regions of the library; each launch
@@ -221,6 +221,9 @@ def cache_rates(L, H, n, flush, reps=10):
                 L.check(lib.lace_timing_enable(1))
                 fn()
                 L.check(lib.lace_timing_enable(0))
+                # each call starts on an idle queue, so a launch's start
+                # event never waits behind the previous call's kernels
+                L.check(lib.lace_device_synchronize())
             L.check(lib.lace_device_synchronize())
             L.check(lib.lace_timing_only(None))
             # one launch per region per call: the call's kernel time and
@@ -525,12 +528,20 @@ def main() -> int:
                          "block-sparse 5x5 workload on the 64^3 block grid; block: 32 "
                          "right-hand sides, forward and adjoint block FGMRES; reorder: the "
                          "RCM pre-step against natural order")
+    ap.add_argument("--set", action="append", default=[], metavar="NAME=VALUE",
+                    help="library kernel-shape option (lace_set_option) applied before the run; "
+                         "repeatable; every choice gives bit-identical results")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: --gpus {args.gpus} is not supported; this implementation runs on one "
               "GPU (multi-GPU sharding of the path is described in DESIGN.md, not built)",
               file=sys.stderr)
         return 2
+    if args.set:
+        import lace_amd as L
+        for kv in args.set:
+            name, _, value = kv.partition("=")
+            L.check(L.lib().lace_set_option(name.encode(), int(value)), f"option {name}")
     if args.mode == "bcsr":
         return bench_bcsr(args)
     if args.mode == "block":
@@ -678,6 +689,7 @@ def main() -> int:
         # the timed steps carry events around every TIMING_SAMPLE-th hot
         # launch; the same steps without any events:
         "timing_sample": TIMING_SAMPLE,
+        "options": dict(kv.partition("=")[::2] for kv in args.set),
         "ms_per_step_without_events": round(t_plain * 1e3, 3),
         "cpu_baseline": cpu,
     }

@@ -404,8 +404,9 @@ int lace_parilu_forms(const lace_parilu *P, int upper, int *ndiag, int *ell_widt
  * streaming kernel with four positions per thread (16-byte accesses).
  * All give identical factors. */
 int lace_parilu_set_variant(lace_parilu *P, int variant);
-/* Non-zero if a sync-free level solve gave up waiting (a wait exceeded its
- * poll budget); the result of that solve is then invalid. */
+/* Non-zero if a sync-free level solve or a pipelined Jacobi launch gave up
+ * waiting (a wait exceeded its poll budget); the result of that solve is
+ * then invalid.  The flag stays set for the handle's lifetime. */
 int lace_syncfree_error(lace_parilu *P, int *err);
 
 /* Device BCSR matrix (A->storage_type Magma_BCSR, num_rows = block rows,
@@ -587,8 +588,16 @@ int lace_block_panel_qr(int n, int r0, const double *W, double *Y, double *Tb, d
  * "spmv_dia" (0 or 1, default 1): the same with the diagonal form, which
  * takes precedence over ELLPACK where both apply.  "jacobi_fuse" (0 or 1,
  * default 1): with an odd sweep count, the first Jacobi sweep of U rides on
- * the last sweep of L (diagonal form only).  Every choice gives
- * bit-identical results.  "matlab_log" (0 or 1, default 1): the
+ * the last sweep of L (diagonal form only).  "jacobi_dia_vec" (0 or 1,
+ * default 1): Jacobi sweeps on the diagonal form take a 16-byte word of
+ * rows per thread (4 FP32, 2 FP64) where n is a multiple of a word and the
+ * vectors are 16-byte aligned (0: one row per thread).  "jacobi_pipe" (0 or
+ * 1, default 0): a ParILU apply with k >= 3 Jacobi sweeps, both factors on
+ * the diagonal form, runs sweeps 2..k of each factor in one launch (tiles
+ * in ticket order, each waiting on the tiles it reads through progress
+ * flags; the first sweep of U rides on L's launch); slower than a launch
+ * per sweep on the 128^3 operator.  Every choice gives bit-identical
+ * results.  "matlab_log" (0 or 1, default 1): the
  * drop-in FGMRES entry points print the reference's MATLAB-syntax progress
  * lines to stdout (src/fgmres.cpp:71,178,430; src/fgmres_householder_restart.cpp:
  * 70,81,196,231,707) -- "FGMRES_Householders_partrsv_search(j) = |g_j|;" etc.,

@@ -44,6 +44,11 @@ struct TriPlan {
   // The same off-diagonal entries in diagonal form when they lie on at most
   // 8 diagonals (stencil factors); gathered with the other forms.
   DiaForm dia;
+  // Pipelined Jacobi on the diagonal form (several sweeps per launch): the
+  // per-tile progress flags, the tile ticket counter, the intermediate
+  // sweep vectors, and the next launch's ticket and flag bases.
+  mutable DevBuf pipe_flags, pipe_ctr, pipe_buf;
+  mutable unsigned pipe_tickets = 0, pipe_epoch = 0;
   int build(int n, const int* row, const int* col, bool upper);
   int refresh_strict(const void* val, size_t elem_size);
 };

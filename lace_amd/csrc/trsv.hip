@@ -461,24 +461,303 @@ __global__ __launch_bounds__(256) void tri_jacobi_dia_kernel(int n, DiaOffsets o
   if (EPI) y2[i] = o / d2[i];
 }
 
+// R consecutive elements (16 bytes) at a 16-byte aligned address.
+template <typename T>
+struct alignas(16) Pack16 {
+  static constexpr int R = 16 / (int)sizeof(T);
+  T v[R];
+};
+template <typename T>
+__device__ __forceinline__ Pack16<T> ld16(const T* p) {
+  return *reinterpret_cast<const Pack16<T>*>(p);
+}
+template <typename T>
+__device__ __forceinline__ void st16(T* p, const Pack16<T>& a) {
+  *reinterpret_cast<Pack16<T>*>(p) = a;
+}
+
+// y[k] = (lo ++ hi)[k + sh] for k < R, sh in [0, R) the same in every lane.
+template <typename T>
+__device__ __forceinline__ void window16(const Pack16<T>& lo, const Pack16<T>& hi, int sh,
+                                         T (&y)[Pack16<T>::R]) {
+  constexpr int R = Pack16<T>::R;
+  T c[2 * R];
+#pragma unroll
+  for (int k = 0; k < R; k++) {
+    c[k] = lo.v[k];
+    c[R + k] = hi.v[k];
+  }
+  if constexpr (R == 4) {
+    switch (sh) {
+      case 0: y[0] = c[0]; y[1] = c[1]; y[2] = c[2]; y[3] = c[3]; break;
+      case 1: y[0] = c[1]; y[1] = c[2]; y[2] = c[3]; y[3] = c[4]; break;
+      case 2: y[0] = c[2]; y[1] = c[3]; y[2] = c[4]; y[3] = c[5]; break;
+      default: y[0] = c[3]; y[1] = c[4]; y[2] = c[5]; y[3] = c[6]; break;
+    }
+  } else {
+    if (sh == 0) {
+      y[0] = c[0];
+      y[1] = c[1];
+    } else {
+      y[0] = c[1];
+      y[1] = c[2];
+    }
+  }
+}
+
+// The diagonal-form sweep with R = 16 / sizeof(T) consecutive rows per
+// thread (4 FP32, 2 FP64): the diagonals, r, the masks, the result and
+// the epilogue move as 16-byte words, and y at rows i0 + off + k comes from
+// the one or two aligned words that hold them (off mod R is the same in
+// every lane).  A word wholly outside [0, n) is not read: no row that reads
+// it has an entry there.  Each row's sum is formed exactly as in the kernel
+// above, so results are bit-identical.  Needs n % R == 0 and 16-byte
+// aligned vectors.
+template <typename T, bool UNIT, int D, bool EPI>
+__global__ __launch_bounds__(256) void tri_jacobi_dia_vec_kernel(int n, DiaOffsets off,
+                                                                 const uint8_t* __restrict__ mask,
+                                                                 const T* __restrict__ dia,
+                                                                 const T* __restrict__ dval,
+                                                                 const T* __restrict__ r,
+                                                                 const T* __restrict__ yin,
+                                                                 T* __restrict__ yout, T* __restrict__ y2,
+                                                                 const T* __restrict__ d2) {
+#pragma clang fp contract(off)
+  constexpr int R = Pack16<T>::R;
+  const int i0 = (xcd_block(blockIdx.x, gridDim.x) * 256 + (int)threadIdx.x) * R;
+  if (i0 >= n) return;
+  unsigned mk[R];
+  if constexpr (R == 4) {
+    const uint32_t m4 = *reinterpret_cast<const uint32_t*>(mask + i0);
+#pragma unroll
+    for (int k = 0; k < R; k++) mk[k] = (m4 >> (8 * k)) & 0xffu;
+  } else {
+    const uint16_t m2 = *reinterpret_cast<const uint16_t*>(mask + i0);
+#pragma unroll
+    for (int k = 0; k < R; k++) mk[k] = (m2 >> (8 * k)) & 0xffu;
+  }
+  Pack16<T> v[D];
+#pragma unroll
+  for (int q = 0; q < D; q++) v[q] = ld16(dia + (size_t)q * n + i0);
+  const Pack16<T> rr = ld16(r + i0);
+  T s[R];
+#pragma unroll
+  for (int k = 0; k < R; k++) s[k] = T(0);
+#pragma unroll
+  for (int q = 0; q < D; q++) {
+    const int sh = ((off.o[q] % R) + R) % R;
+    const int a = i0 + off.o[q] - sh;  // aligned word holding row i0 + off
+    Pack16<T> lo{}, hi{};
+    if (a >= 0 && a < n) lo = ld16(yin + a);
+    if (sh != 0 && a + R >= 0 && a + R < n) hi = ld16(yin + a + R);
+    T y[R];
+    window16<T>(lo, hi, sh, y);
+#pragma unroll
+    for (int k = 0; k < R; k++)
+      if ((mk[k] >> q) & 1u) s[k] = s[k] + v[q].v[k] * y[k];
+  }
+  Pack16<T> o;
+  if constexpr (UNIT) {
+#pragma unroll
+    for (int k = 0; k < R; k++) o.v[k] = rr.v[k] - s[k];
+  } else {
+    const Pack16<T> dv = ld16(dval + i0);
+#pragma unroll
+    for (int k = 0; k < R; k++) o.v[k] = (rr.v[k] - s[k]) / dv.v[k];
+  }
+  st16(yout + i0, o);
+  if constexpr (EPI) {
+    const Pack16<T> dd = ld16(d2 + i0);
+    Pack16<T> e;
+#pragma unroll
+    for (int k = 0; k < R; k++) e.v[k] = o.v[k] / dd.v[k];
+    st16(y2 + i0, e);
+  }
+}
+
+// Device-coherent 16-byte word: two 8-byte agent-scope accesses, which
+// bypass the per-CU and per-XCD caches that other workgroups do not share.
+template <typename T>
+__device__ __forceinline__ Pack16<T> ld16_coherent(const T* p) {
+  const uint64_t* q = reinterpret_cast<const uint64_t*>(p);
+  const uint64_t a = __hip_atomic_load(q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  const uint64_t b = __hip_atomic_load(q + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  Pack16<T> w;
+  __builtin_memcpy(&w.v[0], &a, 8);
+  __builtin_memcpy(reinterpret_cast<char*>(&w.v[0]) + 8, &b, 8);
+  return w;
+}
+template <typename T>
+__device__ __forceinline__ void st16_coherent(T* p, const Pack16<T>& w) {
+  uint64_t a, b;
+  __builtin_memcpy(&a, &w.v[0], 8);
+  __builtin_memcpy(&b, reinterpret_cast<const char*>(&w.v[0]) + 8, 8);
+  uint64_t* q = reinterpret_cast<uint64_t*>(p);
+  __hip_atomic_store(q, a, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  __hip_atomic_store(q + 1, b, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// S Jacobi sweeps of a triangular factor on its diagonal form in one
+// launch.  Tiles of 256 R rows (R = 16 / sizeof(T)) go to workgroups in
+// ticket order (an atomic counter): ascending for a lower factor, whose
+// rows read rows before them, descending for an upper one.  A tile keeps
+// its factor rows, masks and r in registers for all S sweeps.  Sweep 1
+// reads yin; sweep s > 1 of tile t reads sweep s-1 of its own rows from LDS
+// and of tiles t-K..t-1 (t+1..t+K upper; K = ceil(max |offset| / tile
+// rows)) from bufs[s-2], published by each tile's flag: every thread's device-coherent stores of
+// sweep s-1 complete, then the workgroup barrier, then the flag store.
+// The waiting tiles hold smaller tickets, so they started earlier and wait
+// only on earlier tickets in turn: every wait ends.  Each wait also gives up
+// after spin_limit polls (setting *err), so every wave exits whatever
+// happens.  Flags count up across launches (flag_base + s), so nothing is
+// reset between launches.  Sweep S writes yout (and, with EPI, y2 =
+// yout / d2).  yin must not alias yout, y2 or bufs: tiles finish in any
+// order.  Per row the arithmetic is that of the one-sweep kernels:
+// bit-identical results.
+template <typename T, bool UNIT, int D, bool EPI>
+__global__ __launch_bounds__(256) void tri_jacobi_dia_pipe_kernel(
+    int n, DiaOffsets off, int ntiles, int K, int upper, int S, const uint8_t* __restrict__ mask,
+    const T* __restrict__ dia, const T* __restrict__ dval, const T* __restrict__ r, const T* __restrict__ yin,
+    T* bufs, T* __restrict__ yout, T* __restrict__ y2, const T* __restrict__ d2, unsigned* flags,
+    unsigned* ctr, unsigned ticket_base, unsigned flag_base, int* err, int spin_limit) {
+#pragma clang fp contract(off)
+  constexpr int R = Pack16<T>::R;
+  constexpr int TR = 256 * R;
+  __shared__ int s_tile;
+  __shared__ Pack16<T> ys[256];  // this tile's previous sweep, word per thread
+  if (threadIdx.x == 0) {
+    const int tk = (int)(atomicAdd(ctr, 1u) - ticket_base);
+    s_tile = upper ? ntiles - 1 - tk : tk;
+  }
+  __syncthreads();
+  const int t = s_tile;
+  const int t0 = t * TR;  // the tile's first row
+  const int i0 = t0 + (int)threadIdx.x * R;
+  const bool act = t >= 0 && t < ntiles && i0 < n;
+  // word at aligned row a of the previous sweep: this tile's from LDS,
+  // another tile's through device-coherent loads
+  auto prev_word = [&](const T* src, int a) {
+    return (a >= t0 && a < t0 + TR) ? ys[(a - t0) / R] : ld16_coherent(src + a);
+  };
+  unsigned mk[R];
+  Pack16<T> v[D], rr{}, dv{};
+#pragma unroll
+  for (int k = 0; k < R; k++) mk[k] = 0u;
+  if (act) {
+    if constexpr (R == 4) {
+      const uint32_t m4 = *reinterpret_cast<const uint32_t*>(mask + i0);
+#pragma unroll
+      for (int k = 0; k < R; k++) mk[k] = (m4 >> (8 * k)) & 0xffu;
+    } else {
+      const uint16_t m2 = *reinterpret_cast<const uint16_t*>(mask + i0);
+#pragma unroll
+      for (int k = 0; k < R; k++) mk[k] = (m2 >> (8 * k)) & 0xffu;
+    }
+#pragma unroll
+    for (int q = 0; q < D; q++) v[q] = ld16(dia + (size_t)q * n + i0);
+    rr = ld16(r + i0);
+    if constexpr (!UNIT) dv = ld16(dval + i0);
+  }
+  for (int s = 1; s <= S; s++) {
+    const T* src = s == 1 ? yin : bufs + (size_t)(s - 2) * n;
+    if (s > 1) {
+      // the K neighbouring tiles on the dependent side have finished s-1
+      if (threadIdx.x < 64 && (int)threadIdx.x < K) {
+        const int tt = upper ? t + 1 + (int)threadIdx.x : t - 1 - (int)threadIdx.x;
+        if (tt >= 0 && tt < ntiles) {
+          const unsigned want = flag_base + (unsigned)(s - 1);
+          int spins = 0;
+          while ((int)(__hip_atomic_load(flags + tt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) - want) < 0) {
+            if (++spins > spin_limit) {
+              __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+              break;
+            }
+            __builtin_amdgcn_s_sleep(2);
+          }
+        }
+      }
+      __syncthreads();
+    }
+    Pack16<T> o{};
+    if (act) {
+      T sum[R];
+#pragma unroll
+      for (int k = 0; k < R; k++) sum[k] = T(0);
+#pragma unroll
+      for (int q = 0; q < D; q++) {
+        const int sh = ((off.o[q] % R) + R) % R;
+        const int a = i0 + off.o[q] - sh;
+        Pack16<T> lo{}, hi{};
+        if (s == 1) {
+          if (a >= 0 && a < n) lo = ld16(src + a);
+          if (sh != 0 && a + R >= 0 && a + R < n) hi = ld16(src + a + R);
+        } else {
+          if (a >= 0 && a < n) lo = prev_word(src, a);
+          if (sh != 0 && a + R >= 0 && a + R < n) hi = prev_word(src, a + R);
+        }
+        T y[R];
+        window16<T>(lo, hi, sh, y);
+#pragma unroll
+        for (int k = 0; k < R; k++)
+          if ((mk[k] >> q) & 1u) sum[k] = sum[k] + v[q].v[k] * y[k];
+      }
+#pragma unroll
+      for (int k = 0; k < R; k++) o.v[k] = UNIT ? rr.v[k] - sum[k] : (rr.v[k] - sum[k]) / dv.v[k];
+    }
+    if (s < S) __syncthreads();  // every thread has read the previous sweep's LDS words
+    if (act) {
+      if (s == S) {
+        st16(yout + i0, o);
+        if constexpr (EPI) {
+          const Pack16<T> dd = ld16(d2 + i0);
+          Pack16<T> e;
+#pragma unroll
+          for (int k = 0; k < R; k++) e.v[k] = o.v[k] / dd.v[k];
+          st16(y2 + i0, e);
+        }
+      } else {
+        ys[threadIdx.x] = o;
+        st16_coherent(bufs + (size_t)(s - 1) * n + i0, o);
+      }
+    }
+    if (s < S) {
+      __builtin_amdgcn_s_waitcnt(0);  // this thread's stores have completed
+      __syncthreads();
+      if (threadIdx.x == 0 && t >= 0 && t < ntiles)
+        __hip_atomic_store(flags + t, flag_base + (unsigned)s, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+  }
+}
+
 template <typename T, bool UNIT>
 static void jacobi_dia(const TriPlan& P, const T* r, const T* yin, T* yout, T* y2, const T* d2,
                        TimedRegion& tr) {
-  const int nb = ceil_div(P.n, 256);
+  constexpr int R = Pack16<T>::R;
   const uint8_t* mk = P.dia.mask.as<uint8_t>();
   const T* dv = P.dia.val.as<T>();
   const T* dg = P.dval.as<T>();
   T* no_y2 = nullptr;
   const T* no_d2 = nullptr;
+  auto al16 = [](const void* p) { return ((uintptr_t)p & 15u) == 0; };
+  // R rows per thread where every vector is 16-byte aligned and R divides n
+  const bool vec = get_option("jacobi_dia_vec", 1) != 0 && P.n % R == 0 && al16(r) && al16(yin) &&
+                   al16(yout) && al16(dv) && (UNIT || al16(dg)) && (!y2 || (al16(y2) && al16(d2)));
+  const int nb = vec ? ceil_div(P.n / R, 256) : ceil_div(P.n, 256);
   switch (P.dia.nd) {
-#define LACE_DIA_CASE(D)                                                                                  \
-  case D:                                                                                                 \
-    if (y2)                                                                                               \
-      tr.launch(tri_jacobi_dia_kernel<T, UNIT, D, true>, dim3(nb), dim3(256), P.n, P.dia.off, mk, dv, dg, \
-                r, yin, yout, y2, d2);                                                                    \
-    else                                                                                                  \
-      tr.launch(tri_jacobi_dia_kernel<T, UNIT, D, false>, dim3(nb), dim3(256), P.n, P.dia.off, mk, dv,    \
-                dg, r, yin, yout, no_y2, no_d2);                                                          \
+#define LACE_DIA_CASE(D)                                                                                   \
+  case D:                                                                                                  \
+    if (vec && y2)                                                                                         \
+      tr.launch(tri_jacobi_dia_vec_kernel<T, UNIT, D, true>, dim3(nb), dim3(256), P.n, P.dia.off, mk, dv,  \
+                dg, r, yin, yout, y2, d2);                                                                 \
+    else if (vec)                                                                                          \
+      tr.launch(tri_jacobi_dia_vec_kernel<T, UNIT, D, false>, dim3(nb), dim3(256), P.n, P.dia.off, mk, dv, \
+                dg, r, yin, yout, no_y2, no_d2);                                                           \
+    else if (y2)                                                                                           \
+      tr.launch(tri_jacobi_dia_kernel<T, UNIT, D, true>, dim3(nb), dim3(256), P.n, P.dia.off, mk, dv, dg,  \
+                r, yin, yout, y2, d2);                                                                     \
+    else                                                                                                   \
+      tr.launch(tri_jacobi_dia_kernel<T, UNIT, D, false>, dim3(nb), dim3(256), P.n, P.dia.off, mk, dv,     \
+                dg, r, yin, yout, no_y2, no_d2);                                                           \
     break;
     LACE_DIA_CASE(1)
     LACE_DIA_CASE(2)
@@ -635,6 +914,104 @@ static int jacobi_sweep(const TriPlan& P, const T* val, const T* r, const T* yin
   return DEV_SUCCESS;
 }
 
+// Tile count, dependency reach in tiles, and whether the pipelined sweeps
+// apply to P with the value array val: diagonal form current, every offset
+// on the factor's side of the diagonal, n a multiple of the 16-byte word,
+// reach within one wavefront of flags.
+template <typename T>
+static bool jacobi_pipe_shape(const TriPlan& P, const void* val, int& ntiles, int& K) {
+  constexpr int R = Pack16<T>::R;
+  constexpr int TR = 256 * R;
+  if (P.dia.nd <= 0 || P.strict_src != val || P.n <= 0 || P.n % R != 0) return false;
+  int reach = 0;
+  for (int q = 0; q < P.dia.nd; q++) {
+    const int o = P.dia.off.o[q];
+    if (o == 0 || (o > 0) != P.upper) return false;
+    reach = std::max(reach, std::abs(o));
+  }
+  ntiles = ceil_div(P.n, TR);
+  K = ceil_div(reach, TR);
+  return K <= 64;
+}
+
+// Whether S sweeps of P from yin can run pipelined: the shape above, S in
+// 2..32, and every vector 16-byte aligned.
+template <typename T>
+static bool jacobi_pipe_ready(const TriPlan& P, const void* val, const T* r, const T* yin, const T* yout,
+                              int S, const T* y2, const T* d2) {
+  int ntiles = 0, K = 0;
+  if (S < 2 || S > 32 || !jacobi_pipe_shape<T>(P, val, ntiles, K)) return false;
+  auto al16 = [](const void* p) { return ((uintptr_t)p & 15u) == 0; };
+  return al16(r) && al16(yin) && al16(yout) && al16(P.dia.val.p) && (P.unit_diag || al16(P.dval.p)) &&
+         (!y2 || (al16(y2) && al16(d2)));
+}
+
+// S sweeps of P from yin in one launch (tri_jacobi_dia_pipe_kernel); yout
+// gets sweep S and, with y2, y2 = yout / d2.  jacobi_pipe_ready must hold.
+template <typename T, bool UNIT>
+static int jacobi_dia_pipe(const TriPlan& P, const T* val, const T* r, const T* yin, T* yout, int S, T* y2,
+                           const T* d2, int* err) {
+  int ntiles = 0, K = 0;
+  LACE_REQUIRE(jacobi_pipe_ready<T>(P, val, r, yin, yout, S, y2, d2) && jacobi_pipe_shape<T>(P, val, ntiles, K),
+               DEV_ERR_ILLEGAL_VALUE, "pipelined Jacobi: shape not supported");
+  const T* dv = P.dia.val.as<T>();
+  const T* dg = P.dval.as<T>();
+  const size_t need = sizeof(T) * (size_t)(S - 1) * P.n;
+  if (P.pipe_buf.bytes < need) LACE_TRY(P.pipe_buf.alloc(need));
+  if (!P.pipe_flags.p || P.pipe_flags.bytes < sizeof(unsigned) * ntiles) {
+    LACE_TRY(P.pipe_flags.alloc(sizeof(unsigned) * ntiles));
+    LACE_TRY(P.pipe_ctr.alloc(sizeof(unsigned)));
+    LACE_HIP(hipMemsetAsync(P.pipe_flags.p, 0, sizeof(unsigned) * ntiles, stream()));
+    LACE_HIP(hipMemsetAsync(P.pipe_ctr.p, 0, sizeof(unsigned), stream()));
+    P.pipe_tickets = 0;
+    P.pipe_epoch = 0;
+  }
+  // flag values of this launch: base + 1 .. base + S - 1, above every value
+  // an earlier launch left
+  P.pipe_epoch += 64;
+  const unsigned tb = P.pipe_tickets, fb = P.pipe_epoch;
+  P.pipe_tickets += (unsigned)ntiles;
+  // algorithmic bytes: the factor's diagonals and masks (and its diagonal),
+  // r and yin once, each intermediate sweep written and read once, yout
+  // (and the epilogue's d2 and y2)
+  const double es = sizeof(T), n = P.n;
+  double bytes = P.dia.bytes(sizeof(T)) + (UNIT ? 0.0 : es * n) + 2.0 * es * n + 2.0 * es * n * (S - 1) +
+                 es * n + (y2 ? 2.0 * es * n : 0.0);
+  TimedRegion tr(P.upper ? "trsv_jacobi_pipe_upper" : "trsv_jacobi_pipe_lower", 0.0, bytes, true);
+  const uint8_t* mk = P.dia.mask.as<uint8_t>();
+  T* bufs = P.pipe_buf.as<T>();
+  unsigned* fl = P.pipe_flags.as<unsigned>();
+  unsigned* ct = P.pipe_ctr.as<unsigned>();
+  T* no_y2 = nullptr;
+  const T* no_d2 = nullptr;
+  switch (P.dia.nd) {
+#define LACE_PIPE_CASE(D)                                                                                  \
+  case D:                                                                                                  \
+    if (y2)                                                                                                \
+      tr.launch(tri_jacobi_dia_pipe_kernel<T, UNIT, D, true>, dim3(ntiles), dim3(256), P.n, P.dia.off,     \
+                ntiles, K, (int)P.upper, S, mk, dv, dg, r, yin, bufs, yout, y2, d2, fl, ct, tb, fb, err,   \
+                g_spin_limit);                                                                             \
+    else                                                                                                   \
+      tr.launch(tri_jacobi_dia_pipe_kernel<T, UNIT, D, false>, dim3(ntiles), dim3(256), P.n, P.dia.off,    \
+                ntiles, K, (int)P.upper, S, mk, dv, dg, r, yin, bufs, yout, no_y2, no_d2, fl, ct, tb, fb,  \
+                err, g_spin_limit);                                                                        \
+    break;
+    LACE_PIPE_CASE(1)
+    LACE_PIPE_CASE(2)
+    LACE_PIPE_CASE(3)
+    LACE_PIPE_CASE(4)
+    LACE_PIPE_CASE(5)
+    LACE_PIPE_CASE(6)
+    LACE_PIPE_CASE(7)
+    LACE_PIPE_CASE(8)
+#undef LACE_PIPE_CASE
+    default:
+      break;
+  }
+  LACE_HIP(hipGetLastError());
+  return DEV_SUCCESS;
+}
+
 template <typename T>
 int tri_solve_jacobi(const TriPlan& P, const T* val, const T* r, T* y, T* tmp, int sweeps, T* epi_y,
                      const T* epi_d, bool* epi_done, bool first_done) {
@@ -777,10 +1154,27 @@ static int apply_t(ParILU* P, lace_trsv_kind_t kind, int sweeps, int trans, cons
     return DEV_SUCCESS;
   }
   // Jacobi: work holds the intermediate vector, work2 the ping-pong buffer.
+  T* tmp2 = P->work2.as<T>();
+  // Pipelined form, on request (unit lower factor, then a non-unit upper
+  // one, both on the diagonal form): the first solve's sweeps 2..k from
+  // y1 = r in one launch, whose last sweep also writes the second solve's
+  // first sweep tmp / diag into work2; then the second solve's sweeps 2..k
+  // from work2 into z in one launch.  Off by default: on the 128^3 FP32
+  // factors one such launch takes 75 us against 45 us for the four
+  // one-sweep launches it replaces (each tile's sweeps wait on a chain of
+  // device-coherent store, flag and load round trips).
+  const T* d2 = second.dval.as<T>();
+  if (sweeps >= 3 && first.unit_diag && !second.unit_diag && get_option("jacobi_pipe", 0) != 0 &&
+      jacobi_pipe_ready<T>(first, v1, r, r, tmp, sweeps - 1, tmp2, d2) &&
+      jacobi_pipe_ready<T>(second, v2, tmp, tmp2, z, sweeps - 1, nullptr, nullptr)) {
+    int* err = P->syncfree_err.as<int>() + 1;
+    LACE_TRY((jacobi_dia_pipe<T, true>(first, v1, r, r, tmp, sweeps - 1, tmp2, d2, err)));
+    LACE_TRY((jacobi_dia_pipe<T, false>(second, v2, tmp, tmp2, z, sweeps - 1, nullptr, nullptr, err)));
+    return DEV_SUCCESS;
+  }
   // For an odd sweep count the second solve's first sweep (tmp / diag,
   // written to z) rides on the first solve's last sweep when that runs on
   // the diagonal form; z is then distinct from every buffer that sweep reads.
-  T* tmp2 = P->work2.as<T>();
   const bool fuse = first.unit_diag && !second.unit_diag && sweeps % 2 == 1 &&
                     second.strict_src == (const void*)v2 && (const void*)z != (const void*)r &&
                     get_option("jacobi_fuse", 1) != 0;
@@ -814,8 +1208,11 @@ int lace_parilu_apply(lace_parilu* Ph, lace_trsv_kind_t kind, int jacobi_sweeps,
 int lace_syncfree_error(lace_parilu* Ph, int* err) {
   LACE_TRY(ensure_init());
   auto* P = reinterpret_cast<ParILU*>(Ph);
-  LACE_HIP(hipMemcpyAsync(err, P->syncfree_err.p, sizeof(int), hipMemcpyDeviceToHost, stream()));
+  // slot 0: the sync-free solve, slot 1: the pipelined Jacobi sweeps
+  int e[2] = {0, 0};
+  LACE_HIP(hipMemcpyAsync(e, P->syncfree_err.p, sizeof(e), hipMemcpyDeviceToHost, stream()));
   LACE_HIP(hipStreamSynchronize(stream()));
+  *err = e[0] | e[1];
   return DEV_SUCCESS;
 }
 

@@ -1,6 +1,10 @@
 """Storage forms of the products and Jacobi sweeps.  The diagonal (DIA),
 ELLPACK and CSR forms give bit-identical results (the oracle's, in FP64);
-which forms a matrix gets follows its pattern."""
+which forms a matrix gets follows its pattern.  The Jacobi sweeps on the
+diagonal form run pipelined (all sweeps of a factor in one launch) or one
+launch per sweep with 16 bytes of rows or one row per thread, also
+bit-identical."""
+import ctypes as C
 import os
 
 import numpy as np
@@ -23,6 +27,8 @@ def forms(gpu):
     for kind in ("spmv", "jacobi"):
         set_forms(kind, "dia")
     gpu.check(gpu.lib().lace_set_option(b"jacobi_fuse", 1))
+    gpu.check(gpu.lib().lace_set_option(b"jacobi_dia_vec", 1))
+    gpu.check(gpu.lib().lace_set_option(b"jacobi_pipe", 0))
 
 
 def _holes(orc, A, frac, seed):
@@ -43,10 +49,15 @@ def problems(orc):
         "convdiff16": c,
         "convdiff16_holes": _holes(orc, c, 0.1, 5),
         "trefethen20": orc.read_mtx(os.path.join(GOLDEN, "Trefethen_20.mtx")),
+        # offsets 1, 15, 180: y windows at every shift within a 16-byte word
+        "convdiff15x12x10": orc.convdiff3d(15, 12, 10),
+        # n = 2730: not a multiple of 4 rows (FP32 keeps one row per thread)
+        "convdiff15x14x13": orc.convdiff3d(15, 14, 13),
     }
 
 
-NAMES = ["laplace64", "convdiff16", "convdiff16_holes", "trefethen20"]
+NAMES = ["laplace64", "convdiff16", "convdiff16_holes", "trefethen20", "convdiff15x12x10",
+         "convdiff15x14x13"]
 
 
 def test_forms_follow_the_pattern(gpu, orc):
@@ -87,8 +98,39 @@ def test_jacobi_forms_bitexact(gpu, orc, forms, name, prec):
             forms("jacobi", form)
             for fuse in (1, 0):
                 gpu.check(gpu.lib().lace_set_option(b"jacobi_fuse", fuse))
-                got = P.apply(r, gpu.TRSV_JACOBI, sweeps=sweeps)
-                assert np.array_equal(got, ref), (form, fuse, sweeps)
+                # diagonal form: all sweeps of a factor in one pipelined
+                # launch, or a launch per sweep with a 16-byte word of rows
+                # per thread or one row
+                for pipe, vec in (((1, 1), (0, 1), (0, 0)) if form == "dia" else ((0, 1),)):
+                    gpu.check(gpu.lib().lace_set_option(b"jacobi_pipe", pipe))
+                    gpu.check(gpu.lib().lace_set_option(b"jacobi_dia_vec", vec))
+                    got = P.apply(r, gpu.TRSV_JACOBI, sweeps=sweeps)
+                    assert np.array_equal(got, ref), (form, fuse, pipe, vec, sweeps)
+
+
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+def test_pipelined_jacobi_many_tiles(gpu, orc, forms, prec):
+    """The pipelined sweeps with many tiles waiting on one another: 64^3
+    (256 FP32 tiles of 1024 rows, 512 FP64 tiles; the z-offset 4096 reaches
+    4 or 8 tiles back), against a launch per sweep, bit for bit, for
+    several sweep counts and repeated applies (flags count up across
+    launches); no wait gave up."""
+    A = orc.convdiff3d(64, 64, 64)
+    lp = gpu.LACE_F64 if prec == "f64" else gpu.LACE_F32
+    dt = np.float64 if prec == "f64" else np.float32
+    P = gpu.ParILU(gpu.HostCSR.from_csr(A), lp)
+    P.sweeps(3)
+    rng = np.random.default_rng(17)
+    for sweeps in (3, 5, 4, 7, 5):
+        r = rng.standard_normal(A.n_rows).astype(dt)
+        gpu.check(gpu.lib().lace_set_option(b"jacobi_pipe", 0))
+        ref = P.apply(r, gpu.TRSV_JACOBI, sweeps=sweeps)
+        gpu.check(gpu.lib().lace_set_option(b"jacobi_pipe", 1))
+        for _ in range(2):
+            assert np.array_equal(P.apply(r, gpu.TRSV_JACOBI, sweeps=sweeps), ref), sweeps
+    err = C.c_int(-1)
+    gpu.check(gpu.lib().lace_syncfree_error(P.h, C.byref(err)))
+    assert err.value == 0
 
 
 def _csr_order_sum(A, x, dt):
