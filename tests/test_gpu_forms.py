@@ -191,4 +191,3 @@ def test_solve_is_identical_across_forms(gpu, orc, forms):
         its.append(st.iterations)
     assert its[0] == its[1] == its[2]
     assert np.array_equal(xs[0], xs[1]) and np.array_equal(xs[0], xs[2])
-
