@@ -347,8 +347,10 @@ def bench_bcsr(args) -> int:
     # sweeps and SpMV bandwidth
     for name, prec in (("f32", L.LACE_F32), ("bf16", L.LACE_BF16), ("f64", L.LACE_F64)):
         # forms of the paired sweep: 0 the default, 2 the device-side merge of
-        # the two block rows, 3 set-up tables, 4 tables + LDS staging
-        for variant, tag in ((0, ""), (2, "_merge"), (3, "_tables"), (4, "_tables_lds")):
+        # the two block rows, 3 set-up tables, 4 tables + LDS staging, 5
+        # tables + LDS staging of L, U and U_JJ only
+        for variant, tag in ((0, ""), (2, "_merge"), (3, "_tables"), (4, "_tables_lds"),
+                             (5, "_tables_lds_lu")):
             P = L.BILU(H, prec)
             if variant:
                 P.set_variant(variant)

@@ -442,13 +442,13 @@ int lace_bilu_apply(lace_bilu *B, lace_trsv_kind_t kind, int jacobi_sweeps,
 int lace_bilu_levels(const lace_bilu *B, int *nlev_L, int *nlev_U);
 /* Sweep kernel: 0 (default) pairs L_IJ with U_JI in one lane group when A's
  * block pattern is structurally symmetric, else one group per block of A;
- * 1 always takes the per-block kernel.  For FP32 and 16-bit storage the
- * paired kernel reads per-position tables built at set-up and stages each
- * workgroup's blocks through LDS (where they align to 16 bytes and fit
- * 64 KiB); otherwise it merges the two block rows on the device.  2-4 force
+ * 1 always takes the per-block kernel.  The paired kernel reads
+ * per-position tables built at set-up and stages each workgroup's L and U
+ * blocks and U_JJ rows through LDS (where they align to 16 bytes and fit
+ * 64 KiB); otherwise it merges the two block rows on the device.  2-5 force
  * a form of the paired kernel: 2 the merge, 3 the tables without staging,
- * 4 the tables with staging wherever it applies.  All give identical
- * factors. */
+ * 4 the tables with A's blocks staged too, 5 the tables with L, U and U_JJ
+ * staged.  All give identical factors. */
 int lace_bilu_set_variant(lace_bilu *B, int variant);
 /* 1 if A's block pattern is structurally symmetric. */
 int lace_bilu_is_symmetric(const lace_bilu *B);

@@ -428,7 +428,7 @@ __device__ __forceinline__ void unstage_range(const T* src, T* __restrict__ dst,
 // it; each lane writes its new rows over the ones it read, and the blocks go
 // out as 16-byte words, so the stride-BS row accesses become LDS accesses.
 // Same arithmetic in the same order as the kernel above: identical results.
-template <int BS, typename TC, typename TS, bool STEP, bool STAGE>
+template <int BS, typename TC, typename TS, bool STEP, int STAGE>
 __global__ __launch_bounds__(kBBS) void bilu_sym_sweep_meta_kernel(
     int np, const int4* __restrict__ meta, const int* __restrict__ pend, const int4* __restrict__ pairs,
     const TC* __restrict__ AL, const TC* __restrict__ AU, const TS* __restrict__ Lin,
@@ -436,17 +436,23 @@ __global__ __launch_bounds__(kBBS) void bilu_sym_sweep_meta_kernel(
     const float* __restrict__ rs, double* __restrict__ partial) {
   constexpr int LB = BS * BS;
   constexpr int PPB = (kBBS / 64) * (64 / BS);  // positions per workgroup
+  // STAGE 2: all four block sets and U_JJ through LDS; 1: L, U and U_JJ
+  // (A's blocks read directly); 0: none
   constexpr int SE = STAGE ? PPB * LB : 1;
-  __shared__ __align__(16) TC shAL[SE];
-  __shared__ __align__(16) TC shAU[SE];
+  constexpr int SEA = STAGE == 2 ? PPB * LB : 1;
+  constexpr bool SA = STAGE == 2;
+  __shared__ __align__(16) TC shAL[SEA];
+  __shared__ __align__(16) TC shAU[SEA];
   __shared__ __align__(16) TS shL[SE];
   __shared__ __align__(16) TS shU[SE];
   __shared__ TC shUjj[SE];
   const int p0 = xcd_block(blockIdx.x, gridDim.x) * PPB;
   const int ne = min(PPB, np - p0) * LB;
-  if constexpr (STAGE) {
+  if constexpr (SA) {
     stage_range<TC>(AL + (size_t)p0 * LB, shAL, ne);
     stage_range<TC>(AU + (size_t)p0 * LB, shAU, ne);
+  }
+  if constexpr (STAGE) {
     stage_range<TS>(Lin + (size_t)p0 * LB, shL, ne);
     stage_range<TS>(Uin + (size_t)p0 * LB, shU, ne);
   }
@@ -480,8 +486,8 @@ __global__ __launch_bounds__(kBBS) void bilu_sym_sweep_meta_kernel(
     TC aL[BS], aU[BS], lold[BS], uold[BS];
 #pragma unroll
     for (int c = 0; c < BS; c++) {
-      aU[c] = STAGE ? shAU[row0 + c] : AU[grow + c];
-      aL[c] = strict ? (STAGE ? shAL[row0 + c] : AL[grow + c]) : TC(0);
+      aU[c] = SA ? shAU[row0 + c] : AU[grow + c];
+      aL[c] = strict ? (SA ? shAL[row0 + c] : AL[grow + c]) : TC(0);
       uold[c] = dequant<TC, TS>(STAGE ? shU[row0 + c] : Uin[grow + c], sJ);
       if (strict) lold[c] = dequant<TC, TS>(STAGE ? shL[row0 + c] : Lin[grow + c], sI);
     }
@@ -583,13 +589,10 @@ constexpr bool sym_lds_ok() {
   return (e * sizeof(TC)) % 16 == 0 && (e * sizeof(TS)) % 16 == 0 &&
          e * (3 * sizeof(TC) + 2 * sizeof(TS)) <= 65536;
 }
-// Measured on 5x5 blocks of a 64^3 grid: staging saves 8 % (FP32) and 14 %
-// (BF16) of the sweep against the merge kernel; FP64 blocks take 48 KB of
-// LDS per workgroup, and the lower occupancy makes the staged sweep 13 %
-// slower than the merge there.
-template <typename TC>
-constexpr bool sym_lds_pays() {
-  return sizeof(TC) == 4;
+template <int BS, typename TC, typename TS>
+constexpr bool sym_lds_lu_ok() {
+  constexpr int e = (kBBS / 64) * (64 / BS) * BS * BS;
+  return (e * sizeof(TS)) % 16 == 0 && e * (sizeof(TC) + 2 * sizeof(TS)) <= 65536;
 }
 
 // General sweep: group (k, r) takes block k = (I, J) of A in BCSR order,
@@ -1015,7 +1018,7 @@ int BILU::upload_general() {
 }
 
 int BILU::set_variant(int v) {
-  LACE_REQUIRE(v >= 0 && v <= 4, DEV_ERR_ILLEGAL_VALUE, "block ParILU: variant %d", v);
+  LACE_REQUIRE(v >= 0 && v <= 5, DEV_ERR_ILLEGAL_VALUE, "block ParILU: variant %d", v);
   variant = v;
   if (v == 1) LACE_TRY(upload_general());
   return DEV_SUCCESS;
@@ -1060,25 +1063,35 @@ static int sweep_bs(BILU* B, bool with_step) {
                                        B->Uv[c].as<TS>(), B->Lv[o].as<TS>(), B->Uv[o].as<TS>(),
                                        B->rs.as<float>(), B->partial.as<double>());
     };
-    // variant 0 takes the tables with staging where staging applies and
-    // pays, else the device-side merge; 2 the merge, 3 the tables, 4 the
-    // tables staged wherever staging applies
+    // variant 0 takes the tables with L, U and U_JJ staged where that
+    // staging applies, else the device-side merge; 2 the merge, 3 the
+    // tables, 4 the tables with every block set staged, 5 with L, U and
+    // U_JJ staged (wherever the staging applies).  Measured on 5x5 blocks of
+    // a 64^3 grid (FP32 / BF16 / FP64 storage): merge 235 / 237 / 434 us,
+    // tables 251 / 257 / 464, all staged 219 / 206 / 493 (48 KB of LDS per
+    // workgroup in FP64), L, U and U_JJ staged 204 / 191 / 399.
     constexpr bool can_stage = sym_lds_ok<BS, TC, TS>();
-    const bool tables = B->variant == 3 || B->variant == 4 || (B->variant == 0 && can_stage && sym_lds_pays<TC>());
+    constexpr bool can_stage_lu = sym_lds_lu_ok<BS, TC, TS>();
+    const bool tables = B->variant >= 3 || (B->variant == 0 && can_stage_lu);
     if (tables) {
-      const bool stage = can_stage && B->variant != 3;
+      const bool stage = can_stage && B->variant == 4;
+      const bool stage_lu = can_stage_lu && (B->variant == 0 || B->variant == 5);
       auto run = [&](auto kern) {
         kern<<<nb, kBBS, 0, stream()>>>(B->nnzL, B->smeta.as<int4>(), B->spend.as<int>(),
                                          B->spairs.as<int4>(), B->AL.as<TC>(), B->AU.as<TC>(),
                                          B->Lv[c].as<TS>(), B->Uv[c].as<TS>(), B->Lv[o].as<TS>(),
                                          B->Uv[o].as<TS>(), B->rs.as<float>(), B->partial.as<double>());
       };
+      constexpr int full = can_stage ? 2 : 0, lu = can_stage_lu ? 1 : 0;
       if (stage) {
-        if (with_step) run(bilu_sym_sweep_meta_kernel<BS, TC, TS, true, can_stage>);
-        else run(bilu_sym_sweep_meta_kernel<BS, TC, TS, false, can_stage>);
+        if (with_step) run(bilu_sym_sweep_meta_kernel<BS, TC, TS, true, full>);
+        else run(bilu_sym_sweep_meta_kernel<BS, TC, TS, false, full>);
+      } else if (stage_lu) {
+        if (with_step) run(bilu_sym_sweep_meta_kernel<BS, TC, TS, true, lu>);
+        else run(bilu_sym_sweep_meta_kernel<BS, TC, TS, false, lu>);
       } else {
-        if (with_step) run(bilu_sym_sweep_meta_kernel<BS, TC, TS, true, false>);
-        else run(bilu_sym_sweep_meta_kernel<BS, TC, TS, false, false>);
+        if (with_step) run(bilu_sym_sweep_meta_kernel<BS, TC, TS, true, 0>);
+        else run(bilu_sym_sweep_meta_kernel<BS, TC, TS, false, 0>);
       }
     } else {
       if (with_step) go(bilu_sym_sweep_kernel<BS, TC, TS, true>);
