@@ -536,7 +536,8 @@ def main() -> int:
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: --gpus {args.gpus} is not supported; this implementation runs on one "
-              "GPU (multi-GPU sharding of the path is described in DESIGN.md, not built)",
+              "GPU (the path would shard by z-slabs of the grid with one plane halo per sweep or "
+              "product; that is not built)",
               file=sys.stderr)
         return 2
     if args.set:

@@ -21,7 +21,8 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 CFLAGS = [
     "-x", "hip", "--offload-arch=gfx950", "-O3", "-fPIC", "-std=c++17",
     "-Wall", "-Wno-unused-result", "-Wno-unused-function",
-    # keep the compiler from forming packed FP32 ops
+    # keep the compiler from forming packed FP32 ops: some of their forms
+    # have given silently wrong results on this hardware
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-fvisibility=hidden", "-DLACE_BUILD",
 ]
