@@ -152,8 +152,9 @@ def pmc_traffic():
 
 
 FLUSH_BYTES = 1 << 30
-# events around every k-th launch of each hot region in the timed steps
-TIMING_SAMPLE = 8
+# events around the first four and then every k-th launch of each hot
+# region in the timed steps
+TIMING_SAMPLE = 16
 # bytes read (and written) by the reference copy: 2 x 120 MiB ~ one FP32 sweep
 COPY_BYTES = 120 << 20
 

@@ -603,7 +603,8 @@ int lace_block_panel_qr(int n, int r0, const double *W, double *Y, double *Tb, d
  * 70,81,196,231,707) -- "FGMRES_Householders_partrsv_search(j) = |g_j|;" etc.,
  * the log format its post-processing scripts read; 0 silences them.
  * "timing_sample" (>= 1, default 1): the timing regions below record events
- * around every k-th entry of each region only. */
+ * around the first four entries of each region after a reset and every
+ * k-th entry after that only. */
 int lace_set_option(const char *name, int value);
 
 /* Kernel timing by class, HIP events on the library stream. */

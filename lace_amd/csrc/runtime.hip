@@ -99,6 +99,10 @@ static std::map<std::string, TimingEntry> g_timing;
 static bool g_timing_on = false;
 static std::string g_timing_only;  // empty: time every region
 static int g_timing_sample = 1;    // time every k-th entry of each region
+// The first entries of each region since the last reset are always timed,
+// so a region entered only a few times still gets several samples.
+constexpr long kTimingFirst = 4;
+static bool sampled(long call) { return call < kTimingFirst || call % g_timing_sample == 0; }
 // Events are reused: creating two per timed region costs several
 // microseconds of host time per launch, which a launch-bound solve loop
 // would otherwise carry into its wall time.
@@ -135,7 +139,7 @@ TimedRegion::TimedRegion(const char* n) : name(n) {
   std::lock_guard<std::mutex> lk(g_tmu);
   if (!timing_selected(n)) return;
   TimingEntry& e = g_timing[name];
-  if (e.calls++ % g_timing_sample != 0) return;
+  if (!sampled(e.calls++)) return;
   a = pooled_event();
   b = pooled_event();
   if (!a || !b) {
@@ -158,7 +162,7 @@ TimedRegion::TimedRegion(const char* n, double flops_, double bytes_, bool kerne
   std::lock_guard<std::mutex> lk(g_tmu);
   if (!timing_selected(n)) return;
   TimingEntry& e = g_timing[name];
-  if (e.calls++ % g_timing_sample != 0) return;
+  if (!sampled(e.calls++)) return;
   a = pooled_event();
   b = pooled_event();
   if (!a || !b) {
