@@ -66,13 +66,53 @@ def timed_solves(L, S, db, dx, zeros, k, flush=None):
     return out
 
 
+def host_cpus():
+    """Logical CPUs this process may run on, and the host's topology from
+    lscpu (model, sockets, physical cores per socket, threads per core)."""
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        usable = os.cpu_count() or 1
+    topo = {"usable_logical_cpus": usable, "host_logical_cpus": os.cpu_count()}
+    try:
+        import subprocess
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=20).stdout
+        keys = {"Model name": "cpu_model", "Socket(s)": "sockets",
+                "Core(s) per socket": "cores_per_socket", "Thread(s) per core": "threads_per_core"}
+        for line in out.splitlines():
+            k, _, v = line.partition(":")
+            if k.strip() in keys:
+                v = v.strip()
+                topo[keys[k.strip()]] = int(v) if v.isdigit() else v
+    except (OSError, ValueError):
+        pass
+    if "cpu_model" not in topo:
+        topo["cpu_model"] = _cpu_model()
+    # physical cores among the usable CPUs: distinct (package, core) pairs
+    try:
+        cores = set()
+        for c in os.sched_getaffinity(0):
+            base = f"/sys/devices/system/cpu/cpu{c}/topology/"
+            with open(base + "physical_package_id") as f1, open(base + "core_id") as f2:
+                cores.add((f1.read().strip(), f2.read().strip()))
+        topo["usable_physical_cores"] = len(cores)
+    except (AttributeError, OSError):
+        pass
+    return topo
+
+
 def cpu_baseline(row, col, val, b, threads):
     """The reference algorithm on the host cores: in-place OpenMP ParILU v0.3
     (3 sweeps) then Householder FGMRES(50) with exact triangular solves, to
     1e-8 relative (the oracle's restatement; the reference's MKL build
-    cannot run here)."""
+    cannot run here).  OpenMP threads are bound one per physical core where
+    the usable CPUs allow it (OMP_PLACES=cores, OMP_PROC_BIND=close, set
+    before the oracle's OpenMP runtime starts)."""
+    os.environ["OMP_NUM_THREADS"] = str(threads)
+    os.environ["OMP_PLACES"] = "cores"
+    os.environ["OMP_PROC_BIND"] = "close"
     import oracle as O
-    os.environ.setdefault("OMP_NUM_THREADS", str(threads))
+    topo = host_cpus()
     A = O.CSR(len(row) - 1, len(row) - 1, row, col, val)
     t0 = time.perf_counter()
     st, sweeps, _, _ = O.parilu_v03_omp(A, 3)
@@ -86,7 +126,13 @@ def cpu_baseline(row, col, val, b, threads):
            "sample": (f"full solve on the same 128^3 problem: ParILU v0.3 in-place OpenMP "
                       f"({sweeps} sweeps, {t1 - t0:.2f} s) + Householder FGMRES(50) with exact "
                       f"substitution, {r.searches} iterations to rel. residual {rel:.2e}"),
-           "cpu_model": _cpu_model(), "host_threads": os.cpu_count()}
+           "omp": {"OMP_NUM_THREADS": threads, "OMP_PLACES": "cores", "OMP_PROC_BIND": "close"},
+           **topo}
+    # cores: the physical cores the threads run on (one thread per core
+    # where the usable CPUs hold that many cores, else SMT siblings share)
+    phys = topo.get("usable_physical_cores") or topo["usable_logical_cpus"]
+    out["cores"] = min(threads, phys)
+    out["threads"] = threads
     # the deterministic one-thread run (in-place sweeps in row order) on a
     # bounded sample: the three sweeps and one FGMRES(50) cycle, scaled to
     # the iteration count of the full solve above
@@ -133,6 +179,8 @@ SPMV32_KERNEL = "spmv_dia_kernel<float, float, float>"
 KERNEL_CLASSES = {
     SWEEP_KERNEL: ["parilu_sweep"],
     JACOBI_KERNEL: ["trsv_jacobi_sweep_lower", "trsv_jacobi_sweep_upper"],
+    # several sweeps per launch (option jacobi_pipe=1)
+    "tri_jacobi_dia_pipe_kernel<float>": ["trsv_jacobi_pipe_lower", "trsv_jacobi_pipe_upper"],
     SPMV64_KERNEL: ["spmv"],
     SPMV32_KERNEL: ["spmv_f32a"],
 }
@@ -228,9 +276,15 @@ def cache_rates(L, H, n, flush, reps=10):
             L.check(lib.lace_device_synchronize())
             L.check(lib.lace_timing_only(None))
             # one launch per region per call: the call's kernel time and
-            # bytes are the sums of the regions' per-launch averages
-            us = sum(L.timing(rg)[0] / L.timing(rg)[1] for rg in regions) * 1e3
-            nbytes = sum(L.timing_work(rg)[1] / L.timing(rg)[1] for rg in regions)
+            # bytes are the sums of the regions' per-launch averages (an
+            # option can route a call to other regions: those record none)
+            timed = [rg for rg in regions if L.timing(rg)[1] > 0]
+            if not timed:
+                per[f"{label}_us"] = None
+                nbytes = 0.0
+                continue
+            us = sum(L.timing(rg)[0] / L.timing(rg)[1] for rg in timed) * 1e3
+            nbytes = sum(L.timing_work(rg)[1] / L.timing(rg)[1] for rg in timed)
             per[f"{label}_us"] = round(us, 2)
             per[f"{label}_GBps"] = round(nbytes / (us * 1e-6) / 1e9, 1)
         per["bytes_per_call"] = round(nbytes)
@@ -634,7 +688,8 @@ def main() -> int:
                          "bytes_per_launch": round(by / cnt), "avg_launch_us": round(avg_s * 1e6, 2),
                          "launches_timed": cnt, "launches": sum(calls[r] for r in regions),
                          "total_ms_in_timed_steps": round(est_ms, 3)}
-    dominant = max(kernels, key=lambda k: kernels[k]["total_ms_in_timed_steps"])
+    dominant = (max(kernels, key=lambda k: kernels[k]["total_ms_in_timed_steps"])
+                if kernels else None)
     sw = kernels.get(SWEEP_KERNEL, {})
     achieved = sw.get("achieved", float("nan"))
     P64 = L.ParILU(H, L.LACE_F64)
@@ -651,7 +706,7 @@ def main() -> int:
 
     cpu = None
     if not args.no_cpu_baseline:
-        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        threads = args.cpu_threads or min(16, host_cpus()["usable_logical_cpus"])
         cpu = cpu_baseline(row, col, val, b, threads)
 
     out = {
@@ -683,7 +738,7 @@ def main() -> int:
         "all_converged": ok,
         "parilu_sweep_GBps": {"f32": round(achieved, 1), "f64": round(ach64, 1)},
         # the kernel with the most time in the timed mixed steps
-        "roofline": dict(kernel=dominant, **kernels[dominant]),
+        "roofline": dict(kernel=dominant, **kernels[dominant]) if dominant else None,
         "kernels": kernels,
         # value and ms_per_step: steps run back to back (each finds what the
         # previous one left in the 256 MB Infinity Cache); cold: each step

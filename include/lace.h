@@ -189,6 +189,12 @@ int data_zwrite_dense(data_d_matrix A, const char *filename);
 /* dy <- alpha dx + dy on host arrays: blas/zaxpy.cpp:52-59. */
 void data_zaxpy(int n, dataType alpha, const dataType *dx, int incx, dataType *dy, int incy);
 
+/* Dot products on host arrays with strides.  sparse.h:507-517;
+ * blas/zdot.cpp:50-78 (data_zdot: OpenMP reduction; data_zdot_mkl:
+ * cblas_ddot).  Both sum in index order here. */
+dataType data_zdot(int n, dataType *dx, int incx, dataType *dy, int incy);
+dataType data_zdot_mkl(int n, dataType *dx, int incx, dataType *dy, int incy);
+
 /* Norms and the ILU residual check.  sparse.h:410,462-468;
  * control/norms.cpp:69-92,301-405.  data_zilures evaluates ||A-LU||_F over
  * LU's pattern (res) and over A's pattern (nonlinres) on the host (a
@@ -242,8 +248,17 @@ int data_PariLU_v0_3_bcsr(data_d_matrix *A, data_d_matrix *L, data_d_matrix *U,
                           dataType reduction, data_d_preconditioner_log *log);
 
 /* Triangular solves.  sparse.h:881-924; src/trisolve.cpp:20-119,256-351.
- * Iterated sweeps until sum((y_new-y_old)^2) <= tol, run synchronously on
- * the GPU (at most one sweep per level plus one confirming sweep). */
+ * The reference sweeps in place (natural row order for L, reverse for U)
+ * until sum((y_new-y_old)^2) <= tol.  Run by one thread -- the only schedule
+ * with a defined result -- that is exact substitution in the first sweep and
+ * an unchanged second sweep.  These entry points return that result: y is
+ * the exact solution, computed on the GPU in level order (bit-identical to
+ * substitution), and *iter is 1 when sum((y0-y)^2) over the caller's initial
+ * y0 is already <= tol (data_parcsrtrsv: or not finite), else 2; with
+ * tol >= 1e8 (the loop's initial step) no sweep runs, y is left as passed
+ * and *iter is 0.  The two
+ * solve wrappers print the reference's "% iteration = k step = s" lines
+ * (option "matlab_log"). */
 int data_forward_solve(data_d_matrix *L, data_d_matrix *x, data_d_matrix *rhs,
                        const dataType tol, int *iter);
 int data_backward_solve(data_d_matrix *U, data_d_matrix *x, data_d_matrix *rhs,
@@ -367,13 +382,17 @@ int lace_csr_spmv(const lace_csr *A, double alpha, const void *x, double beta,
  * diagonal form, else ELLPACK, else CSR (options "spmv_dia", "spmv_ell"). */
 int lace_csr_forms(const lace_csr *A, int *ndiag, int *ell_width);
 
-/* ParILU(0) on device.  Factor storage in prec (F64 or F32; BF16 keeps a
- * per-row scale), iteration arithmetic in F64 for F64 and F32 otherwise. */
+/* ParILU(0) on device.  Factor storage in prec (F64 or F32), iteration
+ * arithmetic in the same precision. */
 int lace_parilu_create(const data_d_matrix *A, lace_prec_t prec,
                        lace_parilu **out);
 int lace_parilu_destroy(lace_parilu *P);
 /* Re-initialise the factors from A (L0 = strict-lower(A) + I, U0 = upper(A)). */
 int lace_parilu_reset(lace_parilu *P);
+/* New values of A on the pattern given at create (host array, nnz entries
+ * in A's CSR order).  The factors are unchanged until the next
+ * lace_parilu_reset, which starts from the new values. */
+int lace_parilu_set_values(lace_parilu *P, const double *vals);
 /* Run nsweeps synchronous sweeps.  If step is non-NULL, the sum of squared
  * updates of the last sweep divided by ||A||_F is written there (this
  * synchronizes with the host). */
@@ -511,8 +530,9 @@ int lace_solver_solve(lace_solver *S, const double *b, double *x,
  * inside the same FGMRES; A^T is assembled on the first transposed call. */
 int lace_solver_solve_trans(lace_solver *S, const double *b, double *x, int trans,
                             lace_solver_stats *st);
-/* Replace the operator's values on A's pattern (host array of nnz values);
- * the ILU factors stay those of the last lace_solver_setup. */
+/* Replace the operator's values on A's pattern (host array of nnz values).
+ * The ILU factors in use stay those of the last lace_solver_setup; the next
+ * lace_solver_setup factors the new values. */
 int lace_solver_set_values(lace_solver *S, const double *vals);
 /* dL/dtheta on the pattern of A: dtheta[k] = -lambda[row(k)] * x[col(k)]
  * for CSR position k; device vectors.  With A x = b and A^T lambda = dL/dx
@@ -602,6 +622,11 @@ int lace_block_panel_qr(int n, int r0, const double *W, double *Y, double *Tb, d
  * lines to stdout (src/fgmres.cpp:71,178,430; src/fgmres_householder_restart.cpp:
  * 70,81,196,231,707) -- "FGMRES_Householders_partrsv_search(j) = |g_j|;" etc.,
  * the log format its post-processing scripts read; 0 silences them.
+ * "trsv_exact" (0 or 1, default 1): the exact (level) triangular solves run
+ * the persistent natural-order kernel (1: blocks of 256 rows taken in
+ * ticket order, dependencies inside a block resolved through LDS, across
+ * blocks through the solution vector with a NaN sentinel per unsolved row)
+ * or the level-ordered sync-free kernel (0: one thread per row).
  * "timing_sample" (>= 1, default 1): the timing regions below record events
  * around the first four entries of each region after a reset and every
  * k-th entry after that only. */

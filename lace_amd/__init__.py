@@ -121,6 +121,8 @@ SIGNATURES = {
     "data_z_csr_mtx": (_i, [_DM, C.c_char_p]),
     "data_zwrite_csr_mtx": (_i, [DMatrix, _i, C.c_char_p]),
     "data_dnrm2": (_d, [_i, _P(_d), _i]),
+    "data_zdot": (_d, [_i, _P(_d), _i, _P(_d), _i]),
+    "data_zdot_mkl": (_d, [_i, _P(_d), _i, _P(_d), _i]),
     "data_z_dense_mtx": (_i, [_DM, _i, C.c_char_p]),
     "data_zwrite_dense_mtx": (_i, [_i, _i, _i, _i, _P(_P(_d)), C.c_char_p]),
     "data_zwrite_dense": (_i, [DMatrix, C.c_char_p]),
@@ -167,6 +169,7 @@ SIGNATURES = {
     "lace_parilu_create": (_i, [_DM, _i, _P(_vp)]),
     "lace_parilu_destroy": (_i, [_vp]),
     "lace_parilu_reset": (_i, [_vp]),
+    "lace_parilu_set_values": (_i, [_vp, _P(_d)]),
     "lace_parilu_sweeps": (_i, [_vp, _i, _P(_d)]),
     "lace_parilu_get": (_i, [_vp, _DM, _DM]),
     "lace_parilu_nonlinres": (_i, [_vp, _P(_d)]),
@@ -431,6 +434,10 @@ class ParILU:
 
     def reset(self):
         check(lib().lace_parilu_reset(self.h), "parilu_reset")
+
+    def set_values(self, vals):
+        v = np.ascontiguousarray(vals, dtype=np.float64)
+        check(lib().lace_parilu_set_values(self.h, _dptr(v)), "parilu_set_values")
 
     def sweeps(self, k: int, want_step=False):
         st = C.c_double()

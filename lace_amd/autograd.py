@@ -1,13 +1,16 @@
 """torch.autograd binding of the adjoint over the C ABI -- for testing.
 
 forward : x = A(theta)^{-1} b          (lace_solver_set_values + solve)
-backward: A^T lambda = dL/dx           (lace_solver_solve_trans, the ILU
-                                        factors of A applied transposed)
+backward: A^T lambda = dL/dx           (lace_solver_set_values(theta) of
+                                        this node, then lace_solver_solve_trans
+                                        with the ILU factors applied transposed)
           dL/dtheta_k = -lambda_i x_j  (lace_solver_grad_values, on A's
                                         pattern), dL/db = lambda
 
 The engine is the C++/HIP library; this wrapper only moves arrays through
-host memory.  theta and b are float64 CPU tensors.
+host memory.  theta and b are float64 CPU tensors.  One solver may serve
+several nodes of a graph: each node keeps its own theta and puts it back
+into the solver before its adjoint solve.
 """
 from __future__ import annotations
 
@@ -26,13 +29,15 @@ class SparseSolve(torch.autograd.Function):
             raise LaceError(f"forward solve did not converge (rel {st.rel_true_residual:.2e})")
         xt = torch.from_numpy(x)
         ctx.solver = solver
-        ctx.save_for_backward(xt)
+        ctx.save_for_backward(xt, theta.detach().clone())
         return xt
 
     @staticmethod
     def backward(ctx, gx: torch.Tensor):
-        (x,) = ctx.saved_tensors
+        x, theta = ctx.saved_tensors
         S = ctx.solver
+        # the solver may hold another node's values by now
+        S.set_values(theta.cpu().numpy())
         lam, st = S.solve(gx.detach().cpu().numpy(), trans=1)
         if st.converged != 1:
             raise LaceError(f"adjoint solve did not converge (rel {st.rel_true_residual:.2e})")

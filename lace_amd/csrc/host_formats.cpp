@@ -787,6 +787,19 @@ void data_zaxpy(int n, dataType alpha, const dataType* dx, int incx, dataType* d
   for (int i = 0; i < n; i++) dy[(size_t)i * incy] += alpha * dx[(size_t)i * incx];
 }
 
+// blas/zdot.cpp:50-65 (OpenMP reduction; one thread sums in index order)
+// and :67-78 (cblas_ddot).  Both are the plain index-order sum here, so the
+// two entry points return the same value.
+dataType data_zdot(int n, dataType* dx, int incx, dataType* dy, int incy) {
+  double s = 0.0;
+  for (int i = 0; i < n; i++) s += dx[(size_t)i * incx] * dy[(size_t)i * incy];
+  return s;
+}
+
+dataType data_zdot_mkl(int n, dataType* dx, int incx, dataType* dy, int incy) {
+  return data_zdot(n, dx, incx, dy, incy);
+}
+
 // control/norms.cpp:7-13
 dataType data_dnrm2(int n, dataType* x, int incx) {
   double s = 0.0;

@@ -279,6 +279,25 @@ __global__ void parilu_reset_kernel(int nnz, const int* __restrict__ dst, const 
   }
 }
 
+// a_ij into the symmetric kernels' position-ordered copies: a strict-lower
+// entry to AL at its L position, an upper entry a_ij (j >= i) to AU at the
+// L position (j, i), a diagonal entry to both.
+template <typename T>
+__global__ void parilu_scatter_sym_kernel(int nnz, const int* __restrict__ dst, const int* __restrict__ acol,
+                                          const int* __restrict__ arowidx, const T* __restrict__ aval,
+                                          T* __restrict__ AL, T* __restrict__ AU) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= nnz) return;
+  const int d = dst[k];
+  const T v = aval[k];
+  if (d >= 0) {
+    AL[d] = v;
+  } else {
+    AU[-1 - d] = v;
+    if (acol[k] == arowidx[k]) AL[-1 - d] = v;
+  }
+}
+
 template <typename T>
 __global__ void parilu_unitdiag_kernel(int n, const int* __restrict__ lrow, const int* __restrict__ lcol,
                                        T* __restrict__ L0, T* __restrict__ L1) {
@@ -695,6 +714,33 @@ int ParILU::reset() {
   return DEV_SUCCESS;
 }
 
+int ParILU::set_values(const double* vals) {
+  LACE_REQUIRE(vals, DEV_ERR_INVALID_PTR, "ParILU set_values: null values");
+  double s = 0.0;
+  for (int k = 0; k < nnz; k++) s += vals[k] * vals[k];
+  anorm = std::sqrt(s);
+  DevBuf a64;
+  LACE_TRY(upload(a64, vals, sizeof(double) * nnz));
+  const int nb = ceil_div(nnz, 256);
+  if (prec == LACE_F64) {
+    LACE_HIP(hipMemcpyAsync(aval.p, a64.p, sizeof(double) * nnz, hipMemcpyDeviceToDevice, stream()));
+    if (sym)
+      parilu_scatter_sym_kernel<double><<<nb, 256, 0, stream()>>>(nnz, dst.as<int>(), acol.as<int>(),
+                                                                   arowidx.as<int>(), aval.as<double>(),
+                                                                   AL.as<double>(), AU.as<double>());
+  } else {
+    convert_kernel<float, double><<<nb, 256, 0, stream()>>>(nnz, a64.as<double>(), aval.as<float>());
+    if (sym)
+      parilu_scatter_sym_kernel<float><<<nb, 256, 0, stream()>>>(nnz, dst.as<int>(), acol.as<int>(),
+                                                                  arowidx.as<int>(), aval.as<float>(),
+                                                                  AL.as<float>(), AU.as<float>());
+  }
+  LACE_HIP(hipGetLastError());
+  // a64 is released on return: finish the copies that read it first
+  LACE_HIP(hipStreamSynchronize(stream()));
+  return DEV_SUCCESS;
+}
+
 template <typename T, int MAXL>
 static void launch_sym(ParILU* P, bool with_step, int in, int out, int nb) {
   if (with_step)
@@ -936,6 +982,12 @@ int lace_parilu_destroy(lace_parilu* P) {
 int lace_parilu_reset(lace_parilu* P) {
   LACE_TRY(ensure_init());
   return reinterpret_cast<ParILU*>(P)->reset();
+}
+
+int lace_parilu_set_values(lace_parilu* P, const double* vals) {
+  LACE_TRY(ensure_init());
+  LACE_REQUIRE(P, DEV_ERR_INVALID_PTR, "lace_parilu_set_values: null handle");
+  return reinterpret_cast<ParILU*>(P)->set_values(vals);
 }
 
 int lace_parilu_sweeps(lace_parilu* Ph, int nsweeps, double* step) {

@@ -110,10 +110,16 @@ struct ParILU {
   DevBuf l_t_from_csr, lval_t;
   bool csr_dirty = true;
   DevBuf work, work2;  // scratch vectors for the solves (n elements each)
+  // [0] an exact solve gave up waiting, [1] a pipelined Jacobi launch gave
+  // up waiting, [2] the persistent exact solve's block ticket
   DevBuf syncfree_err;
 
   size_t elem_size() const { return prec == LACE_F64 ? 8 : 4; }
   int reset();
+  // New values of A on the pattern given at create (host array, nnz
+  // entries in A's CSR order): refreshes ||A||_F and every device copy of A
+  // the sweeps read, so the next reset() starts from them.
+  int set_values(const double* vals);
   int sweep(bool with_step);
   int read_step(double* step);
   int nonlinres(double* out);

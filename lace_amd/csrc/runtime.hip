@@ -351,7 +351,7 @@ extern "C" int lace_set_option(const char* name, int value) {
                  "stream_rows_per_thread must be 1, 2 or 4 (have %d)", value);
   } else if (key == "jacobi_ell" || key == "spmv_ell" || key == "jacobi_dia" || key == "spmv_dia" ||
              key == "jacobi_fuse" || key == "jacobi_dia_vec" || key == "jacobi_pipe" ||
-             key == "matlab_log") {
+             key == "matlab_log" || key == "trsv_exact") {
     LACE_REQUIRE(value == 0 || value == 1, DEV_ERR_ILLEGAL_VALUE, "%s must be 0 or 1 (have %d)", name, value);
   } else if (key == "timing_sample") {
     LACE_REQUIRE(value >= 1, DEV_ERR_ILLEGAL_VALUE, "timing_sample must be >= 1 (have %d)", value);

@@ -415,24 +415,31 @@ static int solver_create(const data_d_matrix* A, const lace_solver_opts* o, Solv
   return DEV_SUCCESS;
 }
 
-// New values for the operator on A's pattern (the factors are kept until the
-// next setup): updates A, its FP32 copy and A^T.
+// New values for the operator on A's pattern: updates A, its FP32 copy, A^T
+// and the ParILU's own copy of A.  The factors in use stay those of the last
+// setup; the next setup factors the new values.
 static int solver_set_values(Solver* S, const double* vals) {
   LACE_REQUIRE(!S->block, DEV_ERR_NOT_SUPPORTED, "solver: value updates are for CSR operators");
+  LACE_REQUIRE(vals, DEV_ERR_INVALID_PTR, "solver: null values");
   S->h_val.assign(vals, vals + S->nnz);
-  LACE_HIP(hipMemcpyAsync(S->A64.val.p, vals, sizeof(double) * S->nnz, hipMemcpyHostToDevice, stream()));
+  // host staging buffers live until the synchronisation below
+  std::vector<float> f32;
+  std::vector<double> tv;
+  std::vector<float> tf32;
+  LACE_HIP(hipMemcpyAsync(S->A64.val.p, S->h_val.data(), sizeof(double) * S->nnz, hipMemcpyHostToDevice,
+                          stream()));
   if (S->inner) {
-    std::vector<float> f(vals, vals + S->nnz);
-    LACE_HIP(hipMemcpyAsync(S->A32.val.p, f.data(), sizeof(float) * S->nnz, hipMemcpyHostToDevice, stream()));
-    LACE_HIP(hipStreamSynchronize(stream()));
+    f32.assign(vals, vals + S->nnz);
+    LACE_HIP(hipMemcpyAsync(S->A32.val.p, f32.data(), sizeof(float) * S->nnz, hipMemcpyHostToDevice, stream()));
   }
   if (S->have_T) {
-    std::vector<double> tv(S->nnz);
+    tv.resize(S->nnz);
     for (int q = 0; q < S->nnz; q++) tv[q] = vals[S->t_from[q]];
     LACE_HIP(hipMemcpyAsync(S->AT64.val.p, tv.data(), sizeof(double) * S->nnz, hipMemcpyHostToDevice, stream()));
     if (S->inner) {
-      std::vector<float> f(tv.begin(), tv.end());
-      LACE_HIP(hipMemcpyAsync(S->AT32.val.p, f.data(), sizeof(float) * S->nnz, hipMemcpyHostToDevice, stream()));
+      tf32.assign(tv.begin(), tv.end());
+      LACE_HIP(hipMemcpyAsync(S->AT32.val.p, tf32.data(), sizeof(float) * S->nnz, hipMemcpyHostToDevice,
+                              stream()));
     }
   }
   LACE_TRY(S->A64.refresh_ell());
@@ -441,6 +448,7 @@ static int solver_set_values(Solver* S, const double* vals) {
     LACE_TRY(S->AT64.refresh_ell());
     if (S->inner) LACE_TRY(S->AT32.refresh_ell());
   }
+  if (S->P) LACE_TRY(S->P->set_values(vals));
   LACE_HIP(hipStreamSynchronize(stream()));
   return DEV_SUCCESS;
 }

@@ -232,6 +232,184 @@ __global__ __launch_bounds__(256) void tri_syncfree_kernel(int nslots, const int
                      __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// Persistent exact solve in natural row order.  Blocks of BS consecutive
+// rows go to workgroups in ticket order (an atomic counter): ascending for a
+// lower factor, descending for an upper one, so every row a block reads from
+// outside itself belongs to a block handed out earlier.  A workgroup has
+// 2 BS lanes: lane t < BS computes row lo + t, lane BS + t fetches that
+// row's external operands.
+//   - The fetch lane polls the solution vector (device-coherent loads; the
+//     NaN sentinel marks a row not yet solved) for the row's first R
+//     off-diagonal operands that lie outside the block and stages each one
+//     in LDS as it arrives.
+//   - The compute lane keeps its row's first R column indices and values in
+//     registers and walks them in CSR order, taking each operand from LDS:
+//     the block's own solved rows or the staged external values.  It never
+//     waits on device memory, so a dependency chain inside the block moves
+//     at LDS speed; the compute lanes of a wave retry until every row they
+//     hold is solved.  Operands past the first R come straight from memory.
+//   - A solved row goes to LDS for the block and to the solution vector
+//     (device-coherent store) for later blocks.
+// The earliest unfinished block always has every external operand solved
+// (they lie in earlier blocks) and its earliest unsolved row has every
+// operand it needs, so every block finishes.  Each lane also gives up after
+// spin_limit rounds (setting *err), so every wave exits whatever happens.
+// The row sum runs over the row's entries in CSR order and then divides, as
+// the level-scheduled kernel does: bit-identical to substitution.
+// y must not alias r; y is filled with the sentinel and *ticket zeroed by
+// tri_persist_init_kernel first.
+template <typename T, bool UPPER, int BS, int R>
+__global__ __launch_bounds__(2 * BS) void tri_persist_kernel(int n, int nblk, const int* __restrict__ row,
+                                                             const int* __restrict__ col,
+                                                             const T* __restrict__ val,
+                                                             const T* __restrict__ r, T* y, unsigned* ticket,
+                                                             int* err, int spin_limit) {
+#pragma clang fp contract(off)
+  using UB = typename Sentinel<T>::U;
+  constexpr UB SENT = Sentinel<T>::bits;
+  __shared__ UB ly[BS];      // the block's solved rows
+  __shared__ UB lx[BS * R];  // staged external operands, slot q of local row t at t R + q
+  __shared__ int s_blk, s_next;
+  const int tid = threadIdx.x;
+  const bool fetcher = tid >= BS;
+  const int t = fetcher ? tid - BS : tid;
+  UB* yb = reinterpret_cast<UB*>(y);
+  if (tid == 0) s_blk = (int)__hip_atomic_fetch_add(ticket, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  __syncthreads();
+  int blk = s_blk;
+  while (blk < nblk) {
+    int lo, hi;
+    if (UPPER) {
+      hi = n - blk * BS;
+      lo = hi - BS > 0 ? hi - BS : 0;
+    } else {
+      lo = blk * BS;
+      hi = lo + BS < n ? lo + BS : n;
+    }
+    if (!fetcher) {
+      __hip_atomic_store(&ly[t], SENT, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+#pragma unroll
+      for (int q = 0; q < R; q++)
+        __hip_atomic_store(&lx[t * R + q], SENT, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    }
+    __syncthreads();
+    const int i = lo + t;
+    const bool active = i < hi;
+    int k0 = 0, len = 0, kd = 0;
+    if (active) {
+      if (UPPER) {
+        kd = row[i];
+        k0 = kd + 1;
+        len = row[i + 1] - k0;
+      } else {
+        k0 = row[i];
+        kd = row[i + 1] - 1;
+        len = kd - k0;
+      }
+    }
+    int cq[R];
+#pragma unroll
+    for (int q = 0; q < R; q++) cq[q] = q < len ? col[k0 + q] : -1;
+    if (fetcher) {
+      unsigned pend = 0u;
+#pragma unroll
+      for (int q = 0; q < R; q++)
+        if (q < len && (cq[q] < lo || cq[q] >= hi)) pend |= 1u << q;
+      int spins = 0;
+      while (true) {
+        UB b[R];
+#pragma unroll
+        for (int q = 0; q < R; q++)
+          b[q] = ((pend >> q) & 1u) ? __hip_atomic_load(yb + cq[q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+                                    : SENT;
+#pragma unroll
+        for (int q = 0; q < R; q++)
+          if (((pend >> q) & 1u) && b[q] != SENT) {
+            __hip_atomic_store(&lx[t * R + q], b[q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            pend &= ~(1u << q);
+          }
+        if (pend != 0u && ++spins > spin_limit) {
+          __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          pend = 0u;
+        }
+        if (!__any(pend != 0u)) break;
+        __builtin_amdgcn_s_sleep(1);
+      }
+      // the next block's ticket, fetched while the compute lanes work
+      if (t == 0)
+        s_next = (int)__hip_atomic_fetch_add(ticket, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    } else {
+      T vq[R];
+#pragma unroll
+      for (int q = 0; q < R; q++) vq[q] = q < len ? val[k0 + q] : T(0);
+      const T ri = active ? r[i] : T(0);
+      const T di = active ? val[kd] : T(1);
+      T s = T(0);
+      int p = 0;
+      bool done = !active;
+      int spins = 0;
+      while (true) {
+        bool moved = false;
+        if (!done) {
+#pragma unroll
+          for (int q = 0; q < R; q++) {
+            if (q == p && q < len) {
+              const int c = cq[q];
+              const UB b = (c >= lo && c < hi)
+                               ? __hip_atomic_load(&ly[c - lo], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)
+                               : __hip_atomic_load(&lx[t * R + q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+              if (b != SENT) {
+                s = s + vq[q] * __builtin_bit_cast(T, b);
+                p++;
+              }
+            }
+          }
+          // operands past the first R slots (rows longer than R)
+          while (p >= R && p < len) {
+            const int k = k0 + p;
+            const int c = col[k];
+            const UB b = (c >= lo && c < hi)
+                             ? __hip_atomic_load(&ly[c - lo], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)
+                             : __hip_atomic_load(yb + c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (b == SENT) break;
+            s = s + val[k] * __builtin_bit_cast(T, b);
+            p++;
+          }
+          if (p == len) {
+            const T yi = (ri - s) / di;
+            const UB ub = __builtin_bit_cast(UB, yi);
+            __hip_atomic_store(&ly[t], ub, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            __hip_atomic_store(yb + i, ub, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            done = true;
+            moved = true;
+          } else if (++spins > spin_limit) {
+            __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            done = true;
+          }
+        }
+        if (__all(done)) break;
+        if (!__any(moved)) __builtin_amdgcn_s_sleep(1);
+      }
+    }
+    __syncthreads();
+    if (tid == 0) s_blk = s_next;
+    __syncthreads();
+    blk = s_blk;
+  }
+}
+
+// y = sentinel everywhere, ticket = 0: the state tri_persist_kernel starts
+// from.
+template <typename T>
+__global__ void tri_persist_init_kernel(int n, T* y, unsigned* ticket) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) {
+    typename Sentinel<T>::U b = Sentinel<T>::bits;
+    y[i] = __builtin_bit_cast(T, b);
+  }
+  if (i == 0) *ticket = 0u;
+}
+
 // One synchronous Jacobi sweep: yout_i = (r_i - sum_{k != diag} T_ik yin_k)/T_ii.
 // yin == nullptr means yin = 0.  STEP accumulates (yout - yin)^2 per block.
 template <typename T, bool UPPER, bool STEP>
@@ -856,6 +1034,59 @@ int tri_solve_syncfree(const TriPlan& P, const T* val, const T* r, T* y, int* er
   return DEV_SUCCESS;
 }
 
+// Rows per block and register slots of the persistent exact solve.
+constexpr int kPersistBS = 256;
+constexpr int kPersistR = 8;
+// Poll rounds before a persistent-solve lane gives up (each round sleeps
+// about 64 cycles when nothing moved): far beyond any legitimate wait.
+static int g_persist_spin_limit = 1 << 22;
+
+template <typename T, bool UPPER>
+static int persist_grid(int nblk) {
+  static int occ = -1;
+  if (occ < 0) {
+    int o = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&o, tri_persist_kernel<T, UPPER, kPersistBS, kPersistR>,
+                                                     2 * kPersistBS, 0) != hipSuccess || o < 1)
+      o = 1;
+    occ = o;
+  }
+  const long g = (long)occ * std::max(1, num_cus());
+  return (int)std::min<long>(g, nblk);
+}
+
+// y = T^{-1} r exact on raw CSR arrays (lower: diagonal last in each row;
+// upper: diagonal first): the persistent natural-order kernel.  ticket and
+// err are device words; the solve resets the ticket itself.
+template <typename T>
+int tri_solve_persist_raw(int n, bool upper, const int* row, const int* col, const T* val, const T* r, T* y,
+                          unsigned* ticket, int* err) {
+  if (n <= 0) return DEV_SUCCESS;
+  LACE_REQUIRE((const void*)r != (const void*)y, DEV_ERR_ILLEGAL_VALUE, "exact solve needs distinct r and y");
+  const int nblk = ceil_div(n, kPersistBS);
+  tri_persist_init_kernel<T><<<ceil_div(n, 256), 256, 0, stream()>>>(n, y, ticket);
+  if (upper)
+    tri_persist_kernel<T, true, kPersistBS, kPersistR><<<persist_grid<T, true>(nblk), 2 * kPersistBS, 0, stream()>>>(
+        n, nblk, row, col, val, r, y, ticket, err, g_persist_spin_limit);
+  else
+    tri_persist_kernel<T, false, kPersistBS, kPersistR><<<persist_grid<T, false>(nblk), 2 * kPersistBS, 0,
+                                                           stream()>>>(n, nblk, row, col, val, r, y, ticket, err,
+                                                                       g_persist_spin_limit);
+  LACE_HIP(hipGetLastError());
+  return DEV_SUCCESS;
+}
+
+// Exact solve with a plan: the persistent kernel (option "trsv_exact" 1,
+// the default) or the level-ordered sync-free kernel (0).
+template <typename T>
+int tri_solve_exact(const TriPlan& P, const T* val, const T* r, T* y, unsigned* ticket, int* err) {
+  if (get_option("trsv_exact", 1) == 0) return tri_solve_syncfree<T>(P, val, r, y, err);
+  const double es = sizeof(T);
+  const double bytes = (double)P.nnz * (es + 4.0) + 4.0 * (P.n + 1) + 3.0 * es * P.n;
+  TimedRegion tr(P.upper ? "trsv_exact_upper" : "trsv_exact_lower", 0.0, bytes);
+  return tri_solve_persist_raw<T>(P.n, P.upper, P.row.as<int>(), P.col.as<int>(), val, r, y, ticket, err);
+}
+
 // One sweep yout = D^{-1}(r - N yin).  With y2 and d2 given, the sweep on
 // the diagonal form also writes y2 = yout / d2 and sets *epi.
 template <typename T>
@@ -1149,8 +1380,9 @@ static int apply_t(ParILU* P, lace_trsv_kind_t kind, int sweeps, int trans, cons
   }
   if (kind == LACE_TRSV_LEVEL) {
     int* err = P->syncfree_err.as<int>();
-    LACE_TRY(tri_solve_syncfree<T>(first, v1, r, tmp, err));
-    LACE_TRY(tri_solve_syncfree<T>(second, v2, tmp, z, err));
+    unsigned* ticket = reinterpret_cast<unsigned*>(P->syncfree_err.as<int>() + 2);
+    LACE_TRY(tri_solve_exact<T>(first, v1, r, tmp, ticket, err));
+    LACE_TRY(tri_solve_exact<T>(second, v2, tmp, z, ticket, err));
     return DEV_SUCCESS;
   }
   // Jacobi: work holds the intermediate vector, work2 the ping-pong buffer.
@@ -1216,11 +1448,92 @@ int lace_syncfree_error(lace_parilu* Ph, int* err) {
   return DEV_SUCCESS;
 }
 
-// Iterated synchronous sweeps until sum (y_new - y_old)^2 <= tol or the
-// step is not finite (src/trisolve.cpp:274,317), starting from the y
-// passed in.  A triangular Jacobi iteration is exact after nlev sweeps, so
-// at most nlev+1 sweeps run.  storage/uplo must name CSRL+Lower or
-// CSRU+Upper as in the reference (:264-347).
+}  // extern "C"
+
+namespace {
+
+// The reference loop (src/trisolve.cpp:20-119,256-351) sweeps in place, in
+// natural row order for L and reverse order for U, until the sum of squared
+// changes is <= tol.  Run by one thread, which is the only schedule with a
+// defined result, its first sweep is exact forward / backward substitution
+// and its second sweep changes nothing (step 0), so it returns the exact
+// solution with iter = 1 when the first step is already <= tol and iter = 2
+// otherwise (and no sweep at all when tol >= 1e8, the loop's initial step).
+// Here the exact solve runs on the device (level-ordered,
+// bit-identical to substitution) and the first step, sum (y0_i - y_i)^2 in
+// the reference's row order, is formed on the host from the caller's y0.
+// check_finite: data_parcsrtrsv also stops on a non-finite step (:274,317);
+// the two solve wrappers do not.  log: the wrappers print each sweep's
+// "% iteration = k step = s" line as the reference does (:57,104).
+int exact_parcsrtrsv(bool upper, int n, const double* Aval, const int* row, const int* col,
+                     const double* rhs, double* y, double tol, bool check_finite, bool log, int* iter) {
+  LACE_REQUIRE(n >= 0 && (n == 0 || (Aval && row && col && rhs && y && iter)), DEV_ERR_INVALID_PTR,
+               "parcsrtrsv: null argument");
+  // the loop starts from step = 1e8: a tol at or above it (or NaN) runs no
+  // sweep and leaves y as it is
+  if (!(1.e8 > tol)) {
+    if (iter) *iter = 0;
+    return 0;
+  }
+  if (n == 0) {
+    // an empty sweep has step 0 <= tol
+    if (iter) *iter = 1;
+    return 0;
+  }
+  for (int i = 0; i < n; i++) {
+    const int kd = upper ? row[i] : row[i + 1] - 1;
+    LACE_REQUIRE(row[i + 1] > row[i] && col[kd] == i, DEV_ERR_ILLEGAL_VALUE,
+                 "parcsrtrsv: row %d has no diagonal in the %s slot", i, upper ? "first" : "last");
+  }
+  const int nnz = row[n];
+  DevBuf val, drow, dcol, r, yd, err;
+  LACE_TRY(upload(val, Aval, sizeof(double) * std::max(nnz, 1)));
+  LACE_TRY(upload(drow, row, sizeof(int) * (n + 1)));
+  LACE_TRY(upload(dcol, col, sizeof(int) * std::max(nnz, 1)));
+  LACE_TRY(upload(r, rhs, sizeof(double) * n));
+  LACE_TRY(yd.alloc(sizeof(double) * n));
+  // [0] the solve gave up waiting, [1] the block ticket
+  LACE_TRY(err.alloc(sizeof(int) * 2));
+  LACE_HIP(hipMemsetAsync(err.p, 0, sizeof(int) * 2, stream()));
+  LACE_TRY(tri_solve_persist_raw<double>(n, upper, drow.as<int>(), dcol.as<int>(), val.as<double>(),
+                                         r.as<double>(), yd.as<double>(),
+                                         reinterpret_cast<unsigned*>(err.as<int>() + 1), err.as<int>()));
+  std::vector<double> ynew(n);
+  int e = 0;
+  LACE_HIP(hipMemcpyAsync(ynew.data(), yd.p, sizeof(double) * n, hipMemcpyDeviceToHost, stream()));
+  LACE_HIP(hipMemcpyAsync(&e, err.p, sizeof(int), hipMemcpyDeviceToHost, stream()));
+  LACE_HIP(hipStreamSynchronize(stream()));
+  LACE_REQUIRE(e == 0, DEV_ERR, "parcsrtrsv: the device solve gave up waiting on a row");
+  double step = 0.0;
+  for (int t = 0; t < n; t++) {
+    const int i = upper ? n - 1 - t : t;
+    const double d = y[i] - ynew[i];
+    step += d * d;
+  }
+  memcpy(y, ynew.data(), sizeof(double) * n);
+  const bool again = check_finite ? (step > tol && std::isfinite(step)) : step > tol;
+  if (log) printf("%% iteration = %d step = %e\n", 1, step);
+  if (again) {
+    // the confirming sweep: every row recomputes the value it holds, so the
+    // change is 0, or NaN where a value is not finite
+    double step2 = 0.0;
+    for (int t = 0; t < n; t++) {
+      const int i = upper ? n - 1 - t : t;
+      const double d = ynew[i] - ynew[i];
+      step2 += d * d;
+    }
+    if (log) printf("%% iteration = %d step = %e\n", 2, step2);
+  }
+  *iter = again ? 2 : 1;
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+// src/trisolve.cpp:256-351.  storage/uplo must name CSRL+Lower or
+// CSRU+Upper as in the reference (:264-347); info -1 otherwise.
 int data_parcsrtrsv(const data_uplo_t uplo, const data_storage_t storage,
                     const data_diagorder_t diag, const int num_rows, const dataType* Aval,
                     const int* row, const int* col, const dataType* rhsval, dataType* yval,
@@ -1234,57 +1547,33 @@ int data_parcsrtrsv(const data_uplo_t uplo, const data_storage_t storage,
               (int)storage, (int)uplo);
     return -1;
   }
-  const int n = num_rows;
-  TriPlan P;
-  LACE_TRY(P.build(n, row, col, upper));
-  DevBuf val, r, y[2], partial, sum;
-  LACE_TRY(upload(val, Aval, sizeof(double) * std::max(P.nnz, 1)));
-  LACE_TRY(upload(r, rhsval, sizeof(double) * n));
-  LACE_TRY(upload(y[0], yval, sizeof(double) * n));
-  LACE_TRY(y[1].alloc(sizeof(double) * n));
-  const int nb = ceil_div(n, 256);
-  LACE_TRY(partial.alloc(sizeof(double) * nb));
-  std::vector<double> ph(nb);
-  int it = 0, cur = 0;
-  double step = 1.e8;
-  while (step > tol && std::isfinite(step) && it <= P.nlev + 1) {
-    LACE_TRY(jacobi_sweep<double>(P, val.as<double>(), r.as<double>(), y[cur].as<double>(),
-                                  y[1 - cur].as<double>(), partial.as<double>()));
-    LACE_HIP(hipMemcpyAsync(ph.data(), partial.p, sizeof(double) * nb, hipMemcpyDeviceToHost,
-                            stream()));
-    LACE_HIP(hipStreamSynchronize(stream()));
-    step = 0.0;
-    for (int b = 0; b < nb; b++) step += ph[b];
-    cur = 1 - cur;
-    it++;
-  }
-  LACE_TRY(download(yval, y[cur], sizeof(double) * n));
-  *iter = it;
-  return 0;
+  return exact_parcsrtrsv(upper, num_rows, Aval, row, col, rhsval, yval, tol, true, false, iter);
 }
 
 // src/trisolve.cpp:20-67
 int data_forward_solve(data_d_matrix* L, data_d_matrix* x, data_d_matrix* rhs, const dataType tol,
                        int* iter) {
+  LACE_TRY(ensure_init());
   if (!(L->storage_type == Magma_CSRL && L->fill_mode == MagmaLower &&
         L->diagorder_type != Magma_NODIAG)) {
     set_error("data_forward_solve: L must be CSRL / lower with a diagonal");
     return -1;
   }
-  return data_parcsrtrsv(MagmaLower, Magma_CSRL, L->diagorder_type, L->num_rows, L->val, L->row,
-                         L->col, rhs->val, x->val, tol, iter);
+  return exact_parcsrtrsv(false, L->num_rows, L->val, L->row, L->col, rhs->val, x->val, tol, false,
+                          get_option("matlab_log", 1) != 0, iter);
 }
 
 // src/trisolve.cpp:72-119
 int data_backward_solve(data_d_matrix* U, data_d_matrix* x, data_d_matrix* rhs,
                         const dataType tol, int* iter) {
+  LACE_TRY(ensure_init());
   if (!(U->storage_type == Magma_CSRU && U->fill_mode == MagmaUpper &&
         U->diagorder_type != Magma_NODIAG)) {
     set_error("data_backward_solve: U must be CSRU / upper with a diagonal");
     return -1;
   }
-  return data_parcsrtrsv(MagmaUpper, Magma_CSRU, U->diagorder_type, U->num_rows, U->val, U->row,
-                         U->col, rhs->val, x->val, tol, iter);
+  return exact_parcsrtrsv(true, U->num_rows, U->val, U->row, U->col, rhs->val, x->val, tol, false,
+                          get_option("matlab_log", 1) != 0, iter);
 }
 
 }  // extern "C"

@@ -399,3 +399,24 @@ def test_zaxpy_strided(lace):
     ref = np.full(12, 0.5)
     ref[0:8:2] += -2.0 * x[0:12:3]
     assert np.array_equal(y, ref)
+
+
+def test_zdot_dropins_match_reference_goldens(lace, orc):
+    """data_zdot / data_zdot_mkl (blas/zdot.cpp:50-78) on the reference's own
+    dense fixture and expected values (4-ulp tolerance, as DoubleEq)."""
+    import json
+    gold = json.load(open(os.path.join(GOLDEN, "reference_goldens.json")))["dense_dot"]
+    D = lace.DMatrix()
+    D.storage_type = lace.Magma_DENSE
+    path = os.path.join(GOLDEN, "io_dense_test.mtx")
+    lace.check(lace.lib().data_z_dense_mtx(C.byref(D), lace.MagmaRowMajor, path.encode()))
+    a = np.ascontiguousarray(lace.owned_to_numpy(D, kind="dense"), dtype=np.float64)
+    assert a.size == 35
+    P = C.POINTER(C.c_double)
+    for case in gold["cases"]:
+        sub = a[case["offset"]:]
+        ptr = sub.ctypes.data_as(P)
+        exp = case["expected"]
+        for fn in (lace.lib().data_zdot, lace.lib().data_zdot_mkl):
+            got = fn(case["n"], ptr, case["inc"], ptr, case["inc"])
+            assert abs(got - exp) <= 4 * np.spacing(abs(exp)), (case, got)

@@ -253,7 +253,8 @@ def test_block_fgmres_converges(gpu, orc, prec, trsv):
 def test_config3_full_size(gpu, orc):
     """Config 3: 64^3 block grid, 5x5 blocks (n = 1,310,720), off-diagonal
     entries 0.1 N(0,1) (seed 1), diagonal blocks + 5 I, b ~ N(0,1) seed 2;
-    FP32 and BF16 factors under FP64 FGMRES(40) to 1e-8.  The error of
+    FP32 and BF16 factors under FP64 FGMRES(40) to 1e-8, the true residual
+    recomputed on the host with the oracle's block product.  The error of
     each solution is stated against an FP64 solve to 1e-13."""
     import lace_amd as L
     row, col, val = L.gen_bcsr7(64, 64, 64, 5, seed=1, sigma=0.1, shift=5.0)
@@ -261,9 +262,15 @@ def test_config3_full_size(gpu, orc):
     b = L.gen_normal(2, B.nbr * 5)
     xr, st, _ = _solve(gpu, B, gpu.LACE_F64, gpu.TRSV_JACOBI, rtol=1e-13, b=b)
     assert st.converged == 1
+    assert np.linalg.norm(b - orc.bcsr_spmv(B, xr)) <= 1.01e-13 * np.linalg.norm(b)
     for prec in ("f32", "bf16"):
         x, st, _ = _solve(gpu, B, _PREC[prec][0], gpu.TRSV_JACOBI, b=b)
         assert st.converged == 1 and st.rel_true_residual <= 1.01e-8, prec
+        # the FP64 true residual recomputed on the host with the oracle's
+        # block product
+        rel = np.linalg.norm(b - orc.bcsr_spmv(B, x)) / np.linalg.norm(b)
+        assert rel <= 1e-8, (prec, rel)
+        assert abs(rel - st.rel_true_residual) <= 1e-3 * rel
         err = np.linalg.norm(x - xr) / np.linalg.norm(xr)
         print(f"config3 {prec}: its={st.iterations} rel_res={st.rel_true_residual:.3e} "
               f"err_vs_fp64={err:.3e}")
