@@ -9,6 +9,22 @@
 
 namespace lace {
 
+// Marker of an unsolved row in the exact solves' solution vectors: a
+// signalling-NaN bit pattern, which arithmetic never produces (it yields
+// quiet NaNs).
+template <typename T>
+struct Sentinel;
+template <>
+struct Sentinel<double> {
+  using U = unsigned long long;
+  static constexpr U bits = 0x7FF4DEADBEEF5A5AULL;
+};
+template <>
+struct Sentinel<float> {
+  using U = unsigned int;
+  static constexpr U bits = 0x7FA5A5A5U;
+};
+
 // Pattern of a triangular factor in CSR with its level schedule.
 //   upper = false: diagonal last in each row (CSRL layout)
 //   upper = true : diagonal first in each row (CSRU layout)
@@ -60,6 +76,16 @@ int tri_solve_levels(const TriPlan& P, const T* val, const T* r, T* y);
 // rows they depend on (value slots are pre-filled with a NaN sentinel).
 template <typename T>
 int tri_solve_syncfree(const TriPlan& P, const T* val, const T* r, T* y, int* err);
+// y = T^{-1} r exact on raw device CSR arrays (upper = false: diagonal last
+// in each row, CSRL; true: diagonal first, CSRU), one persistent launch:
+// blocks of rows in ticket order, dependencies inside a block through LDS,
+// across blocks through y itself (a NaN sentinel marks unsolved rows).
+// ticket and err are device words; err is set if a wait gave up.  init:
+// fill y with the sentinel and zero the ticket first (false: the caller
+// has done so).  r and y must be distinct.
+template <typename T>
+int tri_solve_persist_raw(int n, bool upper, const int* row, const int* col, const T* val, const T* r, T* y,
+                          unsigned* ticket, int* err, bool init = true);
 // `sweeps` synchronous Jacobi sweeps from y = 0; tmp is a scratch vector.
 // epi_y / epi_d: the last sweep may also write epi_y = y / epi_d (the first
 // sweep of a following solve with diagonal epi_d) and then sets *epi_done;

@@ -165,19 +165,6 @@ __global__ __launch_bounds__(256) void tri_level_kernel(int begin, int count,
 }
 
 template <typename T>
-struct Sentinel;
-template <>
-struct Sentinel<double> {
-  using U = unsigned long long;
-  static constexpr U bits = 0x7FF4DEADBEEF5A5AULL;  // signalling-NaN payload
-};
-template <>
-struct Sentinel<float> {
-  using U = unsigned int;
-  static constexpr U bits = 0x7FA5A5A5U;
-};
-
-template <typename T>
 __global__ void fill_sentinel_kernel(int n, T* y) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) {
@@ -1055,16 +1042,13 @@ static int persist_grid(int nblk) {
   return (int)std::min<long>(g, nblk);
 }
 
-// y = T^{-1} r exact on raw CSR arrays (lower: diagonal last in each row;
-// upper: diagonal first): the persistent natural-order kernel.  ticket and
-// err are device words; the solve resets the ticket itself.
 template <typename T>
 int tri_solve_persist_raw(int n, bool upper, const int* row, const int* col, const T* val, const T* r, T* y,
-                          unsigned* ticket, int* err) {
+                          unsigned* ticket, int* err, bool init) {
   if (n <= 0) return DEV_SUCCESS;
   LACE_REQUIRE((const void*)r != (const void*)y, DEV_ERR_ILLEGAL_VALUE, "exact solve needs distinct r and y");
   const int nblk = ceil_div(n, kPersistBS);
-  tri_persist_init_kernel<T><<<ceil_div(n, 256), 256, 0, stream()>>>(n, y, ticket);
+  if (init) tri_persist_init_kernel<T><<<ceil_div(n, 256), 256, 0, stream()>>>(n, y, ticket);
   if (upper)
     tri_persist_kernel<T, true, kPersistBS, kPersistR><<<persist_grid<T, true>(nblk), 2 * kPersistBS, 0, stream()>>>(
         n, nblk, row, col, val, r, y, ticket, err, g_persist_spin_limit);
@@ -1086,6 +1070,11 @@ int tri_solve_exact(const TriPlan& P, const T* val, const T* r, T* y, unsigned* 
   TimedRegion tr(P.upper ? "trsv_exact_upper" : "trsv_exact_lower", 0.0, bytes);
   return tri_solve_persist_raw<T>(P.n, P.upper, P.row.as<int>(), P.col.as<int>(), val, r, y, ticket, err);
 }
+
+template int tri_solve_persist_raw<double>(int, bool, const int*, const int*, const double*, const double*,
+                                           double*, unsigned*, int*, bool);
+template int tri_solve_persist_raw<float>(int, bool, const int*, const int*, const float*, const float*, float*,
+                                          unsigned*, int*, bool);
 
 // One sweep yout = D^{-1}(r - N yin).  With y2 and d2 given, the sweep on
 // the diagonal form also writes y2 = yout / d2 and sets *epi.
