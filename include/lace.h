@@ -623,10 +623,15 @@ int lace_block_panel_qr(int n, int r0, const double *W, double *Y, double *Tb, d
  * 70,81,196,231,707) -- "FGMRES_Householders_partrsv_search(j) = |g_j|;" etc.,
  * the log format its post-processing scripts read; 0 silences them.
  * "trsv_exact" (0 or 1, default 1): the exact (level) triangular solves run
- * the persistent natural-order kernel (1: blocks of 256 rows taken in
- * ticket order, dependencies inside a block resolved through LDS, across
+ * the persistent natural-order kernel (1: blocks of consecutive rows taken
+ * in ticket order, dependencies inside a block resolved through LDS, across
  * blocks through the solution vector with a NaN sentinel per unsolved row)
  * or the level-ordered sync-free kernel (0: one thread per row).
+ * "trsv_exact_rows" (256 or 512, default 512): rows per block of the
+ * persistent kernel.  "trsv_exact_wgs_per_cu" (0..16, default 0 = as many
+ * as fit): its resident workgroups per CU.  "trsv_exact_poll_sleep" (0..64,
+ * default 1): its sleeps of 64 cycles between two polls of the solution
+ * vector.
  * "timing_sample" (>= 1, default 1): the timing regions below record events
  * around the first four entries of each region after a reset and every
  * k-th entry after that only. */

@@ -82,10 +82,14 @@ int tri_solve_syncfree(const TriPlan& P, const T* val, const T* r, T* y, int* er
 // across blocks through y itself (a NaN sentinel marks unsolved rows).
 // ticket and err are device words; err is set if a wait gave up.  init:
 // fill y with the sentinel and zero the ticket first (false: the caller
-// has done so).  r and y must be distinct.
+// has done so).  r and y must be distinct.  wgs_per_cu: resident
+// workgroups per CU (0: as many as fit); poll_sleep: sleeps of 64 cycles
+// between two polls of the solution vector; rows: rows per block (512 or
+// 256).
 template <typename T>
 int tri_solve_persist_raw(int n, bool upper, const int* row, const int* col, const T* val, const T* r, T* y,
-                          unsigned* ticket, int* err, bool init = true);
+                          unsigned* ticket, int* err, bool init = true, int wgs_per_cu = 0, int poll_sleep = 1,
+                          int rows = 512);
 // `sweeps` synchronous Jacobi sweeps from y = 0; tmp is a scratch vector.
 // epi_y / epi_d: the last sweep may also write epi_y = y / epi_d (the first
 // sweep of a following solve with diagonal epi_d) and then sets *epi_done;

@@ -351,8 +351,17 @@ extern "C" int lace_set_option(const char* name, int value) {
                  "stream_rows_per_thread must be 1, 2 or 4 (have %d)", value);
   } else if (key == "jacobi_ell" || key == "spmv_ell" || key == "jacobi_dia" || key == "spmv_dia" ||
              key == "jacobi_fuse" || key == "jacobi_dia_vec" || key == "jacobi_pipe" ||
-             key == "matlab_log" || key == "trsv_exact") {
+             key == "matlab_log") {
     LACE_REQUIRE(value == 0 || value == 1, DEV_ERR_ILLEGAL_VALUE, "%s must be 0 or 1 (have %d)", name, value);
+  } else if (key == "trsv_exact_wgs_per_cu") {
+    LACE_REQUIRE(value >= 0 && value <= 16, DEV_ERR_ILLEGAL_VALUE, "%s must be 0..16 (have %d)", name, value);
+  } else if (key == "trsv_exact") {
+    LACE_REQUIRE(value == 0 || value == 1, DEV_ERR_ILLEGAL_VALUE, "%s must be 0 or 1 (have %d)", name, value);
+  } else if (key == "trsv_exact_rows") {
+    LACE_REQUIRE(value == 256 || value == 512, DEV_ERR_ILLEGAL_VALUE, "%s must be 256 or 512 (have %d)", name,
+                 value);
+  } else if (key == "trsv_exact_poll_sleep") {
+    LACE_REQUIRE(value >= 0 && value <= 64, DEV_ERR_ILLEGAL_VALUE, "%s must be 0..64 (have %d)", name, value);
   } else if (key == "timing_sample") {
     LACE_REQUIRE(value >= 1, DEV_ERR_ILLEGAL_VALUE, "timing_sample must be >= 1 (have %d)", value);
     std::lock_guard<std::mutex> lk(g_tmu);

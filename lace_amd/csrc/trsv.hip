@@ -225,16 +225,23 @@ __global__ __launch_bounds__(256) void tri_syncfree_kernel(int nslots, const int
 // outside itself belongs to a block handed out earlier.  A workgroup has
 // 2 BS lanes: lane t < BS computes row lo + t, lane BS + t fetches that
 // row's external operands.
+//   - LDS holds the block's solved rows (cells 0..BS-1), the staged external
+//     operands (cell BS + q BS + t for slot q of local row t: consecutive
+//     lanes touch consecutive cells) and one cell that never holds the
+//     sentinel.
 //   - The fetch lane polls the solution vector (device-coherent loads; the
 //     NaN sentinel marks a row not yet solved) for the row's first R
 //     off-diagonal operands that lie outside the block and stages each one
-//     in LDS as it arrives.
-//   - The compute lane keeps its row's first R column indices and values in
-//     registers and walks them in CSR order, taking each operand from LDS:
-//     the block's own solved rows or the staged external values.  It never
-//     waits on device memory, so a dependency chain inside the block moves
-//     at LDS speed; the compute lanes of a wave retry until every row they
-//     hold is solved.  Operands past the first R come straight from memory.
+//     as it arrives.
+//   - The compute lane keeps, for its row's first R slots, the LDS cell of
+//     the operand (a solved row of the block or a staged external value)
+//     and the matrix value in registers.  Each round it reads all R cells
+//     at once; when none holds the sentinel it sums the products in CSR
+//     order and divides.  It never waits on device memory, so a dependency
+//     chain inside the block moves at LDS speed.  Operands past the first R
+//     (rows longer than R) come straight from memory afterwards.  Waves
+//     with nothing to do back off (up to 4 x 64 cycles) to leave the SIMD to
+//     waves that can move.
 //   - A solved row goes to LDS for the block and to the solution vector
 //     (device-coherent store) for later blocks.
 // The earliest unfinished block always has every external operand solved
@@ -250,18 +257,21 @@ __global__ __launch_bounds__(2 * BS) void tri_persist_kernel(int n, int nblk, co
                                                              const int* __restrict__ col,
                                                              const T* __restrict__ val,
                                                              const T* __restrict__ r, T* y, unsigned* ticket,
-                                                             int* err, int spin_limit) {
+                                                             int* err, int spin_limit, int poll_sleep) {
 #pragma clang fp contract(off)
   using UB = typename Sentinel<T>::U;
   constexpr UB SENT = Sentinel<T>::bits;
-  __shared__ UB ly[BS];      // the block's solved rows
-  __shared__ UB lx[BS * R];  // staged external operands, slot q of local row t at t R + q
+  constexpr int kOne = BS + BS * R;  // the cell that never holds the sentinel
+  __shared__ UB lv[BS + BS * R + 1];
   __shared__ int s_blk, s_next;
   const int tid = threadIdx.x;
   const bool fetcher = tid >= BS;
   const int t = fetcher ? tid - BS : tid;
   UB* yb = reinterpret_cast<UB*>(y);
-  if (tid == 0) s_blk = (int)__hip_atomic_fetch_add(ticket, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  if (tid == 0) {
+    s_blk = (int)__hip_atomic_fetch_add(ticket, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(&lv[kOne], UB(0), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+  }
   __syncthreads();
   int blk = s_blk;
   while (blk < nblk) {
@@ -274,10 +284,10 @@ __global__ __launch_bounds__(2 * BS) void tri_persist_kernel(int n, int nblk, co
       hi = lo + BS < n ? lo + BS : n;
     }
     if (!fetcher) {
-      __hip_atomic_store(&ly[t], SENT, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+      __hip_atomic_store(&lv[t], SENT, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 #pragma unroll
       for (int q = 0; q < R; q++)
-        __hip_atomic_store(&lx[t * R + q], SENT, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        __hip_atomic_store(&lv[BS + q * BS + t], SENT, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
     }
     __syncthreads();
     const int i = lo + t;
@@ -312,7 +322,7 @@ __global__ __launch_bounds__(2 * BS) void tri_persist_kernel(int n, int nblk, co
 #pragma unroll
         for (int q = 0; q < R; q++)
           if (((pend >> q) & 1u) && b[q] != SENT) {
-            __hip_atomic_store(&lx[t * R + q], b[q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            __hip_atomic_store(&lv[BS + q * BS + t], b[q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
             pend &= ~(1u << q);
           }
         if (pend != 0u && ++spins > spin_limit) {
@@ -320,35 +330,48 @@ __global__ __launch_bounds__(2 * BS) void tri_persist_kernel(int n, int nblk, co
           pend = 0u;
         }
         if (!__any(pend != 0u)) break;
-        __builtin_amdgcn_s_sleep(1);
+        for (int z = 0; z < poll_sleep; z++) __builtin_amdgcn_s_sleep(1);
       }
       // the next block's ticket, fetched while the compute lanes work
       if (t == 0)
         s_next = (int)__hip_atomic_fetch_add(ticket, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     } else {
+      int li[R];
       T vq[R];
 #pragma unroll
-      for (int q = 0; q < R; q++) vq[q] = q < len ? val[k0 + q] : T(0);
+      for (int q = 0; q < R; q++) {
+        const int c = cq[q];
+        li[q] = q < len ? ((c >= lo && c < hi) ? c - lo : BS + q * BS + t) : kOne;
+        vq[q] = q < len ? val[k0 + q] : T(0);
+      }
       const T ri = active ? r[i] : T(0);
       const T di = active ? val[kd] : T(1);
+      const int head = len < R ? len : R;  // slots held in registers
       T s = T(0);
-      int p = 0;
+      int p = 0;  // slots summed so far
       bool done = !active;
-      int spins = 0;
+      int spins = 0, idle = 0;
+      // the row's loads complete here, so no wait inside the loop covers the
+      // solved values' stores of earlier rounds
+      __builtin_amdgcn_s_waitcnt(0);
       while (true) {
         bool moved = false;
         if (!done) {
+          if (p == 0) {
+            UB b[R];
 #pragma unroll
-          for (int q = 0; q < R; q++) {
-            if (q == p && q < len) {
-              const int c = cq[q];
-              const UB b = (c >= lo && c < hi)
-                               ? __hip_atomic_load(&ly[c - lo], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)
-                               : __hip_atomic_load(&lx[t * R + q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-              if (b != SENT) {
-                s = s + vq[q] * __builtin_bit_cast(T, b);
-                p++;
+            for (int q = 0; q < R; q++)
+              b[q] = __hip_atomic_load(&lv[li[q]], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            bool ready = true;
+#pragma unroll
+            for (int q = 0; q < R; q++) ready = ready && b[q] != SENT;
+            if (ready) {
+#pragma unroll
+              for (int q = 0; q < R; q++) {
+                const T a = s + vq[q] * __builtin_bit_cast(T, b[q]);
+                s = q < head ? a : s;
               }
+              p = head;
             }
           }
           // operands past the first R slots (rows longer than R)
@@ -356,7 +379,7 @@ __global__ __launch_bounds__(2 * BS) void tri_persist_kernel(int n, int nblk, co
             const int k = k0 + p;
             const int c = col[k];
             const UB b = (c >= lo && c < hi)
-                             ? __hip_atomic_load(&ly[c - lo], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)
+                             ? __hip_atomic_load(&lv[c - lo], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)
                              : __hip_atomic_load(yb + c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             if (b == SENT) break;
             s = s + val[k] * __builtin_bit_cast(T, b);
@@ -365,7 +388,7 @@ __global__ __launch_bounds__(2 * BS) void tri_persist_kernel(int n, int nblk, co
           if (p == len) {
             const T yi = (ri - s) / di;
             const UB ub = __builtin_bit_cast(UB, yi);
-            __hip_atomic_store(&ly[t], ub, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            __hip_atomic_store(&lv[t], ub, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
             __hip_atomic_store(yb + i, ub, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             done = true;
             moved = true;
@@ -375,7 +398,12 @@ __global__ __launch_bounds__(2 * BS) void tri_persist_kernel(int n, int nblk, co
           }
         }
         if (__all(done)) break;
-        if (!__any(moved)) __builtin_amdgcn_s_sleep(1);
+        if (__any(moved)) {
+          idle = 0;
+        } else {
+          idle = idle < 4 ? idle + 1 : 4;
+          for (int z = 0; z < idle; z++) __builtin_amdgcn_s_sleep(1);
+        }
       }
     }
     __syncthreads();
@@ -1021,60 +1049,67 @@ int tri_solve_syncfree(const TriPlan& P, const T* val, const T* r, T* y, int* er
   return DEV_SUCCESS;
 }
 
-// Rows per block and register slots of the persistent exact solve.
-constexpr int kPersistBS = 256;
+// Register slots of the persistent exact solve (rows per block: 512 by
+// default, 256 on request).
 constexpr int kPersistR = 8;
 // Poll rounds before a persistent-solve lane gives up (each round sleeps
 // about 64 cycles when nothing moved): far beyond any legitimate wait.
 static int g_persist_spin_limit = 1 << 22;
 
-template <typename T, bool UPPER>
-static int persist_grid(int nblk) {
+template <typename T, bool UPPER, int BS>
+static void persist_launch(int n, const int* row, const int* col, const T* val, const T* r, T* y,
+                           unsigned* ticket, int* err, int wgs_per_cu, int poll_sleep) {
   static int occ = -1;
   if (occ < 0) {
     int o = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&o, tri_persist_kernel<T, UPPER, kPersistBS, kPersistR>,
-                                                     2 * kPersistBS, 0) != hipSuccess || o < 1)
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&o, tri_persist_kernel<T, UPPER, BS, kPersistR>, 2 * BS, 0) !=
+            hipSuccess ||
+        o < 1)
       o = 1;
     occ = o;
   }
-  const long g = (long)occ * std::max(1, num_cus());
-  return (int)std::min<long>(g, nblk);
+  const int nblk = ceil_div(n, BS);
+  const int per = wgs_per_cu > 0 ? std::min(wgs_per_cu, occ) : occ;
+  const int grid = (int)std::min<long>((long)per * std::max(1, num_cus()), nblk);
+  tri_persist_kernel<T, UPPER, BS, kPersistR><<<grid, 2 * BS, 0, stream()>>>(
+      n, nblk, row, col, val, r, y, ticket, err, g_persist_spin_limit, poll_sleep);
 }
 
 template <typename T>
 int tri_solve_persist_raw(int n, bool upper, const int* row, const int* col, const T* val, const T* r, T* y,
-                          unsigned* ticket, int* err, bool init) {
+                          unsigned* ticket, int* err, bool init, int wgs_per_cu, int poll_sleep, int rows) {
   if (n <= 0) return DEV_SUCCESS;
   LACE_REQUIRE((const void*)r != (const void*)y, DEV_ERR_ILLEGAL_VALUE, "exact solve needs distinct r and y");
-  const int nblk = ceil_div(n, kPersistBS);
+  LACE_REQUIRE(rows == 256 || rows == 512, DEV_ERR_ILLEGAL_VALUE, "exact solve: %d rows per block", rows);
   if (init) tri_persist_init_kernel<T><<<ceil_div(n, 256), 256, 0, stream()>>>(n, y, ticket);
-  if (upper)
-    tri_persist_kernel<T, true, kPersistBS, kPersistR><<<persist_grid<T, true>(nblk), 2 * kPersistBS, 0, stream()>>>(
-        n, nblk, row, col, val, r, y, ticket, err, g_persist_spin_limit);
-  else
-    tri_persist_kernel<T, false, kPersistBS, kPersistR><<<persist_grid<T, false>(nblk), 2 * kPersistBS, 0,
-                                                           stream()>>>(n, nblk, row, col, val, r, y, ticket, err,
-                                                                       g_persist_spin_limit);
+  if (rows == 512) {
+    if (upper) persist_launch<T, true, 512>(n, row, col, val, r, y, ticket, err, wgs_per_cu, poll_sleep);
+    else persist_launch<T, false, 512>(n, row, col, val, r, y, ticket, err, wgs_per_cu, poll_sleep);
+  } else {
+    if (upper) persist_launch<T, true, 256>(n, row, col, val, r, y, ticket, err, wgs_per_cu, poll_sleep);
+    else persist_launch<T, false, 256>(n, row, col, val, r, y, ticket, err, wgs_per_cu, poll_sleep);
+  }
   LACE_HIP(hipGetLastError());
   return DEV_SUCCESS;
 }
 
-// Exact solve with a plan: the persistent kernel (option "trsv_exact" 1,
-// the default) or the level-ordered sync-free kernel (0).
+// Exact solve with a plan: the persistent natural-order kernel (option
+// "trsv_exact" 1, the default) or the level-ordered sync-free kernel (0).
 template <typename T>
 int tri_solve_exact(const TriPlan& P, const T* val, const T* r, T* y, unsigned* ticket, int* err) {
   if (get_option("trsv_exact", 1) == 0) return tri_solve_syncfree<T>(P, val, r, y, err);
   const double es = sizeof(T);
   const double bytes = (double)P.nnz * (es + 4.0) + 4.0 * (P.n + 1) + 3.0 * es * P.n;
   TimedRegion tr(P.upper ? "trsv_exact_upper" : "trsv_exact_lower", 0.0, bytes);
-  return tri_solve_persist_raw<T>(P.n, P.upper, P.row.as<int>(), P.col.as<int>(), val, r, y, ticket, err);
+  return tri_solve_persist_raw<T>(P.n, P.upper, P.row.as<int>(), P.col.as<int>(), val, r, y, ticket, err, true,
+                                  get_option("trsv_exact_wgs_per_cu", 0), get_option("trsv_exact_poll_sleep", 1),
+                                  get_option("trsv_exact_rows", 512));
 }
 
 template int tri_solve_persist_raw<double>(int, bool, const int*, const int*, const double*, const double*,
-                                           double*, unsigned*, int*, bool);
+                                           double*, unsigned*, int*, bool, int, int, int);
 template int tri_solve_persist_raw<float>(int, bool, const int*, const int*, const float*, const float*, float*,
-                                          unsigned*, int*, bool);
+                                          unsigned*, int*, bool, int, int, int);
 
 // One sweep yout = D^{-1}(r - N yin).  With y2 and d2 given, the sweep on
 // the diagonal form also writes y2 = yout / d2 and sets *epi.
