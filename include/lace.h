@@ -269,6 +269,25 @@ int data_parcsrtrsv(const data_uplo_t uplo, const data_storage_t storage,
                     const dataType *rhsval, dataType *yval,
                     const dataType tol, int *iter);
 
+/* Allocation-free data_parcsrtrsv on device arrays.  No counterpart in the
+ * reference: for callers that cannot allocate device memory (code under HIP
+ * graph capture, serving loops, harnesses that forbid hipMalloc).  Same
+ * arguments and result as data_parcsrtrsv, except that Aval, row, col,
+ * rhsval, yval and iter are device pointers and the call is asynchronous on
+ * the library stream: *iter (device) receives 0, 1 or 2 as data_parcsrtrsv's
+ * does, with the step summed in a fixed tree order, or -1 if the solve gave
+ * up waiting.  It allocates nothing: its scratch is a fixed array on each
+ * GPU (the caller's initial y for the step, partial sums, a block ticket),
+ * so num_rows is at most LACE_NOALLOC_MAX_ROWS (DEV_ERR_ILLEGAL_VALUE
+ * beyond).  Calls on one GPU share that array: calls that overlap on one
+ * GPU race.  Call lace_init first. */
+#define LACE_NOALLOC_MAX_ROWS 4194304
+int lace_parcsrtrsv_noalloc(const data_uplo_t uplo, const data_storage_t storage,
+                            const data_diagorder_t diag, const int num_rows,
+                            const dataType *Aval, const int *row, const int *col,
+                            const dataType *rhsval, dataType *yval,
+                            const dataType tol, int *iter);
+
 /* FGMRES.  sparse.h:1027-1052; src/fgmres.cpp:64-517,
  * src/fgmres_householder_restart.cpp:63-788.  L: CSRL, U: CSRU.  x0 is
  * freed and re-allocated with the solution (src/fgmres.cpp:491-492).  The

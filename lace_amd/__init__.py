@@ -140,6 +140,7 @@ SIGNATURES = {
     "data_forward_solve": (_i, [_DM, _DM, _DM, _d, _P(_i)]),
     "data_backward_solve": (_i, [_DM, _DM, _DM, _d, _P(_i)]),
     "data_parcsrtrsv": (_i, [_i, _i, _i, _i, _P(_d), _P(_i), _P(_i), _P(_d), _P(_d), _d, _P(_i)]),
+    "lace_parcsrtrsv_noalloc": (_i, [_i, _i, _i, _i, _vp, _vp, _vp, _vp, _vp, _d, _vp]),
     "data_fgmres": (_i, [_DM, _DM, _DM, _DM, _DM, _P(GmresParam), _P(GmresLog)]),
     "data_fgmres_restart": (_i, [_DM, _DM, _DM, _DM, _DM, _P(GmresParam), _P(GmresLog)]),
     "data_fgmres_householder": (_i, [_DM, _DM, _DM, _DM, _DM, _P(GmresParam), _P(GmresLog)]),

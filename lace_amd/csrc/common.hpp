@@ -53,6 +53,9 @@ int ensure_init();
 hipStream_t stream();
 int device_id();
 int num_cus();
+// Device address of the allocation-free entry point's scratch array on
+// this GPU, looked up once per device (noalloc.hip).
+int noalloc_setup(int device);
 // Kernel-shape option set through lace_set_option (dflt when unset).
 int get_option(const char* name, int dflt);
 

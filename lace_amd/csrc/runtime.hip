@@ -52,6 +52,8 @@ static int init_device(int dev) {
   LACE_HIP(hipStreamCreateWithFlags(&g_stream, hipStreamNonBlocking));
   g_device = dev;
   g_cus = prop.multiProcessorCount;
+  // this GPU's allocation-free scratch address (noalloc.hip)
+  LACE_TRY(noalloc_setup(dev));
   return DEV_SUCCESS;
 }
 
