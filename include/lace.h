@@ -401,8 +401,12 @@ int lace_csr_spmv(const lace_csr *A, double alpha, const void *x, double beta,
  * diagonal form, else ELLPACK, else CSR (options "spmv_dia", "spmv_ell"). */
 int lace_csr_forms(const lace_csr *A, int *ndiag, int *ell_width);
 
-/* ParILU(0) on device.  Factor storage in prec (F64 or F32), iteration
- * arithmetic in the same precision. */
+/* ParILU(0) on device.  Factor storage in prec: F64 (FP64 iteration), F32,
+ * or BF16 / F16 (FP32 iteration; at reset and after every sweep each factor
+ * value is rounded to 16 bits relative to the power-of-two scale of its row,
+ * 2^e <= max_j |a_ij| < 2^(e+1), with L's unit diagonal kept exact; the
+ * Jacobi sweeps of L and U then read 16-bit words).  Vectors passed to the
+ * solves are FP64 for F64 and FP32 otherwise. */
 int lace_parilu_create(const data_d_matrix *A, lace_prec_t prec,
                        lace_parilu **out);
 int lace_parilu_destroy(lace_parilu *P);
@@ -509,7 +513,7 @@ typedef struct lace_solver_opts {
   int    search_max;    /* restart length m */
   int    restart_max;
   double rtol;          /* relative to ||b - A x0|| */
-  lace_prec_t prec_precond;   /* factor / trsv precision */
+  lace_prec_t prec_precond;   /* factor storage: F64, F32, BF16 or F16 */
   lace_trsv_kind_t trsv;
   int    jacobi_sweeps;
   int    parilu_sweeps;
@@ -641,6 +645,9 @@ int lace_block_panel_qr(int n, int r0, const double *W, double *Y, double *Tb, d
  * lines to stdout (src/fgmres.cpp:71,178,430; src/fgmres_householder_restart.cpp:
  * 70,81,196,231,707) -- "FGMRES_Householders_partrsv_search(j) = |g_j|;" etc.,
  * the log format its post-processing scripts read; 0 silences them.
+ * "jacobi_dia16" (0 or 1, default 1): with 16-bit factor storage the Jacobi
+ * sweeps on the diagonal form read the 16-bit words (0: the FP32 copy of
+ * the same values).
  * "trsv_exact" (0 or 1, default 1): the exact (level) triangular solves run
  * the persistent natural-order kernel (1: blocks of consecutive rows taken
  * in ticket order, dependencies inside a block resolved through LDS, across

@@ -38,49 +38,6 @@ constexpr bool is16() {
   return sizeof(TS) == 2;
 }
 
-// bfloat16 from float, round to nearest even (NaN kept quiet).
-__device__ __forceinline__ uint16_t f2bf(float v) {
-  uint32_t u = __float_as_uint(v);
-  if ((u & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((u >> 16) | 0x40u);
-  u += 0x7fffu + ((u >> 16) & 1u);
-  return (uint16_t)(u >> 16);
-}
-
-// Storage load/store: 16-bit entries carry the scale of their scalar row.
-template <typename TC, typename TS>
-__device__ __forceinline__ TC ldv(const TS* p, size_t i, float s) {
-  if constexpr (std::is_same<TS, uint16_t>::value) {
-    return __uint_as_float((uint32_t)p[i] << 16) * s;
-  } else if constexpr (std::is_same<TS, _Float16>::value) {
-    return (float)p[i] * s;
-  } else {
-    (void)s;
-    return (TC)p[i];
-  }
-}
-template <typename TC, typename TS>
-__device__ __forceinline__ TS quant(TC v, float sinv) {
-  if constexpr (std::is_same<TS, uint16_t>::value) {
-    return f2bf(v * sinv);
-  } else if constexpr (std::is_same<TS, _Float16>::value) {
-    return (_Float16)(v * sinv);
-  } else {
-    (void)sinv;
-    return (TS)v;
-  }
-}
-template <typename TC, typename TS>
-__device__ __forceinline__ TC dequant(TS q, float s) {
-  if constexpr (std::is_same<TS, uint16_t>::value) {
-    return __uint_as_float((uint32_t)q << 16) * s;
-  } else if constexpr (std::is_same<TS, _Float16>::value) {
-    return (float)q * s;
-  } else {
-    (void)s;
-    return (TC)q;
-  }
-}
-
 // Gauss-Jordan inverse without pivoting, in the fixed order of the CPU
 // restatement: pivot p scales its row, then every other row eliminates
 // column p.

@@ -5,6 +5,9 @@
 #include <hip/hip_runtime.h>
 #include <hip/hip_bf16.h>
 
+#include <cstdint>
+#include <type_traits>
+
 namespace lace {
 
 // Full-wave (64 lanes) sum with DPP/permute shuffles.  Deterministic: the
@@ -155,6 +158,53 @@ __device__ __forceinline__ double as_double(__hip_bfloat16 v) { return (double)_
 template <typename To, typename From>
 __device__ __forceinline__ To convert(From v) {
   return static_cast<To>(v);
+}
+
+// 16-bit factor storage with a power-of-two scale s per scalar row: the
+// stored word is the value times 1/s rounded to BF16 (uint16_t, round to
+// nearest even) or FP16 (_Float16); scaling by a power of two is exact, so
+// dequant(quant(v / s)) * s is v rounded to 16 bits relative to its row.
+// bfloat16 from float, round to nearest even (NaN kept quiet).
+__device__ __forceinline__ uint16_t f2bf(float v) {
+  uint32_t u = __float_as_uint(v);
+  if ((u & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((u >> 16) | 0x40u);
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+
+// Storage load/store: 16-bit entries carry the scale of their scalar row.
+template <typename TC, typename TS>
+__device__ __forceinline__ TC ldv(const TS* p, size_t i, float s) {
+  if constexpr (std::is_same<TS, uint16_t>::value) {
+    return __uint_as_float((uint32_t)p[i] << 16) * s;
+  } else if constexpr (std::is_same<TS, _Float16>::value) {
+    return (float)p[i] * s;
+  } else {
+    (void)s;
+    return (TC)p[i];
+  }
+}
+template <typename TC, typename TS>
+__device__ __forceinline__ TS quant(TC v, float sinv) {
+  if constexpr (std::is_same<TS, uint16_t>::value) {
+    return f2bf(v * sinv);
+  } else if constexpr (std::is_same<TS, _Float16>::value) {
+    return (_Float16)(v * sinv);
+  } else {
+    (void)sinv;
+    return (TS)v;
+  }
+}
+template <typename TC, typename TS>
+__device__ __forceinline__ TC dequant(TS q, float s) {
+  if constexpr (std::is_same<TS, uint16_t>::value) {
+    return __uint_as_float((uint32_t)q << 16) * s;
+  } else if constexpr (std::is_same<TS, _Float16>::value) {
+    return (float)q * s;
+  } else {
+    (void)s;
+    return (TC)q;
+  }
 }
 
 }  // namespace lace

@@ -310,6 +310,49 @@ __global__ void parilu_unitdiag_kernel(int n, const int* __restrict__ lrow, cons
   }
 }
 
+// 16-bit storage: each factor value becomes its 16-bit word relative to
+// the scale s of its row, times s (exact: s is a power of two), in place;
+// L's diagonal (the unit) is left alone.  STEP: per-block partial sums of
+// (old - rounded new)^2 over both factors.
+template <typename TS, bool STEP>
+__global__ __launch_bounds__(256) void parilu_round16_kernel(int nnzL, int nnzU, const int* __restrict__ lrowof,
+                                                             const int* __restrict__ lcol,
+                                                             const int* __restrict__ uri,
+                                                             const float* __restrict__ rs, float* __restrict__ L,
+                                                             float* __restrict__ U, const float* __restrict__ Lold,
+                                                             const float* __restrict__ Uold,
+                                                             double* __restrict__ partial) {
+  const int b = xcd_block(blockIdx.x, gridDim.x);
+  const int t = b * 256 + threadIdx.x;
+  double d2 = 0.0;
+  if (t < nnzL) {
+    const int i = lrowof[t];
+    if (lcol[t] != i) {
+      const float sc = rs[i];
+      const float v = dequant<float, TS>(quant<float, TS>(L[t], 1.0f / sc), sc);
+      L[t] = v;
+      if (STEP) {
+        const double d = (double)Lold[t] - (double)v;
+        d2 += d * d;
+      }
+    }
+  }
+  if (t < nnzU) {
+    const float sc = rs[uri[t]];
+    const float v = dequant<float, TS>(quant<float, TS>(U[t], 1.0f / sc), sc);
+    U[t] = v;
+    if (STEP) {
+      const double d = (double)Uold[t] - (double)v;
+      d2 += d * d;
+    }
+  }
+  if (STEP) {
+    __shared__ double sm[4];
+    const double tot = block_sum<256>(d2, sm);
+    if (threadIdx.x == 0) partial[b] = tot;
+  }
+}
+
 // Residual of the fixed point on A's pattern: (LU)_ij - a_ij with the full
 // merge (no add-back), summed as squares per block.
 template <typename T>
@@ -535,6 +578,23 @@ __global__ void convert_kernel(long n, const From* __restrict__ in, To* __restri
 }
 
 // ---------------------------------------------------------------- host side
+// Power-of-two scale of each row for 16-bit factor storage: 2^e with
+// 2^e <= max_j |a_ij| < 2^(e+1) (1 for a zero row), e clamped to the FP32
+// normal range.
+static void scalar_row_scales(int n, const int* row, const double* val, std::vector<float>& rs) {
+  rs.assign(n, 1.0f);
+  for (int i = 0; i < n; i++) {
+    double m = 0.0;
+    for (int k = row[i]; k < row[i + 1]; k++) m = std::max(m, std::fabs(val[k]));
+    if (m > 0.0 && std::isfinite(m)) {
+      int e = 0;
+      (void)std::frexp(m, &e);  // m = f 2^e, f in [0.5, 1)
+      e = std::max(-126, std::min(127, e - 1));
+      rs[i] = std::ldexp(1.0f, e);
+    }
+  }
+}
+
 static int build_pattern(const data_d_matrix* A, ParILU* P) {
   const int n = A->num_rows;
   LACE_REQUIRE(A->num_rows == A->num_cols, DEV_ERR_ILLEGAL_VALUE, "ParILU needs a square matrix");
@@ -594,6 +654,15 @@ static int build_pattern(const data_d_matrix* A, ParILU* P) {
   double s = 0.0;
   for (int k = 0; k < nnz; k++) s += A->val[k] * A->val[k];
   P->anorm = std::sqrt(s);
+  if (P->s16()) {
+    std::vector<float> rs;
+    scalar_row_scales(n, A->row, A->val, rs);
+    LACE_TRY(upload(P->rs, rs.data(), sizeof(float) * n));
+    std::vector<int> lr(P->nnzL);
+    for (int i = 0; i < n; i++)
+      for (int p = P->h_lrow[i]; p < P->h_lrow[i + 1]; p++) lr[p] = i;
+    LACE_TRY(upload(P->lrowof, lr.data(), sizeof(int) * P->nnzL));
+  }
   // symmetric pattern: the CSCU arrays of U equal the CSRL arrays of L
   P->sym = P->h_lrow == P->h_ucp && P->h_lcol == P->h_uri;
   if (P->sym) {
@@ -711,6 +780,7 @@ int ParILU::reset() {
   LACE_HIP(hipGetLastError());
   cur = 0;
   csr_dirty = true;
+  if (s16()) LACE_TRY(round16(true, false));
   return DEV_SUCCESS;
 }
 
@@ -719,6 +789,15 @@ int ParILU::set_values(const double* vals) {
   double s = 0.0;
   for (int k = 0; k < nnz; k++) s += vals[k] * vals[k];
   anorm = std::sqrt(s);
+  if (s16()) {
+    std::vector<int> arow(n + 1);
+    LACE_HIP(hipMemcpyAsync(arow.data(), this->arow.p, sizeof(int) * (n + 1), hipMemcpyDeviceToHost, stream()));
+    LACE_HIP(hipStreamSynchronize(stream()));
+    std::vector<float> h_rs;
+    scalar_row_scales(n, arow.data(), vals, h_rs);
+    LACE_HIP(hipMemcpyAsync(rs.p, h_rs.data(), sizeof(float) * n, hipMemcpyHostToDevice, stream()));
+    LACE_HIP(hipStreamSynchronize(stream()));
+  }
   DevBuf a64;
   LACE_TRY(upload(a64, vals, sizeof(double) * nnz));
   const int nb = ceil_div(nnz, 256);
@@ -851,8 +930,45 @@ static int sweep_t(ParILU* P, bool with_step) {
   return DEV_SUCCESS;
 }
 
+int ParILU::round16(bool both_buffers, bool with_step) {
+  const int nb = ceil_div(std::max(nnzL, nnzU), 256);
+  for (int bsel = 0; bsel < (both_buffers ? 2 : 1); bsel++) {
+    const int buf = both_buffers ? bsel : cur;
+    float* Lv = lval[buf].as<float>();
+    float* Uv = uval[buf].as<float>();
+    const float* Lo = lval[1 - buf].as<float>();
+    const float* Uo = uval[1 - buf].as<float>();
+    const int* lr = lrowof.as<int>();
+    const int* lc = lcol.as<int>();
+    const int* ur = uri.as<int>();
+    const float* r = rs.as<float>();
+    double* pt = partial.as<double>();
+    if (prec == LACE_BF16) {
+      if (with_step)
+        parilu_round16_kernel<uint16_t, true><<<nb, 256, 0, stream()>>>(nnzL, nnzU, lr, lc, ur, r, Lv, Uv, Lo, Uo, pt);
+      else
+        parilu_round16_kernel<uint16_t, false><<<nb, 256, 0, stream()>>>(nnzL, nnzU, lr, lc, ur, r, Lv, Uv, Lo, Uo,
+                                                                       pt);
+    } else {
+      if (with_step)
+        parilu_round16_kernel<_Float16, true><<<nb, 256, 0, stream()>>>(nnzL, nnzU, lr, lc, ur, r, Lv, Uv, Lo, Uo, pt);
+      else
+        parilu_round16_kernel<_Float16, false><<<nb, 256, 0, stream()>>>(nnzL, nnzU, lr, lc, ur, r, Lv, Uv, Lo, Uo,
+                                                                       pt);
+    }
+  }
+  LACE_HIP(hipGetLastError());
+  if (with_step) last_nb = nb;
+  return DEV_SUCCESS;
+}
+
 int ParILU::sweep(bool with_step) {
-  return prec == LACE_F64 ? sweep_t<double>(this, with_step) : sweep_t<float>(this, with_step);
+  if (prec == LACE_F64) return sweep_t<double>(this, with_step);
+  if (!s16()) return sweep_t<float>(this, with_step);
+  // FP32 sweep, then the rounding to 16-bit storage (which forms the step
+  // from the stored values)
+  LACE_TRY(sweep_t<float>(this, false));
+  return round16(false, with_step);
 }
 
 int ParILU::read_step(double* step) {
@@ -945,8 +1061,8 @@ int ParILU::values_f64(double* lv, double* uv) {
 
 int parilu_create(const data_d_matrix* A, lace_prec_t prec, ParILU** out) {
   LACE_TRY(ensure_init());
-  LACE_REQUIRE(prec == LACE_F64 || prec == LACE_F32, DEV_ERR_NOT_SUPPORTED,
-               "ParILU precision %d not supported", (int)prec);
+  LACE_REQUIRE(prec == LACE_F64 || prec == LACE_F32 || prec == LACE_BF16 || prec == LACE_F16,
+               DEV_ERR_NOT_SUPPORTED, "ParILU precision %d not supported", (int)prec);
   auto* P = new ParILU();
   P->prec = prec;
   int rc = build_pattern(A, P);

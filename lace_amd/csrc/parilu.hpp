@@ -60,6 +60,13 @@ struct TriPlan {
   // The same off-diagonal entries in diagonal form when they lie on at most
   // 8 diagonals (stencil factors); gathered with the other forms.
   DiaForm dia;
+  // For factors stored in 16 bits (BF16 or F16): the diagonal form's values
+  // and the diagonal as 16-bit words relative to each row's power-of-two
+  // scale, and that scale's exponent per row; refreshed from the FP32 forms
+  // by refresh_dia16 (the Jacobi sweeps use them when called with dia16_src).
+  lace_prec_t dia16_fmt = LACE_F32;
+  DevBuf dia16, dexp, dval16;
+  const void* dia16_src = nullptr;
   // Pipelined Jacobi on the diagonal form (several sweeps per launch): the
   // per-tile progress flags, the tile ticket counter, the intermediate
   // sweep vectors, and the next launch's ticket and flag bases.
@@ -67,6 +74,9 @@ struct TriPlan {
   mutable unsigned pipe_tickets = 0, pipe_epoch = 0;
   int build(int n, const int* row, const int* col, bool upper);
   int refresh_strict(const void* val, size_t elem_size);
+  // 16-bit diagonal form (fmt BF16 or F16) from the current FP32 forms,
+  // whose values are already 16-bit values times the row scales rs.
+  int refresh_dia16(lace_prec_t fmt, const float* rs);
 };
 
 // y = T^{-1} r, exact.  Level-scheduled: one launch per level.
@@ -144,6 +154,15 @@ struct ParILU {
   // up waiting, [2] the persistent exact solve's block ticket
   DevBuf syncfree_err;
 
+  // 16-bit factor storage (prec BF16 or F16): the sweeps iterate in FP32
+  // on the FP32 value arrays, and at reset and after every sweep each
+  // factor value is rounded to 16 bits relative to the power-of-two scale
+  // of its row (rs, from A's row maxima); L's unit diagonal stays exact.
+  // The triangular solves read the rounded values (the Jacobi sweeps of L
+  // and U as 16-bit words); lrowof is the row of each L position.
+  DevBuf rs, lrowof;
+  bool s16() const { return prec == LACE_BF16 || prec == LACE_F16; }
+  int round16(bool both_buffers, bool with_step);
   size_t elem_size() const { return prec == LACE_F64 ? 8 : 4; }
   int reset();
   // New values of A on the pattern given at create (host array, nnz

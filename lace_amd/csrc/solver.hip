@@ -390,8 +390,8 @@ static int solver_create(const data_d_matrix* A, const lace_solver_opts* o, Solv
       S->bilu = std::make_unique<PrecBILU>(S->BP, o->trsv, o->jacobi_sweeps);
       base = S->bilu.get();
     } else {
-      LACE_REQUIRE(fp == LACE_F64 || fp == LACE_F32, DEV_ERR_NOT_SUPPORTED,
-                   "solver: CSR factors are stored in F64 or F32 (have %d)", (int)fp);
+      LACE_REQUIRE(fp == LACE_F64 || fp == LACE_F32 || fp == LACE_BF16 || fp == LACE_F16, DEV_ERR_NOT_SUPPORTED,
+                   "solver: CSR factor storage %d not supported", (int)fp);
       LACE_TRY(parilu_create(A, fp, &S->P));
       S->ilu = std::make_unique<PrecILU>(S->P, o->trsv, o->jacobi_sweeps, 0);
       base = S->ilu.get();

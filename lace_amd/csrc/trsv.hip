@@ -131,6 +131,47 @@ int TriPlan::refresh_strict(const void* val, size_t es) {
   return DEV_SUCCESS;
 }
 
+// 16-bit copies of the diagonal form and the diagonal, row i relative to
+// its scale rs[i] = 2^e (stored as the exponent e).
+template <typename TS>
+__global__ void dia16_encode_kernel(int n, int nd, const float* __restrict__ dia, const float* __restrict__ dval,
+                                    const float* __restrict__ rs, TS* __restrict__ dia16, TS* __restrict__ dval16,
+                                    int8_t* __restrict__ dexp) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float sc = rs[i];
+  const float sinv = 1.0f / sc;
+  for (int q = 0; q < nd; q++) dia16[(size_t)q * n + i] = quant<float, TS>(dia[(size_t)q * n + i], sinv);
+  if (dval16) dval16[i] = quant<float, TS>(dval[i], sinv);
+  dexp[i] = (int8_t)((int)((__float_as_uint(sc) >> 23) & 0xffu) - 127);
+}
+
+int TriPlan::refresh_dia16(lace_prec_t fmt, const float* rs) {
+  dia16_fmt = LACE_F32;
+  dia16_src = nullptr;
+  if (dia.nd == 0 || !strict_src) return DEV_SUCCESS;
+  LACE_REQUIRE(fmt == LACE_BF16 || fmt == LACE_F16, DEV_ERR_ILLEGAL_VALUE, "16-bit DIA form: format %d", (int)fmt);
+  const size_t m = (size_t)dia.nd * n;
+  if (dia16.bytes < 2 * m) LACE_TRY(dia16.alloc(2 * m));
+  if (dexp.bytes < (size_t)n) LACE_TRY(dexp.alloc(n));
+  if (!unit_diag && dval16.bytes < 2 * (size_t)n) LACE_TRY(dval16.alloc(2 * (size_t)n));
+  const unsigned nb = (unsigned)ceil_div(n, 256);
+  if (fmt == LACE_BF16)
+    dia16_encode_kernel<uint16_t><<<nb, 256, 0, stream()>>>(n, dia.nd, dia.val.as<float>(), dval.as<float>(), rs,
+                                                             dia16.as<uint16_t>(),
+                                                             unit_diag ? nullptr : dval16.as<uint16_t>(),
+                                                             dexp.as<int8_t>());
+  else
+    dia16_encode_kernel<_Float16><<<nb, 256, 0, stream()>>>(n, dia.nd, dia.val.as<float>(), dval.as<float>(), rs,
+                                                             dia16.as<_Float16>(),
+                                                             unit_diag ? nullptr : dval16.as<_Float16>(),
+                                                             dexp.as<int8_t>());
+  LACE_HIP(hipGetLastError());
+  dia16_fmt = fmt;
+  dia16_src = strict_src;
+  return DEV_SUCCESS;
+}
+
 // ---------------------------------------------------------------- kernels
 template <typename T, bool UPPER>
 __device__ __forceinline__ void row_range(const int* __restrict__ row, int i, int& k0, int& k1,
@@ -768,6 +809,90 @@ __global__ __launch_bounds__(256) void tri_jacobi_dia_vec_kernel(int n, DiaOffse
   }
 }
 
+// Four 16-bit words from an 8-byte aligned address.
+template <typename TS>
+__device__ __forceinline__ void ld4x16(const TS* p, TS (&v)[4]) {
+  const uint2 w = *reinterpret_cast<const uint2*>(p);
+  const uint16_t h[4] = {(uint16_t)(w.x & 0xffffu), (uint16_t)(w.x >> 16), (uint16_t)(w.y & 0xffffu),
+                         (uint16_t)(w.y >> 16)};
+#pragma unroll
+  for (int k = 0; k < 4; k++) v[k] = __builtin_bit_cast(TS, h[k]);
+}
+
+// The FP32 diagonal-form sweep (four rows per thread) on 16-bit factor
+// words: each value is its word times the row's scale 2^e, which is exactly
+// the FP32 factor value, and the sums run as in the FP32 kernel, so the
+// result is bit-identical to it.  A row moves 2 bytes per diagonal, one
+// mask byte and one exponent byte (and 2 bytes of diagonal) instead of 4
+// bytes per diagonal (and 4 of diagonal).
+template <typename TS, bool UNIT, int D, bool EPI>
+__global__ __launch_bounds__(256) void tri_jacobi_dia16_vec_kernel(int n, DiaOffsets off,
+                                                                   const uint8_t* __restrict__ mask,
+                                                                   const int8_t* __restrict__ dexp,
+                                                                   const TS* __restrict__ dia,
+                                                                   const TS* __restrict__ dval,
+                                                                   const float* __restrict__ r,
+                                                                   const float* __restrict__ yin,
+                                                                   float* __restrict__ yout, float* __restrict__ y2,
+                                                                   const float* __restrict__ d2) {
+#pragma clang fp contract(off)
+  constexpr int R = 4;
+  const int i0 = (xcd_block(blockIdx.x, gridDim.x) * 256 + (int)threadIdx.x) * R;
+  if (i0 >= n) return;
+  const uint32_t m4 = *reinterpret_cast<const uint32_t*>(mask + i0);
+  const uint32_t e4 = *reinterpret_cast<const uint32_t*>(dexp + i0);
+  unsigned mk[R];
+  float sc[R];
+#pragma unroll
+  for (int k = 0; k < R; k++) {
+    mk[k] = (m4 >> (8 * k)) & 0xffu;
+    sc[k] = ldexpf(1.0f, (int)(int8_t)((e4 >> (8 * k)) & 0xffu));
+  }
+  float v[D][R];
+#pragma unroll
+  for (int q = 0; q < D; q++) {
+    TS h[R];
+    ld4x16<TS>(dia + (size_t)q * n + i0, h);
+#pragma unroll
+    for (int k = 0; k < R; k++) v[q][k] = dequant<float, TS>(h[k], sc[k]);
+  }
+  const Pack16<float> rr = ld16(r + i0);
+  float s[R];
+#pragma unroll
+  for (int k = 0; k < R; k++) s[k] = 0.0f;
+#pragma unroll
+  for (int q = 0; q < D; q++) {
+    const int sh = ((off.o[q] % R) + R) % R;
+    const int a = i0 + off.o[q] - sh;
+    Pack16<float> lo{}, hi{};
+    if (a >= 0 && a < n) lo = ld16(yin + a);
+    if (sh != 0 && a + R >= 0 && a + R < n) hi = ld16(yin + a + R);
+    float y[R];
+    window16<float>(lo, hi, sh, y);
+#pragma unroll
+    for (int k = 0; k < R; k++)
+      if ((mk[k] >> q) & 1u) s[k] = s[k] + v[q][k] * y[k];
+  }
+  Pack16<float> o;
+  if constexpr (UNIT) {
+#pragma unroll
+    for (int k = 0; k < R; k++) o.v[k] = rr.v[k] - s[k];
+  } else {
+    TS h[R];
+    ld4x16<TS>(dval + i0, h);
+#pragma unroll
+    for (int k = 0; k < R; k++) o.v[k] = (rr.v[k] - s[k]) / dequant<float, TS>(h[k], sc[k]);
+  }
+  st16(yout + i0, o);
+  if constexpr (EPI) {
+    const Pack16<float> dd = ld16(d2 + i0);
+    Pack16<float> e;
+#pragma unroll
+    for (int k = 0; k < R; k++) e.v[k] = o.v[k] / dd.v[k];
+    st16(y2 + i0, e);
+  }
+}
+
 // Device-coherent 16-byte word: two 8-byte agent-scope accesses, which
 // bypass the per-CU and per-XCD caches that other workgroups do not share.
 template <typename T>
@@ -966,6 +1091,47 @@ static void jacobi_dia(const TriPlan& P, const T* r, const T* yin, T* yout, T* y
   }
 }
 
+// The diagonal-form sweep on the 16-bit words of P (float vectors; four
+// rows per thread).  Returns false (launching nothing) where the shape does
+// not allow it: the caller then runs the FP32 form, which gives the same
+// result.
+template <typename TS, bool UNIT>
+static bool jacobi_dia16(const TriPlan& P, const float* r, const float* yin, float* yout, float* y2,
+                         const float* d2, TimedRegion& tr) {
+  auto al16 = [](const void* p) { return ((uintptr_t)p & 15u) == 0; };
+  if (P.n % 4 != 0 || !al16(r) || !al16(yin) || !al16(yout) || (y2 && (!al16(y2) || !al16(d2)))) return false;
+  const uint8_t* mk = P.dia.mask.as<uint8_t>();
+  const int8_t* ex = P.dexp.as<int8_t>();
+  const TS* dv = P.dia16.as<TS>();
+  const TS* dg = UNIT ? nullptr : P.dval16.as<TS>();
+  float* no_y2 = nullptr;
+  const float* no_d2 = nullptr;
+  const int nb = ceil_div(P.n / 4, 256);
+  switch (P.dia.nd) {
+#define LACE_DIA16_CASE(D)                                                                                   \
+  case D:                                                                                                    \
+    if (y2)                                                                                                  \
+      tr.launch(tri_jacobi_dia16_vec_kernel<TS, UNIT, D, true>, dim3(nb), dim3(256), P.n, P.dia.off, mk, ex, \
+                dv, dg, r, yin, yout, y2, d2);                                                               \
+    else                                                                                                     \
+      tr.launch(tri_jacobi_dia16_vec_kernel<TS, UNIT, D, false>, dim3(nb), dim3(256), P.n, P.dia.off, mk,    \
+                ex, dv, dg, r, yin, yout, no_y2, no_d2);                                                     \
+    break;
+    LACE_DIA16_CASE(1)
+    LACE_DIA16_CASE(2)
+    LACE_DIA16_CASE(3)
+    LACE_DIA16_CASE(4)
+    LACE_DIA16_CASE(5)
+    LACE_DIA16_CASE(6)
+    LACE_DIA16_CASE(7)
+    LACE_DIA16_CASE(8)
+#undef LACE_DIA16_CASE
+    default:
+      return false;
+  }
+  return true;
+}
+
 // First sweep from zero: y = r / diag.
 template <typename T>
 __global__ void tri_diag_div_kernel(int n, const T* __restrict__ dval, const T* __restrict__ r,
@@ -1122,18 +1288,37 @@ static int jacobi_sweep(const TriPlan& P, const T* val, const T* r, const T* yin
     const bool dia = strict && P.dia.nd > 0 && get_option("jacobi_dia", 1) != 0;
     const bool ell = !dia && strict && P.ew > 0 && get_option("jacobi_ell", 1) != 0;
     const bool fuse = dia && y2 && d2;
+    // 16-bit words of the diagonal form where the factors are stored in 16
+    // bits (four rows per thread; it needs n % 4 == 0 and aligned vectors)
+    const bool d16 = dia && sizeof(T) == 4 && P.dia16_fmt != LACE_F32 && P.dia16_src == (const void*)val &&
+                     P.n % 4 == 0 && get_option("jacobi_dia16", 1) != 0;
     // bytes the chosen form reads once each, plus r, the previous iterate
     // and the new one (and the diagonal unless it is a unit one)
     const double es = sizeof(T), n = P.n;
-    const double diag = P.unit_diag ? 0.0 : es * n;
+    const double diag = P.unit_diag ? 0.0 : (d16 ? 2.0 : es) * n;
     double bytes = 3.0 * es * n + (fuse ? 2.0 * es * n : 0.0);
-    if (dia) bytes += P.dia.bytes(sizeof(T)) + diag;
+    if (d16) bytes += 2.0 * P.dia.nd * n + 2.0 * n + diag;
+    else if (dia) bytes += P.dia.bytes(sizeof(T)) + diag;
     else if (ell) bytes += (double)P.ew * n * (es + 4.0) + diag;
     else if (strict) bytes += (double)P.snnz * (es + 4.0) + 4.0 * (n + 1) + diag;
     else bytes += (double)P.nnz * (es + 4.0) + 4.0 * (n + 1);
     // the diagonal-form launch carries the events itself
     TimedRegion tr(P.upper ? "trsv_jacobi_sweep_upper" : "trsv_jacobi_sweep_lower", 0.0, bytes, dia);
-    if (dia && P.unit_diag) jacobi_dia<T, true>(P, r, yin, yout, fuse ? y2 : nullptr, d2, tr);
+    bool done16 = false;
+    if constexpr (sizeof(T) == 4) {
+      if (d16) {
+        float* wy2 = fuse ? (float*)y2 : nullptr;
+        if (P.dia16_fmt == LACE_BF16)
+          done16 = P.unit_diag ? jacobi_dia16<uint16_t, true>(P, r, yin, yout, wy2, d2, tr)
+                               : jacobi_dia16<uint16_t, false>(P, r, yin, yout, wy2, d2, tr);
+        else
+          done16 = P.unit_diag ? jacobi_dia16<_Float16, true>(P, r, yin, yout, wy2, d2, tr)
+                               : jacobi_dia16<_Float16, false>(P, r, yin, yout, wy2, d2, tr);
+      }
+    }
+    if (done16) {
+      // launched above
+    } else if (dia && P.unit_diag) jacobi_dia<T, true>(P, r, yin, yout, fuse ? y2 : nullptr, d2, tr);
     else if (dia) jacobi_dia<T, false>(P, r, yin, yout, fuse ? y2 : nullptr, d2, tr);
     else if (ell && P.unit_diag) jacobi_ell<T, true>(P, r, yin, yout);
     else if (ell) jacobi_ell<T, false>(P, r, yin, yout);
@@ -1386,6 +1571,12 @@ int ParILU::refresh_csr() {
   LACE_TRY(upper.refresh_strict(uval_csr.p, es));
   LACE_TRY(lowerT.refresh_strict(uval[cur].p, es));
   LACE_TRY(upperT.refresh_strict(lval_t.p, es));
+  if (s16()) {
+    // the non-transposed Jacobi sweeps read the 16-bit words (a row of L or
+    // U shares its row's scale; the transposed factors do not)
+    LACE_TRY(lower.refresh_dia16(prec, rs.as<float>()));
+    LACE_TRY(upper.refresh_dia16(prec, rs.as<float>()));
+  }
   csr_dirty = false;
   return DEV_SUCCESS;
 }

@@ -282,13 +282,43 @@ def ilures(A: CSR, Lf: CSR, Ucsr: CSR):
 
 
 # ---------------------------------------------------------------- ParILU
+def scalar_row_scales(A: CSR) -> np.ndarray:
+    """Power-of-two scale per row: 2^e with 2^e <= max_j |a_ij| < 2^(e+1)
+    (1 for a zero row), e clamped to the FP32 normal range."""
+    rs = np.ones(A.n_rows, np.float32)
+    for i in range(A.n_rows):
+        v = A.val[A.row[i]:A.row[i + 1]]
+        if len(v) == 0:
+            continue
+        m = np.abs(v).max()
+        if m > 0 and np.isfinite(m):
+            _, e = np.frexp(m)
+            rs[i] = np.float32(2.0) ** int(np.clip(e - 1, -126, 127))
+    return rs
+
+
+def round16(v, s, storage: str) -> np.ndarray:
+    """The 16-bit storage model: v / s rounded to BF16 or FP16 (nearest
+    even), times s, all in FP32 (s a power of two, so the scalings are
+    exact)."""
+    t = (np.asarray(v, np.float32) * (np.float32(1.0) / s)).astype(np.float32)
+    q = bf16_round(t) if storage == "bf16" else t.astype(np.float16).astype(np.float32)
+    return (q * s).astype(np.float32)
+
+
 class ParILUState:
     """L (CSRL, unit diagonal last) and U (CSCU: column pointer in U.row,
     row indices in U.col, diagonal last per column) initialised from A as
-    src/parilu_v0_3.cpp:23-27 does."""
+    src/parilu_v0_3.cpp:23-27 does.  storage 'bf16' / 'f16' iterates in
+    FP32 and, at the start and after every sweep, rounds each factor value
+    to 16 bits relative to the power-of-two scale of its row (L's unit
+    diagonal is kept exact)."""
 
-    def __init__(self, A: CSR, dtype=np.float64):
+    def __init__(self, A: CSR, dtype=np.float64, storage=None):
         self.A = A
+        self.storage = storage
+        if storage:
+            dtype = np.float32
         self.dtype = dtype
         self.L = csr_split(A, False, DIAG_UNIT)
         Ucsr = csr_split(A, True, DIAG_VALUE)
@@ -296,6 +326,17 @@ class ParILUState:
         self.Lval = self.L.val.astype(dtype)
         self.Uval = self.U.val.astype(dtype)
         self.Aval = A.val.astype(dtype)
+        if storage:
+            self.rs = scalar_row_scales(A)
+            self._lrow = np.repeat(np.arange(A.n_rows), np.diff(self.L.row))
+            self._ldiag = self.L.col == self._lrow
+            self._urow = np.asarray(self.U.col)  # CSC: row index of each U entry
+            self.Lval, self.Uval = self._round(self.Lval, self.Uval)
+
+    def _round(self, Lv, Uv):
+        Lq = round16(Lv, self.rs[self._lrow], self.storage)
+        Lq[self._ldiag] = Lv[self._ldiag]
+        return Lq, round16(Uv, self.rs[self._urow], self.storage)
 
     def sweep(self, mode: str = "jacobi") -> float:
         A, L, U = self.A, self.L, self.U
@@ -310,6 +351,12 @@ class ParILUState:
             raise ValueError(mode)
         step = fn(A.n_rows, *map(_ptr, a), *map(_ptr, lrc), _ptr(self.Lval), _ptr(Lout),
                   *map(_ptr, urc), _ptr(self.Uval), _ptr(Uout))
+        if self.storage:
+            if mode != "jacobi":
+                raise ValueError("16-bit storage runs Jacobi sweeps only")
+            Lout, Uout = self._round(Lout, Uout)
+            step = float(np.sum((self.Lval.astype(np.float64) - Lout) ** 2) +
+                         np.sum((self.Uval.astype(np.float64) - Uout) ** 2))
         self.Lval, self.Uval = Lout, Uout
         return step
 
@@ -320,8 +367,8 @@ class ParILUState:
         return Lf, transpose(Uc)
 
 
-def parilu(A: CSR, sweeps: int, mode: str = "jacobi", dtype=np.float64):
-    st = ParILUState(A, dtype)
+def parilu(A: CSR, sweeps: int, mode: str = "jacobi", dtype=np.float64, storage=None):
+    st = ParILUState(A, dtype, storage)
     for _ in range(sweeps):
         st.sweep(mode)
     return st

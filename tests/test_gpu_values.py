@@ -16,7 +16,7 @@ def _perturbed(A, seed=21):
     return A.val * (1.0 + 0.05 * rng.standard_normal(A.nnz)) + 2.0 * (A.col == ri)
 
 
-@pytest.mark.parametrize("prec", ["f64", "f32"])
+@pytest.mark.parametrize("prec", ["f64", "f32", "bf16"])
 @pytest.mark.parametrize("name", ["convdiff", "nonsym_pattern"])
 def test_parilu_set_values_then_reset(gpu, orc, prec, name):
     """create(A) -> set_values(A') -> reset -> k sweeps equals the oracle's
@@ -33,13 +33,17 @@ def test_parilu_set_values_then_reset(gpu, orc, prec, name):
         rows = ri[keep]
         A = orc._coo_to_csr(A.n_rows, rows, A.col[keep], A.val[keep])
     vals = _perturbed(A)
-    lp, dt = (gpu.LACE_F64, np.float64) if prec == "f64" else (gpu.LACE_F32, np.float32)
+    if prec == "bf16":
+        # new row maxima: the 16-bit row scales follow the new values
+        vals = vals * 2.0 ** (np.repeat(np.arange(A.n_rows), np.diff(A.row)) % 5)
+    lp, dt = {"f64": (gpu.LACE_F64, np.float64), "f32": (gpu.LACE_F32, np.float32),
+              "bf16": (gpu.LACE_BF16, np.float32)}[prec]
     P = gpu.ParILU(gpu.HostCSR.from_csr(A), lp)
     P.sweeps(2)
     P.set_values(vals)
     P.reset()
     A2 = orc.CSR(A.n_rows, A.n_cols, A.row, A.col, vals)
-    st = orc.ParILUState(A2, dt)
+    st = orc.ParILUState(A2, dt, storage="bf16" if prec == "bf16" else None)
     for k in range(4):
         P.sweeps(1)
         st.sweep("jacobi")
@@ -47,7 +51,7 @@ def test_parilu_set_values_then_reset(gpu, orc, prec, name):
     assert np.array_equal(lv.astype(dt), st.Lval) and np.array_equal(uv.astype(dt), st.Uval)
     # ||A'||_F scales the step the drop-in stop rules use
     s = P.sweeps(1, want_step=True)
-    st2 = orc.ParILUState(A2, dt)
+    st2 = orc.ParILUState(A2, dt, storage="bf16" if prec == "bf16" else None)
     for _ in range(4):
         st2.sweep("jacobi")
     step = st2.sweep("jacobi") / np.linalg.norm(vals)

@@ -440,3 +440,75 @@ def test_hqrd_magic7_fixture(orc):
     for k in range(7):
         assert abs(np.linalg.norm(U[k:, k]) - np.sqrt(2)) < 1e-13
     assert np.allclose(np.abs(R), np.abs(np.linalg.qr(X)[1]), atol=1e-10)
+
+
+def _frexp_scale(m):
+    import math
+    if m == 0 or not np.isfinite(m):
+        return 1.0
+    _, e = math.frexp(m)
+    return 2.0 ** max(-126, min(127, e - 1))
+
+
+@pytest.mark.parametrize("storage", ["bf16", "f16"])
+def test_parilu_16bit_storage_model(orc, storage):
+    """Scalar 16-bit factor storage: each value rounded relative to the
+    power-of-two scale of its own row.  Rows span 2^-12..2^12 in magnitude,
+    so a scale taken from the wrong row (or column) would lose the value.
+    Per sweep, the oracle's state equals a dense float64 Jacobi sweep from
+    the oracle's previous state, rounded with independently computed row
+    scales (math.frexp) and numpy's / torch's 16-bit conversions, to within
+    one 16-bit step of the row (the sweeps' FP32 rounding may move a value
+    across a rounding boundary); every stored value sits on the 16-bit grid
+    of its row, and L's unit diagonal stays exact."""
+    rng = np.random.default_rng(12)
+    n = 10
+    M = (rng.random((n, n)) < 0.4) * rng.standard_normal((n, n))
+    M += np.diag(5 + rng.random(n))
+    M *= (2.0 ** rng.integers(-12, 13, size=n))[:, None]
+    import scipy.sparse as sp
+    a = sp.csr_matrix(M)
+    a.sort_indices()
+    A = orc.CSR(n, n, a.indptr, a.indices, a.data)
+    s = np.array([_frexp_scale(np.abs(M[i]).max()) for i in range(n)])
+    assert np.array_equal(orc.scalar_row_scales(A), s.astype(np.float32))
+
+    def to16(x):
+        x = np.asarray(x, np.float32)
+        if storage == "f16":
+            return x.astype(np.float16).astype(np.float32)
+        torch = pytest.importorskip("torch")
+        return torch.from_numpy(x).to(torch.bfloat16).to(torch.float32).numpy()
+
+    def q(D):  # round each entry with its row's scale
+        return to16(D / s[:, None]).astype(np.float64) * s[:, None]
+
+    step = (2.0 ** -7 if storage == "bf16" else 2.0 ** -10) * 2.0
+    st = orc.ParILUState(A, storage=storage)
+    mask = M != 0
+    for k in range(4):
+        Lf, Uf = st.factors()
+        L, U = Lf.dense(), Uf.dense()
+        assert np.all(np.diag(L) == 1.0)
+        # every stored value on its row's 16-bit grid
+        assert np.array_equal(q(np.where(mask, np.tril(L, -1) + np.triu(U), 0)),
+                              np.where(mask, np.tril(L, -1) + np.triu(U), 0))
+        Ln, Un = L.copy(), U.copy()
+        for i in range(n):
+            for j in range(n):
+                if not mask[i, j]:
+                    continue
+                m = min(i, j)
+                t = M[i, j] - sum(L[i, kk] * U[kk, j] for kk in range(m))
+                if i > j:
+                    Ln[i, j] = t / U[j, j]
+                else:
+                    Un[i, j] = t
+        Lw = np.where(np.tril(mask, -1), q(Ln), np.eye(n))
+        Uw = np.where(np.triu(mask), q(Un), 0.0)
+        st.sweep("jacobi")
+        Lf, Uf = st.factors()
+        tol = step * np.maximum(np.abs(Lw), s[:, None] * 2.0 ** -20)
+        assert np.all(np.abs(Lf.dense() - Lw) <= tol), k
+        tol = step * np.maximum(np.abs(Uw), s[:, None] * 2.0 ** -20)
+        assert np.all(np.abs(Uf.dense() - Uw) <= tol), k
