@@ -48,6 +48,11 @@ def make_opts(L, **kw):
 
 MIXED = dict(search_max=10, inner_iters=10, trsv=1, jacobi_sweeps=5, prec_precond=1)
 FP64 = dict(search_max=50, trsv=1, jacobi_sweeps=5, prec_precond=0)
+# the mixed path's nested structure entirely in FP64: outer FGMRES(10) x
+# inner FP64 GMRES(10) with FP64 factors
+FP64_NESTED = dict(search_max=10, inner_iters=10, inner_fp64=1, trsv=1, jacobi_sweeps=5, prec_precond=0)
+# the mixed path with BF16 factor storage (per-row power-of-two scales)
+MIXED_BF16 = dict(MIXED, prec_precond=2)
 
 
 def timed_solves(L, S, db, dx, zeros, k, flush=None):
@@ -379,7 +384,7 @@ def bench_reorder(args) -> int:
     return 0 if good else 1
 
 
-def bench_bcsr(args) -> int:
+def bench_bcsr(args, compact=False):
     """Config 3: BCSR 5x5 blocks on the 64^3 7-point block grid
     (n = 1,310,720; 1,810,432 blocks = 45,260,800 entries); entries
     0.1 N(0,1) (seed 1), diagonal blocks + 5 I, b ~ N(0,1) (seed 2).
@@ -400,12 +405,12 @@ def bench_bcsr(args) -> int:
     out = {"mode": "bcsr", "workload": f"config 3: BCSR 5x5, 7-point block grid {N}^3",
            "n": n, "blocks": int(row[-1])}
     # sweeps and SpMV bandwidth
+    variants = ((0, ""), (2, "_merge"), (3, "_tables"), (4, "_tables_lds"), (5, "_tables_lds_lu"))
     for name, prec in (("f32", L.LACE_F32), ("bf16", L.LACE_BF16), ("f64", L.LACE_F64)):
         # forms of the paired sweep: 0 the default, 2 the device-side merge of
         # the two block rows, 3 set-up tables, 4 tables + LDS staging, 5
         # tables + LDS staging of L, U and U_JJ only
-        for variant, tag in ((0, ""), (2, "_merge"), (3, "_tables"), (4, "_tables_lds"),
-                             (5, "_tables_lds_lu")):
+        for variant, tag in (variants[:1] if compact else variants):
             P = L.BILU(H, prec)
             if variant:
                 P.set_variant(variant)
@@ -441,8 +446,13 @@ def bench_bcsr(args) -> int:
             "rel_true_residual": st.rel_true_residual,
             "err_vs_fp64_solution": float(np.linalg.norm(xs - xref) / np.linalg.norm(xref))}
         del S
-    print(json.dumps(out))
-    return 0
+    sw = out["bilu_sweep_f32"]
+    out["roofline"] = {"kernel": "block ParILU sweep (FP32)", "bound": "hbm", "achieved": sw["GBps"],
+                       "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(sw["GBps"] / HBM_PEAK_GBS, 4),
+                       "bytes_per_launch": sw["bytes"], "avg_launch_us": sw["us"]}
+    out["all_converged"] = all(out[f"solve_{k}"]["converged"] == 1 and
+                               out[f"solve_{k}"]["rel_true_residual"] <= 1e-8 for k in ("f32", "bf16", "f64"))
+    return out
 
 
 MFMA_F64_PEAK_TFS = 78.6
@@ -451,7 +461,7 @@ BLOCK_REGIONS = ["tsgemm_tn", "tsgemm_nn", "spmm32_f32x", "spmm32_f32a", "spmm32
                  "inner_update", "block_inner"]
 
 
-def bench_block(args) -> int:
+def bench_block(args):
     """Config 5: 32 N(0,1) right-hand sides (Philox seeds 0-31) on the 128^3
     convection-diffusion matrix.  One step = FP32 ParILU setup (3 sweeps) +
     block Householder FGMRES(30) for the 32 right-hand sides to 1e-8 per
@@ -565,8 +575,7 @@ def bench_block(args) -> int:
                      "mfma_frac": tn.get("mfma_frac")},
         "kernels": kernels,
     }
-    print(json.dumps(out))
-    return 0 if ok else 1
+    return out
 
 
 def main() -> int:
@@ -576,6 +585,8 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--size", type=int, default=128)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-configs", action="store_true",
+                    help="solve mode: skip the compact config-3 (BCSR) and config-5 (32 RHS) summaries")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--block-inner", type=int, default=10,
                     help="block mode: inner FP32 GMRES steps per outer block step (0: ILU only)")
@@ -601,9 +612,13 @@ def main() -> int:
             name, _, value = kv.partition("=")
             L.check(L.lib().lace_set_option(name.encode(), int(value)), f"option {name}")
     if args.mode == "bcsr":
-        return bench_bcsr(args)
+        out = bench_bcsr(args)
+        print(json.dumps(out))
+        return 0 if out["all_converged"] else 1
     if args.mode == "block":
-        return bench_block(args)
+        out = bench_block(args)
+        print(json.dumps(out))
+        return 0 if out["all_converged"] else 1
     if args.mode == "reorder":
         return bench_reorder(args)
 
@@ -635,7 +650,9 @@ def main() -> int:
 
     mixed = L.Solver(H, make_opts(L, **MIXED))
     fp64 = L.Solver(H, make_opts(L, **FP64))
-    for S in (mixed, fp64):
+    fp64n = L.Solver(H, make_opts(L, **FP64_NESTED))
+    mixed16 = L.Solver(H, make_opts(L, **MIXED_BF16))
+    for S in (mixed, fp64, fp64n, mixed16):
         timed_solves(L, S, db, dx, zeros, args.warmup)
 
     # timed region: mixed steps with the hot kernels timed by HIP events on
@@ -652,6 +669,11 @@ def main() -> int:
     calls = {r: L.timing_calls(r) for r in HOT_REGIONS}
     work = {r: L.timing_work(r) for r in HOT_REGIONS}
     res_fp64 = timed_solves(L, fp64, db, dx, zeros, args.steps)
+    x_fp64 = dx.download()
+    res_fp64n = timed_solves(L, fp64n, db, dx, zeros, args.steps)
+    x_fp64n = dx.download()
+    res_m16 = timed_solves(L, mixed16, db, dx, zeros, args.steps)
+    x_m16 = dx.download()
     # the same mixed steps without events (the events' cost), and each
     # started with nothing of its data in the caches
     res_plain = timed_solves(L, mixed, db, dx, zeros, args.steps)
@@ -660,11 +682,23 @@ def main() -> int:
     t_cold = float(np.mean([t for t, _ in res_cold]))
     t_plain = float(np.mean([t for t, _ in res_plain]))
 
+    x_mixed = dx.download()
     t_mixed = float(np.mean([t for t, _ in res_mixed]))
     t_fp64 = float(np.mean([t for t, _ in res_fp64]))
+    t_fp64n = float(np.mean([t for t, _ in res_fp64n]))
+    t_m16 = float(np.mean([t for t, _ in res_m16]))
     st_m, st_f = res_mixed[-1][1], res_fp64[-1][1]
+    st_fn, st_m16 = res_fp64n[-1][1], res_m16[-1][1]
     ok = all(s.converged == 1 and s.rel_true_residual <= 1e-8
-             for _, s in res_mixed + res_fp64 + res_plain + res_cold)
+             for _, s in res_mixed + res_fp64 + res_plain + res_cold + res_fp64n + res_m16)
+    # FP64 oracle error of each solution: against an FP64 solve to 1e-12
+    ref_o = make_opts(L, **FP64)
+    ref_o.rtol = 1e-12
+    ref = L.Solver(H, ref_o)
+    timed_solves(L, ref, db, dx, zeros, 1)
+    x_ref = dx.download()
+    del ref
+    err = lambda x: float(np.linalg.norm(x - x_ref) / np.linalg.norm(x_ref))
 
     # per-kernel rates from the live timings, algorithmic bytes per launch
     traffic = pmc_traffic()
@@ -704,6 +738,21 @@ def main() -> int:
     del P64
     cache = cache_rates(L, H, n, flush)
 
+    # configs 3 and 5 in compact form: solve times, iterations, residuals
+    # and the roofline of each one's dominant kernel
+    del mixed, fp64, fp64n, mixed16
+    extra = {}
+    if not args.no_configs:
+        c3 = bench_bcsr(args, compact=True)
+        extra["config3"] = {k: c3[k] for k in ("workload", "n", "blocks", "solve_f32", "solve_bf16", "solve_f64",
+                                                "bilu_sweep_f32", "bilu_sweep_bf16", "bilu_sweep_f64",
+                                                "bcsr_spmv_f64", "roofline", "all_converged")}
+        c5 = bench_block(args)
+        extra["config5"] = {k: c5[k] for k in ("metric", "value", "unit", "config", "solves_per_s", "setup_s",
+                                                "forward_s", "adjoint_and_gradient_s", "forward", "adjoint",
+                                                "err_vs_fp64_solution", "roofline", "all_converged")}
+        ok = ok and extra["config3"]["all_converged"] and extra["config5"]["all_converged"]
+
     cpu = None
     if not args.no_cpu_baseline:
         threads = args.cpu_threads or min(16, host_cpus()["usable_logical_cpus"])
@@ -730,6 +779,19 @@ def main() -> int:
         },
         "fp64_time_s": round(t_fp64, 5),
         "speedup_mixed_vs_fp64": round(t_fp64 / t_mixed, 3),
+        # the same nested structure in FP64 (outer FGMRES(10) x inner FP64
+        # GMRES(10), FP64 factors): the share of the speed-up the precision
+        # accounts for
+        "fp64_nested_time_s": round(t_fp64n, 5),
+        "speedup_mixed_vs_fp64_nested": round(t_fp64n / t_mixed, 3),
+        "fp64_nested_outer_iterations": st_fn.iterations,
+        "fp64_nested_inner_iterations": st_fn.inner_iterations,
+        # the mixed path with BF16 factor storage (per-row scales)
+        "mixed_bf16_time_s": round(t_m16, 5),
+        "mixed_bf16_outer_iterations": st_m16.iterations,
+        "mixed_bf16_rel_true_residual": st_m16.rel_true_residual,
+        "err_vs_fp64_solution": {"mixed": err(x_mixed), "mixed_bf16": err(x_m16), "fp64": err(x_fp64),
+                                 "fp64_nested": err(x_fp64n), "reference": "FP64 FGMRES(50) to 1e-12"},
         "mixed_outer_iterations": st_m.iterations,
         "mixed_inner_iterations": st_m.inner_iterations,
         "fp64_iterations": st_f.iterations,
@@ -751,6 +813,7 @@ def main() -> int:
         "options": dict(kv.partition("=")[::2] for kv in args.set),
         "ms_per_step_without_events": round(t_plain * 1e3, 3),
         "cpu_baseline": cpu,
+        **extra,
     }
     print(json.dumps(out))
     return 0 if ok else 1

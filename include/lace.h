@@ -520,6 +520,11 @@ typedef struct lace_solver_opts {
   int    inner_iters;   /* >0: nested mixed mode -- each outer FP64 step
                            preconditions with inner_iters FP32 GMRES steps */
   int    transpose;     /* 1: solve A^T x = b with transposed factors */
+  int    inner_fp64;    /* with inner_iters > 0: 0 (default) inner GMRES in
+                           FP32 on an FP32 copy of A (factors at most FP32);
+                           1: inner GMRES in FP64 on A with FP64 factors
+                           (prec_precond F64), the same nested structure
+                           without the precision change */
 } lace_solver_opts;
 
 typedef struct lace_solver_stats {

@@ -80,7 +80,7 @@ class SolverOpts(C.Structure):
     _fields_ = [("krylov", C.c_int), ("search_max", C.c_int), ("restart_max", C.c_int),
                 ("rtol", C.c_double), ("prec_precond", C.c_int), ("trsv", C.c_int),
                 ("jacobi_sweeps", C.c_int), ("parilu_sweeps", C.c_int),
-                ("inner_iters", C.c_int), ("transpose", C.c_int)]
+                ("inner_iters", C.c_int), ("transpose", C.c_int), ("inner_fp64", C.c_int)]
 
 
 class SolverStats(C.Structure):
@@ -102,7 +102,7 @@ class BlockStats(C.Structure):
 
 
 assert C.sizeof(PrecondLog) == 128 and C.sizeof(GmresLog) == 64 and C.sizeof(GmresParam) == 64
-assert C.sizeof(SolverOpts) == 48 and C.sizeof(SolverStats) == 72
+assert C.sizeof(SolverOpts) == 56 and C.sizeof(SolverStats) == 72
 assert C.sizeof(BlockStats) == 24 + 8 + 2 * 8 * NRHS + 8
 
 _P = C.POINTER

@@ -1173,5 +1173,5 @@ static_assert(sizeof(data_d_gmres_log) == 64, "layout");
 static_assert(offsetof(data_d_gmres_param, parilu_reduction) == 40, "layout");
 static_assert(offsetof(data_d_gmres_param, reorder) == 48, "layout");
 static_assert(sizeof(data_d_gmres_param) == 64, "layout");
-static_assert(sizeof(lace_solver_opts) == 48, "layout");
+static_assert(sizeof(lace_solver_opts) == 56, "layout");
 static_assert(sizeof(lace_solver_stats) == 72, "layout");

@@ -121,29 +121,39 @@ struct PrecIdentity : Precond {
   }
 };
 
-// Inner FP32 Householder GMRES with a fixed FP32 ILU preconditioner,
-// `iters` steps from x = 0, no host synchronisation.  Output in FP32.
+// Inner Householder GMRES with a fixed ILU preconditioner, `iters` steps
+// from x = 0, no host synchronisation.  K = float: FP32 engine on the FP32
+// copy of A with FP32 (or 16-bit) factors, output in FP32; K = double: the
+// same in FP64 on A with FP64 factors (the nested structure without the
+// precision change).
+template <typename K>
 struct PrecInner : Precond {
-  const LinOp* A32;
+  const LinOp* Aop;
   Precond* M;
-  HHEngine<float> E;
+  HHEngine<K> E;
   int n, iters;
   double rtol;
   PrecInner(const LinOp* a, Precond* m, int n_, int it, double rt)
-      : A32(a), M(m), n(n_), iters(it), rtol(rt) {}
-  int init() { return E.init(n, iters, true); }
-  bool z_is_float() const override { return true; }
+      : Aop(a), M(m), n(n_), iters(it), rtol(rt) {}
+  int init() { return E.init(n, iters, sizeof(K) == 4); }
+  bool z_is_float() const override { return sizeof(K) == 4; }
   int apply_d(const double* q, void* z) override {
     TimedRegion tr("inner_gmres");
-    LACE_TRY(cast_d2f(n, q, E.r.as<float>()));
-    LACE_HIP(hipMemsetAsync(z, 0, sizeof(float) * n, stream()));
-    LACE_TRY(hh_start<float>(E, 1, rtol));
+    if constexpr (sizeof(K) == 4) LACE_TRY(cast_d2f(n, q, E.r.template as<float>()));
+    else LACE_HIP(hipMemcpyAsync(E.r.p, q, sizeof(double) * n, hipMemcpyDeviceToDevice, stream()));
+    LACE_HIP(hipMemsetAsync(z, 0, sizeof(K) * n, stream()));
+    LACE_TRY(hh_start<K>(E, 1, rtol));
     for (int j = 0; j < iters; j++) {
-      LACE_TRY(M->apply_f(E.q.as<float>(), E.zcol(j)));
-      LACE_TRY(A32->apply_ff(static_cast<const float*>(E.zcol(j)), E.w.as<float>()));
-      LACE_TRY(hh_step<float>(E, j, j < iters - 1));
+      if constexpr (sizeof(K) == 4) {
+        LACE_TRY(M->apply_f(E.q.template as<float>(), E.zcol(j)));
+        LACE_TRY(Aop->apply_ff(static_cast<const float*>(E.zcol(j)), E.w.template as<float>()));
+      } else {
+        LACE_TRY(M->apply_d(E.q.template as<double>(), E.zcol(j)));
+        LACE_TRY(Aop->apply_dd(static_cast<const double*>(E.zcol(j)), E.w.template as<double>()));
+      }
+      LACE_TRY(hh_step<K>(E, j, j < iters - 1));
     }
-    return hh_finish<float, float>(E, static_cast<float*>(z));
+    return hh_finish<K, K>(E, static_cast<K*>(z));
   }
 };
 
@@ -279,7 +289,8 @@ struct Solver {
   ParILU* P = nullptr;
   std::unique_ptr<PrecILU> ilu;
   std::unique_ptr<PrecIdentity> ident;
-  std::unique_ptr<PrecInner> inner;
+  std::unique_ptr<PrecInner<float>> inner;
+  std::unique_ptr<PrecInner<double>> inner64;  // inner_fp64
   Precond* M = nullptr;
   HHEngine<double> E;
   std::unique_ptr<BlockEngine> beng;  // 32-right-hand-side solver, on first use
@@ -342,7 +353,8 @@ static int select_direction(Solver* S, int trans) {
   if (trans) LACE_TRY(ensure_transpose(S));
   S->op64 = trans ? S->opAT64.get() : S->opA64.get();
   if (S->ilu) S->ilu->trans = trans ? 1 : 0;
-  if (S->inner) S->inner->A32 = trans ? S->opAT32.get() : S->opA32.get();
+  if (S->inner) S->inner->Aop = trans ? S->opAT32.get() : S->opA32.get();
+  if (S->inner64) S->inner64->Aop = trans ? S->opAT64.get() : S->opA64.get();
   return DEV_SUCCESS;
 }
 
@@ -353,6 +365,10 @@ static int solver_create(const data_d_matrix* A, const lace_solver_opts* o, Solv
   auto S = std::make_unique<Solver>();
   S->o = *o;
   S->block = A->storage_type == Magma_BCSR;
+  // nested FP64: the inner GMRES on A itself with FP64 factors
+  const bool in64 = o->inner_iters > 0 && o->inner_fp64 != 0;
+  LACE_REQUIRE(!in64 || (!S->block && o->prec_precond == LACE_F64), DEV_ERR_NOT_SUPPORTED,
+               "solver: an FP64 inner GMRES needs a CSR operator and FP64 factors");
   if (S->block) {
     // block operator: FP64 BCSR for the outer products, block ILU factors
     LACE_TRY(S->B64.create(A, LACE_F64));
@@ -371,7 +387,7 @@ static int solver_create(const data_d_matrix* A, const lace_solver_opts* o, Solv
     S->h_val.assign(A->val, A->val + S->nnz);
     LACE_TRY(S->A64.create(A, LACE_F64));
     S->opA64 = std::make_unique<CsrOp>(&S->A64);
-    if (o->inner_iters > 0) {
+    if (o->inner_iters > 0 && !in64) {
       LACE_TRY(S->A32.create(A, LACE_F32));
       S->opA32 = std::make_unique<CsrOp>(&S->A32);
     }
@@ -380,9 +396,9 @@ static int solver_create(const data_d_matrix* A, const lace_solver_opts* o, Solv
     S->ident = std::make_unique<PrecIdentity>(S->n);
     S->M = S->ident.get();
   } else {
-    // the inner solver iterates in FP32, so its factors are at most FP32
+    // the FP32 inner solver iterates in FP32, so its factors are at most FP32
     lace_prec_t fp = o->prec_precond;
-    if (o->inner_iters > 0 && fp == LACE_F64) fp = LACE_F32;
+    if (o->inner_iters > 0 && !in64 && fp == LACE_F64) fp = LACE_F32;
     Precond* base = nullptr;
     if (S->block) {
       S->BP = new BILU();
@@ -397,8 +413,12 @@ static int solver_create(const data_d_matrix* A, const lace_solver_opts* o, Solv
       base = S->ilu.get();
     }
     S->M = base;
-    if (o->inner_iters > 0) {
-      S->inner = std::make_unique<PrecInner>(S->opA32.get(), base, S->n, o->inner_iters, 0.0);
+    if (in64) {
+      S->inner64 = std::make_unique<PrecInner<double>>(S->opA64.get(), base, S->n, o->inner_iters, 0.0);
+      LACE_TRY(S->inner64->init());
+      S->M = S->inner64.get();
+    } else if (o->inner_iters > 0) {
+      S->inner = std::make_unique<PrecInner<float>>(S->opA32.get(), base, S->n, o->inner_iters, 0.0);
       LACE_TRY(S->inner->init());
       S->M = S->inner.get();
     }
@@ -532,7 +552,7 @@ static int solver_solve(Solver* S, const double* b, double* x, lace_solver_stats
     if (E.host_misc[MS_BREAK] != 0.0) break;
     converged = E.host_misc[MS_RES] < rtol_abs;
   }
-  if (S->inner) local.inner_iterations = local.iterations * S->o.inner_iters;
+  if (S->inner || S->inner64) local.inner_iterations = local.iterations * S->o.inner_iters;
   // true residual in FP64 (against the operator in use)
   LACE_TRY(S->op64->residual_dd(x, b, S->rbuf.as<double>()));
   LACE_TRY(norm2(S, S->rbuf.as<double>(), &local.true_residual));
@@ -900,6 +920,7 @@ int lace_solver_solve_block(lace_solver* Sh, const double* B, double* X, int tra
   auto* S = reinterpret_cast<Solver*>(Sh);
   const int m = S->o.search_max;
   LACE_REQUIRE(!S->block && S->P, DEV_ERR_NOT_SUPPORTED, "block solve: CSR operator with ILU factors");
+  LACE_REQUIRE(!S->inner64, DEV_ERR_NOT_SUPPORTED, "block solve: the inner GMRES runs in FP32 only");
   LACE_REQUIRE(S->o.inner_iters >= 0 && S->o.inner_iters <= kMaxInner, DEV_ERR_NOT_SUPPORTED,
                "block solve: inner iterations %d outside 0..%d", S->o.inner_iters, kMaxInner);
   LACE_REQUIRE(S->o.trsv == LACE_TRSV_JACOBI || S->o.trsv == LACE_TRSV_NONE, DEV_ERR_NOT_SUPPORTED,

@@ -10,9 +10,10 @@ seed 0, x0 = 0.
      the oracle's factors of the same sweeps.  The bar is the one the
      reference's users get: convergence to the same 1e-8 within +-2 search
      directions.
-(ii) The bench's two configurations (mixed: FP64 FGMRES(10) x FP32
-     GMRES(10), FP32 ILU, Jacobi-5; FP64: FGMRES(50), FP64 ILU, Jacobi-5)
-     reach an FP64 true relative residual <= 1e-8 recomputed on the host.
+(ii) The bench's configurations (mixed: FP64 FGMRES(10) x FP32 GMRES(10),
+     FP32 ILU, Jacobi-5; the same with BF16 factor storage; the same nesting
+     in FP64; FP64: FGMRES(50), FP64 ILU, Jacobi-5) reach an FP64 true
+     relative residual <= 1e-8 recomputed on the host.
 """
 import numpy as np
 import pytest
@@ -80,12 +81,15 @@ def test_config2_fp64_exact_solves_vs_oracle(gpu, orc, c2):
         assert dx <= 1e-5, dx
 
 
-@pytest.mark.parametrize("path", ["mixed", "fp64"])
+@pytest.mark.parametrize("path", ["mixed", "mixed_bf16", "fp64_nested", "fp64"])
 def test_config2_bench_paths_true_residual(gpu, c2, path):
     A, b = c2
-    if path == "mixed":
+    if path in ("mixed", "mixed_bf16"):
         o = _opts(gpu, search_max=10, inner_iters=10, trsv=gpu.TRSV_JACOBI, jacobi_sweeps=5,
-                  prec_precond=gpu.LACE_F32)
+                  prec_precond=gpu.LACE_F32 if path == "mixed" else gpu.LACE_BF16)
+    elif path == "fp64_nested":
+        o = _opts(gpu, search_max=10, inner_iters=10, inner_fp64=1, trsv=gpu.TRSV_JACOBI, jacobi_sweeps=5,
+                  prec_precond=gpu.LACE_F64)
     else:
         o = _opts(gpu, search_max=50, trsv=gpu.TRSV_JACOBI, jacobi_sweeps=5,
                   prec_precond=gpu.LACE_F64)

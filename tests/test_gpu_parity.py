@@ -436,3 +436,28 @@ def test_mkl_fgmres_dropin(gpu, orc, tol_type):
     assert info == 0
     ref = orc.fgmres_mgs(A, b, m=30, restart_max=7, rtol=par.rtol, tol_type=tol_type, choice=2)
     assert np.linalg.norm(b - A.scipy() @ X2.v) <= max(bound, 1.01 * np.linalg.norm(b - A.scipy() @ ref.x))
+
+
+def test_fp64_nested_inner_solver(gpu, orc):
+    """FP64 FGMRES(10) preconditioned by 8 steps of FP64 GMRES with FP64
+    ILU factors: converges to 1e-10 on the host-recomputed residual, in no
+    more outer steps than the FP32 inner solver; the block solver refuses
+    an FP64 inner solver."""
+    A = orc.convdiff3d(20, 20, 20)
+    b = gpu.gen_normal(3, A.n_rows)
+    res = {}
+    for in64 in (0, 1):
+        o = gpu.default_opts()
+        o.search_max, o.restart_max, o.rtol, o.parilu_sweeps = 10, 100, 1e-10, 3
+        o.trsv, o.jacobi_sweeps, o.inner_iters, o.inner_fp64 = gpu.TRSV_JACOBI, 5, 8, in64
+        o.prec_precond = gpu.LACE_F64 if in64 else gpu.LACE_F32
+        S = gpu.Solver(gpu.HostCSR.from_csr(A), o)
+        S.setup()
+        x, st = S.solve(b)
+        assert st.converged == 1
+        assert np.linalg.norm(b - A.scipy() @ x) <= 1.01e-10 * np.linalg.norm(b)
+        res[in64] = (st.iterations, st.inner_iterations)
+        if in64:
+            with pytest.raises(gpu.LaceError):
+                S.solve_block(np.zeros((A.n_rows, 32)))
+    assert res[1][0] <= res[0][0] and res[1][1] == 8 * res[1][0]
