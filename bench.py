@@ -405,11 +405,12 @@ def bench_bcsr(args, compact=False):
     out = {"mode": "bcsr", "workload": f"config 3: BCSR 5x5, 7-point block grid {N}^3",
            "n": n, "blocks": int(row[-1])}
     # sweeps and SpMV bandwidth
-    variants = ((0, ""), (2, "_merge"), (3, "_tables"), (4, "_tables_lds"), (5, "_tables_lds_lu"))
+    variants = ((0, ""), (2, "_merge"), (3, "_tables"), (4, "_tables_lds"), (5, "_tables_lds_lu"),
+                (6, "_wave"))
     for name, prec in (("f32", L.LACE_F32), ("bf16", L.LACE_BF16), ("f64", L.LACE_F64)):
         # forms of the paired sweep: 0 the default, 2 the device-side merge of
         # the two block rows, 3 set-up tables, 4 tables + LDS staging, 5
-        # tables + LDS staging of L, U and U_JJ only
+        # tables + LDS staging of L, U and U_JJ only, 6 whole-word wave runs
         for variant, tag in (variants[:1] if compact else variants):
             P = L.BILU(H, prec)
             if variant:

@@ -485,12 +485,14 @@ int lace_bilu_levels(const lace_bilu *B, int *nlev_L, int *nlev_U);
 /* Sweep kernel: 0 (default) pairs L_IJ with U_JI in one lane group when A's
  * block pattern is structurally symmetric, else one group per block of A;
  * 1 always takes the per-block kernel.  The paired kernel reads
- * per-position tables built at set-up and stages each workgroup's L and U
- * blocks and U_JJ rows through LDS (where they align to 16 bytes and fit
- * 64 KiB); otherwise it merges the two block rows on the device.  2-5 force
- * a form of the paired kernel: 2 the merge, 3 the tables without staging,
- * 4 the tables with A's blocks staged too, 5 the tables with L, U and U_JJ
- * staged.  All give identical factors. */
+ * per-position tables built at set-up; each wave moves its positions' A, L
+ * and U blocks as runs of whole words through its own slice of LDS (where
+ * the slices fit 64 KiB), else the workgroup stages L, U and U_JJ through
+ * LDS (where they align to 16 bytes and fit), else it merges the two block
+ * rows on the device.  2-6 force a form of the paired kernel: 2 the merge,
+ * 3 the tables without staging, 4 the tables with A's blocks staged too,
+ * 5 the tables with L, U and U_JJ staged, 6 the wave runs.  All give
+ * identical factors. */
 int lace_bilu_set_variant(lace_bilu *B, int variant);
 /* 1 if A's block pattern is structurally symmetric. */
 int lace_bilu_is_symmetric(const lace_bilu *B);

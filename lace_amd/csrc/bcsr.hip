@@ -552,6 +552,364 @@ constexpr bool sym_lds_lu_ok() {
   return (e * sizeof(TS)) % 16 == 0 && e * (sizeof(TC) + 2 * sizeof(TS)) <= 65536;
 }
 
+// ---------------------------------------------------------------- wave-run sweep
+// A wave's PPW consecutive positions occupy one contiguous run of CE = PPW
+// BS^2 elements in each block array (the arrays are padded to whole
+// workgroups of runs, so every run is complete).  WaveRun moves such a run
+// between global memory and LDS as the widest words that tile it (16 bytes
+// for FP32 and FP64 runs of 5x5 blocks, 8 for 16-bit ones): consecutive
+// lanes take consecutive words, so each load or store instruction of the
+// wave covers one contiguous stretch of memory.
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+template <typename T, int CE>
+struct WaveRun {
+  static constexpr int kBytes = CE * (int)sizeof(T);
+  using W = std::conditional_t<
+      kBytes % 16 == 0, u32x4,
+      std::conditional_t<kBytes % 8 == 0, u32x2,
+                         std::conditional_t<kBytes % 4 == 0, unsigned, unsigned short>>>;
+  static constexpr int kWords = kBytes / (int)sizeof(W);
+  static constexpr int kR = (kWords + 63) / 64;  // words per lane
+  W w[kR];
+  __device__ __forceinline__ void load(const T* __restrict__ src, int lane) {
+#pragma unroll
+    for (int k = 0; k < kR; k++)
+      if (lane + 64 * k < kWords) w[k] = reinterpret_cast<const W*>(src)[lane + 64 * k];
+  }
+  __device__ __forceinline__ void to_lds(T* dst, int lane) const {
+#pragma unroll
+    for (int k = 0; k < kR; k++)
+      if (lane + 64 * k < kWords) reinterpret_cast<W*>(dst)[lane + 64 * k] = w[k];
+  }
+  static __device__ __forceinline__ void from_lds(const T* src, T* __restrict__ dst, int lane) {
+#pragma unroll
+    for (int k = 0; k < kR; k++)
+      if (lane + 64 * k < kWords)
+        reinterpret_cast<W*>(dst)[lane + 64 * k] = reinterpret_cast<const W*>(src)[lane + 64 * k];
+  }
+};
+
+// Positions per workgroup of the wave-run form, and a position count
+// rounded up to whole workgroups (the length the block arrays and tables
+// are padded to).
+inline int wave_ppb(int bs) { return (kBBS / 64) * (64 / bs); }
+inline int wave_pad(int bs, int np) { return (np + wave_ppb(bs) - 1) / wave_ppb(bs) * wave_ppb(bs); }
+
+// Positions staged before a wave's run (the halo): a diagonal position's
+// products come from the earlier positions of its block row, three of them
+// on a 7-point block pattern, and the row may begin in the previous run.
+constexpr int kHalo = 3;
+
+// Where the products of the wave-run sweep find their operands: the old L
+// and U blocks of positions [pw - kHalo, pw + PPW) staged in LDS (sL, sU
+// indexed from the run's first position, the halo at negative offsets, the
+// halo empty in the first run) with the scales of U's rows (sS, 16-bit
+// storage), else memory.
+template <int BS, int PPW, typename TS>
+struct StagedFactors {
+  const TS* sL;
+  const TS* sU;
+  const float* sS;
+  const TS* __restrict__ L;
+  const TS* __restrict__ U;
+  const float* __restrict__ rs;
+  int pw;
+  __device__ __forceinline__ bool staged(int q) const {
+    return (unsigned)(q - pw + kHalo) < (unsigned)(PPW + kHalo);
+  }
+};
+
+// Row e0 / BS of factor block q (row scale s), staged or from memory.
+template <int BS, int PPW, typename TC, typename TS>
+__device__ __forceinline__ void staged_row(bool in_lds, const TS* sh, const TS* __restrict__ g, int pw, int q,
+                                           int e0, float s, TC (&out)[BS]) {
+  if (in_lds) {
+#pragma unroll
+    for (int c = 0; c < BS; c++) out[c] = dequant<TC, TS>(sh[(q - pw) * BS * BS + e0 + c], s);
+  } else {
+#pragma unroll
+    for (int c = 0; c < BS; c++) out[c] = ldv<TC>(g, (size_t)q * BS * BS + e0 + c, s);
+  }
+}
+
+// s -= L(qr, row r) U(qb) (U_qb's rows are block row kb's): each entry of
+// the product summed from zero over the inner index in ascending order (as
+// the products of load_row and load_block are), U's rows fetched one at a
+// time.
+template <int BS, int PPW, typename TC, typename TS, bool LOCAL>
+__device__ __forceinline__ void staged_row_times_block_sub(const StagedFactors<BS, PPW, TS>& F, int qr, int r,
+                                                           float sr, int qb, int kb, TC (&s)[BS]) {
+  TC lr[BS], acc[BS];
+  staged_row<BS, PPW, TC, TS>(LOCAL || F.staged(qr), F.sL, F.L, F.pw, qr, r * BS, sr, lr);
+  const bool ub = LOCAL || F.staged(qb);
+#pragma unroll
+  for (int c = 0; c < BS; c++) acc[c] = TC(0);
+#pragma unroll
+  for (int mm = 0; mm < BS; mm++) {
+    TC ur[BS];
+    const float sm = !is16<TS>() ? 1.0f : (ub ? F.sS[(qb - F.pw) * BS + mm] : F.rs[kb * BS + mm]);
+    staged_row<BS, PPW, TC, TS>(ub, F.sU, F.U, F.pw, qb, mm * BS, sm, ur);
+#pragma unroll
+    for (int c = 0; c < BS; c++) acc[c] = acc[c] + lr[mm] * ur[c];
+  }
+#pragma unroll
+  for (int c = 0; c < BS; c++) s[c] = s[c] - acc[c];
+}
+
+// Orders this wave's LDS accesses: the LDS executes one wave's operations
+// in issue order, so a wave-private buffer needs no workgroup barrier, only
+// the LDS counter drained and the compiler kept from moving accesses across.
+__device__ __forceinline__ void wave_lds_sync() { asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory"); }
+
+// Column col of inv(M) by the elimination of binv: the operations on that
+// column's entries are binv's, in binv's order (binv updates each column
+// of the inverse from its own entries and the multipliers alone), so the
+// column is bit-identical to binv's.  The multipliers are the same in
+// every lane; work on columns of M that no later step reads is dead code.
+template <int BS, typename T>
+__device__ __forceinline__ void binv_col(const T (&M)[BS * BS], int col, T (&x)[BS]) {
+  T a[BS * BS];
+#pragma unroll
+  for (int e = 0; e < BS * BS; e++) a[e] = M[e];
+#pragma unroll
+  for (int r = 0; r < BS; r++) x[r] = r == col ? T(1) : T(0);
+#pragma unroll
+  for (int p = 0; p < BS; p++) {
+    const T iv = T(1) / a[p * BS + p];
+#pragma unroll
+    for (int c = 0; c < BS; c++) a[p * BS + c] = a[p * BS + c] * iv;
+    x[p] = x[p] * iv;
+#pragma unroll
+    for (int r = 0; r < BS; r++) {
+      if (r == p) continue;
+      const T f = a[r * BS + p];
+#pragma unroll
+      for (int c = 0; c < BS; c++) a[r * BS + c] = a[r * BS + c] - f * a[p * BS + c];
+      x[r] = x[r] - f * x[p];
+    }
+  }
+}
+
+// The table-driven symmetric sweep with every block array moved as
+// whole-word wave runs.  Each wave owns PPW consecutive positions and a
+// private slice of four LDS buffers: A (a_ij, then a_ji), the old L and U
+// runs (which the new rows overwrite, so diagonal and padding positions
+// write their blocks back unchanged), and U_JJ (each lane stages its row),
+// which then carries inv(U_JJ): lane r of a group computes column r of the
+// inverse and reads the whole inverse back.  Every lane runs the same
+// straight-line code (the 64 - BS PPW lanes past the last group shadow it);
+// only the writes depend on whether a lane's position is real, so no load
+// waits on a branch.  No workgroup barrier on the way; the arithmetic is
+// that of the kernels above, operation by operation.
+template <int BS, typename TC, typename TS, bool STEP, bool LOCAL>
+__global__ __launch_bounds__(kBBS) void bilu_sym_sweep_wave_kernel(
+    int np, const int4* __restrict__ meta, const int* __restrict__ pend, const int4* __restrict__ pairs,
+    const TC* __restrict__ AL, const TC* __restrict__ AU, const TS* __restrict__ Lin,
+    const TS* __restrict__ Uin, TS* __restrict__ Lout, TS* __restrict__ Uout,
+    const float* __restrict__ rs, double* __restrict__ partial) {
+  constexpr int LB = BS * BS, PPW = 64 / BS, CE = PPW * LB, NWV = kBBS / 64;
+  constexpr int HE = kHalo * LB;  // halo elements
+  constexpr int HS = (HE * (int)sizeof(TS) + 15) / 16 * 16 / (int)sizeof(TS);  // halo slots (16-byte multiple)
+  constexpr int SS = is16<TS>() ? (kHalo + PPW) * BS : 1;
+  __shared__ __align__(16) TC shA[NWV * CE];
+  __shared__ __align__(16) TC shJ[NWV * CE];
+  __shared__ __align__(16) TS shL[NWV * (HS + CE)];
+  __shared__ __align__(16) TS shU[NWV * (HS + CE)];
+  __shared__ float shS[NWV * SS];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int pw = (xcd_block(blockIdx.x, gridDim.x) * NWV + wv) * PPW;  // the wave's first position
+  const size_t off = (size_t)pw * LB;
+  TC* sA = shA + wv * CE;
+  TS* sL = shL + wv * (HS + CE) + HS;  // the run; the halo at [-HE, 0)
+  TS* sU = shU + wv * (HS + CE) + HS;
+  float* sS = shS + wv * SS + kHalo * BS;  // U row scales, indexed (q - pw) * BS + row
+  const int slot = lane / BS, r = lane - slot * BS;
+  const bool lane_on = slot < PPW;
+  const int lp = min(slot, PPW - 1);
+  const int4 m = meta[pw + lp];
+  const int pe = pend[pw + lp];
+  WaveRun<TC, CE> ra, rb;
+  WaveRun<TS, CE> rl, ru;
+  ra.load(AL + off, lane);
+  rl.load(Lin + off, lane);
+  ru.load(Uin + off, lane);
+  const int I = m.x, J = m.y, qd = m.z;
+  const bool active = lane_on && pw + lp < np;
+  const bool strict = active && I != J;
+  const float sI = is16<TS>() ? rs[I * BS + r] : 1.0f;
+  const float sJs = is16<TS>() ? rs[J * BS + r] : 1.0f;
+  TS ujr[BS];  // this lane's row of U_JJ
+#pragma unroll
+  for (int c = 0; c < BS; c++) ujr[c] = Uin[(size_t)qd * LB + r * BS + c];
+  // the first three pair-table entries (a diagonal position of a 7-point
+  // block pattern has three), fetched alongside U_JJ so the product loop
+  // below does not wait on a dependent load per step (indices clamped into
+  // the table; entries past pend are never used)
+  const int4 pf0 = pairs[max(0, min(m.w, pe - 1))];
+  const int4 pf1 = pairs[max(0, min(m.w + 1, pe - 1))];
+  const int4 pf2 = pairs[max(0, min(m.w + 2, pe - 1))];
+  rb.load(AU + off, lane);
+  // the halo's old L and U blocks, element by element (the halo does not
+  // start on a word boundary); the first run has none
+  // (loaded unconditionally from clamped indices, so no branch splits the
+  // loads; what is not halo is never written to LDS)
+  constexpr int HR = (HE + 63) / 64;
+  const size_t hoff = pw > 0 ? off - HE : off;
+  TS hl[HR], hu[HR];
+#pragma unroll
+  for (int k = 0; k < HR; k++) {
+    const int e = min(lane + 64 * k, HE - 1);
+    hl[k] = Lin[hoff + e];
+    hu[k] = Uin[hoff + e];
+  }
+  // the scales of U's rows at the halo positions (lanes h * BS + row)
+  float hs = 1.0f;
+  if constexpr (is16<TS>()) {
+    const int h = min(lane, kHalo * BS - 1);
+    hs = rs[meta[max(pw - kHalo, 0) + h / BS].y * BS + h % BS];
+  }
+  const int row0 = lp * LB + r * BS;
+  TC* jb = shJ + wv * CE + lp * LB;  // this group's U_JJ, later inv(U_JJ)
+  ra.to_lds(sA, lane);
+  rl.to_lds(sL, lane);
+  ru.to_lds(sU, lane);
+#pragma unroll
+  for (int k = 0; k < HR; k++) {
+    const int e = lane + 64 * k;
+    if (pw > 0 && e < HE) {
+      sL[e - HE] = hl[k];
+      sU[e - HE] = hu[k];
+    }
+  }
+  if (is16<TS>()) {
+    if (lane_on) sS[lp * BS + r] = sJs;
+    if (lane < kHalo * BS) sS[lane - kHalo * BS] = hs;
+  }
+  if (lane_on) {
+#pragma unroll
+    for (int c = 0; c < BS; c++) jb[r * BS + c] = dequant<TC, TS>(ujr[c], sJs);
+  }
+  wave_lds_sync();
+  TC aL[BS], aU[BS], lold[BS], uold[BS];
+#pragma unroll
+  for (int c = 0; c < BS; c++) {
+    aL[c] = sA[row0 + c];
+    lold[c] = dequant<TC, TS>(sL[row0 + c], sI);
+    uold[c] = dequant<TC, TS>(sU[row0 + c], sJs);
+  }
+  wave_lds_sync();
+  rb.to_lds(sA, lane);
+  wave_lds_sync();
+#pragma unroll
+  for (int c = 0; c < BS; c++) aU[c] = sA[row0 + c];
+  // products over K < J (on a 7-point block pattern only the diagonal
+  // positions have them, three each, all from the position's own block row,
+  // which the run and its halo hold)
+  const StagedFactors<BS, PPW, TS> F{sL, sU, sS, Lin, Uin, rs, pw};
+  if constexpr (LOCAL) {
+    // every position has at most three products and all their operands are
+    // staged (checked at set-up)
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+      if (j < pe - m.w) {
+        const int4 t = j == 0 ? pf0 : (j == 1 ? pf1 : pf2);  // (q_IK, q_JK, K)
+        // L_IJ row r -= L_IK(r,:) U_KJ (U_KJ at position (J, K)); U_JI row
+        // r -= L_JK(r,:) U_KI (U_KI at position (I, K))
+        if (I != J) staged_row_times_block_sub<BS, PPW, TC, TS, true>(F, t.x, r, sI, t.y, t.z, aL);
+        staged_row_times_block_sub<BS, PPW, TC, TS, true>(F, t.y, r, sJs, t.x, t.z, aU);
+      }
+    }
+  } else {
+    for (int k = m.w; k < pe; k++) {
+      const int j = k - m.w;
+      int4 t;  // (q_IK, q_JK, K)
+      if (j == 0) t = pf0;
+      else if (j == 1) t = pf1;
+      else if (j == 2) t = pf2;
+      else t = pairs[k];
+      if (I != J) staged_row_times_block_sub<BS, PPW, TC, TS, false>(F, t.x, r, sI, t.y, t.z, aL);
+      staged_row_times_block_sub<BS, PPW, TC, TS, false>(F, t.y, r, sJs, t.x, t.z, aU);
+    }
+  }
+  // U's row first (its registers are free for what follows); its squared
+  // changes are added after L's, in the order of the other kernels
+  double du2[STEP ? BS : 1];
+  {
+    const float sinv = 1.0f / sJs;
+#pragma unroll
+    for (int c = 0; c < BS; c++) {
+      // L_JJ is the identity: the closing product is U_JI's own row
+      TC v = aU[c] - uold[c];
+      v = v + uold[c];
+      const TS q = quant<TC, TS>(v, sinv);
+      if (active) sU[row0 + c] = q;
+      if constexpr (STEP) {
+        const double d = (double)uold[c] - (double)dequant<TC, TS>(q, sJs);
+        du2[c] = active ? d * d : 0.0;
+      }
+    }
+  }
+  // the closing product K = J, subtracted and added back
+  TC ujj[LB];
+#pragma unroll
+  for (int e = 0; e < LB; e++) ujj[e] = jb[e];
+#pragma unroll
+  for (int c = 0; c < BS; c++) {
+    TC acc = TC(0);
+#pragma unroll
+    for (int mm = 0; mm < BS; mm++) acc = acc + lold[mm] * ujj[mm * BS + c];
+    aL[c] = aL[c] - acc;
+    aL[c] = aL[c] + acc;
+  }
+  TC icol[BS];
+  binv_col<BS, TC>(ujj, r, icol);
+  if (lane_on) {
+#pragma unroll
+    for (int mm = 0; mm < BS; mm++) jb[mm * BS + r] = icol[mm];
+  }
+  wave_lds_sync();
+  TC inv[LB];
+#pragma unroll
+  for (int e = 0; e < LB; e++) inv[e] = jb[e];
+  double d2 = 0.0;
+  {
+    const float sinv = 1.0f / sI;
+#pragma unroll
+    for (int c = 0; c < BS; c++) {
+      TC acc = TC(0);
+#pragma unroll
+      for (int mm = 0; mm < BS; mm++) acc = acc + aL[mm] * inv[mm * BS + c];
+      const TS q = quant<TC, TS>(acc, sinv);
+      if (strict) sL[row0 + c] = q;
+      if (STEP && strict) {
+        const double d = (double)lold[c] - (double)dequant<TC, TS>(q, sI);
+        d2 += d * d;
+      }
+    }
+  }
+  if constexpr (STEP) {
+#pragma unroll
+    for (int c = 0; c < BS; c++) d2 += du2[c];
+  }
+  wave_lds_sync();
+  WaveRun<TS, CE>::from_lds(sL, Lout + off, lane);
+  WaveRun<TS, CE>::from_lds(sU, Uout + off, lane);
+  if (STEP) {
+    __shared__ double sm[kBBS / 64];
+    const double tot = block_sum<kBBS>(d2, sm);
+    if (threadIdx.x == 0) partial[xcd_block(blockIdx.x, gridDim.x)] = tot;
+  }
+}
+
+// The wave-run form applies when its four buffers fit 64 KiB of LDS.
+template <int BS, typename TC, typename TS>
+constexpr bool sym_wave_ok() {
+  constexpr int e = (kBBS / 64) * (64 / BS) * BS * BS;
+  return e * (2 * sizeof(TC) + 2 * sizeof(TS)) + (kBBS / 64) * (2 * 16 * BS * BS * kHalo + 4 * (kHalo + 64) * BS) +
+             64 <= 65536;
+}
+
 // General sweep: group (k, r) takes block k = (I, J) of A in BCSR order,
 // merges L row I (BCSRL) with U column J (BCSCU) and writes row r of L_IJ
 // (I > J) or U_IJ (I <= J), as src/parilu_v0_3_bcsr.cpp:130-247 does.  The
@@ -857,8 +1215,12 @@ int BILU::create(const data_d_matrix* A, lace_prec_t storage) {
   // kernel (then L / U initial values come from AL / AU), A itself for the
   // general kernel
   std::vector<int> lfrom_h(nnzL), ufrom_h(nnzU);
+  // the symmetric kernel's block arrays and tables are padded to whole
+  // workgroups of wave runs (padding: zero blocks, positions (0, 0) with
+  // no products, never written back as factors)
+  const int npad = sym ? wave_pad(bs, nnzL) : nnzL;
   if (sym) {
-    std::vector<double> al((size_t)nnzL * lb, 0.0), au((size_t)nnzL * lb, 0.0);
+    std::vector<double> al((size_t)npad * lb, 0.0), au((size_t)npad * lb, 0.0);
     for (int q = 0; q < nnzL; q++) {
       if (lf[q] >= 0)
         memcpy(&al[(size_t)q * lb], A->val + (size_t)lf[q] * lb, sizeof(double) * lb);
@@ -876,7 +1238,7 @@ int BILU::create(const data_d_matrix* A, lace_prec_t storage) {
     // per-position tables of the symmetric kernel: (I, J, q_JJ, first
     // pair), end of pairs, and the pairs (q_IK, q_JK, K) for K < J in
     // increasing K, where rows I and J both hold block column K
-    std::vector<int> mt((size_t)nnzL * 4), pe(nnzL), pr;
+    std::vector<int> mt((size_t)npad * 4, 0), pe(npad, 0), pr;
     for (int q = 0; q < nnzL; q++) {
       const int I = h_lrow[q], J = h_lcol[q];
       mt[(size_t)q * 4] = I;
@@ -894,6 +1256,22 @@ int BILU::create(const data_d_matrix* A, lace_prec_t storage) {
       LACE_REQUIRE(pr.size() / 4 <= (size_t)INT_MAX, DEV_ERR_ILLEGAL_VALUE,
                    "block ParILU: too many block products");
       pe[q] = (int)(pr.size() / 4);
+    }
+    // whether the wave-run sweep finds every product's operands in the
+    // staged run of its position or the halo before it, with at most three
+    // products per position
+    wave_local = true;
+    {
+      const int ppw = 64 / bs;
+      for (int q = 0; q < nnzL && wave_local; q++) {
+        const int w0 = q / ppw * ppw;
+        if (pe[q] - mt[(size_t)q * 4 + 3] > 3) wave_local = false;
+        for (int k = mt[(size_t)q * 4 + 3]; k < pe[q]; k++)
+          for (int e = 0; e < 2; e++) {
+            const int x = pr[(size_t)k * 4 + e];
+            if (x < w0 - kHalo || x >= w0 + ppw) wave_local = false;
+          }
+      }
     }
     if (pr.empty()) pr.assign(4, 0);
     LACE_TRY(upload_ints(smeta, mt));
@@ -923,8 +1301,10 @@ int BILU::create(const data_d_matrix* A, lace_prec_t storage) {
   LACE_TRY(upload_ints(uidx, h_uidx));
   LACE_TRY(upload_ints(ucolk, h_ucolk));
   for (int s = 0; s < 2; s++) {
-    LACE_TRY(Lv[s].alloc((size_t)nnzL * lb * selem()));
-    LACE_TRY(Uv[s].alloc((size_t)nnzU * lb * selem()));
+    LACE_TRY(Lv[s].alloc((size_t)std::max(nnzL, npad) * lb * selem()));
+    LACE_TRY(Uv[s].alloc((size_t)std::max(nnzU, npad) * lb * selem()));
+    LACE_HIP(hipMemsetAsync(Lv[s].p, 0, Lv[s].bytes, stream()));
+    LACE_HIP(hipMemsetAsync(Uv[s].p, 0, Uv[s].bytes, stream()));
   }
   LACE_TRY(Dinv.alloc((size_t)nbr * lb * celem()));
   LACE_TRY(work.alloc((size_t)2 * n() * celem()));
@@ -975,7 +1355,7 @@ int BILU::upload_general() {
 }
 
 int BILU::set_variant(int v) {
-  LACE_REQUIRE(v >= 0 && v <= 5, DEV_ERR_ILLEGAL_VALUE, "block ParILU: variant %d", v);
+  LACE_REQUIRE(v >= 0 && v <= 6, DEV_ERR_ILLEGAL_VALUE, "block ParILU: variant %d", v);
   variant = v;
   if (v == 1) LACE_TRY(upload_general());
   return DEV_SUCCESS;
@@ -1020,17 +1400,36 @@ static int sweep_bs(BILU* B, bool with_step) {
                                        B->Uv[c].as<TS>(), B->Lv[o].as<TS>(), B->Uv[o].as<TS>(),
                                        B->rs.as<float>(), B->partial.as<double>());
     };
-    // variant 0 takes the tables with L, U and U_JJ staged where that
-    // staging applies, else the device-side merge; 2 the merge, 3 the
-    // tables, 4 the tables with every block set staged, 5 with L, U and
-    // U_JJ staged (wherever the staging applies).  Measured on 5x5 blocks of
-    // a 64^3 grid (FP32 / BF16 / FP64 storage): merge 235 / 237 / 434 us,
-    // tables 251 / 257 / 464, all staged 219 / 206 / 493 (48 KB of LDS per
-    // workgroup in FP64), L, U and U_JJ staged 204 / 191 / 399.
+    // variant 0 takes the wave-run form where its LDS fits, else the
+    // tables with L, U and U_JJ staged where that staging applies, else the
+    // device-side merge; 2 the merge, 3 the tables, 4 the tables with every
+    // block set staged, 5 with L, U and U_JJ staged (wherever the staging
+    // applies), 6 the wave-run form.  Measured on 5x5 blocks of a 64^3 grid
+    // (FP32 / BF16 / FP64 storage): merge 235 / 237 / 434 us, tables 251 /
+    // 257 / 464, all staged 219 / 206 / 493 (48 KB of LDS per workgroup in
+    // FP64), L, U and U_JJ staged 204 / 191 / 399, wave runs 138 / 155 / 297.
     constexpr bool can_stage = sym_lds_ok<BS, TC, TS>();
     constexpr bool can_stage_lu = sym_lds_lu_ok<BS, TC, TS>();
+    constexpr bool can_wave = sym_wave_ok<BS, TC, TS>();
+    const bool wave = can_wave && (B->variant == 0 || B->variant == 6);
     const bool tables = B->variant >= 3 || (B->variant == 0 && can_stage_lu);
-    if (tables) {
+    if (wave) {
+      auto run = [&](auto kern) {
+        kern<<<nb, kBBS, 0, stream()>>>(B->nnzL, B->smeta.as<int4>(), B->spend.as<int>(),
+                                         B->spairs.as<int4>(), B->AL.as<TC>(), B->AU.as<TC>(),
+                                         B->Lv[c].as<TS>(), B->Uv[c].as<TS>(), B->Lv[o].as<TS>(),
+                                         B->Uv[o].as<TS>(), B->rs.as<float>(), B->partial.as<double>());
+      };
+      if constexpr (can_wave) {
+        if (B->wave_local) {
+          if (with_step) run(bilu_sym_sweep_wave_kernel<BS, TC, TS, true, true>);
+          else run(bilu_sym_sweep_wave_kernel<BS, TC, TS, false, true>);
+        } else {
+          if (with_step) run(bilu_sym_sweep_wave_kernel<BS, TC, TS, true, false>);
+          else run(bilu_sym_sweep_wave_kernel<BS, TC, TS, false, false>);
+        }
+      }
+    } else if (tables) {
       const bool stage = can_stage && B->variant == 4;
       const bool stage_lu = can_stage_lu && (B->variant == 0 || B->variant == 5);
       auto run = [&](auto kern) {

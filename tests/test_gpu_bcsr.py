@@ -122,8 +122,9 @@ def test_paired_sweep_forms_identical(gpu, orc, prec, bs, pattern):
     """The forms of the paired sweep: the device-side merge of the two block
     rows (variant 2) against the set-up tables without staging (3), with
     each workgroup's blocks staged through LDS wherever that applies (4),
-    with only L, U and U_JJ staged (5), and the default choice (0):
-    identical factors and step norms over many
+    with only L, U and U_JJ staged (5), the wave-run form (6: whole-word
+    runs per wave, columns of inv(U_JJ) split over a group's lanes), and
+    the default choice (0): identical factors and step norms over many
     workgroups with a partial last one.  A 7-point grid has no products
     over K < J (no triangles); the random pattern has many."""
     if pattern == "grid":
@@ -131,7 +132,7 @@ def test_paired_sweep_forms_identical(gpu, orc, prec, bs, pattern):
     else:
         B = _block_sym_random(orc, 300, bs, seed=40 + bs)
     code = _PREC[prec][0]
-    Ps = {v: gpu.BILU(_host(gpu, B), code) for v in (2, 3, 4, 5, 0)}
+    Ps = {v: gpu.BILU(_host(gpu, B), code) for v in (2, 3, 4, 5, 6, 0)}
     for v, P in Ps.items():
         assert P.symmetric()
         if v:
@@ -139,7 +140,7 @@ def test_paired_sweep_forms_identical(gpu, orc, prec, bs, pattern):
     for k in range(3):
         steps = {v: P.sweeps(1, want_step=True) for v, P in Ps.items()}
         (_, _, l2, _), (_, _, u2, _) = Ps[2].factors()
-        for v in (3, 4, 5, 0):
+        for v in (3, 4, 5, 6, 0):
             assert steps[v] == steps[2], (v, k)
             (_, _, lv, _), (_, _, uv, _) = Ps[v].factors()
             assert np.array_equal(lv, l2) and np.array_equal(uv, u2), (v, k)
