@@ -155,7 +155,12 @@ __global__ __launch_bounds__(kSweepBS) void parilu_stream_sweep_kernel(
 // a_ij, a_ji, l_ij and u_ji come in as 16-byte words and both outputs go out
 // as 16-byte words; each position's arithmetic is the one above (a diagonal
 // position also stores its unchanged unit l_ii).  np must be a multiple of 4
-// for the vector path; the last partial group takes the scalar form.
+// for the vector path; the last partial group takes the scalar form.  A
+// diagonal position's products l_iq u_qi run over the strict positions of
+// its own row, which precede it: those among the thread's four positions
+// and the previous lane's four (shuffled in before any branch) come from
+// registers, only earlier ones (rows longer than the window, lane 0) from
+// memory, in the same ascending order.
 template <typename T, bool STEP, bool MATCH>
 __global__ __launch_bounds__(kSweepBS) void parilu_stream4_sweep_kernel(
     int np, const int* __restrict__ pmeta, const int* __restrict__ mptr,
@@ -165,31 +170,44 @@ __global__ __launch_bounds__(kSweepBS) void parilu_stream4_sweep_kernel(
 #pragma clang fp contract(off)
   const int b = xcd_block(blockIdx.x, gridDim.x);
   const int p0 = (b * kSweepBS + (int)threadIdx.x) * 4;
+  const int lane = threadIdx.x & 63;
+  const bool full = p0 + 3 < np;
+  int mv[4];
+  T al[4], au[4], lo[4], uo[4], lout[4], uout[4], ujj[4];
+  // u_jj of each strict position is gathered as soon as pmeta is in, in
+  // the branch that loads it (every index valid: a diagonal position reads
+  // entry 0 and ignores it)
+  if (full) {
+    vload<int, 4>(pmeta + p0, mv);
+    vload<T, 4>(AL + p0, al);
+    vload<T, 4>(AU + p0, au);
+    vload<T, 4>(VLin + p0, lo);
+    vload<T, 4>(VUin + p0, uo);
+#pragma unroll
+    for (int e = 0; e < 4; e++) ujj[e] = VUin[max(mv[e], 0)];
+  } else {
+#pragma unroll
+    for (int e = 0; e < 4; e++) {
+      const int p = max(0, min(p0 + e, np - 1));
+      mv[e] = pmeta[p];
+      al[e] = AL[p];
+      au[e] = AU[p];
+      lo[e] = VLin[p];
+      uo[e] = VUin[p];
+    }
+#pragma unroll
+    for (int e = 0; e < 4; e++) ujj[e] = VUin[max(mv[e], 0)];
+  }
+  // the previous lane's old l and u (positions p0 - 4 .. p0 - 1), every lane
+  // taking part in the shuffles
+  T plo[4], puo[4];
+#pragma unroll
+  for (int e = 0; e < 4; e++) {
+    plo[e] = __shfl_up(lo[e], 1);
+    puo[e] = __shfl_up(uo[e], 1);
+  }
   double d2 = 0.0;
   if (p0 < np) {
-    const bool full = p0 + 3 < np;
-    int mv[4];
-    T al[4], au[4], lo[4], uo[4], lout[4], uout[4];
-    if (full) {
-      vload<int, 4>(pmeta + p0, mv);
-      vload<T, 4>(AL + p0, al);
-      vload<T, 4>(AU + p0, au);
-      vload<T, 4>(VLin + p0, lo);
-      vload<T, 4>(VUin + p0, uo);
-    } else {
-#pragma unroll
-      for (int e = 0; e < 4; e++) {
-        const int p = min(p0 + e, np - 1);
-        mv[e] = pmeta[p];
-        al[e] = AL[p];
-        au[e] = AU[p];
-        lo[e] = VLin[p];
-        uo[e] = VUin[p];
-      }
-    }
-    T ujj[4];
-#pragma unroll
-    for (int e = 0; e < 4; e++) ujj[e] = mv[e] >= 0 ? VUin[mv[e]] : T(1);
 #pragma unroll
     for (int e = 0; e < 4; e++) {
       const int p = p0 + e;
@@ -198,13 +216,32 @@ __global__ __launch_bounds__(kSweepBS) void parilu_stream4_sweep_kernel(
       if (p >= np) {
         uout[e] = lout[e] = T(0);
       } else if (m < 0) {
+        const int i0 = -m - 1;
         T s = au[e];
-        for (int q = -m - 1; q < p; q++) s = s - VLin[q] * VUin[q];
+        for (int q = i0; q < min(p, p0 - 4); q++) s = s - VLin[q] * VUin[q];
+#pragma unroll
+        for (int k = -4; k < e; k++) {
+          const int q = p0 + k;
+          if (q >= i0) {
+            T lq, uq;
+            if (k >= 0) {
+              lq = lo[k < 0 ? 0 : k];
+              uq = uo[k < 0 ? 0 : k];
+            } else if (lane > 0) {
+              lq = plo[k < 0 ? k + 4 : 0];
+              uq = puo[k < 0 ? k + 4 : 0];
+            } else {
+              lq = VLin[q];
+              uq = VUin[q];
+            }
+            s = s - lq * uq;
+          }
+        }
         s = s - pu;
         s = s + pu;
         uout[e] = s;
         lout[e] = lo[e];
-        if (STEP && p < np) {
+        if (STEP) {
           const double d = (double)uo[e] - (double)s;
           d2 += d * d;
         }
@@ -225,7 +262,7 @@ __global__ __launch_bounds__(kSweepBS) void parilu_stream4_sweep_kernel(
         sL = sL + pl;
         const T t = sL / ujj[e];
         lout[e] = t;
-        if (STEP && p < np) {
+        if (STEP) {
           const double d1 = (double)uo[e] - (double)sU;
           const double d = (double)lo[e] - (double)t;
           d2 += d1 * d1 + d * d;
