@@ -462,6 +462,66 @@ BLOCK_REGIONS = ["tsgemm_tn", "tsgemm_nn", "spmm32_f32x", "spmm32_f32a", "spmm32
                  "inner_update", "block_inner"]
 
 
+def bench_trsv(args):
+    """Exact triangular solves (the level-scheduled ParILU apply, L then U)
+    on the 128^3 convection-diffusion factors after 3 sweeps, FP32 and FP64:
+    the persistent natural-order kernel (default) and the level-ordered
+    sync-free kernel, each timed by HIP events on the library stream; the
+    level counts of L and U, the time per level, and whether both kernels
+    give the same bits."""
+    import ctypes as C
+    import lace_amd as L
+    lib = L.lib()
+    L.check(lib.lace_init(0), "lace_init")
+    N = args.size
+    row, col, val = L.gen_convdiff3d(N, N, N)
+    n = len(row) - 1
+    H = L.HostCSR(row, col, val)
+    rng = np.random.default_rng(0)
+    out = {"mode": "trsv", "workload": f"exact L and U solves of the 3-sweep ParILU factors, "
+           f"7-point convection-diffusion {N}^3", "n": n}
+    reps = max(args.steps, 5)
+    ok = True
+    for name, prec, dt in (("f32", L.LACE_F32, np.float32), ("f64", L.LACE_F64, np.float64)):
+        P = L.ParILU(H, prec)
+        P.sweeps(3)
+        nl, nu = P.levels()
+        r = L.DevVec(n, dt, rng.standard_normal(n).astype(dt))
+        z = L.DevVec(n, dt)
+        res = {"levels_L": nl, "levels_U": nu}
+        zs = {}
+        for mode, tag, regions in ((1, "persistent", ["trsv_exact_lower", "trsv_exact_upper"]),
+                                   (0, "syncfree", ["trsv_syncfree"])):
+            L.check(lib.lace_set_option(b"trsv_exact", mode))
+            apply = lambda: L.check(lib.lace_parilu_apply(P.h, L.TRSV_LEVEL, 0, 0, r.ptr, z.ptr))
+            apply()
+            L.check(lib.lace_device_synchronize())
+            L.check(lib.lace_timing_reset())
+            L.check(lib.lace_timing_only(",".join(regions).encode()))
+            L.check(lib.lace_timing_enable(1))
+            t0 = time.perf_counter()
+            for _ in range(reps):
+                apply()
+            L.check(lib.lace_device_synchronize())
+            wall = (time.perf_counter() - t0) / reps
+            L.check(lib.lace_timing_enable(0))
+            L.check(lib.lace_timing_only(None))
+            ms = sum(L.timing(rg)[0] for rg in regions if L.timing(rg)[1] > 0) / reps
+            ev = C.c_int()
+            L.check(lib.lace_syncfree_error(P.h, C.byref(ev)))
+            zs[tag] = z.download()
+            res[tag] = {"ms_kernels": round(ms, 3), "ms_call": round(wall * 1e3, 3),
+                        "us_per_level": round(ms * 1e3 / (nl + nu), 3), "wait_error": ev.value}
+            ok = ok and ev.value == 0 and bool(np.isfinite(zs[tag]).all())
+        res["bit_identical"] = bool(np.array_equal(zs["persistent"], zs["syncfree"]))
+        ok = ok and res["bit_identical"]
+        out[name] = res
+        del P
+    L.check(lib.lace_set_option(b"trsv_exact", 1))
+    out["all_ok"] = ok
+    return out
+
+
 def bench_block(args):
     """Config 5: 32 N(0,1) right-hand sides (Philox seeds 0-31) on the 128^3
     convection-diffusion matrix.  One step = FP32 ParILU setup (3 sweeps) +
@@ -591,12 +651,13 @@ def main() -> int:
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--block-inner", type=int, default=10,
                     help="block mode: inner FP32 GMRES steps per outer block step (0: ILU only)")
-    ap.add_argument("--mode", choices=["solve", "kernels", "bcsr", "block", "reorder"], default="solve",
+    ap.add_argument("--mode", choices=["solve", "kernels", "bcsr", "block", "reorder", "trsv"], default="solve",
                     help="kernels: one mixed-path step plus one 256 MiB copy kernel "
                          "(the workload profiled for HBM traffic counters); bcsr: the "
                          "block-sparse 5x5 workload on the 64^3 block grid; block: 32 "
                          "right-hand sides, forward and adjoint block FGMRES; reorder: the "
-                         "RCM pre-step against natural order")
+                         "RCM pre-step against natural order; trsv: the exact L and U "
+                         "solves, persistent and level-ordered kernels")
     ap.add_argument("--set", action="append", default=[], metavar="NAME=VALUE",
                     help="library kernel-shape option (lace_set_option) applied before the run; "
                          "repeatable; every choice gives bit-identical results")
@@ -622,6 +683,10 @@ def main() -> int:
         return 0 if out["all_converged"] else 1
     if args.mode == "reorder":
         return bench_reorder(args)
+    if args.mode == "trsv":
+        out = bench_trsv(args)
+        print(json.dumps(out))
+        return 0 if out["all_ok"] else 1
 
     import lace_amd as L
     lib = L.lib()
@@ -671,6 +736,20 @@ def main() -> int:
     work = {r: L.timing_work(r) for r in HOT_REGIONS}
     res_fp64 = timed_solves(L, fp64, db, dx, zeros, args.steps)
     x_fp64 = dx.download()
+    # one more FP64 FGMRES(50) solve with the Householder multi-dot U_j^T w
+    # (j + 1 columns of 16.8 MB and w, every Arnoldi step) timed by events:
+    # its rate over all the solve's column counts, up to 50
+    L.check(lib.lace_timing_reset())
+    L.check(lib.lace_timing_only(b"hh_mdot"))
+    L.check(lib.lace_timing_enable(1))
+    timed_solves(L, fp64, db, dx, zeros, 1)
+    L.check(lib.lace_timing_enable(0))
+    L.check(lib.lace_timing_only(None))
+    md_ms, md_n = L.timing("hh_mdot")
+    md_bytes = L.timing_work("hh_mdot")[1]
+    hh_mdot = {"launches": md_n, "avg_us": round(md_ms * 1e3 / max(md_n, 1), 2),
+               "GBps": round(md_bytes / max(md_ms * 1e-3, 1e-12) / 1e9, 1),
+               "avg_columns": round(md_bytes / max(md_n, 1) / (8.0 * n) - 1.0, 1)}
     res_fp64n = timed_solves(L, fp64n, db, dx, zeros, args.steps)
     x_fp64n = dx.download()
     res_m16 = timed_solves(L, mixed16, db, dx, zeros, args.steps)
@@ -800,6 +879,7 @@ def main() -> int:
         "fp64_rel_true_residual": st_f.rel_true_residual,
         "all_converged": ok,
         "parilu_sweep_GBps": {"f32": round(achieved, 1), "f64": round(ach64, 1)},
+        "fp64_hh_mdot": hh_mdot,
         # the kernel with the most time in the timed mixed steps
         "roofline": dict(kernel=dominant, **kernels[dominant]) if dominant else None,
         "kernels": kernels,

@@ -229,18 +229,22 @@ __global__ __launch_bounds__(kBS, 4) void mdot_rows_kernel(int n, int ld, const 
 // Launches the multi-dot: the register path for whole slabs of 1024 or
 // 2048 rows (8 columns per pass), the rows-outer form otherwise (NC columns
 // per pass).
+// The step's first multi-dot (U_j^T w, no self term) is the timing region
+// "hh_mdot" with its algorithmic bytes, every column and w once.
 template <typename K, int NC, bool SELF>
 static void launch_mdot_rows(const HHEngine<K>& E, const K* U, int ncols, const K* w, int rowstart,
                              double* part, const double* misc) {
+  TimedRegion tr(SELF ? "hh_mdot_self" : "hh_mdot", 0.0,
+                 (double)sizeof(K) * (double)(E.n - rowstart) * (double)(ncols + 1), true);
   if (E.rpb == 8 * kBS)
-    mdot_rows_kernel<K, 8, SELF, 8><<<E.nb, kBS, 0, stream()>>>(E.n, E.ld, U, ncols, w, rowstart, E.rpb, part,
-                                                              E.pstride, misc);
+    tr.launch(mdot_rows_kernel<K, 8, SELF, 8>, dim3(E.nb), dim3(kBS), E.n, E.ld, U, ncols, w, rowstart, E.rpb,
+              part, E.pstride, misc);
   else if (E.rpb == 4 * kBS)
-    mdot_rows_kernel<K, 8, SELF, 4><<<E.nb, kBS, 0, stream()>>>(E.n, E.ld, U, ncols, w, rowstart, E.rpb, part,
-                                                              E.pstride, misc);
+    tr.launch(mdot_rows_kernel<K, 8, SELF, 4>, dim3(E.nb), dim3(kBS), E.n, E.ld, U, ncols, w, rowstart, E.rpb,
+              part, E.pstride, misc);
   else
-    mdot_rows_kernel<K, NC, SELF, 0><<<E.nb, kBS, 0, stream()>>>(E.n, E.ld, U, ncols, w, rowstart, E.rpb, part,
-                                                               E.pstride, misc);
+    tr.launch(mdot_rows_kernel<K, NC, SELF, 0>, dim3(E.nb), dim3(kBS), E.n, E.ld, U, ncols, w, rowstart, E.rpb,
+              part, E.pstride, misc);
 }
 
 // Column totals of the per-block partials when there are many columns: one
