@@ -151,43 +151,47 @@ __global__ __launch_bounds__(kSweepBS) void parilu_stream_sweep_kernel(
   }
 }
 
-// The streaming sweep with four consecutive positions per thread: pmeta,
-// a_ij, a_ji, l_ij and u_ji come in as 16-byte words and both outputs go out
-// as 16-byte words; each position's arithmetic is the one above (a diagonal
-// position also stores its unchanged unit l_ii).  np must be a multiple of 4
-// for the vector path; the last partial group takes the scalar form.  A
-// diagonal position's products l_iq u_qi run over the strict positions of
-// its own row, which precede it: those among the thread's four positions
-// and the previous lane's four (shuffled in before any branch) come from
-// registers, only earlier ones (rows longer than the window, lane 0) from
-// memory, in the same ascending order.
-template <typename T, bool STEP, bool MATCH>
+// The streaming sweep with P consecutive positions per thread (4 in FP32,
+// 2 in FP64): pmeta, a_ij, a_ji, l_ij and u_ji come in as 16-byte words
+// (pmeta as a P * 4-byte word) and both outputs go out as 16-byte words;
+// each position's arithmetic is the one above (a diagonal position also
+// stores its unchanged unit l_ii).  np must be a multiple of P for the
+// vector path; the last partial group takes the scalar form.  A diagonal
+// position's products l_iq u_qi run over the strict positions of its own
+// row, which precede it: those among the thread's positions and the four
+// before them (the previous lanes', shuffled in before any branch) come
+// from registers, only earlier ones (rows longer than the window, the
+// first lanes of a wave) from memory, in the same ascending order.
+template <typename T, bool STEP, bool MATCH, int P>
 __global__ __launch_bounds__(kSweepBS) void parilu_stream4_sweep_kernel(
     int np, const int* __restrict__ pmeta, const int* __restrict__ mptr,
     const int* __restrict__ mpairs, const T* __restrict__ AL, const T* __restrict__ AU,
     const T* __restrict__ VLin, const T* __restrict__ VUin, T* __restrict__ VLout,
     T* __restrict__ VUout, double* __restrict__ partial) {
 #pragma clang fp contract(off)
+  static_assert(P == 2 || P == 4, "two or four positions per thread");
+  static_assert(P * sizeof(T) % 16 == 0, "16-byte words of P values");
+  constexpr int W = 4;  // positions of the window before the thread's first
   const int b = xcd_block(blockIdx.x, gridDim.x);
-  const int p0 = (b * kSweepBS + (int)threadIdx.x) * 4;
+  const int p0 = (b * kSweepBS + (int)threadIdx.x) * P;
   const int lane = threadIdx.x & 63;
-  const bool full = p0 + 3 < np;
-  int mv[4];
-  T al[4], au[4], lo[4], uo[4], lout[4], uout[4], ujj[4];
+  const bool full = p0 + P - 1 < np;
+  int mv[P];
+  T al[P], au[P], lo[P], uo[P], lout[P], uout[P], ujj[P];
   // u_jj of each strict position is gathered as soon as pmeta is in, in
   // the branch that loads it (every index valid: a diagonal position reads
   // entry 0 and ignores it)
   if (full) {
-    vload<int, 4>(pmeta + p0, mv);
-    vload<T, 4>(AL + p0, al);
-    vload<T, 4>(AU + p0, au);
-    vload<T, 4>(VLin + p0, lo);
-    vload<T, 4>(VUin + p0, uo);
+    wload<int, P>(pmeta + p0, mv);
+    vload<T, P>(AL + p0, al);
+    vload<T, P>(AU + p0, au);
+    vload<T, P>(VLin + p0, lo);
+    vload<T, P>(VUin + p0, uo);
 #pragma unroll
-    for (int e = 0; e < 4; e++) ujj[e] = VUin[max(mv[e], 0)];
+    for (int e = 0; e < P; e++) ujj[e] = VUin[max(mv[e], 0)];
   } else {
 #pragma unroll
-    for (int e = 0; e < 4; e++) {
+    for (int e = 0; e < P; e++) {
       const int p = max(0, min(p0 + e, np - 1));
       mv[e] = pmeta[p];
       al[e] = AL[p];
@@ -196,20 +200,23 @@ __global__ __launch_bounds__(kSweepBS) void parilu_stream4_sweep_kernel(
       uo[e] = VUin[p];
     }
 #pragma unroll
-    for (int e = 0; e < 4; e++) ujj[e] = VUin[max(mv[e], 0)];
+    for (int e = 0; e < P; e++) ujj[e] = VUin[max(mv[e], 0)];
   }
-  // the previous lane's old l and u (positions p0 - 4 .. p0 - 1), every lane
-  // taking part in the shuffles
-  T plo[4], puo[4];
+  // old l and u of positions p0 - W .. p0 - 1 from the W / P previous lanes,
+  // every lane taking part in the shuffles; plo[k] holds position
+  // p0 - W + k, valid when that lane is in this wave (lane >= its distance)
+  T plo[W], puo[W];
 #pragma unroll
-  for (int e = 0; e < 4; e++) {
-    plo[e] = __shfl_up(lo[e], 1);
-    puo[e] = __shfl_up(uo[e], 1);
+  for (int k = 0; k < W; k++) {
+    const int back = (W - k + P - 1) / P;  // lanes back
+    const int e = k - (W - back * P);      // position within that lane
+    plo[k] = __shfl_up(lo[e], back);
+    puo[k] = __shfl_up(uo[e], back);
   }
   double d2 = 0.0;
   if (p0 < np) {
 #pragma unroll
-    for (int e = 0; e < 4; e++) {
+    for (int e = 0; e < P; e++) {
       const int p = p0 + e;
       const int m = mv[e];
       const T pu = T(1) * uo[e];
@@ -218,18 +225,19 @@ __global__ __launch_bounds__(kSweepBS) void parilu_stream4_sweep_kernel(
       } else if (m < 0) {
         const int i0 = -m - 1;
         T s = au[e];
-        for (int q = i0; q < min(p, p0 - 4); q++) s = s - VLin[q] * VUin[q];
+        for (int q = i0; q < min(p, p0 - W); q++) s = s - VLin[q] * VUin[q];
 #pragma unroll
-        for (int k = -4; k < e; k++) {
+        for (int k = -W; k < e; k++) {
           const int q = p0 + k;
           if (q >= i0) {
             T lq, uq;
+            const int back = (-k + P - 1) / P;  // lanes back for k < 0
             if (k >= 0) {
               lq = lo[k < 0 ? 0 : k];
               uq = uo[k < 0 ? 0 : k];
-            } else if (lane > 0) {
-              lq = plo[k < 0 ? k + 4 : 0];
-              uq = puo[k < 0 ? k + 4 : 0];
+            } else if (lane >= back) {
+              lq = plo[k < 0 ? k + W : 0];
+              uq = puo[k < 0 ? k + W : 0];
             } else {
               lq = VLin[q];
               uq = VUin[q];
@@ -270,11 +278,11 @@ __global__ __launch_bounds__(kSweepBS) void parilu_stream4_sweep_kernel(
       }
     }
     if (full) {
-      vstore<T, 4>(VLout + p0, lout);
-      vstore<T, 4>(VUout + p0, uout);
+      vstore<T, P>(VLout + p0, lout);
+      vstore<T, P>(VUout + p0, uout);
     } else {
 #pragma unroll
-      for (int e = 0; e < 4; e++)
+      for (int e = 0; e < P; e++)
         if (p0 + e < np) {
           VLout[p0 + e] = lout[e];
           VUout[p0 + e] = uout[e];
@@ -880,18 +888,21 @@ static int sweep_t(ParILU* P, bool with_step) {
   // symmetric patterns default to the four-position streaming kernel
   // (128^3: FP32 38.8 us vs 43.7 us for one position per thread)
   if (stream4) {
-    const int nb = ceil_div(P->nnzL, kSweepBS * 4);
+    // four positions per thread in FP32, two in FP64 (one 16-byte word per
+    // array either way)
+    constexpr int PP = sizeof(T) == 8 ? 2 : 4;
+    const int nb = ceil_div(P->nnzL, kSweepBS * PP);
     auto go = [&](auto kern) {
       tr.launch(kern, dim3(nb), dim3(kSweepBS), P->nnzL, P->pmeta.as<int>(), P->mptr.as<int>(),
                 P->mpairs.as<int>(), P->AL.as<T>(), P->AU.as<T>(), P->lval[in].as<T>(),
                 P->uval[in].as<T>(), P->lval[out].as<T>(), P->uval[out].as<T>(), P->partial.as<double>());
     };
     if (P->strict_matches) {
-      if (with_step) go(parilu_stream4_sweep_kernel<T, true, true>);
-      else go(parilu_stream4_sweep_kernel<T, false, true>);
+      if (with_step) go(parilu_stream4_sweep_kernel<T, true, true, PP>);
+      else go(parilu_stream4_sweep_kernel<T, false, true, PP>);
     } else {
-      if (with_step) go(parilu_stream4_sweep_kernel<T, true, false>);
-      else go(parilu_stream4_sweep_kernel<T, false, false>);
+      if (with_step) go(parilu_stream4_sweep_kernel<T, true, false, PP>);
+      else go(parilu_stream4_sweep_kernel<T, false, false, PP>);
     }
     LACE_HIP(hipGetLastError());
     P->last_nb = nb;
