@@ -223,6 +223,25 @@ int data_dcsrilu0_mkl(data_d_matrix *A);
  * by step <= reduction*Ares (the reference has no cap; 1000 here). */
 void data_PariLU_v0_2(data_d_matrix *A, data_d_matrix *L, data_d_matrix *U,
                       dataType reduction);
+
+/* Tiled ParILU family.  sparse.h:813-869; src/parilu_v3_0.cpp:15-229 and
+ * src/parilu_v3_{1,2,5,6,7,8,9}.cpp (the same fixed point with other tile
+ * products).  Runs on the host, for CPU comparison: A is padded to whole
+ * tiles in place (pad_rows, pad_cols, one unit entry per padding row, its
+ * arrays replaced; control/magma_zmio.cpp:973-1040), L is returned as CSRL
+ * (unit diagonal last), U as CSRU (diagonal first).  The tile grid is swept
+ * in place in the order of the reference run on one thread, until the step
+ * / ||A||_F is at most 1e-15 (at most 1000 sweeps in v3.0-v3.2, 10000 in
+ * v3.5-v3.9); the log lines are the reference's.  On failure the reason is
+ * in lace_last_error(). */
+void data_PariLU_v3_0(data_d_matrix *A, data_d_matrix *L, data_d_matrix *U, int tile);
+void data_PariLU_v3_1(data_d_matrix *A, data_d_matrix *L, data_d_matrix *U, int tile);
+void data_PariLU_v3_2(data_d_matrix *A, data_d_matrix *L, data_d_matrix *U, int tile);
+void data_PariLU_v3_5(data_d_matrix *A, data_d_matrix *L, data_d_matrix *U, int tile);
+void data_PariLU_v3_6(data_d_matrix *A, data_d_matrix *L, data_d_matrix *U, int tile);
+void data_PariLU_v3_7(data_d_matrix *A, data_d_matrix *L, data_d_matrix *U, int tile);
+void data_PariLU_v3_8(data_d_matrix *A, data_d_matrix *L, data_d_matrix *U, int tile);
+void data_PariLU_v3_9(data_d_matrix *A, data_d_matrix *L, data_d_matrix *U, int tile);
 void data_PariLU_v0_3(data_d_matrix *A, data_d_matrix *L, data_d_matrix *U,
                       dataType reduction, data_d_preconditioner_log *log);
 void data_PariLU_v0_3_prescribedSweeps(data_d_matrix *A, data_d_matrix *L,

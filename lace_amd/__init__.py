@@ -150,6 +150,7 @@ SIGNATURES = {
     "data_housegen": (None, [_i, _P(_d), _P(_d), _P(_d)]),
     "data_hqrd": (None, [_DM, _DM, _DM]),
     "data_PariLU_v0_2": (None, [_DM, _DM, _DM, _d]),
+    **{f"data_PariLU_v3_{v}": (None, [_DM, _DM, _DM, _i]) for v in (0, 1, 2, 5, 6, 7, 8, 9)},
     "data_MKL_FGMRES": (_i, [_DM, _DM, _DM, _P(GmresParam)]),
     "data_reorder_cuthill_mckee": (_i, [_DM, _P(_i), C.c_bool]),
     "data_sparse_reorder": (_i, [_DM, _P(_i), _P(_i), _i]),

@@ -397,6 +397,83 @@ def ilu0(A: CSR):
     return CSR(A.n_rows, A.n_cols, A.row.copy(), A.col.copy(), v), bad
 
 
+def parilu_tiled(A: CSR, tile: int, max_iter: int = 1000):
+    """The tiled ParILU iteration of src/parilu_v3_0.cpp:15-229 as the
+    reference runs it on one thread, restated with dense tiles in pure
+    Python (small matrices only): for each tile (ti, tj) of the padded grid
+    in row-major order, Lblock (tile x span, strict L, columns < span) and
+    Ublock (span x tile, strict U), span = min(ti, tj) + tile, are copied
+    out (control/sparse_sub.cpp:77-96), C = Lblock Ublock is summed over the
+    inner index in ascending order from zero, and then (ti <= tj) the tile's
+    strict U entries become Uc - C, on the diagonal tile D_i = 1/(Dc_i -
+    C_ii) and the tile's L entries (Lc - C) D_j; (ti > tj) the L entries.
+    The step is sum (old - new)^2 / ||A||_F; sweeps run while it exceeds
+    1e-15, at most max_iter.  U's diagonal ends as 1 / D.  Returns (L values
+    in CSRL order with the unit diagonal last, U values in CSRU order with
+    the diagonal first, sweeps)."""
+    import math
+    n = A.n_rows
+    lp = [[] for _ in range(n)]
+    up = [[] for _ in range(n)]
+    Lc = {}; Uc = {}; Dc = [0.0] * n
+    for i in range(n):
+        for k in range(A.row[i], A.row[i + 1]):
+            j = int(A.col[k]); v = float(A.val[k])
+            if j < i:
+                lp[i].append(j); Lc[(i, j)] = v
+            elif j > i:
+                up[i].append(j); Uc[(i, j)] = v
+            else:
+                Dc[i] = v
+    Lv = dict(Lc); Uv = dict(Uc)
+    D = [1.0 / d for d in Dc]
+    pad = -(-n // tile) * tile
+    anorm = math.sqrt(sum(float(v) * float(v) for v in A.val[:A.row[n]]))
+    step, it = 1.0, 0
+    while step > 1e-15 and it < max_iter:
+        step = 0.0
+        for ti in range(0, pad, tile):
+            for tj in range(0, pad, tile):
+                span = min(ti + tile, tj + tile)
+                Lb = [[Lv.get((i, k), 0.0) for k in range(span)] for i in range(ti, ti + tile)]
+                Ub = [[Uv.get((k, j), 0.0) for j in range(tj, tj + tile)] for k in range(span)]
+                Ct = [[0.0] * tile for _ in range(tile)]
+                for a in range(tile):
+                    for b in range(tile):
+                        acc = 0.0
+                        for k in range(span):
+                            acc = acc + Lb[a][k] * Ub[k][b]
+                        Ct[a][b] = acc
+                rows = range(ti, min(ti + tile, n))
+                if ti <= tj:
+                    for i in rows:
+                        for j in up[i]:
+                            if tj <= j < tj + tile:
+                                t = Uc[(i, j)] - Ct[i - ti][j - tj]
+                                step += math.pow(Uv[(i, j)] - t, 2)
+                                Uv[(i, j)] = t
+                    if ti == tj:
+                        for i in rows:
+                            t = 1.0 / (Dc[i] - Ct[i - ti][i - tj])
+                            step += math.pow(D[i] - t, 2)
+                            D[i] = t
+                if ti >= tj:
+                    for i in rows:
+                        for j in lp[i]:
+                            if tj <= j < tj + tile:
+                                t = (Lc[(i, j)] - Ct[i - ti][j - tj]) * D[j]
+                                step += math.pow(Lv[(i, j)] - t, 2)
+                                Lv[(i, j)] = t
+        step /= anorm
+        it += 1
+    lval = []
+    uval = []
+    for i in range(n):
+        lval += [Lv[(i, j)] for j in lp[i]] + [1.0]
+        uval += [1.0 / D[i]] + [Uv[(i, j)] for j in up[i]]
+    return np.array(lval), np.array(uval), it
+
+
 def split_merged(LU: CSR):
     """Merged ILU (data_zmlumerge layout) -> (CSRL unit diag, CSRU)."""
     Lf = csr_split(LU, False, DIAG_UNIT)

@@ -512,3 +512,33 @@ def test_parilu_16bit_storage_model(orc, storage):
         assert np.all(np.abs(Lf.dense() - Lw) <= tol), k
         tol = step * np.maximum(np.abs(Uw), s[:, None] * 2.0 ** -20)
         assert np.all(np.abs(Uf.dense() - Uw) <= tol), k
+
+
+def test_parilu_tiled_one_tile_first_sweep_closed_form(orc):
+    """The tiled ParILU restatement with one tile covering A: every entry of
+    the first sweep reads the initial values (strict L and U of A, D = 1 /
+    diag A), so U1 = A_U - (A_L A_U), D1 = 1 / (diag A - diag(A_L A_U)) and
+    L1 = (A_L - A_L A_U) D1 on A's pattern; with small integer entries the
+    product is exact and the closed form is computed with numpy.  A wrong
+    snapshot (an update read back inside the tile), the L update using the
+    old D, or the product missing the diagonal's terms would all differ."""
+    rng = np.random.default_rng(3)
+    n = 7
+    M = (rng.random((n, n)) < 0.5) * rng.integers(-3, 4, (n, n)).astype(float)
+    M += np.diag(rng.integers(5, 9, n).astype(float))
+    import scipy.sparse as sp
+    a = sp.csr_matrix(M)
+    a.sort_indices()
+    A = orc.CSR(n, n, a.indptr.astype(np.int32), a.indices.astype(np.int32), a.data)
+    lv, uv, it = orc.parilu_tiled(A, 16, max_iter=1)
+    assert it == 1
+    AL, AU, d = np.tril(M, -1), np.triu(M, 1), np.diag(M)
+    Cm = AL @ AU
+    U1 = AU - Cm
+    D1 = 1.0 / (d - np.diag(Cm))
+    L1 = (AL - Cm) * D1[None, :]
+    want_l, want_u = [], []
+    for i in range(n):
+        want_l += [L1[i, j] for j in range(i) if M[i, j] != 0] + [1.0]
+        want_u += [1.0 / D1[i]] + [U1[i, j] for j in range(i + 1, n) if M[i, j] != 0]
+    assert np.array_equal(lv, np.array(want_l)) and np.array_equal(uv, np.array(want_u))
