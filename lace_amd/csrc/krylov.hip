@@ -825,6 +825,211 @@ template int hh_finish<float, float>(HHEngine<float>&, float*);
 template int read_misc<double>(HHEngine<double>&);
 template int read_misc<float>(HHEngine<float>&);
 
+// ------------------------------------------------------------ CGS inner GMRES
+// GMRES with classical Gram-Schmidt on a basis kept unnormalised: column k
+// of U holds W_k, the basis vector v_k = W_k / ||W_k|| times nothing (the
+// scales s_k = 1 / ||W_k|| live in the small state), and Z column k holds
+// M^{-1} W_k.  Step j: w = A Z_j; one pass forms the dots W_k . w (k <= j);
+// one one-workgroup kernel turns them into the Hessenberg column of the
+// basis Z (h_kj = s_k (W_k . w)) and, since ||W_j||^2 arrived with the same
+// partials, finishes column j - 1 (its subdiagonal h_{j,j-1} = ||W_j||, the
+// Givens rotations, the residual estimate); one pass writes W_{j+1} = w -
+// sum_k h_kj s_k W_k and the partial sums of its squared norm.  With Z_j =
+// M^{-1} W_j the relation A Z_j = sum_{k <= j+1} h_kj v_k holds for the
+// vectors as stored, so x = Z y with y the least-squares solution is a GMRES
+// iterate of the preconditioned operator.  Two vector passes and one small
+// launch per step (the Householder engine takes three and two), at the
+// price of classical Gram-Schmidt's orthogonality, which a fixed short
+// inner solve inside a flexible outer FGMRES can afford.  The scales go in
+// the T slot of the small state, the raw Hessenberg column in R's column.
+template <typename K, int NC>
+__global__ __launch_bounds__(kBS) void cgs_update_kernel(int n, int ld, K* __restrict__ U, int ncols,
+                                                         const K* __restrict__ w, int store, int rpb,
+                                                         const double* __restrict__ small, SmallLayout lay,
+                                                         double* __restrict__ partial, int pstride) {
+  const double* misc = small + lay.misc();
+  if (is_done(misc)) return;
+  const int b = xcd_block(blockIdx.x, gridDim.x);
+  const int r0 = b * rpb, r1 = min(r0 + rpb, n);
+  const double* d = small + lay.d();
+  K dk[NC];
+#pragma unroll
+  for (int k = 0; k < NC; k++) dk[k] = k < ncols ? (K)d[k] : K(0);
+  constexpr int V = NC <= 8 ? 16 / (int)sizeof(K) : 1;
+  K ss = K(0);
+  K* Wn = U + (size_t)ncols * ld;
+  for (int i0 = r0 + (int)threadIdx.x * V; i0 < r1; i0 += kBS * V) {
+    if (i0 + V <= r1) {
+      K v[V];
+      wload<K, V>(w + i0, v);
+#pragma unroll
+      for (int k = 0; k < NC; k++) {
+        if (k < ncols) {
+          K u[V];
+          wload<K, V>(U + (size_t)k * ld + i0, u);
+#pragma unroll
+          for (int e = 0; e < V; e++) v[e] -= u[e] * dk[k];
+        }
+      }
+#pragma unroll
+      for (int e = 0; e < V; e++) ss += v[e] * v[e];
+      if (store) {
+#pragma unroll
+        for (int e = 0; e < V; e++) Wn[i0 + e] = v[e];
+      }
+    } else {
+      for (int i = i0; i < r1; i++) {
+        K v = w[i];
+#pragma unroll
+        for (int k = 0; k < NC; k++)
+          if (k < ncols) v -= U[(size_t)k * ld + i] * dk[k];
+        ss += v * v;
+        if (store) Wn[i] = v;
+      }
+    }
+  }
+  __shared__ double sm[kBS / 64];
+  const double tot = block_sum<kBS>((double)ss, sm);
+  if (threadIdx.x == 0) partial[(size_t)b * pstride + pstride - 1] = tot;
+}
+
+__global__ void cgs_reset_kernel(double* small, SmallLayout lay) {
+  double* misc = small + lay.misc();
+  misc[MS_DONE] = 0.0;
+  misc[MS_BREAK] = 0.0;
+  misc[MS_NCOLS] = 0.0;
+}
+
+// Column j - 1 finished with its subdiagonal h = ||W_j|| (j >= 1): the
+// previous rotations, a new one (src/givens.cpp:46-67), g, the residual.
+__device__ void cgs_close_column(double* small, SmallLayout lay, int jc, double sub) {
+  double* misc = small + lay.misc();
+  double* R = small + lay.R();
+  const int ldR = lay.m + 1;
+  double* cs = small + lay.cs();
+  double* sn = small + lay.sn();
+  double* g = small + lay.g();
+  double* h = R + (size_t)jc * ldR;
+  for (int i = 0; i < jc; i++) {
+    const double t0 = h[i], t1 = h[i + 1];
+    h[i + 1] = -sn[i] * t0 + cs[i] * t1;
+    h[i] = cs[i] * t0 + sn[i] * t1;
+  }
+  const double a = h[jc], bb = sub;
+  const double tol = 4.9406564584124654e-324;  // nextafter(0,1)
+  double c, s;
+  if (fabs(bb) <= tol) {
+    c = 1.0;
+    s = 0.0;
+  } else if (fabs(bb) > fabs(a)) {
+    const double t = a / bb;
+    s = 1.0 / sqrt(1.0 + t * t);
+    c = t * s;
+  } else {
+    const double t = bb / a;
+    c = 1.0 / sqrt(1.0 + t * t);
+    s = t * c;
+  }
+  cs[jc] = c;
+  sn[jc] = s;
+  h[jc] = c * a + s * bb;
+  const double gj = g[jc];
+  g[jc + 1] = -s * gj;
+  g[jc] = c * gj;
+  const double res = fabs(g[jc + 1]);
+  misc[MS_RES] = res;
+  misc[MS_NCOLS] = (double)(jc + 1);
+  if (!(res >= misc[MS_RTOL]) || !isfinite(res) || !(sub > 0.0)) misc[MS_DONE] = 1.0;
+  if (!isfinite(res)) misc[MS_BREAK] = 1.0;
+}
+
+// Step j's small part (j = m: the closing of column m - 1 only).
+__global__ __launch_bounds__(256) void cgs_small_kernel(int nb, int pstride, const double* partial, int j,
+                                                        double* small, SmallLayout lay, int set_rtol,
+                                                        double rtol_rel) {
+  double* misc = small + lay.misc();
+  if (is_done(misc)) return;
+  __shared__ double s[72];
+  const int cnt = j < lay.m ? j + 1 : 0;
+  finalize_partials(nb, cnt, true, pstride, partial, s);  // s[0..cnt), s[cnt] = ||W_j||^2
+  __syncthreads();
+  double* sc = small + lay.T();  // 1 / ||W_k||
+  if (threadIdx.x == 0) {
+    const double nrm = sqrt(s[cnt]);
+    if (j == 0) {
+      misc[MS_RNORM] = nrm;
+      misc[MS_RES] = nrm;
+      if (set_rtol) misc[MS_RTOL] = rtol_rel * nrm;
+      double* g = small + lay.g();
+      for (int t = 0; t < lay.m + 2; t++) g[t] = 0.0;
+      g[0] = nrm;
+      if (!(nrm >= misc[MS_RTOL]) || !isfinite(nrm) || nrm == 0.0) misc[MS_DONE] = 1.0;
+    } else {
+      cgs_close_column(small, lay, j - 1, nrm);
+    }
+    sc[j] = nrm > 0.0 ? 1.0 / nrm : 0.0;
+  }
+  __syncthreads();
+  if (j >= lay.m || is_done(misc)) return;
+  // column j: h_kj = s_k (W_k . w) into R's column (rotated when it closes)
+  // and the update's coefficients d_k = h_kj s_k
+  double* R = small + lay.R();
+  const int ldR = lay.m + 1;
+  double* d = small + lay.d();
+  for (int k = threadIdx.x; k <= j; k += 256) {
+    const double h = sc[k] * s[k];
+    R[k + (size_t)j * ldR] = h;
+    d[k] = h * sc[k];
+  }
+}
+
+int cast_d2f(int n, const double* in, float* out);
+
+// W_0 = r (cast), the partial sums of ||r||^2, and a fresh cycle state.
+template <typename K>
+int cgs_start(HHEngine<K>& E, const double* r) {
+  TimedRegion tr("cgs_start");
+  K* U = E.U.template as<K>();
+  if constexpr (sizeof(K) == 4) {
+    LACE_TRY(cast_d2f(E.n, r, U));
+  } else {
+    LACE_HIP(hipMemcpyAsync(U, r, sizeof(double) * E.n, hipMemcpyDeviceToDevice, stream()));
+  }
+  cgs_reset_kernel<<<1, 1, 0, stream()>>>(E.small.template as<double>(), E.lay);
+  launch_mdot_rows<K, 8, true>(E, U, 0, U, 0, E.partial.template as<double>(), nullptr);
+  LACE_HIP(hipGetLastError());
+  return DEV_SUCCESS;
+}
+
+// Step j after w = A Z_j (Z_j = M^{-1} W_j).  last: the step's new vector
+// is not stored, only its norm (for closing column j), and the closing
+// follows.
+template <typename K>
+int cgs_step(HHEngine<K>& E, int j, bool last, double rtol_rel) {
+  TimedRegion tr("cgs_step");
+  const int nc = j + 1;
+  K* U = E.U.template as<K>();
+  K* w = E.w.template as<K>();
+  double* small = E.small.template as<double>();
+  double* part = E.partial.template as<double>();
+  const double* misc = small + E.lay.misc();
+  if (nc <= 8)
+    launch_mdot_rows<K, 8, false>(E, U, nc, w, 0, part, misc);
+  else
+    launch_mdot_rows<K, 16, false>(E, U, nc, w, 0, part, misc);
+  cgs_small_kernel<<<1, 256, 0, stream()>>>(E.nb, E.pstride, part, j, small, E.lay, 1, rtol_rel);
+  LACE_NC_DISPATCH(nc, (cgs_update_kernel<K, NCB><<<E.nb, kBS, 0, stream()>>>(
+                           E.n, E.ld, U, nc, w, last ? 0 : 1, E.rpb, small, E.lay, part, E.pstride)));
+  if (last) cgs_small_kernel<<<1, 256, 0, stream()>>>(E.nb, E.pstride, part, E.m, small, E.lay, 1, rtol_rel);
+  LACE_HIP(hipGetLastError());
+  return DEV_SUCCESS;
+}
+
+template int cgs_start<float>(HHEngine<float>&, const double*);
+template int cgs_start<double>(HHEngine<double>&, const double*);
+template int cgs_step<float>(HHEngine<float>&, int, bool, double);
+template int cgs_step<double>(HHEngine<double>&, int, bool, double);
+
 // Helpers used by the solver for norms and casts.
 int norm2_sq_partials(int n, int rpb, const double* x, double* partial, int pstride, int nb) {
   mdot_kernel<double, 8, true><<<nb, kBS, 0, stream()>>>(n, n, x, 0, x, 0, rpb, partial, pstride,

@@ -59,6 +59,10 @@ int hh_step(HHEngine<K>& E, int j, bool write_q);
 template <typename K, typename XT>
 int hh_finish(HHEngine<K>& E, XT* x);
 template <typename K>
+int cgs_start(HHEngine<K>& E, const double* r);
+template <typename K>
+int cgs_step(HHEngine<K>& E, int j, bool last, double rtol_rel);
+template <typename K>
 int read_misc(HHEngine<K>& E);
 int norm2_sq_partials(int n, int rpb, const double* x, double* partial, int pstride, int nb);
 int sum_slot(int nb, int pstride, const double* partial, double* out);
@@ -139,6 +143,7 @@ struct PrecInner : Precond {
   bool z_is_float() const override { return sizeof(K) == 4; }
   int apply_d(const double* q, void* z) override {
     TimedRegion tr("inner_gmres");
+    if (get_option("inner_ortho", 1) == 1) return apply_cgs(q, static_cast<K*>(z));
     if constexpr (sizeof(K) == 4) LACE_TRY(cast_d2f(n, q, E.r.template as<float>()));
     else LACE_HIP(hipMemcpyAsync(E.r.p, q, sizeof(double) * n, hipMemcpyDeviceToDevice, stream()));
     LACE_HIP(hipMemsetAsync(z, 0, sizeof(K) * n, stream()));
@@ -154,6 +159,27 @@ struct PrecInner : Precond {
       LACE_TRY(hh_step<K>(E, j, j < iters - 1));
     }
     return hh_finish<K, K>(E, static_cast<K*>(z));
+  }
+  // The same fixed inner GMRES with classical Gram-Schmidt on an
+  // unnormalised basis (option "inner_ortho" 1, the default; 0 takes the
+  // Householder engine): the preconditioner reads the basis vector W_j as
+  // stored.
+  int apply_cgs(const double* q, K* z) {
+    LACE_HIP(hipMemsetAsync(z, 0, sizeof(K) * n, stream()));
+    LACE_TRY(cgs_start<K>(E, q));
+    K* W = E.U.template as<K>();
+    for (int j = 0; j < iters; j++) {
+      const K* wj = W + (size_t)j * E.ld;
+      if constexpr (sizeof(K) == 4) {
+        LACE_TRY(M->apply_f(wj, E.zcol(j)));
+        LACE_TRY(Aop->apply_ff(static_cast<const float*>(E.zcol(j)), E.w.template as<float>()));
+      } else {
+        LACE_TRY(M->apply_d(wj, E.zcol(j)));
+        LACE_TRY(Aop->apply_dd(static_cast<const double*>(E.zcol(j)), E.w.template as<double>()));
+      }
+      LACE_TRY(cgs_step<K>(E, j, j == iters - 1, rtol));
+    }
+    return hh_finish<K, K>(E, z);
   }
 };
 
