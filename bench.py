@@ -660,7 +660,7 @@ def main() -> int:
                          "solves, persistent and level-ordered kernels")
     ap.add_argument("--set", action="append", default=[], metavar="NAME=VALUE",
                     help="library kernel-shape option (lace_set_option) applied before the run; "
-                         "repeatable; every choice gives bit-identical results")
+                         "repeatable; every choice but inner_ortho gives bit-identical results")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: --gpus {args.gpus} is not supported; this implementation runs on one "

@@ -684,6 +684,12 @@ int lace_block_panel_qr(int n, int r0, const double *W, double *Y, double *Tb, d
  * as fit): its resident workgroups per CU.  "trsv_exact_poll_sleep" (0..64,
  * default 1): its sleeps of 64 cycles between two polls of the solution
  * vector.
+ * "inner_ortho" (0 or 1, default 1): the inner GMRES of a nested solve
+ * (inner_iters > 0) orthogonalises by classical Gram-Schmidt on an
+ * unnormalised basis (1: two vector passes and one small launch per step)
+ * or by Householder reflectors (0: the outer engine's).  The two are
+ * different orthogonalisations of the same Krylov space: results agree to
+ * rounding, not bit for bit.
  * "timing_sample" (>= 1, default 1): the timing regions below record events
  * around the first four entries of each region after a reset and every
  * k-th entry after that only. */
