@@ -707,7 +707,7 @@ __global__ __launch_bounds__(kBBS) void bilu_sym_sweep_wave_kernel(
     int np, const int4* __restrict__ meta, const int* __restrict__ pend, const int4* __restrict__ pairs,
     const TC* __restrict__ AL, const TC* __restrict__ AU, const TS* __restrict__ Lin,
     const TS* __restrict__ Uin, TS* __restrict__ Lout, TS* __restrict__ Uout,
-    const float* __restrict__ rs, double* __restrict__ partial) {
+    const float* __restrict__ rs, double* __restrict__ partial, const uint8_t* __restrict__ whalo) {
   constexpr int LB = BS * BS, PPW = 64 / BS, CE = PPW * LB, NWV = kBBS / 64;
   constexpr int HE = kHalo * LB;  // halo elements
   constexpr int HS = (HE * (int)sizeof(TS) + 15) / 16 * 16 / (int)sizeof(TS);  // halo slots (16-byte multiple)
@@ -752,39 +752,44 @@ __global__ __launch_bounds__(kBBS) void bilu_sym_sweep_wave_kernel(
   rb.load(AU + off, lane);
   // the halo's old L and U blocks, element by element (the halo does not
   // start on a word boundary); the first run has none
-  // (loaded unconditionally from clamped indices, so no branch splits the
-  // loads; what is not halo is never written to LDS)
+  // (clamped indices, every lane loading; only the runs whose products
+  // reach before the run -- a set-up flag per run, the same in every lane --
+  // load one)
   constexpr int HR = (HE + 63) / 64;
+  const bool halo = __builtin_amdgcn_readfirstlane(whalo[pw / PPW]) != 0;
   const size_t hoff = pw > 0 ? off - HE : off;
   TS hl[HR], hu[HR];
+  float hs = 1.0f;  // the scales of U's rows at the halo positions (lanes h * BS + row)
+  if (halo) {
 #pragma unroll
-  for (int k = 0; k < HR; k++) {
-    const int e = min(lane + 64 * k, HE - 1);
-    hl[k] = Lin[hoff + e];
-    hu[k] = Uin[hoff + e];
-  }
-  // the scales of U's rows at the halo positions (lanes h * BS + row)
-  float hs = 1.0f;
-  if constexpr (is16<TS>()) {
-    const int h = min(lane, kHalo * BS - 1);
-    hs = rs[meta[max(pw - kHalo, 0) + h / BS].y * BS + h % BS];
+    for (int k = 0; k < HR; k++) {
+      const int e = min(lane + 64 * k, HE - 1);
+      hl[k] = Lin[hoff + e];
+      hu[k] = Uin[hoff + e];
+    }
+    if constexpr (is16<TS>()) {
+      const int h = min(lane, kHalo * BS - 1);
+      hs = rs[meta[max(pw - kHalo, 0) + h / BS].y * BS + h % BS];
+    }
   }
   const int row0 = lp * LB + r * BS;
   TC* jb = shJ + wv * CE + lp * LB;  // this group's U_JJ, later inv(U_JJ)
   ra.to_lds(sA, lane);
   rl.to_lds(sL, lane);
   ru.to_lds(sU, lane);
+  if (halo) {
 #pragma unroll
-  for (int k = 0; k < HR; k++) {
-    const int e = lane + 64 * k;
-    if (pw > 0 && e < HE) {
-      sL[e - HE] = hl[k];
-      sU[e - HE] = hu[k];
+    for (int k = 0; k < HR; k++) {
+      const int e = lane + 64 * k;
+      if (pw > 0 && e < HE) {
+        sL[e - HE] = hl[k];
+        sU[e - HE] = hu[k];
+      }
     }
   }
   if (is16<TS>()) {
     if (lane_on) sS[lp * BS + r] = sJs;
-    if (lane < kHalo * BS) sS[lane - kHalo * BS] = hs;
+    if (halo && lane < kHalo * BS) sS[lane - kHalo * BS] = hs;
   }
   if (lane_on) {
 #pragma unroll
@@ -1260,19 +1265,24 @@ int BILU::create(const data_d_matrix* A, lace_prec_t storage) {
     // whether the wave-run sweep finds every product's operands in the
     // staged run of its position or the halo before it, with at most three
     // products per position
+    // and, per run, whether any product reaches before it (the runs that
+    // load the halo)
     wave_local = true;
+    std::vector<uint8_t> wh(npad / (64 / bs), 0);
     {
       const int ppw = 64 / bs;
-      for (int q = 0; q < nnzL && wave_local; q++) {
+      for (int q = 0; q < nnzL; q++) {
         const int w0 = q / ppw * ppw;
         if (pe[q] - mt[(size_t)q * 4 + 3] > 3) wave_local = false;
         for (int k = mt[(size_t)q * 4 + 3]; k < pe[q]; k++)
           for (int e = 0; e < 2; e++) {
             const int x = pr[(size_t)k * 4 + e];
             if (x < w0 - kHalo || x >= w0 + ppw) wave_local = false;
+            if (x < w0) wh[w0 / ppw] = 1;
           }
       }
     }
+    LACE_TRY(upload(swhalo, wh.data(), wh.size()));
     if (pr.empty()) pr.assign(4, 0);
     LACE_TRY(upload_ints(smeta, mt));
     LACE_TRY(upload_ints(spend, pe));
@@ -1418,7 +1428,8 @@ static int sweep_bs(BILU* B, bool with_step) {
         kern<<<nb, kBBS, 0, stream()>>>(B->nnzL, B->smeta.as<int4>(), B->spend.as<int>(),
                                          B->spairs.as<int4>(), B->AL.as<TC>(), B->AU.as<TC>(),
                                          B->Lv[c].as<TS>(), B->Uv[c].as<TS>(), B->Lv[o].as<TS>(),
-                                         B->Uv[o].as<TS>(), B->rs.as<float>(), B->partial.as<double>());
+                                         B->Uv[o].as<TS>(), B->rs.as<float>(), B->partial.as<double>(),
+                                         B->swhalo.as<uint8_t>());
       };
       if constexpr (can_wave) {
         if (B->wave_local) {

@@ -65,6 +65,7 @@ struct BILU {
   DevBuf AL, AU;         // A_IJ and A_JI at each L position (symmetric kernel)
   DevBuf smeta, spend, spairs;  // symmetric kernel's per-position tables
   bool wave_local = false;      // every product operand within a wave run or its halo
+  DevBuf swhalo;                // per wave run: 1 if a product reaches before the run
   DevBuf AV;             // A's blocks in its own order (general kernel)
   DevBuf lfrom, ufrom;   // A block feeding each L / U entry (-1: none)
   DevBuf Lv[2], Uv[2];
