@@ -461,3 +461,42 @@ def test_fp64_nested_inner_solver(gpu, orc):
             with pytest.raises(gpu.LaceError):
                 S.solve_block(np.zeros((A.n_rows, 32)))
     assert res[1][0] <= res[0][0] and res[1][1] == 8 * res[1][0]
+
+
+@pytest.mark.parametrize("inner_fp64", [0, 1])
+def test_inner_orthogonalisations_agree(gpu, orc, inner_fp64):
+    """The inner GMRES by classical Gram-Schmidt on an unnormalised basis
+    (default) and by Householder reflectors: the same Krylov space, so one
+    inner solve from the same vector gives the same correction to rounding
+    (checked through the nested solver's first outer step: x after one
+    outer iteration), and both nested solves reach 1e-10 on the host
+    residual in the same number of outer steps (give or take one)."""
+    A = orc.convdiff3d(18, 16, 14)
+    b = gpu.gen_normal(5, A.n_rows)
+    out = {}
+    try:
+        for ortho in (1, 0):
+            gpu.check(gpu.lib().lace_set_option(b"inner_ortho", ortho))
+            o = gpu.default_opts()
+            o.search_max, o.restart_max, o.rtol, o.parilu_sweeps = 10, 100, 1e-10, 3
+            o.trsv, o.jacobi_sweeps, o.inner_iters, o.inner_fp64 = gpu.TRSV_JACOBI, 5, 10, inner_fp64
+            o.prec_precond = gpu.LACE_F64 if inner_fp64 else gpu.LACE_F32
+            S = gpu.Solver(gpu.HostCSR.from_csr(A), o)
+            S.setup()
+            x, st = S.solve(b)
+            assert st.converged == 1
+            assert np.linalg.norm(b - A.scipy() @ x) <= 1.01e-10 * np.linalg.norm(b)
+            o1 = gpu.default_opts()
+            o1.search_max, o1.restart_max, o1.rtol, o1.parilu_sweeps = 1, 1, 1e-30, 3
+            o1.trsv, o1.jacobi_sweeps, o1.inner_iters, o1.inner_fp64 = gpu.TRSV_JACOBI, 5, 10, inner_fp64
+            o1.prec_precond = o.prec_precond
+            S1 = gpu.Solver(gpu.HostCSR.from_csr(A), o1)
+            S1.setup()
+            x1, _ = S1.solve(b)
+            out[ortho] = (st.iterations, x1)
+    finally:
+        gpu.check(gpu.lib().lace_set_option(b"inner_ortho", 1))
+    assert abs(out[1][0] - out[0][0]) <= 1
+    tol = 1e-9 if inner_fp64 else 2e-4
+    rel = np.linalg.norm(out[1][1] - out[0][1]) / np.linalg.norm(out[0][1])
+    assert rel <= tol, rel
