@@ -465,10 +465,10 @@ BLOCK_REGIONS = ["tsgemm_tn", "tsgemm_nn", "spmm32_f32x", "spmm32_f32a", "spmm32
 def bench_trsv(args):
     """Exact triangular solves (the level-scheduled ParILU apply, L then U)
     on the 128^3 convection-diffusion factors after 3 sweeps, FP32 and FP64:
-    the persistent natural-order kernel (default) and the level-ordered
-    sync-free kernel, each timed by HIP events on the library stream; the
-    level counts of L and U, the time per level, and whether both kernels
-    give the same bits."""
+    the persistent kernel with blocks taken box by box (default) and as runs
+    of consecutive rows, and the level-ordered sync-free kernel, each timed
+    by HIP events on the library stream; the level counts of L and U, the
+    time per level, and whether all kernels give the same bits."""
     import ctypes as C
     import lace_amd as L
     lib = L.lib()
@@ -490,9 +490,12 @@ def bench_trsv(args):
         z = L.DevVec(n, dt)
         res = {"levels_L": nl, "levels_U": nu}
         zs = {}
-        for mode, tag, regions in ((1, "persistent", ["trsv_exact_lower", "trsv_exact_upper"]),
-                                   (0, "syncfree", ["trsv_syncfree"])):
+        res["tiles_L"], res["tiles_U"] = P.tiles(0), P.tiles(1)
+        for mode, tiles, tag, regions in ((1, 1, "persistent", ["trsv_exact_lower", "trsv_exact_upper"]),
+                                          (1, 0, "persistent_rows", ["trsv_exact_lower", "trsv_exact_upper"]),
+                                          (0, 1, "syncfree", ["trsv_syncfree"])):
             L.check(lib.lace_set_option(b"trsv_exact", mode))
+            L.check(lib.lace_set_option(b"trsv_exact_tiles", tiles))
             apply = lambda: L.check(lib.lace_parilu_apply(P.h, L.TRSV_LEVEL, 0, 0, r.ptr, z.ptr))
             apply()
             L.check(lib.lace_device_synchronize())
@@ -513,11 +516,13 @@ def bench_trsv(args):
             res[tag] = {"ms_kernels": round(ms, 3), "ms_call": round(wall * 1e3, 3),
                         "us_per_level": round(ms * 1e3 / (nl + nu), 3), "wait_error": ev.value}
             ok = ok and ev.value == 0 and bool(np.isfinite(zs[tag]).all())
-        res["bit_identical"] = bool(np.array_equal(zs["persistent"], zs["syncfree"]))
+        res["bit_identical"] = bool(np.array_equal(zs["persistent"], zs["syncfree"]) and
+                                    np.array_equal(zs["persistent"], zs["persistent_rows"]))
         ok = ok and res["bit_identical"]
         out[name] = res
         del P
     L.check(lib.lace_set_option(b"trsv_exact", 1))
+    L.check(lib.lace_set_option(b"trsv_exact_tiles", 1))
     out["all_ok"] = ok
     return out
 

@@ -455,6 +455,11 @@ int lace_parilu_levels(const lace_parilu *P, int *nlev_L, int *nlev_U);
  * (upper = 1) for the Jacobi sweeps, as for lace_csr_forms (options
  * "jacobi_dia", "jacobi_ell"). */
 int lace_parilu_forms(const lace_parilu *P, int upper, int *ndiag, int *ell_width);
+/* Block order of the exact solves with L (upper = 0), U (1), U^T (2) or
+ * L^T (3): the box sides dims[0..2] where the rows are taken box by box
+ * (option "trsv_exact_tiles"), 0 0 0 where blocks are consecutive rows.
+ * A diagnostic; no counterpart in the reference. */
+int lace_parilu_tiles(const lace_parilu *P, int which, int *dims);
 /* Sweep kernel selection: 0 (default) runs the four-position streaming
  * symmetric kernel (5) when A's pattern is structurally symmetric, else the
  * general one;
@@ -675,15 +680,20 @@ int lace_block_panel_qr(int n, int r0, const double *W, double *Y, double *Tb, d
  * sweeps on the diagonal form read the 16-bit words (0: the FP32 copy of
  * the same values).
  * "trsv_exact" (0 or 1, default 1): the exact (level) triangular solves run
- * the persistent natural-order kernel (1: blocks of consecutive rows taken
- * in ticket order, dependencies inside a block resolved through LDS, across
+ * the persistent kernel (1: blocks of rows taken in a topological ticket
+ * order, dependencies inside a block resolved through LDS, across
  * blocks through the solution vector with a NaN sentinel per unsolved row)
  * or the level-ordered sync-free kernel (0: one thread per row).
  * "trsv_exact_rows" (256 or 512, default 512): rows per block of the
  * persistent kernel.  "trsv_exact_wgs_per_cu" (0..16, default 0 = as many
  * as fit): its resident workgroups per CU.  "trsv_exact_poll_sleep" (0..64,
  * default 1): its sleeps of 64 cycles between two polls of the solution
- * vector.
+ * vector.  "trsv_exact_tiles" (0 or 1, default 1): where the factor's rows
+ * are the points of a grid whose dependencies all step backwards along
+ * each axis (7- and 5-point stencils, recognised from the pattern alone),
+ * the persistent kernel's blocks are boxes of the grid (8 x 8 x 8, or
+ * 32 x 16 in 2-D) taken in order of their diagonal (0: blocks of
+ * consecutive rows).  Either way the result is bit-identical.
  * "inner_ortho" (0 or 1, default 1): the inner GMRES of a nested solve
  * (inner_iters > 0) orthogonalises by classical Gram-Schmidt on an
  * unnormalised basis (1: two vector passes and one small launch per step)

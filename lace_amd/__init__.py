@@ -179,6 +179,7 @@ SIGNATURES = {
     "lace_parilu_apply": (_i, [_vp, _i, _i, _i, _vp, _vp]),
     "lace_parilu_levels": (_i, [_vp, _P(_i), _P(_i)]),
     "lace_parilu_forms": (_i, [_vp, _i, _P(_i), _P(_i)]),
+    "lace_parilu_tiles": (_i, [_vp, _i, _P(_i)]),
     "lace_parilu_set_variant": (_i, [_vp, _i]),
     "lace_syncfree_error": (_i, [_vp, _P(_i)]),
     "lace_bcsr_create": (_i, [_DM, _i, _P(_vp)]),
@@ -470,6 +471,13 @@ class ParILU:
         d = C.c_int(); w = C.c_int()
         check(lib().lace_parilu_forms(self.h, int(upper), C.byref(d), C.byref(w)), "forms")
         return d.value, w.value
+
+    def tiles(self, which: int):
+        """Box sides of the exact solve's block order for L (0), U (1),
+        U^T (2), L^T (3); (0, 0, 0) for blocks of consecutive rows."""
+        d = (C.c_int * 3)()
+        check(lib().lace_parilu_tiles(self.h, int(which), d), "tiles")
+        return tuple(d)
 
     def apply(self, r: np.ndarray, kind=TRSV_LEVEL, sweeps=5, trans=0):
         dt = np.float64 if self.prec == LACE_F64 else np.float32

@@ -1201,6 +1201,15 @@ int lace_parilu_forms(const lace_parilu* Ph, int upper, int* ndiag, int* ell_wid
   return DEV_SUCCESS;
 }
 
+int lace_parilu_tiles(const lace_parilu* Ph, int which, int* dims) {
+  LACE_REQUIRE(Ph && dims, DEV_ERR_INVALID_PTR, "lace_parilu_tiles: null argument");
+  LACE_REQUIRE(which >= 0 && which <= 3, DEV_ERR_ILLEGAL_VALUE, "lace_parilu_tiles: which = %d", which);
+  auto* P = reinterpret_cast<const ParILU*>(Ph);
+  const TriPlan* T[4] = {&P->lower, &P->upper, &P->lowerT, &P->upperT};
+  for (int d = 0; d < 3; d++) dims[d] = T[which]->tile_dims[d];
+  return DEV_SUCCESS;
+}
+
 // control/norms.cpp:301-405: res = ||A - LU||_F over LU's pattern,
 // nonlinres = the same restricted to A's pattern.  Host evaluation (a
 // reporting routine called once per factorization).  L: CSRL with unit

@@ -72,7 +72,29 @@ struct TriPlan {
   // sweep vectors, and the next launch's ticket and flag bases.
   mutable DevBuf pipe_flags, pipe_ctr, pipe_buf;
   mutable unsigned pipe_tickets = 0, pipe_epoch = 0;
+  // Exact solves in box order.  When the rows are the points of an
+  // nx x ny x nz grid (row = x + nx (y + ny z)) and every off-diagonal entry
+  // lies at non-negative coordinate steps from its row (towards lower rows
+  // for a lower factor, higher ones for an upper factor: the factors of a
+  // 7- or 5-point stencil), the persistent exact solve runs on the rows
+  // renumbered box by box -- boxes of tile_dims rows per side taken in
+  // order of their x+y+z, rows inside a box in natural order.  That order
+  // is a topological one, so the permuted factor stays triangular, and a
+  // block of the persistent solve is one box: its in-block dependency
+  // chains are as long as the box's diagonal instead of a grid line, and
+  // the blocks' wavefront crosses the grid diagonally.  tperm[p] is the row
+  // at position p; trow / tcol the CSR in that order (columns renumbered,
+  // each row's entries in their original order, so every row sum is the
+  // same); tmap[k] is entry k's position in the original CSR.  tval holds
+  // the values gathered from the array tval_src (cleared by refresh_strict,
+  // which every change of the factor values passes through); tsol is the
+  // solution in box order.
+  int tile_dims[3] = {0, 0, 0};  // box sides; 0: no box order
+  DevBuf tperm, trow, tcol, tmap;
+  mutable DevBuf tval, tsol;
+  mutable const void* tval_src = nullptr;
   int build(int n, const int* row, const int* col, bool upper);
+  int build_tiles(const int* row, const int* col);
   int refresh_strict(const void* val, size_t elem_size);
   // 16-bit diagonal form (fmt BF16 or F16) from the current FP32 forms,
   // whose values are already 16-bit values times the row scales rs.

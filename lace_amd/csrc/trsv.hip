@@ -82,6 +82,99 @@ int TriPlan::build(int n_, const int* row_, const int* col_, bool upper_) {
   // diagonal form of the same entries
   LACE_TRY(dia.build(n, row_, col_, [&](int i, int k) { return k != dp[i]; }));
   strict_src = nullptr;
+  return build_tiles(row_, col_);
+}
+
+// Grid dimensions under which every off-diagonal entry of the triangular
+// pattern (row, col) lies at non-negative coordinate steps from its row
+// (towards lower rows for a lower factor, higher ones for an upper one),
+// with row = x + nx (y + ny z).  The candidates come from the distinct
+// distances |i - j| of the off-diagonal entries: {1, a, b} with a | b and
+// b | n gives (a, b / a, n / b); {1, a} with a | n gives (a, n / a, 1).
+static bool tri_grid(int n, const int* row, const int* col, bool upper, long dims[3]) {
+  std::vector<long> offs;
+  for (int i = 0; i < n; i++)
+    for (int k = row[i]; k < row[i + 1]; k++) {
+      const long d = upper ? (long)col[k] - i : (long)i - col[k];
+      if (d == 0) continue;
+      if (d < 0) return false;
+      if (std::find(offs.begin(), offs.end(), d) == offs.end()) {
+        if (offs.size() == 3) return false;
+        offs.push_back(d);
+      }
+    }
+  std::sort(offs.begin(), offs.end());
+  if (offs.size() < 2 || offs[0] != 1) return false;
+  long nx, ny, nz;
+  if (offs.size() == 3) {
+    if (offs[2] % offs[1] != 0 || n % offs[2] != 0) return false;
+    nx = offs[1];
+    ny = offs[2] / offs[1];
+    nz = n / offs[2];
+  } else {
+    if (n % offs[1] != 0) return false;
+    nx = offs[1];
+    ny = n / offs[1];
+    nz = 1;
+  }
+  for (int i = 0; i < n; i++)
+    for (int k = row[i]; k < row[i + 1]; k++) {
+      const long a = upper ? (long)col[k] : i, b = upper ? (long)i : col[k];  // a >= b entrywise
+      if (a % nx < b % nx || (a / nx) % ny < (b / nx) % ny || a / (nx * ny) < b / (nx * ny)) return false;
+    }
+  dims[0] = nx;
+  dims[1] = ny;
+  dims[2] = nz;
+  return true;
+}
+
+int TriPlan::build_tiles(const int* row_, const int* col_) {
+  tile_dims[0] = tile_dims[1] = tile_dims[2] = 0;
+  tval_src = nullptr;
+  long g[3];
+  if (n < 2 || !tri_grid(n, row_, col_, upper, g)) return DEV_SUCCESS;
+  // 512 rows per box: 8 x 8 x 8 in 3-D, 32 x 16 in 2-D (sides clipped to
+  // the grid)
+  const long want[3] = {g[2] > 1 ? 8 : 32, g[2] > 1 ? 8 : 16, 8};
+  long b[3], nbx[3];
+  for (int d = 0; d < 3; d++) {
+    b[d] = std::min(want[d], g[d]);
+    nbx[d] = (g[d] + b[d] - 1) / b[d];
+  }
+  std::vector<int> pm;
+  pm.reserve(n);
+  for (long s = 0; s <= nbx[0] + nbx[1] + nbx[2] - 3; s++)
+    for (long Z = 0; Z < nbx[2]; Z++)
+      for (long Y = 0; Y < nbx[1]; Y++) {
+        const long X = s - Y - Z;
+        if (X < 0 || X >= nbx[0]) continue;
+        for (long z = Z * b[2]; z < std::min(g[2], (Z + 1) * b[2]); z++)
+          for (long y = Y * b[1]; y < std::min(g[1], (Y + 1) * b[1]); y++)
+            for (long x = X * b[0]; x < std::min(g[0], (X + 1) * b[0]); x++)
+              pm.push_back((int)(x + g[0] * (y + g[1] * z)));
+      }
+  if ((int)pm.size() != n) return DEV_SUCCESS;
+  std::vector<int> inv(n);
+  for (int p = 0; p < n; p++) inv[pm[p]] = p;
+  std::vector<int> tr(n + 1), tc(std::max(nnz, 1)), tm(std::max(nnz, 1));
+  tr[0] = 0;
+  for (int p = 0; p < n; p++) {
+    const int i = pm[p];
+    int q = tr[p];
+    for (int k = row_[i]; k < row_[i + 1]; k++, q++) {
+      tc[q] = inv[col_[k]];
+      tm[q] = k;
+      // the renumbered factor must stay triangular (true for the order
+      // above; checked so that a mistake cannot stall the solve)
+      if (upper ? tc[q] < p : tc[q] > p) return DEV_SUCCESS;
+    }
+    tr[p + 1] = q;
+  }
+  LACE_TRY(upload(tperm, pm.data(), sizeof(int) * n));
+  LACE_TRY(upload(trow, tr.data(), sizeof(int) * (n + 1)));
+  LACE_TRY(upload(tcol, tc.data(), sizeof(int) * tc.size()));
+  LACE_TRY(upload(tmap, tm.data(), sizeof(int) * tm.size()));
+  for (int d = 0; d < 3; d++) tile_dims[d] = (int)b[d];
   return DEV_SUCCESS;
 }
 
@@ -128,6 +221,7 @@ int TriPlan::refresh_strict(const void* val, size_t es) {
   }
   LACE_TRY(dia.gather(val, es));
   strict_src = val;
+  tval_src = nullptr;
   return DEV_SUCCESS;
 }
 
@@ -292,13 +386,18 @@ __global__ __launch_bounds__(256) void tri_syncfree_kernel(int nslots, const int
 // The row sum runs over the row's entries in CSR order and then divides, as
 // the level-scheduled kernel does: bit-identical to substitution.
 // y must not alias r; y is filled with the sentinel and *ticket zeroed by
-// tri_persist_init_kernel first.
+// tri_persist_init_kernel first.  With rperm given, the rows are a
+// renumbering of the caller's (row i is the caller's row rperm[i]): the
+// right-hand side is read as r[rperm[i]] and each solved value is also
+// stored to out[rperm[i]].
 template <typename T, bool UPPER, int BS, int R>
 __global__ __launch_bounds__(2 * BS) void tri_persist_kernel(int n, int nblk, const int* __restrict__ row,
                                                              const int* __restrict__ col,
                                                              const T* __restrict__ val,
                                                              const T* __restrict__ r, T* y, unsigned* ticket,
-                                                             int* err, int spin_limit, int poll_sleep) {
+                                                             int* err, int spin_limit, int poll_sleep,
+                                                             const int* __restrict__ rperm,
+                                                             T* __restrict__ out) {
 #pragma clang fp contract(off)
   using UB = typename Sentinel<T>::U;
   constexpr UB SENT = Sentinel<T>::bits;
@@ -385,7 +484,8 @@ __global__ __launch_bounds__(2 * BS) void tri_persist_kernel(int n, int nblk, co
         li[q] = q < len ? ((c >= lo && c < hi) ? c - lo : BS + q * BS + t) : kOne;
         vq[q] = q < len ? val[k0 + q] : T(0);
       }
-      const T ri = active ? r[i] : T(0);
+      const int oi = active ? (rperm ? rperm[i] : i) : 0;
+      const T ri = active ? r[oi] : T(0);
       const T di = active ? val[kd] : T(1);
       const int head = len < R ? len : R;  // slots held in registers
       T s = T(0);
@@ -431,6 +531,7 @@ __global__ __launch_bounds__(2 * BS) void tri_persist_kernel(int n, int nblk, co
             const UB ub = __builtin_bit_cast(UB, yi);
             __hip_atomic_store(&lv[t], ub, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
             __hip_atomic_store(yb + i, ub, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (rperm) out[oi] = yi;
             done = true;
             moved = true;
           } else if (++spins > spin_limit) {
@@ -1224,7 +1325,8 @@ static int g_persist_spin_limit = 1 << 22;
 
 template <typename T, bool UPPER, int BS>
 static void persist_launch(int n, const int* row, const int* col, const T* val, const T* r, T* y,
-                           unsigned* ticket, int* err, int wgs_per_cu, int poll_sleep) {
+                           unsigned* ticket, int* err, int wgs_per_cu, int poll_sleep, const int* rperm,
+                           T* out) {
   static int occ = -1;
   if (occ < 0) {
     int o = 0;
@@ -1238,25 +1340,33 @@ static void persist_launch(int n, const int* row, const int* col, const T* val, 
   const int per = wgs_per_cu > 0 ? std::min(wgs_per_cu, occ) : occ;
   const int grid = (int)std::min<long>((long)per * std::max(1, num_cus()), nblk);
   tri_persist_kernel<T, UPPER, BS, kPersistR><<<grid, 2 * BS, 0, stream()>>>(
-      n, nblk, row, col, val, r, y, ticket, err, g_persist_spin_limit, poll_sleep);
+      n, nblk, row, col, val, r, y, ticket, err, g_persist_spin_limit, poll_sleep, rperm, out);
 }
 
 template <typename T>
-int tri_solve_persist_raw(int n, bool upper, const int* row, const int* col, const T* val, const T* r, T* y,
-                          unsigned* ticket, int* err, bool init, int wgs_per_cu, int poll_sleep, int rows) {
+static int persist_dispatch(int n, bool upper, const int* row, const int* col, const T* val, const T* r, T* y,
+                            unsigned* ticket, int* err, bool init, int wgs_per_cu, int poll_sleep, int rows,
+                            const int* rperm, T* out) {
   if (n <= 0) return DEV_SUCCESS;
   LACE_REQUIRE((const void*)r != (const void*)y, DEV_ERR_ILLEGAL_VALUE, "exact solve needs distinct r and y");
   LACE_REQUIRE(rows == 256 || rows == 512, DEV_ERR_ILLEGAL_VALUE, "exact solve: %d rows per block", rows);
   if (init) tri_persist_init_kernel<T><<<ceil_div(n, 256), 256, 0, stream()>>>(n, y, ticket);
   if (rows == 512) {
-    if (upper) persist_launch<T, true, 512>(n, row, col, val, r, y, ticket, err, wgs_per_cu, poll_sleep);
-    else persist_launch<T, false, 512>(n, row, col, val, r, y, ticket, err, wgs_per_cu, poll_sleep);
+    if (upper) persist_launch<T, true, 512>(n, row, col, val, r, y, ticket, err, wgs_per_cu, poll_sleep, rperm, out);
+    else persist_launch<T, false, 512>(n, row, col, val, r, y, ticket, err, wgs_per_cu, poll_sleep, rperm, out);
   } else {
-    if (upper) persist_launch<T, true, 256>(n, row, col, val, r, y, ticket, err, wgs_per_cu, poll_sleep);
-    else persist_launch<T, false, 256>(n, row, col, val, r, y, ticket, err, wgs_per_cu, poll_sleep);
+    if (upper) persist_launch<T, true, 256>(n, row, col, val, r, y, ticket, err, wgs_per_cu, poll_sleep, rperm, out);
+    else persist_launch<T, false, 256>(n, row, col, val, r, y, ticket, err, wgs_per_cu, poll_sleep, rperm, out);
   }
   LACE_HIP(hipGetLastError());
   return DEV_SUCCESS;
+}
+
+template <typename T>
+int tri_solve_persist_raw(int n, bool upper, const int* row, const int* col, const T* val, const T* r, T* y,
+                          unsigned* ticket, int* err, bool init, int wgs_per_cu, int poll_sleep, int rows) {
+  return persist_dispatch<T>(n, upper, row, col, val, r, y, ticket, err, init, wgs_per_cu, poll_sleep, rows,
+                             nullptr, nullptr);
 }
 
 // Exact solve with a plan: the persistent natural-order kernel (option
@@ -1265,11 +1375,29 @@ template <typename T>
 int tri_solve_exact(const TriPlan& P, const T* val, const T* r, T* y, unsigned* ticket, int* err) {
   if (get_option("trsv_exact", 1) == 0) return tri_solve_syncfree<T>(P, val, r, y, err);
   const double es = sizeof(T);
+  const int wpc = get_option("trsv_exact_wgs_per_cu", 0), sleep = get_option("trsv_exact_poll_sleep", 1),
+            rows = get_option("trsv_exact_rows", 512);
+  if (P.tile_dims[0] > 0 && P.n > 0 && get_option("trsv_exact_tiles", 1) != 0) {
+    // box order: values gathered once per change of the factor values
+    if (P.tval_src != (const void*)val) {
+      if (P.tval.bytes < sizeof(T) * std::max(P.nnz, 1)) LACE_TRY(P.tval.alloc(sizeof(T) * std::max(P.nnz, 1)));
+      if (P.nnz > 0) {
+        gather_kernel<T><<<ceil_div(P.nnz, 256), 256, 0, stream()>>>(P.nnz, P.tmap.as<int>(), val, P.tval.as<T>());
+        LACE_HIP(hipGetLastError());
+      }
+      P.tval_src = val;
+    }
+    if (P.tsol.bytes < sizeof(T) * P.n) LACE_TRY(P.tsol.alloc(sizeof(T) * P.n));
+    // + the permutation read and the scattered copy of the solution
+    const double bytes = (double)P.nnz * (es + 4.0) + 4.0 * (P.n + 1) + (4.0 + 4.0 * es) * P.n;
+    TimedRegion tr(P.upper ? "trsv_exact_upper" : "trsv_exact_lower", 0.0, bytes);
+    return persist_dispatch<T>(P.n, P.upper, P.trow.as<int>(), P.tcol.as<int>(), P.tval.as<T>(), r,
+                               P.tsol.as<T>(), ticket, err, true, wpc, sleep, rows, P.tperm.as<int>(), y);
+  }
   const double bytes = (double)P.nnz * (es + 4.0) + 4.0 * (P.n + 1) + 3.0 * es * P.n;
   TimedRegion tr(P.upper ? "trsv_exact_upper" : "trsv_exact_lower", 0.0, bytes);
   return tri_solve_persist_raw<T>(P.n, P.upper, P.row.as<int>(), P.col.as<int>(), val, r, y, ticket, err, true,
-                                  get_option("trsv_exact_wgs_per_cu", 0), get_option("trsv_exact_poll_sleep", 1),
-                                  get_option("trsv_exact_rows", 512));
+                                  wpc, sleep, rows);
 }
 
 template int tri_solve_persist_raw<double>(int, bool, const int*, const int*, const double*, const double*,
