@@ -687,8 +687,9 @@ int lace_block_panel_qr(int n, int r0, const double *W, double *Y, double *Tb, d
  * "trsv_exact_rows" (256 or 512, default 512): rows per block of the
  * persistent kernel.  "trsv_exact_wgs_per_cu" (0..16, default 0 = as many
  * as fit): its resident workgroups per CU.  "trsv_exact_poll_sleep" (0..64,
- * default 1): its sleeps of 64 cycles between two polls of the solution
- * vector.  "trsv_exact_tiles" (0 or 1, default 1): where the factor's rows
+ * default 0): its sleeps of 64 cycles between two polls of the solution
+ * vector.  "trsv_exact_idle" (0..16, default 8): the most sleeps of 64
+ * cycles a wave whose rows could not move takes before it polls again.  "trsv_exact_tiles" (0 or 1, default 1): where the factor's rows
  * are the points of a grid whose dependencies all step backwards along
  * each axis (7- and 5-point stencils, recognised from the pattern alone),
  * the persistent kernel's blocks are boxes of the grid (8 x 8 x 8, or

@@ -38,6 +38,7 @@ struct TriPlan {
   int nlev = 0;
   int nslots = 0;
   bool unit_diag = false;  // every diagonal value is exactly 1 (ParILU's L, L^T)
+  int maxoff = 0;          // most off-diagonal entries in one row
   DevBuf row, col;
   DevBuf perm, lptr;        // rows by level, level pointers (unpadded)
   DevBuf sched;             // padded schedule for the sync-free kernel
@@ -120,7 +121,7 @@ int tri_solve_syncfree(const TriPlan& P, const T* val, const T* r, T* y, int* er
 // 256).
 template <typename T>
 int tri_solve_persist_raw(int n, bool upper, const int* row, const int* col, const T* val, const T* r, T* y,
-                          unsigned* ticket, int* err, bool init = true, int wgs_per_cu = 0, int poll_sleep = 1,
+                          unsigned* ticket, int* err, bool init = true, int wgs_per_cu = 0, int poll_sleep = 0,
                           int rows = 512);
 // `sweeps` synchronous Jacobi sweeps from y = 0; tmp is a scratch vector.
 // epi_y / epi_d: the last sweep may also write epi_y = y / epi_d (the first

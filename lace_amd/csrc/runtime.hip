@@ -362,6 +362,8 @@ extern "C" int lace_set_option(const char* name, int value) {
   } else if (key == "trsv_exact_rows") {
     LACE_REQUIRE(value == 256 || value == 512, DEV_ERR_ILLEGAL_VALUE, "%s must be 256 or 512 (have %d)", name,
                  value);
+  } else if (key == "trsv_exact_idle") {
+    LACE_REQUIRE(value >= 0 && value <= 16, DEV_ERR_ILLEGAL_VALUE, "%s must be 0..16 (have %d)", name, value);
   } else if (key == "trsv_exact_poll_sleep") {
     LACE_REQUIRE(value >= 0 && value <= 64, DEV_ERR_ILLEGAL_VALUE, "%s must be 0..64 (have %d)", name, value);
   } else if (key == "timing_sample") {

@@ -67,6 +67,7 @@ int TriPlan::build(int n_, const int* row_, const int* col_, bool upper_) {
   // ELLPACK form for short rows
   int wmax = 0;
   for (int i = 0; i < n; i++) wmax = std::max(wmax, sr[i + 1] - sr[i]);
+  maxoff = wmax;
   ew = 0;
   if (n > 0 && wmax >= 1 && wmax <= 8 && (double)wmax * n <= 1.25 * snnz) {
     ew = wmax;
@@ -375,8 +376,8 @@ __global__ __launch_bounds__(256) void tri_syncfree_kernel(int nslots, const int
 //     order and divides.  It never waits on device memory, so a dependency
 //     chain inside the block moves at LDS speed.  Operands past the first R
 //     (rows longer than R) come straight from memory afterwards.  Waves
-//     with nothing to do back off (up to 4 x 64 cycles) to leave the SIMD to
-//     waves that can move.
+//     with nothing to do back off (up to idle_max x 64 cycles) to leave
+//     the SIMD to waves that can move.
 //   - A solved row goes to LDS for the block and to the solution vector
 //     (device-coherent store) for later blocks.
 // The earliest unfinished block always has every external operand solved
@@ -397,7 +398,7 @@ __global__ __launch_bounds__(2 * BS) void tri_persist_kernel(int n, int nblk, co
                                                              const T* __restrict__ r, T* y, unsigned* ticket,
                                                              int* err, int spin_limit, int poll_sleep,
                                                              const int* __restrict__ rperm,
-                                                             T* __restrict__ out) {
+                                                             T* __restrict__ out, int idle_max) {
 #pragma clang fp contract(off)
   using UB = typename Sentinel<T>::U;
   constexpr UB SENT = Sentinel<T>::bits;
@@ -543,7 +544,7 @@ __global__ __launch_bounds__(2 * BS) void tri_persist_kernel(int n, int nblk, co
         if (__any(moved)) {
           idle = 0;
         } else {
-          idle = idle < 4 ? idle + 1 : 4;
+          idle = idle < idle_max ? idle + 1 : idle_max;
           for (int z = 0; z < idle; z++) __builtin_amdgcn_s_sleep(1);
         }
       }
@@ -1317,20 +1318,21 @@ int tri_solve_syncfree(const TriPlan& P, const T* val, const T* r, T* y, int* er
 }
 
 // Register slots of the persistent exact solve (rows per block: 512 by
-// default, 256 on request).
+// default, 256 on request): 8, or 4 for factors whose rows have at most 4
+// off-diagonal entries (a shorter poll of LDS per round).
 constexpr int kPersistR = 8;
 // Poll rounds before a persistent-solve lane gives up (each round sleeps
 // about 64 cycles when nothing moved): far beyond any legitimate wait.
 static int g_persist_spin_limit = 1 << 22;
 
-template <typename T, bool UPPER, int BS>
+template <typename T, bool UPPER, int BS, int R>
 static void persist_launch(int n, const int* row, const int* col, const T* val, const T* r, T* y,
                            unsigned* ticket, int* err, int wgs_per_cu, int poll_sleep, const int* rperm,
                            T* out) {
   static int occ = -1;
   if (occ < 0) {
     int o = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&o, tri_persist_kernel<T, UPPER, BS, kPersistR>, 2 * BS, 0) !=
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&o, tri_persist_kernel<T, UPPER, BS, R>, 2 * BS, 0) !=
             hipSuccess ||
         o < 1)
       o = 1;
@@ -1339,25 +1341,39 @@ static void persist_launch(int n, const int* row, const int* col, const T* val, 
   const int nblk = ceil_div(n, BS);
   const int per = wgs_per_cu > 0 ? std::min(wgs_per_cu, occ) : occ;
   const int grid = (int)std::min<long>((long)per * std::max(1, num_cus()), nblk);
-  tri_persist_kernel<T, UPPER, BS, kPersistR><<<grid, 2 * BS, 0, stream()>>>(
-      n, nblk, row, col, val, r, y, ticket, err, g_persist_spin_limit, poll_sleep, rperm, out);
+  tri_persist_kernel<T, UPPER, BS, R><<<grid, 2 * BS, 0, stream()>>>(
+      n, nblk, row, col, val, r, y, ticket, err, g_persist_spin_limit, poll_sleep, rperm, out,
+      get_option("trsv_exact_idle", 8));
 }
 
 template <typename T>
 static int persist_dispatch(int n, bool upper, const int* row, const int* col, const T* val, const T* r, T* y,
                             unsigned* ticket, int* err, bool init, int wgs_per_cu, int poll_sleep, int rows,
-                            const int* rperm, T* out) {
+                            const int* rperm, T* out, int slots = kPersistR) {
   if (n <= 0) return DEV_SUCCESS;
   LACE_REQUIRE((const void*)r != (const void*)y, DEV_ERR_ILLEGAL_VALUE, "exact solve needs distinct r and y");
   LACE_REQUIRE(rows == 256 || rows == 512, DEV_ERR_ILLEGAL_VALUE, "exact solve: %d rows per block", rows);
   if (init) tri_persist_init_kernel<T><<<ceil_div(n, 256), 256, 0, stream()>>>(n, y, ticket);
-  if (rows == 512) {
-    if (upper) persist_launch<T, true, 512>(n, row, col, val, r, y, ticket, err, wgs_per_cu, poll_sleep, rperm, out);
-    else persist_launch<T, false, 512>(n, row, col, val, r, y, ticket, err, wgs_per_cu, poll_sleep, rperm, out);
+#define LACE_PERSIST(UP, BS, R) \
+  persist_launch<T, UP, BS, R>(n, row, col, val, r, y, ticket, err, wgs_per_cu, poll_sleep, rperm, out)
+  if (slots == 4) {
+    if (rows == 512) {
+      if (upper) LACE_PERSIST(true, 512, 4);
+      else LACE_PERSIST(false, 512, 4);
+    } else {
+      if (upper) LACE_PERSIST(true, 256, 4);
+      else LACE_PERSIST(false, 256, 4);
+    }
   } else {
-    if (upper) persist_launch<T, true, 256>(n, row, col, val, r, y, ticket, err, wgs_per_cu, poll_sleep, rperm, out);
-    else persist_launch<T, false, 256>(n, row, col, val, r, y, ticket, err, wgs_per_cu, poll_sleep, rperm, out);
+    if (rows == 512) {
+      if (upper) LACE_PERSIST(true, 512, kPersistR);
+      else LACE_PERSIST(false, 512, kPersistR);
+    } else {
+      if (upper) LACE_PERSIST(true, 256, kPersistR);
+      else LACE_PERSIST(false, 256, kPersistR);
+    }
   }
+#undef LACE_PERSIST
   LACE_HIP(hipGetLastError());
   return DEV_SUCCESS;
 }
@@ -1375,8 +1391,9 @@ template <typename T>
 int tri_solve_exact(const TriPlan& P, const T* val, const T* r, T* y, unsigned* ticket, int* err) {
   if (get_option("trsv_exact", 1) == 0) return tri_solve_syncfree<T>(P, val, r, y, err);
   const double es = sizeof(T);
-  const int wpc = get_option("trsv_exact_wgs_per_cu", 0), sleep = get_option("trsv_exact_poll_sleep", 1),
+  const int wpc = get_option("trsv_exact_wgs_per_cu", 0), sleep = get_option("trsv_exact_poll_sleep", 0),
             rows = get_option("trsv_exact_rows", 512);
+  const int slots = P.maxoff <= 4 ? 4 : kPersistR;
   if (P.tile_dims[0] > 0 && P.n > 0 && get_option("trsv_exact_tiles", 1) != 0) {
     // box order: values gathered once per change of the factor values
     if (P.tval_src != (const void*)val) {
@@ -1392,12 +1409,12 @@ int tri_solve_exact(const TriPlan& P, const T* val, const T* r, T* y, unsigned* 
     const double bytes = (double)P.nnz * (es + 4.0) + 4.0 * (P.n + 1) + (4.0 + 4.0 * es) * P.n;
     TimedRegion tr(P.upper ? "trsv_exact_upper" : "trsv_exact_lower", 0.0, bytes);
     return persist_dispatch<T>(P.n, P.upper, P.trow.as<int>(), P.tcol.as<int>(), P.tval.as<T>(), r,
-                               P.tsol.as<T>(), ticket, err, true, wpc, sleep, rows, P.tperm.as<int>(), y);
+                               P.tsol.as<T>(), ticket, err, true, wpc, sleep, rows, P.tperm.as<int>(), y, slots);
   }
   const double bytes = (double)P.nnz * (es + 4.0) + 4.0 * (P.n + 1) + 3.0 * es * P.n;
   TimedRegion tr(P.upper ? "trsv_exact_upper" : "trsv_exact_lower", 0.0, bytes);
-  return tri_solve_persist_raw<T>(P.n, P.upper, P.row.as<int>(), P.col.as<int>(), val, r, y, ticket, err, true,
-                                  wpc, sleep, rows);
+  return persist_dispatch<T>(P.n, P.upper, P.row.as<int>(), P.col.as<int>(), val, r, y, ticket, err, true, wpc,
+                             sleep, rows, nullptr, nullptr, slots);
 }
 
 template int tri_solve_persist_raw<double>(int, bool, const int*, const int*, const double*, const double*,
