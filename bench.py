@@ -53,6 +53,10 @@ FP64 = dict(search_max=50, trsv=1, jacobi_sweeps=5, prec_precond=0)
 FP64_NESTED = dict(search_max=10, inner_iters=10, inner_fp64=1, trsv=1, jacobi_sweeps=5, prec_precond=0)
 # the mixed path with BF16 factor storage (per-row power-of-two scales)
 MIXED_BF16 = dict(MIXED, prec_precond=2)
+# the reference's own method on the GPU: FP64 FGMRES(50), FP64 factors,
+# exact triangular solves
+FP64_EXACT = dict(FP64, trsv=0)
+EXACT_STEPS = 3
 
 
 def timed_solves(L, S, db, dx, zeros, k, flush=None):
@@ -231,8 +235,10 @@ def make_flush(L):
 def cache_rates(L, H, n, flush, reps=10):
     """Per-launch duration and rate of each hot kernel class run on its own:
     warm (reps launches back to back, data left in the caches by the
-    previous launch) and cold (each launch after a flush).  Bytes per call
-    are those the launches recorded for the storage form they read."""
+    previous launch) and cold (each launch after a flush), the median over
+    the reps calls (a call that meets a stall of the device's queue, as
+    after the release of large allocations, does not move it).  Bytes per
+    call are those the launches recorded for the storage form they read."""
     lib = L.lib()
     rng = np.random.default_rng(0)
     Ps = L.ParILU(H, L.LACE_F32)
@@ -267,29 +273,30 @@ def cache_rates(L, H, n, flush, reps=10):
         L.check(lib.lace_device_synchronize())
         per = {}
         for label in ("warm", "cold"):
-            L.check(lib.lace_timing_reset())
             L.check(lib.lace_timing_only(",".join(regions).encode()))
+            calls_us, nbytes = [], 0.0
             for _ in range(reps):
                 if label == "cold":
                     flush()
+                L.check(lib.lace_timing_reset())
                 L.check(lib.lace_timing_enable(1))
                 fn()
                 L.check(lib.lace_timing_enable(0))
                 # each call starts on an idle queue, so a launch's start
                 # event never waits behind the previous call's kernels
                 L.check(lib.lace_device_synchronize())
-            L.check(lib.lace_device_synchronize())
+                # one launch per region per call: the call's kernel time
+                # and bytes are the sums over its regions (an option can
+                # route a call to other regions: those record none)
+                timed = [rg for rg in regions if L.timing(rg)[1] > 0]
+                if timed:
+                    calls_us.append(sum(L.timing(rg)[0] for rg in timed) * 1e3)
+                    nbytes = sum(L.timing_work(rg)[1] for rg in timed)
             L.check(lib.lace_timing_only(None))
-            # one launch per region per call: the call's kernel time and
-            # bytes are the sums of the regions' per-launch averages (an
-            # option can route a call to other regions: those record none)
-            timed = [rg for rg in regions if L.timing(rg)[1] > 0]
-            if not timed:
+            if not calls_us:
                 per[f"{label}_us"] = None
-                nbytes = 0.0
                 continue
-            us = sum(L.timing(rg)[0] / L.timing(rg)[1] for rg in timed) * 1e3
-            nbytes = sum(L.timing_work(rg)[1] / L.timing(rg)[1] for rg in timed)
+            us = float(np.median(calls_us))
             per[f"{label}_us"] = round(us, 2)
             per[f"{label}_GBps"] = round(nbytes / (us * 1e-6) / 1e9, 1)
         per["bytes_per_call"] = round(nbytes)
@@ -759,6 +766,11 @@ def main() -> int:
     x_fp64n = dx.download()
     res_m16 = timed_solves(L, mixed16, db, dx, zeros, args.steps)
     x_m16 = dx.download()
+    fp64x = L.Solver(H, make_opts(L, **FP64_EXACT))
+    timed_solves(L, fp64x, db, dx, zeros, 1)
+    res_fp64x = timed_solves(L, fp64x, db, dx, zeros, EXACT_STEPS)
+    x_fp64x = dx.download()
+    del fp64x
     # the same mixed steps without events (the events' cost), and each
     # started with nothing of its data in the caches
     res_plain = timed_solves(L, mixed, db, dx, zeros, args.steps)
@@ -772,10 +784,11 @@ def main() -> int:
     t_fp64 = float(np.mean([t for t, _ in res_fp64]))
     t_fp64n = float(np.mean([t for t, _ in res_fp64n]))
     t_m16 = float(np.mean([t for t, _ in res_m16]))
+    t_fp64x = float(np.mean([t for t, _ in res_fp64x]))
     st_m, st_f = res_mixed[-1][1], res_fp64[-1][1]
-    st_fn, st_m16 = res_fp64n[-1][1], res_m16[-1][1]
+    st_fn, st_m16, st_fx = res_fp64n[-1][1], res_m16[-1][1], res_fp64x[-1][1]
     ok = all(s.converged == 1 and s.rel_true_residual <= 1e-8
-             for _, s in res_mixed + res_fp64 + res_plain + res_cold + res_fp64n + res_m16)
+             for _, s in res_mixed + res_fp64 + res_plain + res_cold + res_fp64n + res_m16 + res_fp64x)
     # FP64 oracle error of each solution: against an FP64 solve to 1e-12
     ref_o = make_opts(L, **FP64)
     ref_o.rtol = 1e-12
@@ -861,6 +874,7 @@ def main() -> int:
             "n": n, "nnz": nnz, "parilu_sweeps": 3,
             "mixed": "outer FP64 Householder FGMRES(10) x inner FP32 GMRES(10), FP32 ParILU, Jacobi-5",
             "fp64": "FP64 Householder FGMRES(50), FP64 ParILU, Jacobi-5",
+            "fp64_exact": "FP64 Householder FGMRES(50), FP64 ParILU, exact triangular solves",
         },
         "fp64_time_s": round(t_fp64, 5),
         "speedup_mixed_vs_fp64": round(t_fp64 / t_mixed, 3),
@@ -875,8 +889,14 @@ def main() -> int:
         "mixed_bf16_time_s": round(t_m16, 5),
         "mixed_bf16_outer_iterations": st_m16.iterations,
         "mixed_bf16_rel_true_residual": st_m16.rel_true_residual,
+        # the reference's method (FP64 FGMRES(50), FP64 factors, exact
+        # triangular solves) on the GPU, EXACT_STEPS solves
+        "fp64_exact_time_s": round(t_fp64x, 5),
+        "fp64_exact_iterations": st_fx.iterations,
+        "speedup_mixed_vs_fp64_exact": round(t_fp64x / t_mixed, 3),
         "err_vs_fp64_solution": {"mixed": err(x_mixed), "mixed_bf16": err(x_m16), "fp64": err(x_fp64),
-                                 "fp64_nested": err(x_fp64n), "reference": "FP64 FGMRES(50) to 1e-12"},
+                                 "fp64_nested": err(x_fp64n), "fp64_exact": err(x_fp64x),
+                                 "reference": "FP64 FGMRES(50) to 1e-12"},
         "mixed_outer_iterations": st_m.iterations,
         "mixed_inner_iterations": st_m.inner_iterations,
         "fp64_iterations": st_f.iterations,

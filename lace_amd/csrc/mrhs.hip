@@ -379,7 +379,7 @@ int block_precond(ParILU* P, lace_trsv_kind_t kind, int sweeps, int trans, const
   }
   LACE_REQUIRE(kind == LACE_TRSV_JACOBI, DEV_ERR_NOT_SUPPORTED,
                "block solves take Jacobi (or no) triangular solves");
-  LACE_TRY(P->refresh_csr());
+  LACE_TRY(P->prepare(kind, trans, true));
   const TriPlan& first = trans ? P->lowerT : P->lower;
   const TriPlan& second = trans ? P->upperT : P->upper;
   const T* v1 = trans ? P->uval[P->cur].as<T>() : P->lval[P->cur].as<T>();

@@ -47,8 +47,11 @@ struct TriPlan {
   // entries in row order (srow, scol), each one's position in the full CSR
   // (smap) and each row's diagonal position (dpos); sval / dval hold the
   // off-diagonal and diagonal values gathered from the value array
-  // strict_src by refresh_strict (the sweeps use them when called with
-  // that array).
+  // strict_src (the sweeps use them when called with that array).
+  // refresh_strict records strict_src and marks every form below stale;
+  // ensure_forms gathers the stale ones a solve is about to read, so the
+  // forms no solve reads (the transposed factors' in a forward solve, the
+  // ELLPACK and CSR copies where the diagonal form runs) cost nothing.
   int snnz = 0;
   DevBuf srow, scol, smap, dpos, sval, dval;
   const void* strict_src = nullptr;
@@ -94,9 +97,13 @@ struct TriPlan {
   DevBuf tperm, trow, tcol, tmap;
   mutable DevBuf tval, tsol;
   mutable const void* tval_src = nullptr;
+  static constexpr unsigned kFormStrict = 1, kFormDiag = 2, kFormEll = 4, kFormDia = 8, kFormAll = 15;
+  unsigned stale = 0;  // forms not yet gathered from strict_src
+  size_t src_es = 0;   // element size of strict_src
   int build(int n, const int* row, const int* col, bool upper);
   int build_tiles(const int* row, const int* col);
   int refresh_strict(const void* val, size_t elem_size);
+  int ensure_forms(unsigned forms);
   // 16-bit diagonal form (fmt BF16 or F16) from the current FP32 forms,
   // whose values are already 16-bit values times the row scales rs.
   int refresh_dia16(lace_prec_t fmt, const float* rs);
@@ -200,6 +207,12 @@ struct ParILU {
   int values_f64(double* lv, double* uv);
   int build_trsv();
   int refresh_csr();
+  // Everything a solve of this kind and direction reads (block: the
+  // 32-column solves), current for the present factor values: L^T's values
+  // (lval_t) on a first transposed solve, and the off-diagonal forms the
+  // Jacobi sweeps will read.
+  bool lval_t_dirty = true;
+  int prepare(lace_trsv_kind_t kind, int trans, bool block);
   // z = M^{-1} r (trans = 0) or M^{-T} r (trans = 1); r and z in the
   // factor precision (double for F64, float otherwise).
   int apply(lace_trsv_kind_t kind, int sweeps, int trans, const void* r, void* z);

@@ -197,21 +197,37 @@ __global__ void strict_gather_kernel(int snnz, int n, const int* __restrict__ sm
 int TriPlan::refresh_strict(const void* val, size_t es) {
   if (sval.bytes < es * std::max(snnz, 1)) LACE_TRY(sval.alloc(es * std::max(snnz, 1)));
   if (dval.bytes < es * std::max(n, 1)) LACE_TRY(dval.alloc(es * std::max(n, 1)));
-  const int m = std::max(snnz, n);
+  if (ew > 0 && eval.bytes < es * (size_t)ew * n) LACE_TRY(eval.alloc(es * (size_t)ew * n));
+  strict_src = val;
+  src_es = es;
+  stale = kFormAll;
+  tval_src = nullptr;
+  return DEV_SUCCESS;
+}
+
+int TriPlan::ensure_forms(unsigned forms) {
+  const unsigned todo = stale & forms;
+  if (!todo || !strict_src) return DEV_SUCCESS;
+  const size_t es = src_es;
+  const void* val = strict_src;
+  // off-diagonal CSR values and the diagonal (one launch: the diagonal
+  // alone where the CSR copy is not wanted)
+  const int sn = (todo & kFormStrict) ? snnz : 0;
+  const int dn = (todo & (kFormStrict | kFormDiag)) ? n : 0;
+  const int m = std::max(sn, dn);
   if (m > 0) {
     if (es == 8)
       strict_gather_kernel<double><<<ceil_div(m, 256), 256, 0, stream()>>>(
-          snnz, n, smap.as<int>(), dpos.as<int>(), static_cast<const double*>(val), sval.as<double>(),
+          sn, dn, smap.as<int>(), dpos.as<int>(), static_cast<const double*>(val), sval.as<double>(),
           dval.as<double>());
     else
       strict_gather_kernel<float><<<ceil_div(m, 256), 256, 0, stream()>>>(
-          snnz, n, smap.as<int>(), dpos.as<int>(), static_cast<const float*>(val), sval.as<float>(),
+          sn, dn, smap.as<int>(), dpos.as<int>(), static_cast<const float*>(val), sval.as<float>(),
           dval.as<float>());
     LACE_HIP(hipGetLastError());
   }
-  if (ew > 0) {
+  if ((todo & kFormEll) && ew > 0) {
     const size_t m2 = (size_t)ew * n;
-    if (eval.bytes < es * m2) LACE_TRY(eval.alloc(es * m2));
     if (es == 8)
       ell_gather_kernel<double><<<ceil_div((long)m2, 256), 256, 0, stream()>>>(
           m2, emap.as<int>(), static_cast<const double*>(val), eval.as<double>());
@@ -220,9 +236,9 @@ int TriPlan::refresh_strict(const void* val, size_t es) {
           m2, emap.as<int>(), static_cast<const float*>(val), eval.as<float>());
     LACE_HIP(hipGetLastError());
   }
-  LACE_TRY(dia.gather(val, es));
-  strict_src = val;
-  tval_src = nullptr;
+  if (todo & kFormDia) LACE_TRY(dia.gather(val, es));
+  stale &= ~todo;
+  if (dn > 0) stale &= ~kFormDiag;
   return DEV_SUCCESS;
 }
 
@@ -245,6 +261,7 @@ int TriPlan::refresh_dia16(lace_prec_t fmt, const float* rs) {
   dia16_fmt = LACE_F32;
   dia16_src = nullptr;
   if (dia.nd == 0 || !strict_src) return DEV_SUCCESS;
+  LACE_TRY(ensure_forms(kFormDia | kFormDiag));
   LACE_REQUIRE(fmt == LACE_BF16 || fmt == LACE_F16, DEV_ERR_ILLEGAL_VALUE, "16-bit DIA form: format %d", (int)fmt);
   const size_t m = (size_t)dia.nd * n;
   if (dia16.bytes < 2 * m) LACE_TRY(dia16.alloc(2 * m));
@@ -1698,19 +1715,16 @@ int ParILU::build_trsv() {
 
 int ParILU::refresh_csr() {
   if (!csr_dirty) return DEV_SUCCESS;
-  if (prec == LACE_F64) {
+  if (prec == LACE_F64)
     gather_kernel<double><<<ceil_div(nnzU, 256), 256, 0, stream()>>>(
         nnzU, u_csr_from_csc.as<int>(), uval[cur].as<double>(), uval_csr.as<double>());
-    gather_kernel<double><<<ceil_div(nnzL, 256), 256, 0, stream()>>>(
-        nnzL, l_t_from_csr.as<int>(), lval[cur].as<double>(), lval_t.as<double>());
-  } else {
+  else
     gather_kernel<float><<<ceil_div(nnzU, 256), 256, 0, stream()>>>(
         nnzU, u_csr_from_csc.as<int>(), uval[cur].as<float>(), uval_csr.as<float>());
-    gather_kernel<float><<<ceil_div(nnzL, 256), 256, 0, stream()>>>(
-        nnzL, l_t_from_csr.as<int>(), lval[cur].as<float>(), lval_t.as<float>());
-  }
   LACE_HIP(hipGetLastError());
-  // off-diagonal forms for the Jacobi sweeps, from the arrays the solves pass
+  lval_t_dirty = true;  // gathered by the first transposed solve
+  // off-diagonal forms for the Jacobi sweeps, from the arrays the solves
+  // pass (gathered on first use)
   const size_t es = elem_size();
   LACE_TRY(lower.refresh_strict(lval[cur].p, es));
   LACE_TRY(upper.refresh_strict(uval_csr.p, es));
@@ -1726,9 +1740,45 @@ int ParILU::refresh_csr() {
   return DEV_SUCCESS;
 }
 
+// The off-diagonal forms the Jacobi sweeps read on P under the present
+// options: the diagonal always (the first sweep from zero divides by it),
+// the diagonal form where there is one (the one-sweep, fused and pipelined
+// kernels), else the ELLPACK form, else the off-diagonal CSR; the 32-column
+// sweeps read the ELLPACK form or the full CSR.
+static unsigned jacobi_forms(const TriPlan& P, bool block) {
+  unsigned m = TriPlan::kFormDiag;
+  const bool ell = P.ew > 0 && get_option("jacobi_ell", 1) != 0;
+  if (block) return ell ? m | TriPlan::kFormEll : m;
+  if (P.dia.nd > 0) m |= TriPlan::kFormDia;
+  if (!(P.dia.nd > 0 && get_option("jacobi_dia", 1) != 0)) m |= ell ? TriPlan::kFormEll : TriPlan::kFormStrict;
+  return m;
+}
+
+int ParILU::prepare(lace_trsv_kind_t kind, int trans, bool block) {
+  LACE_TRY(refresh_csr());
+  if (kind == LACE_TRSV_NONE) return DEV_SUCCESS;
+  if (trans && lval_t_dirty) {
+    if (prec == LACE_F64)
+      gather_kernel<double><<<ceil_div(nnzL, 256), 256, 0, stream()>>>(
+          nnzL, l_t_from_csr.as<int>(), lval[cur].as<double>(), lval_t.as<double>());
+    else
+      gather_kernel<float><<<ceil_div(nnzL, 256), 256, 0, stream()>>>(
+          nnzL, l_t_from_csr.as<int>(), lval[cur].as<float>(), lval_t.as<float>());
+    LACE_HIP(hipGetLastError());
+    lval_t_dirty = false;
+  }
+  if (kind == LACE_TRSV_JACOBI) {
+    TriPlan& a = trans ? lowerT : lower;
+    TriPlan& b = trans ? upperT : upper;
+    LACE_TRY(a.ensure_forms(jacobi_forms(a, block)));
+    LACE_TRY(b.ensure_forms(jacobi_forms(b, block)));
+  }
+  return DEV_SUCCESS;
+}
+
 template <typename T>
 static int apply_t(ParILU* P, lace_trsv_kind_t kind, int sweeps, int trans, const T* r, T* z) {
-  LACE_TRY(P->refresh_csr());
+  LACE_TRY(P->prepare(kind, trans, false));
   T* tmp = P->work.as<T>();
   const TriPlan& first = trans ? P->lowerT : P->lower;
   const TriPlan& second = trans ? P->upperT : P->upper;
