@@ -304,20 +304,49 @@ def cache_rates(L, H, n, flush, reps=10):
     return out
 
 
-def _region_rate(L, name, nbytes, fn, reps):
-    """Average duration of the named region over reps calls of fn (HIP
-    events on the library stream) and the rate nbytes / duration."""
+def copy_rate(L, nbytes, reps=10):
+    """GB/s of a plain streaming copy moving nbytes (reading and writing
+    nbytes / 2 each), warm, the median of reps event-timed launches: the
+    rate a streaming kernel of that length reaches on this device (short
+    launches do not reach the long-copy rate, and working sets beyond the
+    256 MB Infinity Cache run at the HBM rate)."""
     lib = L.lib()
-    L.check(lib.lace_timing_reset())
-    L.check(lib.lace_timing_only(name.encode()))
-    L.check(lib.lace_timing_enable(1))
+    half = int(nbytes) // 2 // 16 * 16
+    src = L.DevVec(half // 8 + 2, np.float64)
+    dst = L.DevVec(half // 8 + 2, np.float64)
+    L.check(lib.lace_stream_copy(dst.ptr, src.ptr, half))
+    L.check(lib.lace_timing_only(b"stream_copy"))
+    ts = []
     for _ in range(reps):
-        fn()
-    L.check(lib.lace_device_synchronize())
-    L.check(lib.lace_timing_enable(0))
+        L.check(lib.lace_timing_reset())
+        L.check(lib.lace_timing_enable(1))
+        L.check(lib.lace_stream_copy(dst.ptr, src.ptr, half))
+        L.check(lib.lace_timing_enable(0))
+        L.check(lib.lace_device_synchronize())
+        ms, cnt = L.timing("stream_copy")
+        ts.append(ms / cnt)
     L.check(lib.lace_timing_only(None))
-    ms, cnt = L.timing(name)
-    avg = ms / cnt * 1e-3
+    return round(2.0 * half / (float(np.median(ts)) * 1e-3) / 1e9, 1)
+
+
+def _region_rate(L, name, nbytes, fn, reps):
+    """Duration of the named region per call of fn (HIP events on the
+    library stream), the median over reps calls (a call that meets a stall
+    of the device queue does not move it), and the rate nbytes / duration."""
+    lib = L.lib()
+    L.check(lib.lace_timing_only(name.encode()))
+    ts = []
+    for _ in range(reps):
+        L.check(lib.lace_timing_reset())
+        L.check(lib.lace_timing_enable(1))
+        fn()
+        L.check(lib.lace_timing_enable(0))
+        L.check(lib.lace_device_synchronize())
+        ms, cnt = L.timing(name)
+        if cnt:
+            ts.append(ms / cnt * 1e-3)
+    L.check(lib.lace_timing_only(None))
+    avg = float(np.median(ts))
     return avg, nbytes / avg / 1e9
 
 
@@ -457,7 +486,8 @@ def bench_bcsr(args, compact=False):
     sw = out["bilu_sweep_f32"]
     out["roofline"] = {"kernel": "block ParILU sweep (FP32)", "bound": "hbm", "achieved": sw["GBps"],
                        "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(sw["GBps"] / HBM_PEAK_GBS, 4),
-                       "bytes_per_launch": sw["bytes"], "avg_launch_us": sw["us"]}
+                       "bytes_per_launch": sw["bytes"], "avg_launch_us": sw["us"],
+                       "copy_same_bytes_GBps": copy_rate(L, sw["bytes"])}
     out["all_converged"] = all(out[f"solve_{k}"]["converged"] == 1 and
                                out[f"solve_{k}"]["rel_true_residual"] <= 1e-8 for k in ("f32", "bf16", "f64"))
     return out
@@ -833,6 +863,7 @@ def main() -> int:
     s64_ms, s64_cnt = L.timing("parilu_sweep")
     ach64 = P64.sweep_bytes() / (s64_ms / s64_cnt * 1e-3) / 1e9
     L.check(lib.lace_timing_only(None))
+    copy64 = copy_rate(L, P64.sweep_bytes())
     del P64
     cache = cache_rates(L, H, n, flush)
 
@@ -903,10 +934,13 @@ def main() -> int:
         "mixed_rel_true_residual": st_m.rel_true_residual,
         "fp64_rel_true_residual": st_f.rel_true_residual,
         "all_converged": ok,
-        "parilu_sweep_GBps": {"f32": round(achieved, 1), "f64": round(ach64, 1)},
+        "parilu_sweep_GBps": {"f32": round(achieved, 1), "f64": round(ach64, 1),
+                              "f64_copy_same_bytes": copy64},
         "fp64_hh_mdot": hh_mdot,
         # the kernel with the most time in the timed mixed steps
-        "roofline": dict(kernel=dominant, **kernels[dominant]) if dominant else None,
+        "roofline": (dict(kernel=dominant, **kernels[dominant],
+                          copy_same_bytes_GBps=copy_rate(L, kernels[dominant]["bytes_per_launch"]))
+                     if dominant else None),
         "kernels": kernels,
         # value and ms_per_step: steps run back to back (each finds what the
         # previous one left in the 256 MB Infinity Cache); cold: each step
