@@ -946,55 +946,14 @@ __device__ void cgs_close_column(double* small, SmallLayout lay, int jc, double 
 }
 
 // Step j's small part (j = m: the closing of column m - 1 only).
-// finalize_partials with a workgroup of BS lanes (a power of two, 64 to
-// 1024): each lane sums fewer slabs, the waves' totals are added in a fixed
-// pairwise tree.
-template <int BS>
-__device__ void finalize_partials_wide(int nb, int cnt, bool self, int pstride,
-                                       const double* __restrict__ partial, double* out) {
-  constexpr int CH = 16, NW = BS / 64;
-  __shared__ double seg[NW][CH];
-  const int tot = cnt + (self ? 1 : 0);
-  const int wv = threadIdx.x >> 6;
-  for (int base = 0; base < tot; base += CH) {
-    double acc[CH];
-#pragma unroll
-    for (int k = 0; k < CH; k++) acc[k] = 0.0;
-    for (int bb = threadIdx.x; bb < nb; bb += BS) {
-      const double* pr = partial + (size_t)bb * pstride;
-#pragma unroll
-      for (int k = 0; k < CH; k++) {
-        const int c = base + k;
-        if (c < tot) acc[k] += pr[(self && c == cnt) ? pstride - 1 : c];
-      }
-    }
-    wave_multi_sum<CH>(acc, seg[wv]);
-    __syncthreads();
-    if (threadIdx.x < CH && base + (int)threadIdx.x < tot) {
-      double t[NW];
-#pragma unroll
-      for (int q = 0; q < NW; q++) t[q] = seg[q][threadIdx.x];
-#pragma unroll
-      for (int h = NW / 2; h >= 1; h >>= 1)
-#pragma unroll
-        for (int q = 0; q < h; q++) t[q] = t[2 * q] + t[2 * q + 1];
-      out[base + threadIdx.x] = t[0];
-    }
-    __syncthreads();
-  }
-}
-
-constexpr int kCgsSmallBS = 1024;
-
-__global__ __launch_bounds__(kCgsSmallBS) void cgs_small_kernel(int nb, int pstride, const double* partial, int j,
-                                                                double* small, SmallLayout lay, int set_rtol,
-                                                                double rtol_rel) {
+__global__ __launch_bounds__(256) void cgs_small_kernel(int nb, int pstride, const double* partial, int j,
+                                                        double* small, SmallLayout lay, int set_rtol,
+                                                        double rtol_rel) {
   double* misc = small + lay.misc();
   if (is_done(misc)) return;
   __shared__ double s[72];
   const int cnt = j < lay.m ? j + 1 : 0;
-  // s[0..cnt), s[cnt] = ||W_j||^2
-  finalize_partials_wide<kCgsSmallBS>(nb, cnt, true, pstride, partial, s);
+  finalize_partials(nb, cnt, true, pstride, partial, s);  // s[0..cnt), s[cnt] = ||W_j||^2
   __syncthreads();
   double* sc = small + lay.T();  // 1 / ||W_k||
   if (threadIdx.x == 0) {
@@ -1019,7 +978,7 @@ __global__ __launch_bounds__(kCgsSmallBS) void cgs_small_kernel(int nb, int pstr
   double* R = small + lay.R();
   const int ldR = lay.m + 1;
   double* d = small + lay.d();
-  for (int k = threadIdx.x; k <= j; k += kCgsSmallBS) {
+  for (int k = threadIdx.x; k <= j; k += 256) {
     const double h = sc[k] * s[k];
     R[k + (size_t)j * ldR] = h;
     d[k] = h * sc[k];
@@ -1060,11 +1019,10 @@ int cgs_step(HHEngine<K>& E, int j, bool last, double rtol_rel) {
     launch_mdot_rows<K, 8, false>(E, U, nc, w, 0, part, misc);
   else
     launch_mdot_rows<K, 16, false>(E, U, nc, w, 0, part, misc);
-  cgs_small_kernel<<<1, kCgsSmallBS, 0, stream()>>>(E.nb, E.pstride, part, j, small, E.lay, 1, rtol_rel);
+  cgs_small_kernel<<<1, 256, 0, stream()>>>(E.nb, E.pstride, part, j, small, E.lay, 1, rtol_rel);
   LACE_NC_DISPATCH(nc, (cgs_update_kernel<K, NCB><<<E.nb, kBS, 0, stream()>>>(
                            E.n, E.ld, U, nc, w, last ? 0 : 1, E.rpb, small, E.lay, part, E.pstride)));
-  if (last)
-    cgs_small_kernel<<<1, kCgsSmallBS, 0, stream()>>>(E.nb, E.pstride, part, E.m, small, E.lay, 1, rtol_rel);
+  if (last) cgs_small_kernel<<<1, 256, 0, stream()>>>(E.nb, E.pstride, part, E.m, small, E.lay, 1, rtol_rel);
   LACE_HIP(hipGetLastError());
   return DEV_SUCCESS;
 }
