@@ -32,7 +32,8 @@ struct InnerLayout {
   __host__ __device__ int y() const { return R() + m * m * 32; }
   __host__ __device__ int h() const { return y() + m * 32; }
   __host__ __device__ int nrm2() const { return h() + (m + 1) * 32; }
-  __host__ __device__ int total() const { return nrm2() + 32; }
+  __host__ __device__ int inv() const { return nrm2() + 32; }  // 1 / ||w_c|| (0 for a zero norm)
+  __host__ __device__ int total() const { return inv() + 32; }
 };
 
 __device__ __forceinline__ void load4f(const float* p, double (&v)[4]) {
@@ -196,19 +197,28 @@ static void launch_update(int nq, int nslab, int n, int rpb, const float* Q, siz
 #undef LACE_UPD
 }
 
-// q = w / sqrt(nrm2[c]) per column (zero where the norm is zero).
+// q = w * inv[c] per column (inv = 1 / ||w_c||, 0 where the norm is zero),
+// four consecutive columns per thread: a streaming pass (the reciprocal is
+// formed once per column by the small kernels).  m is a multiple of 4.
 __global__ __launch_bounds__(256) void bi_scale_kernel(size_t m, const float* __restrict__ w,
-                                                       const double* __restrict__ nrm2, float* __restrict__ q) {
-  const size_t e = (size_t)blockIdx.x * 256 + threadIdx.x;
-  if (e >= m) return;
-  const double s2 = nrm2[e & 31];
-  q[e] = s2 > 0.0 ? (float)((double)w[e] / sqrt(s2)) : 0.0f;
+                                                       const double* __restrict__ inv, float* __restrict__ q) {
+  const size_t e4 = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (e4 * 4 >= m) return;
+  const int c0 = (int)((e4 * 4) & 31);
+  const float4 a = *reinterpret_cast<const float4*>(w + e4 * 4);
+  const float4 o = make_float4((float)((double)a.x * inv[c0]), (float)((double)a.y * inv[c0 + 1]),
+                               (float)((double)a.z * inv[c0 + 2]), (float)((double)a.w * inv[c0 + 3]));
+  *reinterpret_cast<float4*>(q + e4 * 4) = o;
 }
+
+__device__ __forceinline__ double inv_norm(double nrm) { return nrm > 0.0 ? 1.0 / nrm : 0.0; }
 
 __global__ void bi_init_kernel(InnerLayout L, double* __restrict__ st) {
   const int c = threadIdx.x;
   if (c >= 32) return;
-  st[L.g() + c] = sqrt(st[L.nrm2() + c]);
+  const double nr = sqrt(st[L.nrm2() + c]);
+  st[L.g() + c] = nr;
+  st[L.inv() + c] = inv_norm(nr);
   for (int i = 1; i <= L.m; i++) st[L.g() + i * 32 + c] = 0.0;
 }
 
@@ -219,6 +229,7 @@ __global__ void bi_givens_kernel(InnerLayout L, int j, double* __restrict__ st) 
   double h[kMaxInner + 1];
   for (int i = 0; i <= j; i++) h[i] = st[L.h() + i * 32 + c];
   h[j + 1] = sqrt(st[L.nrm2() + c]);
+  st[L.inv() + c] = inv_norm(h[j + 1]);
   for (int i = 0; i < j; i++) {
     const double cs = st[L.cs() + i * 32 + c], sn = st[L.sn() + i * 32 + c];
     const double a = cs * h[i] + sn * h[i + 1];
@@ -295,12 +306,12 @@ int InnerBatch::solve(const DevCSR& A32, ParILU* F, lace_trsv_kind_t kind, int s
   double* st = state.as<double>();
   double* part = partial.as<double>();
   InnerLayout L{m};
-  const int eb = ceil_div((long)bs, 256);
+  const int eb = ceil_div((long)bs / 4, 256);
   // beta_c = ||v_c||, q_0 = v / beta
   launch_dots(1, nslab, n, rpb, V, bs, V, part);
   LACE_TRY(sd_slab_reduce(nslab, 32, part, st + L.nrm2(), false));
   bi_init_kernel<<<1, 32, 0, stream()>>>(L, st);
-  bi_scale_kernel<<<eb, 256, 0, stream()>>>(bs, V, st + L.nrm2(), Qb);
+  bi_scale_kernel<<<eb, 256, 0, stream()>>>(bs, V, st + L.inv(), Qb);
   for (int j = 0; j < m; j++) {
     float* pj = Pb + j * bs;
     LACE_TRY(block_precond<float>(F, kind, sweeps, trans, Qb + j * bs, pj, t1.as<float>(), t2.as<float>()));
@@ -318,7 +329,7 @@ int InnerBatch::solve(const DevCSR& A32, ParILU* F, lace_trsv_kind_t kind, int s
     }
     LACE_TRY(sd_slab_reduce(nslab, 32, part, st + L.nrm2(), false));
     bi_givens_kernel<<<1, 32, 0, stream()>>>(L, j, st);
-    if (j + 1 < m) bi_scale_kernel<<<eb, 256, 0, stream()>>>(bs, wb, st + L.nrm2(), Qb + (j + 1) * bs);
+    if (j + 1 < m) bi_scale_kernel<<<eb, 256, 0, stream()>>>(bs, wb, st + L.inv(), Qb + (j + 1) * bs);
   }
   bi_solve_kernel<<<1, 32, 0, stream()>>>(L, m, st);
   bi_combine_kernel<<<ceil_div((long)n * 8, 256), 256, 0, stream()>>>(n, m, Pb, bs, st + L.y(), Z);
