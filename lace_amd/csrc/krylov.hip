@@ -658,6 +658,40 @@ __global__ __launch_bounds__(kBS) void hh_xupdate_kernel(int n, int ld, const ZT
   x[i] = (XT)v;
 }
 
+// The same update with four consecutive rows per thread, moved as 16-byte
+// words (x and each column of Z); each row's sum is the one above, term by
+// term, so the result is bit-identical.  Needs n % 4 == 0 and 16-byte
+// aligned x, Z and ld (the engine's buffers are).
+template <typename XT, typename ZT, int NC>
+__global__ __launch_bounds__(kBS) void hh_xupdate4_kernel(int n, int ld, const ZT* __restrict__ Z,
+                                                          const double* __restrict__ small,
+                                                          SmallLayout lay, XT* __restrict__ x) {
+  const double* misc = small + lay.misc();
+  const int nc = (int)misc[MS_NCOLS];
+  const int b = xcd_block(blockIdx.x, gridDim.x);
+  const int i0 = (b * kBS + (int)threadIdx.x) * 4;
+  if (i0 >= n) return;
+  const double* y = small + lay.y();
+  using ACC = typename std::conditional<sizeof(XT) == 8, double, float>::type;
+  XT xv[4];
+  vload<XT, 4>(x + i0, xv);
+  ACC v[4];
+#pragma unroll
+  for (int e = 0; e < 4; e++) v[e] = (ACC)xv[e];
+#pragma unroll
+  for (int k = 0; k < NC; k++)
+    if (k < nc) {
+      ZT z[4];
+      vload<ZT, 4>(Z + (size_t)k * ld + i0, z);
+      const ACC yk = (ACC)y[k];
+#pragma unroll
+      for (int e = 0; e < 4; e++) v[e] += (ACC)z[e] * yk;
+    }
+#pragma unroll
+  for (int e = 0; e < 4; e++) xv[e] = (XT)v[e];
+  vstore<XT, 4>(x + i0, xv);
+}
+
 template <typename To, typename From>
 __global__ void cast_kernel(int n, const From* __restrict__ in, To* __restrict__ out) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -798,12 +832,23 @@ int hh_finish(HHEngine<K>& E, XT* x) {
   double* small = E.small.template as<double>();
   hh_lsq_kernel<<<1, 64, 0, stream()>>>(small, E.lay);
   const int nbv = ceil_div(E.n, kBS);
-  if (E.z_float)
-    LACE_NC_DISPATCH(E.m, (hh_xupdate_kernel<XT, float, NCB><<<nbv, kBS, 0, stream()>>>(
-                              E.n, E.ld, E.Z.template as<float>(), small, E.lay, x)));
-  else
-    LACE_NC_DISPATCH(E.m, (hh_xupdate_kernel<XT, K, NCB><<<nbv, kBS, 0, stream()>>>(
-                              E.n, E.ld, E.Z.template as<K>(), small, E.lay, x)));
+  const bool vec = E.n % 4 == 0 && reinterpret_cast<uintptr_t>(x) % 16 == 0;
+  const int nb4 = ceil_div(E.n / 4, kBS);
+  if (E.z_float) {
+    if (vec)
+      LACE_NC_DISPATCH(E.m, (hh_xupdate4_kernel<XT, float, NCB><<<nb4, kBS, 0, stream()>>>(
+                                E.n, E.ld, E.Z.template as<float>(), small, E.lay, x)));
+    else
+      LACE_NC_DISPATCH(E.m, (hh_xupdate_kernel<XT, float, NCB><<<nbv, kBS, 0, stream()>>>(
+                                E.n, E.ld, E.Z.template as<float>(), small, E.lay, x)));
+  } else {
+    if (vec)
+      LACE_NC_DISPATCH(E.m, (hh_xupdate4_kernel<XT, K, NCB><<<nb4, kBS, 0, stream()>>>(
+                                E.n, E.ld, E.Z.template as<K>(), small, E.lay, x)));
+    else
+      LACE_NC_DISPATCH(E.m, (hh_xupdate_kernel<XT, K, NCB><<<nbv, kBS, 0, stream()>>>(
+                                E.n, E.ld, E.Z.template as<K>(), small, E.lay, x)));
+  }
   LACE_HIP(hipGetLastError());
   return DEV_SUCCESS;
 }

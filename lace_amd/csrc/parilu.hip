@@ -343,6 +343,25 @@ __global__ void parilu_scatter_sym_kernel(int nnz, const int* __restrict__ dst, 
   }
 }
 
+// The same initial factors for a structurally symmetric pattern, from the
+// position-ordered copies of A the streaming sweep reads: L's and U's
+// positions coincide (L in CSR, U in CSC), AL holds a_ij and AU a_ji at the
+// position of (i, j), so L0 = AL with a unit diagonal and U0 = AU; reads and
+// writes are coalesced (the scatter from A's CSR order is not).
+template <typename T>
+__global__ void parilu_reset_sym_kernel(int np, const int* __restrict__ pmeta, const T* __restrict__ AL,
+                                        const T* __restrict__ AU, T* __restrict__ L0, T* __restrict__ L1,
+                                        T* __restrict__ U0, T* __restrict__ U1) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= np) return;
+  const T l = pmeta[p] < 0 ? T(1) : AL[p];
+  const T u = AU[p];
+  L0[p] = l;
+  L1[p] = l;
+  U0[p] = u;
+  U1[p] = u;
+}
+
 template <typename T>
 __global__ void parilu_unitdiag_kernel(int n, const int* __restrict__ lrow, const int* __restrict__ lcol,
                                        T* __restrict__ L0, T* __restrict__ L1) {
@@ -809,7 +828,17 @@ static int build_pattern(const data_d_matrix* A, ParILU* P) {
 
 int ParILU::reset() {
   const int nb = ceil_div(nnz, 256);
-  if (prec == LACE_F64) {
+  if (sym && pmeta.p && AL.p && AU.p) {
+    const int nbl = ceil_div(nnzL, 256);
+    if (prec == LACE_F64)
+      parilu_reset_sym_kernel<double><<<nbl, 256, 0, stream()>>>(
+          nnzL, pmeta.as<int>(), AL.as<double>(), AU.as<double>(), lval[0].as<double>(), lval[1].as<double>(),
+          uval[0].as<double>(), uval[1].as<double>());
+    else
+      parilu_reset_sym_kernel<float><<<nbl, 256, 0, stream()>>>(
+          nnzL, pmeta.as<int>(), AL.as<float>(), AU.as<float>(), lval[0].as<float>(), lval[1].as<float>(),
+          uval[0].as<float>(), uval[1].as<float>());
+  } else if (prec == LACE_F64) {
     parilu_reset_kernel<double><<<nb, 256, 0, stream()>>>(nnz, dst.as<int>(), aval.as<double>(),
                                                           lval[0].as<double>(), lval[1].as<double>(),
                                                           uval[0].as<double>(), uval[1].as<double>());
