@@ -304,6 +304,37 @@ def cache_rates(L, H, n, flush, reps=10):
     return out
 
 
+# time-to-solution broken down by what the GPU does: the ParILU setup, the
+# preconditioner applies (ILU triangular solves), the sparse products and the
+# orthogonalisation (outer Householder, inner classical Gram-Schmidt)
+BREAKDOWN = {
+    "setup": ["setup"],
+    "preconditioner": ["precond"],
+    "spmv": ["spmv", "spmv_f32a", "spmv_residual"],
+    "orthogonalization": ["hh_start", "householder", "hh_finish", "cgs_start", "cgs_step"],
+}
+
+
+def breakdown(L, S, db, dx, zeros):
+    """One more solve with every launch of the category regions timed
+    (events around each region on the library stream): milliseconds per
+    category, the step's time with those events, and the rest (casts, the
+    host's convergence checks, event overhead)."""
+    lib = L.lib()
+    regions = [r for rs in BREAKDOWN.values() for r in rs]
+    L.check(lib.lace_timing_reset())
+    L.check(lib.lace_timing_only(",".join(regions).encode()))
+    L.check(lib.lace_timing_enable(1))
+    res = timed_solves(L, S, db, dx, zeros, 1)
+    L.check(lib.lace_timing_enable(0))
+    L.check(lib.lace_timing_only(None))
+    out = {f"{cat}_ms": round(sum(L.timing(r)[0] for r in rs), 3) for cat, rs in BREAKDOWN.items()}
+    total = res[0][0] * 1e3
+    out["total_ms"] = round(total, 3)
+    out["other_ms"] = round(total - sum(v for k, v in out.items() if k != "total_ms"), 3)
+    return out
+
+
 def copy_rate(L, nbytes, reps=10):
     """GB/s of a plain streaming copy moving nbytes (reading and writing
     nbytes / 2 each), warm, the median of reps event-timed launches: the
@@ -792,6 +823,7 @@ def main() -> int:
     hh_mdot = {"launches": md_n, "avg_us": round(md_ms * 1e3 / max(md_n, 1), 2),
                "GBps": round(md_bytes / max(md_ms * 1e-3, 1e-12) / 1e9, 1),
                "avg_columns": round(md_bytes / max(md_n, 1) / (8.0 * n) - 1.0, 1)}
+    split = {"mixed": breakdown(L, mixed, db, dx, zeros), "fp64": breakdown(L, fp64, db, dx, zeros)}
     res_fp64n = timed_solves(L, fp64n, db, dx, zeros, args.steps)
     x_fp64n = dx.download()
     res_m16 = timed_solves(L, mixed16, db, dx, zeros, args.steps)
@@ -937,6 +969,8 @@ def main() -> int:
         "parilu_sweep_GBps": {"f32": round(achieved, 1), "f64": round(ach64, 1),
                               "f64_copy_same_bytes": copy64},
         "fp64_hh_mdot": hh_mdot,
+        # one more solve of each path with its category regions timed
+        "breakdown": split,
         # the kernel with the most time in the timed mixed steps
         "roofline": (dict(kernel=dominant, **kernels[dominant],
                           copy_same_bytes_GBps=copy_rate(L, kernels[dominant]["bytes_per_launch"]))
