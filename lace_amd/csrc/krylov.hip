@@ -949,20 +949,23 @@ __global__ void cgs_reset_kernel(double* small, SmallLayout lay) {
 
 // Column j - 1 finished with its subdiagonal h = ||W_j|| (j >= 1): the
 // previous rotations, a new one (src/givens.cpp:46-67), g, the residual.
-__device__ void cgs_close_column(double* small, SmallLayout lay, int jc, double sub) {
+// hs: column jc of R staged in LDS (rotated in place, written back by the
+// caller); css, sns: the earlier rotations, gj = g[jc] and rtol staged the
+// same way, so the serial chain below waits on LDS, not on memory.
+__device__ void cgs_close_column(double* small, SmallLayout lay, int jc, double sub, double* hs,
+                                 const double* css, const double* sns, double gj, double rtol) {
   double* misc = small + lay.misc();
-  double* R = small + lay.R();
-  const int ldR = lay.m + 1;
   double* cs = small + lay.cs();
   double* sn = small + lay.sn();
   double* g = small + lay.g();
-  double* h = R + (size_t)jc * ldR;
+  double t0 = hs[0];  // h_i as rotated by the steps before i
   for (int i = 0; i < jc; i++) {
-    const double t0 = h[i], t1 = h[i + 1];
-    h[i + 1] = -sn[i] * t0 + cs[i] * t1;
-    h[i] = cs[i] * t0 + sn[i] * t1;
+    const double t1 = hs[i + 1];
+    const double h1 = -sns[i] * t0 + css[i] * t1;
+    hs[i] = css[i] * t0 + sns[i] * t1;
+    t0 = h1;
   }
-  const double a = h[jc], bb = sub;
+  const double a = t0, bb = sub;
   const double tol = 4.9406564584124654e-324;  // nextafter(0,1)
   double c, s;
   if (fabs(bb) <= tol) {
@@ -979,24 +982,42 @@ __device__ void cgs_close_column(double* small, SmallLayout lay, int jc, double 
   }
   cs[jc] = c;
   sn[jc] = s;
-  h[jc] = c * a + s * bb;
-  const double gj = g[jc];
+  hs[jc] = c * a + s * bb;
   g[jc + 1] = -s * gj;
   g[jc] = c * gj;
-  const double res = fabs(g[jc + 1]);
+  const double res = fabs(-s * gj);
   misc[MS_RES] = res;
   misc[MS_NCOLS] = (double)(jc + 1);
-  if (!(res >= misc[MS_RTOL]) || !isfinite(res) || !(sub > 0.0)) misc[MS_DONE] = 1.0;
+  if (!(res >= rtol) || !isfinite(res) || !(sub > 0.0)) misc[MS_DONE] = 1.0;
   if (!isfinite(res)) misc[MS_BREAK] = 1.0;
 }
 
-// Step j's small part (j = m: the closing of column m - 1 only).
+// Step j's small part (j = m: the closing of column m - 1 only).  The state
+// the closing of column j - 1 reads is staged in LDS by every lane while
+// the partials are summed.
 __global__ __launch_bounds__(256) void cgs_small_kernel(int nb, int pstride, const double* partial, int j,
                                                         double* small, SmallLayout lay, int set_rtol,
                                                         double rtol_rel) {
   double* misc = small + lay.misc();
   if (is_done(misc)) return;
   __shared__ double s[72];
+  __shared__ double hs[64], css[64], sns[64], pre[2];
+  double* R = small + lay.R();
+  const int ldR = lay.m + 1;
+  const int jc = j - 1;
+  if (j >= 1) {
+    const double* cs = small + lay.cs();
+    const double* sn = small + lay.sn();
+    for (int k = threadIdx.x; k <= jc; k += 256) hs[k] = R[k + (size_t)jc * ldR];
+    for (int k = threadIdx.x; k < jc; k += 256) {
+      css[k] = cs[k];
+      sns[k] = sn[k];
+    }
+    if (threadIdx.x == 0) {
+      pre[0] = small[lay.g() + jc];
+      pre[1] = misc[MS_RTOL];
+    }
+  }
   const int cnt = j < lay.m ? j + 1 : 0;
   finalize_partials(nb, cnt, true, pstride, partial, s);  // s[0..cnt), s[cnt] = ||W_j||^2
   __syncthreads();
@@ -1012,16 +1033,16 @@ __global__ __launch_bounds__(256) void cgs_small_kernel(int nb, int pstride, con
       g[0] = nrm;
       if (!(nrm >= misc[MS_RTOL]) || !isfinite(nrm) || nrm == 0.0) misc[MS_DONE] = 1.0;
     } else {
-      cgs_close_column(small, lay, j - 1, nrm);
+      cgs_close_column(small, lay, jc, nrm, hs, css, sns, pre[0], pre[1]);
     }
     sc[j] = nrm > 0.0 ? 1.0 / nrm : 0.0;
   }
   __syncthreads();
+  if (j >= 1)
+    for (int k = threadIdx.x; k <= jc; k += 256) R[k + (size_t)jc * ldR] = hs[k];
   if (j >= lay.m || is_done(misc)) return;
   // column j: h_kj = s_k (W_k . w) into R's column (rotated when it closes)
   // and the update's coefficients d_k = h_kj s_k
-  double* R = small + lay.R();
-  const int ldR = lay.m + 1;
   double* d = small + lay.d();
   for (int k = threadIdx.x; k <= j; k += 256) {
     const double h = sc[k] * s[k];
