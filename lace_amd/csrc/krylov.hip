@@ -626,17 +626,30 @@ __global__ __launch_bounds__(kBS) void hh_newcol_vec_kernel(int n, int ld, K* __
 
 // Least squares y = R^{-1} g by the column-oriented back substitution of
 // src/fgmres_householder_restart.cpp:721-729, for the ncols columns done.
-__global__ void hh_lsq_kernel(double* small, SmallLayout lay) {
-  if (threadIdx.x != 0) return;
-  double* misc = small + lay.misc();
+// R's leading nc x nc block and g are staged in LDS; each step jj updates
+// y_i (i < jj) in parallel, one lane per i, so every y_i receives the same
+// operations in the same order as in the serial loop.  One wave (64 lanes):
+// nc <= 63.
+__global__ __launch_bounds__(64) void hh_lsq_kernel(double* small, SmallLayout lay) {
+  __shared__ double Rs[64 * 64];
+  __shared__ double ys[64];
+  const double* misc = small + lay.misc();
   const int nc = (int)misc[MS_NCOLS];
   const double* R = small + lay.R();
   const double* g = small + lay.g();
   double* y = small + lay.y();
   const int ldR = lay.m + 1;
-  for (int i = 0; i < nc; i++) y[i] = g[i] / R[i + i * ldR];
-  for (int jj = nc - 1; jj > 0; jj--)
-    for (int i = jj - 1; i >= 0; i--) y[i] = y[i] - R[i + jj * ldR] * y[jj] / R[i + i * ldR];
+  const int t = threadIdx.x;
+  for (int e = t; e < nc * nc; e += 64) Rs[e] = R[e % nc + (e / nc) * ldR];
+  __syncthreads();
+  if (t < nc) ys[t] = g[t] / Rs[t + t * nc];
+  __syncthreads();
+  for (int jj = nc - 1; jj > 0; jj--) {
+    const double yj = ys[jj];
+    if (t < jj) ys[t] = ys[t] - Rs[t + jj * nc] * yj / Rs[t + t * nc];
+    __syncthreads();
+  }
+  if (t < nc) y[t] = ys[t];
 }
 
 // x += Z y over the ncols columns done.
