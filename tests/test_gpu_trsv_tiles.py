@@ -32,6 +32,10 @@ def _wrap(orc):
 def _problem(orc, name):
     if name == "convdiff":
         return orc.convdiff3d(20, 13, 11)  # sides not multiples of 8
+    if name == "tiny":
+        return orc.convdiff3d(3, 3, 2)  # one box, clipped to the grid
+    if name == "chain":
+        return orc.laplace2d(50, 1)  # distances {1} only: no grid to tile
     if name == "laplace2d":
         return orc.laplace2d(40, 37)
     if name == "wrap":
@@ -41,14 +45,15 @@ def _problem(orc, name):
     return orc.read_mtx(os.path.join(GOLDEN, "Trefethen_20.mtx"))
 
 
-WANT = {"convdiff": (8, 8, 8), "laplace2d": (32, 16, 1), "wrap": (0, 0, 0), "trefethen20": (0, 0, 0)}
+WANT = {"convdiff": (8, 8, 8), "laplace2d": (32, 16, 1), "wrap": (0, 0, 0), "trefethen20": (0, 0, 0),
+        "tiny": (3, 3, 2), "chain": (0, 0, 0)}
 
 
 def _opt(gpu, name, value):
     gpu.check(gpu.lib().lace_set_option(name.encode(), value))
 
 
-@pytest.mark.parametrize("name", ["convdiff", "laplace2d", "wrap", "trefethen20"])
+@pytest.mark.parametrize("name", ["convdiff", "laplace2d", "wrap", "trefethen20", "tiny", "chain"])
 @pytest.mark.parametrize("prec", ["f64", "f32"])
 def test_exact_solve_box_order(gpu, orc, name, prec):
     A = _problem(orc, name)
