@@ -1338,8 +1338,9 @@ int tri_solve_syncfree(const TriPlan& P, const T* val, const T* r, T* y, int* er
 // default, 256 on request): 8, or 4 for factors whose rows have at most 4
 // off-diagonal entries (a shorter poll of LDS per round).
 constexpr int kPersistR = 8;
-// Poll rounds before a persistent-solve lane gives up (each round sleeps
-// about 64 cycles when nothing moved): far beyond any legitimate wait.
+// Poll rounds before a persistent-solve lane gives up (a round takes a
+// memory round trip, or up to trsv_exact_idle x 64 cycles of sleep when
+// nothing moved): seconds, far beyond any legitimate wait.
 static int g_persist_spin_limit = 1 << 22;
 
 template <typename T, bool UPPER, int BS, int R>
