@@ -1403,8 +1403,10 @@ int tri_solve_persist_raw(int n, bool upper, const int* row, const int* col, con
                              nullptr, nullptr);
 }
 
-// Exact solve with a plan: the persistent natural-order kernel (option
-// "trsv_exact" 1, the default) or the level-ordered sync-free kernel (0).
+// Exact solve with a plan: the persistent kernel (option "trsv_exact" 1,
+// the default) with its blocks in box order where the plan has one (option
+// "trsv_exact_tiles" 1), else blocks of consecutive rows, or the
+// level-ordered sync-free kernel (0).
 template <typename T>
 int tri_solve_exact(const TriPlan& P, const T* val, const T* r, T* y, unsigned* ticket, int* err) {
   if (get_option("trsv_exact", 1) == 0) return tri_solve_syncfree<T>(P, val, r, y, err);
