@@ -42,8 +42,9 @@ __device__ __forceinline__ void load4f(const float* p, double (&v)[4]) {
 }
 
 // partial[slab][i][c] = sum over the slab's rows of q_i[row][c] w[row][c].
-// Thread t: column group 4 (t % 8), rows t / 8 + 32 k of the slab, two rows
-// per iteration with every load of both issued first.  NQ = nq.
+// Thread t: column group 4 (t % 8), rows t / 8 + 32 k of the slab, one row
+// per iteration (the loads of one row in flight per thread keep the
+// register count, and so the waves per SIMD, up).  NQ = nq.
 template <int NQ>
 __global__ __launch_bounds__(256) void bi_dots_kernel(int n, int rpb, const float* __restrict__ Q,
                                                       size_t bstride, const float* __restrict__ w,
@@ -57,33 +58,18 @@ __global__ __launch_bounds__(256) void bi_dots_kernel(int n, int rpb, const floa
   for (int i = 0; i < NQ; i++)
 #pragma unroll
     for (int e = 0; e < 4; e++) acc[i][e] = 0.0;
-  const float4 z4 = make_float4(0.f, 0.f, 0.f, 0.f);
-  for (int row = r0 + rg; row < r1; row += 64) {
-    const int row2 = row + 32;
-    const bool has2 = row2 < r1;
+  for (int row = r0 + rg; row < r1; row += 32) {
     const float4 wa = *reinterpret_cast<const float4*>(w + (size_t)row * 32 + 4 * cg);
-    const float4 wb = has2 ? *reinterpret_cast<const float4*>(w + (size_t)row2 * 32 + 4 * cg) : z4;
-    float4 qa[NQ], qb[NQ];
+    float4 qa[NQ];
 #pragma unroll
-    for (int i = 0; i < NQ; i++) {
+    for (int i = 0; i < NQ; i++)
       qa[i] = *reinterpret_cast<const float4*>(Q + i * bstride + (size_t)row * 32 + 4 * cg);
-      qb[i] = has2 ? *reinterpret_cast<const float4*>(Q + i * bstride + (size_t)row2 * 32 + 4 * cg) : z4;
-    }
 #pragma unroll
     for (int i = 0; i < NQ; i++) {
       acc[i][0] += (double)qa[i].x * (double)wa.x;
       acc[i][1] += (double)qa[i].y * (double)wa.y;
       acc[i][2] += (double)qa[i].z * (double)wa.z;
       acc[i][3] += (double)qa[i].w * (double)wa.w;
-    }
-    if (has2) {
-#pragma unroll
-      for (int i = 0; i < NQ; i++) {
-        acc[i][0] += (double)qb[i].x * (double)wb.x;
-        acc[i][1] += (double)qb[i].y * (double)wb.y;
-        acc[i][2] += (double)qb[i].z * (double)wb.z;
-        acc[i][3] += (double)qb[i].w * (double)wb.w;
-      }
     }
   }
   constexpr int nq = NQ;
@@ -106,7 +92,7 @@ __global__ __launch_bounds__(256) void bi_dots_kernel(int n, int rpb, const floa
 }
 
 // w <- w - sum_{i < nq} q_i h_i (per column); partial[slab][c] = sum of the
-// squares of the new (FP32-rounded) w.
+// squares of the new (FP32-rounded) w.  One row per iteration, as above.
 template <int NQ>
 __global__ __launch_bounds__(256) void bi_update_kernel(int n, int rpb, const float* __restrict__ Q,
                                                         size_t bstride, const double* __restrict__ h,
@@ -121,37 +107,26 @@ __global__ __launch_bounds__(256) void bi_update_kernel(int n, int rpb, const fl
 #pragma unroll
     for (int e = 0; e < 4; e++) hc[i][e] = h[i * 32 + 4 * cg + e];
   double s2[4] = {0.0, 0.0, 0.0, 0.0};
-  const float4 z4 = make_float4(0.f, 0.f, 0.f, 0.f);
-  for (int row0 = r0 + rg; row0 < r1; row0 += 64) {
-    const int rows[2] = {row0, row0 + 32};
-    const bool has2 = rows[1] < r1;
-    float4 wx[2], qx[2][NQ];
+  for (int row = r0 + rg; row < r1; row += 32) {
+    const float4 wx = *reinterpret_cast<const float4*>(w + (size_t)row * 32 + 4 * cg);
+    float4 qx[NQ];
 #pragma unroll
-    for (int u = 0; u < 2; u++) {
-      const bool in = u == 0 || has2;
-      wx[u] = in ? *reinterpret_cast<const float4*>(w + (size_t)rows[u] * 32 + 4 * cg) : z4;
+    for (int i = 0; i < NQ; i++)
+      qx[i] = *reinterpret_cast<const float4*>(Q + i * bstride + (size_t)row * 32 + 4 * cg);
+    double x[4] = {wx.x, wx.y, wx.z, wx.w};
 #pragma unroll
-      for (int i = 0; i < NQ; i++)
-        qx[u][i] = in ? *reinterpret_cast<const float4*>(Q + i * bstride + (size_t)rows[u] * 32 + 4 * cg) : z4;
+    for (int i = 0; i < NQ; i++) {
+      x[0] -= (double)qx[i].x * hc[i][0];
+      x[1] -= (double)qx[i].y * hc[i][1];
+      x[2] -= (double)qx[i].z * hc[i][2];
+      x[3] -= (double)qx[i].w * hc[i][3];
     }
-#pragma unroll
-    for (int u = 0; u < 2; u++) {
-      if (u == 1 && !has2) break;
-      double x[4] = {wx[u].x, wx[u].y, wx[u].z, wx[u].w};
-#pragma unroll
-      for (int i = 0; i < NQ; i++) {
-        x[0] -= (double)qx[u][i].x * hc[i][0];
-        x[1] -= (double)qx[u][i].y * hc[i][1];
-        x[2] -= (double)qx[u][i].z * hc[i][2];
-        x[3] -= (double)qx[u][i].w * hc[i][3];
-      }
-      const float4 o = make_float4((float)x[0], (float)x[1], (float)x[2], (float)x[3]);
-      *reinterpret_cast<float4*>(w + (size_t)rows[u] * 32 + 4 * cg) = o;
-      s2[0] += (double)o.x * o.x;
-      s2[1] += (double)o.y * o.y;
-      s2[2] += (double)o.z * o.z;
-      s2[3] += (double)o.w * o.w;
-    }
+    const float4 o = make_float4((float)x[0], (float)x[1], (float)x[2], (float)x[3]);
+    *reinterpret_cast<float4*>(w + (size_t)row * 32 + 4 * cg) = o;
+    s2[0] += (double)o.x * o.x;
+    s2[1] += (double)o.y * o.y;
+    s2[2] += (double)o.z * o.z;
+    s2[3] += (double)o.w * o.w;
   }
 #pragma unroll
   for (int e = 0; e < 4; e++) {
