@@ -65,21 +65,28 @@ __global__ __launch_bounds__(256) void tn_partial_kernel(int n, int rpb, BlockLi
     xb[q] = q < gb ? static_cast<const double*>(X.p[b]) : nullptr;
     rsb[q] = b == nb - 1 ? rs_last : rs_main;
   }
-  // staging: 16 rows x 16 words = 256 words per operand, one per thread
+  // staging: 16 rows x 16 words = 256 words per operand, one per thread;
+  // the next chunk's words are loaded into registers while the MFMAs run on
+  // the staged one
   const int srow = t >> 4, scol = (t & 15) * 2;
-  for (int r0 = r_begin; r0 < r_end; r0 += kTnRows) {
-    const int row = r0 + srow;
-    const bool in = row < r_end;
-    const double2 z2 = make_double2(0.0, 0.0);
-    double2 wv = in ? ld2(W + (size_t)row * 32 + scol) : z2;
-    double2 xv[kTnG];
+  const double2 z2 = make_double2(0.0, 0.0);
+  double2 wv = z2, xv[kTnG];
+  for (int r0 = r_begin - kTnRows; r0 < r_end; r0 += kTnRows) {
+    if (r0 >= r_begin) {
+      st2(&Ws[srow][scol], wv);
 #pragma unroll
-    for (int q = 0; q < kTnG; q++)
-      xv[q] = (q < gb && in && row >= rsb[q]) ? ld2(xb[q] + (size_t)row * 32 + scol) : z2;
-    st2(&Ws[srow][scol], wv);
+      for (int q = 0; q < kTnG; q++) st2(&Xs[q][srow][scol], xv[q]);
+      __syncthreads();
+    }
+    {
+      const int row = r0 + kTnRows + srow;  // the next chunk
+      const bool in = row < r_end;
+      wv = in ? ld2(W + (size_t)row * 32 + scol) : z2;
 #pragma unroll
-    for (int q = 0; q < kTnG; q++) st2(&Xs[q][srow][scol], xv[q]);
-    __syncthreads();
+      for (int q = 0; q < kTnG; q++)
+        xv[q] = (q < gb && in && row >= rsb[q]) ? ld2(xb[q] + (size_t)row * 32 + scol) : z2;
+    }
+    if (r0 < r_begin) continue;
 #pragma unroll
     for (int k0 = 0; k0 < kTnRows; k0 += 4) {
       const int kr = k0 + (lane >> 4);
