@@ -151,13 +151,16 @@ struct Stage;
 template <>
 struct Stage<double> {
   static constexpr int WORDS = 16;  // 16-byte words per 32-column row
-  __device__ __forceinline__ static void put(const double* src, double* dst) { st2(dst, ld2(src)); }
+  // a 16-byte word of the row, held in registers, into LDS as doubles
+  __device__ __forceinline__ static void store(const uint4& u, double* dst) {
+    st2(dst, __builtin_bit_cast(double2, u));
+  }
 };
 template <>
 struct Stage<float> {
   static constexpr int WORDS = 8;
-  __device__ __forceinline__ static void put(const float* src, double* dst) {
-    const float4 v = *reinterpret_cast<const float4*>(src);
+  __device__ __forceinline__ static void store(const uint4& u, double* dst) {
+    const float4 v = __builtin_bit_cast(float4, u);
     st2(dst, make_double2(v.x, v.y));
     st2(dst + 2, make_double2(v.z, v.w));
   }
@@ -177,20 +180,41 @@ __global__ __launch_bounds__(256) void nn_kernel(int n, BlockList X,
   const int band = xcd_block(blockIdx.x, gridDim.x) * kNnRows;
   constexpr int WPR = Stage<TX>::WORDS, EPW = 32 / WPR;  // words per row, elements per word
   d4 acc0 = d4{0.0, 0.0, 0.0, 0.0}, acc1 = d4{0.0, 0.0, 0.0, 0.0};
-  for (int b = 0; b < X.nb; b++) {
-    const TX* Xb = static_cast<const TX*>(X.p[b]);
-    const double* Db = D + (size_t)b * 1024;
-    for (int e = t; e < kNnRows * WPR; e += 256) {
-      const int rr = e / WPR, cw = (e % WPR) * EPW;
-      const int row = band + rr;
-      if (row < n && row >= rlo) {
-        Stage<TX>::put(Xb + (size_t)row * 32 + cw, &Xs[rr][cw]);
-      } else {
-        for (int q = 0; q < EPW; q++) Xs[rr][cw + q] = 0.0;
+  // block b + 1's words of the band and of D are loaded into registers
+  // while the MFMAs run on block b's staged ones
+  constexpr int NXW = kNnRows * WPR / 256, NDW = 512 / 256;
+  uint4 xw[NXW], dw[NDW];
+  for (int b = -1; b < X.nb; b++) {
+    if (b >= 0) {
+#pragma unroll
+      for (int k = 0; k < NXW; k++) {
+        const int e = t + 256 * k;
+        Stage<TX>::store(xw[k], &Xs[e / WPR][(e % WPR) * EPW]);
+      }
+#pragma unroll
+      for (int k = 0; k < NDW; k++) {
+        const int e = t + 256 * k;
+        st2(&Ds[e >> 4][(e & 15) * 2], __builtin_bit_cast(double2, dw[k]));
+      }
+      __syncthreads();
+    }
+    if (b + 1 < X.nb) {
+      const TX* Xb = static_cast<const TX*>(X.p[b + 1]);
+      const double* Db = D + (size_t)(b + 1) * 1024;
+#pragma unroll
+      for (int k = 0; k < NXW; k++) {
+        const int e = t + 256 * k;
+        const int row = band + e / WPR, cw = (e % WPR) * EPW;
+        xw[k] = make_uint4(0u, 0u, 0u, 0u);
+        if (row < n && row >= rlo) xw[k] = *reinterpret_cast<const uint4*>(Xb + (size_t)row * 32 + cw);
+      }
+#pragma unroll
+      for (int k = 0; k < NDW; k++) {
+        const int e = t + 256 * k;
+        dw[k] = *reinterpret_cast<const uint4*>(Db + (e >> 4) * 32 + (e & 15) * 2);
       }
     }
-    for (int e = t; e < 512; e += 256) st2(&Ds[e >> 4][(e & 15) * 2], ld2(Db + (e >> 4) * 32 + (e & 15) * 2));
-    __syncthreads();
+    if (b < 0) continue;
 #pragma unroll
     for (int k0 = 0; k0 < 32; k0 += 4) {
       const int kk = k0 + (lane >> 4);
