@@ -268,32 +268,45 @@ __global__ __launch_bounds__(256) void reduce_cols_kernel(int nb, int cnt, int s
 }
 
 // Sum of the per-block partials in a fixed order: out[k] for k < cnt and,
-// if self, out[cnt] = the self slot.  One block of 256 threads.  Thread t
-// owns the partial rows t, t+256, ... and accumulates 16 columns at a time,
-// so all of its loads for a chunk are independent and in flight together;
-// the 16 per-thread sums then meet in a butterfly per wave and a fixed
-// 4-wave combination.
-__device__ void finalize_partials(int nb, int cnt, bool self, int pstride,
-                                  const double* __restrict__ partial, double* out) {
-  constexpr int CH = 16;
-  __shared__ double seg[kBS / 64][CH];
+// if self, out[cnt] = the self slot.  One block of 256 threads, ch columns
+// at a time (ch: the column count rounded up to a power of two, at most
+// 16): lane t reads column t % ch of the partial rows t / ch, t / ch +
+// 256 / ch, ..., so the lanes of a row read its consecutive words and a
+// round touches each cache line once, and a thread issues the loads of 64
+// of its rows before adding any of them, so the 1024 rows of a 2M-row
+// vector's 2048-row slabs take one wait for memory.  The row groups' sums
+// then meet in a fixed butterfly per wave and a fixed 4-wave combination.
+__device__ __forceinline__ void finalize_partials(int nb, int cnt, bool self, int pstride,
+                                                  const double* __restrict__ partial, double* out) {
+  constexpr int RB = 64;  // rows in flight per thread
+  __shared__ double seg[kBS / 64][16];
   const int tot = cnt + (self ? 1 : 0);
+  int ch = 1;
+  while (ch < tot && ch < 16) ch <<= 1;
+  const int G = kBS / ch;  // row groups
+  const int k = threadIdx.x & (ch - 1), g = threadIdx.x / ch;
   const int wv = threadIdx.x >> 6;
-  for (int base = 0; base < tot; base += CH) {
-    double acc[CH];
+  for (int base = 0; base < tot; base += ch) {
+    const int c = base + k;
+    const unsigned col = (self && c == cnt) ? pstride - 1 : c;
+    double acc = 0.0;
+    if (c < tot) {
+      for (int b0 = g; b0 < nb; b0 += RB * G) {
+        double v[RB];
 #pragma unroll
-    for (int k = 0; k < CH; k++) acc[k] = 0.0;
-    for (int bb = threadIdx.x; bb < nb; bb += kBS) {
-      const double* pr = partial + (size_t)bb * pstride;
+        for (int q = 0; q < RB; q++) {
+          v[q] = 0.0;
+          if (b0 + q * G < nb) v[q] = partial[(unsigned)(b0 + q * G) * (unsigned)pstride + col];
+        }
 #pragma unroll
-      for (int k = 0; k < CH; k++) {
-        const int c = base + k;
-        if (c < tot) acc[k] += pr[(self && c == cnt) ? pstride - 1 : c];
+        for (int q = 0; q < RB; q++)
+          if (b0 + q * G < nb) acc += v[q];
       }
     }
-    wave_multi_sum<CH>(acc, seg[wv]);
+    for (int off = ch; off < 64; off <<= 1) acc += __shfl_xor(acc, off);
+    if ((int)(threadIdx.x & 63) < ch) seg[wv][k] = acc;
     __syncthreads();
-    if (threadIdx.x < CH && base + (int)threadIdx.x < tot)
+    if ((int)threadIdx.x < ch && base + (int)threadIdx.x < tot)
       out[base + threadIdx.x] = (seg[0][threadIdx.x] + seg[1][threadIdx.x]) +
                                 (seg[2][threadIdx.x] + seg[3][threadIdx.x]);
     __syncthreads();
@@ -961,11 +974,12 @@ __global__ void cgs_reset_kernel(double* small, SmallLayout lay) {
 }
 
 // Column j - 1 finished with its subdiagonal h = ||W_j|| (j >= 1): the
-// previous rotations, a new one (src/givens.cpp:46-67), g, the residual.
+// previous rotations, a new one (src/givens.cpp:46-67), g, the residual;
+// returns whether the cycle is done (also recorded in misc).
 // hs: column jc of R staged in LDS (rotated in place, written back by the
 // caller); css, sns: the earlier rotations, gj = g[jc] and rtol staged the
 // same way, so the serial chain below waits on LDS, not on memory.
-__device__ void cgs_close_column(double* small, SmallLayout lay, int jc, double sub, double* hs,
+__device__ bool cgs_close_column(double* small, SmallLayout lay, int jc, double sub, double* hs,
                                  const double* css, const double* sns, double gj, double rtol) {
   double* misc = small + lay.misc();
   double* cs = small + lay.cs();
@@ -1001,8 +1015,10 @@ __device__ void cgs_close_column(double* small, SmallLayout lay, int jc, double 
   const double res = fabs(-s * gj);
   misc[MS_RES] = res;
   misc[MS_NCOLS] = (double)(jc + 1);
-  if (!(res >= rtol) || !isfinite(res) || !(sub > 0.0)) misc[MS_DONE] = 1.0;
+  const bool done = !(res >= rtol) || !isfinite(res) || !(sub > 0.0);
+  if (done) misc[MS_DONE] = 1.0;
   if (!isfinite(res)) misc[MS_BREAK] = 1.0;
+  return done;
 }
 
 // Step j's small part (j = m: the closing of column m - 1 only).  The state
@@ -1012,10 +1028,16 @@ __global__ __launch_bounds__(256) void cgs_small_kernel(int nb, int pstride, con
                                                         double* small, SmallLayout lay, int set_rtol,
                                                         double rtol_rel) {
   double* misc = small + lay.misc();
-  if (is_done(misc)) return;
+  // the flag is tested once the partials are summed, so that its load and
+  // theirs are in flight together (a finished cycle's partials are stale
+  // but in bounds, and nothing is written before the test)
+  const bool done = is_done(misc);
   __shared__ double s[72];
-  __shared__ double hs[64], css[64], sns[64], pre[2];
+  __shared__ double hs[64], css[64], sns[64], pre[2], scs[72];
+  __shared__ int fin;
   double* R = small + lay.R();
+  double* sc = small + lay.T();  // 1 / ||W_k||, staged in scs
+  for (int k = threadIdx.x; k < j; k += 256) scs[k] = sc[k];
   const int ldR = lay.m + 1;
   const int jc = j - 1;
   if (j >= 1) {
@@ -1026,41 +1048,42 @@ __global__ __launch_bounds__(256) void cgs_small_kernel(int nb, int pstride, con
       css[k] = cs[k];
       sns[k] = sn[k];
     }
-    if (threadIdx.x == 0) {
-      pre[0] = small[lay.g() + jc];
-      pre[1] = misc[MS_RTOL];
-    }
+    if (threadIdx.x == 0) pre[0] = small[lay.g() + jc];
   }
+  if (threadIdx.x == 0) pre[1] = misc[MS_RTOL];
   const int cnt = j < lay.m ? j + 1 : 0;
   finalize_partials(nb, cnt, true, pstride, partial, s);  // s[0..cnt), s[cnt] = ||W_j||^2
   __syncthreads();
-  double* sc = small + lay.T();  // 1 / ||W_k||
+  if (done) return;
   if (threadIdx.x == 0) {
     const double nrm = sqrt(s[cnt]);
     if (j == 0) {
       misc[MS_RNORM] = nrm;
       misc[MS_RES] = nrm;
-      if (set_rtol) misc[MS_RTOL] = rtol_rel * nrm;
+      const double rtol = set_rtol ? rtol_rel * nrm : pre[1];
+      if (set_rtol) misc[MS_RTOL] = rtol;
       double* g = small + lay.g();
       for (int t = 0; t < lay.m + 2; t++) g[t] = 0.0;
       g[0] = nrm;
-      if (!(nrm >= misc[MS_RTOL]) || !isfinite(nrm) || nrm == 0.0) misc[MS_DONE] = 1.0;
+      fin = !(nrm >= rtol) || !isfinite(nrm) || nrm == 0.0;
+      if (fin) misc[MS_DONE] = 1.0;
     } else {
-      cgs_close_column(small, lay, jc, nrm, hs, css, sns, pre[0], pre[1]);
+      fin = cgs_close_column(small, lay, jc, nrm, hs, css, sns, pre[0], pre[1]);
     }
-    sc[j] = nrm > 0.0 ? 1.0 / nrm : 0.0;
+    scs[j] = nrm > 0.0 ? 1.0 / nrm : 0.0;
+    sc[j] = scs[j];
   }
   __syncthreads();
   if (j >= 1)
     for (int k = threadIdx.x; k <= jc; k += 256) R[k + (size_t)jc * ldR] = hs[k];
-  if (j >= lay.m || is_done(misc)) return;
+  if (j >= lay.m || fin) return;
   // column j: h_kj = s_k (W_k . w) into R's column (rotated when it closes)
   // and the update's coefficients d_k = h_kj s_k
   double* d = small + lay.d();
   for (int k = threadIdx.x; k <= j; k += 256) {
-    const double h = sc[k] * s[k];
+    const double h = scs[k] * s[k];
     R[k + (size_t)j * ldR] = h;
-    d[k] = h * sc[k];
+    d[k] = h * scs[k];
   }
 }
 
@@ -1119,7 +1142,7 @@ int norm2_sq_partials(int n, int rpb, const double* x, double* partial, int pstr
   return DEV_SUCCESS;
 }
 
-__global__ void sum_slot_kernel(int nb, int pstride, const double* partial, double* out) {
+__global__ __launch_bounds__(256) void sum_slot_kernel(int nb, int pstride, const double* partial, double* out) {
   __shared__ double v[1];
   finalize_partials(nb, 0, true, pstride, partial, v);
   __syncthreads();
