@@ -670,8 +670,13 @@ int lace_block_panel_qr(int n, int r0, const double *W, double *Y, double *Tb, d
  * the diagonal form, runs sweeps 2..k of each factor in one launch (tiles
  * in ticket order, each waiting on the tiles it reads through progress
  * flags; the first sweep of U rides on L's launch); slower than a launch
- * per sweep on the 128^3 operator.  Every choice gives bit-identical
- * results.  "matlab_log" (0 or 1, default 1): the
+ * per sweep on the 128^3 operator.  "mrhs_plane_order" (0 or 1, default 1):
+ * the 32-right-hand-side product and Jacobi sweeps on ELLPACK forms run
+ * their row blocks plane segment by plane segment when the pattern's
+ * largest diagonal offset splits the rows into planes of more than 1 MB of
+ * right-hand rows (3-D grids), so that the rows a row reads at +-1 plane
+ * are still in the L2 cache (0: natural row order).  Every choice gives
+ * bit-identical results.  "matlab_log" (0 or 1, default 1): the
  * drop-in FGMRES entry points print the reference's MATLAB-syntax progress
  * lines to stdout (src/fgmres.cpp:71,178,430; src/fgmres_householder_restart.cpp:
  * 70,81,196,231,707) -- "FGMRES_Householders_partrsv_search(j) = |g_j|;" etc.,

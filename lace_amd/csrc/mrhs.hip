@@ -36,6 +36,59 @@ struct Lanes {
   static constexpr int RPB = 256 / LPR;
 };
 
+// Order in which the row blocks of the ELLPACK kernels run.  On a 3-D grid
+// (rows x + nx (y + ny z)) row i reads the block rows i +- nx ny, so in
+// natural order a row of the right-hand block is read again two planes
+// after its first read -- 4 MB of an FP32 n x 32 block at 128^3, an XCD's
+// whole L2 -- and comes from HBM up to three times.  When the largest
+// diagonal offset m of the pattern divides n into planes of more than 1 MB
+// of right-hand rows, each plane is cut into segments of about 256 KB of
+// rows and the blocks run segment by segment, each segment through all
+// planes: an XCD (which runs a contiguous range of the logical blocks)
+// then marches one segment through z, and a right-hand row's reads all
+// fall within a few segment-planes of each other.  Only the order changes:
+// every row is computed by the same code from the same operands.
+struct PlaneOrder {
+  int nplane = 0;  // rows per plane; 0: natural order
+  int seg = 0;     // rows per segment (a multiple of the rows per block)
+  int useg = 0;    // blocks per segment and plane
+  int nseg = 0;    // segments per plane
+};
+
+// The row that thread group g of logical block lb computes, or -1.
+template <int RPB>
+__device__ __forceinline__ int plane_row(const PlaneOrder& po, int lb, int g, int n) {
+  if (po.nplane == 0) {
+    const int i = lb * RPB + g;
+    return i < n ? i : -1;
+  }
+  const int per_seg = (n / po.nplane) * po.useg;  // blocks of one segment over all planes
+  const int sg = lb / per_seg, rem = lb - sg * per_seg;
+  const int z = rem / po.useg, k = rem - z * po.useg;
+  const int p = sg * po.seg + k * RPB + g;  // row within the plane
+  return (sg < po.nseg && p < po.nplane) ? z * po.nplane + p : -1;
+}
+
+// The order for a pattern with diagonal offsets dia (its form need not be
+// gathered), rows of rpb per block and right-hand rows of row_bytes, and
+// the number of blocks to launch.
+static PlaneOrder plane_order(const DiaForm& dia, int n, int rpb, size_t row_bytes, int* nblocks) {
+  PlaneOrder po;
+  *nblocks = ceil_div(n, rpb);
+  if (dia.nd < 3 || n <= 0 || get_option("mrhs_plane_order", 1) == 0) return po;
+  int m = 0;
+  for (int d = 0; d < dia.nd; d++) m = std::max(m, std::abs(dia.off.o[d]));
+  if (m <= 0 || n % m != 0 || n / m < 2 || (double)m * (double)row_bytes <= 1048576.0) return po;
+  int seg = std::max(rpb, (int)(262144 / row_bytes) / rpb * rpb);
+  seg = std::min(seg, ceil_div(m, rpb) * rpb);
+  po.nplane = m;
+  po.seg = seg;
+  po.useg = seg / rpb;
+  po.nseg = ceil_div(m, seg);
+  *nblocks = po.nseg * (n / m) * po.useg;
+  return po;
+}
+
 // Y = A X (MODE 0) or Y = B - A X (MODE 1).  Accumulation in double when the
 // values or the output are double (the single-vector rule).
 template <typename TV, typename TX, typename TY, int MODE>
@@ -88,13 +141,14 @@ template <typename TV, typename TX, typename TY, int MODE, int WS>
 __global__ __launch_bounds__(256) void spmm32_ell_kernel(int n, const int* __restrict__ ecol,
                                                          const TV* __restrict__ eval,
                                                          const TX* __restrict__ X,
-                                                         const TY* __restrict__ B, TY* __restrict__ Y) {
+                                                         const TY* __restrict__ B, TY* __restrict__ Y,
+                                                         PlaneOrder po) {
 #pragma clang fp contract(off)
   using ACC = typename std::conditional<sizeof(TV) == 8 || sizeof(TY) == 8, double, float>::type;
   constexpr int W = Lanes<TX>::W, LPR = Lanes<TX>::LPR, RPB = Lanes<TX>::RPB;
-  const int i = xcd_block(blockIdx.x, gridDim.x) * RPB + (int)threadIdx.x / LPR;
+  const int i = plane_row<RPB>(po, xcd_block(blockIdx.x, gridDim.x), (int)threadIdx.x / LPR, n);
   const int c0 = ((int)threadIdx.x % LPR) * W;
-  if (i >= n) return;
+  if (i < 0) return;
   int cc[WS];
   TV vv[WS];
   TX xv[WS][W];
@@ -124,13 +178,15 @@ __global__ __launch_bounds__(256) void spmm32_ell_kernel(int n, const int* __res
 }
 
 template <typename TV, typename TX, typename TY, int MODE>
-static bool spmm32_ell(const DevCSR& A, const TX* X, const TY* B, TY* Y, int nb) {
+static bool spmm32_ell(const DevCSR& A, const TX* X, const TY* B, TY* Y) {
   const int* ec = A.ecol.as<int>();
   const TV* ev = A.eval.as<TV>();
+  int nb = 0;
+  const PlaneOrder po = plane_order(A.dia, A.n, Lanes<TX>::RPB, 32 * sizeof(TX), &nb);
   switch (A.ew) {
 #define LACE_SPMM_ELL(WS)                                                                                \
   case WS:                                                                                               \
-    spmm32_ell_kernel<TV, TX, TY, MODE, WS><<<nb, 256, 0, stream()>>>(A.n, ec, ev, X, B, Y);             \
+    spmm32_ell_kernel<TV, TX, TY, MODE, WS><<<nb, 256, 0, stream()>>>(A.n, ec, ev, X, B, Y, po);         \
     return true;
     LACE_SPMM_ELL(1)
     LACE_SPMM_ELL(2)
@@ -159,7 +215,7 @@ static int spmm32_launch(const DevCSR& A, const TX* X, const TY* B, TY* Y, bool 
                               : (sizeof(TV) == 4 ? "spmm32_f32a" : (sizeof(TX) == 4 ? "spmm32_f32x" : "spmm32"));
   TimedRegion tr(name, flops, bytes);
   const bool ell = A.ew > 0 && get_option("spmv_ell", 1) != 0;
-  if (ell && (residual ? spmm32_ell<TV, TX, TY, 1>(A, X, B, Y, nb) : spmm32_ell<TV, TX, TY, 0>(A, X, B, Y, nb))) {
+  if (ell && (residual ? spmm32_ell<TV, TX, TY, 1>(A, X, B, Y) : spmm32_ell<TV, TX, TY, 0>(A, X, B, Y))) {
     LACE_HIP(hipGetLastError());
     return DEV_SUCCESS;
   }
@@ -254,12 +310,12 @@ __global__ __launch_bounds__(256) void jacobi32_ell_kernel(int n, const int* __r
                                                            const T* __restrict__ dval,
                                                            const T* __restrict__ r,
                                                            const T* __restrict__ yin,
-                                                           T* __restrict__ yout) {
+                                                           T* __restrict__ yout, PlaneOrder po) {
 #pragma clang fp contract(off)
   constexpr int W = Lanes<T>::W, LPR = Lanes<T>::LPR, RPB = Lanes<T>::RPB;
-  const int i = xcd_block(blockIdx.x, gridDim.x) * RPB + (int)threadIdx.x / LPR;
+  const int i = plane_row<RPB>(po, xcd_block(blockIdx.x, gridDim.x), (int)threadIdx.x / LPR, n);
   const int c0 = ((int)threadIdx.x % LPR) * W;
-  if (i >= n) return;
+  if (i < 0) return;
   int cc[WS];
   T vv[WS], yv[WS][W], ri[W], s[W];
 #pragma unroll
@@ -300,13 +356,15 @@ static int jacobi32_sweep(const TriPlan& P, const T* val, const T* r, const T* y
     const int* ec = P.ecol.as<int>();
     const T* ev = P.eval.as<T>();
     const T* dv = P.dval.as<T>();
+    int nbe = 0;
+    const PlaneOrder po = plane_order(P.dia, P.n, Lanes<T>::RPB, 32 * sizeof(T), &nbe);
     switch (P.ew) {
 #define LACE_J32_ELL(W)                                                                                      \
   case W:                                                                                                    \
     if (P.unit_diag)                                                                                         \
-      jacobi32_ell_kernel<T, true, W><<<nb, 256, 0, stream()>>>(P.n, ec, ev, dv, r, yin, yout);              \
+      jacobi32_ell_kernel<T, true, W><<<nbe, 256, 0, stream()>>>(P.n, ec, ev, dv, r, yin, yout, po);         \
     else                                                                                                     \
-      jacobi32_ell_kernel<T, false, W><<<nb, 256, 0, stream()>>>(P.n, ec, ev, dv, r, yin, yout);             \
+      jacobi32_ell_kernel<T, false, W><<<nbe, 256, 0, stream()>>>(P.n, ec, ev, dv, r, yin, yout, po);        \
     break;
       LACE_J32_ELL(1)
       LACE_J32_ELL(2)

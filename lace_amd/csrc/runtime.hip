@@ -354,7 +354,8 @@ extern "C" int lace_set_option(const char* name, int value) {
                  "stream_rows_per_thread must be 1, 2 or 4 (have %d)", value);
   } else if (key == "jacobi_ell" || key == "spmv_ell" || key == "jacobi_dia" || key == "spmv_dia" ||
              key == "jacobi_fuse" || key == "jacobi_dia_vec" || key == "jacobi_pipe" ||
-             key == "matlab_log" || key == "jacobi_dia16" || key == "inner_ortho" || key == "trsv_exact_tiles") {
+             key == "matlab_log" || key == "jacobi_dia16" || key == "inner_ortho" || key == "trsv_exact_tiles" ||
+             key == "mrhs_plane_order") {
     LACE_REQUIRE(value == 0 || value == 1, DEV_ERR_ILLEGAL_VALUE, "%s must be 0 or 1 (have %d)", name, value);
   } else if (key == "trsv_exact_wgs_per_cu") {
     LACE_REQUIRE(value >= 0 && value <= 16, DEV_ERR_ILLEGAL_VALUE, "%s must be 0..16 (have %d)", name, value);
