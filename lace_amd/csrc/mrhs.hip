@@ -177,6 +177,77 @@ __global__ __launch_bounds__(256) void spmm32_ell_kernel(int n, const int* __res
   vstore<TY, W>(Y + (size_t)i * 32 + c0, out);
 }
 
+// The product on A's diagonal form (offsets ascending, values diagonal-
+// major, a mask bit per entry): the value and right-hand loads of a row do
+// not wait on column indices, so a row takes one round trip to memory.
+// Every diagonal's right-hand row is loaded (clamped into the block; only
+// the masked entries are added), and a row's entries by ascending offset
+// are its CSR entries by ascending column, so the sums are the ELLPACK
+// kernel's, term by term.
+template <typename TV, typename TX, typename TY, int MODE, int ND>
+__global__ __launch_bounds__(256) void spmm32_dia_kernel(int n, DiaOffsets off, const uint8_t* __restrict__ mask,
+                                                         const TV* __restrict__ dval,
+                                                         const TX* __restrict__ X,
+                                                         const TY* __restrict__ B, TY* __restrict__ Y,
+                                                         PlaneOrder po) {
+#pragma clang fp contract(off)
+  using ACC = typename std::conditional<sizeof(TV) == 8 || sizeof(TY) == 8, double, float>::type;
+  constexpr int W = Lanes<TX>::W, LPR = Lanes<TX>::LPR, RPB = Lanes<TX>::RPB;
+  const int i = plane_row<RPB>(po, xcd_block(blockIdx.x, gridDim.x), (int)threadIdx.x / LPR, n);
+  const int c0 = ((int)threadIdx.x % LPR) * W;
+  if (i < 0) return;
+  const unsigned mk = mask[i];
+  TV vv[ND];
+  TX xv[ND][W];
+#pragma unroll
+  for (int d = 0; d < ND; d++) vv[d] = dval[(size_t)d * n + i];
+#pragma unroll
+  for (int d = 0; d < ND; d++) {
+    const int c = min(max(i + off.o[d], 0), n - 1);
+    vload<TX, W>(X + (size_t)c * 32 + c0, xv[d]);
+  }
+  TY add[W];
+  if (MODE == 1) vload<TY, W>(B + (size_t)i * 32 + c0, add);
+  ACC s[W];
+#pragma unroll
+  for (int w = 0; w < W; w++) s[w] = ACC(0);
+#pragma unroll
+  for (int d = 0; d < ND; d++)
+    if ((mk >> d) & 1u) {
+#pragma unroll
+      for (int w = 0; w < W; w++) s[w] = s[w] + (ACC)vv[d] * (ACC)xv[d][w];
+    }
+  TY out[W];
+#pragma unroll
+  for (int w = 0; w < W; w++) out[w] = MODE == 1 ? (TY)((ACC)add[w] - s[w]) : (TY)s[w];
+  vstore<TY, W>(Y + (size_t)i * 32 + c0, out);
+}
+
+template <typename TV, typename TX, typename TY, int MODE>
+static bool spmm32_dia(const DevCSR& A, const TX* X, const TY* B, TY* Y) {
+  const uint8_t* mk = A.dia.mask.as<uint8_t>();
+  const TV* dv = A.dia.val.as<TV>();
+  int nb = 0;
+  const PlaneOrder po = plane_order(A.dia, A.n, Lanes<TX>::RPB, 32 * sizeof(TX), &nb);
+  switch (A.dia.nd) {
+#define LACE_SPMM_DIA(ND)                                                                                \
+  case ND:                                                                                               \
+    spmm32_dia_kernel<TV, TX, TY, MODE, ND><<<nb, 256, 0, stream()>>>(A.n, A.dia.off, mk, dv, X, B, Y, po); \
+    return true;
+    LACE_SPMM_DIA(1)
+    LACE_SPMM_DIA(2)
+    LACE_SPMM_DIA(3)
+    LACE_SPMM_DIA(4)
+    LACE_SPMM_DIA(5)
+    LACE_SPMM_DIA(6)
+    LACE_SPMM_DIA(7)
+    LACE_SPMM_DIA(8)
+#undef LACE_SPMM_DIA
+    default:
+      return false;
+  }
+}
+
 template <typename TV, typename TX, typename TY, int MODE>
 static bool spmm32_ell(const DevCSR& A, const TX* X, const TY* B, TY* Y) {
   const int* ec = A.ecol.as<int>();
@@ -206,14 +277,21 @@ template <typename TV, typename TX, typename TY>
 static int spmm32_launch(const DevCSR& A, const TX* X, const TY* B, TY* Y, bool residual) {
   if (A.n == 0) return DEV_SUCCESS;
   const int nb = ceil_div(A.n, Lanes<TX>::RPB);
-  // algorithmic work: 2 flops per nonzero and right-hand side; values and
-  // columns, row pointers, X, Y (and B) once
+  const bool dia = A.dia.nd > 0 && get_option("spmv_dia", 1) != 0;
+  // algorithmic work: 2 flops per nonzero and right-hand side; the matrix
+  // form the launch reads (diagonal form: values and masks; else values and
+  // columns, row pointers), X, Y (and B) once
   const double flops = 64.0 * A.nnz;
-  const double bytes = (4.0 + sizeof(TV)) * A.nnz + 4.0 * (A.n + 1) +
-                       32.0 * A.n * (sizeof(TX) + sizeof(TY)) + (residual ? 32.0 * sizeof(TY) * A.n : 0.0);
+  const double mat = dia ? A.dia.bytes(sizeof(TV)) : (4.0 + sizeof(TV)) * A.nnz + 4.0 * (A.n + 1);
+  const double bytes =
+      mat + 32.0 * A.n * (sizeof(TX) + sizeof(TY)) + (residual ? 32.0 * sizeof(TY) * A.n : 0.0);
   const char* name = residual ? "spmm32_residual"
                               : (sizeof(TV) == 4 ? "spmm32_f32a" : (sizeof(TX) == 4 ? "spmm32_f32x" : "spmm32"));
   TimedRegion tr(name, flops, bytes);
+  if (dia && (residual ? spmm32_dia<TV, TX, TY, 1>(A, X, B, Y) : spmm32_dia<TV, TX, TY, 0>(A, X, B, Y))) {
+    LACE_HIP(hipGetLastError());
+    return DEV_SUCCESS;
+  }
   const bool ell = A.ew > 0 && get_option("spmv_ell", 1) != 0;
   if (ell && (residual ? spmm32_ell<TV, TX, TY, 1>(A, X, B, Y) : spmm32_ell<TV, TX, TY, 0>(A, X, B, Y))) {
     LACE_HIP(hipGetLastError());
@@ -342,17 +420,93 @@ __global__ __launch_bounds__(256) void jacobi32_ell_kernel(int n, const int* __r
   vstore<T, W>(yout + (size_t)i * 32 + c0, out);
 }
 
+// The sweep on the factor's off-diagonal diagonal form: as the product on
+// A's diagonal form, one round trip per row, the same sums in the same
+// order as the ELLPACK sweep.
+template <typename T, bool UNIT, int ND>
+__global__ __launch_bounds__(256) void jacobi32_dia_kernel(int n, DiaOffsets off,
+                                                           const uint8_t* __restrict__ mask,
+                                                           const T* __restrict__ oval,
+                                                           const T* __restrict__ dval,
+                                                           const T* __restrict__ r,
+                                                           const T* __restrict__ yin,
+                                                           T* __restrict__ yout, PlaneOrder po) {
+#pragma clang fp contract(off)
+  constexpr int W = Lanes<T>::W, LPR = Lanes<T>::LPR, RPB = Lanes<T>::RPB;
+  const int i = plane_row<RPB>(po, xcd_block(blockIdx.x, gridDim.x), (int)threadIdx.x / LPR, n);
+  const int c0 = ((int)threadIdx.x % LPR) * W;
+  if (i < 0) return;
+  const unsigned mk = mask[i];
+  T vv[ND], yv[ND][W], ri[W], s[W];
+#pragma unroll
+  for (int d = 0; d < ND; d++) vv[d] = oval[(size_t)d * n + i];
+#pragma unroll
+  for (int d = 0; d < ND; d++) {
+    const int c = min(max(i + off.o[d], 0), n - 1);
+    vload<T, W>(yin + (size_t)c * 32 + c0, yv[d]);
+  }
+  vload<T, W>(r + (size_t)i * 32 + c0, ri);
+  const T di = UNIT ? T(1) : dval[i];
+#pragma unroll
+  for (int w = 0; w < W; w++) s[w] = T(0);
+#pragma unroll
+  for (int d = 0; d < ND; d++)
+    if ((mk >> d) & 1u) {
+#pragma unroll
+      for (int w = 0; w < W; w++) s[w] = s[w] + vv[d] * yv[d][w];
+    }
+  T out[W];
+#pragma unroll
+  for (int w = 0; w < W; w++) out[w] = UNIT ? ri[w] - s[w] : (ri[w] - s[w]) / di;
+  vstore<T, W>(yout + (size_t)i * 32 + c0, out);
+}
+
 template <typename T>
 static int jacobi32_sweep(const TriPlan& P, const T* val, const T* r, const T* yin, T* yout) {
   const int nb = ceil_div(P.n, Lanes<T>::RPB);
-  // algorithmic work: off-diagonal values and columns, the diagonal, row
-  // pointers, r, the previous iterate (when read) and the new one
+  const bool dia = yin && P.dia.nd > 0 && P.strict_src == (const void*)val && !(P.stale & TriPlan::kFormDia) &&
+                   get_option("jacobi_dia", 1) != 0;
+  // algorithmic work: the off-diagonal form the launch reads (diagonal
+  // form: values and masks, and the diagonal unless it is a unit one; else
+  // values and columns, the diagonal and row pointers), r, the previous
+  // iterate (when read) and the new one
   const double offd = (double)P.nnz - P.n;
-  const double bytes = (4.0 + sizeof(T)) * (yin ? offd : 0.0) + sizeof(T) * (double)P.n +
-                       4.0 * (P.n + 1) + 32.0 * sizeof(T) * P.n * (yin ? 3.0 : 2.0);
+  const double mat = dia ? P.dia.bytes(sizeof(T)) + (P.unit_diag ? 0.0 : sizeof(T) * (double)P.n)
+                         : (4.0 + sizeof(T)) * (yin ? offd : 0.0) + sizeof(T) * (double)P.n + 4.0 * (P.n + 1);
+  const double bytes = mat + 32.0 * sizeof(T) * P.n * (yin ? 3.0 : 2.0);
   TimedRegion tr(P.upper ? "block_jacobi_sweep_upper" : "block_jacobi_sweep_lower",
                  64.0 * (yin ? offd : 0.0) + 32.0 * P.n, bytes);
-  if (yin && P.ew > 0 && P.strict_src == (const void*)val && get_option("jacobi_ell", 1) != 0) {
+  if (dia) {
+    const uint8_t* mk = P.dia.mask.as<uint8_t>();
+    const T* ov = P.dia.val.as<T>();
+    const T* dv = P.dval.as<T>();
+    int nbd = 0;
+    const PlaneOrder po = plane_order(P.dia, P.n, Lanes<T>::RPB, 32 * sizeof(T), &nbd);
+    switch (P.dia.nd) {
+#define LACE_J32_DIA(ND)                                                                                     \
+  case ND:                                                                                                   \
+    if (P.unit_diag)                                                                                         \
+      jacobi32_dia_kernel<T, true, ND><<<nbd, 256, 0, stream()>>>(P.n, P.dia.off, mk, ov, dv, r, yin, yout, po); \
+    else                                                                                                     \
+      jacobi32_dia_kernel<T, false, ND><<<nbd, 256, 0, stream()>>>(P.n, P.dia.off, mk, ov, dv, r, yin, yout, po); \
+    break;
+      LACE_J32_DIA(1)
+      LACE_J32_DIA(2)
+      LACE_J32_DIA(3)
+      LACE_J32_DIA(4)
+      LACE_J32_DIA(5)
+      LACE_J32_DIA(6)
+      LACE_J32_DIA(7)
+      LACE_J32_DIA(8)
+#undef LACE_J32_DIA
+      default:
+        break;
+    }
+    LACE_HIP(hipGetLastError());
+    return DEV_SUCCESS;
+  }
+  if (yin && P.ew > 0 && P.strict_src == (const void*)val && !(P.stale & TriPlan::kFormEll) &&
+      get_option("jacobi_ell", 1) != 0) {
     const int* ec = P.ecol.as<int>();
     const T* ev = P.eval.as<T>();
     const T* dv = P.dval.as<T>();

@@ -1747,11 +1747,14 @@ int ParILU::refresh_csr() {
 // options: the diagonal always (the first sweep from zero divides by it),
 // the diagonal form where there is one (the one-sweep, fused and pipelined
 // kernels), else the ELLPACK form, else the off-diagonal CSR; the 32-column
-// sweeps read the ELLPACK form or the full CSR.
+// sweeps read the diagonal form, else the ELLPACK form, else the full CSR.
 static unsigned jacobi_forms(const TriPlan& P, bool block) {
   unsigned m = TriPlan::kFormDiag;
   const bool ell = P.ew > 0 && get_option("jacobi_ell", 1) != 0;
-  if (block) return ell ? m | TriPlan::kFormEll : m;
+  if (block) {
+    if (P.dia.nd > 0 && get_option("jacobi_dia", 1) != 0) return m | TriPlan::kFormDia;
+    return ell ? m | TriPlan::kFormEll : m;
+  }
   if (P.dia.nd > 0) m |= TriPlan::kFormDia;
   if (!(P.dia.nd > 0 && get_option("jacobi_dia", 1) != 0)) m |= ell ? TriPlan::kFormEll : TriPlan::kFormStrict;
   return m;

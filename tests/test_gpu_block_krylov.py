@@ -104,15 +104,19 @@ def test_block_precond_bitexact(gpu, orc, prec, trans):
         assert np.array_equal(Z[:, 7], z)
 
 
-def _with_plane_order(gpu, fn):
-    """fn() with the plane-segment block order on, then off."""
+def _variants(gpu, fn, form_opt):
+    """fn() on the diagonal form (form_opt 1) and on the ELLPACK form
+    (form_opt 0), each with the plane-segment block order on and off."""
     lib = gpu.lib()
     outs = []
     try:
-        for on in (1, 0):
-            gpu.check(lib.lace_set_option(b"mrhs_plane_order", on))
-            outs.append(fn())
+        for form in (1, 0):
+            gpu.check(lib.lace_set_option(form_opt, form))
+            for order in (1, 0):
+                gpu.check(lib.lace_set_option(b"mrhs_plane_order", order))
+                outs.append(fn())
     finally:
+        gpu.check(lib.lace_set_option(form_opt, 1))
         gpu.check(lib.lace_set_option(b"mrhs_plane_order", 1))
     return outs
 
@@ -122,15 +126,18 @@ def test_spmm32_plane_order(gpu, orc, xdt):
     """On a 96 x 96 x 4 grid (planes of 9216 rows: more than 1 MB of
     right-hand rows in either precision, and a short last segment) the block
     product runs plane segment by plane segment: the same bits as in natural
-    row order, and column c is still the serial CSR sum of column c."""
+    row order and on the ELLPACK form, and column c is still the serial CSR
+    sum of column c."""
     A = orc.convdiff3d(96, 96, 4)
     M = gpu.DevCSRMatrix(gpu.HostCSR.from_csr(A))
     assert M.forms()[0] == 7
     rng = np.random.default_rng(23)
     X = rng.standard_normal((A.n_rows, 32)).astype(xdt)
     B = rng.standard_normal((A.n_rows, 32))
-    (y1, r1), (y0, r0) = _with_plane_order(gpu, lambda: (M.spmm32(X), M.spmm32(X, B)))
-    assert np.array_equal(y1, y0) and np.array_equal(r1, r0)
+    outs = _variants(gpu, lambda: (M.spmm32(X), M.spmm32(X, B)), b"spmv_dia")
+    y1, r1 = outs[0]
+    for y0, r0 in outs[1:]:
+        assert np.array_equal(y1, y0) and np.array_equal(r1, r0)
     for c in (0, 17, 31):
         want = orc.spmv(A, X[:, c].astype(np.float64))
         assert np.array_equal(y1[:, c], want), c
@@ -141,15 +148,18 @@ def test_spmm32_plane_order(gpu, orc, xdt):
 @pytest.mark.parametrize("trans", [0, 1])
 def test_block_precond_plane_order(gpu, orc, prec, trans):
     """The 32-RHS Jacobi sweeps on the 96 x 96 x 4 factors in plane-segment
-    order: the same bits as natural order and as the single-vector solves."""
+    order: the same bits as natural order, as the ELLPACK form and as the
+    single-vector solves."""
     A = orc.convdiff3d(96, 96, 4)
     lp = gpu.LACE_F64 if prec == "f64" else gpu.LACE_F32
     dt = np.float64 if prec == "f64" else np.float32
     P = gpu.ParILU(gpu.HostCSR.from_csr(A), lp)
     P.sweeps(3)
     R = np.random.default_rng(29 + trans).standard_normal((A.n_rows, 32)).astype(dt)
-    Z1, Z0 = _with_plane_order(gpu, lambda: P.apply_block(R, gpu.TRSV_JACOBI, sweeps=4, trans=trans))
-    assert np.array_equal(Z1, Z0)
+    outs = _variants(gpu, lambda: P.apply_block(R, gpu.TRSV_JACOBI, sweeps=4, trans=trans), b"jacobi_dia")
+    Z1 = outs[0]
+    for Z0 in outs[1:]:
+        assert np.array_equal(Z1, Z0)
     z = P.apply(R[:, 9].copy(), gpu.TRSV_JACOBI, sweeps=4, trans=trans)
     assert np.array_equal(Z1[:, 9], z)
 
