@@ -6,10 +6,16 @@
 // steps), here for the 32 columns of the block solver at once: the ILU
 // sweeps and the FP32 product read the factors and A once for all columns,
 // and every vector access is a whole 128-byte row.  Per step j (column c):
-//   p_j = M^{-1} q_j, w = A p_j, h_ij = q_i . w (i <= j, classical
-//   Gram-Schmidt, FP64 sums), w <- w - sum_i h_ij q_i, h_{j+1,j} = ||w||,
-//   q_{j+1} = w / h_{j+1,j}, Givens rotations on the Hessenberg column;
+//   p_j = M^{-1} W_j, w = A p_j, h_ij = s_i (W_i . w) (i <= j, classical
+//   Gram-Schmidt, FP64 sums), W_{j+1} = w - sum_i h_ij s_i W_i,
+//   h_{j+1,j} = ||W_{j+1}||, Givens rotations on the Hessenberg column;
 // after m steps z = sum_j p_j y_j with y from the triangular least squares.
+// The basis is kept unnormalised past W_0 = v / ||v||: column k of the
+// basis block holds W_k, and the scales s_k = 1 / ||W_k|| (s_0 = 1) enter
+// the coefficients instead, so no pass rescales a block.  With p_j =
+// M^{-1} W_j as stored, A p_j = sum_{i <= j+1} h_ij (s_i W_i) holds for the
+// stored vectors, and z = P y is a GMRES iterate of the preconditioned
+// operator.
 // Dot products and norms are reduced per 1024-row slab and then over the
 // slabs in a fixed order, so results do not depend on scheduling.
 #include <algorithm>
@@ -22,7 +28,8 @@
 namespace lace {
 
 // FP64 state per column c at [x * 32 + c]: g (m+1), cs (m), sn (m),
-// R (m x m, R[i][j] at i*m + j), y (m), h (m+1), nrm2 (1).
+// R (m x m, R[i][j] at i*m + j), y (m), h (m+1), nrm2 (1), inv (1),
+// scales s (m+1).
 struct InnerLayout {
   int m;
   __host__ __device__ int g() const { return 0; }
@@ -33,7 +40,8 @@ struct InnerLayout {
   __host__ __device__ int h() const { return y() + m * 32; }
   __host__ __device__ int nrm2() const { return h() + (m + 1) * 32; }
   __host__ __device__ int inv() const { return nrm2() + 32; }  // 1 / ||w_c|| (0 for a zero norm)
-  __host__ __device__ int total() const { return inv() + 32; }
+  __host__ __device__ int s() const { return inv() + 32; }     // s_k = 1 / ||W_k|| of the basis
+  __host__ __device__ int total() const { return s() + (m + 1) * 32; }
 };
 
 __device__ __forceinline__ void load4f(const float* p, double (&v)[4]) {
@@ -96,16 +104,21 @@ __global__ __launch_bounds__(256) void bi_dots_kernel(int n, int rpb, const floa
 template <int NQ>
 __global__ __launch_bounds__(256) void bi_update_kernel(int n, int rpb, const float* __restrict__ Q,
                                                         size_t bstride, const double* __restrict__ h,
-                                                        float* __restrict__ w, double* __restrict__ partial) {
+                                                        const double* __restrict__ sc,
+                                                        const float* __restrict__ w, float* __restrict__ wn,
+                                                        double* __restrict__ partial) {
   __shared__ double red[4][32];
+  __shared__ __align__(16) double coef[NQ * 32];
   const int t = threadIdx.x, lane = t & 63, wv = t >> 6, cg = t & 7, rg = t >> 3;
   const int slab = blockIdx.x;
   const int r0 = slab * rpb, r1 = min(n, r0 + rpb);
-  double hc[NQ][4];
-#pragma unroll
-  for (int i = 0; i < NQ; i++)
-#pragma unroll
-    for (int e = 0; e < 4; e++) hc[i][e] = h[i * 32 + 4 * cg + e];
+  // the multiplier of W_i in column c: h_ic s_i with h_ic = s_i (W_i . w),
+  // formed once per workgroup
+  for (int k = t; k < NQ * 32; k += 256) {
+    const double si = sc[k];
+    coef[k] = (si * h[k]) * si;
+  }
+  __syncthreads();
   double s2[4] = {0.0, 0.0, 0.0, 0.0};
   for (int row = r0 + rg; row < r1; row += 32) {
     const float4 wx = *reinterpret_cast<const float4*>(w + (size_t)row * 32 + 4 * cg);
@@ -114,15 +127,21 @@ __global__ __launch_bounds__(256) void bi_update_kernel(int n, int rpb, const fl
     for (int i = 0; i < NQ; i++)
       qx[i] = *reinterpret_cast<const float4*>(Q + i * bstride + (size_t)row * 32 + 4 * cg);
     double x[4] = {wx.x, wx.y, wx.z, wx.w};
+    // the multipliers are read from LDS in every row (a compiler-only
+    // barrier keeps them from being hoisted into 8 NQ registers), which
+    // leaves room for more waves per SIMD
+    asm volatile("" ::: "memory");
 #pragma unroll
     for (int i = 0; i < NQ; i++) {
-      x[0] -= (double)qx[i].x * hc[i][0];
-      x[1] -= (double)qx[i].y * hc[i][1];
-      x[2] -= (double)qx[i].z * hc[i][2];
-      x[3] -= (double)qx[i].w * hc[i][3];
+      const double2 h01 = *reinterpret_cast<const double2*>(&coef[i * 32 + 4 * cg]);
+      const double2 h23 = *reinterpret_cast<const double2*>(&coef[i * 32 + 4 * cg + 2]);
+      x[0] -= (double)qx[i].x * h01.x;
+      x[1] -= (double)qx[i].y * h01.y;
+      x[2] -= (double)qx[i].z * h23.x;
+      x[3] -= (double)qx[i].w * h23.y;
     }
     const float4 o = make_float4((float)x[0], (float)x[1], (float)x[2], (float)x[3]);
-    *reinterpret_cast<float4*>(w + (size_t)row * 32 + 4 * cg) = o;
+    *reinterpret_cast<float4*>(wn + (size_t)row * 32 + 4 * cg) = o;
     s2[0] += (double)o.x * o.x;
     s2[1] += (double)o.y * o.y;
     s2[2] += (double)o.z * o.z;
@@ -166,8 +185,8 @@ static void launch_dots(int nq, int nslab, int n, int rpb, const float* Q, size_
 }
 
 static void launch_update(int nq, int nslab, int n, int rpb, const float* Q, size_t bs, const double* h,
-                          float* w, double* part) {
-#define LACE_UPD(K) bi_update_kernel<K><<<nslab, 256, 0, stream()>>>(n, rpb, Q, bs, h, w, part)
+                          const double* sc, const float* w, float* wn, double* part) {
+#define LACE_UPD(K) bi_update_kernel<K><<<nslab, 256, 0, stream()>>>(n, rpb, Q, bs, h, sc, w, wn, part)
   LACE_NQ_SWITCH(nq, LACE_UPD)
 #undef LACE_UPD
 }
@@ -194,17 +213,21 @@ __global__ void bi_init_kernel(InnerLayout L, double* __restrict__ st) {
   const double nr = sqrt(st[L.nrm2() + c]);
   st[L.g() + c] = nr;
   st[L.inv() + c] = inv_norm(nr);
+  st[L.s() + c] = 1.0;  // W_0 = v / ||v|| is stored normalised
   for (int i = 1; i <= L.m; i++) st[L.g() + i * 32 + c] = 0.0;
 }
 
-// Givens step j on each column's Hessenberg column h[0..j], ||w||.
+// Givens step j on each column's Hessenberg column h[0..j] (h_ij =
+// s_i (W_i . w)) and h_{j+1,j} = ||W_{j+1}||, whose reciprocal is
+// s_{j+1}.
 __global__ void bi_givens_kernel(InnerLayout L, int j, double* __restrict__ st) {
   const int c = threadIdx.x;
   if (c >= 32) return;
   double h[kMaxInner + 1];
-  for (int i = 0; i <= j; i++) h[i] = st[L.h() + i * 32 + c];
+  for (int i = 0; i <= j; i++) h[i] = st[L.s() + i * 32 + c] * st[L.h() + i * 32 + c];
   h[j + 1] = sqrt(st[L.nrm2() + c]);
   st[L.inv() + c] = inv_norm(h[j + 1]);
+  st[L.s() + (j + 1) * 32 + c] = inv_norm(h[j + 1]);
   for (int i = 0; i < j; i++) {
     const double cs = st[L.cs() + i * 32 + c], sn = st[L.sn() + i * 32 + c];
     const double a = cs * h[i] + sn * h[i + 1];
@@ -298,13 +321,13 @@ int InnerBatch::solve(const DevCSR& A32, ParILU* F, lace_trsv_kind_t kind, int s
     }
     LACE_TRY(sd_slab_reduce(nslab, (j + 1) * 32, part, st + L.h(), false));
     {
-      // q_0..q_j and w read once, w written once
+      // W_0..W_j and w read once, W_{j+1} written once into the basis
+      // block (which has room for W_m, whose norm closes the last step)
       TimedRegion tu("inner_update", 2.0 * (j + 1) * bs, 4.0 * (j + 3) * bs);
-      launch_update(j + 1, nslab, n, rpb, Qb, bs, st + L.h(), wb, part);
+      launch_update(j + 1, nslab, n, rpb, Qb, bs, st + L.h(), st + L.s(), wb, Qb + (j + 1) * bs, part);
     }
     LACE_TRY(sd_slab_reduce(nslab, 32, part, st + L.nrm2(), false));
     bi_givens_kernel<<<1, 32, 0, stream()>>>(L, j, st);
-    if (j + 1 < m) bi_scale_kernel<<<eb, 256, 0, stream()>>>(bs, wb, st + L.inv(), Qb + (j + 1) * bs);
   }
   bi_solve_kernel<<<1, 32, 0, stream()>>>(L, m, st);
   bi_combine_kernel<<<ceil_div((long)n * 8, 256), 256, 0, stream()>>>(n, m, Pb, bs, st + L.y(), Z);
