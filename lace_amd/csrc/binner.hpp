@@ -12,7 +12,8 @@ constexpr int kMaxInner = 12;
 // z_c ~ A^{-1} v_c for the 32 columns independently: `m` steps of
 // right-preconditioned GMRES from zero with the fixed FP32 ILU
 // preconditioner and the FP32 copy of A, classical Gram-Schmidt with
-// FP64 dot products, Givens least squares per column, z = sum_j M^{-1} q_j y_j.
+// FP64 dot products on a basis kept unnormalised past its first vector,
+// Givens least squares per column, z = sum_j M^{-1} W_j y_j.
 // Used as the preconditioner of the outer FP64 block FGMRES (flexible in
 // the preconditioner, so the inner solve may change from step to step).
 struct InnerBatch {
