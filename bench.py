@@ -526,7 +526,8 @@ def bench_bcsr(args, compact=False):
 
 MFMA_F64_PEAK_TFS = 78.6
 BLOCK_REGIONS = ["tsgemm_tn", "tsgemm_nn", "spmm32_f32x", "spmm32_f32a", "spmm32_residual",
-                 "block_jacobi_sweep_lower", "block_jacobi_sweep_upper", "inner_dots",
+                 "block_jacobi_sweep_lower", "block_jacobi_sweep_upper", "block_jacobi_pair_lower",
+                 "block_jacobi_pair_upper", "inner_dots",
                  "inner_update", "block_inner"]
 
 

@@ -675,8 +675,11 @@ int lace_block_panel_qr(int n, int r0, const double *W, double *Y, double *Tb, d
  * their row blocks plane segment by plane segment when the pattern's
  * largest diagonal offset splits the rows into planes of more than 1 MB of
  * right-hand rows (3-D grids), so that the rows a row reads at +-1 plane
- * are still in the L2 cache (0: natural row order).  Every choice gives
- * bit-identical results.  "matlab_log" (0 or 1, default 1): the
+ * are still in the L2 cache (0: natural row order).  "jacobi_pair" (0 or 1,
+ * default 1): the 32-right-hand-side Jacobi sweeps on the diagonal form run
+ * two to a launch, each row forming the first sweep's result at the rows
+ * it reads (0: one sweep per launch).  Every choice gives bit-identical
+ * results.  "matlab_log" (0 or 1, default 1): the
  * drop-in FGMRES entry points print the reference's MATLAB-syntax progress
  * lines to stdout (src/fgmres.cpp:71,178,430; src/fgmres_householder_restart.cpp:
  * 70,81,196,231,707) -- "FGMRES_Householders_partrsv_search(j) = |g_j|;" etc.,

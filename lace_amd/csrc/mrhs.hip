@@ -461,6 +461,122 @@ __global__ __launch_bounds__(256) void jacobi32_dia_kernel(int n, DiaOffsets off
   vstore<T, W>(yout + (size_t)i * 32 + c0, out);
 }
 
+// Two sweeps in one launch on the diagonal form: row i's second-sweep sum
+// reads the first sweep's result at rows i + off_q, which the thread forms
+// itself from the previous iterate (the rows of a neighbour are formed by
+// each row that reads them).  Every load is a whole 128- or 256-byte block
+// row, so the repeated first-sweep reads are L2 hits, and the factor
+// values, r and the previous iterate come from HBM once for two sweeps
+// instead of twice, and the intermediate iterate is never written.  Each
+// first-sweep row is formed exactly as jacobi32_dia_kernel forms it (the
+// same sums in the same order; an entry that a mask excludes is computed
+// from a clamped row and not added), so the result is bit-identical to two
+// launches of that kernel.
+template <typename T, bool UNIT, int ND>
+__global__ __launch_bounds__(256) void jacobi32_dia_pair_kernel(int n, DiaOffsets off,
+                                                                const uint8_t* __restrict__ mask,
+                                                                const T* __restrict__ oval,
+                                                                const T* __restrict__ dval,
+                                                                const T* __restrict__ r,
+                                                                const T* __restrict__ yin,
+                                                                T* __restrict__ yout, PlaneOrder po) {
+#pragma clang fp contract(off)
+  constexpr int W = Lanes<T>::W, LPR = Lanes<T>::LPR, RPB = Lanes<T>::RPB;
+  const int i = plane_row<RPB>(po, xcd_block(blockIdx.x, gridDim.x), (int)threadIdx.x / LPR, n);
+  const int c0 = ((int)threadIdx.x % LPR) * W;
+  if (i < 0) return;
+  const unsigned mi = mask[i];
+  T s[W];
+#pragma unroll
+  for (int w = 0; w < W; w++) s[w] = T(0);
+#pragma unroll
+  for (int q = 0; q < ND; q++) {
+    // the first sweep at row j = i + off_q
+    const int j = min(max(i + off.o[q], 0), n - 1);
+    const unsigned mj = mask[j];
+    T t[W], rj[W];
+#pragma unroll
+    for (int w = 0; w < W; w++) t[w] = T(0);
+#pragma unroll
+    for (int p = 0; p < ND; p++) {
+      const int k = min(max(j + off.o[p], 0), n - 1);
+      const T vp = oval[(size_t)p * n + j];
+      T yk[W];
+      vload<T, W>(yin + (size_t)k * 32 + c0, yk);
+      const bool on = (mj >> p) & 1u;
+#pragma unroll
+      for (int w = 0; w < W; w++) {
+        const T a = t[w] + vp * yk[w];
+        t[w] = on ? a : t[w];
+      }
+    }
+    vload<T, W>(r + (size_t)j * 32 + c0, rj);
+    const T dj = UNIT ? T(1) : dval[j];
+    const T vq = oval[(size_t)q * n + i];
+    const bool onq = (mi >> q) & 1u;
+#pragma unroll
+    for (int w = 0; w < W; w++) {
+      const T y1 = UNIT ? rj[w] - t[w] : (rj[w] - t[w]) / dj;
+      const T b = s[w] + vq * y1;
+      s[w] = onq ? b : s[w];
+    }
+  }
+  T ri[W], out[W];
+  vload<T, W>(r + (size_t)i * 32 + c0, ri);
+  const T di = UNIT ? T(1) : dval[i];
+#pragma unroll
+  for (int w = 0; w < W; w++) out[w] = UNIT ? ri[w] - s[w] : (ri[w] - s[w]) / di;
+  vstore<T, W>(yout + (size_t)i * 32 + c0, out);
+}
+
+// Whether two sweeps of P with the value array val can share a launch:
+// the diagonal form is current for val (the 32-column sweeps would read it).
+template <typename T>
+static bool jacobi32_pair_applies(const TriPlan& P, const T* val) {
+  return P.dia.nd > 0 && P.strict_src == (const void*)val && !(P.stale & TriPlan::kFormDia) &&
+         get_option("jacobi_dia", 1) != 0 && get_option("jacobi_pair", 1) != 0;
+}
+
+// yout = two sweeps from yin (yin, r and yout distinct).
+template <typename T>
+static int jacobi32_pair(const TriPlan& P, const T* r, const T* yin, T* yout) {
+  // algorithmic work: the diagonal form (and the diagonal unless it is a
+  // unit one), r and the previous iterate once, the result once; flops of
+  // two sweeps
+  const double offd = (double)P.nnz - P.n;
+  const double bytes = P.dia.bytes(sizeof(T)) + (P.unit_diag ? 0.0 : sizeof(T) * (double)P.n) +
+                       32.0 * sizeof(T) * P.n * 3.0;
+  TimedRegion tr(P.upper ? "block_jacobi_pair_upper" : "block_jacobi_pair_lower", 2.0 * (64.0 * offd + 32.0 * P.n),
+                 bytes);
+  const uint8_t* mk = P.dia.mask.as<uint8_t>();
+  const T* ov = P.dia.val.as<T>();
+  const T* dv = P.dval.as<T>();
+  int nb = 0;
+  const PlaneOrder po = plane_order(P.dia, P.n, Lanes<T>::RPB, 32 * sizeof(T), &nb);
+  switch (P.dia.nd) {
+#define LACE_J32_PAIR(ND)                                                                                    \
+  case ND:                                                                                                   \
+    if (P.unit_diag)                                                                                         \
+      jacobi32_dia_pair_kernel<T, true, ND><<<nb, 256, 0, stream()>>>(P.n, P.dia.off, mk, ov, dv, r, yin, yout, po); \
+    else                                                                                                     \
+      jacobi32_dia_pair_kernel<T, false, ND><<<nb, 256, 0, stream()>>>(P.n, P.dia.off, mk, ov, dv, r, yin, yout, po); \
+    break;
+    LACE_J32_PAIR(1)
+    LACE_J32_PAIR(2)
+    LACE_J32_PAIR(3)
+    LACE_J32_PAIR(4)
+    LACE_J32_PAIR(5)
+    LACE_J32_PAIR(6)
+    LACE_J32_PAIR(7)
+    LACE_J32_PAIR(8)
+#undef LACE_J32_PAIR
+    default:
+      break;
+  }
+  LACE_HIP(hipGetLastError());
+  return DEV_SUCCESS;
+}
+
 template <typename T>
 static int jacobi32_sweep(const TriPlan& P, const T* val, const T* r, const T* yin, T* yout) {
   const int nb = ceil_div(P.n, Lanes<T>::RPB);
@@ -555,27 +671,36 @@ static int jacobi32_solve(const TriPlan& P, const T* val, const T* r, T* y, T* t
     LACE_HIP(hipMemsetAsync(y, 0, bytes, stream()));
     return DEV_SUCCESS;
   }
-  if (P.unit_diag) {
-    if (sweeps == 1) {
-      LACE_HIP(hipMemcpyAsync(y, r, bytes, hipMemcpyDeviceToDevice, stream()));
-      return DEV_SUCCESS;
-    }
-    const int rest = sweeps - 1;
-    T* a = (rest % 2 == 1) ? y : tmp;
-    T* b = (rest % 2 == 1) ? tmp : y;
-    LACE_TRY(jacobi32_sweep<T>(P, val, r, r, a));
-    for (int s = 1; s < rest; s++) {
-      LACE_TRY(jacobi32_sweep<T>(P, val, r, a, b));
-      std::swap(a, b);
-    }
+  if (P.unit_diag && sweeps == 1) {
+    LACE_HIP(hipMemcpyAsync(y, r, bytes, hipMemcpyDeviceToDevice, stream()));
     return DEV_SUCCESS;
   }
-  T* a = (sweeps % 2 == 1) ? y : tmp;
-  T* b = (sweeps % 2 == 1) ? tmp : y;
-  LACE_TRY(jacobi32_sweep<T>(P, val, r, nullptr, a));
-  for (int s = 1; s < sweeps; s++) {
-    LACE_TRY(jacobi32_sweep<T>(P, val, r, a, b));
-    std::swap(a, b);
+  // The first sweep from zero is r (unit diagonal: no launch) or r / diag;
+  // the K sweeps after it read the previous iterate and run two to a
+  // launch where they can (a single one last when K is odd).  Launch idx
+  // of L writes y when L - 1 - idx is even, tmp otherwise, so the last
+  // lands in y and no launch reads its own output.
+  const int K = sweeps - 1;
+  const bool pair = jacobi32_pair_applies<T>(P, val);
+  const int L = pair ? (K + 1) / 2 : K;
+  auto out_of = [&](int idx) { return ((L - 1 - idx) % 2 == 0) ? y : tmp; };
+  const T* src = r;
+  if (!P.unit_diag) {
+    T* first = L == 0 ? y : (out_of(0) == y ? tmp : y);
+    LACE_TRY(jacobi32_sweep<T>(P, val, r, nullptr, first));
+    src = first;
+  }
+  int left = K;
+  for (int idx = 0; idx < L; idx++) {
+    T* o = out_of(idx);
+    if (pair && left >= 2) {
+      LACE_TRY(jacobi32_pair<T>(P, r, src, o));
+      left -= 2;
+    } else {
+      LACE_TRY(jacobi32_sweep<T>(P, val, r, src, o));
+      left -= 1;
+    }
+    src = o;
   }
   return DEV_SUCCESS;
 }

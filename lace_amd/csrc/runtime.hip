@@ -355,7 +355,7 @@ extern "C" int lace_set_option(const char* name, int value) {
   } else if (key == "jacobi_ell" || key == "spmv_ell" || key == "jacobi_dia" || key == "spmv_dia" ||
              key == "jacobi_fuse" || key == "jacobi_dia_vec" || key == "jacobi_pipe" ||
              key == "matlab_log" || key == "jacobi_dia16" || key == "inner_ortho" || key == "trsv_exact_tiles" ||
-             key == "mrhs_plane_order") {
+             key == "mrhs_plane_order" || key == "jacobi_pair") {
     LACE_REQUIRE(value == 0 || value == 1, DEV_ERR_ILLEGAL_VALUE, "%s must be 0 or 1 (have %d)", name, value);
   } else if (key == "trsv_exact_wgs_per_cu") {
     LACE_REQUIRE(value >= 0 && value <= 16, DEV_ERR_ILLEGAL_VALUE, "%s must be 0..16 (have %d)", name, value);

@@ -164,6 +164,33 @@ def test_block_precond_plane_order(gpu, orc, prec, trans):
     assert np.array_equal(Z1[:, 9], z)
 
 
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+def test_block_precond_sweep_pairs(gpu, orc, prec):
+    """Two Jacobi sweeps per launch (each first-sweep row formed by the rows
+    that read it) give the bits of one sweep per launch, for even and odd
+    sweep counts, both factors and both directions, and match the
+    single-vector solves."""
+    A = orc.convdiff3d(96, 96, 4)
+    lp = gpu.LACE_F64 if prec == "f64" else gpu.LACE_F32
+    dt = np.float64 if prec == "f64" else np.float32
+    P = gpu.ParILU(gpu.HostCSR.from_csr(A), lp)
+    P.sweeps(3)
+    R = np.random.default_rng(31).standard_normal((A.n_rows, 32)).astype(dt)
+    lib = gpu.lib()
+    for trans in (0, 1):
+        for sweeps in (2, 3, 4, 5):
+            outs = []
+            try:
+                for on in (1, 0):
+                    gpu.check(lib.lace_set_option(b"jacobi_pair", on))
+                    outs.append(P.apply_block(R, gpu.TRSV_JACOBI, sweeps=sweeps, trans=trans))
+            finally:
+                gpu.check(lib.lace_set_option(b"jacobi_pair", 1))
+            assert np.array_equal(outs[0], outs[1]), (trans, sweeps)
+            z = P.apply(R[:, 21].copy(), gpu.TRSV_JACOBI, sweeps=sweeps, trans=trans)
+            assert np.array_equal(outs[0][:, 21], z), (trans, sweeps)
+
+
 def _check_panel(gpu, W, r0, tol, unique_r=True):
     n = W.shape[0]
     Y, Tb, Rt, passes = gpu.block_panel_qr(W, r0)
