@@ -485,45 +485,55 @@ __global__ __launch_bounds__(256) void jacobi32_dia_pair_kernel(int n, DiaOffset
   const int i = plane_row<RPB>(po, xcd_block(blockIdx.x, gridDim.x), (int)threadIdx.x / LPR, n);
   const int c0 = ((int)threadIdx.x % LPR) * W;
   if (i < 0) return;
+  // every load first (they depend on i alone), so a row waits on memory
+  // once: for each neighbour j = i + off_q its mask, values, r row,
+  // diagonal and the previous iterate's rows j + off_p
   const unsigned mi = mask[i];
+  unsigned mj[ND];
+  T vq[ND], dj[ND], vp[ND][ND], rj[ND][W], yk[ND][ND][W], ri[W];
+#pragma unroll
+  for (int q = 0; q < ND; q++) {
+    const int j = min(max(i + off.o[q], 0), n - 1);
+    mj[q] = mask[j];
+    vq[q] = oval[(size_t)q * n + i];
+    dj[q] = UNIT ? T(1) : dval[j];
+    vload<T, W>(r + (size_t)j * 32 + c0, rj[q]);
+#pragma unroll
+    for (int p = 0; p < ND; p++) {
+      const int k = min(max(j + off.o[p], 0), n - 1);
+      vp[q][p] = oval[(size_t)p * n + j];
+      vload<T, W>(yin + (size_t)k * 32 + c0, yk[q][p]);
+    }
+  }
+  vload<T, W>(r + (size_t)i * 32 + c0, ri);
+  const T di = UNIT ? T(1) : dval[i];
   T s[W];
 #pragma unroll
   for (int w = 0; w < W; w++) s[w] = T(0);
 #pragma unroll
   for (int q = 0; q < ND; q++) {
     // the first sweep at row j = i + off_q
-    const int j = min(max(i + off.o[q], 0), n - 1);
-    const unsigned mj = mask[j];
-    T t[W], rj[W];
+    T t[W];
 #pragma unroll
     for (int w = 0; w < W; w++) t[w] = T(0);
 #pragma unroll
     for (int p = 0; p < ND; p++) {
-      const int k = min(max(j + off.o[p], 0), n - 1);
-      const T vp = oval[(size_t)p * n + j];
-      T yk[W];
-      vload<T, W>(yin + (size_t)k * 32 + c0, yk);
-      const bool on = (mj >> p) & 1u;
+      const bool on = (mj[q] >> p) & 1u;
 #pragma unroll
       for (int w = 0; w < W; w++) {
-        const T a = t[w] + vp * yk[w];
+        const T a = t[w] + vp[q][p] * yk[q][p][w];
         t[w] = on ? a : t[w];
       }
     }
-    vload<T, W>(r + (size_t)j * 32 + c0, rj);
-    const T dj = UNIT ? T(1) : dval[j];
-    const T vq = oval[(size_t)q * n + i];
     const bool onq = (mi >> q) & 1u;
 #pragma unroll
     for (int w = 0; w < W; w++) {
-      const T y1 = UNIT ? rj[w] - t[w] : (rj[w] - t[w]) / dj;
-      const T b = s[w] + vq * y1;
+      const T y1 = UNIT ? rj[q][w] - t[w] : (rj[q][w] - t[w]) / dj[q];
+      const T b = s[w] + vq[q] * y1;
       s[w] = onq ? b : s[w];
     }
   }
-  T ri[W], out[W];
-  vload<T, W>(r + (size_t)i * 32 + c0, ri);
-  const T di = UNIT ? T(1) : dval[i];
+  T out[W];
 #pragma unroll
   for (int w = 0; w < W; w++) out[w] = UNIT ? ri[w] - s[w] : (ri[w] - s[w]) / di;
   vstore<T, W>(yout + (size_t)i * 32 + c0, out);
